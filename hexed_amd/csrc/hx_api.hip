@@ -1,0 +1,705 @@
+// C ABI of include/hexed_amd.h: contexts, device meshes, and the kernel sequences of the reference's free
+// functions (kernels_convective.cpp, kernels_diffusive.cpp, kernels_max_dt.cpp) and Solver::update.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "basis_tables.inc"
+#include "hx_launch.hpp"
+
+using namespace hx;
+
+struct hx_ctx
+{
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  void* ops[HX_MAX_ROW_SIZE + 1] = {};  // device Ops<rs> tables
+  double* scalar = nullptr;             // device scratch for reductions
+};
+
+struct hx_mesh
+{
+  hx_ctx* ctx = nullptr;
+  Dev_mesh dm{};
+  std::vector<void*> allocs;
+  const Kernel_set* ks = nullptr;
+  int precision = HX_FP64;
+};
+
+namespace {
+
+hx_status fail(hx_ctx* ctx, hx_status code, const std::string& msg)
+{
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+hx_status hip_check(hx_ctx* ctx, hipError_t e, const char* what)
+{
+  if (e == hipSuccess) return HX_OK;
+  return fail(ctx, HX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HX_TRY(ctx, expr, what)                                     \
+  do {                                                              \
+    hx_status s_ = hip_check((ctx), (expr), (what));                \
+    if (s_ != HX_OK) return s_;                                     \
+  } while (0)
+
+template <int RS>
+void fill_ops(Ops<RS>& o, int rs_index)
+{
+  using namespace hx_basis_tables;
+  const double* d = diff_mats[rs_index];
+  const double* b = boundarys[rs_index];
+  const double* w = weights[rs_index];
+  const double* f = filters[rs_index];
+  const double* p = prolongs[rs_index];
+  const double* r = restricts[rs_index];
+  for (int i = 0; i < RS; ++i) {
+    for (int j = 0; j < RS; ++j) {
+      o.diff[i][j] = d[i * RS + j];
+      o.filt[i][j] = f[i * RS + j];
+      for (int h = 0; h < 2; ++h) {
+        o.prolong[h][i][j] = p[(h * RS + i) * RS + j];
+        o.restrict_[h][i][j] = r[(h * RS + i) * RS + j];
+      }
+    }
+    for (int s = 0; s < 2; ++s) o.bound[s][i] = b[s * RS + i];
+    // Derivative.hpp:27-28: lift = diag(1/w) * boundary^T * diag(-1, 1)
+    const double inv_w = 1. / w[i];
+    o.lift[i][0] = -(inv_w * b[i]);
+    o.lift[i][1] = inv_w * b[RS + i];
+    o.node[i] = nodes[rs_index][i];
+  }
+}
+
+template <int RS>
+hx_status upload_ops(hx_ctx* ctx, void** out)
+{
+  Ops<RS> o;
+  fill_ops<RS>(o, RS - 2);
+  HX_TRY(ctx, hipMalloc(out, sizeof(o)), "hipMalloc(ops)");
+  HX_TRY(ctx, hipMemcpy(*out, &o, sizeof(o), hipMemcpyHostToDevice), "hipMemcpy(ops)");
+  return HX_OK;
+}
+
+hx_status ensure_ops(hx_ctx* ctx, int rs)
+{
+  if (ctx->ops[rs]) return HX_OK;
+  switch (rs) {
+    case 2: return upload_ops<2>(ctx, &ctx->ops[2]);
+    case 3: return upload_ops<3>(ctx, &ctx->ops[3]);
+    case 4: return upload_ops<4>(ctx, &ctx->ops[4]);
+    case 5: return upload_ops<5>(ctx, &ctx->ops[5]);
+    case 6: return upload_ops<6>(ctx, &ctx->ops[6]);
+    case 7: return upload_ops<7>(ctx, &ctx->ops[7]);
+    case 8: return upload_ops<8>(ctx, &ctx->ops[8]);
+  }
+  return fail(ctx, HX_ERR_INVALID, "demand for invalid kernel");
+}
+
+const Kernel_set* kernel_set(int nd, int rs)
+{
+  switch (nd) {
+    case 1: return kernel_set_nd1(rs);
+    case 2: return kernel_set_nd2(rs);
+    case 3: return kernel_set_nd3(rs);
+  }
+  return nullptr;
+}
+
+int ipow_h(int b, int e)
+{
+  int r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+
+struct Sizes
+{
+  int nd, rs, nq, nfq, nv, nvert, face_sz;
+  explicit Sizes(int n_dim, int row_size)
+  : nd{n_dim}, rs{row_size}, nq{ipow_h(row_size, n_dim)}, nfq{ipow_h(row_size, n_dim - 1)}, nv{n_dim + 2},
+    nvert{1 << n_dim}, face_sz{std::max(2 * (n_dim + 2), n_dim + row_size) * ipow_h(row_size, n_dim - 1)}
+  {}
+};
+
+// number of doubles of a field of class cls, or -1 for an unknown field
+long long field_size(const Dev_mesh& m, int field, int cls)
+{
+  const long long n = (cls == 0 || cls == 1) ? m.n_elem[cls] : 0;
+  switch (field) {
+    case HX_FIELD_STATE:
+    case HX_FIELD_RCACHE: return n * m.nv * m.nq;
+    case HX_FIELD_TSS: return n * m.nq;
+    case HX_FIELD_AV: return n * 2 * m.nq;
+    case HX_FIELD_FORCING: return n * 4 * m.nq;
+    case HX_FIELD_ADVECTION: return n * m.rs * m.nq;
+    case HX_FIELD_VERTEX_TSS: return n * (1 << m.nd);
+    case HX_FIELD_METRICS: return cls == HX_DEFORMED ? n * (m.nd * m.nd + 1) * m.nq : -1;
+    case HX_FIELD_FACES: return (long long)m.n_slot * m.face_sz;
+    case HX_FIELD_NORMALS: return (long long)m.n_nslot * m.nd * m.nfq;
+  }
+  return -1;
+}
+
+double** field_ptr(Dev_mesh& m, int field, int cls)
+{
+  const bool c_ok = cls == 0 || cls == 1;
+  switch (field) {
+    case HX_FIELD_STATE: return c_ok ? &m.state[cls] : nullptr;
+    case HX_FIELD_RCACHE: return c_ok ? &m.rcache[cls] : nullptr;
+    case HX_FIELD_TSS: return c_ok ? &m.tss[cls] : nullptr;
+    case HX_FIELD_AV: return c_ok ? &m.av[cls] : nullptr;
+    case HX_FIELD_FORCING: return c_ok ? &m.forcing[cls] : nullptr;
+    case HX_FIELD_ADVECTION: return c_ok ? &m.advection[cls] : nullptr;
+    case HX_FIELD_VERTEX_TSS: return c_ok ? &m.vertex_tss[cls] : nullptr;
+    case HX_FIELD_METRICS: return cls == HX_DEFORMED ? &m.metrics : nullptr;
+    case HX_FIELD_FACES: return &m.faces;
+    case HX_FIELD_NORMALS: return &m.normals;
+  }
+  return nullptr;
+}
+
+template <typename T>
+hx_status device_copy(hx_mesh* mesh, T** dst, const T* src, long long count, bool zero_if_null)
+{
+  *dst = nullptr;
+  if (count <= 0) return HX_OK;
+  if (!src && !zero_if_null) return HX_OK;
+  void* p = nullptr;
+  HX_TRY(mesh->ctx, hipMalloc(&p, size_t(count) * sizeof(T)), "hipMalloc(mesh)");
+  mesh->allocs.push_back(p);
+  if (src) HX_TRY(mesh->ctx, hipMemcpy(p, src, size_t(count) * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(mesh)");
+  else HX_TRY(mesh->ctx, hipMemset(p, 0, size_t(count) * sizeof(T)), "hipMemset(mesh)");
+  *dst = static_cast<T*>(p);
+  return HX_OK;
+}
+
+// Every index a kernel dereferences is checked here, so a malformed mesh is an error rather than a fault.
+std::string validate(const hx_host_mesh* d, const Sizes& z)
+{
+  auto in = [](long long v, long long lo, long long hi) { return v >= lo && v < hi; };
+  for (int c = 0; c < 2; ++c) {
+    if (d->n_elem[c] < 0 || d->n_con[c] < 0) return "negative element or connection count";
+    if (d->n_elem[c] && (!d->state[c] || !d->nom_size[c] || !d->vertex_tss[c] || !d->face_slot[c]))
+      return "element arrays missing";
+    for (long long k = 0; k < (long long)d->n_elem[c] * 2 * z.nd; ++k)
+      if (!in(d->face_slot[c][k], 0, d->n_slot)) return "face slot out of range";
+    if (d->n_con[c] && (!d->con_slot[c] || !d->con_dir[c])) return "connection arrays missing";
+    for (long long k = 0; k < (long long)d->n_con[c]; ++k) {
+      if (!in(d->con_slot[c][2 * k], 0, d->n_slot) || !in(d->con_slot[c][2 * k + 1], 0, d->n_slot))
+        return "connection slot out of range";
+      int p = d->con_dir[c][k];
+      if ((p & 3) >= z.nd || ((p >> 2) & 3) >= z.nd || (p >> 6) != 0) return "connection direction out of range";
+    }
+  }
+  if (d->n_elem[HX_DEFORMED]) {
+    if (!d->metrics || !d->face_nslot) return "deformed metrics missing";
+    for (long long k = 0; k < (long long)d->n_elem[1] * 2 * z.nd; ++k)
+      if (!in(d->face_nslot[k], -1, d->n_nslot)) return "face normal slot out of range";
+  }
+  if (d->n_con[HX_DEFORMED] && (!d->con_nslot || !d->normals)) return "deformed connection normals missing";
+  if (d->n_slot && !d->faces) return "face buffer missing";
+  if (d->n_ref && (!d->ref_slot || !d->ref_stretch)) return "refined face arrays missing";
+  if (d->n_bc_con && (!d->bc_con || !d->bc_serial || !d->bcs)) return "boundary arrays missing";
+  for (long long k = 0; k < 2LL * d->n_con[HX_DEFORMED]; ++k)
+    if (!in(d->con_nslot[k], 0, d->n_nslot)) return "connection normal slot out of range";
+  for (long long k = 0; k < d->n_ref; ++k) {
+    if (!in(d->ref_slot[5 * k], 0, d->n_slot)) return "refined coarse slot out of range";
+    int nf = 1 << (z.nd - 1);
+    for (int dd = 0; dd < z.nd - 1; ++dd) nf /= 1 + (d->ref_stretch[2 * k + dd] != 0);
+    for (int f = 0; f < nf; ++f)
+      if (!in(d->ref_slot[5 * k + 1 + f], 0, d->n_slot)) return "refined fine slot out of range";
+  }
+  for (long long k = 0; k < d->n_bc_con; ++k) {
+    if (!in(d->bc_con[k], 0, d->n_con[HX_DEFORMED])) return "boundary connection out of range";
+    if (!in(d->bc_serial[k], 0, d->n_bc)) return "boundary condition serial out of range";
+  }
+  if (d->n_slot < 0 || d->n_nslot < 0 || d->n_ref < 0 || d->n_bc_con < 0 || d->n_bc < 0) return "negative count";
+  return "";
+}
+
+Launch make_launch(hx_mesh* mesh)
+{
+  Launch l{};
+  l.stream = mesh->ctx->stream;
+  l.m = mesh->dm;
+  l.ops = mesh->ctx->ops[mesh->dm.rs];
+  l.scalar = mesh->ctx->scalar;
+  l.precision = mesh->precision;
+  return l;
+}
+
+Transport to_dev(const hx_transport& t)
+{
+  return Transport{t.const_val, t.ref_val, t.ref_temp, std::sqrt(t.ref_temp), t.temp_offset};
+}
+
+hx_status pde_check(hx_mesh* mesh, const hx_pde* pde, bool* visc, Launch* l)
+{
+  int kind = pde ? pde->kind : HX_PDE_EULER;
+  if (kind != HX_PDE_EULER && kind != HX_PDE_NAVIER_STOKES)
+    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "PDE kind not built into this library");
+  *visc = kind == HX_PDE_NAVIER_STOKES;
+  if (*visc) l->ns = Ns_params{to_dev(pde->visc), to_dev(pde->cond)};
+  return HX_OK;
+}
+
+bool valid_cls(int cls) { return cls == HX_CARTESIAN || cls == HX_DEFORMED; }
+
+__global__ void k_fill(double* p, double v) { p[threadIdx.x] = v; }
+
+}  // namespace
+
+extern "C" {
+
+int hx_kernels_built(int n_dim, int row_size, int pde_kind)
+{
+  if (pde_kind != HX_PDE_EULER && pde_kind != HX_PDE_NAVIER_STOKES) return 0;
+  return kernel_set(n_dim, row_size) ? 1 : 0;
+}
+
+hx_status hx_ctx_create(int device, hx_ctx** out)
+{
+  if (!out) return HX_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return HX_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return HX_ERR_INVALID;
+  auto* ctx = new hx_ctx;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&ctx->scalar, 8 * sizeof(double)) != hipSuccess) {
+    delete ctx;
+    return HX_ERR_HIP;
+  }
+  *out = ctx;
+  return HX_OK;
+}
+
+hx_status hx_ctx_destroy(hx_ctx* ctx)
+{
+  if (!ctx) return HX_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->ops)
+    if (p) (void)hipFree(p);
+  (void)hipFree(ctx->scalar);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return HX_OK;
+}
+
+const char* hx_last_error(const hx_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+hx_status hx_ctx_synchronize(hx_ctx* ctx)
+{
+  if (!ctx) return HX_ERR_INVALID;
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return HX_OK;
+}
+
+void* hx_ctx_stream(hx_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int hx_basis_table(int row_size, const char* name, double* out, int n)
+{
+  using namespace hx_basis_tables;
+  if (row_size < HX_MIN_ROW_SIZE || row_size > HX_MAX_ROW_SIZE || !name) return -1;
+  const int k = row_size - 2, rs = row_size;
+  const double* src = nullptr;
+  int size = 0;
+  std::string s(name);
+  if (s == "node") { src = nodes[k]; size = rs; }
+  else if (s == "weight") { src = weights[k]; size = rs; }
+  else if (s == "diff_mat") { src = diff_mats[k]; size = rs * rs; }
+  else if (s == "boundary") { src = boundarys[k]; size = 2 * rs; }
+  else if (s == "orthogonal") { src = orthogonals[k]; size = rs * rs; }
+  else if (s == "filter") { src = filters[k]; size = rs * rs; }
+  else if (s == "prolong") { src = prolongs[k]; size = 2 * rs * rs; }
+  else if (s == "restrict") { src = restricts[k]; size = 2 * rs * rs; }
+  else if (s == "eigenvals") { src = eigenvalss[k]; size = 2; }
+  else return -1;
+  if (out) std::memcpy(out, src, sizeof(double) * std::min(std::max(n, 0), size));
+  return size;
+}
+
+double hx_basis_max_cfl(int row_size)
+{
+  if (row_size < HX_MIN_ROW_SIZE || row_size > HX_MAX_ROW_SIZE) return NAN;
+  return -2 * 0.9 / hx_basis_tables::eigenvalss[row_size - 2][0];
+}
+
+double hx_basis_step_ratio(int) { return .5 / 0.9; }
+
+hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* d, hx_mesh** out)
+{
+  if (!ctx || !d || !out) return HX_ERR_INVALID;
+  *out = nullptr;
+  if (d->n_dim < 1 || d->n_dim > HX_MAX_N_DIM || d->row_size < HX_MIN_ROW_SIZE || d->row_size > HX_MAX_ROW_SIZE)
+    return fail(ctx, HX_ERR_INVALID, "demand for invalid kernel");
+  const Kernel_set* ks = kernel_set(d->n_dim, d->row_size);
+  if (!ks) return fail(ctx, HX_ERR_UNSUPPORTED, "kernels for this n_dim/row_size are not built");
+  Sizes z(d->n_dim, d->row_size);
+  std::string bad = validate(d, z);
+  if (!bad.empty()) return fail(ctx, HX_ERR_INVALID, "invalid mesh: " + bad);
+  HX_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  hx_status s = ensure_ops(ctx, d->row_size);
+  if (s != HX_OK) return s;
+  auto* mesh = new hx_mesh;
+  mesh->ctx = ctx;
+  mesh->ks = ks;
+  Dev_mesh& m = mesh->dm;
+  m.nd = z.nd; m.rs = z.rs; m.nq = z.nq; m.nfq = z.nfq; m.nv = z.nv; m.face_sz = z.face_sz;
+  auto bail = [&](hx_status st) {
+    hx_mesh_destroy(mesh);
+    return st;
+  };
+  for (int c = 0; c < 2; ++c) {
+    const long long n = d->n_elem[c];
+    m.n_elem[c] = int(n);
+    if ((s = device_copy(mesh, &m.state[c], d->state[c], n * z.nv * z.nq, true))) return bail(s);
+    if ((s = device_copy(mesh, &m.rcache[c], d->rcache[c], n * z.nv * z.nq, true))) return bail(s);
+    if ((s = device_copy(mesh, &m.tss[c], d->tss[c], n * z.nq, true))) return bail(s);
+    if ((s = device_copy(mesh, &m.av[c], d->av[c], n * 2 * z.nq, false))) return bail(s);
+    if ((s = device_copy(mesh, &m.forcing[c], d->forcing[c], n * 4 * z.nq, false))) return bail(s);
+    if ((s = device_copy(mesh, &m.advection[c], d->advection[c], n * z.rs * z.nq, false))) return bail(s);
+    if ((s = device_copy(mesh, &m.nom_size[c], d->nom_size[c], n, true))) return bail(s);
+    if ((s = device_copy(mesh, &m.vertex_tss[c], d->vertex_tss[c], n * z.nvert, true))) return bail(s);
+    if ((s = device_copy(mesh, &m.face_slot[c], d->face_slot[c], n * 2 * z.nd, true))) return bail(s);
+    m.n_con[c] = d->n_con[c];
+    if ((s = device_copy(mesh, &m.con_slot[c], d->con_slot[c], 2LL * d->n_con[c], true))) return bail(s);
+    if ((s = device_copy(mesh, &m.con_dir[c], d->con_dir[c], (long long)d->n_con[c], true))) return bail(s);
+  }
+  const long long n_def = d->n_elem[HX_DEFORMED];
+  if ((s = device_copy(mesh, &m.metrics, d->metrics, n_def * (z.nd * z.nd + 1) * z.nq, true))) return bail(s);
+  if ((s = device_copy(mesh, &m.face_nslot, d->face_nslot, n_def * 2 * z.nd, true))) return bail(s);
+  m.n_slot = d->n_slot;
+  if ((s = device_copy(mesh, &m.faces, d->faces, (long long)d->n_slot * z.face_sz, true))) return bail(s);
+  m.n_nslot = d->n_nslot;
+  if ((s = device_copy(mesh, &m.normals, d->normals, (long long)d->n_nslot * z.nd * z.nfq, true))) return bail(s);
+  if ((s = device_copy(mesh, &m.con_nslot, d->con_nslot, 2LL * d->n_con[HX_DEFORMED], true))) return bail(s);
+  m.n_ref = d->n_ref;
+  if ((s = device_copy(mesh, &m.ref_slot, d->ref_slot, 5LL * d->n_ref, true))) return bail(s);
+  if ((s = device_copy(mesh, &m.ref_stretch, d->ref_stretch, 2LL * d->n_ref, true))) return bail(s);
+  m.n_bc_con = d->n_bc_con;
+  if ((s = device_copy(mesh, &m.bc_con, d->bc_con, (long long)d->n_bc_con, true))) return bail(s);
+  if ((s = device_copy(mesh, &m.bc_serial, d->bc_serial, (long long)d->n_bc_con, true))) return bail(s);
+  m.n_bc = d->n_bc;
+  if ((s = device_copy(mesh, &m.bcs, d->bcs, (long long)d->n_bc, true))) return bail(s);
+  *out = mesh;
+  return HX_OK;
+}
+
+hx_status hx_mesh_destroy(hx_mesh* mesh)
+{
+  if (!mesh) return HX_OK;
+  (void)hipSetDevice(mesh->ctx->device);
+  (void)hipStreamSynchronize(mesh->ctx->stream);
+  for (void* p : mesh->allocs) (void)hipFree(p);
+  delete mesh;
+  return HX_OK;
+}
+
+hx_status hx_mesh_upload(hx_mesh* mesh, int field, int cls, const double* host, size_t n)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  double** p = field_ptr(mesh->dm, field, cls);
+  long long sz = field_size(mesh->dm, field, cls);
+  if (!p || sz < 0) return fail(mesh->ctx, HX_ERR_INVALID, "unknown field or class");
+  if ((long long)n != sz) return fail(mesh->ctx, HX_ERR_INVALID, "field size mismatch");
+  if (sz == 0) return HX_OK;
+  if (!*p) {
+    // optional fields (artificial viscosity, forcing, advection) are allocated on first upload
+    void* q = nullptr;
+    HX_TRY(mesh->ctx, hipMalloc(&q, n * sizeof(double)), "hipMalloc(field)");
+    mesh->allocs.push_back(q);
+    *p = static_cast<double*>(q);
+  }
+  HX_TRY(mesh->ctx, hipMemcpyAsync(*p, host, n * sizeof(double), hipMemcpyHostToDevice, mesh->ctx->stream), "upload");
+  HX_TRY(mesh->ctx, hipStreamSynchronize(mesh->ctx->stream), "upload sync");
+  return HX_OK;
+}
+
+hx_status hx_mesh_download(hx_mesh* mesh, int field, int cls, double* host, size_t n)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  double** p = field_ptr(mesh->dm, field, cls);
+  long long sz = field_size(mesh->dm, field, cls);
+  if (!p || sz < 0) return fail(mesh->ctx, HX_ERR_INVALID, "unknown field or class");
+  if ((long long)n != sz) return fail(mesh->ctx, HX_ERR_INVALID, "field size mismatch");
+  if (sz == 0) return HX_OK;
+  if (!*p) {
+    std::memset(host, 0, n * sizeof(double));
+    return HX_OK;
+  }
+  HX_TRY(mesh->ctx, hipMemcpyAsync(host, *p, n * sizeof(double), hipMemcpyDeviceToHost, mesh->ctx->stream), "download");
+  HX_TRY(mesh->ctx, hipStreamSynchronize(mesh->ctx->stream), "download sync");
+  return HX_OK;
+}
+
+hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  if (precision == HX_FP64) {
+    mesh->precision = precision;
+    return HX_OK;
+  }
+  if (precision == HX_FP32_MFMA) return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "FP32-MFMA form not built");
+  return fail(mesh->ctx, HX_ERR_INVALID, "unknown precision");
+}
+
+hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size_t* n)
+{
+  if (!mesh || !out || !n) return HX_ERR_INVALID;
+  double** p = field_ptr(mesh->dm, field, cls);
+  long long sz = field_size(mesh->dm, field, cls);
+  if (!p || sz < 0) return fail(mesh->ctx, HX_ERR_INVALID, "unknown field or class");
+  *out = *p;
+  *n = size_t(sz);
+  return HX_OK;
+}
+
+hx_status hx_write_face(hx_mesh* mesh, int cls, const hx_pde* pde)
+{
+  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  HX_TRY(mesh->ctx, mesh->ks->write_face(l, cls), "write_face");
+  return HX_OK;
+}
+
+hx_status hx_neighbor(hx_mesh* mesh, int cls, const hx_pde* pde, int)
+{
+  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  HX_TRY(mesh->ctx, mesh->ks->neighbor(l, cls, visc), "neighbor");
+  return HX_OK;
+}
+
+hx_status hx_local(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual, int use_filter)
+{
+  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  // HEXED_ASSERTs of the Local constructor (Spatial.hpp:322-323)
+  if (visc && i_stage)
+    return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
+  if (i_stage && compute_residual)
+    return fail(mesh->ctx, HX_ERR_INVALID, "residual calculation is a single-stage operation");
+  const double coef = i_stage ? dt * hx_basis_step_ratio(mesh->dm.rs) : dt;
+  HX_TRY(mesh->ctx, mesh->ks->local(l, cls, visc, coef, i_stage, compute_residual, use_filter), "local");
+  return HX_OK;
+}
+
+hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde*)
+{
+  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  HX_TRY(mesh->ctx, mesh->ks->neighbor_reconcile(l, cls), "neighbor_reconcile");
+  return HX_OK;
+}
+
+hx_status hx_reconcile_ldg_flux(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual,
+                                int use_filter)
+{
+  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  if (visc && i_stage)
+    return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
+  HX_TRY(mesh->ctx, mesh->ks->reconcile_ldg_flux(l, cls, dt, compute_residual, use_filter), "reconcile_ldg_flux");
+  return HX_OK;
+}
+
+hx_status hx_max_dt(hx_mesh* mesh, int cls, const hx_pde* pde, int is_local, int, double safety_conv, double safety_diff,
+                    double* dt_out)
+{
+  if (!mesh || !valid_cls(cls) || !dt_out) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  const double max_cfl_c = hx_basis_max_cfl(mesh->dm.rs) * safety_conv;
+  const double max_cfl_d = -2 / hx_basis_tables::eigenvalss[mesh->dm.rs - 2][1] * safety_diff;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, l.stream, l.scalar, DBL_MAX);
+  HX_TRY(mesh->ctx, hipGetLastError(), "fill");
+  HX_TRY(mesh->ctx, mesh->ks->max_dt(l, cls, visc, is_local, max_cfl_c, max_cfl_d), "max_dt");
+  double dt = DBL_MAX;
+  HX_TRY(mesh->ctx, hipMemcpyAsync(&dt, l.scalar, sizeof(double), hipMemcpyDeviceToHost, l.stream), "max_dt readback");
+  HX_TRY(mesh->ctx, hipStreamSynchronize(l.stream), "max_dt sync");
+  *dt_out = is_local ? 1. : dt;
+  return HX_OK;
+}
+
+hx_status hx_prolong_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int offset)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  HX_TRY(mesh->ctx, mesh->ks->prolong(l, mesh->dm.nv, scale, offset), "prolong");
+  return HX_OK;
+}
+
+hx_status hx_restrict_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int offset)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  HX_TRY(mesh->ctx, mesh->ks->restrict_(l, mesh->dm.nv, scale, offset), "restrict");
+  return HX_OK;
+}
+
+hx_status hx_apply_state_bcs(hx_mesh* mesh)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  HX_TRY(mesh->ctx, mesh->ks->state_bcs(make_launch(mesh)), "state_bcs");
+  return HX_OK;
+}
+
+hx_status hx_apply_flux_bcs(hx_mesh* mesh)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  HX_TRY(mesh->ctx, mesh->ks->flux_bcs(make_launch(mesh)), "flux_bcs");
+  return HX_OK;
+}
+
+hx_status hx_compute_euler(hx_mesh* mesh, const hx_kernel_options* o)
+{
+  if (!mesh || !o) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_EULER;
+  hx_status s;
+  if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, o->i_stage))) return s;
+  if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, o->i_stage))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
+  if ((s = hx_local(mesh, HX_CARTESIAN, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_local(mesh, HX_DEFORMED, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
+  return hx_prolong_refined(mesh, &p, 0, 0);
+}
+
+hx_status hx_compute_navier_stokes(hx_mesh* mesh, const hx_kernel_options* o, hx_transport visc, hx_transport cond)
+{
+  if (!mesh || !o) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_NAVIER_STOKES;
+  p.visc = visc;
+  p.cond = cond;
+  if (o->i_stage)
+    return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
+  hx_status s;
+  if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, o->i_stage))) return s;
+  if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, o->i_stage))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 0, 1))) return s;
+  if ((s = hx_local(mesh, HX_CARTESIAN, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_local(mesh, HX_DEFORMED, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
+  if (!o->i_stage) {
+    if ((s = hx_prolong_refined(mesh, &p, 1, 1))) return s;
+    if ((s = hx_apply_flux_bcs(mesh))) return s;
+    if ((s = hx_neighbor_reconcile(mesh, HX_CARTESIAN, &p))) return s;
+    if ((s = hx_neighbor_reconcile(mesh, HX_DEFORMED, &p))) return s;
+    if ((s = hx_restrict_refined(mesh, &p, 1, 1))) return s;
+    if ((s = hx_reconcile_ldg_flux(mesh, HX_CARTESIAN, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
+    if ((s = hx_reconcile_ldg_flux(mesh, HX_DEFORMED, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
+  }
+  return hx_prolong_refined(mesh, &p, 0, 0);
+}
+
+hx_status hx_max_dt_euler(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time, double* dt_out)
+{
+  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_EULER;
+  double a, b;
+  hx_status s;
+  const int uf = o ? o->use_filter : 0;
+  if ((s = hx_max_dt(mesh, HX_CARTESIAN, &p, local_time, uf, msc, msd, &a))) return s;
+  if ((s = hx_max_dt(mesh, HX_DEFORMED, &p, local_time, uf, msc, msd, &b))) return s;
+  *dt_out = std::min(a, b);
+  return HX_OK;
+}
+
+hx_status hx_max_dt_navier_stokes(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time,
+                                  hx_transport visc, hx_transport cond, double* dt_out)
+{
+  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_NAVIER_STOKES;
+  p.visc = visc;
+  p.cond = cond;
+  double a, b;
+  hx_status s;
+  const int uf = o ? o->use_filter : 0;
+  if ((s = hx_max_dt(mesh, HX_CARTESIAN, &p, local_time, uf, msc, msd, &a))) return s;
+  if ((s = hx_max_dt(mesh, HX_DEFORMED, &p, local_time, uf, msc, msd, &b))) return s;
+  *dt_out = std::min(a, b);
+  return HX_OK;
+}
+
+hx_status hx_compute_write_face(hx_mesh* mesh)
+{
+  hx_status s;
+  if ((s = hx_write_face(mesh, HX_CARTESIAN, nullptr))) return s;
+  return hx_write_face(mesh, HX_DEFORMED, nullptr);
+}
+
+hx_status hx_compute_prolong(hx_mesh* mesh, int scale, int offset) { return hx_prolong_refined(mesh, nullptr, scale, offset); }
+hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset) { return hx_restrict_refined(mesh, nullptr, scale, offset); }
+
+// Solver::update (Solver.cpp:834-886); fix_admissibility is off by default (Solver.cpp:143) and not on this path
+hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
+{
+  if (!mesh || !o || !st) return HX_ERR_INVALID;
+  for (int i_flow = 0; i_flow < o->flow_iters; ++i_flow) {
+    const double safety = o->max_safety;
+    const double n_cheby = o->n_cheby_flow;
+    // math::chebyshev_step (math.cpp:104-107)
+    auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
+    const double max_cheby = cheby(n_cheby, n_cheby - 1);
+    for (int i_cheby = 0; i_cheby < n_cheby; ++i_cheby) {
+      hx_kernel_options ko{0., 0, 0, o->use_filter};
+      double mdt;
+      hx_status s = o->use_ldg
+                        ? hx_max_dt_navier_stokes(mesh, &ko, safety / max_cheby, safety, o->local_time, o->visc, o->cond, &mdt)
+                        : hx_max_dt_euler(mesh, &ko, safety / max_cheby, safety, o->local_time, &mdt);
+      if (s) return s;
+      const double dt = std::min(mdt, o->max_time_step) * cheby(n_cheby, i_cheby);
+      for (int i = 0; i < 2; ++i) {
+        hx_kernel_options opts{dt, i, 0, o->use_filter};
+        if ((s = hx_apply_state_bcs(mesh))) return s;
+        if (o->use_ldg && !i) s = hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
+        else s = hx_compute_euler(mesh, &opts);
+        if (s) return s;
+      }
+      st->time_step = dt;
+      st->flow_time += dt;
+    }
+  }
+  ++st->iteration;
+  return HX_OK;
+}
+
+// Solver::compute_residual (Solver.cpp:899-913)
+hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* o)
+{
+  if (!mesh || !o) return HX_ERR_INVALID;
+  hx_status s = hx_apply_state_bcs(mesh);
+  if (s) return s;
+  hx_kernel_options opts{1., 0, 1, o->use_filter};
+  if (o->use_ldg) return hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
+  return hx_compute_euler(mesh, &opts);
+}
+
+}  // extern "C"

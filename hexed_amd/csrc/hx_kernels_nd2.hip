@@ -1,0 +1,3 @@
+// Instantiates the DG kernels for n_dim = 2, row sizes 2..8.
+#include "hx_kernels.hpp"
+HX_DEFINE_KERNEL_SET(2)

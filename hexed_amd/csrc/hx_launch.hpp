@@ -1,0 +1,39 @@
+// Host-side launch interface between the C ABI (hx_api.cpp) and the templated gfx950 kernels
+// (hx_kernels_nd*.hip). One Kernel_set exists per (n_dim, row_size).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hx_device.hpp"
+
+namespace hx {
+
+struct Launch
+{
+  hipStream_t stream;
+  Dev_mesh m;
+  const void* ops;   // device Ops<row_size>
+  Ns_params ns;      // transport models (Navier-Stokes)
+  double* scalar;    // device scratch: [0] time step reduction
+  int precision;     // hx_precision
+};
+
+struct Kernel_set
+{
+  hipError_t (*write_face)(const Launch&, int cls);
+  hipError_t (*neighbor)(const Launch&, int cls, bool visc);
+  hipError_t (*local)(const Launch&, int cls, bool visc, double coef, int stage, int compute_residual, int use_filter);
+  hipError_t (*neighbor_reconcile)(const Launch&, int cls);
+  hipError_t (*reconcile_ldg_flux)(const Launch&, int cls, double dt, int compute_residual, int use_filter);
+  hipError_t (*max_dt)(const Launch&, int cls, bool visc, int is_local, double max_cfl_c, double max_cfl_d);
+  hipError_t (*prolong)(const Launch&, int n_var, int scale, int offset);
+  hipError_t (*restrict_)(const Launch&, int n_var, int scale, int offset);
+  hipError_t (*state_bcs)(const Launch&);
+  hipError_t (*flux_bcs)(const Launch&);
+};
+
+const Kernel_set* kernel_set_nd1(int rs);
+const Kernel_set* kernel_set_nd2(int rs);
+const Kernel_set* kernel_set_nd3(int rs);
+
+}  // namespace hx

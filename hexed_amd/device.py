@@ -1,0 +1,155 @@
+"""Python handle on the C ABI: one HIP context plus one device mesh built from a `mesh.HostMesh`.
+
+This is the host-side mirror used by the tests, the smoke check and the benchmark; it calls only the C ABI of
+libhexed_amd.so (include/hexed_amd.h) and fails loudly when the library or a GPU is missing.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+FIELDS = {
+    "state": abi.HX_FIELD_STATE, "rcache": abi.HX_FIELD_RCACHE, "tss": abi.HX_FIELD_TSS, "av": abi.HX_FIELD_AV,
+    "vertex_tss": abi.HX_FIELD_VERTEX_TSS, "metrics": abi.HX_FIELD_METRICS, "faces": abi.HX_FIELD_FACES,
+    "normals": abi.HX_FIELD_NORMALS,
+}
+
+
+class HxError(RuntimeError):
+    pass
+
+
+class Context:
+    def __init__(self, device=0):
+        self.lib = abi.load_library()
+        self.ptr = ctypes.c_void_p()
+        rc = self.lib.hx_ctx_create(device, ctypes.byref(self.ptr))
+        if rc != abi.HX_OK:
+            raise HxError(f"hx_ctx_create failed with status {rc}")
+
+    def check(self, rc, what=""):
+        if rc != abi.HX_OK:
+            raise HxError(f"{what}: status {rc}: {self.lib.hx_last_error(self.ptr).decode()}")
+
+    def synchronize(self):
+        self.check(self.lib.hx_ctx_synchronize(self.ptr), "synchronize")
+
+    @property
+    def stream(self):
+        return self.lib.hx_ctx_stream(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            self.lib.hx_ctx_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Device_mesh:
+    """A mesh resident in device memory; `host` is the HostMesh it was created from (and downloads go into)."""
+
+    def __init__(self, ctx, host):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.host = host
+        self.ptr = ctypes.c_void_p()
+        self._struct = host.struct()
+        ctx.check(self.lib.hx_mesh_create(ctx.ptr, ctypes.byref(self._struct), ctypes.byref(self.ptr)), "hx_mesh_create")
+
+    def close(self):
+        if self.ptr:
+            self.lib.hx_mesh_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, precision):
+        self.ctx.check(self.lib.hx_mesh_set_precision(self.ptr, precision), "set_precision")
+
+    def _field_array(self, name, cls):
+        h = self.host
+        if name in ("faces", "normals", "metrics"):
+            return getattr(h, name)
+        arr = getattr(h, name)[cls]
+        return arr
+
+    def download(self, name, cls=0, out=None):
+        arr = self._field_array(name, cls) if out is None else out
+        if arr is None:
+            n = ctypes.c_size_t()
+            p = ctypes.c_void_p()
+            self.ctx.check(self.lib.hx_mesh_device_ptr(self.ptr, FIELDS[name], cls, ctypes.byref(p), ctypes.byref(n)))
+            arr = np.zeros(n.value)
+        self.ctx.check(self.lib.hx_mesh_download(self.ptr, FIELDS[name], cls, abi.dptr(arr), arr.size), f"download {name}")
+        return arr
+
+    def upload(self, name, cls=0, arr=None):
+        arr = self._field_array(name, cls) if arr is None else np.ascontiguousarray(arr, dtype=np.float64)
+        self.ctx.check(self.lib.hx_mesh_upload(self.ptr, FIELDS[name], cls, abi.dptr(arr), arr.size), f"upload {name}")
+
+    # kernels -------------------------------------------------------------------------------------------------
+    def write_face(self, cls, pde=None):
+        self.ctx.check(self.lib.hx_write_face(self.ptr, cls, ctypes.byref(pde or abi.euler_pde())), "write_face")
+
+    def neighbor(self, cls, pde=None, i_stage=0):
+        self.ctx.check(self.lib.hx_neighbor(self.ptr, cls, ctypes.byref(pde or abi.euler_pde()), i_stage), "neighbor")
+
+    def local(self, cls, dt, i_stage=0, compute_residual=False, use_filter=False, pde=None):
+        self.ctx.check(self.lib.hx_local(self.ptr, cls, ctypes.byref(pde or abi.euler_pde()), dt, i_stage,
+                                         int(compute_residual), int(use_filter)), "local")
+
+    def neighbor_reconcile(self, cls, pde):
+        self.ctx.check(self.lib.hx_neighbor_reconcile(self.ptr, cls, ctypes.byref(pde)), "neighbor_reconcile")
+
+    def reconcile_ldg_flux(self, cls, pde, dt, compute_residual=False, use_filter=False):
+        self.ctx.check(self.lib.hx_reconcile_ldg_flux(self.ptr, cls, ctypes.byref(pde), dt, 0, int(compute_residual),
+                                                      int(use_filter)), "reconcile_ldg_flux")
+
+    def max_dt(self, cls, pde=None, is_local=False, safety_conv=0.7, safety_diff=0.7):
+        out = ctypes.c_double()
+        self.ctx.check(self.lib.hx_max_dt(self.ptr, cls, ctypes.byref(pde or abi.euler_pde()), int(is_local), 0,
+                                          safety_conv, safety_diff, ctypes.byref(out)), "max_dt")
+        return out.value
+
+    def prolong(self, scale=False, offset=False, pde=None):
+        self.ctx.check(self.lib.hx_prolong_refined(self.ptr, ctypes.byref(pde or abi.euler_pde()), int(scale),
+                                                   int(offset)), "prolong")
+
+    def restrict(self, scale=True, offset=False, pde=None):
+        self.ctx.check(self.lib.hx_restrict_refined(self.ptr, ctypes.byref(pde or abi.euler_pde()), int(scale),
+                                                    int(offset)), "restrict")
+
+    def apply_state_bcs(self):
+        self.ctx.check(self.lib.hx_apply_state_bcs(self.ptr), "state_bcs")
+
+    def apply_flux_bcs(self):
+        self.ctx.check(self.lib.hx_apply_flux_bcs(self.ptr), "flux_bcs")
+
+    def compute_euler(self, dt, i_stage, compute_residual=False, use_filter=False):
+        o = abi.hx_kernel_options(dt, i_stage, int(compute_residual), int(use_filter))
+        self.ctx.check(self.lib.hx_compute_euler(self.ptr, ctypes.byref(o)), "compute_euler")
+
+    def compute_navier_stokes(self, dt, visc, cond, compute_residual=False, use_filter=False):
+        o = abi.hx_kernel_options(dt, 0, int(compute_residual), int(use_filter))
+        self.ctx.check(self.lib.hx_compute_navier_stokes(self.ptr, ctypes.byref(o), visc, cond), "compute_navier_stokes")
+
+    def compute_write_face(self):
+        self.ctx.check(self.lib.hx_compute_write_face(self.ptr), "compute_write_face")
+
+    def update(self, opts, status=None):
+        status = status if status is not None else abi.hx_update_status()
+        self.ctx.check(self.lib.hx_update(self.ptr, ctypes.byref(opts), ctypes.byref(status)), "update")
+        return status
+
+    def compute_residual(self, opts):
+        self.ctx.check(self.lib.hx_compute_residual(self.ptr, ctypes.byref(opts)), "compute_residual")

@@ -1,0 +1,354 @@
+"""Host-side mesh description (the `hx_host_mesh` of include/hexed_amd.h) and builders for the box meshes the
+benchmarks and tests use.
+
+The reference builds meshes through `Mesh::add_element` / `connect_*` (Mesh.hpp:24-230) and derives geometry in
+`Solver::calc_jacobian` (Solver.cpp:273-381) and `Deformed_element::set_jacobian` (Deformed_element.cpp:60-136).
+Mesh construction is setup-time work outside the accelerated path; this module produces the same kernel-visible
+data directly:
+  - element state / time step scale / nominal size / vertex time-step scale / face slots;
+  - connection slots and packed `Connection_direction`s, Cartesian connections oriented {{d, d}, {1, 0}}
+    (connection.hpp:29-36), deformed ones as given, boundary connections {{d, d}, {s, !s}} (connection.hpp:377);
+  - for deformed elements, the reference-level normals and Jacobian determinant at every quadrature point and the
+    area-weighted face normals, shared across each connection by the sign-corrected average of
+    Solver.cpp:327-352, and the vertex time-step scale of Deformed_element.cpp:116-135 reduced by minimum over
+    the elements sharing each vertex (Solver.cpp:380, Vertex::vector_min).
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+from .basis_gen import load_tables
+
+_TABLES = None
+
+
+def basis(rs):
+    global _TABLES
+    if _TABLES is None:
+        _TABLES = load_tables()
+    return _TABLES[rs]
+
+
+def pack_dir(i_dim0, i_dim1, sign0, sign1):
+    return int(i_dim0) | (int(i_dim1) << 2) | (int(sign0) << 4) | (int(sign1) << 5)
+
+
+def unpack_dir(p):
+    return (p & 3, (p >> 2) & 3), (bool((p >> 4) & 1), bool((p >> 5) & 1))
+
+
+def flip_normal(face_sign, i_side):
+    return face_sign[i_side] == bool(i_side)
+
+
+def face_permutation_index(nd, rs, packed):
+    """Index map `perm` with match_faces(face1)[i] = face1[perm[i]] (Spatial.hpp:85-129): transpose, then flip."""
+    i_dim, face_sign = unpack_dir(packed)
+    nfq = rs ** (nd - 1)
+    idx = np.arange(nfq)
+    transpose = (i_dim[0] == 0 and i_dim[1] == 2) or (i_dim[0] == 2 and i_dim[1] == 0)
+    flip_t = (i_dim[0] != i_dim[1]) and (flip_normal(face_sign, 0) == flip_normal(face_sign, 1))
+    if nd == 3:
+        m = idx.reshape(rs, rs)  # m[c, r] = index of column-major (r, c)
+        if transpose:
+            m = m.T
+        if flip_t:
+            colwise = (i_dim[0] > 3 - i_dim[0] - i_dim[1]) != transpose
+            m = m[:, ::-1] if colwise else m[::-1, :]
+        return np.ascontiguousarray(m).reshape(-1)
+    if nd == 2 and flip_t:
+        return idx[::-1].copy()
+    return idx
+
+
+class HostMesh:
+    """numpy arrays laid out as `hx_host_mesh`; `struct()` returns the ctypes view (arrays must stay alive)."""
+
+    def __init__(self, nd, rs, n_car, n_def, n_slot, n_nslot, n_con_car, n_con_def, n_ref=0, with_av=False):
+        self.n_dim, self.row_size = nd, rs
+        self.nq, self.nfq, self.nv, self.nvert = rs ** nd, rs ** (nd - 1), nd + 2, 2 ** nd
+        self.face_sz = max(2 * self.nv, nd + rs) * self.nfq
+        self.n_elem = [n_car, n_def]
+        self.state = [np.zeros((n, self.nv, self.nq)) for n in self.n_elem]
+        self.rcache = [np.zeros((n, self.nv, self.nq)) for n in self.n_elem]
+        self.tss = [np.ones((n, self.nq)) for n in self.n_elem]
+        self.av = [np.zeros((n, 2, self.nq)) for n in self.n_elem] if with_av else [None, None]
+        self.nom_size = [np.zeros(n) for n in self.n_elem]
+        self.vertex_tss = [np.zeros((n, self.nvert)) for n in self.n_elem]
+        self.face_slot = [np.zeros((n, 2 * nd), np.int32) for n in self.n_elem]
+        self.metrics = np.zeros((n_def, nd * nd + 1, self.nq))
+        self.face_nslot = np.full((n_def, 2 * nd), -1, np.int32)
+        self.faces = np.zeros((n_slot, self.face_sz))
+        self.normals = np.zeros((n_nslot, nd, self.nfq))
+        self.con_slot = [np.zeros((n_con_car, 2), np.int32), np.zeros((n_con_def, 2), np.int32)]
+        self.con_dir = [np.zeros(n_con_car, np.int32), np.zeros(n_con_def, np.int32)]
+        self.con_nslot = np.zeros((n_con_def, 2), np.int32)
+        self.ref_slot = np.full((n_ref, 5), -1, np.int32)
+        self.ref_stretch = np.zeros((n_ref, 2), np.int32)
+        self.bc_con = np.zeros(0, np.int32)
+        self.bc_serial = np.zeros(0, np.int32)
+        self.bcs = []
+        # geometry kept for initial conditions: qpoint positions per class [n][nd][nq]
+        self.qpos = [None, None]
+
+    @property
+    def n_dof(self):
+        return sum(self.n_elem) * self.nq
+
+    def copy(self):
+        out = HostMesh.__new__(HostMesh)
+        for k, v in self.__dict__.items():
+            if isinstance(v, np.ndarray):
+                setattr(out, k, v.copy())
+            elif isinstance(v, list):
+                setattr(out, k, [x.copy() if isinstance(x, np.ndarray) else x for x in v])
+            else:
+                setattr(out, k, v)
+        return out
+
+    def struct(self):
+        m = abi.hx_host_mesh()
+        m.n_dim, m.row_size = self.n_dim, self.row_size
+        for c in range(2):
+            m.n_elem[c] = self.n_elem[c]
+            m.state[c] = abi.dptr(self.state[c])
+            m.rcache[c] = abi.dptr(self.rcache[c])
+            m.tss[c] = abi.dptr(self.tss[c])
+            m.av[c] = abi.dptr(self.av[c])
+            m.nom_size[c] = abi.dptr(self.nom_size[c])
+            m.vertex_tss[c] = abi.dptr(self.vertex_tss[c])
+            m.face_slot[c] = abi.iptr(self.face_slot[c])
+            m.n_con[c] = len(self.con_dir[c])
+            m.con_slot[c] = abi.iptr(self.con_slot[c])
+            m.con_dir[c] = abi.iptr(self.con_dir[c])
+        m.metrics = abi.dptr(self.metrics)
+        m.face_nslot = abi.iptr(self.face_nslot)
+        m.n_slot = self.faces.shape[0]
+        m.faces = abi.dptr(self.faces)
+        m.n_nslot = self.normals.shape[0]
+        m.normals = abi.dptr(self.normals)
+        m.con_nslot = abi.iptr(self.con_nslot)
+        m.n_ref = self.ref_slot.shape[0]
+        m.ref_slot = abi.iptr(self.ref_slot)
+        m.ref_stretch = abi.iptr(self.ref_stretch)
+        m.n_bc_con = len(self.bc_con)
+        m.bc_con = abi.iptr(self.bc_con)
+        m.bc_serial = abi.iptr(self.bc_serial)
+        self._bc_arr = (abi.hx_bc * max(1, len(self.bcs)))()
+        for i, (kind, st) in enumerate(self.bcs):
+            self._bc_arr[i].kind = kind
+            for k, v in enumerate(st):
+                self._bc_arr[i].state[k] = v
+        m.n_bc = len(self.bcs)
+        m.bcs = ctypes.cast(self._bc_arr, ctypes.POINTER(abi.hx_bc))
+        return m
+
+
+# ---- geometry helpers -----------------------------------------------------------------------------------------
+
+def node_coords(nd, rs):
+    """[nq][nd] reference coordinates of the quadrature points (dimension 0 slowest)."""
+    nodes = basis(rs)["node"]
+    q = np.arange(rs ** nd)
+    return np.stack([nodes[(q // rs ** (nd - 1 - d)) % rs] for d in range(nd)], axis=1)
+
+
+def vertex_weights(nd, rs):
+    """[nq][nvert] n-linear interpolation weights; vertex index bit (nd-1-d) is the coordinate along d."""
+    x = node_coords(nd, rs)
+    w = np.ones((rs ** nd, 2 ** nd))
+    for v in range(2 ** nd):
+        for d in range(nd):
+            bit = (v >> (nd - 1 - d)) & 1
+            w[:, v] *= x[:, d] if bit else 1 - x[:, d]
+    return w
+
+
+def _along(mat, arr, axis):
+    """Apply `mat` (m x rs) along `axis` of `arr`."""
+    moved = np.moveaxis(arr, axis, -1)
+    return np.moveaxis(moved @ mat.T, -1, axis)
+
+
+def _cofactor(J, r, c):
+    """Signed cofactor (r, c) of the stacked nd x nd matrices J[..., i, j] = J[i][j] given as J[i][j] arrays."""
+    nd = len(J)
+    if nd == 1:
+        return np.ones_like(J[0][0])
+    if nd == 2:
+        return (1 - 2 * ((r + c) % 2)) * J[1 - r][1 - c]
+    r1, r2, c1, c2 = (r + 1) % 3, (r + 2) % 3, (c + 1) % 3, (c + 2) % 3
+    return J[r1][c1] * J[r2][c2] - J[r1][c2] * J[r2][c1]
+
+
+def deformed_metrics(nd, rs, vert_pos, nom_size):
+    """Deformed_element::set_jacobian restated (Deformed_element.cpp:60-136) for elements whose faces are not
+    warped (node adjustments zero), vectorised over elements.
+
+    vert_pos: [n][nvert][nd]. Returns (metrics [n][nd*nd+1][nq], face_normals [n][2nd][nd][nfq],
+    vertex_tss [n][nvert], qpos [n][nd][nq]).
+    """
+    b = basis(rs)
+    n = vert_pos.shape[0]
+    nq = rs ** nd
+    W = vertex_weights(nd, rs)                                    # [nq][nvert]
+    qpos = np.einsum("qv,nvd->ndq", W, vert_pos)                   # [n][nd][nq]
+    shape = (n,) + (rs,) * nd
+    J = [[None] * nd for _ in range(nd)]
+    for i in range(nd):
+        p = qpos[:, i].reshape(shape)
+        for j in range(nd):
+            J[i][j] = _along(b["diff_mat"], p, 1 + j) / nom_size.reshape((n,) + (1,) * nd)
+    det = sum(J[0][c] * _cofactor(J, 0, c) for c in range(nd))
+    metrics = np.zeros((n, nd * nd + 1, nq))
+    for i in range(nd):
+        for j in range(nd):
+            metrics[:, i * nd + j] = _cofactor(J, j, i).reshape(n, nq)
+    metrics[:, nd * nd] = det.reshape(n, nq)
+    nfq = rs ** (nd - 1)
+    fn = np.zeros((n, 2 * nd, nd, nfq))
+    for i in range(nd):
+        for s in range(2):
+            FJ = [[_along(b["boundary"][s:s + 1], J[a][c], 1 + i) for c in range(nd)] for a in range(nd)]
+            for j in range(nd):
+                fn[:, 2 * i + s, j] = _cofactor(FJ, j, i).reshape(n, nfq)
+    # vertex time step scale: extrapolate normals and determinant to the vertices
+    def to_vertices(arr):
+        a = arr.reshape(shape)
+        for d in range(nd):
+            a = _along(b["boundary"], a, 1 + d)
+        return a.reshape(n, 2 ** nd)
+    vdet = to_vertices(metrics[:, nd * nd])
+    norm_sum = np.zeros((n, 2 ** nd))
+    for i in range(nd):
+        sq = np.zeros((n, 2 ** nd))
+        for j in range(nd):
+            sq += to_vertices(metrics[:, i * nd + j]) ** 2
+        norm_sum += np.sqrt(sq)
+    vtss = nom_size[:, None] * vdet / norm_sum
+    return metrics, fn, vtss, qpos
+
+
+# ---- box builder ----------------------------------------------------------------------------------------------
+
+def box(nd, rs, n, lo=0.0, hi=1.0, deformed=False, jitter=0.0, seed=0, bc=(abi.HX_BC_FREESTREAM, None), with_av=False):
+    """A box of n^nd equal elements on [lo, hi]^nd with one flow boundary condition on every boundary face.
+
+    Cartesian elements are connected by Cartesian connections; deformed elements by deformed connections in the
+    standard orientation {{d, d}, {1, 0}}. With `deformed`, interior vertices are moved by seeded uniform
+    jitter*h per component (boundary vertices stay put). Element order is row-major over the grid position
+    (dimension 0 slowest), matching the test meshes of test_Solver.cpp:416-456.
+    """
+    n = int(n)
+    h = (hi - lo) / n
+    n_e = n ** nd
+    grid = np.stack(np.unravel_index(np.arange(n_e), (n,) * nd), axis=1)  # [n_e][nd]
+    strides = [n ** (nd - 1 - d) for d in range(nd)]
+    # connections between grid neighbours: for each dim, elements with index < n-1 along d
+    con_a, con_b, con_d = [], [], []
+    for d in range(nd):
+        a = np.nonzero(grid[:, d] < n - 1)[0]
+        con_a.append(a)
+        con_b.append(a + strides[d])
+        con_d.append(np.full(len(a), d))
+    con_a, con_b, con_d = np.concatenate(con_a), np.concatenate(con_b), np.concatenate(con_d)
+    n_int = len(con_a)
+    bnd_e, bnd_d, bnd_s = [], [], []
+    for d in range(nd):
+        for s in (0, 1):
+            e = np.nonzero(grid[:, d] == (n - 1 if s else 0))[0]
+            bnd_e.append(e)
+            bnd_d.append(np.full(len(e), d))
+            bnd_s.append(np.full(len(e), s))
+    bnd_e, bnd_d, bnd_s = np.concatenate(bnd_e), np.concatenate(bnd_d), np.concatenate(bnd_s)
+    n_bnd = len(bnd_e)
+    cls = abi.HX_DEFORMED if deformed else abi.HX_CARTESIAN
+    n_con_car = 0 if deformed else n_int
+    n_con_def = (n_int if deformed else 0) + n_bnd
+    n_slot = 2 * (n_int + n_bnd)
+    n_nslot = 2 * n_con_def
+    m = HostMesh(nd, rs, 0 if deformed else n_e, n_e if deformed else 0, n_slot, n_nslot, n_con_car, n_con_def,
+                 with_av=with_av)
+    # slots: interior connections first (2 each), then boundary connections (inside, ghost)
+    int_slots = np.arange(2 * n_int, dtype=np.int32).reshape(n_int, 2)
+    bnd_slots = (2 * n_int + np.arange(2 * n_bnd, dtype=np.int32)).reshape(n_bnd, 2)
+    fs = m.face_slot[cls]
+    fs[con_a, 2 * con_d + 1] = int_slots[:, 0]
+    fs[con_b, 2 * con_d] = int_slots[:, 1]
+    fs[bnd_e, 2 * bnd_d + bnd_s] = bnd_slots[:, 0]
+    std_dir = np.array([pack_dir(d, d, 1, 0) for d in range(nd)], np.int32)
+    bnd_dir = np.array([pack_dir(d, d, s, 1 - s) for d, s in zip(bnd_d, bnd_s)], np.int32)
+    if deformed:
+        m.con_slot[1][:n_int] = int_slots
+        m.con_dir[1][:n_int] = std_dir[con_d]
+        off = n_int
+    else:
+        m.con_slot[0][:] = int_slots
+        m.con_dir[0][:] = std_dir[con_d]
+        off = 0
+    m.con_slot[1][off:] = bnd_slots
+    m.con_dir[1][off:] = bnd_dir
+    m.con_nslot[:] = np.arange(n_nslot, dtype=np.int32).reshape(-1, 2)
+    m.bc_con = np.arange(off, off + n_bnd, dtype=np.int32)
+    m.bc_serial = np.zeros(n_bnd, np.int32)
+    kind, fs_state = bc
+    m.bcs = [(kind, list(fs_state) if fs_state is not None else [0.0] * (nd + 2))]
+    m.nom_size[cls][:] = h
+    # geometry
+    rng = np.random.default_rng(seed)
+    vgrid = np.stack(np.unravel_index(np.arange((n + 1) ** nd), (n + 1,) * nd), axis=1)
+    vpos = lo + vgrid * h
+    if deformed and jitter > 0:
+        interior = np.all((vgrid > 0) & (vgrid < n), axis=1)
+        vpos[interior] += rng.uniform(-jitter * h, jitter * h, size=(int(interior.sum()), nd))
+    vstrides = np.array([(n + 1) ** (nd - 1 - d) for d in range(nd)])
+    corner = np.stack(np.unravel_index(np.arange(2 ** nd), (2,) * nd), axis=1)  # vertex index -> offsets
+    vid = (grid[:, None, :] + corner[None, :, :]) @ vstrides                 # [n_e][nvert] global vertex ids
+    if deformed:
+        metrics, fn, vtss, qpos = deformed_metrics(nd, rs, vpos[vid], m.nom_size[1])
+        m.metrics[:] = metrics
+        m.qpos[1] = qpos
+        # connection normals: average of the two element face normals with sign correction (Solver.cpp:327-352);
+        # standard orientation needs no permutation and has sign +1 on both sides
+        nrm = np.zeros((n_con_def, 2, nd, m.nfq))
+        n0 = fn[con_a, 2 * con_d + 1]
+        n1 = fn[con_b, 2 * con_d]
+        avg = 0.5 * n0 + 0.5 * n1
+        nrm[:n_int, 0] = avg
+        nrm[:n_int, 1] = avg
+        # boundary: ghost face holds a copy of the inside normal, both sides flipped iff the inside face is negative
+        nin = fn[bnd_e, 2 * bnd_d + bnd_s]
+        sgn = np.where(bnd_s == 0, -1.0, 1.0)[:, None, None]
+        avgb = 0.5 * sgn * nin + 0.5 * sgn * nin
+        nrm[n_int:, 0] = sgn * avgb
+        nrm[n_int:, 1] = sgn * avgb
+        m.normals[:] = nrm.reshape(n_nslot, nd, m.nfq)
+        m.face_nslot[con_a, 2 * con_d + 1] = m.con_nslot[:n_int, 0]
+        m.face_nslot[con_b, 2 * con_d] = m.con_nslot[:n_int, 1]
+        m.face_nslot[bnd_e, 2 * bnd_d + bnd_s] = m.con_nslot[n_int:, 0]
+        elem_vtss = vtss
+    else:
+        nodes = node_coords(nd, rs)
+        m.qpos[0] = ((nodes[None, :, :] + grid[:, None, :]) * h + lo).transpose(0, 2, 1).copy()
+        # boundary connections of Cartesian elements carry the unit normal of Element::set_jacobian (Element.cpp:99-112)
+        unit = np.zeros((n_bnd, nd, m.nfq))
+        unit[np.arange(n_bnd), bnd_d] = 1.0
+        m.normals[:] = np.repeat(unit, 2, axis=0)
+        elem_vtss = np.full((n_e, 2 ** nd), h / nd)  # Element.cpp:17
+    # share vertex time step scale by minimum over the elements using each vertex (Solver.cpp:380)
+    vmin = np.full((n + 1) ** nd, np.inf)
+    np.minimum.at(vmin, vid.reshape(-1), elem_vtss.reshape(-1))
+    m.vertex_tss[cls][:] = vmin[vid]
+    return m
+
+
+def set_state(m, func):
+    """Fill the state of every element from func(pos[nd][..]) -> list of nv arrays (Solver::initialize)."""
+    for c in range(2):
+        if m.n_elem[c] == 0:
+            continue
+        pos = m.qpos[c]
+        vals = func([pos[:, d] for d in range(m.n_dim)])
+        for v in range(m.nv):
+            m.state[c][:, v] = vals[v]

@@ -1,0 +1,274 @@
+/*
+ * hexed_amd C ABI: the MI355X (gfx950) implementation of hexed's DG spatial discretization and explicit
+ * time step, behind plain pointers and sizes.
+ *
+ * Every kernel entry point replaces one reference kernel class or free function on the hot path; the
+ * citation next to each names it (paths relative to the hexed source tree). Adapters that implement the
+ * reference's own C++ interfaces (kernel_factory, Spatial<Pde, is_deformed>::Local/Neighbor/..., Solver::update)
+ * sit on top of this ABI in hexed_amd/host.
+ *
+ * Data model. A mesh is described once by `hx_host_mesh` (host arrays, element-major, the reference's
+ * per-element layouts) and copied to device memory by `hx_mesh_create`; afterwards all kernels run on the
+ * device copy, stream-ordered on the context's stream, and fields move with hx_mesh_upload/hx_mesh_download.
+ *
+ * Errors. Every call returns an `hx_status`; on failure `hx_last_error` returns the message. Argument
+ * errors that the reference raises as std::runtime_error (kernel_factory.hpp:114-116, HEXED_ASSERT in
+ * Spatial.hpp:322-323,539-540) return HX_ERR_INVALID with the reference's message text.
+ *
+ * Threading: one host thread per context; calls on one mesh are not re-entrant (face buffers are shared).
+ */
+#ifndef HEXED_AMD_H_
+#define HEXED_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hx_status {
+  HX_OK = 0,
+  HX_ERR_INVALID = 1,     /* bad argument (the reference throws std::runtime_error) */
+  HX_ERR_HIP = 2,         /* HIP runtime failure */
+  HX_ERR_NOT_FINITE = 3,  /* non-finite state detected */
+  HX_ERR_UNSUPPORTED = 4, /* combination not built */
+  HX_ERR_NO_DEVICE = 5
+} hx_status;
+
+/* Limits of kernel_factory (kernel_factory.hpp:111-119; config::max_row_size = 8, CMakeLists.txt:74-77). */
+#define HX_MAX_N_DIM 3
+#define HX_MIN_ROW_SIZE 2
+#define HX_MAX_ROW_SIZE 8
+
+/* Element / connection classes: the reference keeps Cartesian and deformed elements in separate sequences
+ * (Kernel_mesh.hpp:14-25). Boundary connections always belong to the deformed class (connection.hpp:321-327). */
+#define HX_CARTESIAN 0
+#define HX_DEFORMED 1
+
+/* Transport_model (Transport_model.hpp:14-56): coefficient(sqrt_T) =
+ * const_val + ref_val*(sqrt_T/sqrt(ref_temp))^3*(ref_temp + temp_offset)/(sqrt_T^2 + temp_offset). */
+typedef struct hx_transport {
+  double const_val;
+  double ref_val;
+  double ref_temp;
+  double temp_offset;
+  int is_viscous;
+} hx_transport;
+
+/* The PDE a kernel is instantiated with (the Pde_templ argument of Spatial, pde.hpp). */
+typedef enum hx_pde_kind {
+  HX_PDE_EULER = 0,          /* pde::Navier_stokes<false>::Pde  (pde.hpp:27-259) */
+  HX_PDE_NAVIER_STOKES = 1,  /* pde::Navier_stokes<true>::Pde */
+  HX_PDE_ADVECTION = 2,      /* pde::Advection                  (pde.hpp:265-349) */
+  HX_PDE_SMOOTH_AV = 3,      /* pde::Smooth_art_visc            (pde.hpp:355-431) */
+  HX_PDE_FIX_ADMIS = 4       /* pde::Fix_therm_admis            (pde.hpp:437-493) */
+} hx_pde_kind;
+
+/* Constructor arguments of the PDE (the `pde_args...` of every Spatial kernel). */
+typedef struct hx_pde {
+  int kind;                /* hx_pde_kind */
+  hx_transport visc;       /* Navier_stokes: dynamic viscosity */
+  hx_transport cond;       /* Navier_stokes: thermal conductivity */
+  double advect_length;    /* Advection */
+  double diff_time;        /* Smooth_art_visc */
+  double cheby_step;       /* Smooth_art_visc */
+} hx_pde;
+
+/* Arithmetic form of the flux/derivative stages. HX_FP64 is the reference arithmetic. HX_FP32_MFMA computes
+ * the volume fluxes and the interior derivative contractions in FP32 on the matrix cores (FP32 accumulate) and
+ * keeps the state, stage accumulation, face-flux lifting terms, metrics and time step in FP64. */
+typedef enum hx_precision { HX_FP64 = 0, HX_FP32_MFMA = 1 } hx_precision;
+
+/* Flow boundary conditions applied on device to boundary connections (Boundary_condition.cpp). */
+typedef enum hx_bc_kind {
+  HX_BC_FREESTREAM = 0,      /* Freestream      (Boundary_condition.cpp:62-76, 299)  */
+  HX_BC_COPY = 1,            /* Copy            (Boundary_condition.cpp:450-463)     */
+  HX_BC_NONPENETRATION = 2,  /* Nonpenetration  (Boundary_condition.cpp:302-345)     */
+  HX_BC_OUTFLOW = 3          /* Outflow         (Boundary_condition.cpp:465-477)     */
+} hx_bc_kind;
+
+typedef struct hx_bc {
+  int kind;                  /* hx_bc_kind */
+  double state[HX_MAX_N_DIM + 2];  /* freestream conserved state: momentum[n_dim], density, energy */
+} hx_bc;
+
+/*
+ * Host description of a mesh. nd = n_dim, rs = row_size, nv = nd + 2, nq = rs^nd, nfq = rs^(nd-1),
+ * nvert = 2^nd. Arrays indexed [c] exist per element/connection class (HX_CARTESIAN, HX_DEFORMED); an array
+ * may be NULL when its count is 0 or the field is unused (see each field).
+ *
+ * Layouts follow the reference:
+ *  - element fields are [i_elem][i_var][i_qpoint], i_qpoint = (i*rs + j)*rs + k with dimension 0 slowest
+ *    (doc/conventions.dox:104-116, Element.cpp:114-194);
+ *  - face slots hold `face_sz` doubles; the extrapolated state / flux block is at offset 0 and the LDG
+ *    block at offset nv*nfq, each [i_var][i_face_qpoint] (connection.hpp:52-87);
+ *    face_sz = max(2*nv, nd + rs)*nfq (connection.hpp:61);
+ *  - element face i_face = 2*i_dim + is_positive points at a slot (Element.hpp:44);
+ *  - deformed metrics are [i_elem][nd*nd + 1][nq]: reference-level normals [i_dim][j_dim] then the Jacobian
+ *    determinant (Deformed_element.cpp:84-90,138-146);
+ *  - normal slots hold nd*nfq doubles [i_dim][i_face_qpoint] (connection.hpp:77-86).
+ */
+typedef struct hx_host_mesh {
+  int n_dim;
+  int row_size;
+  int n_elem[2];
+  double* state[2];       /* [n][nv][nq] conserved state: momentum[nd], density, energy       */
+  double* rcache[2];      /* [n][nv][nq] stage-0 residual cache (Element.cpp:188)            */
+  double* tss[2];         /* [n][nq] local time step scale (Element.cpp:119)                  */
+  double* av[2];          /* [n][2][nq] bulk and Laplacian artificial viscosity; NULL => zero */
+  double* forcing[2];     /* [n][4][nq] AV forcing (Element.cpp:134); NULL unless AV driver   */
+  double* advection[2];   /* [n][rs][nq] AV advection state (Element.cpp:139); NULL unless AV  */
+  double* nom_size[2];    /* [n] nominal edge length h (Element.cpp:11)                      */
+  double* vertex_tss[2];  /* [n][nvert] vertex time step scale (Element.cpp:163)             */
+  int* face_slot[2];      /* [n][2*nd] face slot of each element face                         */
+  double* metrics;        /* deformed only: [n][nd*nd + 1][nq]                                */
+  int* face_nslot;        /* deformed only: [n][2*nd] normal slot of each face, -1 => identity */
+  int n_slot;
+  double* faces;          /* [n_slot][face_sz] */
+  int n_nslot;
+  double* normals;        /* [n_nslot][nd][nfq] */
+  int n_con[2];
+  int* con_slot[2];       /* [n][2] face slot of side 0 and side 1 */
+  int* con_dir[2];        /* [n] packed Connection_direction: i_dim[0] | i_dim[1] << 2 | face_sign[0] << 4 |
+                             face_sign[1] << 5 (Kernel_connection.hpp:7-37). Cartesian: {{d, d}, {1, 0}} */
+  int* con_nslot;         /* deformed only: [n][2] normal slot of side 0 and side 1 */
+  int n_ref;
+  int* ref_slot;          /* [n_ref][5]: coarse slot, fine slots 0..3 (-1 when unused) (Refined_face.hpp:9-15) */
+  int* ref_stretch;       /* [n_ref][2] */
+  int n_bc_con;
+  int* bc_con;            /* [n_bc_con] index into the deformed connections */
+  int* bc_serial;         /* [n_bc_con] index into bcs */
+  int n_bc;
+  hx_bc* bcs;
+} hx_host_mesh;
+
+/* Fields that move between host and device (hx_mesh_upload / hx_mesh_download). */
+typedef enum hx_field {
+  HX_FIELD_STATE = 0,
+  HX_FIELD_RCACHE = 1,
+  HX_FIELD_TSS = 2,
+  HX_FIELD_AV = 3,
+  HX_FIELD_FORCING = 4,
+  HX_FIELD_ADVECTION = 5,
+  HX_FIELD_VERTEX_TSS = 6,
+  HX_FIELD_METRICS = 7,   /* class must be HX_DEFORMED */
+  HX_FIELD_FACES = 8,     /* class ignored */
+  HX_FIELD_NORMALS = 9    /* class ignored */
+} hx_field;
+
+/* Kernel_options (kernels.hpp:11-20), minus the stopwatch references. */
+typedef struct hx_kernel_options {
+  double dt;
+  int i_stage;
+  int compute_residual;
+  int use_filter;
+} hx_kernel_options;
+
+/* Solver settings read by Solver::update (Solver.cpp:834-886, defaults at Solver.cpp:151-174). */
+typedef struct hx_update_options {
+  double max_safety;      /* default 0.7 */
+  double max_time_step;   /* default huge */
+  int n_cheby_flow;       /* default 1 */
+  int flow_iters;         /* default 1 */
+  int local_time;         /* local time stepping */
+  int use_filter;         /* default 0 */
+  int use_ldg;            /* Solver::use_ldg(): viscous transport or artificial viscosity (Solver.cpp:117-120) */
+  hx_transport visc;
+  hx_transport cond;
+} hx_update_options;
+
+/* Status after an update (Iteration_status fields touched by Solver::update, Solver.cpp:872-881). */
+typedef struct hx_update_status {
+  double time_step;
+  double flow_time;
+  int iteration;
+} hx_update_status;
+
+typedef struct hx_ctx hx_ctx;
+typedef struct hx_mesh hx_mesh;
+
+/* ---- context ---------------------------------------------------------------------------------------- */
+hx_status hx_ctx_create(int device, hx_ctx** out);
+hx_status hx_ctx_destroy(hx_ctx* ctx);
+const char* hx_last_error(const hx_ctx* ctx);
+hx_status hx_ctx_synchronize(hx_ctx* ctx);
+/* The HIP stream (hipStream_t) all work of this context is ordered on. */
+void* hx_ctx_stream(hx_ctx* ctx);
+/* Number of (n_dim, row_size) combinations built into this library for `pde`; 0 if none. */
+int hx_kernels_built(int n_dim, int row_size, int pde_kind);
+
+/* ---- basis (Basis.hpp:14-63, Gauss_legendre.hpp, Basis.cpp:6-14) ------------------------------------- */
+/* Copies one basis table for `row_size` into `out` (capacity `n` doubles). `name` is one of "node", "weight",
+ * "diff_mat", "boundary", "orthogonal", "filter", "prolong", "restrict", "eigenvals". Returns the number of
+ * doubles the table holds (copying min(n, size)), or -1 for an unknown name / row size. */
+int hx_basis_table(int row_size, const char* name, double* out, int n);
+/* Basis::max_cfl() = -2*0.9/min_eig_convection (Basis.cpp:6-9) and Basis::step_ratio() = 0.5/0.9 (Basis.cpp:11-14). */
+double hx_basis_max_cfl(int row_size);
+double hx_basis_step_ratio(int row_size);
+
+/* ---- mesh ------------------------------------------------------------------------------------------- */
+/* Copies the topology and every non-NULL field of `desc` to the device. Replaces walking the reference's
+ * Kernel_mesh sequences (Kernel_mesh.hpp:14-25) with one registration. Fails with the reference's message
+ * "demand for invalid kernel" unless 1 <= n_dim <= 3 and 2 <= row_size <= 8 (kernel_factory.hpp:114-116). */
+hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* desc, hx_mesh** out);
+hx_status hx_mesh_destroy(hx_mesh* mesh);
+hx_status hx_mesh_upload(hx_mesh* mesh, int field, int cls, const double* host, size_t n_doubles);
+hx_status hx_mesh_download(hx_mesh* mesh, int field, int cls, double* host, size_t n_doubles);
+hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision);
+/* Device address of a field (for callers that keep their own device views, e.g. torch tensors). */
+hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size_t* n_doubles);
+
+/* ---- kernels: one per reference kernel class (Spatial.hpp) ------------------------------------------ */
+/* Spatial<Pde, def>::Write_face (Spatial.hpp:29-71): extrapolate element state to its faces. */
+hx_status hx_write_face(hx_mesh* mesh, int cls, const hx_pde* pde);
+/* Spatial<Pde, def>::Neighbor (Spatial.hpp:601-705): numerical flux on every connection of class `cls`. */
+hx_status hx_neighbor(hx_mesh* mesh, int cls, const hx_pde* pde, int i_stage);
+/* Spatial<Pde, def>::Local (Spatial.hpp:293-510): element-local residual, stage update and Write_face. */
+hx_status hx_local(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual,
+                   int use_filter);
+/* Spatial<Pde, def>::Neighbor_reconcile (Spatial.hpp:709-760). */
+hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde* pde);
+/* Spatial<Pde, def>::Reconcile_ldg_flux (Spatial.hpp:513-595). */
+hx_status hx_reconcile_ldg_flux(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_stage,
+                                int compute_residual, int use_filter);
+/* Spatial<Pde, def>::Max_dt (Spatial.hpp:763-829): writes tss and returns the global step in *dt_out
+ * (1.0 when is_local). */
+hx_status hx_max_dt(hx_mesh* mesh, int cls, const hx_pde* pde, int is_local, int use_filter, double safety_conv,
+                    double safety_diff, double* dt_out);
+/* Spatial<Pde, false>::Prolong_refined / Restrict_refined (Spatial.hpp:138-286) over all refined faces. */
+hx_status hx_prolong_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int offset);
+hx_status hx_restrict_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int offset);
+
+/* ---- boundary conditions (Solver.cpp:56-81) --------------------------------------------------------- */
+hx_status hx_apply_state_bcs(hx_mesh* mesh);
+hx_status hx_apply_flux_bcs(hx_mesh* mesh);
+
+/* ---- free functions (kernels.hpp:22-42) -------------------------------------------------------------- */
+/* compute_euler (kernels_convective.cpp:8-18). */
+hx_status hx_compute_euler(hx_mesh* mesh, const hx_kernel_options* opts);
+/* compute_navier_stokes (kernels_diffusive.cpp:8-29); the flux_bc callback is hx_apply_flux_bcs. */
+hx_status hx_compute_navier_stokes(hx_mesh* mesh, const hx_kernel_options* opts, hx_transport visc,
+                                   hx_transport cond);
+/* max_dt_euler / max_dt_navier_stokes (kernels_max_dt.cpp:8-17): min over both classes. */
+hx_status hx_max_dt_euler(hx_mesh* mesh, const hx_kernel_options* opts, double safety_conv, double safety_diff,
+                          int local_time, double* dt_out);
+hx_status hx_max_dt_navier_stokes(hx_mesh* mesh, const hx_kernel_options* opts, double safety_conv,
+                                  double safety_diff, int local_time, hx_transport visc, hx_transport cond,
+                                  double* dt_out);
+/* compute_write_face / compute_prolong / compute_restrict (kernels_convective.cpp:23-46). */
+hx_status hx_compute_write_face(hx_mesh* mesh);
+hx_status hx_compute_prolong(hx_mesh* mesh, int scale, int offset);
+hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset);
+
+/* ---- time step -------------------------------------------------------------------------------------- */
+/* Solver::update (Solver.cpp:834-886): flow_iters x n_cheby_flow x [dt <- max_dt; state BCs + stage 0;
+ * state BCs + stage 1]. `status` is advanced in place. */
+hx_status hx_update(hx_mesh* mesh, const hx_update_options* opts, hx_update_status* status);
+/* Solver::compute_residual (Solver.cpp:899-913): writes the scaled residual into the residual cache. */
+hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* opts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

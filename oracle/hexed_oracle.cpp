@@ -1,0 +1,1293 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- the CPU oracle. It restates the reference's DG hot path loop for loop on the
+ * host mesh layout of include/hexed_amd.h so that the HIP implementation can be checked against it, and it is
+ * the CPU baseline bench.py times. The product never links or calls this file.
+ *
+ * Reference files restated (paths relative to the hexed source tree):
+ *   include/Spatial.hpp          Write_face, Face_permutation, Prolong/Restrict_refined, Local,
+ *                                Reconcile_ldg_flux, Neighbor, Neighbor_reconcile, Max_dt
+ *   include/pde.hpp:27-175       Navier_stokes<visc>::Pde and its Computation
+ *   include/Derivative.hpp       Derivative<row_size>
+ *   include/Row_index.hpp, include/Row_rw.hpp   row iteration and row/face access
+ *   include/Transport_model.hpp  Sutherland / constant transport coefficients
+ *   include/math.hpp:30-36,207-218, src/math.cpp:104-107   pow, interp, chebyshev_step
+ *   src/kernels_convective.cpp, src/kernels_diffusive.cpp, src/kernels_max_dt.cpp   kernel sequences
+ *   src/Solver.cpp:56-81,117-133,834-913   BC application, max_dt, update, compute_residual
+ *   src/Boundary_condition.cpp   Freestream, Copy, Nonpenetration, Outflow
+ *
+ * Parity status of each part is recorded in DESIGN.md's oracle section; the tests under tests/ pin it.
+ */
+#include "hexed_oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../hexed_amd/csrc/basis_tables.inc"
+
+namespace {
+
+std::string g_err;
+
+constexpr int ipow(int b, int e)
+{
+  int r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+
+// math::pow (math.hpp:30-36): repeated multiplication starting from 1
+inline double mpow(double base, int e)
+{
+  double r = 1;
+  for (int i = 0; i < e; ++i) r *= base;
+  for (int i = 0; i > e; --i) r /= base;
+  return r;
+}
+
+// math::chebyshev_step (math.cpp:104-107)
+inline double chebyshev_step(int n, int i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); }
+
+constexpr double heat_rat = 1.4;               // pde.hpp:51
+constexpr double specific_gas_air = 287.05287;  // constants.hpp:44
+
+struct Basis
+{
+  int rs;
+  const double* node;
+  const double* weight;
+  const double* diff;      // [i][j]
+  const double* bound;     // [s][j]
+  const double* ortho;     // [k][i]
+  const double* filt;      // [i][j]
+  const double* prol;      // [h][i][j]
+  const double* restr;     // [h][i][j]
+  const double* eig;       // {min_eig_convection, min_eig_diffusion}
+  explicit Basis(int row_size) : rs{row_size}
+  {
+    int k = row_size - 2;
+    node = hx_basis_tables::nodes[k];
+    weight = hx_basis_tables::weights[k];
+    diff = hx_basis_tables::diff_mats[k];
+    bound = hx_basis_tables::boundarys[k];
+    ortho = hx_basis_tables::orthogonals[k];
+    filt = hx_basis_tables::filters[k];
+    prol = hx_basis_tables::prolongs[k];
+    restr = hx_basis_tables::restricts[k];
+    eig = hx_basis_tables::eigenvalss[k];
+  }
+  double max_cfl() const { return -2 * 0.9 / eig[0]; }  // Basis.cpp:6-9, quadratic_safety 0.9
+  double step_ratio() const { return .5 / 0.9; }        // Basis.cpp:11-14
+  double min_eig_diffusion() const { return eig[1]; }
+};
+
+// Connection_direction (Kernel_connection.hpp:7-37)
+struct Dir
+{
+  int i_dim[2];
+  bool face_sign[2];
+  bool flip_normal(int s) const { return face_sign[s] == bool(s); }
+  bool flip_tangential() const { return (i_dim[0] != i_dim[1]) && (flip_normal(0) == flip_normal(1)); }
+  bool transpose() const { return ((i_dim[0] == 0) && (i_dim[1] == 2)) || ((i_dim[0] == 2) && (i_dim[1] == 0)); }
+};
+
+inline Dir unpack_dir(int p) { return Dir{{p & 3, (p >> 2) & 3}, {bool((p >> 4) & 1), bool((p >> 5) & 1)}}; }
+
+// Transport_model (Transport_model.hpp:14-56)
+struct Transport
+{
+  double const_val, ref_val, ref_temp, sqrt_ref_temp, temp_offset;
+  bool is_viscous;
+  Transport() : const_val{0}, ref_val{0}, ref_temp{1}, sqrt_ref_temp{1}, temp_offset{1}, is_viscous{false} {}
+  explicit Transport(const hx_transport& t)
+  : const_val{t.const_val}, ref_val{t.ref_val}, ref_temp{t.ref_temp}, sqrt_ref_temp{std::sqrt(t.ref_temp)},
+    temp_offset{t.temp_offset}, is_viscous{bool(t.is_viscous)}
+  {}
+  double coefficient(double sqrt_temp) const
+  {
+    return const_val + ref_val * mpow(sqrt_temp / sqrt_ref_temp, 3) * (ref_temp + temp_offset) / (sqrt_temp * sqrt_temp + temp_offset);
+  }
+};
+
+/*
+ * pde::Navier_stokes<visc>::Pde<n_dim, row_size> (pde.hpp:41-175). State vector of a Computation holds the
+ * n_dim + 2 conserved variables followed by the bulk and Laplacian artificial viscosity coefficients.
+ */
+template <int nd, bool visc>
+struct Navier_stokes
+{
+  static constexpr bool has_diffusion = visc;
+  static constexpr bool has_convection = true;
+  static constexpr int n_update = nd + 2;
+  static constexpr int n_extrap = nd + 2;
+  Transport dyn_visc, therm_cond;
+
+  template <int ndf>
+  struct Computation
+  {
+    const Navier_stokes& eq;
+    explicit Computation(const Navier_stokes& e) : eq{e}
+    {
+      for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < ndf; ++j) normal[i][j] = (i == j);
+    }
+    double state[nd + 4];
+    double update_state[nd + 2];
+    // fetch_state (pde.hpp:79-84); the AV coefficients live in their own array in this layout
+    void fetch_state(int stride, const double* data, const double* av)
+    {
+      for (int i = 0; i < nd + 2; ++i) state[i] = data[i * stride];
+      state[nd + 2] = av ? av[0] : 0.;
+      state[nd + 3] = av ? av[stride] : 0.;
+    }
+    // fetch_extrap_state (pde.hpp:86-92)
+    void fetch_extrap_state(int stride, const double* data)
+    {
+      for (int i = 0; i < nd + 2; ++i) state[i] = data[i * stride];
+      state[nd + 2] = 0.;
+      state[nd + 3] = 0.;
+      for (int i = 0; i < nd + 2; ++i) update_state[i] = state[i];
+    }
+    double mass, kin_ener, pressure;
+    void compute_scalars_conv()  // pde.hpp:97-104
+    {
+      mass = state[nd];
+      kin_ener = 0;
+      for (int i = 0; i < nd; ++i) kin_ener += state[i] * state[i];
+      kin_ener *= .5 / mass;
+      pressure = (heat_rat - 1.) * (state[nd + 1] - kin_ener);
+    }
+    double normal[nd][ndf];
+    double flux_conv[nd + 2][ndf];
+    void compute_flux_conv()  // pde.hpp:108-122
+    {
+      compute_scalars_conv();
+      for (int i = 0; i < ndf; ++i) {
+        flux_conv[nd][i] = 0;
+        for (int j = 0; j < nd; ++j) flux_conv[nd][i] += state[j] * normal[j][i];
+        double vol_flux = flux_conv[nd][i] / mass;
+        flux_conv[nd + 1][i] = (state[nd + 1] + pressure) * vol_flux;
+        for (int j = 0; j < nd; ++j) flux_conv[j][i] = state[j] * vol_flux + pressure * normal[j][i];
+      }
+    }
+    double bulk_av, laplacian_av, sqrt_temp, dyn_visc_coef, therm_cond_coef, energy_cond;
+    void compute_scalars_diff()  // pde.hpp:130-138
+    {
+      bulk_av = std::abs(state[nd + 2]);
+      laplacian_av = std::abs(state[nd + 3]);
+      sqrt_temp = std::sqrt(std::max((state[nd + 1] - kin_ener) / mass, 0.) * (heat_rat - 1) / specific_gas_air);
+      dyn_visc_coef = eq.dyn_visc.coefficient(sqrt_temp);
+      therm_cond_coef = eq.therm_cond.coefficient(sqrt_temp);
+      energy_cond = therm_cond_coef * (heat_rat - 1) / specific_gas_air;
+    }
+    double gradient[nd + 2][nd];
+    double flux_diff[nd + 2][nd];
+    void compute_flux_diff()  // pde.hpp:142-158 (requires ndf == nd)
+    {
+      static_assert(ndf == nd || ndf == 1);
+      if constexpr (ndf == nd) {
+        compute_scalars_diff();
+        double veloc[nd];
+        for (int i = 0; i < nd; ++i) veloc[i] = state[i] / mass;
+        double vg[nd][nd];
+        for (int i = 0; i < nd; ++i)
+          for (int j = 0; j < nd; ++j) vg[i][j] = (gradient[i][j] - veloc[i] * gradient[nd][j]) / mass;
+        double trace = 0;
+        for (int i = 0; i < nd; ++i) trace += vg[i][i];
+        double stress[nd][nd];
+        for (int i = 0; i < nd; ++i)
+          for (int j = 0; j < nd; ++j)
+            stress[i][j] = dyn_visc_coef * (vg[i][j] + vg[j][i]) + (i == j ? (bulk_av * mass - 2. / 3. * dyn_visc_coef) * trace : 0.);
+        double fd[nd + 2][nd];
+        for (int v = 0; v < nd + 2; ++v)
+          for (int j = 0; j < nd; ++j) fd[v][j] = -laplacian_av * gradient[v][j];
+        for (int i = 0; i < nd; ++i)
+          for (int j = 0; j < nd; ++j) fd[i][j] -= stress[i][j];
+        for (int j = 0; j < nd; ++j) {
+          double vtvg = 0, vts = 0;
+          for (int i = 0; i < nd; ++i) {
+            vtvg += veloc[i] * vg[i][j];
+            vts += veloc[i] * stress[i][j];
+          }
+          double int_ener_grad = -state[nd + 1] / mass / mass * gradient[nd][j] + gradient[nd + 1][j] / mass - vtvg;
+          fd[nd + 1][j] -= vts + energy_cond * int_ener_grad;
+        }
+        for (int v = 0; v < nd + 2; ++v)
+          for (int j = 0; j < nd; ++j) {
+            double s = 0;
+            for (int k = 0; k < nd; ++k) s += fd[v][k] * normal[k][j];
+            flux_diff[v][j] = s;
+          }
+      }
+    }
+    double char_speed;
+    void compute_char_speed()  // pde.hpp:161-166
+    {
+      const double sound_speed = std::sqrt(heat_rat * (heat_rat - 1) * state[nd + 1] / state[nd]);
+      double sq = 0;
+      for (int i = 0; i < nd; ++i) sq += state[i] * state[i];
+      char_speed = sound_speed + std::sqrt(sq) / state[nd];
+    }
+    double diffusivity;
+    void compute_diffusivity()  // pde.hpp:169-174
+    {
+      compute_scalars_conv();
+      compute_scalars_diff();
+      diffusivity = std::abs(laplacian_av) + std::max(std::abs(bulk_av) + dyn_visc_coef / mass, energy_cond / mass);
+    }
+  };
+};
+
+// Derivative<row_size> (Derivative.hpp:14-61) with lift = W^{-1} B^T diag(-1, 1)
+template <int rs>
+struct Derivative
+{
+  double diff[rs][rs], bound[2][rs], lift[rs][2];
+  explicit Derivative(const Basis& b)
+  {
+    for (int i = 0; i < rs; ++i) {
+      for (int j = 0; j < rs; ++j) diff[i][j] = b.diff[i * rs + j];
+      for (int s = 0; s < 2; ++s) bound[s][i] = b.bound[s * rs + i];
+      double inv_w = 1. / b.weight[i];
+      lift[i][0] = -(inv_w * b.bound[0 * rs + i]);
+      lift[i][1] = inv_w * b.bound[1 * rs + i];
+    }
+  }
+  // operator()(q, b) = diff*q + lift*(b - boundary*q)   (Derivative.hpp:51-55)
+  template <int nv>
+  void apply(const double (&q)[rs][nv], const double (&b)[2][nv], double (&out)[rs][nv]) const
+  {
+    double jump[2][nv];
+    for (int s = 0; s < 2; ++s)
+      for (int v = 0; v < nv; ++v) {
+        double bq = 0;
+        for (int k = 0; k < rs; ++k) bq += bound[s][k] * q[k][v];
+        jump[s][v] = b[s][v] - bq;
+      }
+    for (int i = 0; i < rs; ++i)
+      for (int v = 0; v < nv; ++v) {
+        double dq = 0;
+        for (int k = 0; k < rs; ++k) dq += diff[i][k] * q[k][v];
+        out[i][v] = dq + (lift[i][0] * jump[0][v] + lift[i][1] * jump[1][v]);
+      }
+  }
+  // operator()(q) = diff*q   (Derivative.hpp:56-60)
+  template <int nv>
+  void apply(const double (&q)[rs][nv], double (&out)[rs][nv]) const
+  {
+    for (int i = 0; i < rs; ++i)
+      for (int v = 0; v < nv; ++v) {
+        double dq = 0;
+        for (int k = 0; k < rs; ++k) dq += diff[i][k] * q[k][v];
+        out[i][v] = dq;
+      }
+  }
+  // boundary_term(b) = lift*b   (Derivative.hpp:45-49)
+  template <int nv>
+  void boundary_term(const double (&b)[2][nv], double (&out)[rs][nv]) const
+  {
+    for (int i = 0; i < rs; ++i)
+      for (int v = 0; v < nv; ++v) out[i][v] = lift[i][0] * b[0][v] + lift[i][1] * b[1][v];
+  }
+};
+
+// Row_index (Row_index.hpp:14-64): rows along i_dim; i_fq enumerates rows (= face quadrature point)
+template <int nd, int rs>
+struct Rows
+{
+  static constexpr int nq = ipow(rs, nd);
+  static constexpr int nfq = nq / rs;
+  static int stride(int i_dim) { return ipow(rs, nd - 1 - i_dim); }
+  static int qpoint(int i_dim, int i_fq, int k)
+  {
+    int s = stride(i_dim);
+    return (i_fq / s) * s * rs + i_fq % s + k * s;
+  }
+  // Row_rw::read_row (Row_rw.hpp:27-35)
+  template <int nv>
+  static void read_row(const double* data, int i_dim, int i_fq, double (&r)[rs][nv])
+  {
+    for (int v = 0; v < nv; ++v)
+      for (int k = 0; k < rs; ++k) r[k][v] = data[v * nq + qpoint(i_dim, i_fq, k)];
+  }
+  // Row_rw::write_row (Row_rw.hpp:38-46): d = coef*d + w
+  template <int nv>
+  static void write_row(const double (&w)[rs][nv], double* data, int i_dim, int i_fq, double coef)
+  {
+    for (int v = 0; v < nv; ++v)
+      for (int k = 0; k < rs; ++k) {
+        double& d = data[v * nq + qpoint(i_dim, i_fq, k)];
+        d = coef * d + w[k][v];
+      }
+  }
+  // Row_rw::read_bound (Row_rw.hpp:49-57)
+  template <int nv>
+  static void read_bound(double* const* faces, int i_dim, int i_fq, double (&b)[2][nv])
+  {
+    for (int v = 0; v < nv; ++v)
+      for (int s = 0; s < 2; ++s) b[s][v] = faces[2 * i_dim + s][v * nfq + i_fq];
+  }
+  // Row_rw::write_bound (Row_rw.hpp:60-68)
+  template <int nv>
+  static void write_bound(const double (&b)[2][nv], double* const* faces, int i_dim, int i_fq)
+  {
+    for (int v = 0; v < nv; ++v)
+      for (int s = 0; s < 2; ++s) faces[2 * i_dim + s][v * nfq + i_fq] = b[s][v];
+  }
+};
+
+// Spatial::Face_permutation (Spatial.hpp:73-131) on nv blocks of nfq values
+template <int nd, int rs>
+struct Face_permutation
+{
+  static constexpr int nfq = ipow(rs, nd - 1);
+  Dir dir;
+  double* tgt;
+  int n_var;
+  void transpose()
+  {
+    if (dir.transpose() && nd == 3) {
+      for (int v = 0; v < n_var; ++v) {
+        double* m = tgt + v * nfq;  // column-major rs x rs map: (r, c) at c*rs + r
+        for (int r = 0; r < rs; ++r)
+          for (int c = r + 1; c < rs; ++c) std::swap(m[c * rs + r], m[r * rs + c]);
+      }
+    }
+  }
+  void flip()
+  {
+    if (!dir.flip_tangential()) return;
+    if constexpr (nd == 3) {
+      bool colwise = (dir.i_dim[0] > 3 - dir.i_dim[0] - dir.i_dim[1]) != dir.transpose();
+      for (int v = 0; v < n_var; ++v) {
+        double* m = tgt + v * nfq;
+        if (colwise) {
+          for (int c = 0; c < rs; ++c)
+            for (int r = 0; r < rs / 2; ++r) std::swap(m[c * rs + r], m[c * rs + rs - 1 - r]);
+        } else {
+          for (int r = 0; r < rs; ++r)
+            for (int c = 0; c < rs / 2; ++c) std::swap(m[c * rs + r], m[(rs - 1 - c) * rs + r]);
+        }
+      }
+    } else if constexpr (nd == 2) {
+      for (int v = 0; v < n_var; ++v) {
+        double* m = tgt + v * nfq;
+        for (int r = 0; r < rs / 2; ++r) std::swap(m[r], m[rs - 1 - r]);
+      }
+    }
+  }
+  void match_faces() { transpose(); flip(); }
+  void restore() { flip(); transpose(); }
+};
+
+struct Mesh_view
+{
+  hx_host_mesh* m;
+  int face_sz() const
+  {
+    int nfq = ipow(m->row_size, m->n_dim - 1);
+    return std::max(2 * (m->n_dim + 2), m->n_dim + m->row_size) * nfq;
+  }
+  double* slot(int s) const { return m->faces + size_t(s) * face_sz(); }
+  double* nslot(int s) const { return m->normals + size_t(s) * m->n_dim * ipow(m->row_size, m->n_dim - 1); }
+};
+
+template <int nd, int rs>
+struct Ops
+{
+  static constexpr int nq = ipow(rs, nd);
+  static constexpr int nfq = nq / rs;
+  static constexpr int nv = nd + 2;
+  static constexpr int nvert = 1 << nd;
+  using R = Rows<nd, rs>;
+
+  // Spatial::Write_face (Spatial.hpp:41-57) for the Navier-Stokes extrapolation variables
+  static void write_face_elem(const Basis& basis, const double* read, double* const* faces)
+  {
+    for (int i_dim = 0; i_dim < nd; ++i_dim)
+      for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+        double row[rs][nv];
+        R::template read_row<nv>(read, i_dim, i_fq, row);
+        double b[2][nv];
+        for (int s = 0; s < 2; ++s)
+          for (int v = 0; v < nv; ++v) {
+            double acc = 0;
+            for (int k = 0; k < rs; ++k) acc += basis.bound[s * rs + k] * row[k][v];
+            b[s][v] = acc;
+          }
+        R::template write_bound<nv>(b, faces, i_dim, i_fq);
+      }
+  }
+
+  static void write_face(hx_host_mesh* m, int cls)
+  {
+    Basis basis(rs);
+    Mesh_view mv{m};
+    const int n = m->n_elem[cls];
+    #pragma omp parallel for
+    for (int e = 0; e < n; ++e) {
+      double* faces[6];
+      for (int f = 0; f < 2 * nd; ++f) faces[f] = mv.slot(m->face_slot[cls][e * 2 * nd + f]);
+      write_face_elem(basis, m->state[cls] + size_t(e) * nv * nq, faces);
+    }
+  }
+
+  template <bool visc>
+  static void local(hx_host_mesh* m, int cls, const Navier_stokes<nd, visc>& eq, double dt, int stage,
+                    bool compute_residual, bool use_filter)
+  {
+    using Pde = Navier_stokes<nd, visc>;
+    const bool is_deformed = cls == HX_DEFORMED;
+    Basis basis(rs);
+    Derivative<rs> derivative(basis);
+    const double update_coef = stage ? dt * basis.step_ratio() : dt;
+    Mesh_view mv{m};
+    // dummy face normal for deformed elements in Cartesian connections (Spatial.hpp:331-339)
+    static double cartesian_normal[nd][nd][nfq];
+    for (int i = 0; i < nd; ++i)
+      for (int j = 0; j < nd; ++j)
+        for (int q = 0; q < nfq; ++q) cartesian_normal[i][j][q] = i == j;
+    const int n = m->n_elem[cls];
+    #pragma omp parallel for
+    for (int e = 0; e < n; ++e) {
+      double* state = m->state[cls] + size_t(e) * nv * nq;
+      const double* av = m->av[cls] ? m->av[cls] + size_t(e) * 2 * nq : nullptr;
+      double* faces[6];
+      double* visc_faces[6];
+      for (int f = 0; f < 2 * nd; ++f) {
+        faces[f] = mv.slot(m->face_slot[cls][e * 2 * nd + f]);
+        visc_faces[f] = faces[f] + nv * nfq;
+      }
+      double* tss = m->tss[cls] + size_t(e) * nq;
+      double d_pos = m->nom_size[cls][e];
+      double time_rate[2][nv][nq] = {};
+      const double* nrml = nullptr;
+      const double* elem_det = nullptr;
+      double* face_nrml[6] = {};
+      double visc_storage[nd][nv][nq] = {};
+      if (is_deformed) {
+        nrml = m->metrics + size_t(e) * (nd * nd + 1) * nq;
+        elem_det = nrml + nd * nd * nq;
+        for (int f = 0; f < 2 * nd; ++f) {
+          int ns = m->face_nslot[e * 2 * nd + f];
+          face_nrml[f] = ns >= 0 ? mv.nslot(ns) : cartesian_normal[f / 2][0];
+        }
+      }
+      // gradient times Jacobian determinant (Spatial.hpp:371-402)
+      if constexpr (Pde::has_diffusion) {
+        for (int q = 0; q < nq; ++q)
+          for (int v = 0; v < nv; ++v) time_rate[0][v][q] = state[v * nq + q];
+        for (int i_dim = 0; i_dim < nd; ++i_dim)
+          for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+            double row_r[rs][nv];
+            R::template read_row<nv>(time_rate[0][0], i_dim, i_fq, row_r);
+            double face_state[2][nv];
+            R::template read_bound<nv>(visc_faces, i_dim, i_fq, face_state);
+            if (is_deformed) {
+              double row_n[rs][nd];
+              R::template read_row<nd>(nrml + i_dim * nd * nq, i_dim, i_fq, row_n);
+              double face_n[2][nd];
+              R::template read_bound<nd>(face_nrml, i_dim, i_fq, face_n);
+              for (int j_dim = 0; j_dim < nd; ++j_dim)
+                for (int v = 0; v < nv; ++v) {
+                  double row[rs][1], bound[2][1], d[rs][1];
+                  for (int k = 0; k < rs; ++k) row[k][0] = row_n[k][j_dim] * row_r[k][v];
+                  for (int s = 0; s < 2; ++s) bound[s][0] = face_n[s][j_dim] * face_state[s][v];
+                  derivative.template apply<1>(row, bound, d);
+                  for (int k = 0; k < rs; ++k) d[k][0] = d[k][0] / d_pos;
+                  R::template write_row<1>(d, visc_storage[j_dim][v], i_dim, i_fq, 1.);
+                }
+            } else {
+              double d[rs][nv];
+              derivative.template apply<nv>(row_r, face_state, d);
+              for (int k = 0; k < rs; ++k)
+                for (int v = 0; v < nv; ++v) d[k][v] = d[k][v] / d_pos;
+              R::template write_row<nv>(d, visc_storage[i_dim][0], i_dim, i_fq, 0);
+            }
+          }
+        std::memset(time_rate, 0, sizeof(time_rate));
+      }
+      // flux (Spatial.hpp:404-442)
+      double flux[nd][nv][nq];
+      for (int q = 0; q < nq; ++q) {
+        typename Pde::template Computation<nd> comp(eq);
+        comp.fetch_state(nq, state + q, av ? av + q : nullptr);
+        if (is_deformed)
+          for (int i_dim = 0; i_dim < nd; ++i_dim)
+            for (int j_dim = 0; j_dim < nd; ++j_dim) comp.normal[j_dim][i_dim] = nrml[(i_dim * nd + j_dim) * nq + q];
+        comp.compute_flux_conv();
+        for (int i_dim = 0; i_dim < nd; ++i_dim)
+          for (int v = 0; v < nv; ++v) flux[i_dim][v][q] = comp.flux_conv[v][i_dim];
+        if constexpr (Pde::has_diffusion) {
+          for (int i_dim = 0; i_dim < nd; ++i_dim)
+            for (int v = 0; v < nv; ++v) comp.gradient[v][i_dim] = visc_storage[i_dim][v][q];
+          if (is_deformed)
+            for (int v = 0; v < nv; ++v)
+              for (int i_dim = 0; i_dim < nd; ++i_dim) comp.gradient[v][i_dim] /= elem_det[q];
+          comp.compute_flux_diff();
+          for (int i_dim = 0; i_dim < nd; ++i_dim)
+            for (int v = 0; v < nv; ++v) visc_storage[i_dim][v][q] = comp.flux_diff[v][i_dim];
+        }
+      }
+      // residual (Spatial.hpp:444-470)
+      for (int i_dim = 0; i_dim < nd; ++i_dim)
+        for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+          double row_f[rs][nv], face_f[2][nv], d[rs][nv];
+          R::template read_row<nv>(flux[i_dim][0], i_dim, i_fq, row_f);
+          R::template read_bound<nv>(faces, i_dim, i_fq, face_f);
+          derivative.template apply<nv>(row_f, face_f, d);
+          for (int k = 0; k < rs; ++k)
+            for (int v = 0; v < nv; ++v) d[k][v] = -d[k][v];
+          R::template write_row<nv>(d, time_rate[0][0], i_dim, i_fq, 1.);
+          if constexpr (Pde::has_diffusion) {
+            R::template read_row<nv>(visc_storage[i_dim][0], i_dim, i_fq, row_f);
+            for (int s = 0; s < 2; ++s)
+              for (int v = 0; v < nv; ++v) {
+                double acc = 0;
+                for (int k = 0; k < rs; ++k) acc += basis.bound[s * rs + k] * row_f[k][v];
+                face_f[s][v] = acc;
+              }
+            for (int v = 0; v < nv; ++v)
+              for (int s = 0; s < 2; ++s) visc_faces[i_dim * 2 + s][v * nfq + i_fq] = face_f[s][v];
+            derivative.template apply<nv>(row_f, d);
+            for (int k = 0; k < rs; ++k)
+              for (int v = 0; v < nv; ++v) d[k][v] = -d[k][v];
+            R::template write_row<nv>(d, time_rate[1][0], i_dim, i_fq, 1.);
+          }
+        }
+      // modal filter (Spatial.hpp:472-481) over both halves of time_rate
+      if (use_filter) filter_rows<2 * nv>(basis, time_rate[0][0]);
+      // update (Spatial.hpp:483-503)
+      double* ref_state = m->rcache[cls] + size_t(e) * nv * nq;
+      for (int q = 0; q < nq; ++q) {
+        double update[nv] = {};
+        double mult = update_coef * tss[q] / d_pos;
+        if (is_deformed) mult /= elem_det[q];
+        for (int v = 0; v < nv; ++v) {
+          double u = time_rate[0][v][q];
+          if (stage) {
+            u -= ref_state[v * nq + q];
+          } else {
+            ref_state[v * nq + q] = u;
+            if constexpr (Pde::has_diffusion) u += time_rate[1][v][q];
+          }
+          u *= mult;
+          if (compute_residual) ref_state[v * nq + q] = u;
+          else update[v] = u;
+        }
+        for (int v = 0; v < nv; ++v) state[v * nq + q] += update[v];
+      }
+      if constexpr (!Pde::has_diffusion) write_face_elem(basis, state, faces);
+    }
+  }
+
+  // row-wise modal filter along every dimension: row = filter*row, written with coefficient 0
+  template <int nvar>
+  static void filter_rows(const Basis& basis, double* data)
+  {
+    for (int i_dim = 0; i_dim < nd; ++i_dim)
+      for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+        double row[rs][nvar], out[rs][nvar];
+        R::template read_row<nvar>(data, i_dim, i_fq, row);
+        for (int i = 0; i < rs; ++i)
+          for (int v = 0; v < nvar; ++v) {
+            double acc = 0;
+            for (int k = 0; k < rs; ++k) acc += basis.filt[i * rs + k] * row[k][v];
+            out[i][v] = acc;
+          }
+        R::template write_row<nvar>(out, data, i_dim, i_fq, 0.);
+      }
+  }
+
+  // Spatial::Reconcile_ldg_flux (Spatial.hpp:543-594)
+  template <bool visc>
+  static void reconcile_ldg_flux(hx_host_mesh* m, int cls, double dt, bool compute_residual, bool use_filter)
+  {
+    const bool is_deformed = cls == HX_DEFORMED;
+    Basis basis(rs);
+    Derivative<rs> derivative(basis);
+    Mesh_view mv{m};
+    const int n = m->n_elem[cls];
+    #pragma omp parallel for
+    for (int e = 0; e < n; ++e) {
+      double* state = m->state[cls] + size_t(e) * nv * nq;
+      double* faces[6];
+      double* visc_faces[6];
+      for (int f = 0; f < 2 * nd; ++f) {
+        faces[f] = mv.slot(m->face_slot[cls][e * 2 * nd + f]);
+        visc_faces[f] = faces[f] + nv * nfq;
+      }
+      double* tss = m->tss[cls] + size_t(e) * nq;
+      double d_pos = m->nom_size[cls][e];
+      double time_rate[nv][nq] = {};
+      const double* elem_det = is_deformed ? m->metrics + size_t(e) * (nd * nd + 1) * nq + nd * nd * nq : nullptr;
+      for (int i_dim = 0; i_dim < nd; ++i_dim)
+        for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+          double bound_f[2][nv], d[rs][nv];
+          R::template read_bound<nv>(visc_faces, i_dim, i_fq, bound_f);
+          derivative.template boundary_term<nv>(bound_f, d);
+          for (int k = 0; k < rs; ++k)
+            for (int v = 0; v < nv; ++v) d[k][v] = -d[k][v];
+          R::template write_row<nv>(d, time_rate[0], i_dim, i_fq, 1.);
+        }
+      if (use_filter) filter_rows<nv>(basis, time_rate[0]);
+      double* to_update = compute_residual ? m->rcache[cls] + size_t(e) * nv * nq : state;
+      for (int q = 0; q < nq; ++q) {
+        double mult = dt * tss[q] / d_pos;
+        if (is_deformed) mult /= elem_det[q];
+        for (int v = 0; v < nv; ++v) to_update[v * nq + q] += time_rate[v][q] * mult;
+      }
+      write_face_elem(basis, state, faces);
+    }
+  }
+
+  // Spatial::Neighbor (Spatial.hpp:613-704)
+  template <bool visc>
+  static void neighbor(hx_host_mesh* m, int cls, const Navier_stokes<nd, visc>& eq)
+  {
+    using Pde = Navier_stokes<nd, visc>;
+    const bool is_deformed = cls == HX_DEFORMED;
+    Mesh_view mv{m};
+    const int n = m->n_con[cls];
+    #pragma omp parallel for
+    for (int i_con = 0; i_con < n; ++i_con) {
+      Dir dir = unpack_dir(m->con_dir[cls][i_con]);
+      double* side[2] = {mv.slot(m->con_slot[cls][2 * i_con]), mv.slot(m->con_slot[cls][2 * i_con + 1])};
+      double face[2 + 2 * Pde::has_diffusion][nv * nfq] = {};
+      double face_nrml[nd * nfq];
+      int sign[2] = {1, 1};
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < nv * nfq; ++i) face[s][i] = side[s][i];
+      Face_permutation<nd, rs> perm{dir, face[1], nv};
+      if (is_deformed) {
+        perm.match_faces();
+        for (int s = 0; s < 2; ++s) sign[s] = 1 - 2 * dir.flip_normal(s);
+        const double* nn = mv.nslot(m->con_nslot[2 * i_con]);
+        for (int i = 0; i < nd * nfq; ++i) face_nrml[i] = nn[i];
+      }
+      for (int q = 0; q < nfq; ++q) {
+        if constexpr (Pde::has_diffusion) {
+          for (int v = 0; v < nv; ++v) {
+            double avg = .5 * (face[0][v * nfq + q] + face[1][v * nfq + q]);
+            for (int s = 0; s < 2; ++s) face[2 + s][v * nfq + q] = avg;
+          }
+        }
+        typename Pde::template Computation<1> comp[2] = {typename Pde::template Computation<1>(eq),
+                                                        typename Pde::template Computation<1>(eq)};
+        if (is_deformed) {
+          for (int i = 0; i < nd; ++i) comp[0].normal[i][0] = sign[0] * face_nrml[i * nfq + q];
+        } else {
+          for (int i = 0; i < nd; ++i) comp[0].normal[i][0] = (i == dir.i_dim[0]);
+        }
+        for (int i = 0; i < nd; ++i) comp[1].normal[i][0] = comp[0].normal[i][0];
+        for (int s = 0; s < 2; ++s) comp[s].fetch_extrap_state(nfq, face[s] + q);
+        for (int s = 0; s < 2; ++s) {
+          comp[s].compute_flux_conv();
+          comp[s].compute_char_speed();
+        }
+        double nsq = 0;
+        for (int i = 0; i < nd; ++i) nsq += comp[0].normal[i][0] * comp[0].normal[i][0];
+        double coef = std::max(comp[0].char_speed, comp[1].char_speed) * std::sqrt(nsq);
+        for (int v = 0; v < nv; ++v) {
+          double f = .5 * (comp[0].flux_conv[v][0] + comp[1].flux_conv[v][0] +
+                           coef * (comp[0].update_state[v] - comp[1].update_state[v]));
+          for (int s = 0; s < 2; ++s) face[s][v * nfq + q] = sign[s] * f;
+        }
+      }
+      if (is_deformed) {
+        perm.restore();
+        if constexpr (Pde::has_diffusion) Face_permutation<nd, rs>{dir, face[3], nv}.restore();
+      }
+      for (int s = 0; s < 2; ++s) {
+        for (int i = 0; i < nv * nfq; ++i) side[s][i] = face[s][i];
+        if constexpr (Pde::has_diffusion)
+          for (int i = 0; i < nv * nfq; ++i) side[s][nv * nfq + i] = face[2 + s][i];
+      }
+    }
+  }
+
+  // Spatial::Neighbor_reconcile (Spatial.hpp:716-759)
+  static void neighbor_reconcile(hx_host_mesh* m, int cls)
+  {
+    const bool is_deformed = cls == HX_DEFORMED;
+    Mesh_view mv{m};
+    const int n = m->n_con[cls];
+    #pragma omp parallel for
+    for (int i_con = 0; i_con < n; ++i_con) {
+      Dir dir = unpack_dir(m->con_dir[cls][i_con]);
+      double* side[2] = {mv.slot(m->con_slot[cls][2 * i_con]) + nv * nfq,
+                         mv.slot(m->con_slot[cls][2 * i_con + 1]) + nv * nfq};
+      double face[2][nv * nfq];
+      int sign[2] = {1, 1};
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < nv * nfq; ++i) face[s][i] = side[s][i];
+      Face_permutation<nd, rs> perm{dir, face[1], nv};
+      if (is_deformed) {
+        perm.match_faces();
+        for (int s = 0; s < 2; ++s) sign[s] = 1 - 2 * dir.flip_normal(s);
+      }
+      for (int q = 0; q < nfq; ++q)
+        for (int v = 0; v < nv; ++v) {
+          double avg = 0;
+          for (int s = 0; s < 2; ++s) avg += .5 * sign[s] * face[s][v * nfq + q];
+          for (int s = 0; s < 2; ++s) {
+            double& f = face[s][v * nfq + q];
+            f = sign[s] * avg - f;
+          }
+        }
+      if (is_deformed) perm.restore();
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < nv * nfq; ++i) side[s][i] = face[s][i];
+    }
+  }
+
+  // math::interp (math.hpp:207-218)
+  static double interp(double (&values)[nvert], const double (&coords)[nd])
+  {
+    int stride = nvert;
+    for (int i_dim = 0; i_dim < nd; ++i_dim) {
+      stride /= 2;
+      for (int i = 0; i < stride; ++i) values[i] += coords[i_dim] * (values[i + stride] - values[i]);
+    }
+    return values[0];
+  }
+
+  // Spatial::Max_dt (Spatial.hpp:784-828)
+  template <bool visc>
+  static double max_dt(hx_host_mesh* m, int cls, const Navier_stokes<nd, visc>& eq, bool is_local, double safety_conv,
+                       double safety_diff)
+  {
+    using Pde = Navier_stokes<nd, visc>;
+    Basis basis(rs);
+    const double max_cfl_c = basis.max_cfl() * safety_conv;
+    const double max_cfl_d = -2 / basis.min_eig_diffusion() * safety_diff;
+    double dt = std::numeric_limits<double>::max();
+    const int n = m->n_elem[cls];
+    #pragma omp parallel for reduction(min : dt)
+    for (int e = 0; e < n; ++e) {
+      const double* state = m->state[cls] + size_t(e) * nv * nq;
+      const double* av = m->av[cls] ? m->av[cls] + size_t(e) * 2 * nq : nullptr;
+      double* tss = m->tss[cls] + size_t(e) * nq;
+      for (int q = 0; q < nq; ++q) {
+        double vertex_spacing[nvert];
+        for (int i = 0; i < nvert; ++i) vertex_spacing[i] = m->vertex_tss[cls][size_t(e) * nvert + i];
+        double coords[nd];
+        for (int i_dim = 0; i_dim < nd; ++i_dim) coords[i_dim] = basis.node[(q / ipow(rs, nd - 1 - i_dim)) % rs];
+        double spacing = interp(vertex_spacing, coords);
+        typename Pde::template Computation<nd> comp(eq);
+        comp.fetch_state(nq, state + q, av ? av + q : nullptr);
+        double scale = 0;
+        comp.compute_char_speed();
+        scale += comp.char_speed / max_cfl_c / spacing;
+        if constexpr (Pde::has_diffusion) {
+          comp.compute_diffusivity();
+          scale += comp.diffusivity / max_cfl_d / spacing / spacing;
+        }
+        if (is_local) tss[q] = 1. / scale;
+        else {
+          tss[q] = 1.;
+          dt = std::min(dt, 1. / scale);
+        }
+      }
+    }
+    return is_local ? 1. : dt;
+  }
+
+  // Spatial::Prolong_refined (Spatial.hpp:153-206)
+  static void prolong(hx_host_mesh* m, int n_var, bool scl, bool off)
+  {
+    Basis basis(rs);
+    Mesh_view mv{m};
+    constexpr int n_face = ipow(2, nd - 1);
+    #pragma omp parallel for
+    for (int i_ref = 0; i_ref < m->n_ref; ++i_ref) {
+      const int* sl = m->ref_slot + 5 * i_ref;
+      const bool str[2] = {bool(m->ref_stretch[2 * i_ref]), bool(m->ref_stretch[2 * i_ref + 1])};
+      double* coarse = mv.slot(sl[0]) + off * (nd + 2) * nfq;
+      int nf = n_face;
+      for (int i_dim = 0; i_dim < nd - 1; ++i_dim) nf /= 1 + str[i_dim];
+      for (int i_face = 0; i_face < nf; ++i_face) {
+        double* fine = mv.slot(sl[1 + i_face]) + off * (nd + 2) * nfq;
+        for (int v = 0; v < n_var; ++v) {
+          double* var_face = fine + v * nfq;
+          for (int q = 0; q < nfq; ++q) var_face[q] = coarse[v * nfq + q];
+          for (int i_dim = 0; i_dim < nd - 1; ++i_dim) {
+            if (str[i_dim]) {
+              for (int q = 0; q < nfq; ++q) var_face[q] *= 1 + scl;
+            } else {
+              const int pw = nd - 2 - i_dim;
+              const int face_stride = str[nd - 2] ? 1 : ipow(2, pw);
+              const int qpoint_stride = ipow(rs, pw);
+              const int i_half = (i_face / face_stride) % 2;
+              const double* mat = basis.prol + i_half * rs * rs;
+              for (int i_outer = 0; i_outer < nfq / (rs * qpoint_stride); ++i_outer)
+                for (int i_inner = 0; i_inner < qpoint_stride; ++i_inner) {
+                  double row[rs], out[rs];
+                  for (int k = 0; k < rs; ++k) row[k] = var_face[(i_outer * rs + k) * qpoint_stride + i_inner];
+                  for (int i = 0; i < rs; ++i) {
+                    double acc = 0;
+                    for (int k = 0; k < rs; ++k) acc += mat[i * rs + k] * row[k];
+                    out[i] = acc;
+                  }
+                  for (int k = 0; k < rs; ++k) var_face[(i_outer * rs + k) * qpoint_stride + i_inner] = out[k];
+                }
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // Spatial::Restrict_refined (Spatial.hpp:230-285)
+  static void restrict_(hx_host_mesh* m, int n_var, bool scl, bool off)
+  {
+    Basis basis(rs);
+    Mesh_view mv{m};
+    constexpr int n_face = ipow(2, nd - 1);
+    #pragma omp parallel for
+    for (int i_ref = 0; i_ref < m->n_ref; ++i_ref) {
+      const int* sl = m->ref_slot + 5 * i_ref;
+      const bool str[2] = {bool(m->ref_stretch[2 * i_ref]), bool(m->ref_stretch[2 * i_ref + 1])};
+      double* coarse = mv.slot(sl[0]) + off * (nd + 2) * nfq;
+      for (int i = 0; i < n_var * nfq; ++i) coarse[i] = 0.;
+      int nf = n_face;
+      for (int i_dim = 0; i_dim < nd - 1; ++i_dim) nf /= 1 + str[i_dim];
+      for (int i_face = 0; i_face < nf; ++i_face) {
+        double* fine = mv.slot(sl[1 + i_face]) + off * (nd + 2) * nfq;
+        for (int v = 0; v < n_var; ++v) {
+          double* var_face = fine + v * nfq;
+          for (int i_dim = 0; i_dim < nd - 1; ++i_dim) {
+            if (str[i_dim]) {
+              for (int q = 0; q < nfq; ++q) var_face[q] /= 1 + scl;
+            } else {
+              const int pw = nd - 2 - i_dim;
+              const int face_stride = str[nd - 2] ? 1 : ipow(2, pw);
+              const int qpoint_stride = ipow(rs, pw);
+              const int i_half = (i_face / face_stride) % 2;
+              const double* mat = basis.restr + i_half * rs * rs;
+              for (int i_outer = 0; i_outer < nfq / (rs * qpoint_stride); ++i_outer)
+                for (int i_inner = 0; i_inner < qpoint_stride; ++i_inner) {
+                  double row[rs], out[rs];
+                  for (int k = 0; k < rs; ++k) row[k] = var_face[(i_outer * rs + k) * qpoint_stride + i_inner];
+                  for (int i = 0; i < rs; ++i) {
+                    double acc = 0;
+                    for (int k = 0; k < rs; ++k) acc += mat[i * rs + k] * row[k];
+                    out[i] = acc;
+                  }
+                  for (int k = 0; k < rs; ++k) var_face[(i_outer * rs + k) * qpoint_stride + i_inner] = out[k];
+                }
+            }
+          }
+          for (int q = 0; q < nfq; ++q) coarse[v * nfq + q] += var_face[q];
+        }
+      }
+    }
+  }
+};
+
+// ---- dispatch ---------------------------------------------------------------------------------------------
+
+template <template <int, int> class F, typename... A>
+auto dispatch(int nd, int rs, A&&... args)
+{
+  switch (nd * 16 + rs) {
+#define HX_CASE(ND, RS) \
+  case ND * 16 + RS: return F<ND, RS>::run(std::forward<A>(args)...);
+#define HX_CASES(ND) HX_CASE(ND, 2) HX_CASE(ND, 3) HX_CASE(ND, 4) HX_CASE(ND, 5) HX_CASE(ND, 6) HX_CASE(ND, 7) HX_CASE(ND, 8)
+    HX_CASES(1) HX_CASES(2) HX_CASES(3)
+#undef HX_CASES
+#undef HX_CASE
+  }
+  throw std::runtime_error("demand for invalid kernel");
+}
+
+void check_mesh(const hx_host_mesh* m)
+{
+  if (!m || m->n_dim < 1 || m->n_dim > 3 || m->row_size < 2 || m->row_size > 8)
+    throw std::runtime_error("demand for invalid kernel");
+}
+
+bool is_ns(const hx_pde* pde)
+{
+  if (!pde || pde->kind == HX_PDE_EULER) return false;
+  if (pde->kind == HX_PDE_NAVIER_STOKES) return true;
+  throw std::runtime_error("oracle: PDE kind not restated");
+}
+
+template <int nd, bool visc>
+Navier_stokes<nd, visc> make_eq(const hx_pde* pde)
+{
+  Navier_stokes<nd, visc> eq;
+  if (pde && pde->kind == HX_PDE_NAVIER_STOKES) {
+    eq.dyn_visc = Transport(pde->visc);
+    eq.therm_cond = Transport(pde->cond);
+  }
+  return eq;
+}
+
+template <int nd, int rs>
+struct Run_write_face { static int run(hx_host_mesh* m, int cls) { Ops<nd, rs>::write_face(m, cls); return 0; } };
+
+template <int nd, int rs>
+struct Run_local
+{
+  static int run(hx_host_mesh* m, int cls, const hx_pde* pde, double dt, int stage, int cr, int uf)
+  {
+    if (is_ns(pde)) Ops<nd, rs>::template local<true>(m, cls, make_eq<nd, true>(pde), dt, stage, cr, uf);
+    else Ops<nd, rs>::template local<false>(m, cls, make_eq<nd, false>(pde), dt, stage, cr, uf);
+    return 0;
+  }
+};
+
+template <int nd, int rs>
+struct Run_neighbor
+{
+  static int run(hx_host_mesh* m, int cls, const hx_pde* pde)
+  {
+    if (is_ns(pde)) Ops<nd, rs>::template neighbor<true>(m, cls, make_eq<nd, true>(pde));
+    else Ops<nd, rs>::template neighbor<false>(m, cls, make_eq<nd, false>(pde));
+    return 0;
+  }
+};
+
+template <int nd, int rs>
+struct Run_neighbor_reconcile { static int run(hx_host_mesh* m, int cls) { Ops<nd, rs>::neighbor_reconcile(m, cls); return 0; } };
+
+template <int nd, int rs>
+struct Run_reconcile
+{
+  static int run(hx_host_mesh* m, int cls, double dt, int cr, int uf)
+  {
+    Ops<nd, rs>::template reconcile_ldg_flux<true>(m, cls, dt, cr, uf);
+    return 0;
+  }
+};
+
+template <int nd, int rs>
+struct Run_max_dt
+{
+  static double run(hx_host_mesh* m, int cls, const hx_pde* pde, int is_local, double sc, double sd)
+  {
+    if (is_ns(pde)) return Ops<nd, rs>::template max_dt<true>(m, cls, make_eq<nd, true>(pde), is_local, sc, sd);
+    return Ops<nd, rs>::template max_dt<false>(m, cls, make_eq<nd, false>(pde), is_local, sc, sd);
+  }
+};
+
+template <int nd, int rs>
+struct Run_prolong { static int run(hx_host_mesh* m, int nvar, int s, int o) { Ops<nd, rs>::prolong(m, nvar, s, o); return 0; } };
+template <int nd, int rs>
+struct Run_restrict { static int run(hx_host_mesh* m, int nvar, int s, int o) { Ops<nd, rs>::restrict_(m, nvar, s, o); return 0; } };
+
+int n_extrap(const hx_host_mesh* m, const hx_pde* pde)
+{
+  int kind = pde ? pde->kind : HX_PDE_EULER;
+  if (kind == HX_PDE_EULER || kind == HX_PDE_NAVIER_STOKES) return m->n_dim + 2;
+  throw std::runtime_error("oracle: PDE kind not restated");
+}
+
+// Boundary conditions on the deformed-class boundary connections (Boundary_condition.cpp)
+void apply_state_bcs(hx_host_mesh* m)
+{
+  Mesh_view mv{m};
+  const int nd = m->n_dim, nv = nd + 2, nfq = ipow(m->row_size, nd - 1);
+  #pragma omp parallel for
+  for (int i = 0; i < m->n_bc_con; ++i) {
+    int con = m->bc_con[i];
+    const hx_bc& bc = m->bcs[m->bc_serial[i]];
+    double* in_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con]);
+    double* gh_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con + 1]);
+    const double* nrml = mv.nslot(m->con_nslot[2 * con]);
+    switch (bc.kind) {
+      case HX_BC_FREESTREAM:  // Boundary_condition.cpp:66-76
+        for (int v = 0; v < nv; ++v)
+          for (int q = 0; q < nfq; ++q) gh_f[v * nfq + q] = bc.state[v];
+        break;
+      case HX_BC_COPY:     // copy_state (Boundary_condition.cpp:12-22): both halves
+      case HX_BC_OUTFLOW:
+        for (int l = 0; l < 2; ++l)
+          for (int k = 0; k < nv * nfq; ++k) gh_f[l * nv * nfq + k] = in_f[l * nv * nfq + k];
+        break;
+      case HX_BC_NONPENETRATION:  // reflect_momentum (Boundary_condition.cpp:302-326)
+        for (int k = 0; k < nv * nfq; ++k) gh_f[k] = in_f[k];
+        for (int q = 0; q < nfq; ++q) {
+          double dot = 0., norm_sq = 0.;
+          for (int d = 0; d < nd; ++d) {
+            double n = nrml[d * nfq + q];
+            dot += gh_f[d * nfq + q] * n;
+            norm_sq += n * n;
+          }
+          for (int d = 0; d < nd; ++d) gh_f[d * nfq + q] -= 2 * dot * nrml[d * nfq + q] / norm_sq;
+        }
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+void apply_flux_bcs(hx_host_mesh* m)
+{
+  Mesh_view mv{m};
+  const int nd = m->n_dim, nv = nd + 2, nfq = ipow(m->row_size, nd - 1);
+  #pragma omp parallel for
+  for (int i = 0; i < m->n_bc_con; ++i) {
+    int con = m->bc_con[i];
+    const hx_bc& bc = m->bcs[m->bc_serial[i]];
+    double* in_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con]);
+    double* gh_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con + 1]);
+    const double* nrml = mv.nslot(m->con_nslot[2 * con]);
+    switch (bc.kind) {
+      case HX_BC_FREESTREAM:  // Freestream::apply_flux = copy_state (Boundary_condition.cpp:299)
+      case HX_BC_COPY:
+        for (int l = 0; l < 2; ++l)
+          for (int k = 0; k < nv * nfq; ++k) gh_f[l * nv * nfq + k] = in_f[l * nv * nfq + k];
+        break;
+      case HX_BC_OUTFLOW:  // Boundary_condition.cpp:470-477
+        for (int k = 0; k < nv * nfq; ++k) gh_f[nv * nfq + k] = -in_f[nv * nfq + k];
+        break;
+      case HX_BC_NONPENETRATION: {  // Boundary_condition.cpp:333-345
+        double* g = gh_f + nv * nfq;
+        const double* f = in_f + nv * nfq;
+        for (int k = 0; k < nv * nfq; ++k) g[k] = -f[k];
+        for (int q = 0; q < nfq; ++q) {
+          double dot = 0., norm_sq = 0.;
+          for (int d = 0; d < nd; ++d) {
+            double n = nrml[d * nfq + q];
+            dot += g[d * nfq + q] * n;
+            norm_sq += n * n;
+          }
+          for (int d = 0; d < nd; ++d) g[d * nfq + q] -= 2 * dot * nrml[d * nfq + q] / norm_sq;
+        }
+        break;
+      }
+      default:
+        break;
+    }
+  }
+}
+
+template <typename Fn>
+int guarded(Fn&& fn)
+{
+  try {
+    fn();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+hx_pde euler_pde()
+{
+  hx_pde p{};
+  p.kind = HX_PDE_EULER;
+  return p;
+}
+
+hx_pde ns_pde(hx_transport visc, hx_transport cond)
+{
+  hx_pde p{};
+  p.kind = HX_PDE_NAVIER_STOKES;
+  p.visc = visc;
+  p.cond = cond;
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+void oracle_set_threads(int n) { omp_set_num_threads(n); }
+int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+int oracle_write_face(hx_host_mesh* m, int cls, const hx_pde* pde)
+{
+  return guarded([&] {
+    check_mesh(m);
+    n_extrap(m, pde);
+    dispatch<Run_write_face>(m->n_dim, m->row_size, m, cls);
+  });
+}
+
+int oracle_neighbor(hx_host_mesh* m, int cls, const hx_pde* pde, int)
+{
+  return guarded([&] {
+    check_mesh(m);
+    dispatch<Run_neighbor>(m->n_dim, m->row_size, m, cls, pde);
+  });
+}
+
+int oracle_local(hx_host_mesh* m, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual, int use_filter)
+{
+  return guarded([&] {
+    check_mesh(m);
+    // HEXED_ASSERTs of the Local constructor (Spatial.hpp:322-323)
+    if (is_ns(pde) && i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+    if (i_stage && compute_residual) throw std::runtime_error("residual calculation is a single-stage operation");
+    dispatch<Run_local>(m->n_dim, m->row_size, m, cls, pde, dt, i_stage, compute_residual, use_filter);
+  });
+}
+
+int oracle_neighbor_reconcile(hx_host_mesh* m, int cls, const hx_pde*)
+{
+  return guarded([&] {
+    check_mesh(m);
+    dispatch<Run_neighbor_reconcile>(m->n_dim, m->row_size, m, cls);
+  });
+}
+
+int oracle_reconcile_ldg_flux(hx_host_mesh* m, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual,
+                              int use_filter)
+{
+  return guarded([&] {
+    check_mesh(m);
+    if (is_ns(pde) && i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+    dispatch<Run_reconcile>(m->n_dim, m->row_size, m, cls, dt, compute_residual, use_filter);
+  });
+}
+
+int oracle_max_dt(hx_host_mesh* m, int cls, const hx_pde* pde, int is_local, int, double safety_conv, double safety_diff,
+                  double* dt_out)
+{
+  return guarded([&] {
+    check_mesh(m);
+    *dt_out = dispatch<Run_max_dt>(m->n_dim, m->row_size, m, cls, pde, is_local, safety_conv, safety_diff);
+  });
+}
+
+int oracle_prolong_refined(hx_host_mesh* m, const hx_pde* pde, int scale, int offset)
+{
+  return guarded([&] {
+    check_mesh(m);
+    dispatch<Run_prolong>(m->n_dim, m->row_size, m, n_extrap(m, pde), scale, offset);
+  });
+}
+
+int oracle_restrict_refined(hx_host_mesh* m, const hx_pde* pde, int scale, int offset)
+{
+  return guarded([&] {
+    check_mesh(m);
+    dispatch<Run_restrict>(m->n_dim, m->row_size, m, n_extrap(m, pde), scale, offset);
+  });
+}
+
+int oracle_apply_state_bcs(hx_host_mesh* m) { return guarded([&] { apply_state_bcs(m); }); }
+int oracle_apply_flux_bcs(hx_host_mesh* m) { return guarded([&] { apply_flux_bcs(m); }); }
+
+// compute_euler (kernels_convective.cpp:8-18)
+int oracle_compute_euler(hx_host_mesh* m, const hx_kernel_options* o)
+{
+  return guarded([&] {
+    check_mesh(m);
+    hx_pde p = euler_pde();
+    const int nd = m->n_dim, rs = m->row_size;
+    dispatch<Run_neighbor>(nd, rs, m, HX_CARTESIAN, &p);
+    dispatch<Run_neighbor>(nd, rs, m, HX_DEFORMED, &p);
+    dispatch<Run_restrict>(nd, rs, m, nd + 2, 1, 0);
+    if (o->i_stage && o->compute_residual) throw std::runtime_error("residual calculation is a single-stage operation");
+    dispatch<Run_local>(nd, rs, m, HX_CARTESIAN, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter);
+    dispatch<Run_local>(nd, rs, m, HX_DEFORMED, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter);
+    dispatch<Run_prolong>(nd, rs, m, nd + 2, 0, 0);
+  });
+}
+
+// compute_navier_stokes (kernels_diffusive.cpp:8-29), flux_bc = Solver::apply_flux_bcs
+int oracle_compute_navier_stokes(hx_host_mesh* m, const hx_kernel_options* o, hx_transport visc, hx_transport cond)
+{
+  return guarded([&] {
+    check_mesh(m);
+    hx_pde p = ns_pde(visc, cond);
+    const int nd = m->n_dim, rs = m->row_size;
+    if (o->i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+    dispatch<Run_neighbor>(nd, rs, m, HX_CARTESIAN, &p);
+    dispatch<Run_neighbor>(nd, rs, m, HX_DEFORMED, &p);
+    dispatch<Run_restrict>(nd, rs, m, nd + 2, 1, 0);
+    dispatch<Run_restrict>(nd, rs, m, nd + 2, 0, 1);
+    dispatch<Run_local>(nd, rs, m, HX_CARTESIAN, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter);
+    dispatch<Run_local>(nd, rs, m, HX_DEFORMED, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter);
+    if (!o->i_stage) {
+      dispatch<Run_prolong>(nd, rs, m, nd + 2, 1, 1);
+      apply_flux_bcs(m);
+      dispatch<Run_neighbor_reconcile>(nd, rs, m, HX_CARTESIAN);
+      dispatch<Run_neighbor_reconcile>(nd, rs, m, HX_DEFORMED);
+      dispatch<Run_restrict>(nd, rs, m, nd + 2, 1, 1);
+      dispatch<Run_reconcile>(nd, rs, m, HX_CARTESIAN, o->dt, o->compute_residual, o->use_filter);
+      dispatch<Run_reconcile>(nd, rs, m, HX_DEFORMED, o->dt, o->compute_residual, o->use_filter);
+    }
+    dispatch<Run_prolong>(nd, rs, m, nd + 2, 0, 0);
+  });
+}
+
+// max_dt_euler / max_dt_navier_stokes (kernels_max_dt.cpp:8-17)
+int oracle_max_dt_euler(hx_host_mesh* m, const hx_kernel_options*, double msc, double msd, int local_time, double* dt_out)
+{
+  return guarded([&] {
+    check_mesh(m);
+    hx_pde p = euler_pde();
+    double a = dispatch<Run_max_dt>(m->n_dim, m->row_size, m, HX_CARTESIAN, &p, local_time, msc, msd);
+    double b = dispatch<Run_max_dt>(m->n_dim, m->row_size, m, HX_DEFORMED, &p, local_time, msc, msd);
+    *dt_out = std::min(a, b);
+  });
+}
+
+int oracle_max_dt_navier_stokes(hx_host_mesh* m, const hx_kernel_options*, double msc, double msd, int local_time,
+                                hx_transport visc, hx_transport cond, double* dt_out)
+{
+  return guarded([&] {
+    check_mesh(m);
+    hx_pde p = ns_pde(visc, cond);
+    double a = dispatch<Run_max_dt>(m->n_dim, m->row_size, m, HX_CARTESIAN, &p, local_time, msc, msd);
+    double b = dispatch<Run_max_dt>(m->n_dim, m->row_size, m, HX_DEFORMED, &p, local_time, msc, msd);
+    *dt_out = std::min(a, b);
+  });
+}
+
+// Solver::update (Solver.cpp:834-886) without fix_admissibility (off by default, Solver.cpp:143)
+int oracle_update(hx_host_mesh* m, const hx_update_options* o, hx_update_status* st)
+{
+  for (int i_flow = 0; i_flow < o->flow_iters; ++i_flow) {
+    double safety = o->max_safety;
+    double n_cheby = o->n_cheby_flow;
+    double max_cheby = chebyshev_step(n_cheby, n_cheby - 1);
+    for (int i_cheby = 0; i_cheby < n_cheby; ++i_cheby) {
+      hx_kernel_options ko{0., 0, 0, o->use_filter};
+      double mdt;
+      int rc = o->use_ldg ? oracle_max_dt_navier_stokes(m, &ko, safety / max_cheby, safety, o->local_time, o->visc, o->cond, &mdt)
+                          : oracle_max_dt_euler(m, &ko, safety / max_cheby, safety, o->local_time, &mdt);
+      if (rc) return rc;
+      double nominal_dt = std::min(mdt, o->max_time_step);
+      double dt = nominal_dt * chebyshev_step(n_cheby, i_cheby);
+      for (int i = 0; i < 2; ++i) {
+        hx_kernel_options opts{dt, i, 0, o->use_filter};
+        if ((rc = oracle_apply_state_bcs(m))) return rc;
+        if (o->use_ldg && !i) rc = oracle_compute_navier_stokes(m, &opts, o->visc, o->cond);
+        else rc = oracle_compute_euler(m, &opts);
+        if (rc) return rc;
+      }
+      st->time_step = dt;
+      st->flow_time += dt;
+    }
+  }
+  ++st->iteration;
+  return 0;
+}
+
+// Solver::compute_residual (Solver.cpp:899-913)
+int oracle_compute_residual(hx_host_mesh* m, const hx_update_options* o)
+{
+  int rc = oracle_apply_state_bcs(m);
+  if (rc) return rc;
+  hx_kernel_options opts{1., 0, 1, o->use_filter};
+  if (o->use_ldg) return oracle_compute_navier_stokes(m, &opts, o->visc, o->cond);
+  return oracle_compute_euler(m, &opts);
+}
+
+}  // extern "C"

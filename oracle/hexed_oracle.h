@@ -1,0 +1,47 @@
+/*
+ * TEST INFRASTRUCTURE ONLY. CPU restatement of hexed's DG hot path, used as the parity checker for the
+ * HIP implementation and as the CPU baseline in bench.py. Nothing in the product links or calls it.
+ *
+ * Each function mirrors the reference item named beside it, operating on the same host mesh layout as the
+ * product ABI (include/hexed_amd.h). Functions return 0 on success, nonzero with oracle_last_error() set.
+ */
+#ifndef HEXED_ORACLE_H_
+#define HEXED_ORACLE_H_
+
+#include "../include/hexed_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* oracle_last_error(void);
+void oracle_set_threads(int n);
+int oracle_get_threads(void);
+
+int oracle_write_face(hx_host_mesh* m, int cls, const hx_pde* pde);                        /* Spatial.hpp:29-71   */
+int oracle_neighbor(hx_host_mesh* m, int cls, const hx_pde* pde, int i_stage);             /* Spatial.hpp:601-705 */
+int oracle_local(hx_host_mesh* m, int cls, const hx_pde* pde, double dt, int i_stage,      /* Spatial.hpp:293-510 */
+                 int compute_residual, int use_filter);
+int oracle_neighbor_reconcile(hx_host_mesh* m, int cls, const hx_pde* pde);               /* Spatial.hpp:709-760 */
+int oracle_reconcile_ldg_flux(hx_host_mesh* m, int cls, const hx_pde* pde, double dt,     /* Spatial.hpp:513-595 */
+                              int i_stage, int compute_residual, int use_filter);
+int oracle_max_dt(hx_host_mesh* m, int cls, const hx_pde* pde, int is_local, int use_filter, /* Spatial.hpp:763-829 */
+                  double safety_conv, double safety_diff, double* dt_out);
+int oracle_prolong_refined(hx_host_mesh* m, const hx_pde* pde, int scale, int offset);    /* Spatial.hpp:138-207 */
+int oracle_restrict_refined(hx_host_mesh* m, const hx_pde* pde, int scale, int offset);   /* Spatial.hpp:215-286 */
+int oracle_apply_state_bcs(hx_host_mesh* m);                                               /* Solver.cpp:56-67    */
+int oracle_apply_flux_bcs(hx_host_mesh* m);                                                /* Solver.cpp:69-81    */
+int oracle_compute_euler(hx_host_mesh* m, const hx_kernel_options* o);                    /* kernels_convective.cpp:8-18 */
+int oracle_compute_navier_stokes(hx_host_mesh* m, const hx_kernel_options* o,             /* kernels_diffusive.cpp:8-29  */
+                                 hx_transport visc, hx_transport cond);
+int oracle_max_dt_euler(hx_host_mesh* m, const hx_kernel_options* o, double msc, double msd, /* kernels_max_dt.cpp:8-17 */
+                        int local_time, double* dt_out);
+int oracle_max_dt_navier_stokes(hx_host_mesh* m, const hx_kernel_options* o, double msc, double msd,
+                                int local_time, hx_transport visc, hx_transport cond, double* dt_out);
+int oracle_update(hx_host_mesh* m, const hx_update_options* o, hx_update_status* st);     /* Solver.cpp:834-886  */
+int oracle_compute_residual(hx_host_mesh* m, const hx_update_options* o);                 /* Solver.cpp:899-913  */
+
+#ifdef __cplusplus
+}
+#endif
+#endif
