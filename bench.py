@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""DOF-stage/s of hexed's explicit time step (Solver::update, two stages per step) on one MI355X.
+
+A step is one Solver::update on the whole mesh: max_dt, then for each of the two stages the state boundary
+conditions and the compute_navier_stokes (stage 0) / compute_euler (stage 1) kernel sequence. One DOF is one
+quadrature point carrying all n_dim + 2 conserved variables; value = DOF x 2 stages x steps / wall seconds, with
+the mesh and state resident in HBM before the timed region starts.
+
+The JSON line also carries
+  roofline:     the dominant kernel's algorithmic HBM bytes per launch / its average launch time, measured with
+                HIP events on the library's stream over the timed region, against the 8 TB/s HBM peak;
+  cpu_baseline: the C++/OpenMP oracle (a loop-for-loop restatement of the reference CPU path) timed on this
+                host's cores for a few steps of the same mesh;
+  parity:       relative L2 error per conserved variable between the GPU and the oracle after those steps.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32] [--cpu-steps N]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from hexed_amd import abi, configs  # noqa: E402
+from hexed_amd.device import Context, Device_mesh  # noqa: E402
+
+METRIC = "DOF-stage/s, 3D NS row_size 6: FP32-MFMA vs FP64 GPU at stated L2 err"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "ns48": dict(
+        desc="3D Navier-Stokes, 48^3 deformed hexes (interior vertices jittered +-10% of spacing), row_size 6, "
+             "Re 1e4 on unit length, Mach 0.3, Sutherland air, constant bulk artificial viscosity 1e-3 m^2/s, "
+             "1% seeded Fourier perturbations, freestream boundaries, global time step at max_safety 0.7",
+        build=lambda: configs.ns_deformed(48), visc=True, cpu_steps=6, bytes_per_dof_stage=224),
+    "euler32": dict(
+        desc="3D Euler, 32^3 Cartesian hexes, row_size 6, Mach 0.8 + 1% seeded Fourier perturbations, "
+             "freestream boundaries, global time step at max_safety 0.7",
+        build=lambda: configs.euler_box(32), visc=False, cpu_steps=6, bytes_per_dof_stage=128),
+}
+
+
+def algorithmic_bytes(category, nd, nq, n_elem):
+    """Compulsory HBM bytes of one launch of an element kernel (face buffers and other intermediates are not
+    counted): the state read and written, the residual cache written (stage 0) or read (stage 1), the time step
+    scale, the two artificial-viscosity coefficients for Navier-Stokes, and for deformed elements the nd*nd
+    reference-level normals plus the Jacobian determinant."""
+    nv = nd + 2
+    parts = category.split("/")
+    if len(parts) != 3 or parts[1] != "local":
+        return None
+    doubles = 3 * nv + 1
+    if parts[2] == "navier_stokes":
+        doubles += 2
+    if parts[0] == "deformed":
+        doubles += nd * nd + 1
+    return 8.0 * doubles * nq * n_elem
+
+
+def run_oracle_steps(m, opts, steps):
+    import oracle
+    lib = oracle.load()
+    s = m.struct()
+    st = abi.hx_update_status()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        oracle.check(lib.oracle_update(ctypes.byref(s), ctypes.byref(opts), ctypes.byref(st)))
+    return time.perf_counter() - t0, oracle.load().oracle_get_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="ns48", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps for the CPU baseline (0 to skip)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU figures are not produced "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
+        sys.exit(2)
+    wl = WORKLOADS[args.config]
+    cpu_steps = wl["cpu_steps"] if args.cpu_steps is None else args.cpu_steps
+
+    t_build = time.perf_counter()
+    m = wl["build"]()
+    nd, nq = m.n_dim, m.nq
+    n_elem = sum(m.n_elem)
+    n_dof = n_elem * nq
+    if wl["visc"]:
+        visc, cond = configs.sutherland_air()
+        opts = abi.update_options(max_safety=0.7, visc=visc, cond=cond, use_ldg=True)
+    else:
+        opts = abi.update_options(max_safety=0.7)
+    print(f"[bench] {args.config}: {n_elem} elements, {n_dof} DOF, mesh built in {time.perf_counter() - t_build:.1f} s",
+          file=sys.stderr, flush=True)
+
+    ctx = Context(0)
+    dev = Device_mesh(ctx, m)
+    dev.compute_write_face()  # Solver::initialize extrapolates the initial state to the faces
+    init = {"state": [a.copy() for a in m.state], "faces": None}
+    status = abi.hx_update_status()
+    for _ in range(args.warmup):
+        dev.update(opts, status)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.update(opts, status)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    timings = ctx.timings()
+    value = n_dof * 2 * args.steps / elapsed
+    print(f"[bench] GPU: {args.steps} steps in {elapsed:.3f} s, dt {status.time_step:.3e}", file=sys.stderr, flush=True)
+
+    # dominant kernel and its roofline
+    dominant = max(timings, key=lambda k: timings[k][0])
+    ms, launches, units = timings[dominant]
+    avg_ms = ms / launches
+    cls = abi.HX_DEFORMED if dominant.startswith("deformed") else abi.HX_CARTESIAN
+    alg = algorithmic_bytes(dominant, nd, nq, m.n_elem[cls])
+    roofline = {"bound": "hbm", "kernel": dominant, "launch_ms": round(avg_ms, 5), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "traffic": None}
+    if alg is not None:
+        achieved = alg / (avg_ms * 1e-3) / 1e9
+        roofline.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes": alg})
+    step_gbs = value * wl["bytes_per_dof_stage"] / 1e9
+    roofline["step"] = {"bytes_per_dof_stage": wl["bytes_per_dof_stage"], "achieved": round(step_gbs, 1),
+                        "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
+    breakdown = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches": v[1]} for k, v in sorted(timings.items())}
+
+    # CPU baseline and full-size parity: the oracle runs cpu_steps from the initial state; the GPU repeats them
+    cpu = None
+    parity = None
+    if cpu_steps > 0:
+        import oracle
+        lib = oracle.load()
+        s = m.struct()
+        for c in (0, 1):
+            oracle.check(lib.oracle_write_face(ctypes.byref(s), c, None))
+        init["faces"] = m.faces.copy()
+        cpu_s, threads = run_oracle_steps(m, opts, cpu_steps)
+        cpu = {"value": round(n_dof * 2 * cpu_steps / cpu_s, 1), "unit": "DOF-stage/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{cpu_steps} time steps ({2 * cpu_steps} stages) of the same {n_elem}-element mesh, "
+                         f"FP64, OpenMP over {threads} threads, {cpu_s:.2f} s"}
+        print(f"[bench] CPU oracle: {cpu_steps} steps in {cpu_s:.2f} s on {threads} threads", file=sys.stderr, flush=True)
+        # GPU from the same initial state for the same steps
+        for c in (0, 1):
+            if m.n_elem[c]:
+                dev.upload("state", c, init["state"][c])
+                dev.upload("rcache", c, np.zeros_like(m.rcache[c]))
+                dev.upload("tss", c, np.ones_like(m.tss[c]))
+        dev.upload("faces", 0, init["faces"])
+        st = abi.hx_update_status()
+        for _ in range(cpu_steps):
+            dev.update(opts, st)
+        errs = []
+        for c in (0, 1):
+            if m.n_elem[c]:
+                got = dev.download("state", c, out=np.zeros_like(m.state[c]))
+                for v in range(m.nv):
+                    ref = m.state[c][:, v]
+                    errs.append(float(np.linalg.norm(got[:, v] - ref) / np.linalg.norm(ref)))
+        parity = {"steps": cpu_steps, "rel_l2_per_var": [float(f"{e:.3e}") for e in errs], "max_rel_l2": max(errs)}
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "DOF-stage/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": wl["desc"], "name": args.config, "n_elem": n_elem, "dof": n_dof, "row_size": m.row_size,
+                   "precision": "fp64 (reference arithmetic)"},
+        "roofline": roofline, "cpu_baseline": cpu, "parity_vs_cpu": parity, "kernel_breakdown": breakdown,
+    }
+    print(json.dumps(line))
+    dev.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+"""Workload generators for the benchmark configurations (synthetic, seeded).
+
+Random numbers come from splitmix64 (Steele, Lea & Flood 2014) mapped to [0, 1) by the top 53 bits, so a
+configuration is the same on every platform and library version.
+
+  config 1  2D Euler isentropic vortex, 16x16 Cartesian quads on [-5, 5]^2, row size 6
+  config 2  3D Euler, 32^3 Cartesian hexes on [0, 1]^3, row size 6, Mach 0.8 + 1% Fourier perturbations
+  config 3  3D Navier-Stokes, 48^3 deformed hexes (interior vertices jittered +-10% of spacing), row size 6,
+            Re 1e4 on unit length, Mach 0.3, Sutherland air, constant artificial viscosity
+"""
+import math
+
+import numpy as np
+
+from . import abi, mesh
+
+GAMMA = 1.4
+R_AIR = 287.05287  # constants.hpp:44
+
+
+def splitmix64(seed, n):
+    """n uniform doubles in [0, 1) from the splitmix64 sequence started at `seed`."""
+    mask = (1 << 64) - 1
+    x = seed & mask
+    out = np.empty(n)
+    for i in range(n):
+        x = (x + 0x9E3779B97F4A7C15) & mask
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        z ^= z >> 31
+        out[i] = (z >> 11) * (1.0 / (1 << 53))
+    return out
+
+
+def splitmix64_np(seed, n):
+    """Vectorised splitmix64: the i-th output uses state seed + (i+1)*golden (identical to `splitmix64`)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed & ((1 << 64) - 1)) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def freestream_state(nd, rho, vel, p):
+    ke = 0.5 * rho * sum(v * v for v in vel)
+    return [rho * v for v in vel] + [rho, p / (GAMMA - 1) + ke]
+
+
+def fourier_perturbation(pos, seed, n_modes=8, amplitude=0.01):
+    """Sum over axes and modes 1..n_modes of seeded amplitude/phase sinusoids on the unit box, scaled so the
+    sum of |amplitudes| is `amplitude` (the perturbation never exceeds it)."""
+    nd = len(pos)
+    r = splitmix64_np(seed, 2 * nd * n_modes).reshape(nd, n_modes, 2)
+    amp = r[:, :, 0] + 0.5
+    amp *= amplitude / amp.sum()
+    phase = 2 * math.pi * r[:, :, 1]
+    cp, sp = np.cos(phase), np.sin(phase)
+    flat = [np.ascontiguousarray(p).reshape(-1) for p in pos]
+    out = np.zeros(flat[0].size)
+    chunk = 1 << 16  # keep the temporaries cache resident
+    for lo in range(0, out.size, chunk):
+        acc = out[lo:lo + chunk]
+        for d in range(nd):
+            # sin(k t + p) = sin(k t) cos p + cos(k t) sin p, sin/cos(k t) from the angle-addition recurrence
+            t = 2 * math.pi * flat[d][lo:lo + chunk]
+            s1, c1 = np.sin(t), np.cos(t)
+            sk, ck = s1, c1
+            for k in range(n_modes):
+                acc += amp[d, k] * (sk * cp[d, k] + ck * sp[d, k])
+                sk, ck = sk * c1 + ck * s1, ck * c1 - sk * s1
+    return out.reshape(np.shape(pos[0]))
+
+
+def euler_box(n=32, rs=6, mach=0.8, seed=0, nd=3, deformed=False, jitter=0.0):
+    """Config 2: freestream rho=1, p=1/gamma (sound speed 1), velocity Mach*x, with 1% seeded Fourier
+    perturbations of density and each velocity component; freestream boundary conditions."""
+    fs = freestream_state(nd, 1.0, [mach] + [0.0] * (nd - 1), 1.0 / GAMMA)
+    m = mesh.box(nd, rs, n, 0.0, 1.0, deformed=deformed, jitter=jitter, seed=seed,
+                 bc=(abi.HX_BC_FREESTREAM, fs))
+
+    def init(pos):
+        rho = 1.0 * (1 + fourier_perturbation(pos, seed * 16 + 1))
+        vel = [(mach if d == 0 else 0.0) + mach * fourier_perturbation(pos, seed * 16 + 2 + d) for d in range(nd)]
+        p = 1.0 / GAMMA
+        return [rho * v for v in vel] + [rho, p / (GAMMA - 1) + 0.5 * rho * sum(v * v for v in vel)]
+    mesh.set_state(m, init)
+    return m
+
+
+def sutherland_air():
+    """Sutherland viscosity and conductivity of air (Case.hil:83-90)."""
+    return abi.sutherland(1.716e-5, 273., 111.), abi.sutherland(0.0241, 273., 194.)
+
+
+def ns_freestream(mach=0.3, reynolds=1e4, length=1.0, temperature=288.15):
+    """Dimensional freestream with the requested Mach and Reynolds numbers: returns (rho, U, p, mu)."""
+    c = math.sqrt(GAMMA * R_AIR * temperature)
+    u = mach * c
+    mu = 1.716e-5 * (temperature / 273.) ** 1.5 * (273. + 111.) / (temperature + 111.)
+    rho = reynolds * mu / (u * length)
+    p = rho * R_AIR * temperature
+    return rho, u, p, mu
+
+
+def ns_deformed(n=48, rs=6, mach=0.3, reynolds=1e4, jitter=0.1, seed=0, art_visc=1e-3, nd=3):
+    """Config 3: deformed box with interior vertices jittered by U(+-jitter*h), Navier-Stokes freestream with
+    1% perturbations, freestream boundaries, constant bulk artificial viscosity `art_visc` [m^2/s]
+    (Solver::set_art_visc_constant, Solver.cpp:415-426)."""
+    rho, u, p, _ = ns_freestream(mach, reynolds)
+    fs = freestream_state(nd, rho, [u] + [0.0] * (nd - 1), p)
+    m = mesh.box(nd, rs, n, 0.0, 1.0, deformed=True, jitter=jitter, seed=seed, bc=(abi.HX_BC_FREESTREAM, fs),
+                 with_av=True)
+
+    def init(pos):
+        r = rho * (1 + fourier_perturbation(pos, seed * 16 + 1))
+        vel = [(u if d == 0 else 0.0) + u * fourier_perturbation(pos, seed * 16 + 2 + d) for d in range(nd)]
+        return [r * v for v in vel] + [r, p / (GAMMA - 1) + 0.5 * r * sum(v * v for v in vel)]
+    mesh.set_state(m, init)
+    for c in range(2):
+        if m.n_elem[c]:
+            m.av[c][:, 0, :] = art_visc
+    return m
+
+
+def isentropic_vortex(n=16, rs=6, mach=0.5, strength=5.0, half_width=5.0):
+    """Config 1 (2D): Cartesian quads on [-w, w]^2 with freestream boundaries. Shu's isentropic vortex with
+    rho_inf = p_inf = 1 (sound speed sqrt(gamma)), freestream speed mach*sqrt(gamma) along x:
+    T = p/rho = 1 - (gamma-1) beta^2/(8 gamma pi^2) e^(1-r^2), rho = T^(1/(gamma-1)), perturbation velocity
+    beta/(2 pi) e^((1-r^2)/2) (-y, x). The exact solution is the initial vortex translated with the freestream.
+    Returns (mesh, exact(pos, t) -> list of conserved fields)."""
+    u_inf = mach * math.sqrt(GAMMA)
+    fs = freestream_state(2, 1.0, [u_inf, 0.0], 1.0)
+    m = mesh.box(2, rs, n, -half_width, half_width, bc=(abi.HX_BC_FREESTREAM, fs))
+
+    def exact(pos, t):
+        x = pos[0] - u_inf * t
+        y = pos[1]
+        r2 = x * x + y * y
+        du = strength / (2 * math.pi) * np.exp(0.5 * (1 - r2))
+        temp = 1 - (GAMMA - 1) * strength ** 2 / (8 * GAMMA * math.pi ** 2) * np.exp(1 - r2)
+        rho = temp ** (1 / (GAMMA - 1))
+        pr = rho * temp
+        vx = u_inf - du * y
+        vy = du * x
+        return [rho * vx, rho * vy, rho, pr / (GAMMA - 1) + 0.5 * rho * (vx * vx + vy * vy)]
+    mesh.set_state(m, lambda pos: exact(pos, 0.0))
+    return m, exact

@@ -2,7 +2,9 @@
 // functions (kernels_convective.cpp, kernels_diffusive.cpp, kernels_max_dt.cpp) and Solver::update.
 #include <algorithm>
 #include <cfloat>
+#include <map>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -19,6 +21,22 @@ struct hx_ctx
   std::string err;
   void* ops[HX_MAX_ROW_SIZE + 1] = {};  // device Ops<rs> tables
   double* scalar = nullptr;             // device scratch for reductions
+  // per-category event timing
+  struct Pending
+  {
+    hipEvent_t start, stop;
+    std::string category;
+    long long units;
+  };
+  struct Total
+  {
+    double ms = 0;
+    long long launches = 0, units = 0;
+  };
+  bool timing = false;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<Pending> pending;
+  std::map<std::string, Total> totals;
 };
 
 struct hx_mesh
@@ -253,6 +271,58 @@ hx_status pde_check(hx_mesh* mesh, const hx_pde* pde, bool* visc, Launch* l)
 
 bool valid_cls(int cls) { return cls == HX_CARTESIAN || cls == HX_DEFORMED; }
 
+const char* cls_name(int cls) { return cls == HX_DEFORMED ? "deformed" : "cartesian"; }
+const char* pde_name(bool visc) { return visc ? "navier_stokes" : "euler"; }
+
+hipEvent_t pool_event(hx_ctx* ctx)
+{
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one launch with events on the context stream while timing is enabled.
+struct Timer
+{
+  hx_ctx* ctx;
+  hipEvent_t a = nullptr, b = nullptr;
+  std::string category;
+  long long units;
+  Timer(hx_ctx* c, std::string cat, long long u) : ctx{c}, category{std::move(cat)}, units{u}
+  {
+    if (!ctx->timing) return;
+    a = pool_event(ctx);
+    b = pool_event(ctx);
+    if (a && b && hipEventRecord(a, ctx->stream) != hipSuccess) a = nullptr;
+  }
+  ~Timer()
+  {
+    if (!ctx->timing || !a || !b) return;
+    if (hipEventRecord(b, ctx->stream) == hipSuccess) ctx->pending.push_back({a, b, category, units});
+  }
+};
+
+void resolve_timing(hx_ctx* ctx)
+{
+  for (auto& p : ctx->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+      auto& t = ctx->totals[p.category];
+      t.ms += ms;
+      ++t.launches;
+      t.units += p.units;
+    }
+    ctx->event_pool.push_back(p.start);
+    ctx->event_pool.push_back(p.stop);
+  }
+  ctx->pending.clear();
+}
+
 __global__ void k_fill(double* p, double v) { p[threadIdx.x] = v; }
 
 }  // namespace
@@ -290,6 +360,11 @@ hx_status hx_ctx_destroy(hx_ctx* ctx)
   (void)hipStreamSynchronize(ctx->stream);
   for (void* p : ctx->ops)
     if (p) (void)hipFree(p);
+  for (auto& p : ctx->pending) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   (void)hipFree(ctx->scalar);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -306,6 +381,46 @@ hx_status hx_ctx_synchronize(hx_ctx* ctx)
 }
 
 void* hx_ctx_stream(hx_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+hx_status hx_ctx_set_timing(hx_ctx* ctx, int enable)
+{
+  if (!ctx) return HX_ERR_INVALID;
+  ctx->timing = enable != 0;
+  return HX_OK;
+}
+
+hx_status hx_ctx_timing_reset(hx_ctx* ctx)
+{
+  if (!ctx) return HX_ERR_INVALID;
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "timing sync");
+  resolve_timing(ctx);
+  ctx->totals.clear();
+  return HX_OK;
+}
+
+hx_status hx_ctx_timing(hx_ctx* ctx, const char* category, double* total_ms, long long* launches, long long* units)
+{
+  if (!ctx || !category) return HX_ERR_INVALID;
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "timing sync");
+  resolve_timing(ctx);
+  auto it = ctx->totals.find(category);
+  const hx_ctx::Total t = it == ctx->totals.end() ? hx_ctx::Total{} : it->second;
+  if (total_ms) *total_ms = t.ms;
+  if (launches) *launches = t.launches;
+  if (units) *units = t.units;
+  return HX_OK;
+}
+
+hx_status hx_ctx_timing_list(hx_ctx* ctx, char* out, size_t capacity)
+{
+  if (!ctx || !out || capacity == 0) return HX_ERR_INVALID;
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "timing sync");
+  resolve_timing(ctx);
+  std::string all;
+  for (auto& kv : ctx->totals) all += kv.first + "\n";
+  std::snprintf(out, capacity, "%s", all.c_str());
+  return all.size() < capacity ? HX_OK : HX_ERR_INVALID;
+}
 
 int hx_basis_table(int row_size, const char* name, double* out, int n)
 {
@@ -471,6 +586,7 @@ hx_status hx_write_face(hx_mesh* mesh, int cls, const hx_pde* pde)
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/write face/" + pde_name(visc), mesh->dm.n_elem[cls]);
   HX_TRY(mesh->ctx, mesh->ks->write_face(l, cls), "write_face");
   return HX_OK;
 }
@@ -481,6 +597,7 @@ hx_status hx_neighbor(hx_mesh* mesh, int cls, const hx_pde* pde, int)
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor/" + pde_name(visc), mesh->dm.n_con[cls]);
   HX_TRY(mesh->ctx, mesh->ks->neighbor(l, cls, visc), "neighbor");
   return HX_OK;
 }
@@ -497,6 +614,7 @@ hx_status hx_local(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_s
   if (i_stage && compute_residual)
     return fail(mesh->ctx, HX_ERR_INVALID, "residual calculation is a single-stage operation");
   const double coef = i_stage ? dt * hx_basis_step_ratio(mesh->dm.rs) : dt;
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/local/" + pde_name(visc), mesh->dm.n_elem[cls]);
   HX_TRY(mesh->ctx, mesh->ks->local(l, cls, visc, coef, i_stage, compute_residual, use_filter), "local");
   return HX_OK;
 }
@@ -505,6 +623,7 @@ hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde*)
 {
   if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor reconcile/navier_stokes", mesh->dm.n_con[cls]);
   HX_TRY(mesh->ctx, mesh->ks->neighbor_reconcile(l, cls), "neighbor_reconcile");
   return HX_OK;
 }
@@ -518,6 +637,7 @@ hx_status hx_reconcile_ldg_flux(hx_mesh* mesh, int cls, const hx_pde* pde, doubl
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
   if (visc && i_stage)
     return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/reconcile LDG flux/navier_stokes", mesh->dm.n_elem[cls]);
   HX_TRY(mesh->ctx, mesh->ks->reconcile_ldg_flux(l, cls, dt, compute_residual, use_filter), "reconcile_ldg_flux");
   return HX_OK;
 }
@@ -533,7 +653,10 @@ hx_status hx_max_dt(hx_mesh* mesh, int cls, const hx_pde* pde, int is_local, int
   const double max_cfl_d = -2 / hx_basis_tables::eigenvalss[mesh->dm.rs - 2][1] * safety_diff;
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, l.stream, l.scalar, DBL_MAX);
   HX_TRY(mesh->ctx, hipGetLastError(), "fill");
-  HX_TRY(mesh->ctx, mesh->ks->max_dt(l, cls, visc, is_local, max_cfl_c, max_cfl_d), "max_dt");
+  {
+    Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/compute time step/" + pde_name(visc), mesh->dm.n_elem[cls]);
+    HX_TRY(mesh->ctx, mesh->ks->max_dt(l, cls, visc, is_local, max_cfl_c, max_cfl_d), "max_dt");
+  }
   double dt = DBL_MAX;
   HX_TRY(mesh->ctx, hipMemcpyAsync(&dt, l.scalar, sizeof(double), hipMemcpyDeviceToHost, l.stream), "max_dt readback");
   HX_TRY(mesh->ctx, hipStreamSynchronize(l.stream), "max_dt sync");
@@ -547,6 +670,7 @@ hx_status hx_prolong_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int of
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  Timer tm(mesh->ctx, "prolong/restrict/prolong", mesh->dm.n_ref);
   HX_TRY(mesh->ctx, mesh->ks->prolong(l, mesh->dm.nv, scale, offset), "prolong");
   return HX_OK;
 }
@@ -557,6 +681,7 @@ hx_status hx_restrict_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int o
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  Timer tm(mesh->ctx, "prolong/restrict/restrict", mesh->dm.n_ref);
   HX_TRY(mesh->ctx, mesh->ks->restrict_(l, mesh->dm.nv, scale, offset), "restrict");
   return HX_OK;
 }
@@ -564,6 +689,7 @@ hx_status hx_restrict_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int o
 hx_status hx_apply_state_bcs(hx_mesh* mesh)
 {
   if (!mesh) return HX_ERR_INVALID;
+  Timer tm(mesh->ctx, "boundary conditions/state", mesh->dm.n_bc_con);
   HX_TRY(mesh->ctx, mesh->ks->state_bcs(make_launch(mesh)), "state_bcs");
   return HX_OK;
 }
@@ -571,6 +697,7 @@ hx_status hx_apply_state_bcs(hx_mesh* mesh)
 hx_status hx_apply_flux_bcs(hx_mesh* mesh)
 {
   if (!mesh) return HX_ERR_INVALID;
+  Timer tm(mesh->ctx, "boundary conditions/flux", mesh->dm.n_bc_con);
   HX_TRY(mesh->ctx, mesh->ks->flux_bcs(make_launch(mesh)), "flux_bcs");
   return HX_OK;
 }

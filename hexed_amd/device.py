@@ -35,6 +35,26 @@ class Context:
     def synchronize(self):
         self.check(self.lib.hx_ctx_synchronize(self.ptr), "synchronize")
 
+    def set_timing(self, enable=True):
+        self.check(self.lib.hx_ctx_set_timing(self.ptr, int(enable)), "set_timing")
+
+    def timing_reset(self):
+        self.check(self.lib.hx_ctx_timing_reset(self.ptr), "timing_reset")
+
+    def timings(self):
+        """{category: (total_ms, launches, work_units)} for every category recorded since the last reset."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self.check(self.lib.hx_ctx_timing_list(self.ptr, buf, len(buf)), "timing_list")
+        out = {}
+        for name in buf.value.decode().split("\n"):
+            if not name:
+                continue
+            ms, n, u = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong()
+            self.check(self.lib.hx_ctx_timing(self.ptr, name.encode(), ctypes.byref(ms), ctypes.byref(n),
+                                              ctypes.byref(u)), "timing")
+            out[name] = (ms.value, n.value, u.value)
+        return out
+
     @property
     def stream(self):
         return self.lib.hx_ctx_stream(self.ptr)

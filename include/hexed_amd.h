@@ -198,6 +198,18 @@ void* hx_ctx_stream(hx_ctx* ctx);
 /* Number of (n_dim, row_size) combinations built into this library for `pde`; 0 if none. */
 int hx_kernels_built(int n_dim, int row_size, int pde_kind);
 
+/* Device timing per kernel category, the device-side counterpart of the reference's Stopwatch_tree categories
+ * ("cartesian/local", "deformed/neighbor", "compute time step", "prolong/restrict", "boundary conditions", ...;
+ * kernel_factory.hpp:32-45, Solver.cpp:176-201). While enabled, every kernel launch of the context is bracketed by
+ * HIP events on the context stream. Category names are "<class>/<kernel>/<pde>", e.g. "deformed/local/navier_stokes".
+ * hx_ctx_timing synchronizes the stream, then reports the summed milliseconds, launches and work units (elements or
+ * connections processed) of one category; hx_ctx_timing_list writes the category names, '\n'-separated. */
+hx_status hx_ctx_set_timing(hx_ctx* ctx, int enable);
+hx_status hx_ctx_timing_reset(hx_ctx* ctx);
+hx_status hx_ctx_timing(hx_ctx* ctx, const char* category, double* total_ms, long long* launches,
+                        long long* work_units);
+hx_status hx_ctx_timing_list(hx_ctx* ctx, char* out, size_t capacity);
+
 /* ---- basis (Basis.hpp:14-63, Gauss_legendre.hpp, Basis.cpp:6-14) ------------------------------------- */
 /* Copies one basis table for `row_size` into `out` (capacity `n` doubles). `name` is one of "node", "weight",
  * "diff_mat", "boundary", "orthogonal", "filter", "prolong", "restrict", "eigenvals". Returns the number of
