@@ -63,6 +63,25 @@ def algorithmic_bytes(category, nd, nq, n_elem):
     return 8.0 * doubles * nq * n_elem
 
 
+def kernel_symbol(category, nd, rs):
+    """The HIP kernel a timing category launches (the name rocprofv3 reports), for local categories."""
+    cls, kernel, pde = category.split("/")
+    if kernel != "local":
+        return None
+    name = "k_local_ns" if pde == "navier_stokes" else "k_local_euler"
+    return f"{name}<{nd}, {rs}, {'true' if cls == 'deformed' else 'false'}>"
+
+
+def measured_traffic(config, precision, symbol):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, if one covers the kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if symbol is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f).get(f"{config}_{precision}", {})
+    return table.get("bytes_per_launch", {}).get(symbol)
+
+
 def run_oracle_steps(m, opts, steps):
     import oracle
     lib = oracle.load()
@@ -104,6 +123,7 @@ def main():
           file=sys.stderr, flush=True)
 
     ctx = Context(0)
+    copy_gbs = ctx.stream_copy(1 << 31, 10)  # achievable HBM bandwidth of a plain copy kernel on this box
     dev = Device_mesh(ctx, m)
     dev.compute_write_face()  # Solver::initialize extrapolates the initial state to the faces
     init = {"state": [a.copy() for a in m.state], "faces": None}
@@ -129,8 +149,10 @@ def main():
     avg_ms = ms / launches
     cls = abi.HX_DEFORMED if dominant.startswith("deformed") else abi.HX_CARTESIAN
     alg = algorithmic_bytes(dominant, nd, nq, m.n_elem[cls])
-    roofline = {"bound": "hbm", "kernel": dominant, "launch_ms": round(avg_ms, 5), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "traffic": None}
+    symbol = kernel_symbol(dominant, nd, m.row_size)
+    roofline = {"bound": "hbm", "kernel": dominant, "symbol": symbol, "launch_ms": round(avg_ms, 5),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": measured_traffic(args.config, "fp64", symbol),
+                "stream_copy_gbs": round(copy_gbs, 1)}
     if alg is not None:
         achieved = alg / (avg_ms * 1e-3) / 1e9
         roofline.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -135,6 +135,7 @@ def load_library(path=LIB_PATH):
         "hx_ctx_timing": (ctypes.c_int, [P, ctypes.c_char_p, c_double_p, ctypes.POINTER(ctypes.c_longlong),
                                          ctypes.POINTER(ctypes.c_longlong)]),
         "hx_ctx_timing_list": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_size_t]),
+        "hx_stream_copy": (ctypes.c_int, [P, ctypes.c_size_t, ctypes.c_int, c_double_p]),
         "hx_basis_table": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, c_double_p, ctypes.c_int]),
         "hx_basis_max_cfl": (ctypes.c_double, [ctypes.c_int]),
         "hx_basis_step_ratio": (ctypes.c_double, [ctypes.c_int]),
@@ -182,7 +183,7 @@ def load_library(path=LIB_PATH):
 # names exported by include/hexed_amd.h, checked by the CPU test suite
 EXPORTED_SYMBOLS = [
     "hx_ctx_create", "hx_ctx_destroy", "hx_last_error", "hx_ctx_synchronize", "hx_ctx_stream", "hx_kernels_built",
-    "hx_ctx_set_timing", "hx_ctx_timing_reset", "hx_ctx_timing", "hx_ctx_timing_list",
+    "hx_ctx_set_timing", "hx_ctx_timing_reset", "hx_ctx_timing", "hx_ctx_timing_list", "hx_stream_copy",
     "hx_basis_table", "hx_basis_max_cfl", "hx_basis_step_ratio", "hx_mesh_create", "hx_mesh_destroy",
     "hx_mesh_upload", "hx_mesh_download", "hx_mesh_set_precision", "hx_mesh_device_ptr", "hx_write_face",
     "hx_neighbor", "hx_local", "hx_neighbor_reconcile", "hx_reconcile_ldg_flux", "hx_max_dt", "hx_prolong_refined",

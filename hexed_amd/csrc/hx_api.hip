@@ -325,6 +325,12 @@ void resolve_timing(hx_ctx* ctx)
 
 __global__ void k_fill(double* p, double v) { p[threadIdx.x] = v; }
 
+// plain streaming copy, 16 bytes per thread per iteration
+__global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n)
+{
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -381,6 +387,43 @@ hx_status hx_ctx_synchronize(hx_ctx* ctx)
 }
 
 void* hx_ctx_stream(hx_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+hx_status hx_stream_copy(hx_ctx* ctx, size_t bytes, int reps, double* gbs)
+{
+  if (!ctx || !gbs || reps < 1 || bytes < 16) return HX_ERR_INVALID;
+  const size_t n = bytes / 16;
+  void* a = nullptr;
+  void* b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hx_status st = HX_OK;
+  auto cleanup = [&]() {
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  if ((st = hip_check(ctx, hipMalloc(&a, n * 16), "hipMalloc(copy)")) || (st = hip_check(ctx, hipMalloc(&b, n * 16), "hipMalloc(copy)")) ||
+      (st = hip_check(ctx, hipMemsetAsync(a, 0, n * 16, ctx->stream), "memset(copy)")) ||
+      (st = hip_check(ctx, hipEventCreate(&e0), "event")) || (st = hip_check(ctx, hipEventCreate(&e1), "event"))) {
+    cleanup();
+    return st;
+  }
+  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 256 * 64));
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, (const double2*)a, (double2*)b, n);
+  (void)hipEventRecord(e0, ctx->stream);
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, (const double2*)a, (double2*)b, n);
+  (void)hipEventRecord(e1, ctx->stream);
+  if ((st = hip_check(ctx, hipEventSynchronize(e1), "copy sync"))) {
+    cleanup();
+    return st;
+  }
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *gbs = 2.0 * double(n * 16) * reps / (ms * 1e-3) / 1e9;
+  cleanup();
+  return HX_OK;
+}
 
 hx_status hx_ctx_set_timing(hx_ctx* ctx, int enable)
 {

@@ -55,6 +55,12 @@ class Context:
             out[name] = (ms.value, n.value, u.value)
         return out
 
+    def stream_copy(self, nbytes, reps=10):
+        """Achieved GB/s of a plain copy kernel (one read + one write per byte)."""
+        out = ctypes.c_double()
+        self.check(self.lib.hx_stream_copy(self.ptr, nbytes, reps, ctypes.byref(out)), "stream_copy")
+        return out.value
+
     @property
     def stream(self):
         return self.lib.hx_ctx_stream(self.ptr)

@@ -210,6 +210,11 @@ hx_status hx_ctx_timing(hx_ctx* ctx, const char* category, double* total_ms, lon
                         long long* work_units);
 hx_status hx_ctx_timing_list(hx_ctx* ctx, char* out, size_t capacity);
 
+/* Diagnostic with no counterpart in the reference: a plain device-to-device copy kernel ("hx_stream_copy") over
+ * `bytes` bytes, run `reps` times; returns the achieved bandwidth counting one read and one write of every byte.
+ * Used to measure the achievable HBM bandwidth of the box and to calibrate hardware byte counters. */
+hx_status hx_stream_copy(hx_ctx* ctx, size_t bytes, int reps, double* gbs);
+
 /* ---- basis (Basis.hpp:14-63, Gauss_legendre.hpp, Basis.cpp:6-14) ------------------------------------- */
 /* Copies one basis table for `row_size` into `out` (capacity `n` doubles). `name` is one of "node", "weight",
  * "diff_mat", "boundary", "orthogonal", "filter", "prolong", "restrict", "eigenvals". Returns the number of
