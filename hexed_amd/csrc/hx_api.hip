@@ -554,6 +554,20 @@ hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* d, hx_mesh** out)
   return HX_OK;
 }
 
+hx_status hx_mesh_validate(const hx_host_mesh* d, char* msg, size_t capacity)
+{
+  std::string bad;
+  if (!d) bad = "no mesh";
+  else if (d->n_dim < 1 || d->n_dim > HX_MAX_N_DIM || d->row_size < HX_MIN_ROW_SIZE || d->row_size > HX_MAX_ROW_SIZE)
+    bad = "demand for invalid kernel";
+  else {
+    std::string why = validate(d, Sizes(d->n_dim, d->row_size));
+    if (!why.empty()) bad = "invalid mesh: " + why;
+  }
+  if (msg && capacity) std::snprintf(msg, capacity, "%s", bad.c_str());
+  return bad.empty() ? HX_OK : HX_ERR_INVALID;
+}
+
 hx_status hx_mesh_destroy(hx_mesh* mesh)
 {
   if (!mesh) return HX_OK;

@@ -352,3 +352,205 @@ def set_state(m, func):
         vals = func([pos[:, d] for d in range(m.n_dim)])
         for v in range(m.nv):
             m.state[c][:, v] = vals[v]
+
+
+# ---- general builder (arbitrary connection directions) -------------------------------------------------------
+
+def face_vertex_inds(nd, i_dim, face_sign):
+    """Permutation of the face vertices of side 1 so they align with side 0 (connection.cpp:6-27)."""
+    n_vert = 2 ** (nd - 1)
+    inds = list(range(n_vert))
+    fn = [face_sign[0] == False, face_sign[1] == True]  # noqa: E712  flip_normal(i_side)
+    flip_t = (i_dim[0] != i_dim[1]) and (fn[0] == fn[1])
+    transpose = (i_dim[0] == 0 and i_dim[1] == 2) or (i_dim[0] == 2 and i_dim[1] == 0)
+    if flip_t:
+        stride = 1
+        if nd == 3:
+            unused = 3 - i_dim[0] - i_dim[1]
+            if unused > i_dim[0]:
+                stride = 2
+        inds = [v + stride * (1 - 2 * ((v // stride) % 2)) for v in inds]
+    if transpose:
+        inds[1], inds[2] = inds[2], inds[1]
+    return inds
+
+
+def vertex_inds(nd, i_dim, face_sign):
+    """Vertices of both elements on the connected faces, ordered so that entries correspond (connection.cpp:29-50)."""
+    n_vert = 2 ** (nd - 1)
+    inds = [[], []]
+    for s in range(2):
+        stride = 2 ** (nd - i_dim[s] - 1)
+        inds[s] = [v for v in range(2 * n_vert) if (v // stride) % 2 == int(face_sign[s])]
+    perm = face_vertex_inds(nd, i_dim, face_sign)
+    inds[1] = [inds[1][perm[k]] for k in range(n_vert)]
+    return inds
+
+
+class Builder:
+    """Small meshes through the reference's mesh-building vocabulary (Mesh.hpp:24-230): elements by refinement level
+    and nominal position, Cartesian / deformed / boundary connections. Vertices merge as Vertex::eat does
+    (position = mean of the merged nominal positions); geometry then follows the box builder's formulas."""
+
+    def __init__(self, nd, rs, root_size=1.0):
+        self.nd, self.rs, self.root = nd, rs, root_size
+        self.elems = []      # dict(deformed, level, pos)
+        self.car_cons = []   # (a, b, i_dim)
+        self.def_cons = []   # (a, b, i_dims, face_signs)
+        self.bnd_cons = []   # (elem, i_dim, sign, bc)
+        self.bcs = []
+
+    def add_element(self, ref_level, deformed, pos):
+        self.elems.append(dict(deformed=bool(deformed), level=int(ref_level), pos=list(pos) + [0] * (self.nd - len(pos))))
+        return len(self.elems) - 1
+
+    def add_bc(self, kind, state):
+        self.bcs.append((kind, list(state)))
+        return len(self.bcs) - 1
+
+    def connect_cartesian(self, a, b, i_dim):
+        """Cartesian connection: face +i_dim of a to face -i_dim of b (connection.hpp:29-36)."""
+        self.car_cons.append((a, b, i_dim))
+
+    def connect_deformed(self, a, b, i_dims, face_signs):
+        if i_dims[0] == i_dims[1] and face_signs[0] == face_signs[1]:
+            raise RuntimeError("attempt to connect faces of same sign along same dimension which is forbidden")
+        self.def_cons.append((a, b, tuple(i_dims), tuple(bool(s) for s in face_signs)))
+
+    def connect_boundary(self, elem, i_dim, sign, bc):
+        self.bnd_cons.append((elem, i_dim, int(sign), bc))
+
+    def build(self, with_av=False, jitter=None):
+        nd, rs = self.nd, self.rs
+        nvert = 2 ** nd
+        cls_of = [1 if e["deformed"] else 0 for e in self.elems]
+        idx_of, counts = [], [0, 0]
+        for c in cls_of:
+            idx_of.append(counts[c])
+            counts[c] += 1
+        n_int_def = len(self.def_cons)
+        n_bnd = len(self.bnd_cons)
+        n_car_con = len(self.car_cons)
+        m = HostMesh(nd, rs, counts[0], counts[1], 2 * (n_car_con + n_int_def + n_bnd), 2 * (n_int_def + n_bnd),
+                     n_car_con, n_int_def + n_bnd, with_av=with_av)
+        # nominal vertex positions and union-find merge
+        parent = list(range(len(self.elems) * nvert))
+
+        def find(x):
+            while parent[x] != x:
+                parent[x] = parent[parent[x]]
+                x = parent[x]
+            return x
+
+        def union(x, y):
+            rx, ry = find(x), find(y)
+            if rx != ry:
+                parent[ry] = rx
+        nom = np.zeros((len(self.elems), nvert, nd))
+        for i, e in enumerate(self.elems):
+            h = self.root / 2 ** e["level"]
+            for v in range(nvert):
+                for d in range(nd):
+                    nom[i, v, d] = (e["pos"][d] + ((v >> (nd - 1 - d)) & 1)) * h
+        for a, b, d in self.car_cons:
+            ia, ib = vertex_inds(nd, (d, d), (True, False))
+            for va, vb in zip(ia, ib):
+                union(a * nvert + va, b * nvert + vb)
+        for a, b, dims, signs in self.def_cons:
+            ia, ib = vertex_inds(nd, dims, signs)
+            for va, vb in zip(ia, ib):
+                union(a * nvert + va, b * nvert + vb)
+        roots = np.array([find(x) for x in range(len(parent))])
+        uniq, vid = np.unique(roots, return_inverse=True)
+        vid = vid.reshape(len(self.elems), nvert)
+        sums = np.zeros((len(uniq), nd))
+        cnt = np.zeros(len(uniq))
+        np.add.at(sums, vid.reshape(-1), nom.reshape(-1, nd))
+        np.add.at(cnt, vid.reshape(-1), 1)
+        vpos = sums / cnt[:, None]
+        if jitter is not None:
+            vpos = jitter(vpos)
+        # element data
+        for i, e in enumerate(self.elems):
+            m.nom_size[cls_of[i]][idx_of[i]] = self.root / 2 ** e["level"]
+        def_ids = [i for i in range(len(self.elems)) if cls_of[i] == 1]
+        car_ids = [i for i in range(len(self.elems)) if cls_of[i] == 0]
+        fn = None
+        if def_ids:
+            metrics, fn, vtss_def, qpos = deformed_metrics(nd, rs, vpos[vid[def_ids]], m.nom_size[1])
+            m.metrics[:] = metrics
+            m.qpos[1] = qpos
+        if car_ids:
+            nodes = node_coords(nd, rs)
+            pos = np.array([self.elems[i]["pos"] for i in car_ids], dtype=float)
+            h = m.nom_size[0]
+            m.qpos[0] = ((nodes[None] + pos[:, None, :]) * h[:, None, None]).transpose(0, 2, 1).copy()
+        # slots
+        slot = 0
+        for k, (a, b, d) in enumerate(self.car_cons):
+            m.con_slot[0][k] = (slot, slot + 1)
+            m.con_dir[0][k] = pack_dir(d, d, 1, 0)
+            m.face_slot[cls_of[a]][idx_of[a], 2 * d + 1] = slot
+            m.face_slot[cls_of[b]][idx_of[b], 2 * d] = slot + 1
+            slot += 2
+        nrm = np.zeros((n_int_def + n_bnd, 2, nd, m.nfq))
+        for k, (a, b, dims, signs) in enumerate(self.def_cons):
+            if cls_of[a] != 1 or cls_of[b] != 1:
+                raise RuntimeError("deformed connections join deformed elements")
+            packed = pack_dir(dims[0], dims[1], signs[0], signs[1])
+            m.con_slot[1][k] = (slot, slot + 1)
+            m.con_dir[1][k] = packed
+            fa, fb = 2 * dims[0] + signs[0], 2 * dims[1] + signs[1]
+            m.face_slot[1][idx_of[a], fa] = slot
+            m.face_slot[1][idx_of[b], fb] = slot + 1
+            m.con_nslot[k] = (2 * k, 2 * k + 1)
+            m.face_nslot[idx_of[a], fa] = 2 * k
+            m.face_nslot[idx_of[b], fb] = 2 * k + 1
+            # shared normal (Solver.cpp:327-352)
+            perm = face_permutation_index(nd, rs, packed)
+            _, face_sign = unpack_dir(packed)
+            sg = [1 - 2 * flip_normal(face_sign, s) for s in range(2)]
+            n0 = fn[idx_of[a], fa]
+            n1 = fn[idx_of[b], fb][:, perm]
+            avg = 0.5 * sg[0] * n0 + 0.5 * sg[1] * n1
+            nrm[k, 0] = sg[0] * avg
+            back = np.zeros_like(avg)
+            back[:, perm] = sg[1] * avg
+            nrm[k, 1] = back
+            slot += 2
+        m.bc_con = np.arange(n_int_def, n_int_def + n_bnd, dtype=np.int32)
+        m.bc_serial = np.zeros(n_bnd, np.int32)
+        for j, (e, d, s, bc) in enumerate(self.bnd_cons):
+            k = n_int_def + j
+            m.con_slot[1][k] = (slot, slot + 1)
+            m.con_dir[1][k] = pack_dir(d, d, s, 1 - s)
+            m.con_nslot[k] = (2 * k, 2 * k + 1)
+            m.face_slot[cls_of[e]][idx_of[e], 2 * d + s] = slot
+            m.bc_serial[j] = bc
+            if cls_of[e] == 1:
+                m.face_nslot[idx_of[e], 2 * d + s] = 2 * k
+                nin = fn[idx_of[e], 2 * d + s]
+                sg = -1.0 if s == 0 else 1.0
+                avg = 0.5 * sg * nin + 0.5 * sg * nin
+                nrm[k, 0] = sg * avg
+                nrm[k, 1] = sg * avg
+            else:
+                nrm[k, :, d, :] = 1.0
+            slot += 2
+        m.normals[:] = nrm.reshape(-1, nd, m.nfq)
+        m.bcs = list(self.bcs)
+        # vertex time step scale, shared by minimum
+        per_elem = np.zeros((len(self.elems), nvert))
+        for j, i in enumerate(car_ids):
+            per_elem[i] = m.nom_size[0][j] / nd
+        for j, i in enumerate(def_ids):
+            per_elem[i] = vtss_def[j]
+        vmin = np.full(len(uniq), np.inf)
+        np.minimum.at(vmin, vid.reshape(-1), per_elem.reshape(-1))
+        for j, i in enumerate(car_ids):
+            m.vertex_tss[0][j] = vmin[vid[i]]
+        for j, i in enumerate(def_ids):
+            m.vertex_tss[1][j] = vmin[vid[i]]
+        self.vertex_positions = vpos
+        self.vertex_ids = vid
+        return m

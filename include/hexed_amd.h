@@ -230,6 +230,9 @@ double hx_basis_step_ratio(int row_size);
  * "demand for invalid kernel" unless 1 <= n_dim <= 3 and 2 <= row_size <= 8 (kernel_factory.hpp:114-116). */
 hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* desc, hx_mesh** out);
 hx_status hx_mesh_destroy(hx_mesh* mesh);
+/* The checks hx_mesh_create applies before touching the device (sizes, every slot / normal / connection index in
+ * range, required arrays present); host only. Returns HX_OK or HX_ERR_INVALID with the reason in `msg`. */
+hx_status hx_mesh_validate(const hx_host_mesh* desc, char* msg, size_t capacity);
 hx_status hx_mesh_upload(hx_mesh* mesh, int field, int cls, const double* host, size_t n_doubles);
 hx_status hx_mesh_download(hx_mesh* mesh, int field, int cls, double* host, size_t n_doubles);
 hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision);

@@ -189,3 +189,52 @@ def test_residual(ctx, cfg):
         o_call("oracle_compute_residual", ctypes.byref(s), ctypes.byref(opts))
         dev.compute_residual(opts)
         compare_mesh("residual", host, dev)
+
+
+# ---- every connection orientation and the reference's rotated test meshes ----------------------------------------
+
+import itertools  # noqa: E402
+
+import meshes  # noqa: E402
+
+ORIENTATIONS = [(nd, dims, signs)
+                for nd in (2, 3)
+                for dims in itertools.product(range(nd), repeat=2)
+                for signs in itertools.product((0, 1), repeat=2)
+                if not (dims[0] == dims[1] and signs[0] == signs[1])]
+
+
+def run_both(ctx, host, opts, steps=2):
+    dev = Device_mesh(ctx, host.copy())
+    s = host.struct()
+    st_o, st_d = abi.hx_update_status(), abi.hx_update_status()
+    for it in range(steps):
+        o_call("oracle_update", ctypes.byref(s), ctypes.byref(opts), ctypes.byref(st_o))
+        dev.update(opts, st_d)
+        compare_mesh(f"update {it}", host, dev)
+
+
+@pytest.mark.parametrize("nd,dims,signs", ORIENTATIONS, ids=lambda v: str(v))
+def test_orientation_parity(ctx, nd, dims, signs):
+    """Two deformed elements joined in every orientation (face permutation + normal signs on the device)."""
+    say(f"[gpu] orientation nd{nd} dims {dims} signs {signs}")
+    b, host = meshes.pair(nd, 4, dims, signs, bc_kind=abi.HX_BC_COPY)
+    smooth_state(host, seed=13)
+    host.av = [np.zeros((n, 2, host.nq)) for n in host.n_elem]
+    init_faces(host)
+    run_both(ctx, host, abi.update_options(max_safety=0.05, visc=abi.constant(1e-2), cond=abi.constant(1.4e-2),
+                                           use_ldg=True), steps=1)
+    run_both(ctx, host, abi.update_options(max_safety=0.05), steps=1)
+
+
+@pytest.mark.parametrize("name", ["all_cartesian", "all_deformed_rot0", "all_deformed_rot1"])
+def test_reference_test_meshes(ctx, name):
+    say(f"[gpu] reference test mesh {name}")
+    if name == "all_cartesian":
+        host = meshes.all_cartesian(abi.HX_BC_NONPENETRATION, with_av=True)
+    else:
+        host = meshes.all_deformed(int(name[-1]), abi.HX_BC_NONPENETRATION, with_av=True)
+    smooth_state(host, seed=17)
+    init_faces(host)
+    run_both(ctx, host, abi.update_options(max_safety=0.1, visc=abi.constant(1e-2), cond=abi.constant(1.4e-2),
+                                           use_ldg=True))
