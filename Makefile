@@ -29,6 +29,21 @@ $(ORACLE): oracle/hexed_oracle.cpp oracle/hexed_oracle.h include/hexed_amd.h $(S
 	$(CXX_ORACLE) $(ORACLE_FLAGS) -o $@.tmp $< && mv $@.tmp $@
 
 clean:
+	rm -f $(HOST_LIB) $(DROP_IN_TEST)
 	rm -rf $(BUILD) $(LIB) $(ORACLE)
 
 .PHONY: all clean
+
+# C++ drop-in adapter over the C ABI (the reference's kernel interfaces) and its test driver
+HOST_INC = -Ihexed_amd/host/include
+HOST_LIB = hexed_amd/libhexed_host.so
+DROP_IN_TEST = tests/cpp/test_drop_in
+
+$(HOST_LIB): hexed_amd/host/src/amd.cpp $(wildcard hexed_amd/host/include/hexed/*.hpp) include/hexed_amd.h $(LIB)
+	g++ -O2 -std=c++20 -fPIC -shared -Wall $(HOST_INC) -o $@.tmp $< -Lhexed_amd -lhexed_amd -Wl,-rpath,'$$ORIGIN' && mv $@.tmp $@
+
+$(DROP_IN_TEST): tests/cpp/test_drop_in.cpp $(HOST_LIB) $(ORACLE)
+	g++ -O2 -std=c++20 -Wall $(HOST_INC) -o $@ $< -Lhexed_amd -lhexed_host -lhexed_amd -Loracle -loracle_hexed \
+	  -Wl,-rpath,'$$ORIGIN/../../hexed_amd' -Wl,-rpath,'$$ORIGIN/../../oracle'
+
+all: $(HOST_LIB) $(DROP_IN_TEST)

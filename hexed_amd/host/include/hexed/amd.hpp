@@ -1,0 +1,98 @@
+// The device side of the drop-in: a process-wide HIP context and the registry that maps the reference's
+// sequences (Kernel_mesh) onto one device mesh.
+//
+// An Accelerated_mesh walks the sequences of a Kernel_mesh once, assigns every face buffer a slot (face pointers are
+// shared between elements and connections, exactly as the reference's connections own the face storage the elements
+// point into) and keeps a device copy. Kernels called on registered sequences run on the device copy; host memory is
+// a mirror that is refreshed by pull() and written back by push(). Sequences that were never registered take a slow
+// path: a temporary device mesh is built, the kernel runs, and the results are scattered back before returning.
+#ifndef HEXED_AMD_AMD_HPP_
+#define HEXED_AMD_AMD_HPP_
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../../../include/hexed_amd.h"
+#include "Kernel_mesh.hpp"
+
+namespace hexed::amd
+{
+
+hx_ctx* context();
+void synchronize();
+void check(hx_status s, const char* what);
+
+class Accelerated_mesh
+{
+  public:
+  //! `reference_layout`: element storage follows the reference's Element (time step scale right after the state,
+  //! then the bulk and Laplacian artificial viscosity coefficients), so the AV fields can be read from it
+  Accelerated_mesh(Kernel_mesh mesh, bool reference_layout = true);
+  ~Accelerated_mesh();
+  Accelerated_mesh(const Accelerated_mesh&) = delete;
+  Accelerated_mesh& operator=(const Accelerated_mesh&) = delete;
+  //! device flow boundary condition for a boundary connection of mesh.def_cons (call before the first push)
+  void add_boundary(Kernel_connection& con, hx_bc bc);
+  void push();  // host -> device
+  void pull();  // device -> host
+  hx_mesh* device();
+  int n_dim() const { return nd; }
+  int row_size() const { return rs; }
+  //! registered mesh owning `sequence`, with the element/connection class it holds (-1: the mixed element list)
+  static Accelerated_mesh* find(const void* sequence, int* cls);
+
+  // construction from partial lists (the slow path for unregistered sequences)
+  Accelerated_mesh(int n_dim, int row_size, std::vector<Kernel_element*> car, std::vector<Kernel_element*> def,
+                   std::vector<Kernel_connection*> car_cons, std::vector<Kernel_connection*> def_cons,
+                   std::vector<Refined_face*> refs, bool reference_layout, bool register_self);
+
+  private:
+  int nd, rs, nq, nfq, nv, face_sz;
+  bool ref_layout;
+  std::vector<Kernel_element*> elems[2];
+  std::vector<Kernel_connection*> cons[2];
+  std::vector<Refined_face*> refs;
+  std::vector<double*> slot_ptr;   // host address of every face slot (nullptr: the dummy slot)
+  std::vector<double*> nslot_ptr;  // host address of every normal slot
+  std::unordered_map<double*, int> slot_of, nslot_of;
+  std::vector<std::pair<Kernel_connection*, hx_bc>> bcs;
+  std::vector<const void*> keys;
+  hx_mesh* dev = nullptr;
+  // host mirrors in the layout of hx_host_mesh
+  std::vector<double> state[2], rcache[2], tss[2], av[2], nom[2], vtss[2], metrics, faces, normals;
+  std::vector<int> face_slot[2], face_nslot, con_slot[2], con_dir[2], con_nslot, ref_slot, ref_stretch, bc_con, bc_serial;
+  std::vector<hx_bc> bc_defs;
+  void build_topology();
+  void gather();
+  void scatter();
+  hx_host_mesh description();
+  int slot(double* p);
+  int nslot(double* p);
+};
+
+//! the device mesh a kernel call on `sequence` runs on: the registered one, or a temporary built for this call
+class Mesh_lease
+{
+  Accelerated_mesh* am = nullptr;
+  std::unique_ptr<Accelerated_mesh> temp;
+  int c = 0;
+
+  public:
+  Mesh_lease(Sequence<Kernel_element&>& s, int n_dim, int row_size, bool deformed);
+  Mesh_lease(Sequence<Kernel_connection&>& s, int n_dim, int row_size, bool deformed);
+  Mesh_lease(Sequence<Refined_face&>& s, int n_dim, int row_size);
+  explicit Mesh_lease(Kernel_mesh& mesh);
+  ~Mesh_lease();
+  hx_mesh* device() { return am->device(); }
+  int cls() const { return c; }
+};
+
+//! Solver::update (Solver.cpp:834-886) on a registered mesh
+void update(Accelerated_mesh& mesh, const hx_update_options& opts, hx_update_status& status);
+
+}  // namespace hexed::amd
+#endif
