@@ -6,14 +6,21 @@ conditions and the compute_navier_stokes (stage 0) / compute_euler (stage 1) ker
 quadrature point carrying all n_dim + 2 conserved variables; value = DOF x 2 stages x steps / wall seconds, with
 the mesh and state resident in HBM before the timed region starts.
 
+Both arithmetic forms are timed on the same mesh from the same initial state: FP64 (the reference's arithmetic)
+and FP32-MFMA (volume fluxes, gradients and derivative contractions in FP32 on the matrix cores; state, RK update,
+face fluxes and jumps, metrics scaling and max_dt in FP64). `value` is the form chosen with --precision (default
+fp32_mfma, the headline); the line also carries the other form's rate, the speedup, and the FP32-MFMA form's relative
+L2 error per conserved variable against the FP64 form after the same number of steps.
+
 The JSON line also carries
-  roofline:     the dominant kernel's algorithmic HBM bytes per launch / its average launch time, measured with
-                HIP events on the library's stream over the timed region, against the 8 TB/s HBM peak;
+  roofline:     the chosen form's dominant kernel: algorithmic HBM bytes per launch / its average launch time,
+                measured with HIP events on the library's stream over the timed region, against the 8 TB/s HBM peak;
   cpu_baseline: the C++/OpenMP oracle (a loop-for-loop restatement of the reference CPU path) timed on this
                 host's cores for a few steps of the same mesh;
-  parity:       relative L2 error per conserved variable between the GPU and the oracle after those steps.
+  parity_vs_cpu: relative L2 error per conserved variable between the FP64 GPU form and the oracle after those steps.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32] [--cpu-steps N]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32] [--precision fp32_mfma|fp64]
+                       [--cpu-steps N] [--err-steps N]
 """
 import argparse
 import ctypes
@@ -63,13 +70,16 @@ def algorithmic_bytes(category, nd, nq, n_elem):
     return 8.0 * doubles * nq * n_elem
 
 
-def kernel_symbol(category, nd, rs):
+FORMS = {"fp64": (abi.HX_FP64, "hx::Fp64_form"), "fp32_mfma": (abi.HX_FP32_MFMA, "hx::Fp32_mfma_form")}
+
+
+def kernel_symbol(category, nd, rs, form):
     """The HIP kernel a timing category launches (the name rocprofv3 reports), for local categories."""
     cls, kernel, pde = category.split("/")
     if kernel != "local":
         return None
     name = "k_local_ns" if pde == "navier_stokes" else "k_local_euler"
-    return f"{name}<{nd}, {rs}, {'true' if cls == 'deformed' else 'false'}>"
+    return f"{name}<{nd}, {rs}, {'true' if cls == 'deformed' else 'false'}, {FORMS[form][1]}>"
 
 
 def measured_traffic(config, precision, symbol):
@@ -93,13 +103,58 @@ def run_oracle_steps(m, opts, steps):
     return time.perf_counter() - t0, oracle.load().oracle_get_threads()
 
 
+def reset(dev, m):
+    """Initial state on the device: state uploaded, residual cache and time step scale cleared, faces extrapolated
+    (Solver::initialize)."""
+    for c in (0, 1):
+        if m.n_elem[c]:
+            dev.upload("state", c, m.state[c])
+            dev.upload("rcache", c, np.zeros_like(m.rcache[c]))
+            dev.upload("tss", c, np.ones_like(m.tss[c]))
+    dev.compute_write_face()
+
+
+def download_state(dev, m):
+    return [dev.download("state", c, out=np.zeros_like(m.state[c])) if m.n_elem[c] else None for c in (0, 1)]
+
+
+def rel_l2_per_var(got, ref, m):
+    """Relative L2 error per conserved variable over all elements (momentum components, mass, energy)."""
+    errs = []
+    for v in range(m.nv):
+        num = sum(float(np.linalg.norm(got[c][:, v] - ref[c][:, v])) ** 2 for c in (0, 1) if m.n_elem[c])
+        den = sum(float(np.linalg.norm(ref[c][:, v])) ** 2 for c in (0, 1) if m.n_elem[c])
+        errs.append((num / den) ** 0.5)
+    return errs
+
+
+def timed_run(ctx, dev, m, opts, form, warmup, steps):
+    dev.set_precision(FORMS[form][0])
+    reset(dev, m)
+    status = abi.hx_update_status()
+    for _ in range(warmup):
+        dev.update(opts, status)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.update(opts, status)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    return elapsed, ctx.timings(), status
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="ns48", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="fp32_mfma", choices=sorted(FORMS), help="form reported as `value`")
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps for the CPU baseline (0 to skip)")
+    ap.add_argument("--err-steps", type=int, default=10, help="steps after which FP32-MFMA is compared with FP64")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -125,33 +180,38 @@ def main():
     ctx = Context(0)
     copy_gbs = ctx.stream_copy(1 << 31, 10)  # achievable HBM bandwidth of a plain copy kernel on this box
     dev = Device_mesh(ctx, m)
-    dev.compute_write_face()  # Solver::initialize extrapolates the initial state to the faces
-    init = {"state": [a.copy() for a in m.state], "faces": None}
-    status = abi.hx_update_status()
-    for _ in range(args.warmup):
-        dev.update(opts, status)
-    ctx.synchronize()
-    ctx.set_timing(True)
-    ctx.timing_reset()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dev.update(opts, status)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    ctx.set_timing(False)
-    timings = ctx.timings()
-    value = n_dof * 2 * args.steps / elapsed
-    print(f"[bench] GPU: {args.steps} steps in {elapsed:.3f} s, dt {status.time_step:.3e}", file=sys.stderr, flush=True)
+    runs = {}
+    for form in FORMS:
+        elapsed, timings, status = timed_run(ctx, dev, m, opts, form, args.warmup, args.steps)
+        runs[form] = dict(elapsed=elapsed, timings=timings, value=n_dof * 2 * args.steps / elapsed)
+        print(f"[bench] GPU {form}: {args.steps} steps in {elapsed:.3f} s, dt {status.time_step:.3e}",
+              file=sys.stderr, flush=True)
+    chosen = runs[args.precision]
+    value, elapsed, timings = chosen["value"], chosen["elapsed"], chosen["timings"]
 
-    # dominant kernel and its roofline
+    # FP32-MFMA against FP64 from the same initial state after err_steps steps
+    finals = {}
+    for form in FORMS:
+        dev.set_precision(FORMS[form][0])
+        reset(dev, m)
+        st = abi.hx_update_status()
+        for _ in range(args.err_steps):
+            dev.update(opts, st)
+        finals[form] = download_state(dev, m)
+    fp32_err = rel_l2_per_var(finals["fp32_mfma"], finals["fp64"], m)
+    print(f"[bench] FP32-MFMA vs FP64 after {args.err_steps} steps: rel L2 per variable "
+          + " ".join(f"{e:.2e}" for e in fp32_err), file=sys.stderr, flush=True)
+    dev.set_precision(abi.HX_FP64)
+
+    # dominant kernel of the chosen form and its roofline
     dominant = max(timings, key=lambda k: timings[k][0])
     ms, launches, units = timings[dominant]
     avg_ms = ms / launches
     cls = abi.HX_DEFORMED if dominant.startswith("deformed") else abi.HX_CARTESIAN
     alg = algorithmic_bytes(dominant, nd, nq, m.n_elem[cls])
-    symbol = kernel_symbol(dominant, nd, m.row_size)
+    symbol = kernel_symbol(dominant, nd, m.row_size, args.precision)
     roofline = {"bound": "hbm", "kernel": dominant, "symbol": symbol, "launch_ms": round(avg_ms, 5),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": measured_traffic(args.config, "fp64", symbol),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": measured_traffic(args.config, args.precision, symbol),
                 "stream_copy_gbs": round(copy_gbs, 1)}
     if alg is not None:
         achieved = alg / (avg_ms * 1e-3) / 1e9
@@ -160,49 +220,55 @@ def main():
     step_gbs = value * wl["bytes_per_dof_stage"] / 1e9
     roofline["step"] = {"bytes_per_dof_stage": wl["bytes_per_dof_stage"], "achieved": round(step_gbs, 1),
                         "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
-    breakdown = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches": v[1]} for k, v in sorted(timings.items())}
+    breakdown = {form: {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches": v[1]}
+                        for k, v in sorted(r["timings"].items())} for form, r in runs.items()}
 
-    # CPU baseline and full-size parity: the oracle runs cpu_steps from the initial state; the GPU repeats them
+    # CPU baseline and full-size parity: the oracle runs cpu_steps from the initial state; the FP64 GPU form repeats
+    # them
     cpu = None
     parity = None
     if cpu_steps > 0:
         import oracle
         lib = oracle.load()
+        init_state = [a.copy() for a in m.state]
         s = m.struct()
         for c in (0, 1):
             oracle.check(lib.oracle_write_face(ctypes.byref(s), c, None))
-        init["faces"] = m.faces.copy()
         cpu_s, threads = run_oracle_steps(m, opts, cpu_steps)
         cpu = {"value": round(n_dof * 2 * cpu_steps / cpu_s, 1), "unit": "DOF-stage/s", "cores": threads,
                "kind": "port",
                "sample": f"{cpu_steps} time steps ({2 * cpu_steps} stages) of the same {n_elem}-element mesh, "
                          f"FP64, OpenMP over {threads} threads, {cpu_s:.2f} s"}
         print(f"[bench] CPU oracle: {cpu_steps} steps in {cpu_s:.2f} s on {threads} threads", file=sys.stderr, flush=True)
-        # GPU from the same initial state for the same steps
+        oracle_state = [a.copy() for a in m.state]
         for c in (0, 1):
-            if m.n_elem[c]:
-                dev.upload("state", c, init["state"][c])
-                dev.upload("rcache", c, np.zeros_like(m.rcache[c]))
-                dev.upload("tss", c, np.ones_like(m.tss[c]))
-        dev.upload("faces", 0, init["faces"])
+            m.state[c][...] = init_state[c]
+        reset(dev, m)
         st = abi.hx_update_status()
         for _ in range(cpu_steps):
             dev.update(opts, st)
-        errs = []
-        for c in (0, 1):
-            if m.n_elem[c]:
-                got = dev.download("state", c, out=np.zeros_like(m.state[c]))
-                for v in range(m.nv):
-                    ref = m.state[c][:, v]
-                    errs.append(float(np.linalg.norm(got[:, v] - ref) / np.linalg.norm(ref)))
-        parity = {"steps": cpu_steps, "rel_l2_per_var": [float(f"{e:.3e}") for e in errs], "max_rel_l2": max(errs)}
+        errs = rel_l2_per_var(download_state(dev, m), oracle_state, m)
+        parity = {"steps": cpu_steps, "form": "fp64", "rel_l2_per_var": [float(f"{e:.3e}") for e in errs],
+                  "max_rel_l2": max(errs)}
 
+    other = "fp64" if args.precision == "fp32_mfma" else "fp32_mfma"
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "DOF-stage/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-mfma+f64" if args.precision == "fp32_mfma" else "f64", "data": "synthetic",
         "config": {"workload": wl["desc"], "name": args.config, "n_elem": n_elem, "dof": n_dof, "row_size": m.row_size,
-                   "precision": "fp64 (reference arithmetic)"},
+                   "precision": args.precision},
+        "forms": {
+            args.precision: {"value": round(value, 1), "ms_per_step": round(1e3 * elapsed / args.steps, 4)},
+            other: {"value": round(runs[other]["value"], 1),
+                    "ms_per_step": round(1e3 * runs[other]["elapsed"] / args.steps, 4)},
+            "fp32_mfma_speedup_vs_fp64": round(runs["fp32_mfma"]["value"] / runs["fp64"]["value"], 3),
+            "fp32_mfma_vs_fp64_rel_l2_per_var": {"steps": args.err_steps,
+                                                  "vars": [f"momentum{i}" for i in range(nd)] + ["mass", "energy"],
+                                                  "rel_l2": [float(f"{e:.3e}") for e in fp32_err],
+                                                  "max": float(f"{max(fp32_err):.3e}")},
+        },
         "roofline": roofline, "cpu_baseline": cpu, "parity_vs_cpu": parity, "kernel_breakdown": breakdown,
     }
     print(json.dumps(line))

@@ -20,6 +20,7 @@ struct hx_ctx
   hipStream_t stream = nullptr;
   std::string err;
   void* ops[HX_MAX_ROW_SIZE + 1] = {};  // device Ops<rs> tables
+  std::vector<char> host_ops[HX_MAX_ROW_SIZE + 1];  // host copies, for kernels taking constants by value
   double* scalar = nullptr;             // device scratch for reductions
   // per-category event timing
   struct Pending
@@ -103,6 +104,8 @@ hx_status upload_ops(hx_ctx* ctx, void** out)
   fill_ops<RS>(o, RS - 2);
   HX_TRY(ctx, hipMalloc(out, sizeof(o)), "hipMalloc(ops)");
   HX_TRY(ctx, hipMemcpy(*out, &o, sizeof(o), hipMemcpyHostToDevice), "hipMemcpy(ops)");
+  const char* bytes = reinterpret_cast<const char*>(&o);
+  ctx->host_ops[RS].assign(bytes, bytes + sizeof(o));
   return HX_OK;
 }
 
@@ -249,6 +252,7 @@ Launch make_launch(hx_mesh* mesh)
   l.stream = mesh->ctx->stream;
   l.m = mesh->dm;
   l.ops = mesh->ctx->ops[mesh->dm.rs];
+  l.hops = mesh->ctx->host_ops[mesh->dm.rs].data();
   l.scalar = mesh->ctx->scalar;
   l.precision = mesh->precision;
   return l;
@@ -618,12 +622,9 @@ hx_status hx_mesh_download(hx_mesh* mesh, int field, int cls, double* host, size
 hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision)
 {
   if (!mesh) return HX_ERR_INVALID;
-  if (precision == HX_FP64) {
-    mesh->precision = precision;
-    return HX_OK;
-  }
-  if (precision == HX_FP32_MFMA) return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "FP32-MFMA form not built");
-  return fail(mesh->ctx, HX_ERR_INVALID, "unknown precision");
+  if (precision != HX_FP64 && precision != HX_FP32_MFMA) return fail(mesh->ctx, HX_ERR_INVALID, "unknown precision");
+  mesh->precision = precision;
+  return HX_OK;
 }
 
 hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size_t* n)

@@ -69,10 +69,12 @@ struct Ops
 struct Transport
 {
   double const_val, ref_val, ref_temp, sqrt_ref_temp, temp_offset;
-  __device__ double coefficient(double sqrt_temp) const
+  template <class T>
+  __device__ T coefficient(T sqrt_temp) const
   {
-    double r = sqrt_temp / sqrt_ref_temp;
-    return const_val + ref_val * (((1. * r) * r) * r) * (ref_temp + temp_offset) / (sqrt_temp * sqrt_temp + temp_offset);
+    T r = sqrt_temp / T(sqrt_ref_temp);
+    return T(const_val) + T(ref_val) * (((T(1) * r) * r) * r) * (T(ref_temp) + T(temp_offset)) /
+                              (sqrt_temp * sqrt_temp + T(temp_offset));
   }
 };
 
@@ -113,19 +115,21 @@ __device__ inline int face1_index(const Dir& dir, int i)
   }
 }
 
-// pde::Navier_stokes::Pde::Computation::compute_flux_conv for one flux direction (pde.hpp:97-122)
-template <int ND>
-__device__ inline void flux_conv(const double (&u)[ND + 2], const double (&n)[ND], double (&f)[ND + 2], double& pressure)
+// pde::Navier_stokes::Pde::Computation::compute_flux_conv for one flux direction (pde.hpp:97-122); the
+// divisions by the density are one reciprocal. T is the arithmetic type (double, or float in the FP32 form).
+template <int ND, class T>
+__device__ inline void flux_conv(const T (&u)[ND + 2], const T (&n)[ND], T (&f)[ND + 2])
 {
-  const double mass = u[ND];
-  double kin = 0;
+  constexpr T gm1 = T(heat_rat - 1.);
+  const T inv_mass = T(1) / u[ND];
+  T kin = 0;
   for (int j = 0; j < ND; ++j) kin += u[j] * u[j];
-  kin *= .5 / mass;
-  pressure = (heat_rat - 1.) * (u[ND + 1] - kin);
-  double mf = 0;
+  kin *= T(.5) * inv_mass;
+  const T pressure = gm1 * (u[ND + 1] - kin);
+  T mf = 0;
   for (int j = 0; j < ND; ++j) mf += u[j] * n[j];
   f[ND] = mf;
-  const double vol_flux = mf / mass;
+  const T vol_flux = mf * inv_mass;
   f[ND + 1] = (u[ND + 1] + pressure) * vol_flux;
   for (int j = 0; j < ND; ++j) f[j] = u[j] * vol_flux + pressure * n[j];
 }
@@ -134,60 +138,51 @@ __device__ inline void flux_conv(const double (&u)[ND + 2], const double (&n)[ND
 template <int ND>
 __device__ inline double char_speed(const double (&u)[ND + 2])
 {
-  const double sound = sqrt(heat_rat * (heat_rat - 1) * u[ND + 1] / u[ND]);
+  const double inv_mass = 1. / u[ND];
+  const double sound = sqrt(heat_rat * (heat_rat - 1) * u[ND + 1] * inv_mass);
   double sq = 0;
   for (int j = 0; j < ND; ++j) sq += u[j] * u[j];
-  return sound + sqrt(sq) / u[ND];
+  return sound + sqrt(sq) * inv_mass;
 }
 
-// compute_scalars_diff + compute_flux_diff (pde.hpp:130-158) in the element's reference frame:
-// grad[v][j] is the physical gradient, nrm[j][i] the reference-level normal matrix (identity for Cartesian),
-// result fd[v][i] = (flux_diff * normal)[v][i].
-template <int ND, bool DEF>
-__device__ inline void flux_diff(const Ns_params& p, const double (&u)[ND + 2], double bulk, double lap,
-                                 const double (&grad)[ND + 2][ND], const double (&nrm)[ND][ND], double (&fd)[ND + 2][ND])
+// compute_scalars_diff + the physical part of compute_flux_diff (pde.hpp:130-155): f[v][j], the viscous flux of
+// variable v in physical direction j, from the physical gradient grad[v][j]. The reference's final product with the
+// normal matrix (pde.hpp:156) is applied per reference direction by the caller.
+template <int ND, class T>
+__device__ inline void flux_visc(const Ns_params& p, const T (&u)[ND + 2], T bulk, T lap, const T (&grad)[ND + 2][ND],
+                                 T (&f)[ND + 2][ND])
 {
-  const double mass = u[ND];
-  double kin = 0;
+  const T mass = u[ND];
+  const T inv_mass = T(1) / mass;
+  T kin = 0;
   for (int j = 0; j < ND; ++j) kin += u[j] * u[j];
-  kin *= .5 / mass;
-  const double bulk_av = fabs(bulk), lap_av = fabs(lap);
-  const double sqrt_temp = sqrt(fmax((u[ND + 1] - kin) / mass, 0.) * (heat_rat - 1) / specific_gas_air);
-  const double mu = p.visc.coefficient(sqrt_temp);
-  const double k = p.cond.coefficient(sqrt_temp);
-  const double energy_cond = k * (heat_rat - 1) / specific_gas_air;
-  double veloc[ND];
-  for (int i = 0; i < ND; ++i) veloc[i] = u[i] / mass;
-  double vg[ND][ND];
+  kin *= T(.5) * inv_mass;
+  const T bulk_av = fabs(bulk), lap_av = fabs(lap);
+  constexpr T r_factor = T((heat_rat - 1) / specific_gas_air);
+  const T sqrt_temp = sqrt(fmax((u[ND + 1] - kin) * inv_mass, T(0)) * r_factor);
+  const T mu = p.visc.coefficient(sqrt_temp);
+  const T energy_cond = p.cond.coefficient(sqrt_temp) * r_factor;
+  T veloc[ND];
+  for (int i = 0; i < ND; ++i) veloc[i] = u[i] * inv_mass;
+  T vg[ND][ND];
   for (int i = 0; i < ND; ++i)
-    for (int j = 0; j < ND; ++j) vg[i][j] = (grad[i][j] - veloc[i] * grad[ND][j]) / mass;
-  double trace = 0;
+    for (int j = 0; j < ND; ++j) vg[i][j] = (grad[i][j] - veloc[i] * grad[ND][j]) * inv_mass;
+  T trace = 0;
   for (int i = 0; i < ND; ++i) trace += vg[i][i];
-  const double dil = (bulk_av * mass - 2. / 3. * mu) * trace;
-  double f[ND + 2][ND];
+  const T dil = (bulk_av * mass - T(2. / 3.) * mu) * trace;
   for (int v = 0; v < ND + 2; ++v)
     for (int j = 0; j < ND; ++j) f[v][j] = -lap_av * grad[v][j];
+  const T e_over_m2 = u[ND + 1] * inv_mass * inv_mass;
   for (int j = 0; j < ND; ++j) {
-    double vtvg = 0, vts = 0;
+    T vtvg = 0, vts = 0;
     for (int i = 0; i < ND; ++i) {
-      double stress = mu * (vg[i][j] + vg[j][i]) + (i == j ? dil : 0.);
+      const T stress = mu * (vg[i][j] + vg[j][i]) + (i == j ? dil : T(0));
       f[i][j] -= stress;
       vtvg += veloc[i] * vg[i][j];
       vts += veloc[i] * stress;
     }
-    double ieg = -u[ND + 1] / mass / mass * grad[ND][j] + grad[ND + 1][j] / mass - vtvg;
+    const T ieg = -e_over_m2 * grad[ND][j] + grad[ND + 1][j] * inv_mass - vtvg;
     f[ND + 1][j] -= vts + energy_cond * ieg;
-  }
-  if constexpr (DEF) {
-    for (int v = 0; v < ND + 2; ++v)
-      for (int i = 0; i < ND; ++i) {
-        double s = 0;
-        for (int j = 0; j < ND; ++j) s += f[v][j] * nrm[j][i];
-        fd[v][i] = s;
-      }
-  } else {
-    for (int v = 0; v < ND + 2; ++v)
-      for (int i = 0; i < ND; ++i) fd[v][i] = f[v][i];
   }
 }
 
@@ -195,15 +190,15 @@ __device__ inline void flux_diff(const Ns_params& p, const double (&u)[ND + 2], 
 template <int ND>
 __device__ inline double diffusivity(const Ns_params& p, const double (&u)[ND + 2], double bulk, double lap)
 {
-  const double mass = u[ND];
+  const double inv_mass = 1. / u[ND];
   double kin = 0;
   for (int j = 0; j < ND; ++j) kin += u[j] * u[j];
-  kin *= .5 / mass;
-  const double sqrt_temp = sqrt(fmax((u[ND + 1] - kin) / mass, 0.) * (heat_rat - 1) / specific_gas_air);
+  kin *= .5 * inv_mass;
+  constexpr double r_factor = (heat_rat - 1) / specific_gas_air;
+  const double sqrt_temp = sqrt(fmax((u[ND + 1] - kin) * inv_mass, 0.) * r_factor);
   const double mu = p.visc.coefficient(sqrt_temp);
-  const double k = p.cond.coefficient(sqrt_temp);
-  const double energy_cond = k * (heat_rat - 1) / specific_gas_air;
-  return fabs(fabs(lap)) + fmax(fabs(fabs(bulk)) + mu / mass, energy_cond / mass);
+  const double energy_cond = p.cond.coefficient(sqrt_temp) * r_factor;
+  return fabs(lap) + fmax(fabs(bulk) + mu * inv_mass, energy_cond * inv_mass);
 }
 
 // Row iteration along dimension `i_dim` (Row_index.hpp:14-64)

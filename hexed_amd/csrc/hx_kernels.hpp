@@ -82,72 +82,366 @@ __device__ inline void filter_rows(const Ops<RS>& ops, double* buf, int el, int 
   __syncthreads();
 }
 
-// ---- Local, Euler (Spatial.hpp:326-509 with Navier_stokes<false>) -----------------------------------------
-template <int ND, int RS, bool DEF>
-__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_local_euler(Dev_mesh m, const Ops<RS>* __restrict__ gops,
-                                                                      double coef, int stage, int compute_residual,
-                                                                      int use_filter)
+// ---- element kernels: row contractions ------------------------------------------------------------------
+// The element kernels alternate between point phases (one thread per quadrature point) and row phases, where one
+// thread owns a "column" (an element, a variable and one row along a reference direction) and computes the whole
+// row of outputs from the row's rs values and its two face values. The operator constants are a by-value kernel
+// argument: the fully unrolled contractions read them with uniform scalar loads, so one LDS read of the data feeds
+// rs multiply-adds.
+template <int RS, class T>
+struct Eops
+{
+  T diff[RS][RS];  // D
+  T lift[RS][2];   // L = W^{-1} B^T diag(-1, 1)
+  T bound[2][RS];  // B
+  T filt[RS][RS];  // modal filter
+};
+
+template <int RS, class T>
+Eops<RS, T> make_eops(const Ops<RS>& o)
+{
+  Eops<RS, T> e;
+  for (int i = 0; i < RS; ++i) {
+    for (int k = 0; k < RS; ++k) {
+      e.diff[i][k] = T(o.diff[i][k]);
+      e.filt[i][k] = T(o.filt[i][k]);
+    }
+    e.lift[i][0] = T(o.lift[i][0]);
+    e.lift[i][1] = T(o.lift[i][1]);
+    e.bound[0][i] = T(o.bound[0][i]);
+    e.bound[1][i] = T(o.bound[1][i]);
+  }
+  return e;
+}
+
+// Arithmetic forms of the element kernels. FP64: the reference's arithmetic. FP32-MFMA: volume fluxes, gradients
+// and derivative contractions in FP32, the contractions on the matrix cores with FP32 accumulation; the state, the
+// RK update and residual cache, the face fluxes and their jumps against the interior extrapolation, the Jacobian
+// determinant scaling and the face-buffer extrapolations stay FP64.
+struct Fp64_form
+{
+  using T = double;
+  static constexpr bool mfma = false;
+};
+struct Fp32_mfma_form
+{
+  using T = float;
+  static constexpr bool mfma = true;
+};
+
+// Threads per workgroup: one per quadrature point, rounded up to whole wavefronts for the matrix-core form so that
+// every lane of a wavefront takes part in its matrix instructions.
+template <int ND, int RS, class P>
+constexpr int form_block()
+{
+  constexpr int b = Shape<ND, RS>::BLOCK;
+  return P::mfma ? (b + 63) / 64 * 64 : b;
+}
+
+// Calls f(ok, el, v, r, base, sd) for every column (el, v, r) of direction i_dim: v < NVAR, r the row counter (face
+// qpoint), base the row's first qpoint and sd its stride (Row_index.hpp:14-64). With UNIFORM, every thread takes part
+// in each round (ok is false past the last column and the indices are then those of column 0).
+template <int ND, int RS, int NVAR, int NT, bool UNIFORM, class F>
+__device__ inline void for_columns(int i_dim, F&& f)
 {
   using S = Shape<ND, RS>;
   using R = Row<ND, RS>;
-  constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB;
+  constexpr int PER = NVAR * S::NFQ, TOTAL = S::EPB * PER;
+  const int sd = R::stride(i_dim);
+  if constexpr (UNIFORM) {
+    for (int c0 = 0; c0 < TOTAL; c0 += NT) {
+      const int c = c0 + int(threadIdx.x);
+      const bool ok = c < TOTAL;
+      const int cc = ok ? c : 0;
+      const int el = cc / PER, vr = cc % PER, v = vr / S::NFQ, r = vr % S::NFQ;
+      f(ok, el, v, r, R::base_of_fq(r, i_dim), sd);
+    }
+  } else {
+    for (int c = threadIdx.x; c < TOTAL; c += NT) {
+      const int el = c / PER, vr = c % PER, v = vr / S::NFQ, r = vr % S::NFQ;
+      f(true, el, v, r, R::base_of_fq(r, i_dim), sd);
+    }
+  }
+}
+
+template <int RS, class T>
+__device__ inline void load_row(const T* p, int sd, T (&x)[RS])
+{
+#pragma unroll
+  for (int k = 0; k < RS; ++k) x[k] = p[k * sd];
+}
+
+// y = M x for an rs x rs matrix M
+template <int RS, class T>
+__device__ inline void row_matvec(const T (&mat)[RS][RS], const T (&x)[RS], T (&y)[RS])
+{
+#pragma unroll
+  for (int i = 0; i < RS; ++i) {
+    T a = 0;
+#pragma unroll
+    for (int k = 0; k < RS; ++k) a += mat[i][k] * x[k];
+    y[i] = a;
+  }
+}
+
+// out[base + i sd] = -y[i] (first direction), else out -= y
+template <int RS, class T>
+__device__ inline void sub_row(T* out, int sd, const T (&y)[RS], bool first)
+{
+#pragma unroll
+  for (int i = 0; i < RS; ++i) out[i * sd] = first ? -y[i] : out[i * sd] - y[i];
+}
+
+// boundary(s) . x in FP64 (Spatial.hpp:29-71)
+template <int RS, class T>
+__device__ inline double row_bound(const Eops<RS, double>& o, int s, const T (&x)[RS])
+{
+  double a = 0;
+#pragma unroll
+  for (int k = 0; k < RS; ++k) a += o.bound[s][k] * double(x[k]);
+  return a;
+}
+
+// Row derivatives, Derivative.hpp:51-60: faces(x, b0, b1) = D x + L (b - B x), interior(x) = D x.
+template <int RS, class P>
+struct Contraction;
+
+// FP64: the reference's operation order, D x + L (b - B x), on the vector units. (Folding the lift into
+// (D - L B) x + L b saves 2 rs multiply-adds per column but rounds the two large face terms separately, which
+// measurably loosens agreement with the reference arithmetic.)
+template <int RS>
+struct Contraction<RS, Fp64_form>
+{
+  const Eops<RS, double>& o;
+  __device__ explicit Contraction(const Eops<RS, double>& o_) : o(o_) {}
+  __device__ void faces(const double (&x)[RS], double b0, double b1, double (&y)[RS]) const
+  {
+    const double j0 = b0 - row_bound(o, 0, x), j1 = b1 - row_bound(o, 1, x);
+#pragma unroll
+    for (int i = 0; i < RS; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int k = 0; k < RS; ++k) a += o.diff[i][k] * x[k];
+      y[i] = a + (o.lift[i][0] * j0 + o.lift[i][1] * j1);
+    }
+  }
+  __device__ void interior(const double (&x)[RS], double (&y)[RS]) const
+  {
+#pragma unroll
+    for (int i = 0; i < RS; ++i) {
+      double a = 0;
+#pragma unroll
+      for (int k = 0; k < RS; ++k) a += o.diff[i][k] * x[k];
+      y[i] = a;
+    }
+  }
+  // faces() where `with_faces`, else interior()
+  __device__ void faces_if(bool with_faces, const double (&x)[RS], double b0, double b1, double (&y)[RS]) const
+  {
+    if (with_faces) faces(x, b0, b1, y);
+    else interior(x, y);
+  }
+};
+
+// FP32-MFMA: V_MFMA_F32_4X4X1_16B_F32 computes 16 independent 4x4 outer products per wavefront, lane 4b + j holding
+// column j of block b. Each lane owns one column of data (its B operand) and the operator row 4t + (lane mod 4) of
+// [D | L] (its A operand), so after rs + 2 rank-1 steps lane 4b + j holds outputs 4t..4t+3 of its own row. The face
+// jumps b - B x are formed in FP64 before they enter the matrix cores.
+typedef float hx_f4 __attribute__((ext_vector_type(4)));
+
+template <int RS>
+struct Contraction<RS, Fp32_mfma_form>
+{
+  static constexpr int TILES = (RS + 3) / 4;
+  const Eops<RS, double>& o;
+  float a[TILES][RS + 2];
+  __device__ explicit Contraction(const Eops<RS, double>& o_) : o(o_)
+  {
+    const int lane_row = threadIdx.x & 3;
+#pragma unroll
+    for (int t = 0; t < TILES; ++t)
+#pragma unroll
+      for (int k = 0; k < RS + 2; ++k) {
+        float val = 0.f;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = 4 * t + rr;
+          if (row < RS && lane_row == rr) val = float(k < RS ? o.diff[row][k] : o.lift[row][k - RS]);
+        }
+        a[t][k] = val;
+      }
+  }
+  template <bool FACES>
+  __device__ void run(const float (&x)[RS], float j0, float j1, float (&y)[RS]) const
+  {
+    hx_f4 acc[TILES];
+#pragma unroll
+    for (int t = 0; t < TILES; ++t) acc[t] = hx_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < RS; ++k)
+#pragma unroll
+      for (int t = 0; t < TILES; ++t) acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][k], x[k], acc[t], 0, 0, 0);
+    if constexpr (FACES) {
+#pragma unroll
+      for (int t = 0; t < TILES; ++t) {
+        acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][RS], j0, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][RS + 1], j1, acc[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TILES; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (4 * t + r < RS) y[4 * t + r] = acc[t][r];
+  }
+  __device__ void faces(const float (&x)[RS], double b0, double b1, float (&y)[RS]) const
+  {
+    run<true>(x, float(b0 - row_bound(o, 0, x)), float(b1 - row_bound(o, 1, x)), y);
+  }
+  __device__ void interior(const float (&x)[RS], float (&y)[RS]) const { run<false>(x, 0.f, 0.f, y); }
+  // faces() where `with_faces`, else interior(), without divergence around the matrix instructions: lanes without
+  // faces lift zero jumps
+  __device__ void faces_if(bool with_faces, const float (&x)[RS], double b0, double b1, float (&y)[RS]) const
+  {
+    const float j0 = with_faces ? float(b0 - row_bound(o, 0, x)) : 0.f;
+    const float j1 = with_faces ? float(b1 - row_bound(o, 1, x)) : 0.f;
+    run<true>(x, j0, j1, y);
+  }
+};
+
+// Modal filter of NVAR [NQ] arrays per element at `buf` (element stride `per`), in place, one direction at a time
+// (Spatial.hpp:472-481). Ends synchronized.
+template <int ND, int RS, int NVAR, int NT, class T>
+__device__ inline void filter_columns(const Eops<RS, double>& o, T* buf, int per)
+{
+  using S = Shape<ND, RS>;
+  T filt[RS][RS];
+  for (int i = 0; i < RS; ++i)
+    for (int k = 0; k < RS; ++k) filt[i][k] = T(o.filt[i][k]);
+  for (int i = 0; i < ND; ++i) {
+    __syncthreads();
+    for_columns<ND, RS, NVAR, NT, false>(i, [&](bool, int el, int v, int, int base, int sd) {
+      T* row = buf + el * per + v * S::NQ + base;
+      T x[RS], y[RS];
+      load_row(row, sd, x);
+      row_matvec(filt, x, y);
+#pragma unroll
+      for (int k = 0; k < RS; ++k) row[k * sd] = y[k];
+    });
+  }
+  __syncthreads();
+}
+
+// Loads block `ldg` (0: state/flux, 1: LDG) of every face of the workgroup's elements into dst[EPB][2ND][NV][NFQ]
+// (element stride `per`).
+template <int ND, int RS, int NT>
+__device__ inline void load_faces(const Dev_mesh& m, int cls, int ldg, double* dst, int per)
+{
+  using S = Shape<ND, RS>;
+  constexpr int PF = 2 * ND * S::NV * S::NFQ;
+  const int n = m.n_elem[cls];
+  for (int k = threadIdx.x; k < S::EPB * PF; k += NT) {
+    const int el = k / PF, r = k % PF, f = r / (S::NV * S::NFQ), vq = r % (S::NV * S::NFQ);
+    const int e = blockIdx.x * S::EPB + el;
+    dst[el * per + r] = e < n ? slot_ptr(m, m.face_slot[cls][e * 2 * ND + f])[ldg * S::NV * S::NFQ + vq] : 0.;
+  }
+}
+
+// Write_face of the FP64 [NV][NQ] arrays at `buf` (element stride `per`) into block 0 of every face
+// (Spatial.hpp:29-71)
+template <int ND, int RS, int NT>
+__device__ inline void write_faces_from(const Dev_mesh& m, int cls, const Eops<RS, double>& o, const double* buf, int per)
+{
+  using S = Shape<ND, RS>;
+  const int n = m.n_elem[cls];
+  for (int i = 0; i < ND; ++i) {
+    for_columns<ND, RS, S::NV, NT, false>(i, [&](bool, int el, int v, int r, int base, int sd) {
+      const int e = blockIdx.x * S::EPB + el;
+      if (e >= n) return;
+      double x[RS];
+      load_row(buf + el * per + v * S::NQ + base, sd, x);
+      const int* fs = m.face_slot[cls] + e * 2 * ND + 2 * i;
+      slot_ptr(m, fs[0])[v * S::NFQ + r] = row_bound(o, 0, x);
+      slot_ptr(m, fs[1])[v * S::NFQ + r] = row_bound(o, 1, x);
+    });
+  }
+}
+
+// ---- Local, Euler (Spatial.hpp:326-509 with Navier_stokes<false>) -----------------------------------------
+template <int ND, int RS, class P>
+struct Euler_lds
+{
+  using S = Shape<ND, RS>;
+  using T = typename P::T;
+  // arrays of T
+  static constexpr int F = 0;                               // [NV][NQ] flux along the current direction
+  static constexpr int RT = F + S::NV * S::NQ;              // [NV][NQ] time rate
+  static constexpr int PER_T = RT + S::NV * S::NQ;
+  // arrays of double
+  static constexpr int SB = 0;                              // [2ND][NV][NFQ] numerical face fluxes
+  static constexpr int U = 0;                               // [NV][NQ] updated state, for Write_face (after SB)
+  static constexpr int PER_D = 2 * ND * S::NFQ > S::NQ ? 2 * ND * S::NV * S::NFQ : S::NV * S::NQ;
+};
+
+template <int ND, int RS, bool DEF, class P>
+__global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_mesh m, Eops<RS, double> o,
+                                                                           double coef, int stage,
+                                                                           int compute_residual, int use_filter)
+{
+  using S = Shape<ND, RS>;
+  using L = Euler_lds<ND, RS, P>;
+  using T = typename P::T;
+  constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = form_block<ND, RS, P>();
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
-  __shared__ Ops<RS> ops;
-  __shared__ double s_flux[EPB][ND][NV][NQ];
-  __shared__ double s_face[EPB][2 * ND][NV][NFQ];
-  const int t = threadIdx.x, el = t / NQ, q = t % NQ;
+  __shared__ T lt[EPB * L::PER_T];
+  __shared__ double ld[EPB * L::PER_D];
+  const int t = threadIdx.x;
+  const bool pt = t < EPB * NQ;
+  const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0;
   const int n = m.n_elem[cls];
   const int e = blockIdx.x * EPB + el;
-  const bool valid = e < n;
-  load_ops(ops, gops, t, S::BLOCK);
+  const bool valid = pt && e < n;
+  T* my = lt + el * L::PER_T;
+  const Contraction<RS, P> con(o);
 
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
-  for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : (v == ND ? 1. : 0.);
+  for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
+  T ut[NV];
+  for (int v = 0; v < NV; ++v) ut[v] = T(u[v]);
   double det = 1.;
-  double met[ND][ND];
+  T met[ND][ND];
   if constexpr (DEF) {
     const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
     for (int i = 0; i < ND; ++i)
-      for (int j = 0; j < ND; ++j) met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
+      for (int j = 0; j < ND; ++j) met[i][j] = valid ? T(mt[(i * ND + j) * NQ + q]) : T(i == j);
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
-  // volume flux along each reference direction (Spatial.hpp:404-419)
+  load_faces<ND, RS, NT>(m, cls, 0, ld + L::SB, L::PER_D);
   for (int i = 0; i < ND; ++i) {
-    double nrm[ND];
+    // volume flux along reference direction i (Spatial.hpp:404-419)
+    T nrm[ND], f[NV];
     for (int j = 0; j < ND; ++j) {
       if constexpr (DEF) nrm[j] = met[i][j];
-      else nrm[j] = (i == j);
+      else nrm[j] = T(i == j);
     }
-    double f[NV], p;
-    flux_conv<ND>(u, nrm, f, p);
-    for (int v = 0; v < NV; ++v) s_flux[el][i][v][q] = f[v];
+    flux_conv<ND>(ut, nrm, f);
+    if (i) __syncthreads();
+    if (pt)
+      for (int v = 0; v < NV; ++v) my[L::F + v * NQ + q] = f[v];
+    __syncthreads();
+    // -derivative(row_f, face_f) (Spatial.hpp:444-457)
+    for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
+      T* b = lt + el2 * L::PER_T;
+      const double* fb = ld + el2 * L::PER_D + L::SB;
+      T x[RS], y[RS];
+      load_row(b + L::F + v * NQ + base, sd, x);
+      con.faces(x, fb[((2 * i) * NV + v) * NFQ + r], fb[((2 * i + 1) * NV + v) * NFQ + r], y);
+      if (ok) sub_row(b + L::RT + v * NQ + base, sd, y, i == 0);
+    });
   }
-  // numerical face fluxes written by Neighbor
-  for (int k = t; k < EPB * S::NFACE_DOF; k += S::BLOCK) {
-    const int el2 = k / S::NFACE_DOF, r = k % S::NFACE_DOF, f = r / (NV * NFQ), vq = r % (NV * NFQ);
-    const int e2 = blockIdx.x * EPB + el2;
-    (&s_face[el2][f][0][0])[vq] = e2 < n ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[vq] : 0.;
-  }
-  __syncthreads();
-  // -derivative(row_f, face_f) along every dimension (Spatial.hpp:444-456, Derivative.hpp:51-55)
-  double rate[NV] = {};
-  for (int i = 0; i < ND; ++i) {
-    const int ki = R::node(q, i), b = R::base(q, i), sd = R::stride(i), fq = R::fq(q, i);
-    for (int v = 0; v < NV; ++v) {
-      double dq = 0, b0 = 0, b1 = 0;
-      for (int k = 0; k < RS; ++k) {
-        const double fk = s_flux[el][i][v][b + k * sd];
-        dq += ops.diff[ki][k] * fk;
-        b0 += ops.bound[0][k] * fk;
-        b1 += ops.bound[1][k] * fk;
-      }
-      const double j0 = s_face[el][2 * i][v][fq] - b0;
-      const double j1 = s_face[el][2 * i + 1][v][fq] - b1;
-      rate[v] -= dq + (ops.lift[ki][0] * j0 + ops.lift[ki][1] * j1);
-    }
-  }
-  if (use_filter) filter_rows<ND, RS, NV>(ops, &s_flux[0][0][0][0], el, q, rate);
+  if (use_filter) filter_columns<ND, RS, NV, NT>(o, lt + L::RT, L::PER_T);
+  else __syncthreads();
   // stage update (Spatial.hpp:483-503)
   if (valid) {
     double mult = coef * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
@@ -155,7 +449,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_local_euler(Dev_mesh
     double* rc = m.rcache[cls] + size_t(e) * NV * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
-      double r = rate[v];
+      double r = double(my[L::RT + v * NQ + q]);
       if (stage) r -= rc[v * NQ + q];
       else rc[v * NQ + q] = r;
       r *= mult;
@@ -166,91 +460,85 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_local_euler(Dev_mesh
       for (int v = 0; v < NV; ++v) sw[v * NQ + q] = u[v];
   }
   // Write_face of the updated state (Spatial.hpp:507)
+  if (pt)
+    for (int v = 0; v < NV; ++v) ld[el * L::PER_D + L::U + v * NQ + q] = u[v];
   __syncthreads();
-  for (int v = 0; v < NV; ++v) s_flux[el][0][v][q] = u[v];
-  __syncthreads();
-  double* su = &s_flux[0][0][0][0];
-  // s_flux[el] is ND*NV*NQ long; the state block of element el sits at its start
-  using Sh = S;
-  for (int k = t; k < EPB * Sh::NFACE_DOF; k += Sh::BLOCK) {
-    const int el2 = k / Sh::NFACE_DOF, r = k % Sh::NFACE_DOF;
-    const int f = r / (NV * NFQ), v = (r / NFQ) % NV, fq = r % NFQ;
-    const int e2 = blockIdx.x * EPB + el2;
-    if (e2 >= n) continue;
-    const int i = f / 2, s = f % 2;
-    const int b = R::base_of_fq(fq, i), sd = R::stride(i);
-    const double* row = su + (el2 * ND * NV + v) * NQ + b;
-    double acc = 0;
-    for (int kk = 0; kk < RS; ++kk) acc += ops.bound[s][kk] * row[kk * sd];
-    slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[v * NFQ + fq] = acc;
-  }
+  write_faces_from<ND, RS, NT>(m, cls, o, ld + L::U, L::PER_D);
 }
 
 // ---- Local, Navier-Stokes stage 0 (Spatial.hpp:326-509 with Navier_stokes<true>) -------------------------
-template <int ND, int RS, bool DEF>
+template <int ND, int RS, bool DEF, class P>
 struct Ns_lds
 {
   using S = Shape<ND, RS>;
-  // phase 1: gradient inputs; phase 2: fluxes. Both start at offset 0 of the element's region.
-  static constexpr int P1 = S::NV * S::NQ + 2 * ND * S::NV * S::NFQ + (DEF ? ND * ND * S::NQ + 2 * ND * ND * S::NFQ : 0);
-  static constexpr int P2 = 2 * ND * S::NV * S::NQ + 2 * ND * S::NV * S::NFQ;
-  static constexpr int PER_ELEM = P1 > P2 ? P1 : P2;
+  static constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV;
+  // arrays of T, gradient phase
+  static constexpr int U = 0;                                     // [NV][NQ] state
+  static constexpr int G = U + NV * NQ;                           // [NV][NQ] one gradient component
+  static constexpr int SN = G + NV * NQ;                          // [ND][ND][NQ] reference-level normals
+  static constexpr int P1_T = SN + (DEF ? ND * ND * NQ : 0);
+  // arrays of T, residual phase
+  static constexpr int F = 0;                                     // [2][NV][NQ] convective, viscous flux
+  static constexpr int RT = F + 2 * NV * NQ;                      // [2][NV][NQ] convective, viscous rate
+  static constexpr int P2_T = RT + 2 * NV * NQ;
+  static constexpr int PER_T = P1_T > P2_T ? P1_T : P2_T;
+  // arrays of double
+  static constexpr int SL = 0;                                    // [2ND][NV][NFQ] LDG face state
+  static constexpr int SFN = SL + 2 * ND * NV * NFQ;              // [2ND][ND][NFQ] face normals
+  static constexpr int P1_D = SFN + (DEF ? 2 * ND * ND * NFQ : 0);
+  static constexpr int SB = 0;                                    // [2ND][NV][NFQ] numerical face fluxes
+  static constexpr int P2_D = SB + 2 * ND * NV * NFQ;
+  static constexpr int PER_D = P1_D > P2_D ? P1_D : P2_D;
 };
 
-template <int ND, int RS, bool DEF>
-__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_local_ns(Dev_mesh m, const Ops<RS>* __restrict__ gops,
-                                                                   Ns_params ns, double coef, int compute_residual,
-                                                                   int use_filter)
+template <int ND, int RS, bool DEF, class P>
+__global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh m, Eops<RS, double> o, Ns_params ns,
+                                                                        double coef, int compute_residual,
+                                                                        int use_filter)
 {
   using S = Shape<ND, RS>;
-  using R = Row<ND, RS>;
-  using L = Ns_lds<ND, RS, DEF>;
-  constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB;
+  using L = Ns_lds<ND, RS, DEF, P>;
+  using T = typename P::T;
+  constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = form_block<ND, RS, P>();
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
-  __shared__ Ops<RS> ops;
-  __shared__ double lds[EPB * L::PER_ELEM];
-  const int t = threadIdx.x, el = t / NQ, q = t % NQ;
+  __shared__ T lt[EPB * L::PER_T];
+  __shared__ double ld[EPB * L::PER_D];
+  const int t = threadIdx.x;
+  const bool pt = t < EPB * NQ;
+  const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0;
   const int n = m.n_elem[cls];
   const int e = blockIdx.x * EPB + el;
-  const bool valid = e < n;
-  load_ops(ops, gops, t, S::BLOCK);
-  double* my = lds + el * L::PER_ELEM;
-  // phase 1 layout
-  double* su = my;                                  // [NV][NQ]
-  double* sl = su + NV * NQ;                        // [2ND][NV][NFQ] LDG face state
-  double* sn = sl + 2 * ND * NV * NFQ;              // [ND][ND][NQ] reference-level normals (DEF)
-  double* sfn = sn + ND * ND * NQ;                  // [2ND][ND][NFQ] face normals (DEF)
+  const bool valid = pt && e < n;
+  T* my = lt + el * L::PER_T;
+  const Contraction<RS, P> con(o);
 
   double u[NV];
+  T ut[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) {
-    u[v] = valid ? st[v * NQ + q] : (v == ND ? 1. : (v == ND + 1 ? 1. : 0.));
-    su[v * NQ + q] = u[v];
+    u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
+    ut[v] = T(u[v]);
+    if (pt) my[L::U + v * NQ + q] = ut[v];
   }
-  double bulk = 0., lap = 0.;
+  T bulk = 0, lap = 0;
   if (valid && m.av[cls]) {
-    bulk = m.av[cls][size_t(e) * 2 * NQ + q];
-    lap = m.av[cls][size_t(e) * 2 * NQ + NQ + q];
+    bulk = T(m.av[cls][size_t(e) * 2 * NQ + q]);
+    lap = T(m.av[cls][size_t(e) * 2 * NQ + NQ + q]);
   }
   double det = 1.;
-  double met[ND][ND];
+  T met[ND][ND];
   if constexpr (DEF) {
     const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
     for (int i = 0; i < ND; ++i)
       for (int j = 0; j < ND; ++j) {
-        met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
-        sn[(i * ND + j) * NQ + q] = met[i][j];
+        met[i][j] = valid ? T(mt[(i * ND + j) * NQ + q]) : T(i == j);
+        if (pt) my[L::SN + (i * ND + j) * NQ + q] = met[i][j];
       }
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
-  // LDG face states (and face normals) of every element in the block
-  for (int k = t; k < EPB * 2 * ND * NV * NFQ; k += S::BLOCK) {
-    const int el2 = k / (2 * ND * NV * NFQ), r = k % (2 * ND * NV * NFQ), f = r / (NV * NFQ), vq = r % (NV * NFQ);
-    const int e2 = blockIdx.x * EPB + el2;
-    lds[el2 * L::PER_ELEM + NV * NQ + r] = e2 < n ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[NV * NFQ + vq] : 0.;
-  }
+  load_faces<ND, RS, NT>(m, cls, 1, ld + L::SL, L::PER_D);
   if constexpr (DEF) {
-    for (int k = t; k < EPB * 2 * ND * ND * NFQ; k += S::BLOCK) {
+    for (int k = t; k < EPB * 2 * ND * ND * NFQ; k += NT) {
       const int el2 = k / (2 * ND * ND * NFQ), r = k % (2 * ND * ND * NFQ), f = r / (ND * NFQ), jq = r % (ND * NFQ);
       const int e2 = blockIdx.x * EPB + el2;
       double val = 0.;
@@ -259,129 +547,102 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_local_ns(Dev_mesh m,
         // a face in a Cartesian connection uses the identity normal (Spatial.hpp:331-339,364-367)
         val = ns_ >= 0 ? m.normals[size_t(ns_) * ND * NFQ + jq] : double(jq / NFQ == f / 2);
       }
-      lds[el2 * L::PER_ELEM + NV * NQ + 2 * ND * NV * NFQ + ND * ND * NQ + r] = val;
+      ld[el2 * L::PER_D + L::SFN + r] = val;
     }
   }
   __syncthreads();
-  // gradient times Jacobian determinant (Spatial.hpp:371-402), divided by nominal size
-  const double h = valid ? m.nom_size[cls][e] : 1.;
-  double grad[NV][ND];
-  for (int v = 0; v < NV; ++v)
-    for (int j = 0; j < ND; ++j) grad[v][j] = 0.;
-  for (int i = 0; i < ND; ++i) {
-    const int ki = R::node(q, i), b = R::base(q, i), sd = R::stride(i), fq = R::fq(q, i);
-    if constexpr (DEF) {
-      for (int j = 0; j < ND; ++j) {
-        double nk[RS];
-        for (int k = 0; k < RS; ++k) nk[k] = sn[(i * ND + j) * NQ + b + k * sd];
-        const double fn0 = sfn[((2 * i) * ND + j) * NFQ + fq], fn1 = sfn[((2 * i + 1) * ND + j) * NFQ + fq];
-        for (int v = 0; v < NV; ++v) {
-          double dq = 0, b0 = 0, b1 = 0;
-          for (int k = 0; k < RS; ++k) {
-            const double x = nk[k] * su[v * NQ + b + k * sd];
-            dq += ops.diff[ki][k] * x;
-            b0 += ops.bound[0][k] * x;
-            b1 += ops.bound[1][k] * x;
-          }
-          const double j0 = fn0 * sl[((2 * i) * NV + v) * NFQ + fq] - b0;
-          const double j1 = fn1 * sl[((2 * i + 1) * NV + v) * NFQ + fq] - b1;
-          grad[v][j] += (dq + (ops.lift[ki][0] * j0 + ops.lift[ki][1] * j1)) / h;
+  // gradient times Jacobian determinant, divided by the nominal size (Spatial.hpp:371-402), one physical
+  // component j at a time
+  const double inv_h = valid ? 1. / m.nom_size[cls][e] : 1.;
+  T grad[NV][ND];
+  for (int j = 0; j < ND; ++j) {
+    for (int i = DEF ? 0 : j; i < (DEF ? ND : j + 1); ++i) {
+      for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
+        T* b = lt + el2 * L::PER_T;
+        const double* db = ld + el2 * L::PER_D;
+        T x[RS], y[RS];
+        load_row(b + L::U + v * NQ + base, sd, x);
+        double b0 = db[L::SL + ((2 * i) * NV + v) * NFQ + r], b1 = db[L::SL + ((2 * i + 1) * NV + v) * NFQ + r];
+        if constexpr (DEF) {
+          T nk[RS];
+          load_row(b + L::SN + (i * ND + j) * NQ + base, sd, nk);
+#pragma unroll
+          for (int k = 0; k < RS; ++k) x[k] *= nk[k];
+          b0 *= db[L::SFN + ((2 * i) * ND + j) * NFQ + r];
+          b1 *= db[L::SFN + ((2 * i + 1) * ND + j) * NFQ + r];
         }
-      }
-    } else {
+        con.faces(x, b0, b1, y);
+        if (ok) {
+          T* out = b + L::G + v * NQ + base;
+#pragma unroll
+          for (int k = 0; k < RS; ++k) out[k * sd] = (i == 0 || !DEF) ? y[k] : out[k * sd] + y[k];
+        }
+      });
+      __syncthreads();
+    }
+    for (int v = 0; v < NV; ++v) grad[v][j] = my[L::G + v * NQ + q] * T(inv_h);
+    __syncthreads();
+  }
+  if constexpr (DEF) {
+    const T inv_det = T(1. / det);
+    for (int v = 0; v < NV; ++v)
+      for (int j = 0; j < ND; ++j) grad[v][j] *= inv_det;
+  }
+  // physical viscous flux (Spatial.hpp:420-435); its product with the normals is taken per direction below
+  T fvis[NV][ND];
+  flux_visc<ND>(ns, ut, bulk, lap, grad, fvis);
+  load_faces<ND, RS, NT>(m, cls, 0, ld + L::SB, L::PER_D);
+  for (int i = 0; i < ND; ++i) {
+    T nrm[ND], fc[NV];
+    for (int j = 0; j < ND; ++j) {
+      if constexpr (DEF) nrm[j] = met[i][j];
+      else nrm[j] = T(i == j);
+    }
+    flux_conv<ND>(ut, nrm, fc);
+    if (i) __syncthreads();
+    if (pt)
       for (int v = 0; v < NV; ++v) {
-        double dq = 0, b0 = 0, b1 = 0;
-        for (int k = 0; k < RS; ++k) {
-          const double x = su[v * NQ + b + k * sd];
-          dq += ops.diff[ki][k] * x;
-          b0 += ops.bound[0][k] * x;
-          b1 += ops.bound[1][k] * x;
+        T fv;
+        if constexpr (DEF) {
+          fv = 0;
+          for (int j = 0; j < ND; ++j) fv += fvis[v][j] * nrm[j];
+        } else {
+          fv = fvis[v][i];
         }
-        const double j0 = sl[((2 * i) * NV + v) * NFQ + fq] - b0;
-        const double j1 = sl[((2 * i + 1) * NV + v) * NFQ + fq] - b1;
-        grad[v][i] = (dq + (ops.lift[ki][0] * j0 + ops.lift[ki][1] * j1)) / h;
+        my[L::F + v * NQ + q] = fc[v];
+        my[L::F + (NV + v) * NQ + q] = fv;
       }
-    }
-  }
-  // volume fluxes (Spatial.hpp:404-435)
-  double fc[ND][NV];
-  double fv[NV][ND];
-  {
-    double nrm_t[ND][ND];  // normal(j, i) = met[i][j]
-    for (int i = 0; i < ND; ++i) {
-      double nrm[ND];
-      for (int j = 0; j < ND; ++j) {
-        if constexpr (DEF) nrm[j] = met[i][j];
-        else nrm[j] = (i == j);
-        nrm_t[j][i] = nrm[j];
+    __syncthreads();
+    // residual along direction i and the viscous flux extrapolated to the LDG blocks (Spatial.hpp:444-470)
+    for_columns<ND, RS, 2 * NV, NT, P::mfma>(i, [&](bool ok, int el2, int v2, int r, int base, int sd) {
+      T* b = lt + el2 * L::PER_T;
+      const double* fb = ld + el2 * L::PER_D + L::SB;
+      T x[RS], y[RS];
+      load_row(b + L::F + v2 * NQ + base, sd, x);
+      const bool conv = v2 < NV;
+      const int v = conv ? v2 : v2 - NV;
+      con.faces_if(conv, x, fb[((2 * i) * NV + v) * NFQ + r], fb[((2 * i + 1) * NV + v) * NFQ + r], y);
+      const int e2 = blockIdx.x * EPB + el2;
+      if (!conv && ok && e2 < n) {
+        const int* fs = m.face_slot[cls] + e2 * 2 * ND + 2 * i;
+        slot_ptr(m, fs[0])[(NV + v) * NFQ + r] = row_bound(o, 0, x);
+        slot_ptr(m, fs[1])[(NV + v) * NFQ + r] = row_bound(o, 1, x);
       }
-      double f[NV], p;
-      flux_conv<ND>(u, nrm, f, p);
-      for (int v = 0; v < NV; ++v) fc[i][v] = f[v];
-    }
-    if constexpr (DEF)
-      for (int v = 0; v < NV; ++v)
-        for (int j = 0; j < ND; ++j) grad[v][j] /= det;
-    flux_diff<ND, DEF>(ns, u, bulk, lap, grad, nrm_t, fv);
+      if (ok) sub_row(b + L::RT + v2 * NQ + base, sd, y, i == 0);
+    });
   }
-  __syncthreads();  // phase 1 data is dead from here on
-  double* sf = my;                                  // [ND][NV][NQ] convective flux
-  double* sv = sf + ND * NV * NQ;                   // [ND][NV][NQ] viscous flux
-  double* sb = sv + ND * NV * NQ;                   // [2ND][NV][NFQ] numerical face flux
-  for (int i = 0; i < ND; ++i)
-    for (int v = 0; v < NV; ++v) {
-      sf[(i * NV + v) * NQ + q] = fc[i][v];
-      sv[(i * NV + v) * NQ + q] = fv[v][i];
-    }
-  for (int k = t; k < EPB * 2 * ND * NV * NFQ; k += S::BLOCK) {
-    const int el2 = k / (2 * ND * NV * NFQ), r = k % (2 * ND * NV * NFQ), f = r / (NV * NFQ), vq = r % (NV * NFQ);
-    const int e2 = blockIdx.x * EPB + el2;
-    lds[el2 * L::PER_ELEM + 2 * ND * NV * NQ + r] = e2 < n ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[vq] : 0.;
-  }
-  __syncthreads();
-  // residual (Spatial.hpp:444-470)
-  double rate[2 * NV] = {};
-  for (int i = 0; i < ND; ++i) {
-    const int ki = R::node(q, i), b = R::base(q, i), sd = R::stride(i), fq = R::fq(q, i);
-    for (int v = 0; v < NV; ++v) {
-      double dq = 0, b0 = 0, b1 = 0, dv = 0;
-      for (int k = 0; k < RS; ++k) {
-        const double fk = sf[(i * NV + v) * NQ + b + k * sd];
-        dq += ops.diff[ki][k] * fk;
-        b0 += ops.bound[0][k] * fk;
-        b1 += ops.bound[1][k] * fk;
-        dv += ops.diff[ki][k] * sv[(i * NV + v) * NQ + b + k * sd];
-      }
-      const double j0 = sb[((2 * i) * NV + v) * NFQ + fq] - b0;
-      const double j1 = sb[((2 * i + 1) * NV + v) * NFQ + fq] - b1;
-      rate[v] -= dq + (ops.lift[ki][0] * j0 + ops.lift[ki][1] * j1);
-      rate[NV + v] -= dv;
-    }
-  }
-  // viscous flux extrapolated to the LDG face blocks (Spatial.hpp:460-466)
-  for (int k = t; k < EPB * S::NFACE_DOF; k += S::BLOCK) {
-    const int el2 = k / S::NFACE_DOF, r = k % S::NFACE_DOF;
-    const int f = r / (NV * NFQ), v = (r / NFQ) % NV, fq = r % NFQ;
-    const int e2 = blockIdx.x * EPB + el2;
-    if (e2 >= n) continue;
-    const int i = f / 2, s = f % 2;
-    const int b = R::base_of_fq(fq, i), sd = R::stride(i);
-    const double* row = lds + el2 * L::PER_ELEM + ND * NV * NQ + (i * NV + v) * NQ + b;
-    double acc = 0;
-    for (int kk = 0; kk < RS; ++kk) acc += ops.bound[s][kk] * row[kk * sd];
-    slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[NV * NFQ + v * NFQ + fq] = acc;
-  }
-  if (use_filter) filter_rows<ND, RS, 2 * NV>(ops, lds, el, q, rate);
+  if (use_filter) filter_columns<ND, RS, 2 * NV, NT>(o, lt + L::RT, L::PER_T);
+  else __syncthreads();
   // stage-0 update (Spatial.hpp:483-503): the convective part goes to the residual cache
   if (valid) {
-    double mult = coef * m.tss[cls][size_t(e) * NQ + q] / h;
+    double mult = coef * m.tss[cls][size_t(e) * NQ + q] * inv_h;
     if constexpr (DEF) mult /= det;
     double* rc = m.rcache[cls] + size_t(e) * NV * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
-      double r = rate[v];
+      double r = double(my[L::RT + v * NQ + q]);
       rc[v * NQ + q] = r;
-      r += rate[NV + v];
+      r += double(my[L::RT + (NV + v) * NQ + q]);
       r *= mult;
       if (compute_residual) rc[v * NQ + q] = r;
       else sw[v * NQ + q] = u[v] + r;
@@ -390,35 +651,48 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_local_ns(Dev_mesh m,
 }
 
 // ---- Reconcile_ldg_flux (Spatial.hpp:543-594) ------------------------------------------------------------
+template <int ND, int RS>
+struct Reconcile_lds
+{
+  using S = Shape<ND, RS>;
+  static constexpr int SL = 0;                                // [2ND][NV][NFQ] viscous flux jumps
+  static constexpr int U = SL + 2 * ND * S::NV * S::NFQ;      // [NV][NQ] rate, then the updated state
+  static constexpr int PER = U + S::NV * S::NQ;
+};
+
 template <int ND, int RS, bool DEF>
-__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_reconcile_ldg(Dev_mesh m, const Ops<RS>* __restrict__ gops,
-                                                                        double dt, int compute_residual, int use_filter)
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt,
+                                                                        int compute_residual, int use_filter)
 {
   using S = Shape<ND, RS>;
   using R = Row<ND, RS>;
+  using L = Reconcile_lds<ND, RS>;
   constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB;
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
-  __shared__ Ops<RS> ops;
-  __shared__ double su[EPB][NV][NQ];
+  __shared__ double lds[EPB * L::PER];
   const int t = threadIdx.x, el = t / NQ, q = t % NQ;
   const int n = m.n_elem[cls];
   const int e = blockIdx.x * EPB + el;
   const bool valid = e < n;
-  load_ops(ops, gops, t, S::BLOCK);
-  double rate[NV] = {};
-  if (valid) {
-    for (int i = 0; i < ND; ++i) {
-      const int ki = R::node(q, i), fq = R::fq(q, i);
-      const double* f0 = slot_ptr(m, m.face_slot[cls][e * 2 * ND + 2 * i]) + NV * NFQ;
-      const double* f1 = slot_ptr(m, m.face_slot[cls][e * 2 * ND + 2 * i + 1]) + NV * NFQ;
-      for (int v = 0; v < NV; ++v)
-        rate[v] -= ops.lift[ki][0] * f0[v * NFQ + fq] + ops.lift[ki][1] * f1[v * NFQ + fq];
-    }
-  }
-  if (use_filter) filter_rows<ND, RS, NV>(ops, &su[0][0][0], el, q, rate);
+  double* my = lds + el * L::PER;
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : 0.;
+  load_faces<ND, RS, S::BLOCK>(m, cls, 1, lds + L::SL, L::PER);
+  __syncthreads();
+  // -boundary_term(jump) (Spatial.hpp:560-570)
+  double rate[NV] = {};
+  for (int i = 0; i < ND; ++i) {
+    const int ki = R::node(q, i), fq = R::fq(q, i);
+    const double l0 = o.lift[ki][0], l1 = o.lift[ki][1];
+    for (int v = 0; v < NV; ++v)
+      rate[v] -= l0 * my[L::SL + ((2 * i) * NV + v) * NFQ + fq] + l1 * my[L::SL + ((2 * i + 1) * NV + v) * NFQ + fq];
+  }
+  if (use_filter) {
+    for (int v = 0; v < NV; ++v) my[L::U + v * NQ + q] = rate[v];
+    filter_columns<ND, RS, NV, S::BLOCK>(o, lds + L::U, L::PER);
+    for (int v = 0; v < NV; ++v) rate[v] = my[L::U + v * NQ + q];
+  }
   if (valid) {
     double mult = dt * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
     if constexpr (DEF) mult /= m.metrics[size_t(e) * (ND * ND + 1) * NQ + ND * ND * NQ + q];
@@ -433,10 +707,11 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_reconcile_ldg(Dev_me
       }
     }
   }
+  // Write_face of the updated state (Spatial.hpp:589)
   __syncthreads();
-  for (int v = 0; v < NV; ++v) su[el][v][q] = u[v];
+  for (int v = 0; v < NV; ++v) my[L::U + v * NQ + q] = u[v];
   __syncthreads();
-  extrapolate_faces<ND, RS>(m, cls, ops, &su[0][0][0], 0);
+  write_faces_from<ND, RS, S::BLOCK>(m, cls, o, lds + L::U, L::PER);
 }
 
 // ---- Write_face (Spatial.hpp:60-70) ----------------------------------------------------------------------
@@ -492,9 +767,9 @@ __global__ __launch_bounds__(256) void k_neighbor(Dev_mesh m)
       f1[NV * NFQ + v * NFQ + p1] = avg;
     }
   }
-  double F0[NV], F1[NV], p;
-  flux_conv<ND>(u0, n, F0, p);
-  flux_conv<ND>(u1, n, F1, p);
+  double F0[NV], F1[NV];
+  flux_conv<ND>(u0, n, F0);
+  flux_conv<ND>(u1, n, F1);
   double nsq = 0;
   for (int j = 0; j < ND; ++j) nsq += n[j] * n[j];
   const double c = fmax(char_speed<ND>(u0), char_speed<ND>(u1)) * sqrt(nsq);
@@ -534,18 +809,21 @@ __global__ __launch_bounds__(256) void k_neighbor_reconcile(Dev_mesh m)
   }
 }
 
-// ---- Max_dt (Spatial.hpp:784-828): per-point scale, wavefront + workgroup min, one atomic per workgroup ----
+// ---- Max_dt (Spatial.hpp:784-828): per-point scale; grid-stride loop, wavefront + workgroup min and one atomic
+// per workgroup, with the grid capped so that the atomics on the single result word stay few ----
+constexpr int max_dt_blocks = 2048;
+
 template <int ND, int RS, bool VISC>
 __global__ __launch_bounds__(256) void k_max_dt(Dev_mesh m, const Ops<RS>* __restrict__ gops, Ns_params ns, int cls,
                                                 int is_local, double max_cfl_c, double max_cfl_d, double* result)
 {
   constexpr int NQ = Shape<ND, RS>::NQ, NV = ND + 2, NVERT = 1 << ND;
   __shared__ double wmin[256 / 64];
-  const size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x;
-  const size_t e = gid / NQ;
-  const int q = int(gid % NQ);
+  const size_t n_items = size_t(m.n_elem[cls]) * NQ;
   double inv = DBL_MAX;
-  if (e < size_t(m.n_elem[cls])) {
+  for (size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x; gid < n_items; gid += size_t(gridDim.x) * 256) {
+    const size_t e = gid / NQ;
+    const int q = int(gid % NQ);
     double vs[NVERT];
     for (int i = 0; i < NVERT; ++i) vs[i] = m.vertex_tss[cls][e * NVERT + i];
     // math::interp (math.hpp:207-218)
@@ -572,11 +850,12 @@ __global__ __launch_bounds__(256) void k_max_dt(Dev_mesh m, const Ops<RS>* __res
       m.tss[cls][e * NQ + q] = 1. / scale;
     } else {
       m.tss[cls][e * NQ + q] = 1.;
-      inv = 1. / scale;
+      const double c = 1. / scale;
+      // NaN never wins, as with std::min(dt, NaN) in the reference
+      inv = (c < inv) ? c : inv;
     }
   }
   if (is_local) return;
-  // NaN never wins, as with std::min(dt, NaN) in the reference
   for (int off = 32; off > 0; off /= 2) {
     const double o = __shfl_xor(inv, off, 64);
     inv = (o < inv) ? o : inv;
@@ -760,6 +1039,7 @@ struct Launchers
 {
   using S = Shape<ND, RS>;
   static const Ops<RS>* ops(const Launch& l) { return static_cast<const Ops<RS>*>(l.ops); }
+  static const Ops<RS>& hops(const Launch& l) { return *static_cast<const Ops<RS>*>(l.hops); }
 
   static hipError_t write_face(const Launch& l, int cls)
   {
@@ -782,19 +1062,26 @@ struct Launchers
     }
     return hipGetLastError();
   }
-  static hipError_t local(const Launch& l, int cls, bool visc, double coef, int stage, int cr, int uf)
+  template <class P>
+  static hipError_t local_form(const Launch& l, int cls, bool visc, double coef, int stage, int cr, int uf)
   {
     const int n = l.m.n_elem[cls];
     if (n == 0) return hipSuccess;
-    dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
+    dim3 g(blocks_for(n, S::EPB)), b(form_block<ND, RS, P>());
+    const Eops<RS, double> o = make_eops<RS, double>(hops(l));
     if (visc) {
-      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_ns<ND, RS, true>), g, b, 0, l.stream, l.m, ops(l), l.ns, coef, cr, uf);
-      else hipLaunchKernelGGL((k_local_ns<ND, RS, false>), g, b, 0, l.stream, l.m, ops(l), l.ns, coef, cr, uf);
+      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_ns<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, l.ns, coef, cr, uf);
+      else hipLaunchKernelGGL((k_local_ns<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, l.ns, coef, cr, uf);
     } else {
-      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_euler<ND, RS, true>), g, b, 0, l.stream, l.m, ops(l), coef, stage, cr, uf);
-      else hipLaunchKernelGGL((k_local_euler<ND, RS, false>), g, b, 0, l.stream, l.m, ops(l), coef, stage, cr, uf);
+      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_euler<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, coef, stage, cr, uf);
+      else hipLaunchKernelGGL((k_local_euler<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, coef, stage, cr, uf);
     }
     return hipGetLastError();
+  }
+  static hipError_t local(const Launch& l, int cls, bool visc, double coef, int stage, int cr, int uf)
+  {
+    if (l.precision == HX_FP32_MFMA) return local_form<Fp32_mfma_form>(l, cls, visc, coef, stage, cr, uf);
+    return local_form<Fp64_form>(l, cls, visc, coef, stage, cr, uf);
   }
   static hipError_t neighbor_reconcile(const Launch& l, int cls)
   {
@@ -810,15 +1097,16 @@ struct Launchers
     const int n = l.m.n_elem[cls];
     if (n == 0) return hipSuccess;
     dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
-    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, true>), g, b, 0, l.stream, l.m, ops(l), dt, cr, uf);
-    else hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, false>), g, b, 0, l.stream, l.m, ops(l), dt, cr, uf);
+    const Eops<RS, double> o = make_eops<RS, double>(hops(l));
+    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, cr, uf);
+    else hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, cr, uf);
     return hipGetLastError();
   }
   static hipError_t max_dt(const Launch& l, int cls, bool visc, int is_local, double cc, double cd)
   {
     const int n = l.m.n_elem[cls];
     if (n == 0) return hipSuccess;
-    dim3 g(blocks_for(size_t(n) * S::NQ, 256)), b(256);
+    dim3 g(std::min(blocks_for(size_t(n) * S::NQ, 256), unsigned(max_dt_blocks))), b(256);
     if (visc) hipLaunchKernelGGL((k_max_dt<ND, RS, true>), g, b, 0, l.stream, l.m, ops(l), l.ns, cls, is_local, cc, cd, l.scalar);
     else hipLaunchKernelGGL((k_max_dt<ND, RS, false>), g, b, 0, l.stream, l.m, ops(l), l.ns, cls, is_local, cc, cd, l.scalar);
     return hipGetLastError();

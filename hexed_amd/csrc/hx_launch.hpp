@@ -13,6 +13,7 @@ struct Launch
   hipStream_t stream;
   Dev_mesh m;
   const void* ops;   // device Ops<row_size>
+  const void* hops;  // host copy of the same Ops<row_size>
   Ns_params ns;      // transport models (Navier-Stokes)
   double* scalar;    // device scratch: [0] time step reduction
   int precision;     // hx_precision

@@ -235,6 +235,10 @@ hx_status hx_mesh_destroy(hx_mesh* mesh);
 hx_status hx_mesh_validate(const hx_host_mesh* desc, char* msg, size_t capacity);
 hx_status hx_mesh_upload(hx_mesh* mesh, int field, int cls, const double* host, size_t n_doubles);
 hx_status hx_mesh_download(hx_mesh* mesh, int field, int cls, double* host, size_t n_doubles);
+/* Arithmetic form of the element kernels (Local) of this mesh; no counterpart in the reference, which is FP64 only.
+ * HX_FP64 (default): the reference's arithmetic. HX_FP32_MFMA: volume fluxes, gradients and the derivative
+ * contractions in FP32, the contractions on the matrix cores with FP32 accumulation; the state, RK update, residual
+ * cache, face fluxes (Neighbor), face jumps, LDG reconciliation, Jacobian determinant scaling and max_dt stay FP64. */
 hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision);
 /* Device address of a field (for callers that keep their own device views, e.g. torch tensors). */
 hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size_t* n_doubles);
