@@ -21,6 +21,7 @@ struct hx_ctx
   std::string err;
   void* ops[HX_MAX_ROW_SIZE + 1] = {};  // device Ops<rs> tables
   std::vector<char> host_ops[HX_MAX_ROW_SIZE + 1];  // host copies, for kernels taking constants by value
+  void* eops[HX_MAX_ROW_SIZE + 1] = {};             // device Eops<rs, double> tables
   double* scalar = nullptr;             // device scratch for reductions
   // per-category event timing
   struct Pending
@@ -106,6 +107,9 @@ hx_status upload_ops(hx_ctx* ctx, void** out)
   HX_TRY(ctx, hipMemcpy(*out, &o, sizeof(o), hipMemcpyHostToDevice), "hipMemcpy(ops)");
   const char* bytes = reinterpret_cast<const char*>(&o);
   ctx->host_ops[RS].assign(bytes, bytes + sizeof(o));
+  const Eops<RS, double> e = make_eops<RS, double>(o);
+  HX_TRY(ctx, hipMalloc(&ctx->eops[RS], sizeof(e)), "hipMalloc(eops)");
+  HX_TRY(ctx, hipMemcpy(ctx->eops[RS], &e, sizeof(e), hipMemcpyHostToDevice), "hipMemcpy(eops)");
   return HX_OK;
 }
 
@@ -253,6 +257,7 @@ Launch make_launch(hx_mesh* mesh)
   l.m = mesh->dm;
   l.ops = mesh->ctx->ops[mesh->dm.rs];
   l.hops = mesh->ctx->host_ops[mesh->dm.rs].data();
+  l.eops = mesh->ctx->eops[mesh->dm.rs];
   l.scalar = mesh->ctx->scalar;
   l.precision = mesh->precision;
   return l;
@@ -369,6 +374,8 @@ hx_status hx_ctx_destroy(hx_ctx* ctx)
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (void* p : ctx->ops)
+    if (p) (void)hipFree(p);
+  for (void* p : ctx->eops)
     if (p) (void)hipFree(p);
   for (auto& p : ctx->pending) {
     (void)hipEventDestroy(p.start);

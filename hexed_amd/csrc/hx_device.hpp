@@ -65,6 +65,33 @@ struct Ops
   double restrict_[2][RS][RS];
 };
 
+// Element-kernel operator constants (Derivative.hpp:14-61, Basis filter), in the arithmetic type T
+template <int RS, class T>
+struct Eops
+{
+  T diff[RS][RS];  // D
+  T lift[RS][2];   // L = W^{-1} B^T diag(-1, 1)
+  T bound[2][RS];  // B
+  T filt[RS][RS];  // modal filter
+};
+
+template <int RS, class T>
+Eops<RS, T> make_eops(const Ops<RS>& o)
+{
+  Eops<RS, T> e;
+  for (int i = 0; i < RS; ++i) {
+    for (int k = 0; k < RS; ++k) {
+      e.diff[i][k] = T(o.diff[i][k]);
+      e.filt[i][k] = T(o.filt[i][k]);
+    }
+    e.lift[i][0] = T(o.lift[i][0]);
+    e.lift[i][1] = T(o.lift[i][1]);
+    e.bound[0][i] = T(o.bound[0][i]);
+    e.bound[1][i] = T(o.bound[1][i]);
+  }
+  return e;
+}
+
 // Transport_model::coefficient (Transport_model.hpp:154-157); sqrt_ref_temp precomputed on the host
 struct Transport
 {

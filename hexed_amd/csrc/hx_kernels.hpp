@@ -22,6 +22,10 @@ struct Shape
   static constexpr int EPB = (256 / NQ) > 0 ? 256 / NQ : 1;
   static constexpr int BLOCK = EPB * NQ;
   static constexpr int NFACE_DOF = 2 * ND * NV * NFQ;  // face values per element
+  // LDS copies of per-point arrays carry one pad slot after every row along the last dimension, so that the
+  // columns of that dimension (rows rs apart) fall on different banks
+  static constexpr int NQP = NQ + NQ / RS;
+  static __device__ int pad(int q) { return q + q / RS; }
 };
 
 template <int RS>
@@ -88,32 +92,6 @@ __device__ inline void filter_rows(const Ops<RS>& ops, double* buf, int el, int 
 // row of outputs from the row's rs values and its two face values. The operator constants are a by-value kernel
 // argument: the fully unrolled contractions read them with uniform scalar loads, so one LDS read of the data feeds
 // rs multiply-adds.
-template <int RS, class T>
-struct Eops
-{
-  T diff[RS][RS];  // D
-  T lift[RS][2];   // L = W^{-1} B^T diag(-1, 1)
-  T bound[2][RS];  // B
-  T filt[RS][RS];  // modal filter
-};
-
-template <int RS, class T>
-Eops<RS, T> make_eops(const Ops<RS>& o)
-{
-  Eops<RS, T> e;
-  for (int i = 0; i < RS; ++i) {
-    for (int k = 0; k < RS; ++k) {
-      e.diff[i][k] = T(o.diff[i][k]);
-      e.filt[i][k] = T(o.filt[i][k]);
-    }
-    e.lift[i][0] = T(o.lift[i][0]);
-    e.lift[i][1] = T(o.lift[i][1]);
-    e.bound[0][i] = T(o.bound[0][i]);
-    e.bound[1][i] = T(o.bound[1][i]);
-  }
-  return e;
-}
-
 // Arithmetic forms of the element kernels. FP64: the reference's arithmetic. FP32-MFMA: volume fluxes, gradients
 // and derivative contractions in FP32, the contractions on the matrix cores with FP32 accumulation; the state, the
 // RK update and residual cache, the face fluxes and their jumps against the interior extrapolation, the Jacobian
@@ -139,8 +117,9 @@ constexpr int form_block()
 }
 
 // Calls f(ok, el, v, r, base, sd) for every column (el, v, r) of direction i_dim: v < NVAR, r the row counter (face
-// qpoint), base the row's first qpoint and sd its stride (Row_index.hpp:14-64). With UNIFORM, every thread takes part
-// in each round (ok is false past the last column and the indices are then those of column 0).
+// qpoint), base the row's first point and sd its stride in the padded LDS layout (Row_index.hpp:14-64). With
+// UNIFORM, every thread takes part in each round (ok is false past the last column and the indices are then those of
+// column 0).
 template <int ND, int RS, int NVAR, int NT, bool UNIFORM, class F>
 __device__ inline void for_columns(int i_dim, F&& f)
 {
@@ -148,18 +127,19 @@ __device__ inline void for_columns(int i_dim, F&& f)
   using R = Row<ND, RS>;
   constexpr int PER = NVAR * S::NFQ, TOTAL = S::EPB * PER;
   const int sd = R::stride(i_dim);
+  const int sdp = i_dim == ND - 1 ? 1 : sd + sd / RS;
   if constexpr (UNIFORM) {
     for (int c0 = 0; c0 < TOTAL; c0 += NT) {
       const int c = c0 + int(threadIdx.x);
       const bool ok = c < TOTAL;
       const int cc = ok ? c : 0;
       const int el = cc / PER, vr = cc % PER, v = vr / S::NFQ, r = vr % S::NFQ;
-      f(ok, el, v, r, R::base_of_fq(r, i_dim), sd);
+      f(ok, el, v, r, S::pad(R::base_of_fq(r, i_dim)), sdp);
     }
   } else {
     for (int c = threadIdx.x; c < TOTAL; c += NT) {
       const int el = c / PER, vr = c % PER, v = vr / S::NFQ, r = vr % S::NFQ;
-      f(true, el, v, r, R::base_of_fq(r, i_dim), sd);
+      f(true, el, v, r, S::pad(R::base_of_fq(r, i_dim)), sdp);
     }
   }
 }
@@ -203,114 +183,171 @@ __device__ inline double row_bound(const Eops<RS, double>& o, int s, const T (&x
 }
 
 // Row derivatives, Derivative.hpp:51-60: faces(x, b0, b1) = D x + L (b - B x), interior(x) = D x.
-template <int RS, class P>
+template <int RS, class P, bool RELOAD = false>
 struct Contraction;
 
 // FP64: the reference's operation order, D x + L (b - B x), on the vector units. (Folding the lift into
 // (D - L B) x + L b saves 2 rs multiply-adds per column but rounds the two large face terms separately, which
-// measurably loosens agreement with the reference arithmetic.)
-template <int RS>
-struct Contraction<RS, Fp64_form>
+// measurably loosens agreement with the reference arithmetic.) `bx` returns B x, the row's face extrapolation.
+// For even row sizes the Gauss-Legendre tables are exactly (anti)symmetric -- D[rs-1-i][k] = -D[i][rs-1-k],
+// B[1][k] = B[0][rs-1-k], L[i][1] = -L[rs-1-i][0], checked bit for bit by the CPU tests -- so the mirrored rows take
+// their coefficients from the first half with a sign modifier: the same products in the same order, from half as
+// many scalar registers.
+template <int RS, bool RELOAD>
+struct Contraction<RS, Fp64_form, RELOAD>
 {
-  const Eops<RS, double>& o;
-  __device__ explicit Contraction(const Eops<RS, double>& o_) : o(o_) {}
+  typedef const __attribute__((address_space(4))) Eops<RS, double>* cptr;
+  const Eops<RS, double>& o;   // by-value kernel argument
+  const Eops<RS, double>* g;   // device copy, read when RELOAD
+  __device__ Contraction(const Eops<RS, double>& o_, const Eops<RS, double>* dev) : o(o_), g(dev) {}
+  // The constants come either from the by-value kernel argument, which the compiler loads into scalar registers
+  // once, or with RELOAD (for kernels whose other kernel arguments already fill the scalar registers) through a
+  // constant-address-space pointer that the compiler cannot follow across the empty asm, one row (pair) at a time
+  // behind a scheduling barrier, so that each group is reloaded with scalar loads when it is needed.
+  template <class Get>
+  __device__ void run(Get&& get, bool with_faces, const double (&x)[RS], double b0, double b1, double (&y)[RS],
+                      double (&bx)[2]) const
+  {
+    {
+      const auto& c = get();
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) {
+          if constexpr (RS % 2 == 0) a += (s == 0 ? c.bound[0][k] : c.bound[0][RS - 1 - k]) * x[k];
+          else a += c.bound[s][k] * x[k];
+        }
+        bx[s] = a;
+      }
+    }
+    const double j0 = with_faces ? b0 - bx[0] : 0., j1 = with_faces ? b1 - bx[1] : 0.;
+    if constexpr (RS % 2 == 0) {
+#pragma unroll
+      for (int i = 0; i < RS / 2; ++i) {
+        if constexpr (RELOAD) __builtin_amdgcn_sched_barrier(0);
+        const auto& c = get();
+        const int im = RS - 1 - i;
+        double a = 0, am = 0;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) {
+          a += c.diff[i][k] * x[k];
+          am += -c.diff[i][RS - 1 - k] * x[k];
+        }
+        const double li = c.lift[i][0], lm = c.lift[im][0];
+        y[i] = with_faces ? a + (li * j0 + (-lm) * j1) : a;
+        y[im] = with_faces ? am + (lm * j0 + (-li) * j1) : am;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < RS; ++i) {
+        if constexpr (RELOAD) __builtin_amdgcn_sched_barrier(0);
+        const auto& c = get();
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) a += c.diff[i][k] * x[k];
+        y[i] = with_faces ? a + (c.lift[i][0] * j0 + c.lift[i][1] * j1) : a;
+      }
+    }
+    if constexpr (RELOAD) __builtin_amdgcn_sched_barrier(0);
+  }
+  // D x + L (b - B x) where `with_faces`, else D x
+  __device__ void faces_if(bool with_faces, const double (&x)[RS], double b0, double b1, double (&y)[RS],
+                           double (&bx)[2]) const
+  {
+    if constexpr (RELOAD) {
+      run([&]() -> const auto& {
+            cptr r = (cptr)g;
+            asm volatile("" : "+s"(r));
+            return *r;
+          },
+          with_faces, x, b0, b1, y, bx);
+    } else {
+      run([&]() -> const Eops<RS, double>& { return o; }, with_faces, x, b0, b1, y, bx);
+    }
+  }
   __device__ void faces(const double (&x)[RS], double b0, double b1, double (&y)[RS]) const
   {
-    const double j0 = b0 - row_bound(o, 0, x), j1 = b1 - row_bound(o, 1, x);
-#pragma unroll
-    for (int i = 0; i < RS; ++i) {
-      double a = 0;
-#pragma unroll
-      for (int k = 0; k < RS; ++k) a += o.diff[i][k] * x[k];
-      y[i] = a + (o.lift[i][0] * j0 + o.lift[i][1] * j1);
-    }
-  }
-  __device__ void interior(const double (&x)[RS], double (&y)[RS]) const
-  {
-#pragma unroll
-    for (int i = 0; i < RS; ++i) {
-      double a = 0;
-#pragma unroll
-      for (int k = 0; k < RS; ++k) a += o.diff[i][k] * x[k];
-      y[i] = a;
-    }
-  }
-  // faces() where `with_faces`, else interior()
-  __device__ void faces_if(bool with_faces, const double (&x)[RS], double b0, double b1, double (&y)[RS]) const
-  {
-    if (with_faces) faces(x, b0, b1, y);
-    else interior(x, y);
+    double bx[2];
+    faces_if(true, x, b0, b1, y, bx);
   }
 };
 
 // FP32-MFMA: V_MFMA_F32_4X4X1_16B_F32 computes 16 independent 4x4 outer products per wavefront, lane 4b + j holding
-// column j of block b. Each lane owns one column of data (its B operand) and the operator row 4t + (lane mod 4) of
-// [D | L] (its A operand), so after rs + 2 rank-1 steps lane 4b + j holds outputs 4t..4t+3 of its own row. The face
-// jumps b - B x are formed in FP64 before they enter the matrix cores.
+// column j of block b. Each lane owns one column of data (its B operand) and row 4t + (lane mod 4) of the stacked
+// operator [D; B] (its A operand), so after rs rank-1 steps lane 4b + j holds D x and B x of its own row. The face
+// jumps b - B x are formed in FP64 and lifted by two more rank-1 steps with the rows of L.
 typedef float hx_f4 __attribute__((ext_vector_type(4)));
 
-template <int RS>
-struct Contraction<RS, Fp32_mfma_form>
+template <int RS, bool RELOAD>
+struct Contraction<RS, Fp32_mfma_form, RELOAD>
 {
-  static constexpr int TILES = (RS + 3) / 4;
-  const Eops<RS, double>& o;
-  float a[TILES][RS + 2];
-  __device__ explicit Contraction(const Eops<RS, double>& o_) : o(o_)
+  static constexpr int TA = (RS + 2 + 3) / 4;  // tiles of the stacked [D; B] rows
+  static constexpr int TL = (RS + 3) / 4;      // tiles holding rows of D (the lifted ones)
+  float ax[TA][RS];
+  float aj[TL][2];
+  __device__ Contraction(const Eops<RS, double>& o, const Eops<RS, double>*)
   {
     const int lane_row = threadIdx.x & 3;
 #pragma unroll
-    for (int t = 0; t < TILES; ++t)
+    for (int t = 0; t < TA; ++t)
 #pragma unroll
-      for (int k = 0; k < RS + 2; ++k) {
+      for (int k = 0; k < RS; ++k) {
         float val = 0.f;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
           const int row = 4 * t + rr;
-          if (row < RS && lane_row == rr) val = float(k < RS ? o.diff[row][k] : o.lift[row][k - RS]);
+          if (lane_row == rr) {
+            if (row < RS) val = float(o.diff[row][k]);
+            else if (row < RS + 2) val = float(o.bound[row - RS][k]);
+          }
         }
-        a[t][k] = val;
+        ax[t][k] = val;
+      }
+#pragma unroll
+    for (int t = 0; t < TL; ++t)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        float val = 0.f;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr)
+          if (lane_row == rr && 4 * t + rr < RS) val = float(o.lift[4 * t + rr][s]);
+        aj[t][s] = val;
       }
   }
-  template <bool FACES>
-  __device__ void run(const float (&x)[RS], float j0, float j1, float (&y)[RS]) const
+  // D x + L (b - B x) where `with_faces`, else D x; without divergence around the matrix instructions (lanes
+  // without faces lift zero jumps)
+  __device__ void faces_if(bool with_faces, const float (&x)[RS], double b0, double b1, float (&y)[RS],
+                           float (&bx)[2]) const
   {
-    hx_f4 acc[TILES];
+    hx_f4 acc[TA];
 #pragma unroll
-    for (int t = 0; t < TILES; ++t) acc[t] = hx_f4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < TA; ++t) acc[t] = hx_f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < RS; ++k)
 #pragma unroll
-      for (int t = 0; t < TILES; ++t) acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][k], x[k], acc[t], 0, 0, 0);
-    if constexpr (FACES) {
+      for (int t = 0; t < TA; ++t) acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(ax[t][k], x[k], acc[t], 0, 0, 0);
+    bx[0] = acc[RS / 4][RS % 4];
+    bx[1] = acc[(RS + 1) / 4][(RS + 1) % 4];
+    const float j0 = with_faces ? float(b0 - double(bx[0])) : 0.f;
+    const float j1 = with_faces ? float(b1 - double(bx[1])) : 0.f;
 #pragma unroll
-      for (int t = 0; t < TILES; ++t) {
-        acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][RS], j0, acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t][RS + 1], j1, acc[t], 0, 0, 0);
-      }
+    for (int t = 0; t < TL; ++t) {
+      acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(aj[t][0], j0, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(aj[t][1], j1, acc[t], 0, 0, 0);
     }
 #pragma unroll
-    for (int t = 0; t < TILES; ++t)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (4 * t + r < RS) y[4 * t + r] = acc[t][r];
+    for (int i = 0; i < RS; ++i) y[i] = acc[i / 4][i % 4];
   }
   __device__ void faces(const float (&x)[RS], double b0, double b1, float (&y)[RS]) const
   {
-    run<true>(x, float(b0 - row_bound(o, 0, x)), float(b1 - row_bound(o, 1, x)), y);
-  }
-  __device__ void interior(const float (&x)[RS], float (&y)[RS]) const { run<false>(x, 0.f, 0.f, y); }
-  // faces() where `with_faces`, else interior(), without divergence around the matrix instructions: lanes without
-  // faces lift zero jumps
-  __device__ void faces_if(bool with_faces, const float (&x)[RS], double b0, double b1, float (&y)[RS]) const
-  {
-    const float j0 = with_faces ? float(b0 - row_bound(o, 0, x)) : 0.f;
-    const float j1 = with_faces ? float(b1 - row_bound(o, 1, x)) : 0.f;
-    run<true>(x, j0, j1, y);
+    float bx[2];
+    faces_if(true, x, b0, b1, y, bx);
   }
 };
 
-// Modal filter of NVAR [NQ] arrays per element at `buf` (element stride `per`), in place, one direction at a time
-// (Spatial.hpp:472-481). Ends synchronized.
+// Modal filter of NVAR padded [NQP] arrays per element at `buf` (element stride `per`), in place, one direction at a
+// time (Spatial.hpp:472-481). Ends synchronized.
 template <int ND, int RS, int NVAR, int NT, class T>
 __device__ inline void filter_columns(const Eops<RS, double>& o, T* buf, int per)
 {
@@ -321,7 +358,7 @@ __device__ inline void filter_columns(const Eops<RS, double>& o, T* buf, int per
   for (int i = 0; i < ND; ++i) {
     __syncthreads();
     for_columns<ND, RS, NVAR, NT, false>(i, [&](bool, int el, int v, int, int base, int sd) {
-      T* row = buf + el * per + v * S::NQ + base;
+      T* row = buf + el * per + v * S::NQP + base;
       T x[RS], y[RS];
       load_row(row, sd, x);
       row_matvec(filt, x, y);
@@ -347,7 +384,7 @@ __device__ inline void load_faces(const Dev_mesh& m, int cls, int ldg, double* d
   }
 }
 
-// Write_face of the FP64 [NV][NQ] arrays at `buf` (element stride `per`) into block 0 of every face
+// Write_face of the FP64 padded [NV][NQP] arrays at `buf` (element stride `per`) into block 0 of every face
 // (Spatial.hpp:29-71)
 template <int ND, int RS, int NT>
 __device__ inline void write_faces_from(const Dev_mesh& m, int cls, const Eops<RS, double>& o, const double* buf, int per)
@@ -359,7 +396,7 @@ __device__ inline void write_faces_from(const Dev_mesh& m, int cls, const Eops<R
       const int e = blockIdx.x * S::EPB + el;
       if (e >= n) return;
       double x[RS];
-      load_row(buf + el * per + v * S::NQ + base, sd, x);
+      load_row(buf + el * per + v * S::NQP + base, sd, x);
       const int* fs = m.face_slot[cls] + e * 2 * ND + 2 * i;
       slot_ptr(m, fs[0])[v * S::NFQ + r] = row_bound(o, 0, x);
       slot_ptr(m, fs[1])[v * S::NFQ + r] = row_bound(o, 1, x);
@@ -372,37 +409,37 @@ template <int ND, int RS, class P>
 struct Euler_lds
 {
   using S = Shape<ND, RS>;
-  using T = typename P::T;
-  // arrays of T
-  static constexpr int F = 0;                               // [NV][NQ] flux along the current direction
-  static constexpr int RT = F + S::NV * S::NQ;              // [NV][NQ] time rate
-  static constexpr int PER_T = RT + S::NV * S::NQ;
+  // arrays of T (padded)
+  static constexpr int F = 0;                               // [NV][NQP] flux along the current direction
+  static constexpr int RT = F + S::NV * S::NQP;             // [NV][NQP] time rate
+  static constexpr int PER_T = RT + S::NV * S::NQP;
   // arrays of double
   static constexpr int SB = 0;                              // [2ND][NV][NFQ] numerical face fluxes
-  static constexpr int U = 0;                               // [NV][NQ] updated state, for Write_face (after SB)
-  static constexpr int PER_D = 2 * ND * S::NFQ > S::NQ ? 2 * ND * S::NV * S::NFQ : S::NV * S::NQ;
+  static constexpr int U = 0;                               // [NV][NQP] updated state, for Write_face (after SB)
+  static constexpr int PER_D = 2 * ND * S::NFQ > S::NQP ? 2 * ND * S::NV * S::NFQ : S::NV * S::NQP;
 };
 
 template <int ND, int RS, bool DEF, class P>
 __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_mesh m, Eops<RS, double> o,
-                                                                           double coef, int stage,
+                                                                           const Eops<RS, double>* od, double coef,
+                                                                           int stage,
                                                                            int compute_residual, int use_filter)
 {
   using S = Shape<ND, RS>;
   using L = Euler_lds<ND, RS, P>;
   using T = typename P::T;
-  constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = form_block<ND, RS, P>();
+  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = form_block<ND, RS, P>();
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
   __shared__ T lt[EPB * L::PER_T];
   __shared__ double ld[EPB * L::PER_D];
   const int t = threadIdx.x;
   const bool pt = t < EPB * NQ;
-  const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0;
+  const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0, qp = S::pad(q);
   const int n = m.n_elem[cls];
   const int e = blockIdx.x * EPB + el;
   const bool valid = pt && e < n;
   T* my = lt + el * L::PER_T;
-  const Contraction<RS, P> con(o);
+  const Contraction<RS, P> con(o, od);
 
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
@@ -428,16 +465,16 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
     flux_conv<ND>(ut, nrm, f);
     if (i) __syncthreads();
     if (pt)
-      for (int v = 0; v < NV; ++v) my[L::F + v * NQ + q] = f[v];
+      for (int v = 0; v < NV; ++v) my[L::F + v * NQP + qp] = f[v];
     __syncthreads();
     // -derivative(row_f, face_f) (Spatial.hpp:444-457)
     for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
       T* b = lt + el2 * L::PER_T;
       const double* fb = ld + el2 * L::PER_D + L::SB;
       T x[RS], y[RS];
-      load_row(b + L::F + v * NQ + base, sd, x);
+      load_row(b + L::F + v * NQP + base, sd, x);
       con.faces(x, fb[((2 * i) * NV + v) * NFQ + r], fb[((2 * i + 1) * NV + v) * NFQ + r], y);
-      if (ok) sub_row(b + L::RT + v * NQ + base, sd, y, i == 0);
+      if (ok) sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
     });
   }
   if (use_filter) filter_columns<ND, RS, NV, NT>(o, lt + L::RT, L::PER_T);
@@ -449,7 +486,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
     double* rc = m.rcache[cls] + size_t(e) * NV * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
-      double r = double(my[L::RT + v * NQ + q]);
+      double r = double(my[L::RT + v * NQP + qp]);
       if (stage) r -= rc[v * NQ + q];
       else rc[v * NQ + q] = r;
       r *= mult;
@@ -461,7 +498,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
   }
   // Write_face of the updated state (Spatial.hpp:507)
   if (pt)
-    for (int v = 0; v < NV; ++v) ld[el * L::PER_D + L::U + v * NQ + q] = u[v];
+    for (int v = 0; v < NV; ++v) ld[el * L::PER_D + L::U + v * NQP + qp] = u[v];
   __syncthreads();
   write_faces_from<ND, RS, NT>(m, cls, o, ld + L::U, L::PER_D);
 }
@@ -471,16 +508,16 @@ template <int ND, int RS, bool DEF, class P>
 struct Ns_lds
 {
   using S = Shape<ND, RS>;
-  static constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV;
-  // arrays of T, gradient phase
-  static constexpr int U = 0;                                     // [NV][NQ] state
-  static constexpr int G = U + NV * NQ;                           // [NV][NQ] one gradient component
-  static constexpr int SN = G + NV * NQ;                          // [ND][ND][NQ] reference-level normals
-  static constexpr int P1_T = SN + (DEF ? ND * ND * NQ : 0);
-  // arrays of T, residual phase
-  static constexpr int F = 0;                                     // [2][NV][NQ] convective, viscous flux
-  static constexpr int RT = F + 2 * NV * NQ;                      // [2][NV][NQ] convective, viscous rate
-  static constexpr int P2_T = RT + 2 * NV * NQ;
+  static constexpr int NQP = S::NQP, NFQ = S::NFQ, NV = S::NV;
+  // arrays of T (padded), gradient phase
+  static constexpr int U = 0;                                     // [NV][NQP] state
+  static constexpr int G = U + NV * NQP;                          // [NV][NQP] one gradient component
+  static constexpr int SN = G + NV * NQP;                         // [ND][ND][NQP] reference-level normals
+  static constexpr int P1_T = SN + (DEF ? ND * ND * NQP : 0);
+  // arrays of T (padded), residual phase
+  static constexpr int F = 0;                                     // [NV][NQP] convective, then viscous flux
+  static constexpr int RT = F + NV * NQP;                         // [2][NV][NQP] convective, viscous rate
+  static constexpr int P2_T = RT + 2 * NV * NQP;
   static constexpr int PER_T = P1_T > P2_T ? P1_T : P2_T;
   // arrays of double
   static constexpr int SL = 0;                                    // [2ND][NV][NFQ] LDG face state
@@ -492,25 +529,37 @@ struct Ns_lds
 };
 
 template <int ND, int RS, bool DEF, class P>
-__global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh m, Eops<RS, double> o, Ns_params ns,
+__global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh m, Eops<RS, double> o,
+                                                                        const Eops<RS, double>* od, Ns_params ns,
                                                                         double coef, int compute_residual,
                                                                         int use_filter)
 {
   using S = Shape<ND, RS>;
   using L = Ns_lds<ND, RS, DEF, P>;
   using T = typename P::T;
-  constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = form_block<ND, RS, P>();
+  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = form_block<ND, RS, P>();
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
   __shared__ T lt[EPB * L::PER_T];
   __shared__ double ld[EPB * L::PER_D];
   const int t = threadIdx.x;
   const bool pt = t < EPB * NQ;
-  const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0;
+  const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0, qp = S::pad(q);
   const int n = m.n_elem[cls];
   const int e = blockIdx.x * EPB + el;
   const bool valid = pt && e < n;
   T* my = lt + el * L::PER_T;
-  const Contraction<RS, P> con(o);
+  const Contraction<RS, P, true> con(o, od);
+
+  // the numerical face fluxes are needed only after the gradient; their loads are issued now and held in
+  // registers so that their latency hides behind the gradient phase
+  constexpr int PF = 2 * ND * NV * NFQ, NSB = (EPB * PF + NT - 1) / NT;
+  double sb[NSB];
+#pragma unroll
+  for (int k = 0; k < NSB; ++k) {
+    const int kk = t + k * NT, el2 = kk / PF, r = kk % PF, f = r / (NV * NFQ), vq = r % (NV * NFQ);
+    const int e2 = blockIdx.x * EPB + el2;
+    sb[k] = (kk < EPB * PF && e2 < n) ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[vq] : 0.;
+  }
 
   double u[NV];
   T ut[NV];
@@ -518,7 +567,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
   for (int v = 0; v < NV; ++v) {
     u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
     ut[v] = T(u[v]);
-    if (pt) my[L::U + v * NQ + q] = ut[v];
+    if (pt) my[L::U + v * NQP + qp] = ut[v];
   }
   T bulk = 0, lap = 0;
   if (valid && m.av[cls]) {
@@ -532,7 +581,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
     for (int i = 0; i < ND; ++i)
       for (int j = 0; j < ND; ++j) {
         met[i][j] = valid ? T(mt[(i * ND + j) * NQ + q]) : T(i == j);
-        if (pt) my[L::SN + (i * ND + j) * NQ + q] = met[i][j];
+        if (pt) my[L::SN + (i * ND + j) * NQP + qp] = met[i][j];
       }
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
@@ -561,11 +610,11 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
         T* b = lt + el2 * L::PER_T;
         const double* db = ld + el2 * L::PER_D;
         T x[RS], y[RS];
-        load_row(b + L::U + v * NQ + base, sd, x);
+        load_row(b + L::U + v * NQP + base, sd, x);
         double b0 = db[L::SL + ((2 * i) * NV + v) * NFQ + r], b1 = db[L::SL + ((2 * i + 1) * NV + v) * NFQ + r];
         if constexpr (DEF) {
           T nk[RS];
-          load_row(b + L::SN + (i * ND + j) * NQ + base, sd, nk);
+          load_row(b + L::SN + (i * ND + j) * NQP + base, sd, nk);
 #pragma unroll
           for (int k = 0; k < RS; ++k) x[k] *= nk[k];
           b0 *= db[L::SFN + ((2 * i) * ND + j) * NFQ + r];
@@ -573,14 +622,14 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
         }
         con.faces(x, b0, b1, y);
         if (ok) {
-          T* out = b + L::G + v * NQ + base;
+          T* out = b + L::G + v * NQP + base;
 #pragma unroll
           for (int k = 0; k < RS; ++k) out[k * sd] = (i == 0 || !DEF) ? y[k] : out[k * sd] + y[k];
         }
       });
       __syncthreads();
     }
-    for (int v = 0; v < NV; ++v) grad[v][j] = my[L::G + v * NQ + q] * T(inv_h);
+    for (int v = 0; v < NV; ++v) grad[v][j] = my[L::G + v * NQP + qp] * T(inv_h);
     __syncthreads();
   }
   if constexpr (DEF) {
@@ -591,7 +640,14 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
   // physical viscous flux (Spatial.hpp:420-435); its product with the normals is taken per direction below
   T fvis[NV][ND];
   flux_visc<ND>(ns, ut, bulk, lap, grad, fvis);
-  load_faces<ND, RS, NT>(m, cls, 0, ld + L::SB, L::PER_D);
+#pragma unroll
+  for (int k = 0; k < NSB; ++k)
+    if (t + k * NT < EPB * PF) {
+      const int kk = t + k * NT;
+      ld[(kk / PF) * L::PER_D + L::SB + kk % PF] = sb[k];
+    }
+  // residual along each direction (Spatial.hpp:444-470): the convective flux with its numerical face fluxes, then
+  // the viscous flux, which is also extrapolated to the LDG blocks of the faces
   for (int i = 0; i < ND; ++i) {
     T nrm[ND], fc[NV];
     for (int j = 0; j < ND; ++j) {
@@ -601,6 +657,18 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
     flux_conv<ND>(ut, nrm, fc);
     if (i) __syncthreads();
     if (pt)
+      for (int v = 0; v < NV; ++v) my[L::F + v * NQP + qp] = fc[v];
+    __syncthreads();
+    for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
+      T* b = lt + el2 * L::PER_T;
+      const double* fb = ld + el2 * L::PER_D + L::SB;
+      T x[RS], y[RS];
+      load_row(b + L::F + v * NQP + base, sd, x);
+      con.faces(x, fb[((2 * i) * NV + v) * NFQ + r], fb[((2 * i + 1) * NV + v) * NFQ + r], y);
+      if (ok) sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
+    });
+    __syncthreads();
+    if (pt)
       for (int v = 0; v < NV; ++v) {
         T fv;
         if constexpr (DEF) {
@@ -609,26 +677,21 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
         } else {
           fv = fvis[v][i];
         }
-        my[L::F + v * NQ + q] = fc[v];
-        my[L::F + (NV + v) * NQ + q] = fv;
+        my[L::F + v * NQP + qp] = fv;
       }
     __syncthreads();
-    // residual along direction i and the viscous flux extrapolated to the LDG blocks (Spatial.hpp:444-470)
-    for_columns<ND, RS, 2 * NV, NT, P::mfma>(i, [&](bool ok, int el2, int v2, int r, int base, int sd) {
+    for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
       T* b = lt + el2 * L::PER_T;
-      const double* fb = ld + el2 * L::PER_D + L::SB;
-      T x[RS], y[RS];
-      load_row(b + L::F + v2 * NQ + base, sd, x);
-      const bool conv = v2 < NV;
-      const int v = conv ? v2 : v2 - NV;
-      con.faces_if(conv, x, fb[((2 * i) * NV + v) * NFQ + r], fb[((2 * i + 1) * NV + v) * NFQ + r], y);
+      T x[RS], y[RS], bx[2];
+      load_row(b + L::F + v * NQP + base, sd, x);
+      con.faces_if(false, x, 0., 0., y, bx);
       const int e2 = blockIdx.x * EPB + el2;
-      if (!conv && ok && e2 < n) {
+      if (ok && e2 < n) {
         const int* fs = m.face_slot[cls] + e2 * 2 * ND + 2 * i;
-        slot_ptr(m, fs[0])[(NV + v) * NFQ + r] = row_bound(o, 0, x);
-        slot_ptr(m, fs[1])[(NV + v) * NFQ + r] = row_bound(o, 1, x);
+        slot_ptr(m, fs[0])[(NV + v) * NFQ + r] = double(bx[0]);
+        slot_ptr(m, fs[1])[(NV + v) * NFQ + r] = double(bx[1]);
       }
-      if (ok) sub_row(b + L::RT + v2 * NQ + base, sd, y, i == 0);
+      if (ok) sub_row(b + L::RT + (NV + v) * NQP + base, sd, y, i == 0);
     });
   }
   if (use_filter) filter_columns<ND, RS, 2 * NV, NT>(o, lt + L::RT, L::PER_T);
@@ -640,9 +703,9 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
     double* rc = m.rcache[cls] + size_t(e) * NV * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
-      double r = double(my[L::RT + v * NQ + q]);
+      double r = double(my[L::RT + v * NQP + qp]);
       rc[v * NQ + q] = r;
-      r += double(my[L::RT + (NV + v) * NQ + q]);
+      r += double(my[L::RT + (NV + v) * NQP + qp]);
       r *= mult;
       if (compute_residual) rc[v * NQ + q] = r;
       else sw[v * NQ + q] = u[v] + r;
@@ -656,8 +719,8 @@ struct Reconcile_lds
 {
   using S = Shape<ND, RS>;
   static constexpr int SL = 0;                                // [2ND][NV][NFQ] viscous flux jumps
-  static constexpr int U = SL + 2 * ND * S::NV * S::NFQ;      // [NV][NQ] rate, then the updated state
-  static constexpr int PER = U + S::NV * S::NQ;
+  static constexpr int U = SL + 2 * ND * S::NV * S::NFQ;      // [NV][NQP] rate, then the updated state
+  static constexpr int PER = U + S::NV * S::NQP;
 };
 
 template <int ND, int RS, bool DEF>
@@ -689,9 +752,9 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_reconcile_ldg(Dev_me
       rate[v] -= l0 * my[L::SL + ((2 * i) * NV + v) * NFQ + fq] + l1 * my[L::SL + ((2 * i + 1) * NV + v) * NFQ + fq];
   }
   if (use_filter) {
-    for (int v = 0; v < NV; ++v) my[L::U + v * NQ + q] = rate[v];
+    for (int v = 0; v < NV; ++v) my[L::U + v * S::NQP + S::pad(q)] = rate[v];
     filter_columns<ND, RS, NV, S::BLOCK>(o, lds + L::U, L::PER);
-    for (int v = 0; v < NV; ++v) rate[v] = my[L::U + v * NQ + q];
+    for (int v = 0; v < NV; ++v) rate[v] = my[L::U + v * S::NQP + S::pad(q)];
   }
   if (valid) {
     double mult = dt * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
@@ -709,7 +772,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_reconcile_ldg(Dev_me
   }
   // Write_face of the updated state (Spatial.hpp:589)
   __syncthreads();
-  for (int v = 0; v < NV; ++v) my[L::U + v * NQ + q] = u[v];
+  for (int v = 0; v < NV; ++v) my[L::U + v * S::NQP + S::pad(q)] = u[v];
   __syncthreads();
   write_faces_from<ND, RS, S::BLOCK>(m, cls, o, lds + L::U, L::PER);
 }
@@ -1069,12 +1132,13 @@ struct Launchers
     if (n == 0) return hipSuccess;
     dim3 g(blocks_for(n, S::EPB)), b(form_block<ND, RS, P>());
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
+    const Eops<RS, double>* od = static_cast<const Eops<RS, double>*>(l.eops);
     if (visc) {
-      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_ns<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, l.ns, coef, cr, uf);
-      else hipLaunchKernelGGL((k_local_ns<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, l.ns, coef, cr, uf);
+      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_ns<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, cr, uf);
+      else hipLaunchKernelGGL((k_local_ns<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, cr, uf);
     } else {
-      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_euler<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, coef, stage, cr, uf);
-      else hipLaunchKernelGGL((k_local_euler<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, coef, stage, cr, uf);
+      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_euler<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, od, coef, stage, cr, uf);
+      else hipLaunchKernelGGL((k_local_euler<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, od, coef, stage, cr, uf);
     }
     return hipGetLastError();
   }

@@ -14,6 +14,7 @@ struct Launch
   Dev_mesh m;
   const void* ops;   // device Ops<row_size>
   const void* hops;  // host copy of the same Ops<row_size>
+  const void* eops;  // device Eops<row_size, double>
   Ns_params ns;      // transport models (Navier-Stokes)
   double* scalar;    // device scratch: [0] time step reduction
   int precision;     // hx_precision
