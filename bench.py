@@ -19,7 +19,7 @@ The JSON line also carries
                 host's cores for a few steps of the same mesh;
   parity_vs_cpu: relative L2 error per conserved variable between the FP64 GPU form and the oracle after those steps.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32] [--precision fp32_mfma|fp64]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32|amr16] [--precision fp32_mfma|fp64]
                        [--cpu-steps N] [--err-steps N]
 """
 import argparse
@@ -46,6 +46,12 @@ WORKLOADS = {
              "Re 1e4 on unit length, Mach 0.3, Sutherland air, constant bulk artificial viscosity 1e-3 m^2/s, "
              "1% seeded Fourier perturbations, freestream boundaries, global time step at max_safety 0.7",
         build=lambda: configs.ns_deformed(48), visc=True, cpu_steps=6, bytes_per_dof_stage=224),
+    "amr16": dict(
+        desc="3D Euler on a seeded 2:1-balanced tree: base 16^3 Cartesian hexes, refinement probability 0.085 per "
+             "level up to 3 levels (19,461 leaves, 31% of the base refined, 7,691 hanging faces through "
+             "Prolong/Restrict_refined), row_size 6, Mach 0.8 + 1% seeded Fourier perturbations, freestream "
+             "boundaries, global time step at max_safety 0.7",
+        build=lambda: configs.euler_tree(), visc=False, cpu_steps=6, bytes_per_dof_stage=128),
     "euler32": dict(
         desc="3D Euler, 32^3 Cartesian hexes, row_size 6, Mach 0.8 + 1% seeded Fourier perturbations, "
              "freestream boundaries, global time step at max_safety 0.7",
@@ -70,11 +76,12 @@ def algorithmic_bytes(category, nd, nq, n_elem):
     return 8.0 * doubles * nq * n_elem
 
 
-FORMS = {"fp64": (abi.HX_FP64, "hx::Fp64_form"), "fp32_mfma": (abi.HX_FP32_MFMA, "hx::Fp32_mfma_form")}
+FORMS = {"fp64": (abi.HX_FP64, "Fp64_form"), "fp32_mfma": (abi.HX_FP32_MFMA, "Fp32_mfma_form")}
 
 
 def kernel_symbol(category, nd, rs, form):
-    """The HIP kernel a timing category launches (the name rocprofv3 reports), for local categories."""
+    """The HIP kernel a timing category launches (the name rocprofv3 reports, without the namespace), for local
+    categories."""
     cls, kernel, pde = category.split("/")
     if kernel != "local":
         return None
@@ -82,13 +89,13 @@ def kernel_symbol(category, nd, rs, form):
     return f"{name}<{nd}, {rs}, {'true' if cls == 'deformed' else 'false'}, {FORMS[form][1]}>"
 
 
-def measured_traffic(config, precision, symbol):
+def measured_traffic(config, symbol):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, if one covers the kernel."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if symbol is None or not os.path.exists(path):
         return None
     with open(path) as f:
-        table = json.load(f).get(f"{config}_{precision}", {})
+        table = json.load(f).get(config, {})
     return table.get("bytes_per_launch", {}).get(symbol)
 
 
@@ -104,14 +111,15 @@ def run_oracle_steps(m, opts, steps):
 
 
 def reset(dev, m):
-    """Initial state on the device: state uploaded, residual cache and time step scale cleared, faces extrapolated
-    (Solver::initialize)."""
+    """Initial state on the device: state uploaded, residual cache and time step scale cleared, faces extrapolated and
+    prolonged to the hanging-face mortars (Solver::initialize, Solver.cpp:400-401)."""
     for c in (0, 1):
         if m.n_elem[c]:
             dev.upload("state", c, m.state[c])
             dev.upload("rcache", c, np.zeros_like(m.rcache[c]))
             dev.upload("tss", c, np.ones_like(m.tss[c]))
     dev.compute_write_face()
+    dev.prolong()
 
 
 def download_state(dev, m):
@@ -211,7 +219,7 @@ def main():
     alg = algorithmic_bytes(dominant, nd, nq, m.n_elem[cls])
     symbol = kernel_symbol(dominant, nd, m.row_size, args.precision)
     roofline = {"bound": "hbm", "kernel": dominant, "symbol": symbol, "launch_ms": round(avg_ms, 5),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": measured_traffic(args.config, args.precision, symbol),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": measured_traffic(args.config, symbol),
                 "stream_copy_gbs": round(copy_gbs, 1)}
     if alg is not None:
         achieved = alg / (avg_ms * 1e-3) / 1e9
@@ -234,6 +242,7 @@ def main():
         s = m.struct()
         for c in (0, 1):
             oracle.check(lib.oracle_write_face(ctypes.byref(s), c, None))
+        oracle.check(lib.oracle_prolong_refined(ctypes.byref(s), None, 0, 0))
         cpu_s, threads = run_oracle_steps(m, opts, cpu_steps)
         cpu = {"value": round(n_dof * 2 * cpu_steps / cpu_s, 1), "unit": "DOF-stage/s", "cores": threads,
                "kind": "port",

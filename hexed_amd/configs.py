@@ -148,3 +148,107 @@ def isentropic_vortex(n=16, rs=6, mach=0.5, strength=5.0, half_width=5.0):
         return [rho * vx, rho * vy, rho, pr / (GAMMA - 1) + 0.5 * rho * (vx * vx + vy * vy)]
     mesh.set_state(m, lambda pos: exact(pos, 0.0))
     return m, exact
+
+
+def tree_leaves(base=16, levels=3, frac=0.085, seed=0, nd=3):
+    """Leaves of a seeded, 2:1 face-balanced refinement tree over base^nd level-0 cells: at each level below
+    `levels`, every leaf of that level is refined with probability `frac` (a splitmix64 uniform per leaf in a fixed
+    order), then leaves are split until no leaf has a face neighbour more than one level finer. With the defaults
+    (16^3, 3 levels, 0.085, seed 0) the balancing leaves 31% of the base elements refined: 19,461 leaves over levels
+    0-3 (2835, 9355, 5663, 1608) and 7,691 hanging faces. Returns a dict {(level, integer position tuple): True}."""
+    leaves = {(0, p): True for p in np.ndindex(*([base] * nd))}
+
+    def split(key):
+        lvl, p = key
+        del leaves[key]
+        for c in np.ndindex(*([2] * nd)):
+            leaves[(lvl + 1, tuple(2 * x + y for x, y in zip(p, c)))] = True
+
+    for lvl in range(levels):
+        cur = sorted(k for k in leaves if k[0] == lvl)
+        u = splitmix64(seed * 1009 + lvl, len(cur)) if cur else []
+        for k, r in zip(cur, u):
+            if r < frac:
+                split(k)
+    # 2:1 balance across faces: a leaf at level L needs every face neighbour covered by leaves of level >= L - 1
+    changed = True
+    while changed:
+        changed = False
+        for (lvl, p) in sorted(leaves, key=lambda k: -k[0]):
+            if (lvl, p) not in leaves or lvl < 2:
+                continue
+            for d in range(nd):
+                for s in (-1, 1):
+                    q = list(p)
+                    q[d] += s
+                    if not 0 <= q[d] < base * 2 ** lvl:
+                        continue
+                    # the leaf covering the same-level neighbour cell, if it is coarser
+                    for up in range(1, lvl + 1):
+                        anc = (lvl - up, tuple(x >> up for x in q))
+                        if anc in leaves:
+                            if up >= 2:
+                                split(anc)
+                                changed = True
+                            break
+    return leaves
+
+
+def tree_mesh(base=16, rs=6, levels=3, frac=0.085, seed=0, nd=3, bc=None):
+    """Cartesian tree mesh on the unit cube from `tree_leaves`: same-level neighbours get Cartesian connections,
+    a leaf whose face neighbour is refined gets a hanging-face (refined) connection to the 2^(nd-1) finer leaves
+    on that face (connection.hpp:168-286), domain faces get boundary connections. Root size 1/base."""
+    leaves = tree_leaves(base, levels, frac, seed, nd)
+    b = mesh.Builder(nd, rs, root_size=1.0 / base)
+    keys = sorted(leaves, key=lambda k: (k[0], k[1]))
+    index = {}
+    for k in keys:
+        index[k] = b.add_element(k[0], False, list(k[1]))
+    if bc is None:
+        bc = (abi.HX_BC_FREESTREAM, freestream_state(nd, 1.0, [0.8] + [0.0] * (nd - 1), 1.0 / GAMMA))
+    bc_id = b.add_bc(*bc)
+    for (lvl, p) in keys:
+        e = index[(lvl, p)]
+        n_cells = base * 2 ** lvl
+        for d in range(nd):
+            for s in (0, 1):
+                q = list(p)
+                q[d] += 1 if s else -1
+                if not 0 <= q[d] < n_cells:
+                    b.connect_boundary(e, d, s, bc_id)
+                    continue
+                q = tuple(q)
+                if (lvl, q) in leaves:
+                    if s:
+                        b.connect_cartesian(e, index[(lvl, q)], d)
+                    continue
+                # neighbour refined: its children on the shared face, row-major over the tangential dimensions
+                kids = []
+                tang = [t for t in range(nd) if t != d]
+                for c in np.ndindex(*([2] * (nd - 1))):
+                    child = [2 * x for x in q]
+                    child[d] += 0 if s else 1
+                    for t, ct in zip(tang, c):
+                        child[t] += ct
+                    kids.append((lvl + 1, tuple(child)))
+                if all(k in leaves for k in kids):
+                    b.connect_refined(e, [index[k] for k in kids], d, reverse=not s)
+                # otherwise the neighbour is coarser and owns the connection
+    m = b.build()
+    m.tree_counts = {lvl: sum(1 for k in keys if k[0] == lvl) for lvl in range(levels + 1)}
+    return m
+
+
+def euler_tree(base=16, rs=6, levels=3, frac=0.085, seed=0, mach=0.8, nd=3):
+    """Config 4: 3D Euler on a seeded 2:1-balanced tree (hanging faces through Prolong/Restrict_refined), freestream
+    rho=1, p=1/gamma, velocity Mach*x with 1% seeded Fourier perturbations, freestream boundaries."""
+    fs = freestream_state(nd, 1.0, [mach] + [0.0] * (nd - 1), 1.0 / GAMMA)
+    m = tree_mesh(base, rs, levels, frac, seed, nd, bc=(abi.HX_BC_FREESTREAM, fs))
+
+    def init(pos):
+        rho = 1.0 * (1 + fourier_perturbation(pos, seed * 16 + 1))
+        vel = [(mach if d == 0 else 0.0) + mach * fourier_perturbation(pos, seed * 16 + 2 + d) for d in range(nd)]
+        p = 1.0 / GAMMA
+        return [rho * v for v in vel] + [rho, p / (GAMMA - 1) + 0.5 * rho * sum(v * v for v in vel)]
+    mesh.set_state(m, init)
+    return m

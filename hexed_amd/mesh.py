@@ -398,6 +398,7 @@ class Builder:
         self.car_cons = []   # (a, b, i_dim)
         self.def_cons = []   # (a, b, i_dims, face_signs)
         self.bnd_cons = []   # (elem, i_dim, sign, bc)
+        self.ref_cons = []   # (coarse, [fine...], i_dim, reverse)
         self.bcs = []
 
     def add_element(self, ref_level, deformed, pos):
@@ -420,6 +421,19 @@ class Builder:
     def connect_boundary(self, elem, i_dim, sign, bc):
         self.bnd_cons.append((elem, i_dim, int(sign), bc))
 
+    def connect_refined(self, coarse, fine, i_dim, reverse=False):
+        """Cartesian hanging-face connection between a coarse element and the 2^(nd-1) elements one level finer
+        (Refined_connection, connection.hpp:168-286). `fine` lists the fine elements in row-major order over the
+        tangential dimensions. Not reversed: the coarse element's +i_dim face meets the fine elements' -i_dim faces;
+        reversed: the fine elements come first (their +i_dim faces). Each fine element gets a Cartesian connection to
+        a mortar face; Prolong_refined / Restrict_refined move data between the coarse face and the mortar faces."""
+        if len(fine) != 2 ** (self.nd - 1):
+            raise RuntimeError("wrong number of elements in `Refined_connection`")
+        for e in [coarse] + list(fine):
+            if self.elems[e]["deformed"]:
+                raise RuntimeError("this builder makes Cartesian refined connections only")
+        self.ref_cons.append((coarse, list(fine), i_dim, bool(reverse)))
+
     def build(self, with_av=False, jitter=None):
         nd, rs = self.nd, self.rs
         nvert = 2 ** nd
@@ -430,9 +444,11 @@ class Builder:
             counts[c] += 1
         n_int_def = len(self.def_cons)
         n_bnd = len(self.bnd_cons)
-        n_car_con = len(self.car_cons)
-        m = HostMesh(nd, rs, counts[0], counts[1], 2 * (n_car_con + n_int_def + n_bnd), 2 * (n_int_def + n_bnd),
-                     n_car_con, n_int_def + n_bnd, with_av=with_av)
+        n_fine = sum(len(f) for _, f, _, _ in self.ref_cons)
+        n_car_con = len(self.car_cons) + n_fine
+        n_ref = len(self.ref_cons)
+        m = HostMesh(nd, rs, counts[0], counts[1], 2 * (n_car_con + n_int_def + n_bnd) + n_ref,
+                     2 * (n_int_def + n_bnd), n_car_con, n_int_def + n_bnd, n_ref=n_ref, with_av=with_av)
         # nominal vertex positions and union-find merge
         parent = list(range(len(self.elems) * nvert))
 
@@ -460,6 +476,11 @@ class Builder:
             ia, ib = vertex_inds(nd, dims, signs)
             for va, vb in zip(ia, ib):
                 union(a * nvert + va, b * nvert + vb)
+        for c, fine, d, rev in self.ref_cons:
+            # the coarse face's corners eat the matching corner of each fine element (connection.hpp:241-247)
+            vi = vertex_inds(nd, (d, d), (True, False))
+            for i, f in enumerate(fine):
+                union(c * nvert + vi[int(rev)][i], f * nvert + vi[int(not rev)][i])
         roots = np.array([find(x) for x in range(len(parent))])
         uniq, vid = np.unique(roots, return_inverse=True)
         vid = vid.reshape(len(self.elems), nvert)
@@ -493,6 +514,20 @@ class Builder:
             m.face_slot[cls_of[a]][idx_of[a], 2 * d + 1] = slot
             m.face_slot[cls_of[b]][idx_of[b], 2 * d] = slot + 1
             slot += 2
+        j = len(self.car_cons)
+        for k, (c, fine, d, rev) in enumerate(self.ref_cons):
+            m.ref_slot[k, 0] = slot
+            m.face_slot[cls_of[c]][idx_of[c], 2 * d + (0 if rev else 1)] = slot
+            slot += 1
+            for i, f in enumerate(fine):
+                # fine connection i: the mortar face on the coarse side, the fine element's face on the other
+                mortar, fface = (slot + 1, slot) if rev else (slot, slot + 1)
+                m.con_slot[0][j] = (slot, slot + 1)
+                m.con_dir[0][j] = pack_dir(d, d, 1, 0)
+                m.face_slot[cls_of[f]][idx_of[f], 2 * d + (1 if rev else 0)] = fface
+                m.ref_slot[k, 1 + i] = mortar
+                slot += 2
+                j += 1
         nrm = np.zeros((n_int_def + n_bnd, 2, nd, m.nfq))
         for k, (a, b, dims, signs) in enumerate(self.def_cons):
             if cls_of[a] != 1 or cls_of[b] != 1:

@@ -48,10 +48,12 @@ def o_call(name, *args):
 
 
 def init_faces(m):
-    """Solver::initialize's compute_write_face, run by the oracle on the host mesh."""
+    """Solver::initialize's compute_write_face + compute_prolong (Solver.cpp:400-401), run by the oracle on the host
+    mesh."""
     s = m.struct()
     for c in (0, 1):
         o_call("oracle_write_face", ctypes.byref(s), c, None)
+    o_call("oracle_prolong_refined", ctypes.byref(s), None, 0, 0)
 
 
 def rel_l2(a, b):

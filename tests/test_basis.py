@@ -111,3 +111,18 @@ def test_regeneration_reproduces_committed_tables():
     for rs in ROW_SIZES:
         for k, v in tables[rs].items():
             np.testing.assert_array_equal(np.ravel(v), np.ravel(TABLES[rs][k]), err_msg=f"{rs} {k}")
+
+
+@pytest.mark.parametrize("rs", ROW_SIZES)
+def test_tables_exactly_mirror_symmetric(rs):
+    """The FP64 element kernels take the mirrored half of D, B and L from the first half with a sign flip for even
+    row sizes (same products, same order). That is exact only if the committed tables are bit-for-bit
+    (anti)symmetric, which Gauss-Legendre nodes symmetric about 1/2 give in exact arithmetic."""
+    t = TABLES[rs]
+    d, b, w = t["diff_mat"], t["boundary"], t["weight"]
+    lift0 = -(b[0] * (1.0 / w))
+    lift1 = b[1] * (1.0 / w)
+    assert np.array_equal(b[1], b[0][::-1])
+    assert np.array_equal(lift1, -lift0[::-1])
+    if rs % 2 == 0:
+        assert np.array_equal(d, -d[::-1, ::-1])
