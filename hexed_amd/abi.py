@@ -93,6 +93,36 @@ def update_options(max_safety=0.7, max_time_step=1e300, n_cheby_flow=1, flow_ite
                              int(use_ldg), visc, cond)
 
 
+class hx_av_options(ctypes.Structure):
+    _fields_ = [("advect_length", ctypes.c_double), ("advect_iters", ctypes.c_int), ("advect_max_safety", ctypes.c_double),
+                ("diff_ratio", ctypes.c_double), ("diff_max_safety", ctypes.c_double), ("n_cheby", ctypes.c_int),
+                ("diff_iters", ctypes.c_int), ("visc_mult", ctypes.c_double), ("unscaled_max", ctypes.c_double),
+                ("freestream_mass", ctypes.c_double), ("freestream_energy", ctypes.c_double), ("av_rs", ctypes.c_int)]
+
+
+def av_options(advect_length, row_size, freestream_mass, freestream_energy, advect_iters=1, advect_max_safety=.7,
+               diff_ratio=.3, diff_max_safety=.7, n_cheby=1, diff_iters=1, visc_mult=1e2, unscaled_max=5., av_rs=None):
+    """Settings of Solver::update_art_visc_smoothness with the reference's defaults (Solver.cpp:144,154-163)."""
+    return hx_av_options(advect_length, advect_iters, advect_max_safety, diff_ratio, diff_max_safety, n_cheby, diff_iters,
+                         visc_mult, unscaled_max, freestream_mass, freestream_energy,
+                         row_size if av_rs is None else av_rs)
+
+
+def advection_pde(advect_length):
+    p = hx_pde()
+    p.kind = HX_PDE_ADVECTION
+    p.advect_length = advect_length
+    return p
+
+
+def smooth_av_pde(diff_time, cheby_step):
+    p = hx_pde()
+    p.kind = HX_PDE_SMOOTH_AV
+    p.diff_time = diff_time
+    p.cheby_step = cheby_step
+    return p
+
+
 class hx_update_status(ctypes.Structure):
     _fields_ = [("time_step", ctypes.c_double), ("flow_time", ctypes.c_double), ("iteration", ctypes.c_int)]
 
@@ -172,6 +202,19 @@ def load_library(path=LIB_PATH):
         "hx_compute_restrict": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "hx_update": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.POINTER(hx_update_status)]),
         "hx_compute_residual": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options)]),
+        "hx_apply_advection_bcs": (ctypes.c_int, [P]),
+        "hx_apply_avc_diff_bcs": (ctypes.c_int, [P]),
+        "hx_apply_avc_diff_flux_bcs": (ctypes.c_int, [P]),
+        "hx_compute_advection": (ctypes.c_int, [P, ctypes.POINTER(hx_kernel_options), ctypes.c_double]),
+        "hx_compute_smooth_av": (ctypes.c_int, [P, ctypes.POINTER(hx_kernel_options), ctypes.c_double, ctypes.c_double]),
+        "hx_compute_write_face_advection": (ctypes.c_int, [P, ctypes.c_double]),
+        "hx_compute_write_face_smooth_av": (ctypes.c_int, [P]),
+        "hx_compute_prolong_advection": (ctypes.c_int, [P, ctypes.c_double]),
+        "hx_max_dt_advection": (ctypes.c_int, [P, ctypes.POINTER(hx_kernel_options), ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_int, ctypes.c_double, c_double_p]),
+        "hx_max_dt_smooth_av": (ctypes.c_int, [P, ctypes.POINTER(hx_kernel_options), ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_int, c_double_p]),
+        "hx_update_art_visc_smoothness": (ctypes.c_int, [P, ctypes.POINTER(hx_av_options), c_double_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -190,5 +233,7 @@ EXPORTED_SYMBOLS = [
     "hx_neighbor", "hx_local", "hx_neighbor_reconcile", "hx_reconcile_ldg_flux", "hx_max_dt", "hx_prolong_refined",
     "hx_restrict_refined", "hx_apply_state_bcs", "hx_apply_flux_bcs", "hx_compute_euler", "hx_compute_navier_stokes",
     "hx_max_dt_euler", "hx_max_dt_navier_stokes", "hx_compute_write_face", "hx_compute_prolong", "hx_compute_restrict",
-    "hx_update", "hx_compute_residual",
+    "hx_update", "hx_compute_residual", "hx_apply_advection_bcs", "hx_apply_avc_diff_bcs", "hx_apply_avc_diff_flux_bcs",
+    "hx_compute_advection", "hx_compute_smooth_av", "hx_compute_write_face_advection", "hx_compute_write_face_smooth_av",
+    "hx_compute_prolong_advection", "hx_max_dt_advection", "hx_max_dt_smooth_av", "hx_update_art_visc_smoothness",
 ]

@@ -95,6 +95,8 @@ void fill_ops(Ops<RS>& o, int rs_index)
     o.lift[i][0] = -(inv_w * b[i]);
     o.lift[i][1] = inv_w * b[RS + i];
     o.node[i] = nodes[rs_index][i];
+    o.weight[i] = w[i];
+    for (int k = 0; k < RS; ++k) o.orth[k][i] = orthogonals[rs_index][k * RS + i];
   }
 }
 
@@ -160,7 +162,7 @@ long long field_size(const Dev_mesh& m, int field, int cls)
   const long long n = (cls == 0 || cls == 1) ? m.n_elem[cls] : 0;
   switch (field) {
     case HX_FIELD_STATE:
-    case HX_FIELD_RCACHE: return n * m.nv * m.nq;
+    case HX_FIELD_RCACHE: return n * std::max(m.nv, m.rs) * m.nq;
     case HX_FIELD_TSS: return n * m.nq;
     case HX_FIELD_AV: return n * 2 * m.nq;
     case HX_FIELD_FORCING: return n * 4 * m.nq;
@@ -268,20 +270,50 @@ Transport to_dev(const hx_transport& t)
   return Transport{t.const_val, t.ref_val, t.ref_temp, std::sqrt(t.ref_temp), t.temp_offset};
 }
 
+int pde_kind(const hx_pde* pde) { return pde ? pde->kind : HX_PDE_EULER; }
+bool is_av(int kind) { return kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV; }
+
 hx_status pde_check(hx_mesh* mesh, const hx_pde* pde, bool* visc, Launch* l)
 {
-  int kind = pde ? pde->kind : HX_PDE_EULER;
-  if (kind != HX_PDE_EULER && kind != HX_PDE_NAVIER_STOKES)
+  int kind = pde_kind(pde);
+  if (kind != HX_PDE_EULER && kind != HX_PDE_NAVIER_STOKES && !is_av(kind))
     return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "PDE kind not built into this library");
+  if (is_av(kind)) {
+    for (int c = 0; c < 2; ++c)
+      if (mesh->dm.n_elem[c] && (!mesh->dm.advection[c] || !mesh->dm.forcing[c]))
+        return fail(mesh->ctx, HX_ERR_INVALID, "artificial-viscosity PDEs need the forcing and advection fields");
+    if (kind == HX_PDE_ADVECTION && !(pde->advect_length > 0))
+      return fail(mesh->ctx, HX_ERR_INVALID, "advect_length must be positive");
+    if (kind == HX_PDE_SMOOTH_AV && !(pde->diff_time > 0))
+      return fail(mesh->ctx, HX_ERR_INVALID, "diff_time must be positive");
+  }
   *visc = kind == HX_PDE_NAVIER_STOKES;
   if (*visc) l->ns = Ns_params{to_dev(pde->visc), to_dev(pde->cond)};
   return HX_OK;
+}
+
+// Pde::n_extrap: the variables Write_face / Prolong_refined / Restrict_refined move (Spatial.hpp:141,218)
+int n_extrap(const hx_mesh* mesh, const hx_pde* pde)
+{
+  const int kind = pde_kind(pde);
+  if (kind == HX_PDE_ADVECTION) return mesh->dm.nd + mesh->dm.rs;
+  if (kind == HX_PDE_SMOOTH_AV) return 3;
+  return mesh->dm.nv;
 }
 
 bool valid_cls(int cls) { return cls == HX_CARTESIAN || cls == HX_DEFORMED; }
 
 const char* cls_name(int cls) { return cls == HX_DEFORMED ? "deformed" : "cartesian"; }
 const char* pde_name(bool visc) { return visc ? "navier_stokes" : "euler"; }
+const char* pde_name(const hx_pde* pde)
+{
+  switch (pde_kind(pde)) {
+    case HX_PDE_NAVIER_STOKES: return "navier_stokes";
+    case HX_PDE_ADVECTION: return "advection";
+    case HX_PDE_SMOOTH_AV: return "smooth_av";
+    default: return "euler";
+  }
+}
 
 hipEvent_t pool_event(hx_ctx* ctx)
 {
@@ -346,7 +378,7 @@ extern "C" {
 
 int hx_kernels_built(int n_dim, int row_size, int pde_kind)
 {
-  if (pde_kind != HX_PDE_EULER && pde_kind != HX_PDE_NAVIER_STOKES) return 0;
+  if (pde_kind != HX_PDE_EULER && pde_kind != HX_PDE_NAVIER_STOKES && !is_av(pde_kind)) return 0;
   return kernel_set(n_dim, row_size) ? 1 : 0;
 }
 
@@ -533,7 +565,7 @@ hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* d, hx_mesh** out)
     const long long n = d->n_elem[c];
     m.n_elem[c] = int(n);
     if ((s = device_copy(mesh, &m.state[c], d->state[c], n * z.nv * z.nq, true))) return bail(s);
-    if ((s = device_copy(mesh, &m.rcache[c], d->rcache[c], n * z.nv * z.nq, true))) return bail(s);
+    if ((s = device_copy(mesh, &m.rcache[c], d->rcache[c], n * std::max(z.nv, z.rs) * z.nq, true))) return bail(s);
     if ((s = device_copy(mesh, &m.tss[c], d->tss[c], n * z.nq, true))) return bail(s);
     if ((s = device_copy(mesh, &m.av[c], d->av[c], n * 2 * z.nq, false))) return bail(s);
     if ((s = device_copy(mesh, &m.forcing[c], d->forcing[c], n * 4 * z.nq, false))) return bail(s);
@@ -651,8 +683,9 @@ hx_status hx_write_face(hx_mesh* mesh, int cls, const hx_pde* pde)
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
-  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/write face/" + pde_name(visc), mesh->dm.n_elem[cls]);
-  HX_TRY(mesh->ctx, mesh->ks->write_face(l, cls), "write_face");
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/write face/" + pde_name(pde), mesh->dm.n_elem[cls]);
+  if (is_av(pde_kind(pde))) HX_TRY(mesh->ctx, mesh->ks->av_write_face(l, cls, pde->kind), "write_face");
+  else HX_TRY(mesh->ctx, mesh->ks->write_face(l, cls), "write_face");
   return HX_OK;
 }
 
@@ -662,8 +695,9 @@ hx_status hx_neighbor(hx_mesh* mesh, int cls, const hx_pde* pde, int)
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
-  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor/" + pde_name(visc), mesh->dm.n_con[cls]);
-  HX_TRY(mesh->ctx, mesh->ks->neighbor(l, cls, visc), "neighbor");
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor/" + pde_name(pde), mesh->dm.n_con[cls]);
+  if (is_av(pde_kind(pde))) HX_TRY(mesh->ctx, mesh->ks->av_neighbor(l, cls, pde->kind), "neighbor");
+  else HX_TRY(mesh->ctx, mesh->ks->neighbor(l, cls, visc), "neighbor");
   return HX_OK;
 }
 
@@ -679,17 +713,28 @@ hx_status hx_local(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_s
   if (i_stage && compute_residual)
     return fail(mesh->ctx, HX_ERR_INVALID, "residual calculation is a single-stage operation");
   const double coef = i_stage ? dt * hx_basis_step_ratio(mesh->dm.rs) : dt;
-  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/local/" + pde_name(visc), mesh->dm.n_elem[cls]);
-  HX_TRY(mesh->ctx, mesh->ks->local(l, cls, visc, coef, i_stage, compute_residual, use_filter), "local");
+  const int kind = pde_kind(pde);
+  if (is_av(kind) && use_filter)
+    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the artificial-viscosity PDEs are built without the modal filter");
+  if (kind == HX_PDE_SMOOTH_AV && (i_stage || compute_residual))
+    return fail(mesh->ctx, HX_ERR_INVALID, "Smooth_art_visc runs single-stage without compute_residual");
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/local/" + pde_name(pde), mesh->dm.n_elem[cls]);
+  if (kind == HX_PDE_ADVECTION)
+    HX_TRY(mesh->ctx, mesh->ks->adv_local(l, cls, coef, i_stage, compute_residual, pde->advect_length), "local");
+  else if (kind == HX_PDE_SMOOTH_AV) HX_TRY(mesh->ctx, mesh->ks->sav_local(l, cls, coef, pde->diff_time), "local");
+  else HX_TRY(mesh->ctx, mesh->ks->local(l, cls, visc, coef, i_stage, compute_residual, use_filter), "local");
   return HX_OK;
 }
 
-hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde*)
+hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde* pde)
 {
   if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
-  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor reconcile/navier_stokes", mesh->dm.n_con[cls]);
-  HX_TRY(mesh->ctx, mesh->ks->neighbor_reconcile(l, cls), "neighbor_reconcile");
+  const bool smooth = pde_kind(pde) == HX_PDE_SMOOTH_AV;
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor reconcile/" + (smooth ? "smooth_av" : "navier_stokes"),
+           mesh->dm.n_con[cls]);
+  if (smooth) HX_TRY(mesh->ctx, mesh->ks->sav_neighbor_reconcile(l, cls), "neighbor_reconcile");
+  else HX_TRY(mesh->ctx, mesh->ks->neighbor_reconcile(l, cls), "neighbor_reconcile");
   return HX_OK;
 }
 
@@ -700,8 +745,15 @@ hx_status hx_reconcile_ldg_flux(hx_mesh* mesh, int cls, const hx_pde* pde, doubl
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
-  if (visc && i_stage)
+  if ((visc || pde_kind(pde) == HX_PDE_SMOOTH_AV) && i_stage)
     return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
+  if (pde_kind(pde) == HX_PDE_SMOOTH_AV) {
+    if (compute_residual || use_filter)
+      return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "Smooth_art_visc is built without compute_residual and the filter");
+    Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/reconcile LDG flux/smooth_av", mesh->dm.n_elem[cls]);
+    HX_TRY(mesh->ctx, mesh->ks->sav_reconcile(l, cls, dt, pde->diff_time, pde->cheby_step), "reconcile_ldg_flux");
+    return HX_OK;
+  }
   Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/reconcile LDG flux/navier_stokes", mesh->dm.n_elem[cls]);
   HX_TRY(mesh->ctx, mesh->ks->reconcile_ldg_flux(l, cls, dt, compute_residual, use_filter), "reconcile_ldg_flux");
   return HX_OK;
@@ -719,8 +771,11 @@ hx_status hx_max_dt(hx_mesh* mesh, int cls, const hx_pde* pde, int is_local, int
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, l.stream, l.scalar, DBL_MAX);
   HX_TRY(mesh->ctx, hipGetLastError(), "fill");
   {
-    Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/compute time step/" + pde_name(visc), mesh->dm.n_elem[cls]);
-    HX_TRY(mesh->ctx, mesh->ks->max_dt(l, cls, visc, is_local, max_cfl_c, max_cfl_d), "max_dt");
+    Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/compute time step/" + pde_name(pde), mesh->dm.n_elem[cls]);
+    if (is_av(pde_kind(pde)))
+      HX_TRY(mesh->ctx, mesh->ks->av_max_dt(l, cls, pde->kind == HX_PDE_SMOOTH_AV, is_local, max_cfl_c, max_cfl_d),
+             "max_dt");
+    else HX_TRY(mesh->ctx, mesh->ks->max_dt(l, cls, visc, is_local, max_cfl_c, max_cfl_d), "max_dt");
   }
   double dt = DBL_MAX;
   HX_TRY(mesh->ctx, hipMemcpyAsync(&dt, l.scalar, sizeof(double), hipMemcpyDeviceToHost, l.stream), "max_dt readback");
@@ -736,7 +791,7 @@ hx_status hx_prolong_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int of
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
   Timer tm(mesh->ctx, "prolong/restrict/prolong", mesh->dm.n_ref);
-  HX_TRY(mesh->ctx, mesh->ks->prolong(l, mesh->dm.nv, scale, offset), "prolong");
+  HX_TRY(mesh->ctx, mesh->ks->prolong(l, n_extrap(mesh, pde), scale, offset), "prolong");
   return HX_OK;
 }
 
@@ -747,7 +802,7 @@ hx_status hx_restrict_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int o
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
   Timer tm(mesh->ctx, "prolong/restrict/restrict", mesh->dm.n_ref);
-  HX_TRY(mesh->ctx, mesh->ks->restrict_(l, mesh->dm.nv, scale, offset), "restrict");
+  HX_TRY(mesh->ctx, mesh->ks->restrict_(l, n_extrap(mesh, pde), scale, offset), "restrict");
   return HX_OK;
 }
 
@@ -892,6 +947,201 @@ hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* o)
   hx_kernel_options opts{1., 0, 1, o->use_filter};
   if (o->use_ldg) return hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
   return hx_compute_euler(mesh, &opts);
+}
+
+// ---- artificial-viscosity generator ---------------------------------------------------------------------------
+
+hx_status hx_apply_advection_bcs(hx_mesh* mesh)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  Timer tm(mesh->ctx, "boundary conditions/advection", mesh->dm.n_bc_con);
+  HX_TRY(mesh->ctx, mesh->ks->av_bcs(make_launch(mesh), 0), "advection bcs");
+  return HX_OK;
+}
+
+hx_status hx_apply_avc_diff_bcs(hx_mesh* mesh)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  Timer tm(mesh->ctx, "boundary conditions/av diffusion", mesh->dm.n_bc_con);
+  HX_TRY(mesh->ctx, mesh->ks->av_bcs(make_launch(mesh), 1), "avc diff bcs");
+  return HX_OK;
+}
+
+hx_status hx_apply_avc_diff_flux_bcs(hx_mesh* mesh)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  Timer tm(mesh->ctx, "boundary conditions/av diffusion flux", mesh->dm.n_bc_con);
+  HX_TRY(mesh->ctx, mesh->ks->av_bcs(make_launch(mesh), 2), "avc diff flux bcs");
+  return HX_OK;
+}
+
+hx_status hx_compute_advection(hx_mesh* mesh, const hx_kernel_options* o, double advect_length)
+{
+  if (!mesh || !o) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_ADVECTION;
+  p.advect_length = advect_length;
+  hx_status s;
+  if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, o->i_stage))) return s;
+  if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, o->i_stage))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
+  if ((s = hx_local(mesh, HX_CARTESIAN, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_local(mesh, HX_DEFORMED, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
+  return hx_prolong_refined(mesh, &p, 0, 0);
+}
+
+hx_status hx_compute_smooth_av(hx_mesh* mesh, const hx_kernel_options* o, double diff_time, double cheby_step)
+{
+  if (!mesh || !o) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_SMOOTH_AV;
+  p.diff_time = diff_time;
+  p.cheby_step = cheby_step;
+  if (o->i_stage)
+    return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
+  hx_status s;
+  if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, 0))) return s;
+  if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, 0))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 0, 1))) return s;
+  if ((s = hx_local(mesh, HX_CARTESIAN, &p, o->dt, 0, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_local(mesh, HX_DEFORMED, &p, o->dt, 0, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_prolong_refined(mesh, &p, 1, 1))) return s;
+  if ((s = hx_apply_avc_diff_flux_bcs(mesh))) return s;
+  if ((s = hx_neighbor_reconcile(mesh, HX_CARTESIAN, &p))) return s;
+  if ((s = hx_neighbor_reconcile(mesh, HX_DEFORMED, &p))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 1, 1))) return s;
+  if ((s = hx_reconcile_ldg_flux(mesh, HX_CARTESIAN, &p, o->dt, 0, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_reconcile_ldg_flux(mesh, HX_DEFORMED, &p, o->dt, 0, o->compute_residual, o->use_filter))) return s;
+  return hx_prolong_refined(mesh, &p, 0, 0);
+}
+
+hx_status hx_compute_write_face_advection(hx_mesh* mesh, double advect_length)
+{
+  hx_pde p{};
+  p.kind = HX_PDE_ADVECTION;
+  p.advect_length = advect_length;
+  hx_status s;
+  if ((s = hx_write_face(mesh, HX_CARTESIAN, &p))) return s;
+  return hx_write_face(mesh, HX_DEFORMED, &p);
+}
+
+hx_status hx_compute_write_face_smooth_av(hx_mesh* mesh)
+{
+  hx_pde p{};
+  p.kind = HX_PDE_SMOOTH_AV;
+  p.diff_time = p.cheby_step = 1.;  // Write_face(basis, 1., 1.) (kernels_convective.cpp:53-56)
+  hx_status s;
+  if ((s = hx_write_face(mesh, HX_CARTESIAN, &p))) return s;
+  return hx_write_face(mesh, HX_DEFORMED, &p);
+}
+
+hx_status hx_compute_prolong_advection(hx_mesh* mesh, double advect_length)
+{
+  hx_pde p{};
+  p.kind = HX_PDE_ADVECTION;
+  p.advect_length = advect_length;
+  return hx_prolong_refined(mesh, &p, 0, 0);
+}
+
+hx_status hx_max_dt_advection(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time,
+                              double advect_length, double* dt_out)
+{
+  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_ADVECTION;
+  p.advect_length = advect_length;
+  double a, b;
+  hx_status s;
+  const int uf = o ? o->use_filter : 0;
+  if ((s = hx_max_dt(mesh, HX_CARTESIAN, &p, local_time, uf, msc, msd, &a))) return s;
+  if ((s = hx_max_dt(mesh, HX_DEFORMED, &p, local_time, uf, msc, msd, &b))) return s;
+  *dt_out = std::min(a, b);
+  return HX_OK;
+}
+
+hx_status hx_max_dt_smooth_av(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time,
+                              double* dt_out)
+{
+  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_SMOOTH_AV;
+  p.diff_time = p.cheby_step = 1.;  // Max_dt<Smooth_art_visc>(..., 1., 1.) (kernels_max_dt.cpp:15)
+  double a, b;
+  hx_status s;
+  const int uf = o ? o->use_filter : 0;
+  if ((s = hx_max_dt(mesh, HX_CARTESIAN, &p, local_time, uf, msc, msd, &a))) return s;
+  if ((s = hx_max_dt(mesh, HX_DEFORMED, &p, local_time, uf, msc, msd, &b))) return s;
+  *dt_out = std::min(a, b);
+  return HX_OK;
+}
+
+// Solver::update_art_visc_smoothness (Solver.cpp:457-582) with diffuse_art_visc (Solver.cpp:428-455)
+hx_status hx_update_art_visc_smoothness(hx_mesh* mesh, const hx_av_options* o, double* residual)
+{
+  if (!mesh || !o) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  const Dev_mesh& dm = mesh->dm;
+  for (int c = 0; c < 2; ++c)
+    if (dm.n_elem[c] && (!dm.forcing[c] || !dm.advection[c] || !dm.av[c]))
+      return fail(ctx, HX_ERR_INVALID, "artificial viscosity needs the av, forcing and advection fields");
+  if (!(o->advect_length > 0) || o->advect_iters < 0 || o->diff_iters < 0 || o->n_cheby < 1)
+    return fail(ctx, HX_ERR_INVALID, "invalid artificial-viscosity settings");
+  if (o->av_rs < 1 || o->av_rs > dm.rs) return fail(ctx, HX_ERR_INVALID, "av_rs must lie in [1, row_size]");
+  const double L = o->advect_length;
+  const int n_real = 3;  // Storage_params::n_forcing - 1
+  const double mult = o->visc_mult * L;
+  const double us_max = L * o->unscaled_max * std::sqrt(2 * o->freestream_energy / o->freestream_mass);
+  Launch l = make_launch(mesh);
+  double* resid_dev = l.scalar + 1;
+  auto points = [&](int what) -> hx_status {
+    Timer tm(ctx, "set art visc/points", dm.n_elem[0] + dm.n_elem[1]);
+    for (int c = 0; c < 2; ++c) HX_TRY(ctx, mesh->ks->av_points(l, c, what, o->av_rs, mult, us_max, resid_dev), "av points");
+    return HX_OK;
+  };
+  hx_status s;
+  hx_kernel_options ko{1., 0, 0, 0};
+  double dummy;
+  if ((s = points(0))) return s;
+  // enforce CFL condition
+  if ((s = hx_compute_write_face(mesh))) return s;
+  if ((s = hx_compute_prolong(mesh, 0, 0))) return s;
+  if ((s = hx_max_dt_advection(mesh, &ko, o->advect_max_safety, 1., 1, L, &dummy))) return s;
+  for (int iter = 0; iter < o->advect_iters; ++iter) {
+    if ((s = hx_compute_write_face_advection(mesh, L))) return s;
+    if ((s = hx_compute_prolong_advection(mesh, L))) return s;
+    for (int i = 0; i < 2; ++i) {
+      if ((s = hx_apply_advection_bcs(mesh))) return s;
+      ko.i_stage = i;
+      if ((s = hx_compute_advection(mesh, &ko, L))) return s;
+    }
+  }
+  if ((s = points(1))) return s;
+  // root-smear-square
+  const double diff_time = o->diff_ratio * L * L / n_real;
+  ko = hx_kernel_options{0., 0, 0, 0};
+  if ((s = hx_max_dt_smooth_av(mesh, &ko, 1., o->diff_max_safety, 1, &dummy))) return s;
+  if ((s = hx_compute_write_face_smooth_av(mesh))) return s;
+  if ((s = hx_compute_prolong(mesh, 0, 0))) return s;
+  for (int iter = 0; iter < o->diff_iters; ++iter)
+    for (int i_cheby = 0; i_cheby < o->n_cheby; ++i_cheby) {
+      const double step = 1 / (1 - std::cos((o->n_cheby - i_cheby - 0.5) * M_PI / o->n_cheby));  // math.cpp:104-107
+      if ((s = hx_apply_avc_diff_bcs(mesh))) return s;
+      ko.dt = step;
+      if ((s = hx_compute_smooth_av(mesh, &ko, diff_time, step))) return s;
+    }
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, l.stream, resid_dev, 0.);
+  HX_TRY(ctx, hipGetLastError(), "fill");
+  if ((s = points(2))) return s;
+  if ((s = hx_compute_write_face(mesh))) return s;
+  if ((s = hx_compute_prolong(mesh, 0, 0))) return s;
+  if (residual) {
+    double r = 0;
+    HX_TRY(ctx, hipMemcpyAsync(&r, resid_dev, sizeof(double), hipMemcpyDeviceToHost, l.stream), "residual readback");
+    HX_TRY(ctx, hipStreamSynchronize(l.stream), "residual sync");
+    *residual = std::sqrt(r);
+  }
+  return HX_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,550 @@
+// gfx950 kernels of the artificial-viscosity generator: pde::Advection (pde.hpp:265-349) and pde::Smooth_art_visc
+// (pde.hpp:355-431) through the Spatial kernels, their boundary hooks (Boundary_condition.cpp:12-60,346-367) and the
+// point loops of Solver::update_art_visc_smoothness (Solver.cpp:457-582). Included by hx_kernels.hpp.
+//
+// Element data: the Advection state is the momentum slots of `state` (holding velocity while the generator runs,
+// Solver.cpp:465-477) followed by the rs `advection` arrays; Smooth_art_visc works on the 4 `forcing` arrays
+// ([0] the derivative estimate, [1..3] the diffused chain). Face slots carry the nd + rs Advection variables (or the
+// 3 forcing variables) in block 0 and the LDG variables at offset nv*nfq, as for the flow.
+#pragma once
+
+namespace hx {
+
+constexpr int av_kind_adv = HX_PDE_ADVECTION;
+constexpr int av_kind_sav = HX_PDE_SMOOTH_AV;
+
+// Point constants of the generator, passed by value
+template <int RS>
+struct Av_consts
+{
+  double node[RS];     // Gauss-Legendre nodes on [0, 1]
+  double adv_node[RS]; // Advection's 2 x - 1 (pde.hpp:279-281)
+  double weight[RS];   // quadrature weights
+  double orth[RS];     // normalized Legendre polynomial of degree av_rs - 1 at the nodes (Solver.cpp:529)
+};
+
+template <int ND, int RS, int KIND>
+struct Av_vars
+{
+  static constexpr int N = KIND == av_kind_adv ? ND + RS : 3;  // n_extrap
+};
+
+// Advection::fetch_extrap (pde.hpp:284-290) / Smooth_art_visc::fetch_extrap (pde.hpp:373-378) of variable v
+template <int ND, int RS, int KIND>
+__device__ inline double av_extrap(const Dev_mesh& m, int cls, size_t e, int v, int q)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ;
+  if constexpr (KIND == av_kind_adv) {
+    return v < ND ? m.state[cls][(e * (ND + 2) + v) * NQ + q] : m.advection[cls][(e * RS + (v - ND)) * NQ + q];
+  } else {
+    return m.forcing[cls][(e * 4 + 1 + v) * NQ + q];
+  }
+}
+
+// ---- Write_face<Advection / Smooth_art_visc> (Spatial.hpp:41-57) ------------------------------------------
+template <int ND, int RS, int KIND>
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_av_write_face(Dev_mesh m, Eops<RS, double> o, int cls)
+{
+  using S = Shape<ND, RS>;
+  constexpr int NQ = S::NQ, NQP = S::NQP, EPB = S::EPB, NVAR = Av_vars<ND, RS, KIND>::N;
+  __shared__ double lds[EPB * NVAR * NQP];
+  const int t = threadIdx.x, el = t / NQ, q = t % NQ;
+  const int e = blockIdx.x * EPB + el;
+  const bool valid = e < m.n_elem[cls];
+  for (int v = 0; v < NVAR; ++v)
+    lds[el * NVAR * NQP + v * NQP + S::pad(q)] = valid ? av_extrap<ND, RS, KIND>(m, cls, e, v, q) : 0.;
+  __syncthreads();
+  write_faces_from<ND, RS, S::BLOCK, NVAR>(m, cls, o, lds, NVAR * NQP);
+}
+
+// ---- Neighbor<Advection / Smooth_art_visc> (Spatial.hpp:613-704) ----------------------------------------------
+// Advection: LLF flux of the rs advected scalars with flux node_i (v.n) a_i and characteristic speed max(1, |v|)
+// (pde.hpp:319-347), written over the first rs variables of block 0. Smooth_art_visc: diffusion only, the LDG
+// average of the 3 extrapolated variables.
+template <int ND, int RS, bool DEF, int KIND>
+__global__ __launch_bounds__(256) void k_av_neighbor(Dev_mesh m, Av_consts<RS> ac)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2, NVAR = Av_vars<ND, RS, KIND>::N;
+  constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int con = gid / NFQ, f = gid % NFQ;
+  if (con >= m.n_con[cls]) return;
+  const Dir dir(m.con_dir[cls][con]);
+  double* f0 = slot_ptr(m, m.con_slot[cls][2 * con]);
+  double* f1 = slot_ptr(m, m.con_slot[cls][2 * con + 1]);
+  const int p1 = DEF ? face1_index<ND, RS>(dir, f) : f;
+  if constexpr (KIND == av_kind_sav) {
+    for (int v = 0; v < NVAR; ++v) {
+      const double avg = .5 * (f0[v * NFQ + f] + f1[v * NFQ + p1]);
+      f0[NV * NFQ + v * NFQ + f] = avg;
+      f1[NV * NFQ + v * NFQ + p1] = avg;
+    }
+  } else {
+    double sg0 = 1., sg1 = 1.;
+    double n[ND];
+    if constexpr (DEF) {
+      sg0 = 1 - 2 * int(dir.flip_normal(0));
+      sg1 = 1 - 2 * int(dir.flip_normal(1));
+      const double* nn = m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ;
+      for (int j = 0; j < ND; ++j) n[j] = sg0 * nn[j * NFQ + f];
+    } else {
+      for (int j = 0; j < ND; ++j) n[j] = (j == dir.i_dim0);
+    }
+    double vn0 = 0., vn1 = 0., vsq0 = 0., vsq1 = 0.;
+    for (int j = 0; j < ND; ++j) {
+      const double v0 = f0[j * NFQ + f], v1 = f1[j * NFQ + p1];
+      vn0 += v0 * n[j];
+      vn1 += v1 * n[j];
+      vsq0 += v0 * v0;
+      vsq1 += v1 * v1;
+    }
+    double a0[RS], a1[RS];
+    for (int i = 0; i < RS; ++i) {
+      a0[i] = f0[(ND + i) * NFQ + f];
+      a1[i] = f1[(ND + i) * NFQ + p1];
+    }
+    double nsq = 0;
+    for (int j = 0; j < ND; ++j) nsq += n[j] * n[j];
+    const double c = fmax(fmax(1., sqrt(vsq0)), fmax(1., sqrt(vsq1))) * sqrt(nsq);
+    for (int i = 0; i < RS; ++i) {
+      const double fl = .5 * (ac.adv_node[i] * vn0 * a0[i] + ac.adv_node[i] * vn1 * a1[i] + c * (a0[i] - a1[i]));
+      f0[i * NFQ + f] = sg0 * fl;
+      f1[i * NFQ + p1] = sg1 * fl;
+    }
+  }
+}
+
+// ---- Local<Advection> (Spatial.hpp:326-509) -----------------------------------------------------------------
+// Convective rate along each direction, the constant source 2/advect_length in stage 0, and Advection::write_update
+// (pde.hpp:292-300): stage 0 a = (a + du)/(1 + tss 2/L), stage 1 a += du/(1 + tss 2/L). Then Write_face.
+template <int ND, int RS>
+struct Adv_lds
+{
+  using S = Shape<ND, RS>;
+  static constexpr int F = 0;                                  // [RS][NQP] flux along the current direction
+  static constexpr int RT = F + RS * S::NQP;                   // [RS][NQP] rate
+  static constexpr int U = 0;                                  // [ND + RS][NQP] updated state (after F, RT)
+  static constexpr int SB = (2 * RS > ND + RS ? 2 * RS : ND + RS) * S::NQP;  // [2ND][RS][NFQ] face fluxes
+  static constexpr int PER = SB + 2 * ND * RS * S::NFQ;
+};
+
+template <int ND, int RS, bool DEF>
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_adv_local(Dev_mesh m, Eops<RS, double> o, Av_consts<RS> ac,
+                                                                      double coef, int stage, int compute_residual,
+                                                                      double advect_length)
+{
+  using S = Shape<ND, RS>;
+  using L = Adv_lds<ND, RS>;
+  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, EPB = S::EPB, NT = S::BLOCK;
+  constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
+  __shared__ double lds[EPB * L::PER];
+  const int t = threadIdx.x, el = t / NQ, q = t % NQ, qp = S::pad(q);
+  const int n = m.n_elem[cls];
+  const int e = blockIdx.x * EPB + el;
+  const bool valid = e < n;
+  double* my = lds + el * L::PER;
+  const Contraction<RS, Fp64_form> con(o, nullptr);
+
+  double vel[ND], a[RS];
+  for (int j = 0; j < ND; ++j) vel[j] = valid ? m.state[cls][(size_t(e) * (ND + 2) + j) * NQ + q] : 0.;
+  for (int i = 0; i < RS; ++i) a[i] = valid ? m.advection[cls][(size_t(e) * RS + i) * NQ + q] : 0.;
+  double det = 1.;
+  double met[ND][ND];
+  if constexpr (DEF) {
+    const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
+    for (int i = 0; i < ND; ++i)
+      for (int j = 0; j < ND; ++j) met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
+    det = valid ? mt[ND * ND * NQ + q] : 1.;
+  }
+  // numerical face fluxes: the first rs variables of block 0 of every face
+  for (int k = t; k < EPB * 2 * ND * RS * NFQ; k += NT) {
+    const int el2 = k / (2 * ND * RS * NFQ), r = k % (2 * ND * RS * NFQ), f = r / (RS * NFQ), vq = r % (RS * NFQ);
+    const int e2 = blockIdx.x * EPB + el2;
+    lds[el2 * L::PER + L::SB + r] = e2 < n ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[vq] : 0.;
+  }
+  for (int i = 0; i < ND; ++i) {
+    double vn = 0;
+    for (int j = 0; j < ND; ++j) {
+      if constexpr (DEF) vn += vel[j] * met[i][j];
+      else vn += vel[j] * double(j == i);
+    }
+    if (i) __syncthreads();
+    for (int v = 0; v < RS; ++v) my[L::F + v * NQP + qp] = ac.adv_node[v] * vn * a[v];
+    __syncthreads();
+    for_columns<ND, RS, RS, NT, false>(i, [&](bool, int el2, int v, int r, int base, int sd) {
+      double* b = lds + el2 * L::PER;
+      double x[RS], y[RS];
+      load_row(b + L::F + v * NQP + base, sd, x);
+      con.faces(x, b[L::SB + ((2 * i) * RS + v) * NFQ + r], b[L::SB + ((2 * i + 1) * RS + v) * NFQ + r], y);
+      sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
+    });
+  }
+  __syncthreads();
+  if (valid) {
+    const double d_pos = m.nom_size[cls][e];
+    const double tss = m.tss[cls][size_t(e) * NQ + q];
+    double mult = coef * tss / d_pos;
+    if constexpr (DEF) mult /= det;
+    double src_mult = d_pos;
+    if constexpr (DEF) src_mult *= det;
+    double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
+    const double pseudo = 1 + tss * 2 / advect_length;
+    for (int v = 0; v < RS; ++v) {
+      double u = my[L::RT + v * NQP + qp];
+      if (stage) {
+        u -= rc[v * NQ + q];
+      } else {
+        rc[v * NQ + q] = u;
+        u += src_mult * (2 / advect_length);
+      }
+      u *= mult;
+      double du = 0.;
+      if (compute_residual) rc[v * NQ + q] = u;
+      else du = u;
+      if (!stage) a[v] = (a[v] + du) / pseudo;
+      else a[v] += du / pseudo;
+      m.advection[cls][(size_t(e) * RS + v) * NQ + q] = a[v];
+    }
+  }
+  // Write_face<Advection> of the updated state (Spatial.hpp:507)
+  __syncthreads();
+  for (int j = 0; j < ND; ++j) my[L::U + j * NQP + qp] = vel[j];
+  for (int v = 0; v < RS; ++v) my[L::U + (ND + v) * NQP + qp] = a[v];
+  __syncthreads();
+  write_faces_from<ND, RS, NT, ND + RS>(m, cls, o, lds + L::U, L::PER);
+}
+
+// ---- Local<Smooth_art_visc> (Spatial.hpp:326-509, stage 0) ----------------------------------------------------
+// LDG gradient of forcing[1..3], flux -gradient*normal (pde.hpp:405-410), the chained source
+// |f0|, sqrt|f1|, |f2| over diff_time (pde.hpp:419-427), and the additive update of write_update (pde.hpp:380-388).
+template <int ND, int RS, bool DEF>
+struct Sav_lds
+{
+  using S = Shape<ND, RS>;
+  static constexpr int NQP = S::NQP, NFQ = S::NFQ, NS = 3;
+  static constexpr int U = 0;                                   // [3][NQP] forcing[1..3]
+  static constexpr int G = U + NS * NQP;                        // [3][NQP] one gradient component
+  static constexpr int SN = G + NS * NQP;                       // [ND][ND][NQP] reference-level normals
+  static constexpr int SL = SN + (DEF ? ND * ND * NQP : 0);     // [2ND][3][NFQ] LDG face state
+  static constexpr int SFN = SL + 2 * ND * NS * NFQ;            // [2ND][ND][NFQ] face normals
+  static constexpr int P1 = SFN + (DEF ? 2 * ND * ND * NFQ : 0);
+  static constexpr int F = 0;                                   // [3][NQP] viscous flux along the current direction
+  static constexpr int RT = F + NS * NQP;                       // [3][NQP] rate
+  static constexpr int P2 = RT + NS * NQP;
+  static constexpr int PER = P1 > P2 ? P1 : P2;
+};
+
+template <int ND, int RS, bool DEF>
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m, Eops<RS, double> o, double coef,
+                                                                      double diff_time)
+{
+  using S = Shape<ND, RS>;
+  using L = Sav_lds<ND, RS, DEF>;
+  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = S::BLOCK, NS = 3;
+  constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
+  __shared__ double lds[EPB * L::PER];
+  const int t = threadIdx.x, el = t / NQ, q = t % NQ, qp = S::pad(q);
+  const int n = m.n_elem[cls];
+  const int e = blockIdx.x * EPB + el;
+  const bool valid = e < n;
+  double* my = lds + el * L::PER;
+  const Contraction<RS, Fp64_form> con(o, nullptr);
+
+  double fc[4];
+  for (int v = 0; v < 4; ++v) fc[v] = valid ? m.forcing[cls][(size_t(e) * 4 + v) * NQ + q] : 0.;
+  for (int v = 0; v < NS; ++v) my[L::U + v * NQP + qp] = fc[1 + v];
+  double det = 1.;
+  double met[ND][ND];
+  if constexpr (DEF) {
+    const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
+    for (int i = 0; i < ND; ++i)
+      for (int j = 0; j < ND; ++j) {
+        met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
+        my[L::SN + (i * ND + j) * NQP + qp] = met[i][j];
+      }
+    det = valid ? mt[ND * ND * NQ + q] : 1.;
+  }
+  for (int k = t; k < EPB * 2 * ND * NS * NFQ; k += NT) {
+    const int el2 = k / (2 * ND * NS * NFQ), r = k % (2 * ND * NS * NFQ), f = r / (NS * NFQ), vq = r % (NS * NFQ);
+    const int e2 = blockIdx.x * EPB + el2;
+    lds[el2 * L::PER + L::SL + r] = e2 < n ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[NV * NFQ + vq] : 0.;
+  }
+  if constexpr (DEF) {
+    for (int k = t; k < EPB * 2 * ND * ND * NFQ; k += NT) {
+      const int el2 = k / (2 * ND * ND * NFQ), r = k % (2 * ND * ND * NFQ), f = r / (ND * NFQ), jq = r % (ND * NFQ);
+      const int e2 = blockIdx.x * EPB + el2;
+      double val = 0.;
+      if (e2 < n) {
+        const int ns_ = m.face_nslot[e2 * 2 * ND + f];
+        val = ns_ >= 0 ? m.normals[size_t(ns_) * ND * NFQ + jq] : double(jq / NFQ == f / 2);
+      }
+      lds[el2 * L::PER + L::SFN + r] = val;
+    }
+  }
+  __syncthreads();
+  const double d_pos = valid ? m.nom_size[cls][e] : 1.;
+  const double inv_h = 1. / d_pos;
+  double grad[NS][ND];
+  for (int j = 0; j < ND; ++j) {
+    for (int i = DEF ? 0 : j; i < (DEF ? ND : j + 1); ++i) {
+      for_columns<ND, RS, NS, NT, false>(i, [&](bool, int el2, int v, int r, int base, int sd) {
+        double* b = lds + el2 * L::PER;
+        double x[RS], y[RS];
+        load_row(b + L::U + v * NQP + base, sd, x);
+        double b0 = b[L::SL + ((2 * i) * NS + v) * NFQ + r], b1 = b[L::SL + ((2 * i + 1) * NS + v) * NFQ + r];
+        if constexpr (DEF) {
+          double nk[RS];
+          load_row(b + L::SN + (i * ND + j) * NQP + base, sd, nk);
+#pragma unroll
+          for (int k = 0; k < RS; ++k) x[k] *= nk[k];
+          b0 *= b[L::SFN + ((2 * i) * ND + j) * NFQ + r];
+          b1 *= b[L::SFN + ((2 * i + 1) * ND + j) * NFQ + r];
+        }
+        con.faces(x, b0, b1, y);
+        double* out = b + L::G + v * NQP + base;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) out[k * sd] = (i == 0 || !DEF) ? y[k] * inv_h : out[k * sd] + y[k] * inv_h;
+      });
+      __syncthreads();
+    }
+    for (int v = 0; v < NS; ++v) grad[v][j] = my[L::G + v * NQP + qp];
+    __syncthreads();
+  }
+  if constexpr (DEF)
+    for (int v = 0; v < NS; ++v)
+      for (int j = 0; j < ND; ++j) grad[v][j] /= det;
+  double src_mult = d_pos;
+  if constexpr (DEF) src_mult *= det;
+  for (int v = 0; v < NS; ++v) {
+    const double f = fabs(fc[v]);
+    my[L::RT + v * NQP + qp] = src_mult * (((v == 1) ? sqrt(f) : f) / diff_time);
+  }
+  for (int i = 0; i < ND; ++i) {
+    if (i) __syncthreads();
+    for (int v = 0; v < NS; ++v) {
+      double acc = 0;
+      for (int j = 0; j < ND; ++j) {
+        if constexpr (DEF) acc += -grad[v][j] * met[i][j];
+        else acc += -grad[v][j] * double(j == i);
+      }
+      my[L::F + v * NQP + qp] = acc;
+    }
+    __syncthreads();
+    for_columns<ND, RS, NS, NT, false>(i, [&](bool, int el2, int v, int r, int base, int sd) {
+      double* b = lds + el2 * L::PER;
+      double x[RS], y[RS], bx[2];
+      load_row(b + L::F + v * NQP + base, sd, x);
+      con.faces_if(false, x, 0., 0., y, bx);
+      const int e2 = blockIdx.x * EPB + el2;
+      if (e2 < n) {
+        const int* fs = m.face_slot[cls] + e2 * 2 * ND + 2 * i;
+        slot_ptr(m, fs[0])[(NV + v) * NFQ + r] = bx[0];
+        slot_ptr(m, fs[1])[(NV + v) * NFQ + r] = bx[1];
+      }
+      sub_row(b + L::RT + v * NQP + base, sd, y, false);
+    });
+  }
+  __syncthreads();
+  if (valid) {
+    double mult = coef * m.tss[cls][size_t(e) * NQ + q] / d_pos;
+    if constexpr (DEF) mult /= det;
+    for (int v = 0; v < NS; ++v) {
+      double u = 0.;
+      u += my[L::RT + v * NQP + qp];
+      u *= mult;
+      m.forcing[cls][(size_t(e) * 4 + 1 + v) * NQ + q] = fc[1 + v] + u;
+    }
+  }
+}
+
+// ---- Reconcile_ldg_flux<Smooth_art_visc> (Spatial.hpp:543-594) ------------------------------------------------
+// -boundary_term(jump), the critical write_update (f += du; f /= 1 + tss cheby/diff_time, pde.hpp:380-388), then
+// Write_face of forcing[1..3].
+template <int ND, int RS, bool DEF>
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_reconcile(Dev_mesh m, Eops<RS, double> o, double dt,
+                                                                          double diff_time, double cheby)
+{
+  using S = Shape<ND, RS>;
+  using R = Row<ND, RS>;
+  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NS = 3;
+  constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
+  constexpr int PF = 2 * ND * NS * NFQ, PER = PF + NS * NQP;
+  __shared__ double lds[EPB * PER];
+  const int t = threadIdx.x, el = t / NQ, q = t % NQ;
+  const int n = m.n_elem[cls];
+  const int e = blockIdx.x * EPB + el;
+  const bool valid = e < n;
+  double* my = lds + el * PER;
+  for (int k = t; k < EPB * PF; k += S::BLOCK) {
+    const int el2 = k / PF, r = k % PF, f = r / (NS * NFQ), vq = r % (NS * NFQ);
+    const int e2 = blockIdx.x * EPB + el2;
+    lds[el2 * PER + r] = e2 < n ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[NV * NFQ + vq] : 0.;
+  }
+  __syncthreads();
+  double rate[NS] = {};
+  for (int i = 0; i < ND; ++i) {
+    const int ki = R::node(q, i), fq = R::fq(q, i);
+    for (int v = 0; v < NS; ++v)
+      rate[v] -= o.lift[ki][0] * my[((2 * i) * NS + v) * NFQ + fq] + o.lift[ki][1] * my[((2 * i + 1) * NS + v) * NFQ + fq];
+  }
+  double fv[NS];
+  for (int v = 0; v < NS; ++v) fv[v] = valid ? m.forcing[cls][(size_t(e) * 4 + 1 + v) * NQ + q] : 0.;
+  if (valid) {
+    const double tss = m.tss[cls][size_t(e) * NQ + q];
+    double mult = dt * tss / m.nom_size[cls][e];
+    if constexpr (DEF) mult /= m.metrics[size_t(e) * (ND * ND + 1) * NQ + ND * ND * NQ + q];
+    const double pseudo = 1 + tss * cheby / diff_time;
+    for (int v = 0; v < NS; ++v) {
+      fv[v] += rate[v] * mult;
+      fv[v] /= pseudo;
+      m.forcing[cls][(size_t(e) * 4 + 1 + v) * NQ + q] = fv[v];
+    }
+  }
+  for (int v = 0; v < NS; ++v) my[PF + v * NQP + S::pad(q)] = fv[v];
+  __syncthreads();
+  write_faces_from<ND, RS, S::BLOCK, NS>(m, cls, o, lds + PF, PER);
+}
+
+// ---- Max_dt<Advection / Smooth_art_visc> (Spatial.hpp:784-828) ------------------------------------------------
+template <int ND, int RS, bool SMOOTH>
+__global__ __launch_bounds__(256) void k_av_max_dt(Dev_mesh m, Av_consts<RS> ac, int cls, int is_local,
+                                                   double max_cfl_c, double max_cfl_d, double* result)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ, NVERT = 1 << ND;
+  __shared__ double wmin[256 / 64];
+  const size_t n_items = size_t(m.n_elem[cls]) * NQ;
+  double inv = DBL_MAX;
+  for (size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x; gid < n_items; gid += size_t(gridDim.x) * 256) {
+    const size_t e = gid / NQ;
+    const int q = int(gid % NQ);
+    double vs[NVERT];
+    for (int i = 0; i < NVERT; ++i) vs[i] = m.vertex_tss[cls][e * NVERT + i];
+    int stride = NVERT;
+    for (int d = 0; d < ND; ++d) {
+      stride /= 2;
+      const double x = ac.node[(q / ipow(RS, ND - 1 - d)) % RS];
+      for (int i = 0; i < stride; ++i) vs[i] += x * (vs[i + stride] - vs[i]);
+    }
+    const double spacing = vs[0];
+    double scale = 0;
+    if constexpr (SMOOTH) {
+      scale += 1. / max_cfl_d / spacing / spacing;
+    } else {
+      double vsq = 0;
+      for (int j = 0; j < ND; ++j) {
+        const double v = m.state[cls][(e * (ND + 2) + j) * NQ + q];
+        vsq += v * v;
+      }
+      scale += fmax(1., sqrt(vsq)) / max_cfl_c / spacing;
+    }
+    if (is_local) {
+      m.tss[cls][e * NQ + q] = 1. / scale;
+    } else {
+      m.tss[cls][e * NQ + q] = 1.;
+      const double c = 1. / scale;
+      inv = (c < inv) ? c : inv;
+    }
+  }
+  if (is_local) return;
+  for (int off = 32; off > 0; off /= 2) {
+    const double o = __shfl_xor(inv, off, 64);
+    inv = (o < inv) ? o : inv;
+  }
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x / 64] = inv;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = wmin[0];
+    for (int w = 1; w < 256 / 64; ++w) b = (wmin[w] < b) ? wmin[w] : b;
+    if (b < DBL_MAX) atomicMin(reinterpret_cast<unsigned long long*>(result), __double_as_longlong(b));
+  }
+}
+
+// ---- boundary hooks (Solver.cpp:83-101,508-514; Boundary_condition.cpp:12-60,346-367) -------------------------
+// mode 0: apply_advection; 1: apply_diffusion (ghost state = inside); 2: flux_diffusion (ghost LDG = -inside)
+template <int ND, int RS>
+__global__ __launch_bounds__(256) void k_av_bcs(Dev_mesh m, int mode)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid / NFQ, f = gid % NFQ;
+  if (i >= m.n_bc_con) return;
+  const int con = m.bc_con[i];
+  const double* in_f = slot_ptr(m, m.con_slot[HX_DEFORMED][2 * con]);
+  double* gh_f = slot_ptr(m, m.con_slot[HX_DEFORMED][2 * con + 1]);
+  if (mode == 1) {
+    for (int v = 0; v < NV; ++v) gh_f[v * NFQ + f] = in_f[v * NFQ + f];
+    return;
+  }
+  if (mode == 2) {
+    for (int v = 0; v < NV; ++v) gh_f[(NV + v) * NFQ + f] = -in_f[(NV + v) * NFQ + f];
+    return;
+  }
+  const int kind = m.bcs[m.bc_serial[i]].kind;
+  if (kind == HX_BC_COPY) {  // copy_state: the first nv variables of both blocks
+    for (int l = 0; l < 2; ++l)
+      for (int v = 0; v < NV; ++v) gh_f[(l * NV + v) * NFQ + f] = in_f[(l * NV + v) * NFQ + f];
+  } else if (kind == HX_BC_NONPENETRATION) {
+    const double* nrml = m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ;
+    double dot = 0., nsq = 0.;
+    for (int d = 0; d < ND; ++d) {
+      dot += in_f[d * NFQ + f] * nrml[d * NFQ + f];
+      nsq += nrml[d * NFQ + f] * nrml[d * NFQ + f];
+    }
+    for (int d = 0; d < ND; ++d) gh_f[d * NFQ + f] = in_f[d * NFQ + f] - 2 * dot * nrml[d * NFQ + f] / nsq;
+    for (int a = 0; a < RS; ++a) gh_f[(ND + a) * NFQ + f] = in_f[(ND + a) * NFQ + f];
+  } else {  // Flow_bc::apply_advection: velocity copied, advected scalars 2 - inside
+    for (int d = 0; d < ND; ++d) gh_f[d * NFQ + f] = in_f[d * NFQ + f];
+    for (int a = 0; a < RS; ++a) gh_f[(ND + a) * NFQ + f] = 2. - in_f[(ND + a) * NFQ + f];
+  }
+}
+
+// ---- point loops of update_art_visc_smoothness (Solver.cpp:465-477,525-540,549-574) ---------------------------
+// what 0: momentum /= sqrt(2 rho E); 1: Legendre projection of the advected scalars into forcing[0];
+// 2: new coefficient us_max f/(us_max + f), squared change weighted into `resid`, momentum restored
+template <int ND, int RS>
+__global__ __launch_bounds__(256) void k_av_points(Dev_mesh m, Av_consts<RS> ac, int cls, int what, double mult,
+                                                   double us_max, double* resid)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ, NV = ND + 2;
+  __shared__ double wsum[256 / 64];
+  const size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const size_t e = gid / NQ;
+  const int q = int(gid % NQ);
+  double part = 0.;
+  if (e < size_t(m.n_elem[cls])) {
+    double* st = m.state[cls] + e * NV * NQ;
+    if (what == 0) {
+      const double scale = sqrt(2 * st[ND * NQ + q] * st[(ND + 1) * NQ + q]);
+      for (int d = 0; d < ND; ++d) st[d * NQ + q] /= scale;
+    } else if (what == 1) {
+      const double* adv = m.advection[cls] + e * RS * NQ;
+      double proj = 0;
+      for (int i = 0; i < RS; ++i) proj += adv[i * NQ + q] * ac.weight[i] * ac.orth[i];
+      m.forcing[cls][e * 4 * NQ + q] = proj * proj * 2 * st[(ND + 1) * NQ + q] / st[ND * NQ + q];
+    } else {
+      const double f = mult * m.forcing[cls][(e * 4 + 3) * NQ + q];
+      const double new_av = us_max * f / (us_max + f);
+      double* av = m.av[cls] + e * 2 * NQ;
+      double w = 1;
+      for (int d = 0; d < ND; ++d) w *= ac.weight[(q / ipow(RS, ND - 1 - d)) % RS];
+      double volume = 1;
+      for (int d = 0; d < ND; ++d) volume *= m.nom_size[cls][e];
+      const double diff = av[q] - new_av;
+      part = (1. * diff * diff) * w * volume;
+      av[q] = new_av;
+      const double scale = sqrt(2 * st[ND * NQ + q] * st[(ND + 1) * NQ + q]);
+      for (int d = 0; d < ND; ++d) st[d * NQ + q] *= scale;
+    }
+  }
+  if (what != 2) return;
+  for (int off = 32; off > 0; off /= 2) part += __shfl_xor(part, off, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x / 64] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0;
+    for (int w = 0; w < 256 / 64; ++w) b += wsum[w];
+    atomicAdd(resid, b);
+  }
+}
+
+}  // namespace hx

@@ -61,6 +61,8 @@ struct Ops
   double lift[RS][2];    // W^{-1} B^T diag(-1, 1)  (Derivative.hpp:27-28)
   double filt[RS][RS];   // filter[i][j]
   double node[RS];
+  double weight[RS];     // quadrature weights
+  double orth[RS][RS];   // orthogonal[degree][node]: normalized Legendre polynomials at the nodes
   double prolong[2][RS][RS];
   double restrict_[2][RS][RS];
 };

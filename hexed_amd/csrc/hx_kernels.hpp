@@ -22,6 +22,7 @@ struct Shape
   static constexpr int EPB = (256 / NQ) > 0 ? 256 / NQ : 1;
   static constexpr int BLOCK = EPB * NQ;
   static constexpr int NFACE_DOF = 2 * ND * NV * NFQ;  // face values per element
+  static constexpr int NRC = NV > RS ? NV : RS;         // residual-cache variables per point
   // LDS copies of per-point arrays carry one pad slot after every row along the last dimension, so that the
   // columns of that dimension (rows rs apart) fall on different banks
   static constexpr int NQP = NQ + NQ / RS;
@@ -384,15 +385,15 @@ __device__ inline void load_faces(const Dev_mesh& m, int cls, int ldg, double* d
   }
 }
 
-// Write_face of the FP64 padded [NV][NQP] arrays at `buf` (element stride `per`) into block 0 of every face
+// Write_face of the FP64 padded [NVAR][NQP] arrays at `buf` (element stride `per`) into block 0 of every face
 // (Spatial.hpp:29-71)
-template <int ND, int RS, int NT>
+template <int ND, int RS, int NT, int NVAR = Shape<ND, RS>::NV>
 __device__ inline void write_faces_from(const Dev_mesh& m, int cls, const Eops<RS, double>& o, const double* buf, int per)
 {
   using S = Shape<ND, RS>;
   const int n = m.n_elem[cls];
   for (int i = 0; i < ND; ++i) {
-    for_columns<ND, RS, S::NV, NT, false>(i, [&](bool, int el, int v, int r, int base, int sd) {
+    for_columns<ND, RS, NVAR, NT, false>(i, [&](bool, int el, int v, int r, int base, int sd) {
       const int e = blockIdx.x * S::EPB + el;
       if (e >= n) return;
       double x[RS];
@@ -483,7 +484,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
   if (valid) {
     double mult = coef * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
     if constexpr (DEF) mult /= det;
-    double* rc = m.rcache[cls] + size_t(e) * NV * NQ;
+    double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
       double r = double(my[L::RT + v * NQP + qp]);
@@ -700,7 +701,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
   if (valid) {
     double mult = coef * m.tss[cls][size_t(e) * NQ + q] * inv_h;
     if constexpr (DEF) mult /= det;
-    double* rc = m.rcache[cls] + size_t(e) * NV * NQ;
+    double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
       double r = double(my[L::RT + v * NQP + qp]);
@@ -760,7 +761,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_reconcile_ldg(Dev_me
     double mult = dt * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
     if constexpr (DEF) mult /= m.metrics[size_t(e) * (ND * ND + 1) * NQ + ND * ND * NQ + q];
     if (compute_residual) {
-      double* rc = m.rcache[cls] + size_t(e) * NV * NQ;
+      double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
       for (int v = 0; v < NV; ++v) rc[v * NQ + q] += rate[v] * mult;
     } else {
       double* sw = m.state[cls] + size_t(e) * NV * NQ;
@@ -844,10 +845,10 @@ __global__ __launch_bounds__(256) void k_neighbor(Dev_mesh m)
 }
 
 // ---- Neighbor_reconcile (Spatial.hpp:716-759) ------------------------------------------------------------
-template <int ND, int RS, bool DEF>
+template <int ND, int RS, bool DEF, int NVAR = ND + 2>
 __global__ __launch_bounds__(256) void k_neighbor_reconcile(Dev_mesh m)
 {
-  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
+  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;  // LDG block offset NV * NFQ; NVAR variables reconciled
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
   const int gid = blockIdx.x * 256 + threadIdx.x;
   const int con = gid / NFQ, f = gid % NFQ;
@@ -862,7 +863,7 @@ __global__ __launch_bounds__(256) void k_neighbor_reconcile(Dev_mesh m)
     sg0 = 1 - 2 * int(dir.flip_normal(0));
     sg1 = 1 - 2 * int(dir.flip_normal(1));
   }
-  for (int v = 0; v < NV; ++v) {
+  for (int v = 0; v < NVAR; ++v) {
     const double a0 = f0[v * NFQ + f], a1 = f1[v * NFQ + p1];
     double avg = 0;
     avg += .5 * sg0 * a0;
@@ -1094,6 +1095,12 @@ __global__ __launch_bounds__(256) void k_flux_bcs(Dev_mesh m)
   }
 }
 
+}  // namespace hx
+
+#include "hx_av_kernels.hpp"
+
+namespace hx {
+
 // ---- launchers -------------------------------------------------------------------------------------------
 inline unsigned blocks_for(size_t n_items, int per_block) { return unsigned((n_items + per_block - 1) / per_block); }
 
@@ -1201,10 +1208,118 @@ struct Launchers
     hipLaunchKernelGGL((k_flux_bcs<ND, RS>), dim3(blocks_for(size_t(l.m.n_bc_con) * S::NFQ, 256)), dim3(256), 0, l.stream, l.m);
     return hipGetLastError();
   }
+  // artificial-viscosity generator
+  static Av_consts<RS> av_consts(const Launch& l, int av_rs)
+  {
+    const Ops<RS>& o = hops(l);
+    Av_consts<RS> c;
+    for (int i = 0; i < RS; ++i) {
+      c.node[i] = o.node[i];
+      c.adv_node[i] = 2 * o.node[i] - 1.;
+      c.weight[i] = o.weight[i];
+      c.orth[i] = (av_rs >= 1 && av_rs <= RS) ? o.orth[av_rs - 1][i] : 0.;
+    }
+    return c;
+  }
+  static hipError_t av_write_face(const Launch& l, int cls, int kind)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    const Eops<RS, double> o = make_eops<RS, double>(hops(l));
+    dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
+    if (kind == av_kind_adv) hipLaunchKernelGGL((k_av_write_face<ND, RS, av_kind_adv>), g, b, 0, l.stream, l.m, o, cls);
+    else hipLaunchKernelGGL((k_av_write_face<ND, RS, av_kind_sav>), g, b, 0, l.stream, l.m, o, cls);
+    return hipGetLastError();
+  }
+  static hipError_t av_neighbor(const Launch& l, int cls, int kind)
+  {
+    const int n = l.m.n_con[cls];
+    if (n == 0) return hipSuccess;
+    dim3 g(blocks_for(size_t(n) * S::NFQ, 256)), b(256);
+    const Av_consts<RS> c = av_consts(l, RS);
+    if (cls == HX_DEFORMED) {
+      if (kind == av_kind_adv) hipLaunchKernelGGL((k_av_neighbor<ND, RS, true, av_kind_adv>), g, b, 0, l.stream, l.m, c);
+      else hipLaunchKernelGGL((k_av_neighbor<ND, RS, true, av_kind_sav>), g, b, 0, l.stream, l.m, c);
+    } else {
+      if (kind == av_kind_adv) hipLaunchKernelGGL((k_av_neighbor<ND, RS, false, av_kind_adv>), g, b, 0, l.stream, l.m, c);
+      else hipLaunchKernelGGL((k_av_neighbor<ND, RS, false, av_kind_sav>), g, b, 0, l.stream, l.m, c);
+    }
+    return hipGetLastError();
+  }
+  static hipError_t adv_local(const Launch& l, int cls, double coef, int stage, int cr, double advect_length)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    const Eops<RS, double> o = make_eops<RS, double>(hops(l));
+    const Av_consts<RS> c = av_consts(l, RS);
+    dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
+    if (cls == HX_DEFORMED)
+      hipLaunchKernelGGL((k_adv_local<ND, RS, true>), g, b, 0, l.stream, l.m, o, c, coef, stage, cr, advect_length);
+    else hipLaunchKernelGGL((k_adv_local<ND, RS, false>), g, b, 0, l.stream, l.m, o, c, coef, stage, cr, advect_length);
+    return hipGetLastError();
+  }
+  static hipError_t sav_local(const Launch& l, int cls, double dt, double diff_time)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    const Eops<RS, double> o = make_eops<RS, double>(hops(l));
+    dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
+    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_sav_local<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, diff_time);
+    else hipLaunchKernelGGL((k_sav_local<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, diff_time);
+    return hipGetLastError();
+  }
+  static hipError_t sav_neighbor_reconcile(const Launch& l, int cls)
+  {
+    const int n = l.m.n_con[cls];
+    if (n == 0) return hipSuccess;
+    dim3 g(blocks_for(size_t(n) * S::NFQ, 256)), b(256);
+    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_neighbor_reconcile<ND, RS, true, 3>), g, b, 0, l.stream, l.m);
+    else hipLaunchKernelGGL((k_neighbor_reconcile<ND, RS, false, 3>), g, b, 0, l.stream, l.m);
+    return hipGetLastError();
+  }
+  static hipError_t sav_reconcile(const Launch& l, int cls, double dt, double diff_time, double cheby)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    const Eops<RS, double> o = make_eops<RS, double>(hops(l));
+    dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
+    if (cls == HX_DEFORMED)
+      hipLaunchKernelGGL((k_sav_reconcile<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, diff_time, cheby);
+    else hipLaunchKernelGGL((k_sav_reconcile<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, diff_time, cheby);
+    return hipGetLastError();
+  }
+  static hipError_t av_max_dt(const Launch& l, int cls, int smooth, int is_local, double cc, double cd)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    const Av_consts<RS> c = av_consts(l, RS);
+    dim3 g(std::min(blocks_for(size_t(n) * S::NQ, 256), unsigned(max_dt_blocks))), b(256);
+    if (smooth) hipLaunchKernelGGL((k_av_max_dt<ND, RS, true>), g, b, 0, l.stream, l.m, c, cls, is_local, cc, cd, l.scalar);
+    else hipLaunchKernelGGL((k_av_max_dt<ND, RS, false>), g, b, 0, l.stream, l.m, c, cls, is_local, cc, cd, l.scalar);
+    return hipGetLastError();
+  }
+  static hipError_t av_bcs(const Launch& l, int mode)
+  {
+    if (l.m.n_bc_con == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_av_bcs<ND, RS>), dim3(blocks_for(size_t(l.m.n_bc_con) * S::NFQ, 256)), dim3(256), 0, l.stream,
+                       l.m, mode);
+    return hipGetLastError();
+  }
+  static hipError_t av_points(const Launch& l, int cls, int what, int av_rs, double mult, double us_max, double* resid)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    const Av_consts<RS> c = av_consts(l, av_rs);
+    hipLaunchKernelGGL((k_av_points<ND, RS>), dim3(blocks_for(size_t(n) * S::NQ, 256)), dim3(256), 0, l.stream, l.m, c,
+                       cls, what, mult, us_max, resid);
+    return hipGetLastError();
+  }
   static const Kernel_set* set()
   {
-    static const Kernel_set s{&write_face, &neighbor, &local, &neighbor_reconcile, &reconcile_ldg_flux,
-                              &max_dt, &prolong, &restrict_, &state_bcs, &flux_bcs};
+    static const Kernel_set s{&write_face,  &neighbor,      &local,     &neighbor_reconcile, &reconcile_ldg_flux,
+                              &max_dt,      &prolong,       &restrict_, &state_bcs,          &flux_bcs,
+                              &av_write_face, &av_neighbor, &adv_local, &sav_local,          &sav_neighbor_reconcile,
+                              &sav_reconcile, &av_max_dt,   &av_bcs,    &av_points};
     return &s;
   }
 };

@@ -32,6 +32,16 @@ struct Kernel_set
   hipError_t (*restrict_)(const Launch&, int n_var, int scale, int offset);
   hipError_t (*state_bcs)(const Launch&);
   hipError_t (*flux_bcs)(const Launch&);
+  // artificial-viscosity generator (kind: HX_PDE_ADVECTION or HX_PDE_SMOOTH_AV)
+  hipError_t (*av_write_face)(const Launch&, int cls, int kind);
+  hipError_t (*av_neighbor)(const Launch&, int cls, int kind);
+  hipError_t (*adv_local)(const Launch&, int cls, double coef, int stage, int compute_residual, double advect_length);
+  hipError_t (*sav_local)(const Launch&, int cls, double dt, double diff_time);
+  hipError_t (*sav_neighbor_reconcile)(const Launch&, int cls);
+  hipError_t (*sav_reconcile)(const Launch&, int cls, double dt, double diff_time, double cheby);
+  hipError_t (*av_max_dt)(const Launch&, int cls, int smooth, int is_local, double max_cfl_c, double max_cfl_d);
+  hipError_t (*av_bcs)(const Launch&, int mode);
+  hipError_t (*av_points)(const Launch&, int cls, int what, int av_rs, double mult, double us_max, double* resid);
 };
 
 const Kernel_set* kernel_set_nd1(int rs);

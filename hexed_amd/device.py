@@ -12,7 +12,7 @@ from . import abi
 FIELDS = {
     "state": abi.HX_FIELD_STATE, "rcache": abi.HX_FIELD_RCACHE, "tss": abi.HX_FIELD_TSS, "av": abi.HX_FIELD_AV,
     "vertex_tss": abi.HX_FIELD_VERTEX_TSS, "metrics": abi.HX_FIELD_METRICS, "faces": abi.HX_FIELD_FACES,
-    "normals": abi.HX_FIELD_NORMALS,
+    "normals": abi.HX_FIELD_NORMALS, "forcing": abi.HX_FIELD_FORCING, "advection": abi.HX_FIELD_ADVECTION,
 }
 
 
@@ -171,6 +171,28 @@ class Device_mesh:
 
     def compute_write_face(self):
         self.ctx.check(self.lib.hx_compute_write_face(self.ptr), "compute_write_face")
+
+    # artificial-viscosity generator --------------------------------------------------------------------------
+    def update_art_visc_smoothness(self, av_opts):
+        r = ctypes.c_double()
+        self.ctx.check(self.lib.hx_update_art_visc_smoothness(self.ptr, ctypes.byref(av_opts), ctypes.byref(r)),
+                       "update_art_visc_smoothness")
+        return r.value
+
+    def compute_advection(self, dt, i_stage, advect_length):
+        o = abi.hx_kernel_options(dt, i_stage, 0, 0)
+        self.ctx.check(self.lib.hx_compute_advection(self.ptr, ctypes.byref(o), advect_length), "compute_advection")
+
+    def compute_smooth_av(self, dt, diff_time, cheby_step):
+        o = abi.hx_kernel_options(dt, 0, 0, 0)
+        self.ctx.check(self.lib.hx_compute_smooth_av(self.ptr, ctypes.byref(o), diff_time, cheby_step),
+                       "compute_smooth_av")
+
+    def apply_advection_bcs(self):
+        self.ctx.check(self.lib.hx_apply_advection_bcs(self.ptr), "advection bcs")
+
+    def apply_avc_diff_bcs(self):
+        self.ctx.check(self.lib.hx_apply_avc_diff_bcs(self.ptr), "avc diff bcs")
 
     def update(self, opts, status=None):
         status = status if status is not None else abi.hx_update_status()

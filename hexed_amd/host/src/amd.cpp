@@ -188,7 +188,7 @@ void Accelerated_mesh::build_topology()
   for (int c = 0; c < 2; ++c) {
     const size_t n = elems[c].size();
     state[c].assign(n * nv * nq, 0.);
-    rcache[c].assign(n * nv * nq, 0.);
+    rcache[c].assign(n * std::max(nv, rs) * nq, 0.);
     tss[c].assign(n * nq, 1.);
     av[c].assign(ref_layout ? n * 2 * nq : 0, 0.);
     nom[c].resize(n);
@@ -224,7 +224,8 @@ void Accelerated_mesh::gather()
       const double* st = el.state();
       std::copy(st, st + nv * nq, state[c].begin() + e * nv * nq);
       const double* rc = el.residual_cache();
-      std::copy(rc, rc + nv * nq, rcache[c].begin() + e * nv * nq);
+      const size_t nrc = std::max(nv, rs) * nq;  // residual cache: max(nv, rs) variables (Storage_params.cpp:32-35)
+      std::copy(rc, rc + nrc, rcache[c].begin() + e * nrc);
       const double* ts = el.time_step_scale();
       std::copy(ts, ts + nq, tss[c].begin() + e * nq);
       if (ref_layout) std::copy(st + (nv + 1) * nq, st + (nv + 3) * nq, av[c].begin() + e * 2 * nq);
@@ -237,7 +238,7 @@ void Accelerated_mesh::gather()
     }
   }
   for (size_t s = 0; s < slot_ptr.size(); ++s)
-    if (slot_ptr[s]) std::copy(slot_ptr[s], slot_ptr[s] + 2 * nv * nfq, faces.begin() + s * face_sz);
+    if (slot_ptr[s]) std::copy(slot_ptr[s], slot_ptr[s] + face_sz, faces.begin() + s * face_sz);
   for (size_t s = 0; s < nslot_ptr.size(); ++s)
     std::copy(nslot_ptr[s], nslot_ptr[s] + nd * nfq, normals.begin() + s * nd * nfq);
 }
@@ -248,12 +249,13 @@ void Accelerated_mesh::scatter()
     for (size_t e = 0; e < elems[c].size(); ++e) {
       Kernel_element& el = *elems[c][e];
       std::copy(state[c].begin() + e * nv * nq, state[c].begin() + (e + 1) * nv * nq, el.state());
-      std::copy(rcache[c].begin() + e * nv * nq, rcache[c].begin() + (e + 1) * nv * nq, el.residual_cache());
+      const size_t nrc = std::max(nv, rs) * nq;
+      std::copy(rcache[c].begin() + e * nrc, rcache[c].begin() + (e + 1) * nrc, el.residual_cache());
       std::copy(tss[c].begin() + e * nq, tss[c].begin() + (e + 1) * nq, el.time_step_scale());
     }
   }
   for (size_t s = 0; s < slot_ptr.size(); ++s)
-    if (slot_ptr[s]) std::copy(faces.begin() + s * face_sz, faces.begin() + s * face_sz + 2 * nv * nfq, slot_ptr[s]);
+    if (slot_ptr[s]) std::copy(faces.begin() + s * face_sz, faces.begin() + (s + 1) * face_sz, slot_ptr[s]);
 }
 
 hx_host_mesh Accelerated_mesh::description()

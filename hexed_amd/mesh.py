@@ -71,9 +71,12 @@ class HostMesh:
         self.face_sz = max(2 * self.nv, nd + rs) * self.nfq
         self.n_elem = [n_car, n_def]
         self.state = [np.zeros((n, self.nv, self.nq)) for n in self.n_elem]
-        self.rcache = [np.zeros((n, self.nv, self.nq)) for n in self.n_elem]
+        self.nrc = max(self.nv, rs)  # residual-cache variables (Storage_params.cpp:32-35)
+        self.rcache = [np.zeros((n, self.nrc, self.nq)) for n in self.n_elem]
         self.tss = [np.ones((n, self.nq)) for n in self.n_elem]
         self.av = [np.zeros((n, 2, self.nq)) for n in self.n_elem] if with_av else [None, None]
+        self.forcing = [None, None]    # [n][4][nq] artificial-viscosity forcing (Element.cpp:134)
+        self.advection = [None, None]  # [n][rs][nq] advected scalars of the smoothness indicator (Element.cpp:139)
         self.nom_size = [np.zeros(n) for n in self.n_elem]
         self.vertex_tss = [np.zeros((n, self.nvert)) for n in self.n_elem]
         self.face_slot = [np.zeros((n, 2 * nd), np.int32) for n in self.n_elem]
@@ -91,6 +94,16 @@ class HostMesh:
         self.bcs = []
         # geometry kept for initial conditions: qpoint positions per class [n][nd][nq]
         self.qpos = [None, None]
+
+    def enable_av_generator(self):
+        """Allocate the arrays Solver::update_art_visc_smoothness works on (zeros, as Element.cpp:134-142 does)."""
+        for c in range(2):
+            n = self.n_elem[c]
+            if self.av[c] is None:
+                self.av[c] = np.zeros((n, 2, self.nq))
+            self.forcing[c] = np.zeros((n, 4, self.nq))
+            self.advection[c] = np.zeros((n, self.row_size, self.nq))
+        return self
 
     @property
     def n_dof(self):
@@ -116,6 +129,8 @@ class HostMesh:
             m.rcache[c] = abi.dptr(self.rcache[c])
             m.tss[c] = abi.dptr(self.tss[c])
             m.av[c] = abi.dptr(self.av[c])
+            m.forcing[c] = abi.dptr(self.forcing[c])
+            m.advection[c] = abi.dptr(self.advection[c])
             m.nom_size[c] = abi.dptr(self.nom_size[c])
             m.vertex_tss[c] = abi.dptr(self.vertex_tss[c])
             m.face_slot[c] = abi.iptr(self.face_slot[c])

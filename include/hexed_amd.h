@@ -114,7 +114,8 @@ typedef struct hx_host_mesh {
   int row_size;
   int n_elem[2];
   double* state[2];       /* [n][nv][nq] conserved state: momentum[nd], density, energy       */
-  double* rcache[2];      /* [n][nv][nq] stage-0 residual cache (Element.cpp:188)            */
+  double* rcache[2];      /* [n][max(nv,rs)][nq] stage-0 residual cache (Element.cpp:188; the
+                             Advection PDE of the AV generator caches rs variables, Storage_params.cpp:32-35) */
   double* tss[2];         /* [n][nq] local time step scale (Element.cpp:119)                  */
   double* av[2];          /* [n][2][nq] bulk and Laplacian artificial viscosity; NULL => zero */
   double* forcing[2];     /* [n][4][nq] AV forcing (Element.cpp:134); NULL unless AV driver   */
@@ -177,6 +178,23 @@ typedef struct hx_update_options {
   hx_transport visc;
   hx_transport cond;
 } hx_update_options;
+
+/* Settings of Solver::update_art_visc_smoothness (Solver.cpp:457-582); the reference's defaults are at
+ * Solver.cpp:144,154-163. The mesh needs its forcing and advection arrays (hx_host_mesh::forcing, ::advection). */
+typedef struct hx_av_options {
+  double advect_length;      /* argument of update_art_visc_smoothness                               */
+  int advect_iters;          /* av_advect_iters, default 1                                           */
+  double advect_max_safety;  /* av_advect_max_safety, default 0.7                                    */
+  double diff_ratio;         /* av_diff_ratio, default 0.3                                           */
+  double diff_max_safety;    /* av_diff_max_safety, default 0.7                                      */
+  int n_cheby;               /* n_cheby_av, default 1                                                */
+  int diff_iters;            /* av_diff_iters, default 1                                             */
+  double visc_mult;          /* av_visc_mult, default 1e2                                            */
+  double unscaled_max;       /* av_unscaled_max, default 5                                           */
+  double freestream_mass;    /* freestream<n_dim>   (density)                                        */
+  double freestream_energy;  /* freestream<n_dim+1> (total energy)                                   */
+  int av_rs;                 /* Solver::av_rs: degree + 1 of the Legendre projection, default row_size */
+} hx_av_options;
 
 /* Status after an update (Iteration_status fields touched by Solver::update, Solver.cpp:872-881). */
 typedef struct hx_update_status {
@@ -291,6 +309,27 @@ hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset);
 hx_status hx_update(hx_mesh* mesh, const hx_update_options* opts, hx_update_status* status);
 /* Solver::compute_residual (Solver.cpp:899-913): writes the scaled residual into the residual cache. */
 hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* opts);
+
+/* ---- artificial-viscosity generator (the per-kernel entry points above also take HX_PDE_ADVECTION and
+ * HX_PDE_SMOOTH_AV; the mesh needs its av, forcing and advection fields) ---------------------------------- */
+hx_status hx_apply_advection_bcs(hx_mesh* mesh);                 /* Solver.cpp:508-514, Flow_bc::apply_advection */
+hx_status hx_apply_avc_diff_bcs(hx_mesh* mesh);                  /* Solver.cpp:83-91     */
+hx_status hx_apply_avc_diff_flux_bcs(hx_mesh* mesh);             /* Solver.cpp:93-101    */
+hx_status hx_compute_advection(hx_mesh* mesh, const hx_kernel_options* opts, double advect_length);
+                                                                 /* kernels_convective.cpp:8-19 (compute_advection) */
+hx_status hx_compute_smooth_av(hx_mesh* mesh, const hx_kernel_options* opts, double diff_time, double cheby_step);
+                                                                 /* kernels_diffusive.cpp:8-31 (compute_smooth_av); the
+                                                                    flux_bc callback is hx_apply_avc_diff_flux_bcs */
+hx_status hx_compute_write_face_advection(hx_mesh* mesh, double advect_length);  /* kernels_convective.cpp:48-51 */
+hx_status hx_compute_write_face_smooth_av(hx_mesh* mesh);                         /* kernels_convective.cpp:53-56 */
+hx_status hx_compute_prolong_advection(hx_mesh* mesh, double advect_length);     /* kernels_convective.cpp:38-41 */
+hx_status hx_max_dt_advection(hx_mesh* mesh, const hx_kernel_options* opts, double msc, double msd, int local_time,
+                              double advect_length, double* dt_out);             /* kernels_max_dt.cpp:13-14 */
+hx_status hx_max_dt_smooth_av(hx_mesh* mesh, const hx_kernel_options* opts, double msc, double msd, int local_time,
+                              double* dt_out);                                   /* kernels_max_dt.cpp:15 */
+/* Solver::update_art_visc_smoothness (Solver.cpp:457-582): writes the bulk coefficient av[.][0] and, if `residual`
+ * is not NULL, the art_visc_residual it assigns (Solver.cpp:575). */
+hx_status hx_update_art_visc_smoothness(hx_mesh* mesh, const hx_av_options* opts, double* residual);
 
 #ifdef __cplusplus
 }

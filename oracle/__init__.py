@@ -45,6 +45,12 @@ def load():
         "oracle_max_dt_navier_stokes": (I, [M, KO, D, D, I, abi.hx_transport, abi.hx_transport, ctypes.POINTER(D)]),
         "oracle_update": (I, [M, ctypes.POINTER(abi.hx_update_options), ctypes.POINTER(abi.hx_update_status)]),
         "oracle_compute_residual": (I, [M, ctypes.POINTER(abi.hx_update_options)]),
+        "oracle_apply_advection_bcs": (I, [M]),
+        "oracle_apply_avc_diff_bcs": (I, [M]),
+        "oracle_apply_avc_diff_flux_bcs": (I, [M]),
+        "oracle_compute_advection": (I, [M, KO, D]),
+        "oracle_compute_smooth_av": (I, [M, KO, D, D]),
+        "oracle_update_art_visc_smoothness": (I, [M, ctypes.POINTER(abi.hx_av_options), ctypes.POINTER(D)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

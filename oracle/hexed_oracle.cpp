@@ -564,7 +564,7 @@ struct Ops
       // modal filter (Spatial.hpp:472-481) over both halves of time_rate
       if (use_filter) filter_rows<2 * nv>(basis, time_rate[0][0]);
       // update (Spatial.hpp:483-503)
-      double* ref_state = m->rcache[cls] + size_t(e) * nv * nq;
+      double* ref_state = m->rcache[cls] + size_t(e) * std::max(nv, rs) * nq;
       for (int q = 0; q < nq; ++q) {
         double update[nv] = {};
         double mult = update_coef * tss[q] / d_pos;
@@ -637,7 +637,7 @@ struct Ops
           R::template write_row<nv>(d, time_rate[0], i_dim, i_fq, 1.);
         }
       if (use_filter) filter_rows<nv>(basis, time_rate[0]);
-      double* to_update = compute_residual ? m->rcache[cls] + size_t(e) * nv * nq : state;
+      double* to_update = compute_residual ? m->rcache[cls] + size_t(e) * std::max(nv, rs) * nq : state;
       for (int q = 0; q < nq; ++q) {
         double mult = dt * tss[q] / d_pos;
         if (is_deformed) mult /= elem_det[q];
@@ -891,6 +891,533 @@ struct Ops
   }
 };
 
+// ---- artificial-viscosity generator -------------------------------------------------------------------------
+// pde::Advection (pde.hpp:265-349) and pde::Smooth_art_visc (pde.hpp:355-431) through the Spatial kernels, their
+// boundary hooks (Boundary_condition.cpp:12-60,346-367) and Solver::update_art_visc_smoothness / diffuse_art_visc
+// (Solver.cpp:428-582). Element data: the Advection state is the momentum slots of `state` (holding velocity while
+// the generator runs) followed by the rs `advection` arrays; Smooth_art_visc works on the 4 `forcing` arrays.
+template <int nd, int rs>
+struct Av
+{
+  static constexpr int nq = ipow(rs, nd), nfq = nq / rs, nv = nd + 2, nvert = 1 << nd;
+  static constexpr int n_ext = nd + rs;  // Advection: n_extrap (velocity + rs advected scalars); n_update = rs
+  static constexpr int n_sav = 3;        // Smooth_art_visc: n_extrap = n_update = 3
+  using R = Rows<nd, rs>;
+  using O = Ops<nd, rs>;
+
+  static void element_faces(hx_host_mesh* m, const Mesh_view& mv, int cls, int e, double* (&faces)[6])
+  {
+    for (int f = 0; f < 2 * nd; ++f) faces[f] = mv.slot(m->face_slot[cls][e * 2 * nd + f]);
+  }
+
+  // Write_face (Spatial.hpp:41-57) of nvar gathered [nvar][nq] arrays
+  template <int nvar>
+  static void write_face_vars(const Basis& basis, const double (&ext)[nvar][nq], double* const* faces)
+  {
+    for (int i_dim = 0; i_dim < nd; ++i_dim)
+      for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+        double row[rs][nvar];
+        R::template read_row<nvar>(ext[0], i_dim, i_fq, row);
+        double b[2][nvar];
+        for (int s = 0; s < 2; ++s)
+          for (int v = 0; v < nvar; ++v) {
+            double acc = 0;
+            for (int k = 0; k < rs; ++k) acc += basis.bound[s * rs + k] * row[k][v];
+            b[s][v] = acc;
+          }
+        R::template write_bound<nvar>(b, faces, i_dim, i_fq);
+      }
+  }
+
+  // Advection::fetch_extrap (pde.hpp:284-290)
+  static void adv_gather(const hx_host_mesh* m, int cls, int e, double (&ext)[n_ext][nq])
+  {
+    const double* st = m->state[cls] + size_t(e) * nv * nq;
+    const double* adv = m->advection[cls] + size_t(e) * rs * nq;
+    for (int j = 0; j < nd; ++j)
+      for (int q = 0; q < nq; ++q) ext[j][q] = st[j * nq + q];
+    for (int i = 0; i < rs; ++i)
+      for (int q = 0; q < nq; ++q) ext[nd + i][q] = adv[i * nq + q];
+  }
+
+  // Smooth_art_visc::fetch_extrap (pde.hpp:373-378): forcing[1..3]
+  static void sav_gather(const hx_host_mesh* m, int cls, int e, double (&ext)[n_sav][nq])
+  {
+    const double* f = m->forcing[cls] + size_t(e) * 4 * nq;
+    for (int i = 0; i < n_sav; ++i)
+      for (int q = 0; q < nq; ++q) ext[i][q] = f[(1 + i) * nq + q];
+  }
+
+  static void adv_write_face(hx_host_mesh* m, int cls)
+  {
+    Basis basis(rs);
+    Mesh_view mv{m};
+    #pragma omp parallel for
+    for (int e = 0; e < m->n_elem[cls]; ++e) {
+      double ext[n_ext][nq];
+      adv_gather(m, cls, e, ext);
+      double* faces[6];
+      element_faces(m, mv, cls, e, faces);
+      write_face_vars<n_ext>(basis, ext, faces);
+    }
+  }
+
+  static void sav_write_face(hx_host_mesh* m, int cls)
+  {
+    Basis basis(rs);
+    Mesh_view mv{m};
+    #pragma omp parallel for
+    for (int e = 0; e < m->n_elem[cls]; ++e) {
+      double ext[n_sav][nq];
+      sav_gather(m, cls, e, ext);
+      double* faces[6];
+      element_faces(m, mv, cls, e, faces);
+      write_face_vars<n_sav>(basis, ext, faces);
+    }
+  }
+
+  // Advection nodes 2 x - 1 (pde.hpp:279-281)
+  static void adv_nodes(const Basis& basis, double (&x)[rs])
+  {
+    for (int i = 0; i < rs; ++i) x[i] = 2 * basis.node[i] - 1.;
+  }
+
+  // Neighbor<Advection> (Spatial.hpp:613-704 with Advection::Computation, pde.hpp:319-347)
+  static void adv_neighbor(hx_host_mesh* m, int cls)
+  {
+    const bool is_deformed = cls == HX_DEFORMED;
+    Basis basis(rs);
+    double x[rs];
+    adv_nodes(basis, x);
+    Mesh_view mv{m};
+    #pragma omp parallel for
+    for (int i_con = 0; i_con < m->n_con[cls]; ++i_con) {
+      Dir dir = unpack_dir(m->con_dir[cls][i_con]);
+      double* side[2] = {mv.slot(m->con_slot[cls][2 * i_con]), mv.slot(m->con_slot[cls][2 * i_con + 1])};
+      double face[2][n_ext * nfq];
+      double face_nrml[nd * nfq];
+      int sign[2] = {1, 1};
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < n_ext * nfq; ++i) face[s][i] = side[s][i];
+      Face_permutation<nd, rs> perm{dir, face[1], n_ext};
+      if (is_deformed) {
+        perm.match_faces();
+        for (int s = 0; s < 2; ++s) sign[s] = 1 - 2 * dir.flip_normal(s);
+        const double* nn = mv.nslot(m->con_nslot[2 * i_con]);
+        for (int i = 0; i < nd * nfq; ++i) face_nrml[i] = nn[i];
+      }
+      for (int q = 0; q < nfq; ++q) {
+        double n[nd];
+        for (int j = 0; j < nd; ++j) n[j] = is_deformed ? sign[0] * face_nrml[j * nfq + q] : double(j == dir.i_dim[0]);
+        double nvel[2], cs[2], a[2][rs];
+        for (int s = 0; s < 2; ++s) {
+          double vn = 0, vsq = 0;
+          for (int j = 0; j < nd; ++j) {
+            const double v = face[s][j * nfq + q];
+            vn += v * n[j];
+            vsq += v * v;
+          }
+          nvel[s] = vn;
+          cs[s] = std::max(1., std::sqrt(vsq));
+          for (int i = 0; i < rs; ++i) a[s][i] = face[s][(nd + i) * nfq + q];
+        }
+        double nsq = 0;
+        for (int j = 0; j < nd; ++j) nsq += n[j] * n[j];
+        const double coef = std::max(cs[0], cs[1]) * std::sqrt(nsq);
+        for (int i = 0; i < rs; ++i) {
+          const double f = .5 * (x[i] * nvel[0] * a[0][i] + x[i] * nvel[1] * a[1][i] + coef * (a[0][i] - a[1][i]));
+          for (int s = 0; s < 2; ++s) face[s][i * nfq + q] = sign[s] * f;
+        }
+      }
+      if (is_deformed) perm.restore();
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < rs * nfq; ++i) side[s][i] = face[s][i];
+    }
+  }
+
+  // Neighbor<Smooth_art_visc>: diffusion only, the LDG average of the 3 extrapolated variables
+  static void sav_neighbor(hx_host_mesh* m, int cls)
+  {
+    const bool is_deformed = cls == HX_DEFORMED;
+    Mesh_view mv{m};
+    #pragma omp parallel for
+    for (int i_con = 0; i_con < m->n_con[cls]; ++i_con) {
+      Dir dir = unpack_dir(m->con_dir[cls][i_con]);
+      double* side[2] = {mv.slot(m->con_slot[cls][2 * i_con]), mv.slot(m->con_slot[cls][2 * i_con + 1])};
+      double face[4][n_sav * nfq];
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < n_sav * nfq; ++i) face[s][i] = side[s][i];
+      Face_permutation<nd, rs> perm{dir, face[1], n_sav};
+      if (is_deformed) perm.match_faces();
+      for (int q = 0; q < nfq; ++q)
+        for (int v = 0; v < n_sav; ++v) {
+          const double avg = .5 * (face[0][v * nfq + q] + face[1][v * nfq + q]);
+          for (int s = 0; s < 2; ++s) face[2 + s][v * nfq + q] = avg;
+        }
+      if (is_deformed) {
+        perm.restore();
+        Face_permutation<nd, rs>{dir, face[3], n_sav}.restore();
+      }
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < n_sav * nfq; ++i) side[s][nv * nfq + i] = face[2 + s][i];
+    }
+  }
+
+  // Neighbor_reconcile<Smooth_art_visc> (Spatial.hpp:716-759) on the 3 LDG variables
+  static void sav_neighbor_reconcile(hx_host_mesh* m, int cls)
+  {
+    const bool is_deformed = cls == HX_DEFORMED;
+    Mesh_view mv{m};
+    #pragma omp parallel for
+    for (int i_con = 0; i_con < m->n_con[cls]; ++i_con) {
+      Dir dir = unpack_dir(m->con_dir[cls][i_con]);
+      double* side[2] = {mv.slot(m->con_slot[cls][2 * i_con]) + nv * nfq,
+                         mv.slot(m->con_slot[cls][2 * i_con + 1]) + nv * nfq};
+      double face[2][n_sav * nfq];
+      int sign[2] = {1, 1};
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < n_sav * nfq; ++i) face[s][i] = side[s][i];
+      Face_permutation<nd, rs> perm{dir, face[1], n_sav};
+      if (is_deformed) {
+        perm.match_faces();
+        for (int s = 0; s < 2; ++s) sign[s] = 1 - 2 * dir.flip_normal(s);
+      }
+      for (int q = 0; q < nfq; ++q)
+        for (int v = 0; v < n_sav; ++v) {
+          double avg = 0;
+          for (int s = 0; s < 2; ++s) avg += .5 * sign[s] * face[s][v * nfq + q];
+          for (int s = 0; s < 2; ++s) {
+            double& f = face[s][v * nfq + q];
+            f = sign[s] * avg - f;
+          }
+        }
+      if (is_deformed) perm.restore();
+      for (int s = 0; s < 2; ++s)
+        for (int i = 0; i < n_sav * nfq; ++i) side[s][i] = face[s][i];
+    }
+  }
+
+  // Local<Advection> (Spatial.hpp:326-509): convective rate, the constant source 2/advect_length in stage 0, and
+  // Advection::write_update (pde.hpp:292-300), which divides by the implicit pseudo-time factor 1 + tss 2/L
+  static void adv_local(hx_host_mesh* m, int cls, double dt, int stage, bool compute_residual, double advect_length)
+  {
+    const bool is_deformed = cls == HX_DEFORMED;
+    Basis basis(rs);
+    Derivative<rs> derivative(basis);
+    double x[rs];
+    adv_nodes(basis, x);
+    const double update_coef = stage ? dt * basis.step_ratio() : dt;
+    Mesh_view mv{m};
+    #pragma omp parallel for
+    for (int e = 0; e < m->n_elem[cls]; ++e) {
+      double* state = m->state[cls] + size_t(e) * nv * nq;
+      double* adv = m->advection[cls] + size_t(e) * rs * nq;
+      double* faces[6];
+      element_faces(m, mv, cls, e, faces);
+      double* tss = m->tss[cls] + size_t(e) * nq;
+      const double d_pos = m->nom_size[cls][e];
+      const double* nrml = is_deformed ? m->metrics + size_t(e) * (nd * nd + 1) * nq : nullptr;
+      const double* elem_det = is_deformed ? nrml + nd * nd * nq : nullptr;
+      double time_rate[2][rs][nq] = {};
+      double flux[nd][rs][nq];
+      for (int q = 0; q < nq; ++q) {
+        for (int i_dim = 0; i_dim < nd; ++i_dim) {
+          double vn = 0;
+          for (int j = 0; j < nd; ++j)
+            vn += state[j * nq + q] * (is_deformed ? nrml[(i_dim * nd + j) * nq + q] : double(j == i_dim));
+          for (int i = 0; i < rs; ++i) flux[i_dim][i][q] = x[i] * vn * adv[i * nq + q];
+        }
+        if (!stage) {
+          double mult = d_pos;
+          if (is_deformed) mult *= elem_det[q];
+          for (int i = 0; i < rs; ++i) time_rate[1][i][q] = mult * (2 / advect_length);
+        }
+      }
+      for (int i_dim = 0; i_dim < nd; ++i_dim)
+        for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+          double row_f[rs][rs], face_f[2][rs], d[rs][rs];
+          R::template read_row<rs>(flux[i_dim][0], i_dim, i_fq, row_f);
+          R::template read_bound<rs>(faces, i_dim, i_fq, face_f);
+          derivative.template apply<rs>(row_f, face_f, d);
+          for (int k = 0; k < rs; ++k)
+            for (int v = 0; v < rs; ++v) d[k][v] = -d[k][v];
+          R::template write_row<rs>(d, time_rate[0][0], i_dim, i_fq, 1.);
+        }
+      double* ref_state = m->rcache[cls] + size_t(e) * std::max(nv, rs) * nq;
+      for (int q = 0; q < nq; ++q) {
+        double update[rs] = {};
+        double mult = update_coef * tss[q] / d_pos;
+        if (is_deformed) mult /= elem_det[q];
+        for (int i = 0; i < rs; ++i) {
+          double u = time_rate[0][i][q];
+          if (stage) {
+            u -= ref_state[i * nq + q];
+          } else {
+            ref_state[i * nq + q] = u;
+            u += time_rate[1][i][q];
+          }
+          u *= mult;
+          if (compute_residual) ref_state[i * nq + q] = u;
+          else update[i] = u;
+        }
+        const double pseudo = 1 + tss[q] * 2 / advect_length;
+        for (int i = 0; i < rs; ++i) {
+          double& a = adv[i * nq + q];
+          if (!stage) a = (a + update[i]) / pseudo;
+          else a += update[i] / pseudo;
+        }
+      }
+      double ext[n_ext][nq];
+      adv_gather(m, cls, e, ext);
+      write_face_vars<n_ext>(basis, ext, faces);
+    }
+  }
+
+  // Local<Smooth_art_visc> (Spatial.hpp:326-509, stage 0): LDG gradient of forcing[1..3], flux -gradient*normal
+  // (pde.hpp:405-410), the chained source of pde.hpp:419-427, and an additive update (write_update, pde.hpp:380-388)
+  static void sav_local(hx_host_mesh* m, int cls, double dt, double diff_time)
+  {
+    const bool is_deformed = cls == HX_DEFORMED;
+    Basis basis(rs);
+    Derivative<rs> derivative(basis);
+    Mesh_view mv{m};
+    static double cartesian_normal[nd][nd][nfq];
+    for (int i = 0; i < nd; ++i)
+      for (int j = 0; j < nd; ++j)
+        for (int q = 0; q < nfq; ++q) cartesian_normal[i][j][q] = i == j;
+    #pragma omp parallel for
+    for (int e = 0; e < m->n_elem[cls]; ++e) {
+      double* forcing = m->forcing[cls] + size_t(e) * 4 * nq;
+      double* faces[6];
+      double* visc_faces[6];
+      element_faces(m, mv, cls, e, faces);
+      for (int f = 0; f < 2 * nd; ++f) visc_faces[f] = faces[f] + nv * nfq;
+      double* tss = m->tss[cls] + size_t(e) * nq;
+      const double d_pos = m->nom_size[cls][e];
+      const double* nrml = is_deformed ? m->metrics + size_t(e) * (nd * nd + 1) * nq : nullptr;
+      const double* elem_det = is_deformed ? nrml + nd * nd * nq : nullptr;
+      double* face_nrml[6] = {};
+      if (is_deformed)
+        for (int f = 0; f < 2 * nd; ++f) {
+          int ns = m->face_nslot[e * 2 * nd + f];
+          face_nrml[f] = ns >= 0 ? mv.nslot(ns) : cartesian_normal[f / 2][0];
+        }
+      double time_rate[2][n_sav][nq] = {};
+      double visc_storage[nd][n_sav][nq] = {};
+      for (int q = 0; q < nq; ++q)
+        for (int v = 0; v < n_sav; ++v) time_rate[0][v][q] = forcing[(1 + v) * nq + q];
+      for (int i_dim = 0; i_dim < nd; ++i_dim)
+        for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+          double row_r[rs][n_sav], face_state[2][n_sav];
+          R::template read_row<n_sav>(time_rate[0][0], i_dim, i_fq, row_r);
+          R::template read_bound<n_sav>(visc_faces, i_dim, i_fq, face_state);
+          if (is_deformed) {
+            double row_n[rs][nd], face_n[2][nd];
+            R::template read_row<nd>(nrml + i_dim * nd * nq, i_dim, i_fq, row_n);
+            R::template read_bound<nd>(face_nrml, i_dim, i_fq, face_n);
+            for (int j_dim = 0; j_dim < nd; ++j_dim)
+              for (int v = 0; v < n_sav; ++v) {
+                double row[rs][1], bound[2][1], d[rs][1];
+                for (int k = 0; k < rs; ++k) row[k][0] = row_n[k][j_dim] * row_r[k][v];
+                for (int s = 0; s < 2; ++s) bound[s][0] = face_n[s][j_dim] * face_state[s][v];
+                derivative.template apply<1>(row, bound, d);
+                for (int k = 0; k < rs; ++k) d[k][0] = d[k][0] / d_pos;
+                R::template write_row<1>(d, visc_storage[j_dim][v], i_dim, i_fq, 1.);
+              }
+          } else {
+            double d[rs][n_sav];
+            derivative.template apply<n_sav>(row_r, face_state, d);
+            for (int k = 0; k < rs; ++k)
+              for (int v = 0; v < n_sav; ++v) d[k][v] = d[k][v] / d_pos;
+            R::template write_row<n_sav>(d, visc_storage[i_dim][0], i_dim, i_fq, 0);
+          }
+        }
+      std::memset(time_rate, 0, sizeof(time_rate));
+      for (int q = 0; q < nq; ++q) {
+        double grad[n_sav][nd];
+        for (int i_dim = 0; i_dim < nd; ++i_dim)
+          for (int v = 0; v < n_sav; ++v) grad[v][i_dim] = visc_storage[i_dim][v][q];
+        if (is_deformed)
+          for (int v = 0; v < n_sav; ++v)
+            for (int i_dim = 0; i_dim < nd; ++i_dim) grad[v][i_dim] /= elem_det[q];
+        for (int i_dim = 0; i_dim < nd; ++i_dim)
+          for (int v = 0; v < n_sav; ++v) {
+            double acc = 0;
+            for (int j = 0; j < nd; ++j)
+              acc += -grad[v][j] * (is_deformed ? nrml[(i_dim * nd + j) * nq + q] : double(j == i_dim));
+            visc_storage[i_dim][v][q] = acc;
+          }
+        double mult = d_pos;
+        if (is_deformed) mult *= elem_det[q];
+        for (int v = 0; v < n_sav; ++v) {
+          const double f = std::abs(forcing[v * nq + q]);
+          time_rate[1][v][q] = mult * (((v == 1) ? std::sqrt(f) : f) / diff_time);
+        }
+      }
+      for (int i_dim = 0; i_dim < nd; ++i_dim)
+        for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+          double row_f[rs][n_sav], face_f[2][n_sav], d[rs][n_sav];
+          R::template read_row<n_sav>(visc_storage[i_dim][0], i_dim, i_fq, row_f);
+          for (int s = 0; s < 2; ++s)
+            for (int v = 0; v < n_sav; ++v) {
+              double acc = 0;
+              for (int k = 0; k < rs; ++k) acc += basis.bound[s * rs + k] * row_f[k][v];
+              face_f[s][v] = acc;
+            }
+          for (int v = 0; v < n_sav; ++v)
+            for (int s = 0; s < 2; ++s) visc_faces[i_dim * 2 + s][v * nfq + i_fq] = face_f[s][v];
+          derivative.template apply<n_sav>(row_f, d);
+          for (int k = 0; k < rs; ++k)
+            for (int v = 0; v < n_sav; ++v) d[k][v] = -d[k][v];
+          R::template write_row<n_sav>(d, time_rate[1][0], i_dim, i_fq, 1.);
+        }
+      for (int q = 0; q < nq; ++q) {
+        double mult = dt * tss[q] / d_pos;
+        if (is_deformed) mult /= elem_det[q];
+        for (int v = 0; v < n_sav; ++v) {
+          double u = time_rate[0][v][q];
+          u += time_rate[1][v][q];
+          u *= mult;
+          forcing[(1 + v) * nq + q] += u;
+        }
+      }
+    }
+  }
+
+  // Reconcile_ldg_flux<Smooth_art_visc> (Spatial.hpp:543-594): lift the flux jump, update with the pseudo-time
+  // division (critical write_update), extrapolate forcing[1..3] to the faces
+  static void sav_reconcile(hx_host_mesh* m, int cls, double dt, double diff_time, double cheby)
+  {
+    const bool is_deformed = cls == HX_DEFORMED;
+    Basis basis(rs);
+    Derivative<rs> derivative(basis);
+    Mesh_view mv{m};
+    #pragma omp parallel for
+    for (int e = 0; e < m->n_elem[cls]; ++e) {
+      double* forcing = m->forcing[cls] + size_t(e) * 4 * nq;
+      double* faces[6];
+      double* visc_faces[6];
+      element_faces(m, mv, cls, e, faces);
+      for (int f = 0; f < 2 * nd; ++f) visc_faces[f] = faces[f] + nv * nfq;
+      double* tss = m->tss[cls] + size_t(e) * nq;
+      const double d_pos = m->nom_size[cls][e];
+      const double* elem_det = is_deformed ? m->metrics + size_t(e) * (nd * nd + 1) * nq + nd * nd * nq : nullptr;
+      double time_rate[n_sav][nq] = {};
+      for (int i_dim = 0; i_dim < nd; ++i_dim)
+        for (int i_fq = 0; i_fq < nfq; ++i_fq) {
+          double bound_f[2][n_sav], d[rs][n_sav];
+          R::template read_bound<n_sav>(visc_faces, i_dim, i_fq, bound_f);
+          derivative.template boundary_term<n_sav>(bound_f, d);
+          for (int k = 0; k < rs; ++k)
+            for (int v = 0; v < n_sav; ++v) d[k][v] = -d[k][v];
+          R::template write_row<n_sav>(d, time_rate[0], i_dim, i_fq, 1.);
+        }
+      for (int q = 0; q < nq; ++q) {
+        double mult = dt * tss[q] / d_pos;
+        if (is_deformed) mult /= elem_det[q];
+        const double pseudo = 1 + tss[q] * cheby / diff_time;
+        for (int v = 0; v < n_sav; ++v) {
+          double& f = forcing[(1 + v) * nq + q];
+          f += time_rate[v][q] * mult;
+          f /= pseudo;
+        }
+      }
+      double ext[n_sav][nq];
+      sav_gather(m, cls, e, ext);
+      write_face_vars<n_sav>(basis, ext, faces);
+    }
+  }
+
+  // Max_dt (Spatial.hpp:784-828) for Advection (char_speed max(1, |v|), pde.hpp:341-345) or Smooth_art_visc
+  // (diffusivity 1, pde.hpp:412-416)
+  static double max_dt(hx_host_mesh* m, int cls, bool smooth, bool is_local, double safety_conv, double safety_diff)
+  {
+    Basis basis(rs);
+    const double max_cfl_c = basis.max_cfl() * safety_conv;
+    const double max_cfl_d = -2 / basis.min_eig_diffusion() * safety_diff;
+    double dt = std::numeric_limits<double>::max();
+    #pragma omp parallel for reduction(min : dt)
+    for (int e = 0; e < m->n_elem[cls]; ++e) {
+      const double* state = m->state[cls] + size_t(e) * nv * nq;
+      double* tss = m->tss[cls] + size_t(e) * nq;
+      for (int q = 0; q < nq; ++q) {
+        double vs[nvert], coords[nd];
+        for (int i = 0; i < nvert; ++i) vs[i] = m->vertex_tss[cls][e * nvert + i];
+        for (int d = 0; d < nd; ++d) coords[d] = basis.node[(q / ipow(rs, nd - 1 - d)) % rs];
+        const double spacing = O::interp(vs, coords);
+        double scale = 0;
+        if (smooth) {
+          scale += 1. / max_cfl_d / spacing / spacing;
+        } else {
+          double vsq = 0;
+          for (int j = 0; j < nd; ++j) vsq += state[j * nq + q] * state[j * nq + q];
+          scale += std::max(1., std::sqrt(vsq)) / max_cfl_c / spacing;
+        }
+        if (is_local) tss[q] = 1. / scale;
+        else {
+          tss[q] = 1.;
+          dt = std::min(dt, 1. / scale);
+        }
+      }
+    }
+    return is_local ? 1. : dt;
+  }
+};
+
+// boundary hooks of the generator on the deformed-class boundary connections
+void apply_advection_bcs(hx_host_mesh* m)
+{
+  Mesh_view mv{m};
+  const int nd = m->n_dim, rs = m->row_size, nv = nd + 2, nfq = ipow(rs, nd - 1);
+  #pragma omp parallel for
+  for (int i = 0; i < m->n_bc_con; ++i) {
+    int con = m->bc_con[i];
+    const hx_bc& bc = m->bcs[m->bc_serial[i]];
+    double* in_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con]);
+    double* gh_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con + 1]);
+    const double* nrml = mv.nslot(m->con_nslot[2 * con]);
+    switch (bc.kind) {
+      case HX_BC_COPY:  // Copy::apply_advection = copy_state (Boundary_condition.cpp:12-22,460-463)
+        for (int l = 0; l < 2; ++l)
+          for (int k = 0; k < nv * nfq; ++k) gh_f[l * nv * nfq + k] = in_f[l * nv * nfq + k];
+        break;
+      case HX_BC_NONPENETRATION:  // Boundary_condition.cpp:346-367
+        for (int q = 0; q < nfq; ++q) {
+          double dot = 0., nsq = 0.;
+          for (int d = 0; d < nd; ++d) {
+            dot += in_f[d * nfq + q] * nrml[d * nfq + q];
+            nsq += nrml[d * nfq + q] * nrml[d * nfq + q];
+          }
+          for (int d = 0; d < nd; ++d) gh_f[d * nfq + q] = in_f[d * nfq + q] - 2 * dot * nrml[d * nfq + q] / nsq;
+          for (int a = 0; a < rs; ++a) gh_f[(nd + a) * nfq + q] = in_f[(nd + a) * nfq + q];
+        }
+        break;
+      default:  // Flow_bc::apply_advection (Boundary_condition.cpp:24-41)
+        for (int k = 0; k < nd * nfq; ++k) gh_f[k] = in_f[k];
+        for (int k = 0; k < rs * nfq; ++k) gh_f[nd * nfq + k] = 2. - in_f[nd * nfq + k];
+        break;
+    }
+  }
+}
+
+// Solver::apply_avc_diff_bcs / apply_avc_diff_flux_bcs (Solver.cpp:83-101) with Flow_bc::apply_diffusion and
+// flux_diffusion (Boundary_condition.cpp:43-60), which no condition built here overrides
+void apply_avc_diff_bcs(hx_host_mesh* m, bool flux)
+{
+  Mesh_view mv{m};
+  const int nd = m->n_dim, nv = nd + 2, nfq = ipow(m->row_size, nd - 1);
+  #pragma omp parallel for
+  for (int i = 0; i < m->n_bc_con; ++i) {
+    int con = m->bc_con[i];
+    double* in_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con]);
+    double* gh_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con + 1]);
+    if (flux)
+      for (int k = 0; k < nv * nfq; ++k) gh_f[nv * nfq + k] = -in_f[nv * nfq + k];
+    else
+      for (int k = 0; k < nv * nfq; ++k) gh_f[k] = in_f[k];
+  }
+}
+
 // ---- dispatch ---------------------------------------------------------------------------------------------
 
 template <template <int, int> class F, typename... A>
@@ -984,12 +1511,127 @@ struct Run_prolong { static int run(hx_host_mesh* m, int nvar, int s, int o) { O
 template <int nd, int rs>
 struct Run_restrict { static int run(hx_host_mesh* m, int nvar, int s, int o) { Ops<nd, rs>::restrict_(m, nvar, s, o); return 0; } };
 
+int pde_kind(const hx_pde* pde) { return pde ? pde->kind : HX_PDE_EULER; }
+
 int n_extrap(const hx_host_mesh* m, const hx_pde* pde)
 {
-  int kind = pde ? pde->kind : HX_PDE_EULER;
+  int kind = pde_kind(pde);
   if (kind == HX_PDE_EULER || kind == HX_PDE_NAVIER_STOKES) return m->n_dim + 2;
+  if (kind == HX_PDE_ADVECTION) return m->n_dim + m->row_size;
+  if (kind == HX_PDE_SMOOTH_AV) return 3;
   throw std::runtime_error("oracle: PDE kind not restated");
 }
+
+template <int nd, int rs>
+struct Run_av_write_face
+{
+  static int run(hx_host_mesh* m, int cls, int kind)
+  {
+    if (kind == HX_PDE_ADVECTION) Av<nd, rs>::adv_write_face(m, cls);
+    else Av<nd, rs>::sav_write_face(m, cls);
+    return 0;
+  }
+};
+template <int nd, int rs>
+struct Run_av_neighbor
+{
+  static int run(hx_host_mesh* m, int cls, int kind)
+  {
+    if (kind == HX_PDE_ADVECTION) Av<nd, rs>::adv_neighbor(m, cls);
+    else Av<nd, rs>::sav_neighbor(m, cls);
+    return 0;
+  }
+};
+template <int nd, int rs>
+struct Run_adv_local
+{
+  static int run(hx_host_mesh* m, int cls, double dt, int stage, int cr, double advect_length)
+  {
+    Av<nd, rs>::adv_local(m, cls, dt, stage, cr, advect_length);
+    return 0;
+  }
+};
+template <int nd, int rs>
+struct Run_sav_local
+{
+  static int run(hx_host_mesh* m, int cls, double dt, double diff_time) { Av<nd, rs>::sav_local(m, cls, dt, diff_time); return 0; }
+};
+template <int nd, int rs>
+struct Run_sav_neighbor_reconcile { static int run(hx_host_mesh* m, int cls) { Av<nd, rs>::sav_neighbor_reconcile(m, cls); return 0; } };
+template <int nd, int rs>
+struct Run_sav_reconcile
+{
+  static int run(hx_host_mesh* m, int cls, double dt, double diff_time, double cheby)
+  {
+    Av<nd, rs>::sav_reconcile(m, cls, dt, diff_time, cheby);
+    return 0;
+  }
+};
+template <int nd, int rs>
+struct Run_av_max_dt
+{
+  static double run(hx_host_mesh* m, int cls, bool smooth, int is_local, double sc, double sd)
+  {
+    return Av<nd, rs>::max_dt(m, cls, smooth, is_local, sc, sd);
+  }
+};
+
+// the Solver-level point loops of update_art_visc_smoothness (Solver.cpp:465-477,525-540,549-574)
+template <int nd, int rs>
+struct Run_av_points
+{
+  static constexpr int nq = ipow(rs, nd), nv = nd + 2;
+  // stage 0: momentum /= sqrt(2 rho E); stage 1: Legendre projection of the advected scalars -> forcing[0];
+  // stage 2: new coefficient, residual and momentum restored
+  static double run(hx_host_mesh* m, int what, const hx_av_options* o)
+  {
+    Basis basis(rs);
+    double resid = 0;
+    const int n_real = 3;  // Storage_params::n_forcing - 1
+    const double mult = o->visc_mult * o->advect_length;
+    const double us_max = o->advect_length * o->unscaled_max * std::sqrt(2 * o->freestream_energy / o->freestream_mass);
+    double qw[nq];
+    for (int q = 0; q < nq; ++q) {
+      double w = 1;
+      for (int d = 0; d < nd; ++d) w *= basis.weight[(q / ipow(rs, nd - 1 - d)) % rs];
+      qw[q] = w;
+    }
+    for (int cls = 0; cls < 2; ++cls) {
+      #pragma omp parallel for reduction(+ : resid)
+      for (int e = 0; e < m->n_elem[cls]; ++e) {
+        double* state = m->state[cls] + size_t(e) * nv * nq;
+        if (what == 0) {
+          for (int q = 0; q < nq; ++q) {
+            double scale = std::sqrt(2 * state[nd * nq + q] * state[(nd + 1) * nq + q]);
+            for (int d = 0; d < nd; ++d) state[d * nq + q] /= scale;
+          }
+        } else if (what == 1) {
+          const double* adv = m->advection[cls] + size_t(e) * rs * nq;
+          double* forcing = m->forcing[cls] + size_t(e) * 4 * nq;
+          const double* orth = basis.ortho + (o->av_rs - 1) * rs;
+          for (int q = 0; q < nq; ++q) {
+            double proj = 0;
+            for (int i = 0; i < rs; ++i) proj += adv[i * nq + q] * basis.weight[i] * orth[i];
+            forcing[q] = proj * proj * 2 * state[(nd + 1) * nq + q] / state[nd * nq + q];
+          }
+        } else {
+          double* av = m->av[cls] + size_t(e) * 2 * nq;
+          const double* forcing = m->forcing[cls] + size_t(e) * 4 * nq;
+          const double volume = mpow(m->nom_size[cls][e], nd);
+          for (int q = 0; q < nq; ++q) {
+            const double f = mult * forcing[n_real * nq + q];
+            const double new_av = us_max * f / (us_max + f);
+            resid += mpow(av[q] - new_av, 2) * qw[q] * volume;
+            av[q] = new_av;
+            double scale = std::sqrt(2 * state[nd * nq + q] * state[(nd + 1) * nq + q]);
+            for (int d = 0; d < nd; ++d) state[d * nq + q] *= scale;
+          }
+        }
+      }
+    }
+    return resid;
+  }
+};
 
 // Boundary conditions on the deformed-class boundary connections (Boundary_condition.cpp)
 void apply_state_bcs(hx_host_mesh* m)
@@ -1113,7 +1755,9 @@ int oracle_write_face(hx_host_mesh* m, int cls, const hx_pde* pde)
   return guarded([&] {
     check_mesh(m);
     n_extrap(m, pde);
-    dispatch<Run_write_face>(m->n_dim, m->row_size, m, cls);
+    int kind = pde_kind(pde);
+    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV) dispatch<Run_av_write_face>(m->n_dim, m->row_size, m, cls, kind);
+    else dispatch<Run_write_face>(m->n_dim, m->row_size, m, cls);
   });
 }
 
@@ -1121,7 +1765,9 @@ int oracle_neighbor(hx_host_mesh* m, int cls, const hx_pde* pde, int)
 {
   return guarded([&] {
     check_mesh(m);
-    dispatch<Run_neighbor>(m->n_dim, m->row_size, m, cls, pde);
+    int kind = pde_kind(pde);
+    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV) dispatch<Run_av_neighbor>(m->n_dim, m->row_size, m, cls, kind);
+    else dispatch<Run_neighbor>(m->n_dim, m->row_size, m, cls, pde);
   });
 }
 
@@ -1130,17 +1776,31 @@ int oracle_local(hx_host_mesh* m, int cls, const hx_pde* pde, double dt, int i_s
   return guarded([&] {
     check_mesh(m);
     // HEXED_ASSERTs of the Local constructor (Spatial.hpp:322-323)
-    if (is_ns(pde) && i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+    int kind = pde_kind(pde);
+    if (kind == HX_PDE_NAVIER_STOKES && i_stage)
+      throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
     if (i_stage && compute_residual) throw std::runtime_error("residual calculation is a single-stage operation");
+    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV) {
+      if (use_filter) throw std::runtime_error("oracle: the artificial-viscosity PDEs are restated without the filter");
+      if (kind == HX_PDE_ADVECTION)
+        dispatch<Run_adv_local>(m->n_dim, m->row_size, m, cls, dt, i_stage, compute_residual, pde->advect_length);
+      else {
+        if (i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+        if (compute_residual) throw std::runtime_error("oracle: Smooth_art_visc is restated without compute_residual");
+        dispatch<Run_sav_local>(m->n_dim, m->row_size, m, cls, dt, pde->diff_time);
+      }
+      return;
+    }
     dispatch<Run_local>(m->n_dim, m->row_size, m, cls, pde, dt, i_stage, compute_residual, use_filter);
   });
 }
 
-int oracle_neighbor_reconcile(hx_host_mesh* m, int cls, const hx_pde*)
+int oracle_neighbor_reconcile(hx_host_mesh* m, int cls, const hx_pde* pde)
 {
   return guarded([&] {
     check_mesh(m);
-    dispatch<Run_neighbor_reconcile>(m->n_dim, m->row_size, m, cls);
+    if (pde_kind(pde) == HX_PDE_SMOOTH_AV) dispatch<Run_sav_neighbor_reconcile>(m->n_dim, m->row_size, m, cls);
+    else dispatch<Run_neighbor_reconcile>(m->n_dim, m->row_size, m, cls);
   });
 }
 
@@ -1149,6 +1809,13 @@ int oracle_reconcile_ldg_flux(hx_host_mesh* m, int cls, const hx_pde* pde, doubl
 {
   return guarded([&] {
     check_mesh(m);
+    if (pde_kind(pde) == HX_PDE_SMOOTH_AV) {
+      if (i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+      if (compute_residual || use_filter)
+        throw std::runtime_error("oracle: Smooth_art_visc is restated without compute_residual and filter");
+      dispatch<Run_sav_reconcile>(m->n_dim, m->row_size, m, cls, dt, pde->diff_time, pde->cheby_step);
+      return;
+    }
     if (is_ns(pde) && i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
     dispatch<Run_reconcile>(m->n_dim, m->row_size, m, cls, dt, compute_residual, use_filter);
   });
@@ -1159,7 +1826,11 @@ int oracle_max_dt(hx_host_mesh* m, int cls, const hx_pde* pde, int is_local, int
 {
   return guarded([&] {
     check_mesh(m);
-    *dt_out = dispatch<Run_max_dt>(m->n_dim, m->row_size, m, cls, pde, is_local, safety_conv, safety_diff);
+    int kind = pde_kind(pde);
+    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV)
+      *dt_out = dispatch<Run_av_max_dt>(m->n_dim, m->row_size, m, cls, kind == HX_PDE_SMOOTH_AV, is_local, safety_conv,
+                                        safety_diff);
+    else *dt_out = dispatch<Run_max_dt>(m->n_dim, m->row_size, m, cls, pde, is_local, safety_conv, safety_diff);
   });
 }
 
@@ -1278,6 +1949,98 @@ int oracle_update(hx_host_mesh* m, const hx_update_options* o, hx_update_status*
   }
   ++st->iteration;
   return 0;
+}
+
+// ---- artificial-viscosity generator: kernel sequences and Solver::update_art_visc_smoothness ------------------
+
+int oracle_apply_advection_bcs(hx_host_mesh* m) { return guarded([&] { apply_advection_bcs(m); }); }
+int oracle_apply_avc_diff_bcs(hx_host_mesh* m) { return guarded([&] { apply_avc_diff_bcs(m, false); }); }
+int oracle_apply_avc_diff_flux_bcs(hx_host_mesh* m) { return guarded([&] { apply_avc_diff_bcs(m, true); }); }
+
+// compute_advection (kernels_convective.cpp:8-19 with pde::Advection)
+int oracle_compute_advection(hx_host_mesh* m, const hx_kernel_options* o, double advect_length)
+{
+  return guarded([&] {
+    check_mesh(m);
+    const int nd = m->n_dim, rs = m->row_size;
+    dispatch<Run_av_neighbor>(nd, rs, m, HX_CARTESIAN, int(HX_PDE_ADVECTION));
+    dispatch<Run_av_neighbor>(nd, rs, m, HX_DEFORMED, int(HX_PDE_ADVECTION));
+    dispatch<Run_restrict>(nd, rs, m, nd + rs, 1, 0);
+    dispatch<Run_adv_local>(nd, rs, m, HX_CARTESIAN, o->dt, o->i_stage, o->compute_residual, advect_length);
+    dispatch<Run_adv_local>(nd, rs, m, HX_DEFORMED, o->dt, o->i_stage, o->compute_residual, advect_length);
+    dispatch<Run_prolong>(nd, rs, m, nd + rs, 0, 0);
+  });
+}
+
+// compute_smooth_av (kernels_diffusive.cpp:8-31 with pde::Smooth_art_visc), flux_bc = apply_avc_diff_flux_bcs
+int oracle_compute_smooth_av(hx_host_mesh* m, const hx_kernel_options* o, double diff_time, double cheby_step)
+{
+  return guarded([&] {
+    check_mesh(m);
+    const int nd = m->n_dim, rs = m->row_size;
+    if (o->i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+    dispatch<Run_av_neighbor>(nd, rs, m, HX_CARTESIAN, int(HX_PDE_SMOOTH_AV));
+    dispatch<Run_av_neighbor>(nd, rs, m, HX_DEFORMED, int(HX_PDE_SMOOTH_AV));
+    dispatch<Run_restrict>(nd, rs, m, 3, 1, 0);
+    dispatch<Run_restrict>(nd, rs, m, 3, 0, 1);
+    dispatch<Run_sav_local>(nd, rs, m, HX_CARTESIAN, o->dt, diff_time);
+    dispatch<Run_sav_local>(nd, rs, m, HX_DEFORMED, o->dt, diff_time);
+    dispatch<Run_prolong>(nd, rs, m, 3, 1, 1);
+    apply_avc_diff_bcs(m, true);
+    dispatch<Run_sav_neighbor_reconcile>(nd, rs, m, HX_CARTESIAN);
+    dispatch<Run_sav_neighbor_reconcile>(nd, rs, m, HX_DEFORMED);
+    dispatch<Run_restrict>(nd, rs, m, 3, 1, 1);
+    dispatch<Run_sav_reconcile>(nd, rs, m, HX_CARTESIAN, o->dt, diff_time, cheby_step);
+    dispatch<Run_sav_reconcile>(nd, rs, m, HX_DEFORMED, o->dt, diff_time, cheby_step);
+    dispatch<Run_prolong>(nd, rs, m, 3, 0, 0);
+  });
+}
+
+// Solver::update_art_visc_smoothness (Solver.cpp:457-582) with diffuse_art_visc (Solver.cpp:428-455)
+int oracle_update_art_visc_smoothness(hx_host_mesh* m, const hx_av_options* o, double* residual)
+{
+  return guarded([&] {
+    check_mesh(m);
+    for (int c = 0; c < 2; ++c)
+      if (m->n_elem[c] && (!m->forcing[c] || !m->advection[c] || !m->av[c]))
+        throw std::runtime_error("artificial viscosity needs the av, forcing and advection arrays");
+    const int nd = m->n_dim, rs = m->row_size;
+    const double L = o->advect_length;
+    dispatch<Run_av_points>(nd, rs, m, 0, o);
+    // enforce CFL condition
+    for (int c = 0; c < 2; ++c) dispatch<Run_write_face>(nd, rs, m, c);
+    dispatch<Run_prolong>(nd, rs, m, nd + 2, 0, 0);
+    dispatch<Run_av_max_dt>(nd, rs, m, HX_CARTESIAN, false, 1, o->advect_max_safety, 1.);
+    dispatch<Run_av_max_dt>(nd, rs, m, HX_DEFORMED, false, 1, o->advect_max_safety, 1.);
+    for (int iter = 0; iter < o->advect_iters; ++iter) {
+      for (int c = 0; c < 2; ++c) dispatch<Run_av_write_face>(nd, rs, m, c, int(HX_PDE_ADVECTION));
+      dispatch<Run_prolong>(nd, rs, m, nd + rs, 0, 0);
+      for (int i = 0; i < 2; ++i) {
+        apply_advection_bcs(m);
+        hx_kernel_options ko{1., i, 0, 0};
+        if (oracle_compute_advection(m, &ko, L)) throw std::runtime_error(g_err);
+      }
+    }
+    dispatch<Run_av_points>(nd, rs, m, 1, o);
+    // root-smear-square: diffuse_art_visc
+    const int n_real = 3;
+    const double diff_time = o->diff_ratio * L * L / n_real;
+    dispatch<Run_av_max_dt>(nd, rs, m, HX_CARTESIAN, true, 1, 1., o->diff_max_safety);
+    dispatch<Run_av_max_dt>(nd, rs, m, HX_DEFORMED, true, 1, 1., o->diff_max_safety);
+    for (int c = 0; c < 2; ++c) dispatch<Run_av_write_face>(nd, rs, m, c, int(HX_PDE_SMOOTH_AV));
+    dispatch<Run_prolong>(nd, rs, m, nd + 2, 0, 0);
+    for (int iter = 0; iter < o->diff_iters; ++iter)
+      for (int i_cheby = 0; i_cheby < o->n_cheby; ++i_cheby) {
+        const double s = chebyshev_step(o->n_cheby, i_cheby);
+        apply_avc_diff_bcs(m, false);
+        hx_kernel_options ko{s, 0, 0, 0};
+        if (oracle_compute_smooth_av(m, &ko, diff_time, s)) throw std::runtime_error(g_err);
+      }
+    const double resid = dispatch<Run_av_points>(nd, rs, m, 2, o);
+    if (residual) *residual = std::sqrt(resid);
+    for (int c = 0; c < 2; ++c) dispatch<Run_write_face>(nd, rs, m, c);
+    dispatch<Run_prolong>(nd, rs, m, nd + 2, 0, 0);
+  });
 }
 
 // Solver::compute_residual (Solver.cpp:899-913)

@@ -41,6 +41,15 @@ int oracle_max_dt_navier_stokes(hx_host_mesh* m, const hx_kernel_options* o, dou
 int oracle_update(hx_host_mesh* m, const hx_update_options* o, hx_update_status* st);     /* Solver.cpp:834-886  */
 int oracle_compute_residual(hx_host_mesh* m, const hx_update_options* o);                 /* Solver.cpp:899-913  */
 
+/* artificial-viscosity generator (pde.hpp:265-431, Solver.cpp:83-101,428-582); the per-kernel entry points above
+ * also take HX_PDE_ADVECTION and HX_PDE_SMOOTH_AV */
+int oracle_apply_advection_bcs(hx_host_mesh* m);                                           /* Solver.cpp:508-514  */
+int oracle_apply_avc_diff_bcs(hx_host_mesh* m);                                            /* Solver.cpp:83-91    */
+int oracle_apply_avc_diff_flux_bcs(hx_host_mesh* m);                                       /* Solver.cpp:93-101   */
+int oracle_compute_advection(hx_host_mesh* m, const hx_kernel_options* o, double advect_length);
+int oracle_compute_smooth_av(hx_host_mesh* m, const hx_kernel_options* o, double diff_time, double cheby_step);
+int oracle_update_art_visc_smoothness(hx_host_mesh* m, const hx_av_options* o, double* residual);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,7 +70,7 @@ class Test_element : public Kernel_element
   double* fnrml[6] = {};
   double unc = 0;
   Test_element(Params pp, double size, bool deformed)
-  : p{pp}, data((pp.nv + 3) * pp.nq, 0.), rcache(pp.nv * pp.nq, 0.), vertex(1 << pp.nd, size / pp.nd), h{size}, def{deformed}
+  : p{pp}, data((pp.nv + 3) * pp.nq, 0.), rcache(std::max(pp.nv, pp.rs) * pp.nq, 0.), vertex(1 << pp.nd, size / pp.nd), h{size}, def{deformed}
   {
     for (int q = 0; q < p.nq; ++q) data[p.nv * p.nq + q] = 1.;
     if (def) {

@@ -37,7 +37,7 @@ def set_local_tss(m, opts):
 
 def physical_residual(m, c):
     """Physical_residual (Qpoint_func.cpp:37-45): residual cache divided by the time step scale."""
-    return m.rcache[c] / m.tss[c][:, None, :]
+    return m.rcache[c][:, :m.nv] / m.tss[c][:, None, :]
 
 
 def quadrature_volume_weights(m, c):
@@ -311,7 +311,7 @@ def test_filter_is_tensor_product(nd, rs):
         s = m.struct()
         opts = abi.update_options(use_filter=bool(uf))
         o_call("oracle_compute_residual", ctypes.byref(s), ctypes.byref(opts))
-        res.append(m.rcache[1].copy())
+        res.append(m.rcache[1][:, :m.nv].copy())
     # the filter acts on the rate before the point-wise scale dt*tss/(h*det) (Spatial.hpp:472-489)
     F = BASIS(rs)["filter"]
     n = m0.n_elem[1]
