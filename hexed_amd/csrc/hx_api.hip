@@ -161,7 +161,7 @@ long long field_size(const Dev_mesh& m, int field, int cls)
 {
   const long long n = (cls == 0 || cls == 1) ? m.n_elem[cls] : 0;
   switch (field) {
-    case HX_FIELD_STATE:
+    case HX_FIELD_STATE: return n * m.nv * m.nq;
     case HX_FIELD_RCACHE: return n * std::max(m.nv, m.rs) * m.nq;
     case HX_FIELD_TSS: return n * m.nq;
     case HX_FIELD_AV: return n * 2 * m.nq;

@@ -303,11 +303,12 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
         con.faces(x, b0, b1, y);
         double* out = b + L::G + v * NQP + base;
 #pragma unroll
-        for (int k = 0; k < RS; ++k) out[k * sd] = (i == 0 || !DEF) ? y[k] * inv_h : out[k * sd] + y[k] * inv_h;
+        for (int k = 0; k < RS; ++k) out[k * sd] = (i == 0 || !DEF) ? y[k] : out[k * sd] + y[k];
       });
       __syncthreads();
     }
-    for (int v = 0; v < NS; ++v) grad[v][j] = my[L::G + v * NQP + qp];
+    // the column threads above serve several elements; each point scales by its own element's size
+    for (int v = 0; v < NS; ++v) grad[v][j] = my[L::G + v * NQP + qp] * inv_h;
     __syncthreads();
   }
   if constexpr (DEF)

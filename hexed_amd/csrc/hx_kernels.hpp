@@ -405,6 +405,34 @@ __device__ inline void write_faces_from(const Dev_mesh& m, int cls, const Eops<R
   }
 }
 
+// ---- convective row derivative of one column, both forms ------------------------------------------------------
+// FP64: -(D x + L (b - B x)) of the flux row. FP32-MFMA: the flux row x holds everything but the pressure part p n_v
+// of the momentum fluxes; that part (FP64, row xp) is contracted on the vector units in FP64 and its face
+// extrapolation taken off the face jump before the FP32 matrix-core lift: D x + D xp + L ((b - B xp) - B x).
+template <int ND, int RS, bool DEF, class P, class C>
+__device__ inline void conv_column(const C& con, const Eops<RS, double>& o, int i, int v,
+                                   const typename P::T* frow, const double* prow, int sd, double b0, double b1,
+                                   typename P::T (&y)[RS])
+{
+  using T = typename P::T;
+  T x[RS];
+  load_row(frow, sd, x);
+  if constexpr (!P::mfma) {
+    con.faces(x, b0, b1, y);
+  } else {
+    double yd[RS] = {}, bd[2] = {0., 0.};
+    if (v < ND && (DEF || v == i)) {
+      double xp[RS];
+      load_row(prow, sd, xp);
+      const Contraction<RS, Fp64_form> c64(o, nullptr);
+      c64.faces_if(false, xp, 0., 0., yd, bd);
+    }
+    con.faces(x, b0 - bd[0], b1 - bd[1], y);
+#pragma unroll
+    for (int k = 0; k < RS; ++k) y[k] = T(double(y[k]) + yd[k]);
+  }
+}
+
 // ---- Local, Euler (Spatial.hpp:326-509 with Navier_stokes<false>) -----------------------------------------
 template <int ND, int RS, class P>
 struct Euler_lds
@@ -416,8 +444,10 @@ struct Euler_lds
   static constexpr int PER_T = RT + S::NV * S::NQP;
   // arrays of double
   static constexpr int SB = 0;                              // [2ND][NV][NFQ] numerical face fluxes
-  static constexpr int U = 0;                               // [NV][NQP] updated state, for Write_face (after SB)
-  static constexpr int PER_D = 2 * ND * S::NFQ > S::NQP ? 2 * ND * S::NV * S::NFQ : S::NV * S::NQP;
+  static constexpr int PD = SB + 2 * ND * S::NV * S::NFQ;   // [ND][NQP] FP32 form: pressure part p n of the momentum flux
+  static constexpr int LOOP_D = PD + (P::mfma ? ND * S::NQP : 0);
+  static constexpr int U = 0;                               // [NV][NQP] updated state, for Write_face (after the loop)
+  static constexpr int PER_D = LOOP_D > S::NV * S::NQP ? LOOP_D : S::NV * S::NQP;
 };
 
 template <int ND, int RS, bool DEF, class P>
@@ -448,13 +478,18 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
   T ut[NV];
   for (int v = 0; v < NV; ++v) ut[v] = T(u[v]);
   double det = 1.;
+  double metd[ND][ND];
   T met[ND][ND];
   if constexpr (DEF) {
     const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
     for (int i = 0; i < ND; ++i)
-      for (int j = 0; j < ND; ++j) met[i][j] = valid ? T(mt[(i * ND + j) * NQ + q]) : T(i == j);
+      for (int j = 0; j < ND; ++j) {
+        metd[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
+        met[i][j] = T(metd[i][j]);
+      }
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
+  const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
   load_faces<ND, RS, NT>(m, cls, 0, ld + L::SB, L::PER_D);
   for (int i = 0; i < ND; ++i) {
     // volume flux along reference direction i (Spatial.hpp:404-419)
@@ -463,18 +498,22 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
       if constexpr (DEF) nrm[j] = met[i][j];
       else nrm[j] = T(i == j);
     }
-    flux_conv<ND>(ut, nrm, f);
+    if constexpr (P::mfma) flux_conv_no_pressure<ND>(ut, nrm, f);
+    else flux_conv<ND>(ut, nrm, f);
     if (i) __syncthreads();
-    if (pt)
+    if (pt) {
       for (int v = 0; v < NV; ++v) my[L::F + v * NQP + qp] = f[v];
+      if constexpr (P::mfma)
+        for (int v = 0; v < ND; ++v) ld[el * L::PER_D + L::PD + v * NQP + qp] = p64 * (DEF ? metd[i][v] : double(i == v));
+    }
     __syncthreads();
     // -derivative(row_f, face_f) (Spatial.hpp:444-457)
     for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
       T* b = lt + el2 * L::PER_T;
-      const double* fb = ld + el2 * L::PER_D + L::SB;
-      T x[RS], y[RS];
-      load_row(b + L::F + v * NQP + base, sd, x);
-      con.faces(x, fb[((2 * i) * NV + v) * NFQ + r], fb[((2 * i + 1) * NV + v) * NFQ + r], y);
+      const double* db = ld + el2 * L::PER_D;
+      T y[RS];
+      conv_column<ND, RS, DEF, P>(con, o, i, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
+                                  db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
       if (ok) sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
     });
   }
@@ -525,7 +564,8 @@ struct Ns_lds
   static constexpr int SFN = SL + 2 * ND * NV * NFQ;              // [2ND][ND][NFQ] face normals
   static constexpr int P1_D = SFN + (DEF ? 2 * ND * ND * NFQ : 0);
   static constexpr int SB = 0;                                    // [2ND][NV][NFQ] numerical face fluxes
-  static constexpr int P2_D = SB + 2 * ND * NV * NFQ;
+  static constexpr int PD = SB + 2 * ND * NV * NFQ;               // [ND][NQP] FP32 form: pressure part p n
+  static constexpr int P2_D = PD + (P::mfma ? ND * NQP : 0);
   static constexpr int PER_D = P1_D > P2_D ? P1_D : P2_D;
 };
 
@@ -570,18 +610,21 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
     ut[v] = T(u[v]);
     if (pt) my[L::U + v * NQP + qp] = ut[v];
   }
+  const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
   T bulk = 0, lap = 0;
   if (valid && m.av[cls]) {
     bulk = T(m.av[cls][size_t(e) * 2 * NQ + q]);
     lap = T(m.av[cls][size_t(e) * 2 * NQ + NQ + q]);
   }
   double det = 1.;
+  double metd[ND][ND];
   T met[ND][ND];
   if constexpr (DEF) {
     const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
     for (int i = 0; i < ND; ++i)
       for (int j = 0; j < ND; ++j) {
-        met[i][j] = valid ? T(mt[(i * ND + j) * NQ + q]) : T(i == j);
+        metd[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
+        met[i][j] = T(metd[i][j]);
         if (pt) my[L::SN + (i * ND + j) * NQP + qp] = met[i][j];
       }
     det = valid ? mt[ND * ND * NQ + q] : 1.;
@@ -655,17 +698,21 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
       if constexpr (DEF) nrm[j] = met[i][j];
       else nrm[j] = T(i == j);
     }
-    flux_conv<ND>(ut, nrm, fc);
+    if constexpr (P::mfma) flux_conv_no_pressure<ND>(ut, nrm, fc);
+    else flux_conv<ND>(ut, nrm, fc);
     if (i) __syncthreads();
-    if (pt)
+    if (pt) {
       for (int v = 0; v < NV; ++v) my[L::F + v * NQP + qp] = fc[v];
+      if constexpr (P::mfma)
+        for (int v = 0; v < ND; ++v) ld[el * L::PER_D + L::PD + v * NQP + qp] = p64 * (DEF ? metd[i][v] : double(i == v));
+    }
     __syncthreads();
     for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
       T* b = lt + el2 * L::PER_T;
-      const double* fb = ld + el2 * L::PER_D + L::SB;
-      T x[RS], y[RS];
-      load_row(b + L::F + v * NQP + base, sd, x);
-      con.faces(x, fb[((2 * i) * NV + v) * NFQ + r], fb[((2 * i + 1) * NV + v) * NFQ + r], y);
+      const double* db = ld + el2 * L::PER_D;
+      T y[RS];
+      conv_column<ND, RS, DEF, P>(con, o, i, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
+                                  db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
       if (ok) sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
     });
     __syncthreads();

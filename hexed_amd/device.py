@@ -4,6 +4,7 @@ This is the host-side mirror used by the tests, the smoke check and the benchmar
 libhexed_amd.so (include/hexed_amd.h) and fails loudly when the library or a GPU is missing.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -24,6 +25,7 @@ class Context:
     def __init__(self, device=0):
         self.lib = abi.load_library()
         self.ptr = ctypes.c_void_p()
+        self._meshes = weakref.WeakSet()  # live meshes; a context outlives every mesh made on it
         rc = self.lib.hx_ctx_create(device, ctypes.byref(self.ptr))
         if rc != abi.HX_OK:
             raise HxError(f"hx_ctx_create failed with status {rc}")
@@ -67,6 +69,8 @@ class Context:
 
     def close(self):
         if self.ptr:
+            for m in list(self._meshes):
+                m.close()
             self.lib.hx_ctx_destroy(self.ptr)
             self.ptr = ctypes.c_void_p()
 
@@ -87,11 +91,13 @@ class Device_mesh:
         self.ptr = ctypes.c_void_p()
         self._struct = host.struct()
         ctx.check(self.lib.hx_mesh_create(ctx.ptr, ctypes.byref(self._struct), ctypes.byref(self.ptr)), "hx_mesh_create")
+        ctx._meshes.add(self)
 
     def close(self):
-        if self.ptr:
+        # a mesh whose context is already destroyed has been released with it (Context.close)
+        if self.ptr and self.ctx.ptr:
             self.lib.hx_mesh_destroy(self.ptr)
-            self.ptr = ctypes.c_void_p()
+        self.ptr = ctypes.c_void_p()
 
     def __del__(self):
         try:
