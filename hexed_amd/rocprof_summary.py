@@ -20,6 +20,7 @@ COPY_BYTES = 2 ** 31
 
 
 def _kernel(name):
+    name = name.replace("(anonymous namespace)::", "")
     return name.split("(")[0].replace("void ", "").replace("hx::", "") if name.startswith("void ") else name
 
 
