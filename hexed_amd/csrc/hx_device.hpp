@@ -163,25 +163,6 @@ __device__ inline void flux_conv(const T (&u)[ND + 2], const T (&n)[ND], T (&f)[
   for (int j = 0; j < ND; ++j) f[j] = u[j] * vol_flux + pressure * n[j];
 }
 
-// flux_conv without the isotropic pressure part p n of the momentum flux, which the FP32 form contracts separately in
-// FP64 (its rounding in FP32 would dominate the error of the momentum components transverse to the flow)
-template <int ND, class T>
-__device__ inline void flux_conv_no_pressure(const T (&u)[ND + 2], const T (&n)[ND], T (&f)[ND + 2])
-{
-  constexpr T gm1 = T(heat_rat - 1.);
-  const T inv_mass = T(1) / u[ND];
-  T kin = 0;
-  for (int j = 0; j < ND; ++j) kin += u[j] * u[j];
-  kin *= T(.5) * inv_mass;
-  const T pressure = gm1 * (u[ND + 1] - kin);
-  T mf = 0;
-  for (int j = 0; j < ND; ++j) mf += u[j] * n[j];
-  f[ND] = mf;
-  const T vol_flux = mf * inv_mass;
-  f[ND + 1] = (u[ND + 1] + pressure) * vol_flux;
-  for (int j = 0; j < ND; ++j) f[j] = u[j] * vol_flux;
-}
-
 // pressure (pde.hpp:97-104) in FP64
 template <int ND>
 __device__ inline double pressure_of(const double (&u)[ND + 2])

@@ -405,14 +405,53 @@ __device__ inline void write_faces_from(const Dev_mesh& m, int cls, const Eops<R
   }
 }
 
-// ---- convective row derivative of one column, both forms ------------------------------------------------------
-// FP64: -(D x + L (b - B x)) of the flux row. FP32-MFMA: the flux row x holds everything but the pressure part p n_v
-// of the momentum fluxes; that part (FP64, row xp) is contracted on the vector units in FP64 and its face
-// extrapolation taken off the face jump before the FP32 matrix-core lift: D x + D xp + L ((b - B xp) - B x).
+// ---- convective flux rows of the two forms ---------------------------------------------------------------------
+// FP64: the flux row x of direction i enters -(D x + L (b - B x)) as is.
+// FP32-MFMA: the FP32 operands carry only what FP32 holds accurately relative to the small residual of a nearly
+// uniform flow:
+//  * every flux row is shifted by its numerical face flux b0 before rounding; D 1 = 0 and B 1 = 1 make
+//    D x + L (b - B x) invariant under x -> x - b0, b -> b - b0, so the FP32 operands carry only the variation
+//    along the row;
+//  * except the pressure part p n_iv of the momentum fluxes of deformed elements, which varies along a row as much as
+//    the normal n_iv does, so that no shift makes it small. It stays FP64 (LDS row xp, array PD): its face
+//    extrapolation B xp comes off the jumps of the rest of the row, which goes to the matrix cores unshifted, and
+//    its derivative D xp is summed over the directions in FP64 (array RP). (Summing it in FP32 would round partial
+//    sums many times larger than the total.)
+template <int ND, bool DEF>
+__device__ inline bool has_pressure(int v)
+{
+  return DEF && v < ND;
+}
+
+// point phase: this point's operands of direction i (normal nrm = n_i.), from the state u with pressure p; sb0 is
+// the point's row's numerical flux at face 0 (variable stride nfq)
+template <int ND, bool DEF, class P>
+__device__ inline void conv_point(const double (&u)[ND + 2], double p, const double (&nrm)[ND], const double* sb0, int nfq,
+                                  typename P::T* f, double* pd, int stride)
+{
+  using T = typename P::T;
+  double fl[ND + 2];
+  flux_conv<ND>(u, nrm, fl);
+#pragma unroll
+  for (int v = 0; v < ND + 2; ++v) {
+    if constexpr (!P::mfma) {
+      f[v * stride] = fl[v];
+    } else if (has_pressure<ND, DEF>(v)) {
+      const double xp = p * nrm[v];
+      pd[v * stride] = xp;
+      f[v * stride] = T(fl[v] - xp);
+    } else {
+      f[v * stride] = T(fl[v] - sb0[v * nfq]);
+    }
+  }
+}
+
+// row phase: y = D x + L (b - B x) of the flux row of variable v from the operands conv_point wrote (frow; prow the
+// pressure row), without the FP32-MFMA form's FP64 pressure derivative (pressure_row). The matrix instructions run
+// outside any branch that diverges by variable.
 template <int ND, int RS, bool DEF, class P, class C>
-__device__ inline void conv_column(const C& con, const Eops<RS, double>& o, int i, int v,
-                                   const typename P::T* frow, const double* prow, int sd, double b0, double b1,
-                                   typename P::T (&y)[RS])
+__device__ inline void conv_row(const C& con, const Eops<RS, double>& o, int v, const typename P::T* frow,
+                                const double* prow, int sd, double b0, double b1, typename P::T (&y)[RS])
 {
   using T = typename P::T;
   T x[RS];
@@ -420,21 +459,34 @@ __device__ inline void conv_column(const C& con, const Eops<RS, double>& o, int 
   if constexpr (!P::mfma) {
     con.faces(x, b0, b1, y);
   } else {
-    double yd[RS] = {}, bd[2] = {0., 0.};
-    if (v < ND && (DEF || v == i)) {
+    double s0 = 0., s1 = b1 - b0;
+    if (has_pressure<ND, DEF>(v)) {
       double xp[RS];
       load_row(prow, sd, xp);
-      const Contraction<RS, Fp64_form> c64(o, nullptr);
-      c64.faces_if(false, xp, 0., 0., yd, bd);
+      s0 = b0 - row_bound(o, 0, xp);
+      s1 = b1 - row_bound(o, 1, xp);
     }
-    con.faces(x, b0 - bd[0], b1 - bd[1], y);
+    con.faces(x, s0, s1, y);
+  }
+}
+
+// FP32-MFMA form on deformed elements: acc = D xp (first direction) or acc += D xp, in FP64, for a momentum row
+template <int ND, int RS, bool DEF, class P>
+__device__ inline void pressure_row(const Eops<RS, double>& o, int i, int v, const double* prow, double* acc, int sd)
+{
+  if constexpr (P::mfma && DEF) {
+    if (v < ND) {
+      double xp[RS], yd[RS];
+      load_row(prow, sd, xp);
+      row_matvec(o.diff, xp, yd);
 #pragma unroll
-    for (int k = 0; k < RS; ++k) y[k] = T(double(y[k]) + yd[k]);
+      for (int k = 0; k < RS; ++k) acc[k * sd] = i == 0 ? yd[k] : acc[k * sd] + yd[k];
+    }
   }
 }
 
 // ---- Local, Euler (Spatial.hpp:326-509 with Navier_stokes<false>) -----------------------------------------
-template <int ND, int RS, class P>
+template <int ND, int RS, bool DEF, class P>
 struct Euler_lds
 {
   using S = Shape<ND, RS>;
@@ -444,8 +496,9 @@ struct Euler_lds
   static constexpr int PER_T = RT + S::NV * S::NQP;
   // arrays of double
   static constexpr int SB = 0;                              // [2ND][NV][NFQ] numerical face fluxes
-  static constexpr int PD = SB + 2 * ND * S::NV * S::NFQ;   // [ND][NQP] FP32 form: pressure part p n of the momentum flux
-  static constexpr int LOOP_D = PD + (P::mfma ? ND * S::NQP : 0);
+  static constexpr int PD = SB + 2 * ND * S::NV * S::NFQ;   // [ND][NQP] FP32-MFMA, deformed: pressure part p n_iv
+  static constexpr int RP = PD + ND * S::NQP;               // [ND][NQP] FP32-MFMA, deformed: its derivative, summed
+  static constexpr int LOOP_D = PD + (P::mfma && DEF ? 2 * ND * S::NQP : 0);
   static constexpr int U = 0;                               // [NV][NQP] updated state, for Write_face (after the loop)
   static constexpr int PER_D = LOOP_D > S::NV * S::NQP ? LOOP_D : S::NV * S::NQP;
 };
@@ -457,7 +510,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
                                                                            int compute_residual, int use_filter)
 {
   using S = Shape<ND, RS>;
-  using L = Euler_lds<ND, RS, P>;
+  using L = Euler_lds<ND, RS, DEF, P>;
   using T = typename P::T;
   constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = form_block<ND, RS, P>();
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
@@ -475,50 +528,47 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
-  T ut[NV];
-  for (int v = 0; v < NV; ++v) ut[v] = T(u[v]);
   double det = 1.;
-  double metd[ND][ND];
-  T met[ND][ND];
+  double met[ND][ND];
   if constexpr (DEF) {
     const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
     for (int i = 0; i < ND; ++i)
-      for (int j = 0; j < ND; ++j) {
-        metd[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
-        met[i][j] = T(metd[i][j]);
-      }
+      for (int j = 0; j < ND; ++j) met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
   const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
   load_faces<ND, RS, NT>(m, cls, 0, ld + L::SB, L::PER_D);
   for (int i = 0; i < ND; ++i) {
-    // volume flux along reference direction i (Spatial.hpp:404-419)
-    T nrm[ND], f[NV];
+    // volume flux along reference direction i (Spatial.hpp:404-419), in FP64
+    double nrm[ND];
     for (int j = 0; j < ND; ++j) {
       if constexpr (DEF) nrm[j] = met[i][j];
-      else nrm[j] = T(i == j);
+      else nrm[j] = double(i == j);
     }
-    if constexpr (P::mfma) flux_conv_no_pressure<ND>(ut, nrm, f);
-    else flux_conv<ND>(ut, nrm, f);
-    if (i) __syncthreads();
-    if (pt) {
-      for (int v = 0; v < NV; ++v) my[L::F + v * NQP + qp] = f[v];
-      if constexpr (P::mfma)
-        for (int v = 0; v < ND; ++v) ld[el * L::PER_D + L::PD + v * NQP + qp] = p64 * (DEF ? metd[i][v] : double(i == v));
-    }
+    if (i || P::mfma) __syncthreads();
+    if (pt)
+      conv_point<ND, DEF, P>(u, p64, nrm, ld + el * L::PER_D + L::SB + (2 * i) * NV * NFQ + Row<ND, RS>::fq(q, i), NFQ,
+                             my + L::F + qp, ld + el * L::PER_D + L::PD + qp, NQP);
     __syncthreads();
     // -derivative(row_f, face_f) (Spatial.hpp:444-457)
     for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
       T* b = lt + el2 * L::PER_T;
-      const double* db = ld + el2 * L::PER_D;
+      double* db = ld + el2 * L::PER_D;
       T y[RS];
-      conv_column<ND, RS, DEF, P>(con, o, i, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
-                                  db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
-      if (ok) sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
+      conv_row<ND, RS, DEF, P>(con, o, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
+                               db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
+      if (ok) {
+        sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
+        pressure_row<ND, RS, DEF, P>(o, i, v, db + L::PD + (v % ND) * NQP + base, db + L::RP + (v % ND) * NQP + base, sd);
+      }
     });
   }
-  if (use_filter) filter_columns<ND, RS, NV, NT>(o, lt + L::RT, L::PER_T);
-  else __syncthreads();
+  if (use_filter) {
+    filter_columns<ND, RS, NV, NT>(o, lt + L::RT, L::PER_T);
+    if constexpr (P::mfma && DEF) filter_columns<ND, RS, ND, NT>(o, ld + L::RP, L::PER_D);
+  } else {
+    __syncthreads();
+  }
   // stage update (Spatial.hpp:483-503)
   if (valid) {
     double mult = coef * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
@@ -527,6 +577,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
       double r = double(my[L::RT + v * NQP + qp]);
+      if (has_pressure<ND, DEF>(v) && P::mfma) r -= ld[el * L::PER_D + L::RP + v * NQP + qp];
       if (stage) r -= rc[v * NQ + q];
       else rc[v * NQ + q] = r;
       r *= mult;
@@ -549,11 +600,15 @@ struct Ns_lds
 {
   using S = Shape<ND, RS>;
   static constexpr int NQP = S::NQP, NFQ = S::NFQ, NV = S::NV;
-  // arrays of T (padded), gradient phase
-  static constexpr int U = 0;                                     // [NV][NQP] state
-  static constexpr int G = U + NV * NQP;                          // [NV][NQP] one gradient component
-  static constexpr int SN = G + NV * NQP;                         // [ND][ND][NQP] reference-level normals
-  static constexpr int P1_T = SN + (DEF ? ND * ND * NQP : 0);
+  // arrays of T (padded), gradient phase. FP64 form: the state, one gradient component and the reference-level
+  // normals, contracted row by row. FP32-MFMA form: the face-referenced products u n_ij of the directions i that
+  // contribute to one physical component j, contracted in place.
+  static constexpr int NI = DEF ? ND : 1;                         // reference directions per physical component
+  static constexpr int U = 0;                                     // [NV][NQP] state (FP64 form)
+  static constexpr int G = U + NV * NQP;                          // [NV][NQP] one gradient component (FP64 form)
+  static constexpr int SN = G + NV * NQP;                         // [ND][ND][NQP] reference-level normals (FP64 form)
+  static constexpr int X = 0;                                     // [NI][NV][NQP] row operands (FP32-MFMA form)
+  static constexpr int P1_T = P::mfma ? NI * NV * NQP : SN + (DEF ? ND * ND * NQP : 0);
   // arrays of T (padded), residual phase
   static constexpr int F = 0;                                     // [NV][NQP] convective, then viscous flux
   static constexpr int RT = F + NV * NQP;                         // [2][NV][NQP] convective, viscous rate
@@ -564,8 +619,9 @@ struct Ns_lds
   static constexpr int SFN = SL + 2 * ND * NV * NFQ;              // [2ND][ND][NFQ] face normals
   static constexpr int P1_D = SFN + (DEF ? 2 * ND * ND * NFQ : 0);
   static constexpr int SB = 0;                                    // [2ND][NV][NFQ] numerical face fluxes
-  static constexpr int PD = SB + 2 * ND * NV * NFQ;               // [ND][NQP] FP32 form: pressure part p n
-  static constexpr int P2_D = PD + (P::mfma ? ND * NQP : 0);
+  static constexpr int PD = SB + 2 * ND * NV * NFQ;               // [ND][NQP] FP32-MFMA, deformed: pressure part p n_iv
+  static constexpr int RP = PD + ND * NQP;                        // [ND][NQP] FP32-MFMA, deformed: its derivative, summed
+  static constexpr int P2_D = PD + (P::mfma && DEF ? 2 * ND * NQP : 0);
   static constexpr int PER_D = P1_D > P2_D ? P1_D : P2_D;
 };
 
@@ -608,9 +664,8 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
   for (int v = 0; v < NV; ++v) {
     u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
     ut[v] = T(u[v]);
-    if (pt) my[L::U + v * NQP + qp] = ut[v];
+    if (!P::mfma && pt) my[L::U + v * NQP + qp] = ut[v];
   }
-  const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
   T bulk = 0, lap = 0;
   if (valid && m.av[cls]) {
     bulk = T(m.av[cls][size_t(e) * 2 * NQ + q]);
@@ -625,7 +680,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
       for (int j = 0; j < ND; ++j) {
         metd[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
         met[i][j] = T(metd[i][j]);
-        if (pt) my[L::SN + (i * ND + j) * NQP + qp] = met[i][j];
+        if (!P::mfma && pt) my[L::SN + (i * ND + j) * NQP + qp] = met[i][j];
       }
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
@@ -648,33 +703,75 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
   // component j at a time
   const double inv_h = valid ? 1. / m.nom_size[cls][e] : 1.;
   T grad[NV][ND];
-  for (int j = 0; j < ND; ++j) {
-    for (int i = DEF ? 0 : j; i < (DEF ? ND : j + 1); ++i) {
-      for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
-        T* b = lt + el2 * L::PER_T;
-        const double* db = ld + el2 * L::PER_D;
-        T x[RS], y[RS];
-        load_row(b + L::U + v * NQP + base, sd, x);
-        double b0 = db[L::SL + ((2 * i) * NV + v) * NFQ + r], b1 = db[L::SL + ((2 * i + 1) * NV + v) * NFQ + r];
-        if constexpr (DEF) {
-          T nk[RS];
-          load_row(b + L::SN + (i * ND + j) * NQP + base, sd, nk);
+  if constexpr (!P::mfma) {
+    for (int j = 0; j < ND; ++j) {
+      for (int i = DEF ? 0 : j; i < (DEF ? ND : j + 1); ++i) {
+        for_columns<ND, RS, NV, NT, false>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
+          T* b = lt + el2 * L::PER_T;
+          const double* db = ld + el2 * L::PER_D;
+          T x[RS], y[RS];
+          load_row(b + L::U + v * NQP + base, sd, x);
+          double b0 = db[L::SL + ((2 * i) * NV + v) * NFQ + r], b1 = db[L::SL + ((2 * i + 1) * NV + v) * NFQ + r];
+          if constexpr (DEF) {
+            T nk[RS];
+            load_row(b + L::SN + (i * ND + j) * NQP + base, sd, nk);
 #pragma unroll
-          for (int k = 0; k < RS; ++k) x[k] *= nk[k];
-          b0 *= db[L::SFN + ((2 * i) * ND + j) * NFQ + r];
-          b1 *= db[L::SFN + ((2 * i + 1) * ND + j) * NFQ + r];
-        }
-        con.faces(x, b0, b1, y);
-        if (ok) {
+            for (int k = 0; k < RS; ++k) x[k] *= nk[k];
+            b0 *= db[L::SFN + ((2 * i) * ND + j) * NFQ + r];
+            b1 *= db[L::SFN + ((2 * i + 1) * ND + j) * NFQ + r];
+          }
+          con.faces(x, b0, b1, y);
           T* out = b + L::G + v * NQP + base;
 #pragma unroll
           for (int k = 0; k < RS; ++k) out[k * sd] = (i == 0 || !DEF) ? y[k] : out[k * sd] + y[k];
-        }
-      });
+        });
+        __syncthreads();
+      }
+      for (int v = 0; v < NV; ++v) grad[v][j] = my[L::G + v * NQP + qp] * T(inv_h);
       __syncthreads();
     }
-    for (int v = 0; v < NV; ++v) grad[v][j] = my[L::G + v * NQP + qp] * T(inv_h);
-    __syncthreads();
+  } else {
+    for (int j = 0; j < ND; ++j) {
+      // u n_ij minus its value at the row's face 0 (face state times face normal), in FP64, then rounded
+      if (pt) {
+        const double* db = ld + el * L::PER_D;
+        for (int ii = 0; ii < L::NI; ++ii) {
+          const int i = DEF ? ii : j, r = Row<ND, RS>::fq(q, i);
+          const double nf = DEF ? db[L::SFN + ((2 * i) * ND + j) * NFQ + r] : 1.;
+          const double nq = DEF ? metd[i][j] : 1.;
+          for (int v = 0; v < NV; ++v)
+            my[L::X + (ii * NV + v) * NQP + qp] = T(u[v] * nq - db[L::SL + ((2 * i) * NV + v) * NFQ + r] * nf);
+        }
+      }
+      __syncthreads();
+      // every direction's rows in one pass, each row replaced by its derivative (a row is read and written by the
+      // one thread that owns it)
+      for (int ii = 0; ii < L::NI; ++ii) {
+        const int i = DEF ? ii : j;
+        for_columns<ND, RS, NV, NT, true>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
+          T* row = lt + el2 * L::PER_T + L::X + (ii * NV + v) * NQP + base;
+          const double* db = ld + el2 * L::PER_D;
+          double b0 = db[L::SL + ((2 * i) * NV + v) * NFQ + r], b1 = db[L::SL + ((2 * i + 1) * NV + v) * NFQ + r];
+          if constexpr (DEF) {
+            b0 *= db[L::SFN + ((2 * i) * ND + j) * NFQ + r];
+            b1 *= db[L::SFN + ((2 * i + 1) * ND + j) * NFQ + r];
+          }
+          T x[RS], y[RS];
+          load_row(row, sd, x);
+          con.faces(x, 0., b1 - b0, y);
+          if (ok)
+#pragma unroll
+            for (int k = 0; k < RS; ++k) row[k * sd] = y[k];
+        });
+      }
+      __syncthreads();
+      // (the next component's operands overwrite only this thread's own point)
+      for (int v = 0; v < NV; ++v) {
+        T g = 0;
+        for (int ii = 0; ii < L::NI; ++ii) g += my[L::X + (ii * NV + v) * NQP + qp];
+        grad[v][j] = g * T(inv_h);
+      }
+    }
   }
   if constexpr (DEF) {
     const T inv_det = T(1. / det);
@@ -692,28 +789,30 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
     }
   // residual along each direction (Spatial.hpp:444-470): the convective flux with its numerical face fluxes, then
   // the viscous flux, which is also extrapolated to the LDG blocks of the faces
+  const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
   for (int i = 0; i < ND; ++i) {
-    T nrm[ND], fc[NV];
+    T nrm[ND];
+    double nrmd[ND];
     for (int j = 0; j < ND; ++j) {
-      if constexpr (DEF) nrm[j] = met[i][j];
-      else nrm[j] = T(i == j);
+      if constexpr (DEF) nrm[j] = met[i][j], nrmd[j] = metd[i][j];
+      else nrm[j] = T(i == j), nrmd[j] = double(i == j);
     }
-    if constexpr (P::mfma) flux_conv_no_pressure<ND>(ut, nrm, fc);
-    else flux_conv<ND>(ut, nrm, fc);
-    if (i) __syncthreads();
-    if (pt) {
-      for (int v = 0; v < NV; ++v) my[L::F + v * NQP + qp] = fc[v];
-      if constexpr (P::mfma)
-        for (int v = 0; v < ND; ++v) ld[el * L::PER_D + L::PD + v * NQP + qp] = p64 * (DEF ? metd[i][v] : double(i == v));
-    }
+    // (FP32-MFMA: the face fluxes stored above and the gradient operands they alias need the barrier at i = 0 too)
+    if (i || P::mfma) __syncthreads();
+    if (pt)
+      conv_point<ND, DEF, P>(u, p64, nrmd, ld + el * L::PER_D + L::SB + (2 * i) * NV * NFQ + Row<ND, RS>::fq(q, i), NFQ,
+                             my + L::F + qp, ld + el * L::PER_D + L::PD + qp, NQP);
     __syncthreads();
     for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
       T* b = lt + el2 * L::PER_T;
-      const double* db = ld + el2 * L::PER_D;
+      double* db = ld + el2 * L::PER_D;
       T y[RS];
-      conv_column<ND, RS, DEF, P>(con, o, i, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
-                                  db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
-      if (ok) sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
+      conv_row<ND, RS, DEF, P>(con, o, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
+                               db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
+      if (ok) {
+        sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
+        pressure_row<ND, RS, DEF, P>(o, i, v, db + L::PD + (v % ND) * NQP + base, db + L::RP + (v % ND) * NQP + base, sd);
+      }
     });
     __syncthreads();
     if (pt)
@@ -742,8 +841,12 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
       if (ok) sub_row(b + L::RT + (NV + v) * NQP + base, sd, y, i == 0);
     });
   }
-  if (use_filter) filter_columns<ND, RS, 2 * NV, NT>(o, lt + L::RT, L::PER_T);
-  else __syncthreads();
+  if (use_filter) {
+    filter_columns<ND, RS, 2 * NV, NT>(o, lt + L::RT, L::PER_T);
+    if constexpr (P::mfma && DEF) filter_columns<ND, RS, ND, NT>(o, ld + L::RP, L::PER_D);
+  } else {
+    __syncthreads();
+  }
   // stage-0 update (Spatial.hpp:483-503): the convective part goes to the residual cache
   if (valid) {
     double mult = coef * m.tss[cls][size_t(e) * NQ + q] * inv_h;
@@ -752,6 +855,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
       double r = double(my[L::RT + v * NQP + qp]);
+      if (has_pressure<ND, DEF>(v) && P::mfma) r -= ld[el * L::PER_D + L::RP + v * NQP + qp];
       rc[v * NQ + q] = r;
       r += double(my[L::RT + (NV + v) * NQP + qp]);
       r *= mult;
