@@ -19,8 +19,11 @@ The JSON line also carries
                 host's cores for a few steps of the same mesh;
   parity_vs_cpu: relative L2 error per conserved variable between the FP64 GPU form and the oracle after those steps.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32|amr16] [--precision fp32_mfma|fp64]
-                       [--cpu-steps N] [--err-steps N]
+The batch workload (batch64) holds 64 independent cases in one mesh per row size; a step is one hx_update_cases
+per mesh, every case with its own time step.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32|amr16|batch64]
+                       [--precision fp32_mfma|fp64] [--cpu-steps N] [--err-steps N]
 """
 import argparse
 import ctypes
@@ -56,7 +59,28 @@ WORKLOADS = {
         desc="3D Euler, 32^3 Cartesian hexes, row_size 6, Mach 0.8 + 1% seeded Fourier perturbations, "
              "freestream boundaries, global time step at max_safety 0.7",
         build=lambda: configs.euler_box(32), visc=False, cpu_steps=6, bytes_per_dof_stage=128),
+    "batch64": dict(
+        desc="64 independent 3D Euler boxes on the unit cube, element counts log-uniform in [512, 8192] rounded to "
+             "cubes (142,551 elements), row_size uniform in {4, 5, 6}, Mach uniform in [0.3, 0.9] + 1% seeded "
+             "Fourier perturbations, freestream boundaries, a global time step per case at max_safety 0.7; one "
+             "mesh per row size, each kernel launched once per row size for all its cases",
+        build=None, visc=False, cpu_steps=10, bytes_per_dof_stage=128),
 }
+
+
+def build_groups(config):
+    """[(host mesh, n_cases or None, CPU-sample builder)] of a workload: one mesh, or for the batch one mesh of cases
+    per row size. The CPU sample is a list of meshes the oracle steps one at a time: the mesh itself, or every case
+    of the row size alone (the reference runs one Solver per case)."""
+    if config == "batch64":
+        cases = configs.batch_cases()
+        out = []
+        for rs, (m, ks) in configs.euler_batch(cases).items():
+            out.append((m, len(ks), lambda ks=ks, rs=rs: [configs.euler_box(cases[k]["n"], rs, cases[k]["mach"],
+                                                                            cases[k]["seed"]) for k in ks]))
+        return out
+    m = WORKLOADS[config]["build"]()
+    return [(m, None, lambda: [m])]
 
 
 def algorithmic_bytes(category, nd, nq, n_elem):
@@ -126,32 +150,48 @@ def download_state(dev, m):
     return [dev.download("state", c, out=np.zeros_like(m.state[c])) if m.n_elem[c] else None for c in (0, 1)]
 
 
-def rel_l2_per_var(got, ref, m):
-    """Relative L2 error per conserved variable over all elements (momentum components, mass, energy)."""
+def rel_l2_per_var(got, ref, nv):
+    """Relative L2 error per conserved variable over all elements of all groups (momentum components, mass,
+    energy); got/ref are lists of per-class state arrays."""
     errs = []
-    for v in range(m.nv):
-        num = sum(float(np.linalg.norm(got[c][:, v] - ref[c][:, v])) ** 2 for c in (0, 1) if m.n_elem[c])
-        den = sum(float(np.linalg.norm(ref[c][:, v])) ** 2 for c in (0, 1) if m.n_elem[c])
+    for v in range(nv):
+        pairs = [(g, r) for g, r in zip(got, ref) if r is not None and r.size]
+        num = sum(float(np.linalg.norm(g[:, v] - r[:, v])) ** 2 for g, r in pairs)
+        den = sum(float(np.linalg.norm(r[:, v])) ** 2 for _, r in pairs)
         errs.append((num / den) ** 0.5)
     return errs
 
 
-def timed_run(ctx, dev, m, opts, form, warmup, steps):
-    dev.set_precision(FORMS[form][0])
-    reset(dev, m)
-    status = abi.hx_update_status()
+def step(groups, opts):
+    for m, dev, nc, status in groups:
+        if nc is None:
+            dev.update(opts, status)
+        else:
+            dev.update_cases(opts, status)
+
+
+def fresh_status(nc):
+    return abi.hx_update_status() if nc is None else (abi.hx_update_status * nc)()
+
+
+def timed_run(ctx, groups, opts, form, warmup, steps):
+    groups = [(m, dev, nc, fresh_status(nc)) for m, dev, nc, _ in groups]
+    for m, dev, nc, _ in groups:
+        dev.set_precision(FORMS[form][0])
+        reset(dev, m)
     for _ in range(warmup):
-        dev.update(opts, status)
+        step(groups, opts)
     ctx.synchronize()
     ctx.set_timing(True)
     ctx.timing_reset()
     t0 = time.perf_counter()
     for _ in range(steps):
-        dev.update(opts, status)
+        step(groups, opts)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
-    return elapsed, ctx.timings(), status
+    st = groups[0][3]
+    return elapsed, ctx.timings(), (st if groups[0][2] is None else st[0])
 
 
 def main():
@@ -173,24 +213,29 @@ def main():
     cpu_steps = wl["cpu_steps"] if args.cpu_steps is None else args.cpu_steps
 
     t_build = time.perf_counter()
-    m = wl["build"]()
-    nd, nq = m.n_dim, m.nq
-    n_elem = sum(m.n_elem)
-    n_dof = n_elem * nq
+    hosts = build_groups(args.config)
+    nd, nv = hosts[0][0].n_dim, hosts[0][0].nv
+    n_elem = sum(sum(m.n_elem) for m, _, _ in hosts)
+    n_dof = sum(m.n_dof for m, _, _ in hosts)
     if wl["visc"]:
         visc, cond = configs.sutherland_air()
         opts = abi.update_options(max_safety=0.7, visc=visc, cond=cond, use_ldg=True)
     else:
         opts = abi.update_options(max_safety=0.7)
-    print(f"[bench] {args.config}: {n_elem} elements, {n_dof} DOF, mesh built in {time.perf_counter() - t_build:.1f} s",
-          file=sys.stderr, flush=True)
+    print(f"[bench] {args.config}: {n_elem} elements, {n_dof} DOF in {len(hosts)} mesh(es), built in "
+          f"{time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
 
     ctx = Context(0)
     copy_gbs = ctx.stream_copy(1 << 31, 10)  # achievable HBM bandwidth of a plain copy kernel on this box
-    dev = Device_mesh(ctx, m)
+    groups = []
+    for m, nc, _ in hosts:
+        dev = Device_mesh(ctx, m)
+        if nc is not None:
+            dev.set_cases(nc, m.case_id)
+        groups.append((m, dev, nc, None))
     runs = {}
     for form in FORMS:
-        elapsed, timings, status = timed_run(ctx, dev, m, opts, form, args.warmup, args.steps)
+        elapsed, timings, status = timed_run(ctx, groups, opts, form, args.warmup, args.steps)
         runs[form] = dict(elapsed=elapsed, timings=timings, value=n_dof * 2 * args.steps / elapsed)
         print(f"[bench] GPU {form}: {args.steps} steps in {elapsed:.3f} s, dt {status.time_step:.3e}",
               file=sys.stderr, flush=True)
@@ -200,29 +245,33 @@ def main():
     # FP32-MFMA against FP64 from the same initial state after err_steps steps
     finals = {}
     for form in FORMS:
-        dev.set_precision(FORMS[form][0])
-        reset(dev, m)
-        st = abi.hx_update_status()
+        gs = [(m, dev, nc, fresh_status(nc)) for m, dev, nc, _ in groups]
+        for m, dev, nc, _ in gs:
+            dev.set_precision(FORMS[form][0])
+            reset(dev, m)
         for _ in range(args.err_steps):
-            dev.update(opts, st)
-        finals[form] = download_state(dev, m)
-    fp32_err = rel_l2_per_var(finals["fp32_mfma"], finals["fp64"], m)
+            step(gs, opts)
+        finals[form] = [a for m, dev, _, _ in gs for a in download_state(dev, m)]
+    fp32_err = rel_l2_per_var(finals["fp32_mfma"], finals["fp64"], nv)
     print(f"[bench] FP32-MFMA vs FP64 after {args.err_steps} steps: rel L2 per variable "
           + " ".join(f"{e:.2e}" for e in fp32_err), file=sys.stderr, flush=True)
-    dev.set_precision(abi.HX_FP64)
+    for m, dev, _, _ in groups:
+        dev.set_precision(abi.HX_FP64)
 
-    # dominant kernel of the chosen form and its roofline
+    # dominant kernel of the chosen form and its roofline: algorithmic bytes of every launch in its category over the
+    # time spent in them (for one mesh: bytes per launch over the average launch time)
     dominant = max(timings, key=lambda k: timings[k][0])
     ms, launches, units = timings[dominant]
-    avg_ms = ms / launches
     cls = abi.HX_DEFORMED if dominant.startswith("deformed") else abi.HX_CARTESIAN
-    alg = algorithmic_bytes(dominant, nd, nq, m.n_elem[cls])
-    symbol = kernel_symbol(dominant, nd, m.row_size, args.precision)
-    roofline = {"bound": "hbm", "kernel": dominant, "symbol": symbol, "launch_ms": round(avg_ms, 5),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": measured_traffic(args.config, symbol),
+    per_round = [algorithmic_bytes(dominant, nd, m.nq, m.n_elem[cls]) for m, _, _ in hosts]
+    symbol = kernel_symbol(dominant, nd, hosts[0][0].row_size, args.precision) if len(hosts) == 1 else None
+    roofline = {"bound": "hbm", "kernel": dominant, "symbol": symbol, "launch_ms": round(ms / launches, 5),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "traffic": measured_traffic(args.config, symbol) if symbol else None,
                 "stream_copy_gbs": round(copy_gbs, 1)}
-    if alg is not None:
-        achieved = alg / (avg_ms * 1e-3) / 1e9
+    if None not in per_round:
+        alg = sum(per_round)  # one launch per mesh
+        achieved = alg * (launches / len(hosts)) / (ms * 1e-3) / 1e9
         roofline.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "algorithmic_bytes": alg})
     step_gbs = value * wl["bytes_per_dof_stage"] / 1e9
@@ -231,32 +280,41 @@ def main():
     breakdown = {form: {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches": v[1]}
                         for k, v in sorted(r["timings"].items())} for form, r in runs.items()}
 
-    # CPU baseline and full-size parity: the oracle runs cpu_steps from the initial state; the FP64 GPU form repeats
-    # them
+    # CPU baseline and full-size parity: the oracle runs cpu_steps from the initial state (for the batch, on every case
+    # alone, as one Solver per case would); the FP64 GPU form repeats them
     cpu = None
     parity = None
     if cpu_steps > 0:
         import oracle
         lib = oracle.load()
-        init_state = [a.copy() for a in m.state]
-        s = m.struct()
-        for c in (0, 1):
-            oracle.check(lib.oracle_write_face(ctypes.byref(s), c, None))
-        oracle.check(lib.oracle_prolong_refined(ctypes.byref(s), None, 0, 0))
-        cpu_s, threads = run_oracle_steps(m, opts, cpu_steps)
-        cpu = {"value": round(n_dof * 2 * cpu_steps / cpu_s, 1), "unit": "DOF-stage/s", "cores": threads,
-               "kind": "port",
-               "sample": f"{cpu_steps} time steps ({2 * cpu_steps} stages) of the same {n_elem}-element mesh, "
-                         f"FP64, OpenMP over {threads} threads, {cpu_s:.2f} s"}
+        cpu_s, cpu_dof, threads, got, ref = 0.0, 0, 1, [], []
+        for sample in (x for (_, _, make_sample) in hosts for x in make_sample()):
+            init_state = [a.copy() for a in sample.state]
+            s = sample.struct()
+            for c in (0, 1):
+                oracle.check(lib.oracle_write_face(ctypes.byref(s), c, None))
+            oracle.check(lib.oracle_prolong_refined(ctypes.byref(s), None, 0, 0))
+            t, threads = run_oracle_steps(sample, opts, cpu_steps)
+            cpu_s += t
+            cpu_dof += sample.n_dof
+            ref += [a.copy() for a in sample.state]
+            for c in (0, 1):
+                sample.state[c][...] = init_state[c]
+            d1 = Device_mesh(ctx, sample)
+            reset(d1, sample)
+            st = abi.hx_update_status()
+            for _ in range(cpu_steps):
+                d1.update(opts, st)
+            got += download_state(d1, sample)
+            d1.close()
         print(f"[bench] CPU oracle: {cpu_steps} steps in {cpu_s:.2f} s on {threads} threads", file=sys.stderr, flush=True)
-        oracle_state = [a.copy() for a in m.state]
-        for c in (0, 1):
-            m.state[c][...] = init_state[c]
-        reset(dev, m)
-        st = abi.hx_update_status()
-        for _ in range(cpu_steps):
-            dev.update(opts, st)
-        errs = rel_l2_per_var(download_state(dev, m), oracle_state, m)
+        what = (f"the same {n_elem}-element mesh" if len(groups) == 1 else
+                f"every case of the batch, one at a time ({cpu_dof} DOF)")
+        cpu = {"value": round(cpu_dof * 2 * cpu_steps / cpu_s, 1), "unit": "DOF-stage/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{cpu_steps} time steps ({2 * cpu_steps} stages) of {what}, FP64, OpenMP over {threads} "
+                         f"threads, {cpu_s:.2f} s"}
+        errs = rel_l2_per_var(got, ref, nv)
         parity = {"steps": cpu_steps, "form": "fp64", "rel_l2_per_var": [float(f"{e:.3e}") for e in errs],
                   "max_rel_l2": max(errs)}
 
@@ -266,8 +324,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32-mfma+f64" if args.precision == "fp32_mfma" else "f64", "data": "synthetic",
-        "config": {"workload": wl["desc"], "name": args.config, "n_elem": n_elem, "dof": n_dof, "row_size": m.row_size,
-                   "precision": args.precision},
+        "config": {"workload": wl["desc"], "name": args.config, "n_elem": n_elem, "dof": n_dof,
+                   "row_size": sorted({m.row_size for m, _, _ in hosts}), "precision": args.precision},
         "forms": {
             args.precision: {"value": round(value, 1), "ms_per_step": round(1e3 * elapsed / args.steps, 4)},
             other: {"value": round(runs[other]["value"], 1),
@@ -281,7 +339,8 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "parity_vs_cpu": parity, "kernel_breakdown": breakdown,
     }
     print(json.dumps(line))
-    dev.close()
+    for m, dev, _, _ in groups:
+        dev.close()
     ctx.close()
 
 

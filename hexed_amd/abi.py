@@ -201,6 +201,8 @@ def load_library(path=LIB_PATH):
         "hx_compute_prolong": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "hx_compute_restrict": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "hx_update": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.POINTER(hx_update_status)]),
+        "hx_mesh_set_cases": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "hx_update_cases": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.POINTER(hx_update_status)]),
         "hx_compute_residual": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options)]),
         "hx_apply_advection_bcs": (ctypes.c_int, [P]),
         "hx_apply_avc_diff_bcs": (ctypes.c_int, [P]),
@@ -236,4 +238,5 @@ EXPORTED_SYMBOLS = [
     "hx_update", "hx_compute_residual", "hx_apply_advection_bcs", "hx_apply_avc_diff_bcs", "hx_apply_avc_diff_flux_bcs",
     "hx_compute_advection", "hx_compute_smooth_av", "hx_compute_write_face_advection", "hx_compute_write_face_smooth_av",
     "hx_compute_prolong_advection", "hx_max_dt_advection", "hx_max_dt_smooth_av", "hx_update_art_visc_smoothness",
+    "hx_mesh_set_cases", "hx_update_cases",
 ]

@@ -7,6 +7,8 @@ configuration is the same on every platform and library version.
   config 2  3D Euler, 32^3 Cartesian hexes on [0, 1]^3, row size 6, Mach 0.8 + 1% Fourier perturbations
   config 3  3D Navier-Stokes, 48^3 deformed hexes (interior vertices jittered +-10% of spacing), row size 6,
             Re 1e4 on unit length, Mach 0.3, Sutherland air, constant artificial viscosity
+  config 4  3D Euler on a seeded 2:1-balanced tree with hanging faces
+  config 5  a batch of 64 independent 3D Euler boxes of log-uniform size, row size 4-6, Mach 0.3-0.9
 """
 import math
 
@@ -252,3 +254,27 @@ def euler_tree(base=16, rs=6, levels=3, frac=0.085, seed=0, mach=0.8, nd=3):
         return [rho * v for v in vel] + [rho, p / (GAMMA - 1) + 0.5 * rho * sum(v * v for v in vel)]
     mesh.set_state(m, init)
     return m
+
+
+def batch_cases(n_cases=64, seed=0, min_elem=512, max_elem=8192, row_sizes=(4, 5, 6), mach=(0.3, 0.9)):
+    """Config 5 case list: for each case an element count drawn log-uniform in [min_elem, max_elem] and rounded to a
+    cube n^3, a row size drawn uniformly from `row_sizes` and a freestream Mach number uniform in `mach`; returns
+    dicts {n, rs, mach, seed}."""
+    r = splitmix64_np(seed * 7919 + 12345, 3 * n_cases).reshape(n_cases, 3)
+    out = []
+    for k in range(n_cases):
+        count = math.exp(math.log(min_elem) + r[k, 0] * (math.log(max_elem) - math.log(min_elem)))
+        out.append(dict(n=max(2, int(round(count ** (1.0 / 3.0)))), rs=row_sizes[min(int(r[k, 1] * len(row_sizes)),
+                                                                                   len(row_sizes) - 1)],
+                        mach=mach[0] + r[k, 2] * (mach[1] - mach[0]), seed=seed * 1000 + k + 1))
+    return out
+
+
+def euler_batch(cases):
+    """Config 5 meshes: one batch mesh per row size (mesh.concat of the cases of that row size, each an
+    euler_box of its own size, Mach number and perturbation seed). Returns {rs: (mesh, [case indices])}."""
+    groups = {}
+    for k, c in enumerate(cases):
+        groups.setdefault(c["rs"], []).append(k)
+    return {rs: (mesh.concat([euler_box(cases[k]["n"], rs, cases[k]["mach"], cases[k]["seed"]) for k in ks]), ks)
+            for rs, ks in sorted(groups.items())}

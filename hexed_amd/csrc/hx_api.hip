@@ -48,6 +48,12 @@ struct hx_mesh
   std::vector<void*> allocs;
   const Kernel_set* ks = nullptr;
   int precision = HX_FP64;
+  // batch of independent cases (hx_mesh_set_cases): case of every element per class, and per case the Max_dt
+  // minimum (bits of a positive double) and the time step
+  int n_cases = 0;
+  int* case_id[2] = {};
+  unsigned long long* case_min = nullptr;
+  double* case_dt = nullptr;
 };
 
 namespace {
@@ -365,6 +371,30 @@ void resolve_timing(hx_ctx* ctx)
 }
 
 __global__ void k_fill(double* p, double v) { p[threadIdx.x] = v; }
+
+// batch of cases: per-case time step min(Max_dt, max_time_step) * chebyshev_step (Solver.cpp:843-848), and the time
+// step scale of every point set to its case's time step
+__global__ __launch_bounds__(256) void k_case_dt(const unsigned long long* __restrict__ case_min, double* __restrict__ case_dt,
+                                                 int n_cases, double max_time_step, double cheby)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_cases) return;
+  const double mdt = __longlong_as_double((long long)case_min[i]);
+  case_dt[i] = (max_time_step < mdt ? max_time_step : mdt) * cheby;
+}
+
+__global__ __launch_bounds__(256) void k_case_tss(double* __restrict__ tss, const int* __restrict__ case_id,
+                                                  const double* __restrict__ case_dt, int nq, size_t n)
+{
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256)
+    tss[i] = case_dt[case_id[i / nq]];
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* p, unsigned long long v, int n)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 
 // plain streaming copy, 16 bytes per thread per iteration
 __global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n)
@@ -947,6 +977,152 @@ hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* o)
   hx_kernel_options opts{1., 0, 1, o->use_filter};
   if (o->use_ldg) return hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
   return hx_compute_euler(mesh, &opts);
+}
+
+// ---- batch of independent cases -------------------------------------------------------------------------------
+
+hx_status hx_mesh_set_cases(hx_mesh* mesh, int n_cases, const int* case_cartesian, const int* case_deformed)
+{
+  if (!mesh) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  const Dev_mesh& m = mesh->dm;
+  if (n_cases < 1) return fail(ctx, HX_ERR_INVALID, "n_cases must be positive");
+  const int* ids[2] = {case_cartesian, case_deformed};
+  for (int c = 0; c < 2; ++c) {
+    if (m.n_elem[c] && !ids[c]) return fail(ctx, HX_ERR_INVALID, "missing case ids for a nonempty element class");
+    for (int e = 0; e < m.n_elem[c]; ++e)
+      if (ids[c][e] < 0 || ids[c][e] >= n_cases) return fail(ctx, HX_ERR_INVALID, "case id out of range");
+  }
+  // the cases must be independent: no face slot, connection or hanging face may join two of them
+  HX_TRY(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+  auto fetch = [&](const int* src, size_t n, std::vector<int>& dst) {
+    dst.resize(n);
+    return n ? hipMemcpy(dst.data(), src, n * sizeof(int), hipMemcpyDeviceToHost) : hipSuccess;
+  };
+  std::vector<int> slot_case(m.n_slot, -1), buf;
+  for (int c = 0; c < 2; ++c) {
+    HX_TRY(ctx, fetch(m.face_slot[c], size_t(m.n_elem[c]) * 2 * m.nd, buf), "download face slots");
+    for (size_t k = 0; k < buf.size(); ++k) {
+      int& sc = slot_case[buf[k]];
+      const int id = ids[c][k / (2 * m.nd)];
+      if (sc >= 0 && sc != id) return fail(ctx, HX_ERR_INVALID, "a face slot is shared by two cases");
+      sc = id;
+    }
+  }
+  for (int c = 0; c < 2; ++c) {
+    HX_TRY(ctx, fetch(m.con_slot[c], size_t(m.n_con[c]) * 2, buf), "download connections");
+    for (size_t k = 0; k < buf.size(); k += 2) {
+      int &a = slot_case[buf[k]], &b = slot_case[buf[k + 1]];
+      if (a >= 0 && b >= 0 && a != b) return fail(ctx, HX_ERR_INVALID, "a connection joins two cases");
+      if (a < 0) a = b;  // ghost faces of boundary connections and hanging-face mortars take their partner's case
+      if (b < 0) b = a;
+    }
+  }
+  HX_TRY(ctx, fetch(m.ref_slot, size_t(m.n_ref) * 5, buf), "download refined faces");
+  for (size_t k = 0; k < buf.size(); k += 5)
+    for (int j = 1; j < 5; ++j)
+      if (buf[k + j] >= 0 && slot_case[buf[k + j]] >= 0 && slot_case[buf[k + j]] != slot_case[buf[k]])
+        return fail(ctx, HX_ERR_INVALID, "a hanging face joins two cases");
+  auto alloc = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    mesh->allocs.push_back(*p);
+    return true;
+  };
+  int* dev_ids[2] = {};
+  void *cmin = nullptr, *cdt = nullptr;
+  for (int c = 0; c < 2; ++c) {
+    if (!alloc(reinterpret_cast<void**>(&dev_ids[c]), size_t(m.n_elem[c]) * sizeof(int)))
+      return fail(ctx, HX_ERR_HIP, "hipMalloc(case ids)");
+    if (m.n_elem[c])
+      HX_TRY(ctx, hipMemcpy(dev_ids[c], ids[c], size_t(m.n_elem[c]) * sizeof(int), hipMemcpyHostToDevice), "upload case ids");
+  }
+  if (!alloc(&cmin, size_t(n_cases) * 8) || !alloc(&cdt, size_t(n_cases) * 8))
+    return fail(ctx, HX_ERR_HIP, "hipMalloc(case time steps)");
+  mesh->n_cases = n_cases;
+  mesh->case_id[0] = dev_ids[0];
+  mesh->case_id[1] = dev_ids[1];
+  mesh->case_min = static_cast<unsigned long long*>(cmin);
+  mesh->case_dt = static_cast<double*>(cdt);
+  return HX_OK;
+}
+
+// Solver::update (Solver.cpp:834-886) of every case of the mesh at once. Each case takes its own global time step:
+// the Max_dt minimum is reduced per case, each point's time step scale is set to its case's time step, and the stage
+// kernels run with dt = 1, so that their coefficient times the time step scale is the same product as a single
+// case's dt times a scale of 1.
+hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
+{
+  if (!mesh || !o || !st) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  const int nc = mesh->n_cases;
+  if (nc < 1) return fail(ctx, HX_ERR_INVALID, "no cases registered (hx_mesh_set_cases)");
+  if (o->local_time) {
+    // every case takes the same step, min(1, max_time_step) * chebyshev_step
+    hx_update_status one{};
+    hx_status s = hx_update(mesh, o, &one);
+    if (s) return s;
+    for (int k = 0; k < nc; ++k) {
+      st[k].time_step = one.time_step;
+      st[k].flow_time += one.flow_time;
+      st[k].iteration += one.iteration;
+    }
+    return HX_OK;
+  }
+  hx_pde p{};
+  p.kind = o->use_ldg ? HX_PDE_NAVIER_STOKES : HX_PDE_EULER;
+  p.visc = o->visc;
+  p.cond = o->cond;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, &p, &visc, &l)) return s;
+  const int rs = mesh->dm.rs;
+  const char* name = pde_name(o->use_ldg != 0);
+  std::vector<double> dts(nc);
+  for (int i_flow = 0; i_flow < o->flow_iters; ++i_flow) {
+    const double safety = o->max_safety;
+    const double n_cheby = o->n_cheby_flow;
+    auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
+    const double max_cheby = cheby(n_cheby, n_cheby - 1);
+    for (int i_cheby = 0; i_cheby < n_cheby; ++i_cheby) {
+      // the constants of hx_max_dt with safety_conv = safety / max_cheby, safety_diff = safety
+      const double cc = hx_basis_max_cfl(rs) * (safety / max_cheby);
+      const double cd = -2 / hx_basis_tables::eigenvalss[rs - 2][1] * safety;
+      hipLaunchKernelGGL(k_fill_u64, dim3((nc + 255) / 256), dim3(256), 0, l.stream, mesh->case_min,
+                         (unsigned long long)__builtin_bit_cast(long long, DBL_MAX), nc);
+      HX_TRY(ctx, hipGetLastError(), "fill");
+      for (int c = 0; c < 2; ++c) {
+        Timer tm(ctx, std::string(cls_name(c)) + "/compute time step/" + name, mesh->dm.n_elem[c]);
+        HX_TRY(ctx, mesh->ks->max_dt_cases(l, c, visc, cc, cd, mesh->case_id[c], mesh->case_min), "max_dt_cases");
+      }
+      hipLaunchKernelGGL(k_case_dt, dim3((nc + 255) / 256), dim3(256), 0, l.stream, mesh->case_min, mesh->case_dt, nc,
+                         o->max_time_step, cheby(n_cheby, i_cheby));
+      HX_TRY(ctx, hipGetLastError(), "case dt");
+      for (int c = 0; c < 2; ++c) {
+        const size_t n = size_t(mesh->dm.n_elem[c]) * mesh->dm.nq;
+        if (!n) continue;
+        hipLaunchKernelGGL(k_case_tss, dim3(unsigned(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, l.stream,
+                           mesh->dm.tss[c], mesh->case_id[c], mesh->case_dt, mesh->dm.nq, n);
+        HX_TRY(ctx, hipGetLastError(), "case tss");
+      }
+      for (int i = 0; i < 2; ++i) {
+        hx_kernel_options opts{1., i, 0, o->use_filter};
+        hx_status s;
+        if ((s = hx_apply_state_bcs(mesh))) return s;
+        if (o->use_ldg && !i) s = hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
+        else s = hx_compute_euler(mesh, &opts);
+        if (s) return s;
+      }
+      HX_TRY(ctx, hipMemcpyAsync(dts.data(), mesh->case_dt, nc * sizeof(double), hipMemcpyDeviceToHost, l.stream),
+             "case dt readback");
+      HX_TRY(ctx, hipStreamSynchronize(l.stream), "case dt sync");
+      for (int k = 0; k < nc; ++k) {
+        st[k].time_step = dts[k];
+        st[k].flow_time += dts[k];
+      }
+    }
+  }
+  for (int k = 0; k < nc; ++k) ++st[k].iteration;
+  return HX_OK;
 }
 
 // ---- artificial-viscosity generator ---------------------------------------------------------------------------

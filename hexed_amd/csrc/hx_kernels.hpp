@@ -1028,39 +1028,50 @@ __global__ __launch_bounds__(256) void k_neighbor_reconcile(Dev_mesh m)
 // per workgroup, with the grid capped so that the atomics on the single result word stay few ----
 constexpr int max_dt_blocks = 2048;
 
+// 1/time step at one quadrature point (Spatial.hpp:790-818): the convective scale over the interpolated vertex spacing,
+// plus the diffusive one for Navier-Stokes
+template <int ND, int RS, bool VISC>
+__device__ inline double max_dt_scale(const Dev_mesh& m, const Ops<RS>* __restrict__ gops, const Ns_params& ns, int cls,
+                                      size_t e, int q, double max_cfl_c, double max_cfl_d)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ, NV = ND + 2, NVERT = 1 << ND;
+  double vs[NVERT];
+  for (int i = 0; i < NVERT; ++i) vs[i] = m.vertex_tss[cls][e * NVERT + i];
+  // math::interp (math.hpp:207-218)
+  int stride = NVERT;
+  for (int d = 0; d < ND; ++d) {
+    stride /= 2;
+    const double x = gops->node[(q / ipow(RS, ND - 1 - d)) % RS];
+    for (int i = 0; i < stride; ++i) vs[i] += x * (vs[i + stride] - vs[i]);
+  }
+  const double spacing = vs[0];
+  double u[NV];
+  const double* st = m.state[cls] + e * NV * NQ;
+  for (int v = 0; v < NV; ++v) u[v] = st[v * NQ + q];
+  double scale = char_speed<ND>(u) / max_cfl_c / spacing;
+  if constexpr (VISC) {
+    double bulk = 0., lap = 0.;
+    if (m.av[cls]) {
+      bulk = m.av[cls][e * 2 * NQ + q];
+      lap = m.av[cls][e * 2 * NQ + NQ + q];
+    }
+    scale += diffusivity<ND>(ns, u, bulk, lap) / max_cfl_d / spacing / spacing;
+  }
+  return scale;
+}
+
 template <int ND, int RS, bool VISC>
 __global__ __launch_bounds__(256) void k_max_dt(Dev_mesh m, const Ops<RS>* __restrict__ gops, Ns_params ns, int cls,
                                                 int is_local, double max_cfl_c, double max_cfl_d, double* result)
 {
-  constexpr int NQ = Shape<ND, RS>::NQ, NV = ND + 2, NVERT = 1 << ND;
+  constexpr int NQ = Shape<ND, RS>::NQ;
   __shared__ double wmin[256 / 64];
   const size_t n_items = size_t(m.n_elem[cls]) * NQ;
   double inv = DBL_MAX;
   for (size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x; gid < n_items; gid += size_t(gridDim.x) * 256) {
     const size_t e = gid / NQ;
     const int q = int(gid % NQ);
-    double vs[NVERT];
-    for (int i = 0; i < NVERT; ++i) vs[i] = m.vertex_tss[cls][e * NVERT + i];
-    // math::interp (math.hpp:207-218)
-    int stride = NVERT;
-    for (int d = 0; d < ND; ++d) {
-      stride /= 2;
-      const double x = gops->node[(q / ipow(RS, ND - 1 - d)) % RS];
-      for (int i = 0; i < stride; ++i) vs[i] += x * (vs[i + stride] - vs[i]);
-    }
-    const double spacing = vs[0];
-    double u[NV];
-    const double* st = m.state[cls] + e * NV * NQ;
-    for (int v = 0; v < NV; ++v) u[v] = st[v * NQ + q];
-    double scale = char_speed<ND>(u) / max_cfl_c / spacing;
-    if constexpr (VISC) {
-      double bulk = 0., lap = 0.;
-      if (m.av[cls]) {
-        bulk = m.av[cls][e * 2 * NQ + q];
-        lap = m.av[cls][e * 2 * NQ + NQ + q];
-      }
-      scale += diffusivity<ND>(ns, u, bulk, lap) / max_cfl_d / spacing / spacing;
-    }
+    const double scale = max_dt_scale<ND, RS, VISC>(m, gops, ns, cls, e, q, max_cfl_c, max_cfl_d);
     if (is_local) {
       m.tss[cls][e * NQ + q] = 1. / scale;
     } else {
@@ -1081,6 +1092,54 @@ __global__ __launch_bounds__(256) void k_max_dt(Dev_mesh m, const Ops<RS>* __res
     double b = wmin[0];
     for (int w = 1; w < 256 / 64; ++w) b = (wmin[w] < b) ? wmin[w] : b;
     if (b < DBL_MAX) atomicMin(reinterpret_cast<unsigned long long*>(result), __double_as_longlong(b));
+  }
+}
+
+// atomic minimum of a positive double into case_min[k] (the bits of positive doubles order like the doubles)
+__device__ inline void flush_case_min(unsigned long long* case_min, int k, double v)
+{
+  if (k >= 0 && v < DBL_MAX) atomicMin(case_min + k, (unsigned long long)__double_as_longlong(v));
+}
+
+// Global-time-step Max_dt of a batch of independent cases: as k_max_dt, with the minimum taken per case (element e
+// belongs to case case_id[e]). Each workgroup scans one contiguous chunk of the points, so that a thread meets few
+// case changes: it keeps a running minimum and flushes it into its case's slot whenever the case changes, and at the
+// end once per wavefront where the whole wavefront shares one case.
+template <int ND, int RS, bool VISC>
+__global__ __launch_bounds__(256) void k_max_dt_cases(Dev_mesh m, const Ops<RS>* __restrict__ gops, Ns_params ns,
+                                                      int cls, double max_cfl_c, double max_cfl_d,
+                                                      const int* __restrict__ case_id,
+                                                      unsigned long long* __restrict__ case_min)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ;
+  const size_t n_items = size_t(m.n_elem[cls]) * NQ;
+  const size_t chunk = (n_items + gridDim.x - 1) / gridDim.x;
+  const size_t lo = size_t(blockIdx.x) * chunk;
+  const size_t hi = lo + chunk < n_items ? lo + chunk : n_items;
+  int cur = -1;
+  double inv = DBL_MAX;
+  for (size_t gid = lo + threadIdx.x; gid < hi; gid += 256) {
+    const size_t e = gid / NQ;
+    const int q = int(gid % NQ);
+    const double c = 1. / max_dt_scale<ND, RS, VISC>(m, gops, ns, cls, e, q, max_cfl_c, max_cfl_d);
+    m.tss[cls][e * NQ + q] = 1.;
+    const int k = case_id[e];
+    if (k != cur) {
+      flush_case_min(case_min, cur, inv);
+      cur = k;
+      inv = DBL_MAX;
+    }
+    inv = (c < inv) ? c : inv;
+  }
+  const int c0 = __shfl(cur, 0, 64);
+  if (__all(cur == c0)) {
+    for (int off = 32; off > 0; off /= 2) {
+      const double o = __shfl_xor(inv, off, 64);
+      inv = (o < inv) ? o : inv;
+    }
+    if ((threadIdx.x & 63) == 0) flush_case_min(case_min, cur, inv);
+  } else {
+    flush_case_min(case_min, cur, inv);
   }
 }
 
@@ -1333,6 +1392,18 @@ struct Launchers
     else hipLaunchKernelGGL((k_max_dt<ND, RS, false>), g, b, 0, l.stream, l.m, ops(l), l.ns, cls, is_local, cc, cd, l.scalar);
     return hipGetLastError();
   }
+  static hipError_t max_dt_cases(const Launch& l, int cls, bool visc, double cc, double cd, const int* case_id,
+                                 unsigned long long* case_min)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    dim3 g(std::min(blocks_for(size_t(n) * S::NQ, 256), unsigned(max_dt_blocks))), b(256);
+    if (visc)
+      hipLaunchKernelGGL((k_max_dt_cases<ND, RS, true>), g, b, 0, l.stream, l.m, ops(l), l.ns, cls, cc, cd, case_id, case_min);
+    else
+      hipLaunchKernelGGL((k_max_dt_cases<ND, RS, false>), g, b, 0, l.stream, l.m, ops(l), l.ns, cls, cc, cd, case_id, case_min);
+    return hipGetLastError();
+  }
   static hipError_t prolong(const Launch& l, int n_var, int scale, int offset)
   {
     if (l.m.n_ref == 0) return hipSuccess;
@@ -1470,7 +1541,7 @@ struct Launchers
     static const Kernel_set s{&write_face,  &neighbor,      &local,     &neighbor_reconcile, &reconcile_ldg_flux,
                               &max_dt,      &prolong,       &restrict_, &state_bcs,          &flux_bcs,
                               &av_write_face, &av_neighbor, &adv_local, &sav_local,          &sav_neighbor_reconcile,
-                              &sav_reconcile, &av_max_dt,   &av_bcs,    &av_points};
+                              &sav_reconcile, &av_max_dt,   &av_bcs,    &av_points,          &max_dt_cases};
     return &s;
   }
 };

@@ -42,6 +42,9 @@ struct Kernel_set
   hipError_t (*av_max_dt)(const Launch&, int cls, int smooth, int is_local, double max_cfl_c, double max_cfl_d);
   hipError_t (*av_bcs)(const Launch&, int mode);
   hipError_t (*av_points)(const Launch&, int cls, int what, int av_rs, double mult, double us_max, double* resid);
+  // batch of independent cases: global-time-step Max_dt with a minimum per case
+  hipError_t (*max_dt_cases)(const Launch&, int cls, bool visc, double max_cfl_c, double max_cfl_d, const int* case_id,
+                             unsigned long long* case_min);
 };
 
 const Kernel_set* kernel_set_nd1(int rs);

@@ -205,5 +205,19 @@ class Device_mesh:
         self.ctx.check(self.lib.hx_update(self.ptr, ctypes.byref(opts), ctypes.byref(status)), "update")
         return status
 
+    # batch of independent cases --------------------------------------------------------------------------------
+    def set_cases(self, n_cases, case_id):
+        """case_id: per class, the case of every element (int32 arrays; None for an empty class)."""
+        self._case_id = [None if c is None else np.ascontiguousarray(c, dtype=np.int32) for c in case_id]
+        ptrs = [abi.iptr(c) if c is not None else None for c in self._case_id]
+        self.ctx.check(self.lib.hx_mesh_set_cases(self.ptr, n_cases, ptrs[0], ptrs[1]), "set_cases")
+        self.n_cases = n_cases
+
+    def update_cases(self, opts, status=None):
+        """One Solver::update of every case; `status` is a ctypes array of n_cases hx_update_status."""
+        status = status if status is not None else (abi.hx_update_status * self.n_cases)()
+        self.ctx.check(self.lib.hx_update_cases(self.ptr, ctypes.byref(opts), status), "update_cases")
+        return status
+
     def compute_residual(self, opts):
         self.ctx.check(self.lib.hx_compute_residual(self.ptr, ctypes.byref(opts)), "compute_residual")

@@ -369,6 +369,75 @@ def set_state(m, func):
             m.state[c][:, v] = vals[v]
 
 
+# ---- batches of independent cases -----------------------------------------------------------------------------
+
+def concat(meshes):
+    """One HostMesh holding several independent meshes of the same n_dim and row size (a batch of cases), with
+    `case_id[c]` the case index of every element of class c. Element order, slots, connections, normals, hanging
+    faces and boundary conditions of each mesh follow those of the previous ones; nothing connects two cases."""
+    m0 = meshes[0]
+    nd, rs = m0.n_dim, m0.row_size
+    assert all(m.n_dim == nd and m.row_size == rs for m in meshes), "a batch shares one n_dim and row size"
+    n_elem = [sum(m.n_elem[c] for m in meshes) for c in range(2)]
+    n_slot = sum(m.faces.shape[0] for m in meshes)
+    n_nslot = sum(m.normals.shape[0] for m in meshes)
+    n_con = [sum(len(m.con_dir[c]) for m in meshes) for c in range(2)]
+    n_ref = sum(m.ref_slot.shape[0] for m in meshes)
+    with_av = any(m.av[c] is not None for m in meshes for c in range(2))
+    out = HostMesh(nd, rs, n_elem[0], n_elem[1], n_slot, n_nslot, n_con[0], n_con[1], n_ref, with_av=with_av)
+    out.case_id = [np.zeros(n, np.int32) for n in n_elem]
+    out.n_cases = len(meshes)
+
+    def shifted(a, off):
+        a = a.copy()
+        a[a >= 0] += off
+        return a
+    e_off, c_off = [0, 0], [0, 0]
+    s_off = ns_off = r_off = b_off = 0
+    bc_con, bc_serial, qpos = [], [], [[], []]
+    for k, m in enumerate(meshes):
+        for c in range(2):
+            n, lo = m.n_elem[c], e_off[c]
+            sl = slice(lo, lo + n)
+            out.state[c][sl] = m.state[c]
+            out.rcache[c][sl] = m.rcache[c]
+            out.tss[c][sl] = m.tss[c]
+            if m.av[c] is not None:
+                out.av[c][sl] = m.av[c]
+            out.nom_size[c][sl] = m.nom_size[c]
+            out.vertex_tss[c][sl] = m.vertex_tss[c]
+            out.face_slot[c][sl] = m.face_slot[c] + s_off
+            out.case_id[c][sl] = k
+            if n:
+                qpos[c].append(m.qpos[c])
+            nc = len(m.con_dir[c])
+            out.con_slot[c][c_off[c]:c_off[c] + nc] = m.con_slot[c] + s_off
+            out.con_dir[c][c_off[c]:c_off[c] + nc] = m.con_dir[c]
+            if c == 1:
+                out.con_nslot[c_off[1]:c_off[1] + nc] = shifted(m.con_nslot, ns_off)
+                out.metrics[sl] = m.metrics
+                out.face_nslot[sl] = shifted(m.face_nslot, ns_off)
+        out.faces[s_off:s_off + m.faces.shape[0]] = m.faces
+        out.normals[ns_off:ns_off + m.normals.shape[0]] = m.normals
+        nr = m.ref_slot.shape[0]
+        out.ref_slot[r_off:r_off + nr] = shifted(m.ref_slot, s_off)
+        out.ref_stretch[r_off:r_off + nr] = m.ref_stretch
+        bc_con.append(m.bc_con + c_off[1])
+        bc_serial.append(m.bc_serial + b_off)
+        out.bcs += list(m.bcs)
+        for c in range(2):
+            e_off[c] += m.n_elem[c]
+            c_off[c] += len(m.con_dir[c])
+        s_off += m.faces.shape[0]
+        ns_off += m.normals.shape[0]
+        r_off += nr
+        b_off += len(m.bcs)
+    out.bc_con = np.concatenate(bc_con).astype(np.int32)
+    out.bc_serial = np.concatenate(bc_serial).astype(np.int32)
+    out.qpos = [np.concatenate(q) if q else None for q in qpos]
+    return out
+
+
 # ---- general builder (arbitrary connection directions) -------------------------------------------------------
 
 def face_vertex_inds(nd, i_dim, face_sign):

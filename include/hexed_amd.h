@@ -310,6 +310,20 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* opts, hx_update_stat
 /* Solver::compute_residual (Solver.cpp:899-913): writes the scaled residual into the residual cache. */
 hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* opts);
 
+/* ---- batch of independent cases ------------------------------------------------------------------------------
+ * No counterpart in the reference, which runs one Solver per case: many small cases of one (n_dim, row_size) share
+ * one mesh, and each kernel runs once over all of them. */
+/* Registers the cases: case_cartesian[i] / case_deformed[i] is the case (0 <= id < n_cases) of element i of each class
+ * (NULL for an empty class). Fails unless the cases are independent: no face slot, connection or hanging face may
+ * join two cases. */
+hx_status hx_mesh_set_cases(hx_mesh* mesh, int n_cases, const int* case_cartesian, const int* case_deformed);
+/* Solver::update (Solver.cpp:834-886) of every registered case at once, each case with its own global time step
+ * (its own Max_dt minimum, computed and applied on the device); status[n_cases] is advanced in place. Each case's
+ * result equals hx_update on a mesh holding that case alone. Afterwards the time step scale field holds each case's
+ * last time step (hx_update leaves 1 there). With local time stepping every case takes the same step, as in
+ * hx_update. */
+hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* opts, hx_update_status* status);
+
 /* ---- artificial-viscosity generator (the per-kernel entry points above also take HX_PDE_ADVECTION and
  * HX_PDE_SMOOTH_AV; the mesh needs its av, forcing and advection fields) ---------------------------------- */
 hx_status hx_apply_advection_bcs(hx_mesh* mesh);                 /* Solver.cpp:508-514, Flow_bc::apply_advection */

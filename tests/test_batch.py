@@ -1,0 +1,74 @@
+"""Batches of independent cases (config 5): the case list, the concatenated mesh, and -- through the oracle -- that
+the cases of a concatenated mesh do not interact. The GPU batch update is compared with single-case updates in
+test_gpu_batch.py."""
+import ctypes
+
+import numpy as np
+
+from hexed_amd import abi, configs, mesh
+
+from helpers import init_faces, o_call
+
+
+def small_cases():
+    return [configs.euler_box(2, 4, mach=0.3, seed=1), configs.euler_box(3, 4, mach=0.7, seed=2),
+            configs.euler_box(2, 4, mach=0.5, seed=3, deformed=True, jitter=0.1)]
+
+
+def test_batch_config():
+    cases = configs.batch_cases()
+    assert len(cases) == 64
+    counts = [c["n"] ** 3 for c in cases]
+    assert min(counts) >= 343 and max(counts) <= 9261   # cubes nearest to [512, 8192]
+    assert {c["rs"] for c in cases} == {4, 5, 6}
+    assert all(0.3 <= c["mach"] <= 0.9 for c in cases)
+    assert configs.batch_cases() == cases                # seeded
+
+
+def test_concat_is_valid_and_keeps_cases_apart():
+    cases = small_cases()
+    b = mesh.concat(cases)
+    lib = abi.load_library()
+    msg = ctypes.create_string_buffer(256)
+    assert lib.hx_mesh_validate(ctypes.byref(b.struct()), msg, 256) == abi.HX_OK, msg.value
+    for c in (0, 1):
+        assert b.n_elem[c] == sum(m.n_elem[c] for m in cases)
+        assert len(b.case_id[c]) == b.n_elem[c]
+    # every slot belongs to one case, and every connection and boundary condition stays within it
+    slot_case = -np.ones(b.faces.shape[0], int)
+    for c in (0, 1):
+        for e in range(b.n_elem[c]):
+            for s in b.face_slot[c][e]:
+                assert slot_case[s] in (-1, b.case_id[c][e])
+                slot_case[s] = b.case_id[c][e]
+    for c in (0, 1):
+        for s0, s1 in b.con_slot[c]:
+            assert -1 in (slot_case[s0], slot_case[s1]) or slot_case[s0] == slot_case[s1]
+    assert len(b.bcs) == len(cases)
+    for k, con in enumerate(b.bc_con):
+        s0 = b.con_slot[1][con][0]
+        assert b.bc_serial[k] == slot_case[s0]  # each case has its own freestream
+
+
+def test_concat_steps_like_separate_cases():
+    """With local time stepping nothing couples elements globally, so the oracle stepping the concatenated mesh must
+    reproduce the oracle stepping each case alone, bit for bit."""
+    cases = small_cases()
+    for m in cases:
+        init_faces(m)
+    b = mesh.concat(cases)
+    opts = abi.update_options(max_safety=0.5, local_time=True)
+    sb = b.struct()
+    st = abi.hx_update_status()
+    for _ in range(2):
+        o_call("oracle_update", ctypes.byref(sb), ctypes.byref(opts), ctypes.byref(st))
+    off = [0, 0]
+    for m in cases:
+        s = m.struct()
+        st1 = abi.hx_update_status()
+        for _ in range(2):
+            o_call("oracle_update", ctypes.byref(s), ctypes.byref(opts), ctypes.byref(st1))
+        for c in (0, 1):
+            n = m.n_elem[c]
+            assert np.array_equal(b.state[c][off[c]:off[c] + n], m.state[c])
+            off[c] += n
