@@ -48,23 +48,23 @@ WORKLOADS = {
         desc="3D Navier-Stokes, 48^3 deformed hexes (interior vertices jittered +-10% of spacing), row_size 6, "
              "Re 1e4 on unit length, Mach 0.3, Sutherland air, constant bulk artificial viscosity 1e-3 m^2/s, "
              "1% seeded Fourier perturbations, freestream boundaries, global time step at max_safety 0.7",
-        build=lambda: configs.ns_deformed(48), visc=True, cpu_steps=6, bytes_per_dof_stage=224),
+        build=lambda: configs.ns_deformed(48), visc=True, cpu_steps=20, bytes_per_dof_stage=224),
     "amr16": dict(
         desc="3D Euler on a seeded 2:1-balanced tree: base 16^3 Cartesian hexes, refinement probability 0.085 per "
              "level up to 3 levels (19,461 leaves, 31% of the base refined, 7,691 hanging faces through "
              "Prolong/Restrict_refined), row_size 6, Mach 0.8 + 1% seeded Fourier perturbations, freestream "
              "boundaries, global time step at max_safety 0.7",
-        build=lambda: configs.euler_tree(), visc=False, cpu_steps=6, bytes_per_dof_stage=128),
+        build=lambda: configs.euler_tree(), visc=False, cpu_steps=200, bytes_per_dof_stage=128),
     "euler32": dict(
         desc="3D Euler, 32^3 Cartesian hexes, row_size 6, Mach 0.8 + 1% seeded Fourier perturbations, "
              "freestream boundaries, global time step at max_safety 0.7",
-        build=lambda: configs.euler_box(32), visc=False, cpu_steps=6, bytes_per_dof_stage=128),
+        build=lambda: configs.euler_box(32), visc=False, cpu_steps=150, bytes_per_dof_stage=128),
     "batch64": dict(
         desc="64 independent 3D Euler boxes on the unit cube, element counts log-uniform in [512, 8192] rounded to "
              "cubes (142,551 elements), row_size uniform in {4, 5, 6}, Mach uniform in [0.3, 0.9] + 1% seeded "
              "Fourier perturbations, freestream boundaries, a global time step per case at max_safety 0.7; one "
              "mesh per row size, each kernel launched once per row size for all its cases",
-        build=None, visc=False, cpu_steps=10, bytes_per_dof_stage=128),
+        build=None, visc=False, cpu_steps=50, bytes_per_dof_stage=128),
 }
 
 

@@ -503,8 +503,9 @@ struct Euler_lds
   static constexpr int PER_D = LOOP_D > S::NV * S::NQP ? LOOP_D : S::NV * S::NQP;
 };
 
+// (5 waves per SIMD is what the LDS of rs 6 allows; the hint keeps the FP32-MFMA form's registers within it)
 template <int ND, int RS, bool DEF, class P>
-__global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_mesh m, Eops<RS, double> o,
+__global__ __launch_bounds__((form_block<ND, RS, P>())) __attribute__((amdgpu_waves_per_eu(5))) void k_local_euler(Dev_mesh m, Eops<RS, double> o,
                                                                            const Eops<RS, double>* od, double coef,
                                                                            int stage,
                                                                            int compute_residual, int use_filter)
@@ -528,21 +529,16 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
-  double det = 1.;
-  double met[ND][ND];
-  if constexpr (DEF) {
-    const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
-    for (int i = 0; i < ND; ++i)
-      for (int j = 0; j < ND; ++j) met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
-    det = valid ? mt[ND * ND * NQ + q] : 1.;
-  }
+  // deformed: each direction's reference-level normals are loaded in that direction's point phase, so that they
+  // are not held in registers across the row phases
+  const double* mt = m.metrics + (DEF ? size_t(e) * (ND * ND + 1) * NQ + q : 0);
   const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
   load_faces<ND, RS, NT>(m, cls, 0, ld + L::SB, L::PER_D);
   for (int i = 0; i < ND; ++i) {
     // volume flux along reference direction i (Spatial.hpp:404-419), in FP64
     double nrm[ND];
     for (int j = 0; j < ND; ++j) {
-      if constexpr (DEF) nrm[j] = met[i][j];
+      if constexpr (DEF) nrm[j] = valid ? mt[(i * ND + j) * NQ] : double(i == j);
       else nrm[j] = double(i == j);
     }
     if (i || P::mfma) __syncthreads();
@@ -572,7 +568,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_euler(Dev_m
   // stage update (Spatial.hpp:483-503)
   if (valid) {
     double mult = coef * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
-    if constexpr (DEF) mult /= det;
+    if constexpr (DEF) mult /= mt[ND * ND * NQ];
     double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
