@@ -503,9 +503,20 @@ struct Euler_lds
   static constexpr int PER_D = LOOP_D > S::NV * S::NQP ? LOOP_D : S::NV * S::NQP;
 };
 
-// (5 waves per SIMD is what the LDS of rs 6 allows; the hint keeps the FP32-MFMA form's registers within it)
+// Waves per SIMD that the workgroup's LDS allows (160 KB per CU, 4 SIMDs), at most 5: the register target given to
+// the compiler, so that registers do not lower the occupancy below what LDS permits
 template <int ND, int RS, bool DEF, class P>
-__global__ __launch_bounds__((form_block<ND, RS, P>())) __attribute__((amdgpu_waves_per_eu(5))) void k_local_euler(Dev_mesh m, Eops<RS, double> o,
+constexpr int euler_waves()
+{
+  using L = Euler_lds<ND, RS, DEF, P>;
+  constexpr int lds = Shape<ND, RS>::EPB * (int(sizeof(typename P::T)) * L::PER_T + 8 * L::PER_D);
+  constexpr int w = (163840 / lds) * ((form_block<ND, RS, P>() + 63) / 64) / 4;
+  return w < 1 ? 1 : (w > 5 ? 5 : w);
+}
+
+template <int ND, int RS, bool DEF, class P>
+__global__ __launch_bounds__((form_block<ND, RS, P>())) __attribute__((amdgpu_waves_per_eu(euler_waves<ND, RS, DEF, P>())))
+void k_local_euler(Dev_mesh m, Eops<RS, double> o,
                                                                            const Eops<RS, double>* od, double coef,
                                                                            int stage,
                                                                            int compute_residual, int use_filter)
@@ -1140,93 +1151,173 @@ __global__ __launch_bounds__(256) void k_max_dt_cases(Dev_mesh m, const Ops<RS>*
 }
 
 // ---- Prolong_refined / Restrict_refined (Spatial.hpp:153-285) --------------------------------------------
-// In-place product of `mat` with every row of a face block along tangential dimension `i_dim`
-// (the row loops of Spatial.hpp:183-199 and 259-275).
-template <int ND, int RS>
-__device__ inline void tangential_matvec(double* var_face, const double (&mat)[RS][RS], int i_dim)
+// The tangential operators of one fine face at one face point (a, b): for each tangential dimension d in order, the
+// matrix row of the point's coordinate along d (the row loops of Spatial.hpp:183-199 and 259-275), or for a
+// stretched dimension the factor fac, multiplied (prolongation) or divided (restriction). Per output the products
+// are summed in the reference's order: dimension 0 first, ascending k.
+template <int RS, bool DIVIDE>
+struct Face_rows
 {
-  constexpr int NFQ = ipow(RS, ND - 1);
-  const int qpoint_stride = ipow(RS, ND - 2 - i_dim);
-  for (int i_outer = 0; i_outer < NFQ / (RS * qpoint_stride); ++i_outer)
-    for (int i_inner = 0; i_inner < qpoint_stride; ++i_inner) {
-      double row[RS], out[RS];
-      for (int k = 0; k < RS; ++k) row[k] = var_face[(i_outer * RS + k) * qpoint_stride + i_inner];
-      for (int i = 0; i < RS; ++i) {
-        double acc = 0;
-        for (int k = 0; k < RS; ++k) acc += mat[i][k] * row[k];
-        out[i] = acc;
+  double r0[2][RS];  // row a of the dimension-0 matrix of either half
+  double r1[2][RS];  // row b of the dimension-1 matrix of either half
+  bool str[2];
+  double fac;
+  __device__ double scaled(double x) const { return DIVIDE ? x / fac : x * fac; }
+  // (the halves are selected element by element, so that no register array is indexed at run time)
+  // dimension-0 result at (a, k) for every k, from face data src (rs x rs, dimension 0 slowest)
+  __device__ void dim0(const double* src, int a, bool h0, double (&row)[RS]) const
+  {
+#pragma unroll
+    for (int k = 0; k < RS; ++k) {
+      if (str[0]) {
+        row[k] = scaled(src[a * RS + k]);
+      } else {
+        double acc = 0.;
+#pragma unroll
+        for (int j = 0; j < RS; ++j) acc += (h0 ? r0[1][j] : r0[0][j]) * src[j * RS + k];
+        row[k] = acc;
       }
-      for (int k = 0; k < RS; ++k) var_face[(i_outer * RS + k) * qpoint_stride + i_inner] = out[k];
+    }
+  }
+  // the 3D face value at b from the dimension-0 result of the face's half along dimension 0 (row_h0)
+  __device__ double dim1(const double (&row_0)[RS], const double (&row_1)[RS], bool h0, int b, bool h1) const
+  {
+    double x[RS];
+#pragma unroll
+    for (int k = 0; k < RS; ++k) x[k] = h0 ? row_1[k] : row_0[k];
+    if (str[1]) {
+      double xb = x[0];
+#pragma unroll
+      for (int k = 1; k < RS; ++k) xb = b == k ? x[k] : xb;
+      return scaled(xb);
+    }
+    double acc = 0.;
+#pragma unroll
+    for (int k = 0; k < RS; ++k) acc += (h1 ? r1[1][k] : r1[0][k]) * x[k];
+    return acc;
+  }
+  // the 2D face value at a (one tangential dimension)
+  __device__ double line(const double* src, int a, bool h0) const
+  {
+    if (str[0]) return scaled(src[a]);
+    double acc = 0.;
+#pragma unroll
+    for (int j = 0; j < RS; ++j) acc += (h0 ? r0[1][j] : r0[0][j]) * src[j];
+    return acc;
+  }
+};
+
+// fine faces of refined face i_ref (fewer along stretched dimensions), the stretch flags, and the half (0/1) that
+// tangential dimension d of fine face i_face takes (Spatial.hpp:175-181)
+template <int ND>
+__device__ inline int fine_faces(const Dev_mesh& m, int i_ref, bool (&str)[2])
+{
+  str[0] = ND > 1 && m.ref_stretch[2 * i_ref];
+  str[1] = ND > 2 && m.ref_stretch[2 * i_ref + 1];
+  int nf = 1 << (ND - 1);
+  for (int d = 0; d < ND - 1; ++d) nf /= 1 + str[d];
+  return nf;
+}
+template <int ND>
+__device__ inline int half_of(int i_face, int d, const bool (&str)[2])
+{
+  if constexpr (ND < 2) {
+    return 0;
+  } else {
+    const int face_stride = str[ND - 2] ? 1 : (1 << (ND - 2 - d));
+    return (i_face / face_stride) % 2;
+  }
+}
+
+// the matrix rows of face point p (coordinates a, b) of both halves, from mats[2][rs][rs]
+template <int ND, int RS, bool DIVIDE>
+__device__ inline void face_rows(Face_rows<RS, DIVIDE>& fr, const double (*mats)[RS][RS], int a, int b, int scale)
+{
+  fr.fac = 1. + scale;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int k = 0; k < RS; ++k) {
+      fr.r0[h][k] = ND > 1 ? mats[h][a][k] : 0.;
+      fr.r1[h][k] = ND > 2 ? mats[h][b][k] : 0.;
     }
 }
 
+// one thread per (refined face, variable, face point): the coarse face is staged in LDS and every fine face's value
+// at the point computed from it (the dimension-0 sums shared by the fine faces of the same half)
 template <int ND, int RS>
 __global__ __launch_bounds__(256) void k_prolong(Dev_mesh m, const Ops<RS>* __restrict__ ops, int n_var, int scale,
                                                  int offset)
 {
   constexpr int NFQ = ipow(RS, ND - 1);
-  constexpr int N_FACE = 1 << (ND - 1);
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int per = N_FACE * n_var;
-  const int i_ref = gid / per;
-  if (i_ref >= m.n_ref) return;
-  const int i_face = (gid % per) / n_var, v = gid % n_var;
-  const bool str[2] = {bool(m.ref_stretch[2 * i_ref]), bool(m.ref_stretch[2 * i_ref + 1])};
-  int nf = N_FACE;
-  for (int d = 0; d < ND - 1; ++d) nf /= 1 + str[d];
-  if (i_face >= nf) return;
-  const double* coarse = slot_ptr(m, m.ref_slot[5 * i_ref]) + offset * (ND + 2) * NFQ;
-  double* var_face = slot_ptr(m, m.ref_slot[5 * i_ref + 1 + i_face]) + offset * (ND + 2) * NFQ + v * NFQ;
-  double buf[NFQ];
-  for (int qq = 0; qq < NFQ; ++qq) buf[qq] = coarse[v * NFQ + qq];
-  for (int d = 0; d < ND - 1; ++d) {
-    if (str[d]) {
-      for (int qq = 0; qq < NFQ; ++qq) buf[qq] *= 1 + scale;
-    } else {
-      const int pw = ND - 2 - d;
-      const int face_stride = str[ND - 2] ? 1 : ipow(2, pw);
-      const int i_half = (i_face / face_stride) % 2;
-      tangential_matvec<ND, RS>(buf, ops->prolong[i_half], d);
+  constexpr int G = 256 / NFQ;
+  __shared__ double stage[G][NFQ];
+  const int g = threadIdx.x / NFQ, p = threadIdx.x % NFQ;
+  const size_t gi = size_t(blockIdx.x) * G + g;
+  const bool active = g < G && gi < size_t(m.n_ref) * n_var;
+  const int i_ref = active ? int(gi / n_var) : 0, v = active ? int(gi % n_var) : 0;
+  const int off = (offset * (ND + 2) + v) * NFQ;
+  if (active) stage[g][p] = slot_ptr(m, m.ref_slot[5 * i_ref])[off + p];
+  __syncthreads();
+  if (!active) return;
+  Face_rows<RS, false> fr;
+  const int nf = fine_faces<ND>(m, i_ref, fr.str);
+  const int a = ND > 2 ? p / RS : p, b = ND > 2 ? p % RS : 0;
+  face_rows<ND, RS, false>(fr, ops->prolong, a, b, scale);
+  if constexpr (ND == 1) {
+    slot_ptr(m, m.ref_slot[5 * i_ref + 1])[off + p] = stage[g][p];
+  } else if constexpr (ND == 2) {
+    for (int f = 0; f < nf; ++f) slot_ptr(m, m.ref_slot[5 * i_ref + 1 + f])[off + p] = fr.line(stage[g], a, half_of<ND>(f, 0, fr.str));
+  } else {
+    double row_0[RS], row_1[RS];
+    fr.dim0(stage[g], a, false, row_0);
+    fr.dim0(stage[g], a, !fr.str[0], row_1);
+    for (int f = 0; f < nf; ++f) {
+      const bool h0 = half_of<ND>(f, 0, fr.str), h1 = half_of<ND>(f, 1, fr.str);
+      slot_ptr(m, m.ref_slot[5 * i_ref + 1 + f])[off + p] = fr.dim1(row_0, row_1, h0, b, h1);
     }
   }
-  for (int qq = 0; qq < NFQ; ++qq) var_face[qq] = buf[qq];
 }
 
+// one thread per (refined face, variable, face point): the fine faces are staged in LDS, restricted in place
+// (Restrict_refined leaves the restricted data in the fine faces) and summed into the coarse face in fine-face order
 template <int ND, int RS>
 __global__ __launch_bounds__(256) void k_restrict(Dev_mesh m, const Ops<RS>* __restrict__ ops, int n_var, int scale,
                                                   int offset)
 {
   constexpr int NFQ = ipow(RS, ND - 1);
   constexpr int N_FACE = 1 << (ND - 1);
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int i_ref = gid / n_var, v = gid % n_var;
-  if (i_ref >= m.n_ref) return;
-  const bool str[2] = {bool(m.ref_stretch[2 * i_ref]), bool(m.ref_stretch[2 * i_ref + 1])};
-  int nf = N_FACE;
-  for (int d = 0; d < ND - 1; ++d) nf /= 1 + str[d];
-  double* coarse = slot_ptr(m, m.ref_slot[5 * i_ref]) + offset * (ND + 2) * NFQ + v * NFQ;
-  double acc[NFQ];
-  for (int qq = 0; qq < NFQ; ++qq) acc[qq] = 0.;
-  for (int i_face = 0; i_face < nf; ++i_face) {
-    double* var_face = slot_ptr(m, m.ref_slot[5 * i_ref + 1 + i_face]) + offset * (ND + 2) * NFQ + v * NFQ;
-    double buf[NFQ];
-    for (int qq = 0; qq < NFQ; ++qq) buf[qq] = var_face[qq];
-    for (int d = 0; d < ND - 1; ++d) {
-      if (str[d]) {
-        for (int qq = 0; qq < NFQ; ++qq) buf[qq] /= 1 + scale;
-      } else {
-        const int pw = ND - 2 - d;
-        const int face_stride = str[ND - 2] ? 1 : ipow(2, pw);
-        const int i_half = (i_face / face_stride) % 2;
-        tangential_matvec<ND, RS>(buf, ops->restrict_[i_half], d);
-      }
+  constexpr int G = 256 / NFQ;
+  __shared__ double stage[G][N_FACE][NFQ];
+  const int g = threadIdx.x / NFQ, p = threadIdx.x % NFQ;
+  const size_t gi = size_t(blockIdx.x) * G + g;
+  const bool active = g < G && gi < size_t(m.n_ref) * n_var;
+  const int i_ref = active ? int(gi / n_var) : 0, v = active ? int(gi % n_var) : 0;
+  Face_rows<RS, true> fr;
+  fr.str[0] = fr.str[1] = false;
+  const int nf = active ? fine_faces<ND>(m, i_ref, fr.str) : 0;
+  const int off = (offset * (ND + 2) + v) * NFQ;
+  for (int f = 0; f < nf; ++f) stage[g][f][p] = slot_ptr(m, m.ref_slot[5 * i_ref + 1 + f])[off + p];
+  __syncthreads();
+  if (!active) return;
+  const int a = ND > 2 ? p / RS : p, b = ND > 2 ? p % RS : 0;
+  face_rows<ND, RS, true>(fr, ops->restrict_, a, b, scale);
+  double acc = 0.;
+  for (int f = 0; f < nf; ++f) {
+    double r;
+    if constexpr (ND == 1) {
+      r = stage[g][f][p];
+    } else if constexpr (ND == 2) {
+      r = fr.line(stage[g][f], a, half_of<ND>(f, 0, fr.str));
+    } else {
+      double row[RS];
+      fr.dim0(stage[g][f], a, half_of<ND>(f, 0, fr.str), row);
+      r = fr.dim1(row, row, false, b, half_of<ND>(f, 1, fr.str));
     }
-    for (int qq = 0; qq < NFQ; ++qq) {
-      var_face[qq] = buf[qq];
-      acc[qq] += buf[qq];
-    }
+    slot_ptr(m, m.ref_slot[5 * i_ref + 1 + f])[off + p] = r;
+    acc += r;
   }
-  for (int qq = 0; qq < NFQ; ++qq) coarse[qq] = acc[qq];
+  slot_ptr(m, m.ref_slot[5 * i_ref])[off + p] = acc;
 }
 
 // ---- flow boundary conditions (Boundary_condition.cpp) ---------------------------------------------------
@@ -1403,15 +1494,17 @@ struct Launchers
   static hipError_t prolong(const Launch& l, int n_var, int scale, int offset)
   {
     if (l.m.n_ref == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_prolong<ND, RS>), dim3(blocks_for(size_t(l.m.n_ref) * (1 << (ND - 1)) * n_var, 256)), dim3(256), 0,
-                       l.stream, l.m, ops(l), n_var, scale, offset);
+    constexpr int G = 256 / S::NFQ;
+    hipLaunchKernelGGL((k_prolong<ND, RS>), dim3(blocks_for(size_t(l.m.n_ref) * n_var, G)), dim3(256), 0, l.stream, l.m,
+                       ops(l), n_var, scale, offset);
     return hipGetLastError();
   }
   static hipError_t restrict_(const Launch& l, int n_var, int scale, int offset)
   {
     if (l.m.n_ref == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_restrict<ND, RS>), dim3(blocks_for(size_t(l.m.n_ref) * n_var, 256)), dim3(256), 0, l.stream, l.m,
-                       ops(l), n_var, scale, offset);
+    constexpr int G = 256 / S::NFQ;
+    hipLaunchKernelGGL((k_restrict<ND, RS>), dim3(blocks_for(size_t(l.m.n_ref) * n_var, G)), dim3(256), 0, l.stream,
+                       l.m, ops(l), n_var, scale, offset);
     return hipGetLastError();
   }
   static hipError_t state_bcs(const Launch& l)
