@@ -54,7 +54,23 @@ struct hx_mesh
   int* case_id[2] = {};
   unsigned long long* case_min = nullptr;
   double* case_dt = nullptr;
+  // Max_dt fused into the last stage of hx_update: every call that can change device data (a kernel launch, an
+  // upload, handing out a device pointer) advances `gen`; the next hx_update takes the fused minimum only if `gen` is
+  // where the previous hx_update left it and its time-step settings are the same
+  long long gen = 0;
+  double* dt_block_min = nullptr;  // one minimum per workgroup of the Euler Local, both classes
+  long long n_dt_blocks = 0;
+  Dt_fuse fuse{};                  // the fusion the next Euler Local launch performs (block_min == nullptr: none)
+  struct
+  {
+    bool valid = false;
+    long long gen = 0;
+    double msc = 0, msd = 0;
+    int use_ldg = 0;
+    hx_transport visc{}, cond{};
+  } dt_cache;
 };
+
 
 namespace {
 
@@ -260,6 +276,7 @@ std::string validate(const hx_host_mesh* d, const Sizes& z)
 
 Launch make_launch(hx_mesh* mesh)
 {
+  ++mesh->gen;
   Launch l{};
   l.stream = mesh->ctx->stream;
   l.m = mesh->dm;
@@ -268,6 +285,7 @@ Launch make_launch(hx_mesh* mesh)
   l.eops = mesh->ctx->eops[mesh->dm.rs];
   l.scalar = mesh->ctx->scalar;
   l.precision = mesh->precision;
+  l.fuse = mesh->fuse;
   return l;
 }
 
@@ -394,6 +412,24 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* p, unsigne
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// minimum of v[0..n) by atomic minimum into *out (prefilled with DBL_MAX), one atomic per workgroup
+__global__ __launch_bounds__(256) void k_min_reduce(const double* __restrict__ v, long long n, double* out)
+{
+  __shared__ double w[4];
+  double m = DBL_MAX;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) m = v[i] < m ? v[i] : m;
+  for (int off = 32; off > 0; off /= 2) {
+    const double x = __shfl_xor(m, off, 64);
+    m = x < m ? x : m;
+  }
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) m = w[k] < m ? w[k] : m;
+    if (m < DBL_MAX) atomicMin(reinterpret_cast<unsigned long long*>(out), (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // plain streaming copy, 16 bytes per thread per iteration
@@ -654,6 +690,7 @@ hx_status hx_mesh_destroy(hx_mesh* mesh)
 hx_status hx_mesh_upload(hx_mesh* mesh, int field, int cls, const double* host, size_t n)
 {
   if (!mesh) return HX_ERR_INVALID;
+  ++mesh->gen;
   double** p = field_ptr(mesh->dm, field, cls);
   long long sz = field_size(mesh->dm, field, cls);
   if (!p || sz < 0) return fail(mesh->ctx, HX_ERR_INVALID, "unknown field or class");
@@ -692,6 +729,7 @@ hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision)
 {
   if (!mesh) return HX_ERR_INVALID;
   if (precision != HX_FP64 && precision != HX_FP32_MFMA) return fail(mesh->ctx, HX_ERR_INVALID, "unknown precision");
+  ++mesh->gen;
   mesh->precision = precision;
   return HX_OK;
 }
@@ -699,6 +737,7 @@ hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision)
 hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size_t* n)
 {
   if (!mesh || !out || !n) return HX_ERR_INVALID;
+  ++mesh->gen;  // the caller may write through the pointer
   double** p = field_ptr(mesh->dm, field, cls);
   long long sz = field_size(mesh->dm, field, cls);
   if (!p || sz < 0) return fail(mesh->ctx, HX_ERR_INVALID, "unknown field or class");
@@ -939,32 +978,90 @@ hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset) { return hx_
 hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
 {
   if (!mesh || !o || !st) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  // with a global time step, stage 1's Local also computes the next step's Max_dt (the state it writes is the state
+  // that Max_dt would read next); local time stepping writes the time step scale in Max_dt, so it is not fused
+  const bool fusable = !o->local_time;
+  if (fusable && !mesh->dt_block_min) {
+    const long long epb = std::max(1, 256 / mesh->dm.nq);  // elements per workgroup of the Euler Local
+    mesh->n_dt_blocks = (mesh->dm.n_elem[0] + epb - 1) / epb + (mesh->dm.n_elem[1] + epb - 1) / epb;
+    void* p = nullptr;
+    HX_TRY(ctx, hipMalloc(&p, std::max<long long>(mesh->n_dt_blocks, 1) * sizeof(double)), "hipMalloc(time step minima)");
+    mesh->allocs.push_back(p);
+    mesh->dt_block_min = static_cast<double*>(p);
+  }
+  const double safety = o->max_safety;
+  const double n_cheby = o->n_cheby_flow;
+  // math::chebyshev_step (math.cpp:104-107)
+  auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
+  const double max_cheby = cheby(n_cheby, n_cheby - 1);
+  const double msc = safety / max_cheby, msd = safety;
+  auto& c = mesh->dt_cache;
+  bool pending = fusable && c.valid && c.gen == mesh->gen && c.msc == msc && c.msd == msd &&
+                 c.use_ldg == o->use_ldg && !std::memcmp(&c.visc, &o->visc, sizeof(hx_transport)) &&
+                 !std::memcmp(&c.cond, &o->cond, sizeof(hx_transport));
+  c.valid = false;
+  Ns_params ns{};
+  if (o->use_ldg) {
+    hx_pde p{};
+    p.kind = HX_PDE_NAVIER_STOKES;
+    p.visc = o->visc;
+    p.cond = o->cond;
+    Launch lp{};
+    bool visc;
+    if (hx_status s = pde_check(mesh, &p, &visc, &lp)) return s;
+    ns = lp.ns;
+  }
   for (int i_flow = 0; i_flow < o->flow_iters; ++i_flow) {
-    const double safety = o->max_safety;
-    const double n_cheby = o->n_cheby_flow;
-    // math::chebyshev_step (math.cpp:104-107)
-    auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
-    const double max_cheby = cheby(n_cheby, n_cheby - 1);
     for (int i_cheby = 0; i_cheby < n_cheby; ++i_cheby) {
       hx_kernel_options ko{0., 0, 0, o->use_filter};
       double mdt;
-      hx_status s = o->use_ldg
-                        ? hx_max_dt_navier_stokes(mesh, &ko, safety / max_cheby, safety, o->local_time, o->visc, o->cond, &mdt)
-                        : hx_max_dt_euler(mesh, &ko, safety / max_cheby, safety, o->local_time, &mdt);
-      if (s) return s;
+      hx_status s;
+      if (pending) {
+        {
+          Timer tm(ctx, "fused/compute time step/reduce", mesh->n_dt_blocks);
+          hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, ctx->stream, ctx->scalar, DBL_MAX);
+          const unsigned g = unsigned(std::min<long long>(256, (mesh->n_dt_blocks + 255) / 256));
+          if (g) hipLaunchKernelGGL(k_min_reduce, dim3(g), dim3(256), 0, ctx->stream, mesh->dt_block_min, mesh->n_dt_blocks, ctx->scalar);
+          HX_TRY(ctx, hipGetLastError(), "time step reduction");
+        }
+        HX_TRY(ctx, hipMemcpyAsync(&mdt, ctx->scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "max_dt readback");
+        HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "max_dt sync");
+      } else {
+        s = o->use_ldg ? hx_max_dt_navier_stokes(mesh, &ko, msc, msd, o->local_time, o->visc, o->cond, &mdt)
+                       : hx_max_dt_euler(mesh, &ko, msc, msd, o->local_time, &mdt);
+        if (s) return s;
+      }
       const double dt = std::min(mdt, o->max_time_step) * cheby(n_cheby, i_cheby);
       for (int i = 0; i < 2; ++i) {
         hx_kernel_options opts{dt, i, 0, o->use_filter};
         if ((s = hx_apply_state_bcs(mesh))) return s;
-        if (o->use_ldg && !i) s = hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
-        else s = hx_compute_euler(mesh, &opts);
+        if (o->use_ldg && !i) {
+          s = hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
+        } else {
+          if (i == 1 && fusable)
+            mesh->fuse = Dt_fuse{mesh->dt_block_min, 0, o->use_ldg ? 1 : 0, hx_basis_max_cfl(mesh->dm.rs) * msc,
+                                 -2 / hx_basis_tables::eigenvalss[mesh->dm.rs - 2][1] * msd, ns};
+          s = hx_compute_euler(mesh, &opts);
+          mesh->fuse = Dt_fuse{};
+        }
         if (s) return s;
       }
+      pending = fusable;
       st->time_step = dt;
       st->flow_time += dt;
     }
   }
   ++st->iteration;
+  if (fusable) {
+    c.valid = true;
+    c.gen = mesh->gen;
+    c.msc = msc;
+    c.msd = msd;
+    c.use_ldg = o->use_ldg;
+    c.visc = o->visc;
+    c.cond = o->cond;
+  }
   return HX_OK;
 }
 

@@ -112,6 +112,18 @@ struct Ns_params
   Transport visc, cond;
 };
 
+// Max_dt of the next time step fused into the last stage's Local (the state it writes is the state the next Max_dt
+// reads): each workgroup stores its minimum of 1/scale in block_min[block_offset + blockIdx.x] (a plain store; the
+// minimum over the workgroups is taken by a separate small reduction). block_min == nullptr: not fused.
+struct Dt_fuse
+{
+  double* block_min;
+  long long block_offset;
+  int visc;  // include the diffusive scale (Max_dt of Navier-Stokes)
+  double max_cfl_c, max_cfl_d;
+  Ns_params ns;
+};
+
 // Connection_direction (Kernel_connection.hpp:7-37) from the packed form of hx_host_mesh::con_dir
 struct Dir
 {

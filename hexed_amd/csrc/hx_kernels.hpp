@@ -485,6 +485,35 @@ __device__ inline void pressure_row(const Eops<RS, double>& o, int i, int v, con
   }
 }
 
+// 1/time step at one quadrature point (Spatial.hpp:790-818): the convective scale over the interpolated vertex spacing,
+// plus the diffusive one for Navier-Stokes
+template <int ND, int RS, bool VISC>
+__device__ inline double max_dt_scale_u(const Dev_mesh& m, const Ops<RS>* __restrict__ gops, const Ns_params& ns, int cls,
+                                        size_t e, int q, const double (&u)[ND + 2], double max_cfl_c, double max_cfl_d)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ, NVERT = 1 << ND;
+  double vs[NVERT];
+  for (int i = 0; i < NVERT; ++i) vs[i] = m.vertex_tss[cls][e * NVERT + i];
+  // math::interp (math.hpp:207-218)
+  int stride = NVERT;
+  for (int d = 0; d < ND; ++d) {
+    stride /= 2;
+    const double x = gops->node[(q / ipow(RS, ND - 1 - d)) % RS];
+    for (int i = 0; i < stride; ++i) vs[i] += x * (vs[i + stride] - vs[i]);
+  }
+  const double spacing = vs[0];
+  double scale = char_speed<ND>(u) / max_cfl_c / spacing;
+  if constexpr (VISC) {
+    double bulk = 0., lap = 0.;
+    if (m.av[cls]) {
+      bulk = m.av[cls][e * 2 * NQ + q];
+      lap = m.av[cls][e * 2 * NQ + NQ + q];
+    }
+    scale += diffusivity<ND>(ns, u, bulk, lap) / max_cfl_d / spacing / spacing;
+  }
+  return scale;
+}
+
 // ---- Local, Euler (Spatial.hpp:326-509 with Navier_stokes<false>) -----------------------------------------
 template <int ND, int RS, bool DEF, class P>
 struct Euler_lds
@@ -503,28 +532,36 @@ struct Euler_lds
   static constexpr int PER_D = LOOP_D > S::NV * S::NQP ? LOOP_D : S::NV * S::NQP;
 };
 
+// The Euler Local's workgroup: whole wavefronts in both forms (the fused time-step reduction shuffles across full
+// wavefronts)
+template <int ND, int RS>
+constexpr int euler_block()
+{
+  return (Shape<ND, RS>::BLOCK + 63) / 64 * 64;
+}
+
 // Waves per SIMD that the workgroup's LDS allows (160 KB per CU, 4 SIMDs), at most 5: the register target given to
 // the compiler, so that registers do not lower the occupancy below what LDS permits
 template <int ND, int RS, bool DEF, class P>
 constexpr int euler_waves()
 {
   using L = Euler_lds<ND, RS, DEF, P>;
-  constexpr int lds = Shape<ND, RS>::EPB * (int(sizeof(typename P::T)) * L::PER_T + 8 * L::PER_D);
-  constexpr int w = (163840 / lds) * ((form_block<ND, RS, P>() + 63) / 64) / 4;
+  // (plus the time-step reduction's one double per wavefront)
+  constexpr int lds = Shape<ND, RS>::EPB * (int(sizeof(typename P::T)) * L::PER_T + 8 * L::PER_D) +
+                      8 * (euler_block<ND, RS>() / 64);
+  constexpr int w = (163840 / lds) * (euler_block<ND, RS>() / 64) / 4;
   return w < 1 ? 1 : (w > 5 ? 5 : w);
 }
 
 template <int ND, int RS, bool DEF, class P>
-__global__ __launch_bounds__((form_block<ND, RS, P>())) __attribute__((amdgpu_waves_per_eu(euler_waves<ND, RS, DEF, P>())))
-void k_local_euler(Dev_mesh m, Eops<RS, double> o,
-                                                                           const Eops<RS, double>* od, double coef,
-                                                                           int stage,
-                                                                           int compute_residual, int use_filter)
+__global__ __launch_bounds__((euler_block<ND, RS>())) __attribute__((amdgpu_waves_per_eu(euler_waves<ND, RS, DEF, P>())))
+void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, double coef, int stage,
+                   int compute_residual, int use_filter, Dt_fuse fuse, const Ops<RS>* __restrict__ gops)
 {
   using S = Shape<ND, RS>;
   using L = Euler_lds<ND, RS, DEF, P>;
   using T = typename P::T;
-  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = form_block<ND, RS, P>();
+  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = euler_block<ND, RS>();
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
   __shared__ T lt[EPB * L::PER_T];
   __shared__ double ld[EPB * L::PER_D];
@@ -594,11 +631,33 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o,
     if (!compute_residual)
       for (int v = 0; v < NV; ++v) sw[v * NQ + q] = u[v];
   }
+  // the next time step's Max_dt (Spatial.hpp:790-818, global time step) on the state just written: each point's
+  // 1/scale here (its loads then overlap the face writes below), the workgroup minimum after them
+  double c = DBL_MAX;
+  if (fuse.block_min && valid && !compute_residual) {
+    const double scale = fuse.visc ? max_dt_scale_u<ND, RS, true>(m, gops, fuse.ns, cls, e, q, u, fuse.max_cfl_c, fuse.max_cfl_d)
+                                   : max_dt_scale_u<ND, RS, false>(m, gops, fuse.ns, cls, e, q, u, fuse.max_cfl_c, fuse.max_cfl_d);
+    c = 1. / scale;
+  }
   // Write_face of the updated state (Spatial.hpp:507)
   if (pt)
     for (int v = 0; v < NV; ++v) ld[el * L::PER_D + L::U + v * NQP + qp] = u[v];
   __syncthreads();
   write_faces_from<ND, RS, NT>(m, cls, o, ld + L::U, L::PER_D);
+  if (fuse.block_min) {
+    __shared__ double wmin[NT / 64];
+    for (int off = 32; off > 0; off /= 2) {
+      const double x = __shfl_xor(c, off, 64);
+      c = (x < c) ? x : c;
+    }
+    if ((t & 63) == 0) wmin[t / 64] = c;
+    __syncthreads();
+    if (t == 0) {
+      double b = wmin[0];
+      for (int w = 1; w < NT / 64; ++w) b = (wmin[w] < b) ? wmin[w] : b;
+      fuse.block_min[fuse.block_offset + blockIdx.x] = b;
+    }
+  }
 }
 
 // ---- Local, Navier-Stokes stage 0 (Spatial.hpp:326-509 with Navier_stokes<true>) -------------------------
@@ -1035,36 +1094,15 @@ __global__ __launch_bounds__(256) void k_neighbor_reconcile(Dev_mesh m)
 // per workgroup, with the grid capped so that the atomics on the single result word stay few ----
 constexpr int max_dt_blocks = 2048;
 
-// 1/time step at one quadrature point (Spatial.hpp:790-818): the convective scale over the interpolated vertex spacing,
-// plus the diffusive one for Navier-Stokes
 template <int ND, int RS, bool VISC>
 __device__ inline double max_dt_scale(const Dev_mesh& m, const Ops<RS>* __restrict__ gops, const Ns_params& ns, int cls,
                                       size_t e, int q, double max_cfl_c, double max_cfl_d)
 {
-  constexpr int NQ = Shape<ND, RS>::NQ, NV = ND + 2, NVERT = 1 << ND;
-  double vs[NVERT];
-  for (int i = 0; i < NVERT; ++i) vs[i] = m.vertex_tss[cls][e * NVERT + i];
-  // math::interp (math.hpp:207-218)
-  int stride = NVERT;
-  for (int d = 0; d < ND; ++d) {
-    stride /= 2;
-    const double x = gops->node[(q / ipow(RS, ND - 1 - d)) % RS];
-    for (int i = 0; i < stride; ++i) vs[i] += x * (vs[i + stride] - vs[i]);
-  }
-  const double spacing = vs[0];
+  constexpr int NQ = Shape<ND, RS>::NQ, NV = ND + 2;
   double u[NV];
   const double* st = m.state[cls] + e * NV * NQ;
   for (int v = 0; v < NV; ++v) u[v] = st[v * NQ + q];
-  double scale = char_speed<ND>(u) / max_cfl_c / spacing;
-  if constexpr (VISC) {
-    double bulk = 0., lap = 0.;
-    if (m.av[cls]) {
-      bulk = m.av[cls][e * 2 * NQ + q];
-      lap = m.av[cls][e * 2 * NQ + NQ + q];
-    }
-    scale += diffusivity<ND>(ns, u, bulk, lap) / max_cfl_d / spacing / spacing;
-  }
-  return scale;
+  return max_dt_scale_u<ND, RS, VISC>(m, gops, ns, cls, e, q, u, max_cfl_c, max_cfl_d);
 }
 
 template <int ND, int RS, bool VISC>
@@ -1441,8 +1479,13 @@ struct Launchers
       if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_ns<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, cr, uf);
       else hipLaunchKernelGGL((k_local_ns<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, cr, uf);
     } else {
-      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_euler<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, od, coef, stage, cr, uf);
-      else hipLaunchKernelGGL((k_local_euler<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, od, coef, stage, cr, uf);
+      const dim3 be(euler_block<ND, RS>());
+      Dt_fuse f = l.fuse;  // the deformed class's workgroups store after the Cartesian class's
+      if (f.block_min && cls == HX_DEFORMED) f.block_offset += blocks_for(l.m.n_elem[HX_CARTESIAN], S::EPB);
+      if (cls == HX_DEFORMED)
+        hipLaunchKernelGGL((k_local_euler<ND, RS, true, P>), g, be, 0, l.stream, l.m, o, od, coef, stage, cr, uf, f, ops(l));
+      else
+        hipLaunchKernelGGL((k_local_euler<ND, RS, false, P>), g, be, 0, l.stream, l.m, o, od, coef, stage, cr, uf, f, ops(l));
     }
     return hipGetLastError();
   }

@@ -18,6 +18,7 @@ struct Launch
   Ns_params ns;      // transport models (Navier-Stokes)
   double* scalar;    // device scratch: [0] time step reduction
   int precision;     // hx_precision
+  Dt_fuse fuse;      // next step's Max_dt fused into the Euler Local (slots == nullptr: not fused)
 };
 
 struct Kernel_set
