@@ -54,6 +54,9 @@ struct hx_mesh
   int* case_id[2] = {};
   unsigned long long* case_min = nullptr;
   double* case_dt = nullptr;
+  // when each class lists its cases in order (non-decreasing ids): first element of every case, [n_cases + 1]
+  int* case_start[2] = {};
+  int case_split[2] = {};  // workgroups per case of the per-case reduction
   // Max_dt fused into the last stage of hx_update: every call that can change device data (a kernel launch, an
   // upload, handing out a device pointer) advances `gen`; the next hx_update takes the fused minimum only if `gen` is
   // where the previous hx_update left it and its time-step settings are the same
@@ -399,6 +402,30 @@ __global__ __launch_bounds__(256) void k_case_dt(const unsigned long long* __res
   if (i >= n_cases) return;
   const double mdt = __longlong_as_double((long long)case_min[i]);
   case_dt[i] = (max_time_step < mdt ? max_time_step : mdt) * cheby;
+}
+
+// per-case minimum of the time step scale field (which Max_dt in local mode filled with each point's 1/scale) over
+// each case's contiguous range of elements, `split` workgroups per case, one atomic minimum per workgroup
+__global__ __launch_bounds__(256) void k_case_min_tss(const double* __restrict__ tss, const int* __restrict__ case_start,
+                                                      int nq, int split, unsigned long long* __restrict__ case_min)
+{
+  __shared__ double w[4];
+  const int k = blockIdx.x / split, sl = blockIdx.x % split;
+  const long long lo = (long long)case_start[k] * nq, hi = (long long)case_start[k + 1] * nq;
+  const long long per = (hi - lo + split - 1) / split;
+  const long long a = lo + sl * per, b = a + per < hi ? a + per : hi;
+  double m = DBL_MAX;
+  for (long long i = a + threadIdx.x; i < b; i += 256) m = tss[i] < m ? tss[i] : m;  // NaN never wins
+  for (int off = 32; off > 0; off /= 2) {
+    const double x = __shfl_xor(m, off, 64);
+    m = x < m ? x : m;
+  }
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 1; j < 4; ++j) m = w[j] < m ? w[j] : m;
+    if (m < DBL_MAX) atomicMin(case_min + k, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 __global__ __launch_bounds__(256) void k_case_tss(double* __restrict__ tss, const int* __restrict__ case_id,
@@ -1135,6 +1162,25 @@ hx_status hx_mesh_set_cases(hx_mesh* mesh, int n_cases, const int* case_cartesia
   }
   if (!alloc(&cmin, size_t(n_cases) * 8) || !alloc(&cdt, size_t(n_cases) * 8))
     return fail(ctx, HX_ERR_HIP, "hipMalloc(case time steps)");
+  // cases listed in order within each class: their element ranges, for the streaming per-case reduction
+  for (int c = 0; c < 2; ++c) {
+    mesh->case_start[c] = nullptr;
+    bool sorted = true;
+    for (int e = 1; e < m.n_elem[c]; ++e) sorted = sorted && ids[c][e] >= ids[c][e - 1];
+    if (!sorted || !m.n_elem[c]) continue;
+    std::vector<int> start(n_cases + 1, 0);
+    for (int e = 0; e < m.n_elem[c]; ++e) ++start[ids[c][e] + 1];
+    int biggest = 0;
+    for (int k = 0; k < n_cases; ++k) {
+      biggest = std::max(biggest, start[k + 1]);
+      start[k + 1] += start[k];
+    }
+    int* d = nullptr;
+    if (!alloc(reinterpret_cast<void**>(&d), start.size() * sizeof(int))) return fail(ctx, HX_ERR_HIP, "hipMalloc(case ranges)");
+    HX_TRY(ctx, hipMemcpy(d, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice), "upload case ranges");
+    mesh->case_start[c] = d;
+    mesh->case_split[c] = int(std::max<long long>(1, (long long)biggest * m.nq / 8192));
+  }
   mesh->n_cases = n_cases;
   mesh->case_id[0] = dev_ids[0];
   mesh->case_id[1] = dev_ids[1];
@@ -1188,8 +1234,17 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
                          (unsigned long long)__builtin_bit_cast(long long, DBL_MAX), nc);
       HX_TRY(ctx, hipGetLastError(), "fill");
       for (int c = 0; c < 2; ++c) {
+        if (!mesh->dm.n_elem[c]) continue;
         Timer tm(ctx, std::string(cls_name(c)) + "/compute time step/" + name, mesh->dm.n_elem[c]);
-        HX_TRY(ctx, mesh->ks->max_dt_cases(l, c, visc, cc, cd, mesh->case_id[c], mesh->case_min), "max_dt_cases");
+        if (mesh->case_start[c]) {
+          // each point's 1/scale into the time step scale (Max_dt in local mode), then a streaming minimum per case
+          HX_TRY(ctx, mesh->ks->max_dt(l, c, visc, 1, cc, cd), "max_dt");
+          hipLaunchKernelGGL(k_case_min_tss, dim3(nc * mesh->case_split[c]), dim3(256), 0, l.stream, mesh->dm.tss[c],
+                             mesh->case_start[c], mesh->dm.nq, mesh->case_split[c], mesh->case_min);
+          HX_TRY(ctx, hipGetLastError(), "case minimum");
+        } else {
+          HX_TRY(ctx, mesh->ks->max_dt_cases(l, c, visc, cc, cd, mesh->case_id[c], mesh->case_min), "max_dt_cases");
+        }
       }
       hipLaunchKernelGGL(k_case_dt, dim3((nc + 255) / 256), dim3(256), 0, l.stream, mesh->case_min, mesh->case_dt, nc,
                          o->max_time_step, cheby(n_cheby, i_cheby));

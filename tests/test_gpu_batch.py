@@ -33,13 +33,16 @@ def ns_cases():
             configs.ns_deformed(2, rs=5, mach=0.2, jitter=0.0, seed=3)]
 
 
-def run_batch_vs_single(ctx, cases, opts, precision):
+def run_batch_vs_single(ctx, cases, opts, precision, relabel=None):
+    """relabel: a permutation of the case ids (case k registered as relabel[k]), so that the ids are not in order
+    within a class and the update takes its general per-case reduction."""
     for m in cases:
         init_faces(m)
     b = mesh.concat(cases)
     dev = Device_mesh(ctx, b.copy())
     dev.set_precision(precision)
-    dev.set_cases(len(cases), b.case_id)
+    perm = np.arange(len(cases)) if relabel is None else np.asarray(relabel)
+    dev.set_cases(len(cases), [perm[c].astype(np.int32) for c in b.case_id])
     st = (abi.hx_update_status * len(cases))()
     singles = []
     for m in cases:
@@ -50,8 +53,9 @@ def run_batch_vs_single(ctx, cases, opts, precision):
         dev.update_cases(opts, st)
         for k, (d, s1) in enumerate(singles):
             d.update(opts, s1)
-            assert st[k].time_step == s1.time_step, f"step {it} case {k}: dt {st[k].time_step} vs {s1.time_step}"
-            assert st[k].flow_time == s1.flow_time and st[k].iteration == s1.iteration == it + 1
+            sk = st[int(perm[k])]
+            assert sk.time_step == s1.time_step, f"step {it} case {k}: dt {sk.time_step} vs {s1.time_step}"
+            assert sk.flow_time == s1.flow_time and sk.iteration == s1.iteration == it + 1
     got = [dev.download("state", c, out=np.zeros_like(b.state[c])) for c in (0, 1)]
     off = [0, 0]
     for k, (d, _) in enumerate(singles):
@@ -79,6 +83,11 @@ def test_batch_navier_stokes(ctx, precision):
     visc, cond = configs.sutherland_air()
     run_batch_vs_single(ctx, ns_cases(), abi.update_options(max_safety=0.5, visc=visc, cond=cond, use_ldg=True),
                         precision)
+
+
+def test_batch_unordered_case_ids(ctx):
+    say("[gpu] batch with case ids out of order")
+    run_batch_vs_single(ctx, euler_cases(), abi.update_options(max_safety=0.5), abi.HX_FP64, relabel=[2, 0, 3, 1])
 
 
 def test_batch_filter_and_chebyshev(ctx):
