@@ -258,7 +258,10 @@ hx_status hx_mesh_download(hx_mesh* mesh, int field, int cls, double* host, size
  * contractions in FP32, the contractions on the matrix cores with FP32 accumulation; the state, RK update, residual
  * cache, face fluxes (Neighbor), face jumps, LDG reconciliation, Jacobian determinant scaling and max_dt stay FP64. */
 hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision);
-/* Device address of a field (for callers that keep their own device views, e.g. torch tensors). */
+/* Device address of a field (for callers that keep their own device views, e.g. torch tensors). hx_update reuses
+ * the Max_dt its previous call computed unless the mesh changed in between; a caller that writes through a pointer
+ * it obtained before that call must call hx_mesh_device_ptr again (which marks the mesh changed) before the next
+ * hx_update. */
 hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size_t* n_doubles);
 
 /* ---- kernels: one per reference kernel class (Spatial.hpp) ------------------------------------------ */
