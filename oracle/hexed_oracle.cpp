@@ -7,15 +7,18 @@
  *   include/Spatial.hpp          Write_face, Face_permutation, Prolong/Restrict_refined, Local,
  *                                Reconcile_ldg_flux, Neighbor, Neighbor_reconcile, Max_dt
  *   include/pde.hpp:27-175       Navier_stokes<visc>::Pde and its Computation
+ *   include/pde.hpp:265-431      Advection and Smooth_art_visc (the artificial-viscosity generator's PDEs)
  *   include/Derivative.hpp       Derivative<row_size>
  *   include/Row_index.hpp, include/Row_rw.hpp   row iteration and row/face access
  *   include/Transport_model.hpp  Sutherland / constant transport coefficients
  *   include/math.hpp:30-36,207-218, src/math.cpp:104-107   pow, interp, chebyshev_step
  *   src/kernels_convective.cpp, src/kernels_diffusive.cpp, src/kernels_max_dt.cpp   kernel sequences
  *   src/Solver.cpp:56-81,117-133,834-913   BC application, max_dt, update, compute_residual
+ *   src/Solver.cpp:83-101,428-582          AV boundary conditions, diffuse_art_visc, update_art_visc_smoothness
  *   src/Boundary_condition.cpp   Freestream, Copy, Nonpenetration, Outflow
  *
- * Parity status of each part is recorded in DESIGN.md's oracle section; the tests under tests/ pin it.
+ * What pins each part (closed forms, invariants, the reference's own tests restated) is in the tests under tests/:
+ * test_oracle.py, test_basis.py, test_tree.py, test_av.py, test_batch.py.
  */
 #include "hexed_oracle.h"
 

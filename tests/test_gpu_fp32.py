@@ -15,7 +15,7 @@ from helpers import freestream, init_faces, per_var_rel_l2, say, smooth_state
 
 pytestmark = pytest.mark.gpu
 
-TOL32 = 1e-5   # relative L2 per variable after STEPS steps vs the FP64 form
+TOL32 = 1e-7   # relative L2 per variable after STEPS steps vs the FP64 form (measured up to 2.4e-8)
 STEPS = 3
 
 # (n_dim, row_size, deformed, elements per axis, Navier-Stokes); includes row sizes whose workgroups are not whole
@@ -95,7 +95,7 @@ def test_fp32_mfma_freestream(ctx, nd, rs):
     mesh.set_state(host, lambda pos: fs)
     init_faces(host)
     got = run(ctx, host, abi.update_options(max_safety=0.5), abi.HX_FP32_MFMA, 2)
-    for v in range(nd + 2):
-        dev_v = np.abs(got[1][:, v] - fs[v]).max()
-        scale = max(abs(x) for x in fs)
-        assert dev_v <= 1e-5 * scale, f"variable {v}: max deviation {dev_v:.3e}"
+    scale = max(abs(x) for x in fs)
+    devs = [float(np.abs(got[1][:, v] - fs[v]).max()) / scale for v in range(nd + 2)]
+    say("    max deviation / freestream scale per variable " + " ".join(f"{d:.2e}" for d in devs))
+    assert max(devs) <= 5e-8, f"max deviations {devs}"  # measured up to 4.7e-9
