@@ -12,7 +12,9 @@ HX_OK, HX_ERR_INVALID, HX_ERR_HIP, HX_ERR_NOT_FINITE, HX_ERR_UNSUPPORTED, HX_ERR
 HX_CARTESIAN, HX_DEFORMED = 0, 1
 HX_PDE_EULER, HX_PDE_NAVIER_STOKES, HX_PDE_ADVECTION, HX_PDE_SMOOTH_AV, HX_PDE_FIX_ADMIS = range(5)
 HX_FP64, HX_FP32_MFMA = 0, 1
-HX_BC_FREESTREAM, HX_BC_COPY, HX_BC_NONPENETRATION, HX_BC_OUTFLOW = range(4)
+(HX_BC_FREESTREAM, HX_BC_COPY, HX_BC_NONPENETRATION, HX_BC_OUTFLOW, HX_BC_RIEMANN_INVARIANTS, HX_BC_PRESSURE_OUTFLOW,
+ HX_BC_NO_SLIP) = range(7)
+HX_THERMAL_HEAT_FLUX, HX_THERMAL_ENERGY, HX_THERMAL_EQUILIBRIUM = range(3)
 (HX_FIELD_STATE, HX_FIELD_RCACHE, HX_FIELD_TSS, HX_FIELD_AV, HX_FIELD_FORCING, HX_FIELD_ADVECTION,
  HX_FIELD_VERTEX_TSS, HX_FIELD_METRICS, HX_FIELD_FACES, HX_FIELD_NORMALS) = range(10)
 
@@ -54,7 +56,10 @@ def ns_pde(visc, cond):
 
 
 class hx_bc(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int), ("state", ctypes.c_double * 5)]
+    _fields_ = [("kind", ctypes.c_int), ("state", ctypes.c_double * 5), ("pressure", ctypes.c_double),
+                ("thermal", ctypes.c_int), ("thermal_value", ctypes.c_double), ("emissivity", ctypes.c_double),
+                ("heat_transfer_coef", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("coercion", ctypes.c_double)]
 
 
 class hx_host_mesh(ctypes.Structure):
@@ -67,7 +72,7 @@ class hx_host_mesh(ctypes.Structure):
         ("n_con", ctypes.c_int * 2), ("con_slot", c_int_p * 2), ("con_dir", c_int_p * 2), ("con_nslot", c_int_p),
         ("n_ref", ctypes.c_int), ("ref_slot", c_int_p), ("ref_stretch", c_int_p),
         ("n_bc_con", ctypes.c_int), ("bc_con", c_int_p), ("bc_serial", c_int_p),
-        ("n_bc", ctypes.c_int), ("bcs", ctypes.POINTER(hx_bc)),
+        ("n_bc", ctypes.c_int), ("bcs", ctypes.POINTER(hx_bc)), ("bc_cache", c_double_p),
     ]
 
 

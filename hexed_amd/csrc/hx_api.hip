@@ -273,6 +273,11 @@ std::string validate(const hx_host_mesh* d, const Sizes& z)
     if (!in(d->bc_con[k], 0, d->n_con[HX_DEFORMED])) return "boundary connection out of range";
     if (!in(d->bc_serial[k], 0, d->n_bc)) return "boundary condition serial out of range";
   }
+  for (long long k = 0; k < d->n_bc; ++k) {
+    if (!in(d->bcs[k].kind, HX_BC_FREESTREAM, HX_BC_NO_SLIP + 1)) return "unknown boundary condition kind";
+    if (d->bcs[k].kind == HX_BC_NO_SLIP && !in(d->bcs[k].thermal, HX_THERMAL_HEAT_FLUX, HX_THERMAL_EQUILIBRIUM + 1))
+      return "unknown thermal condition";
+  }
   if (d->n_slot < 0 || d->n_nslot < 0 || d->n_ref < 0 || d->n_bc_con < 0 || d->n_bc < 0) return "negative count";
   return "";
 }
@@ -686,6 +691,7 @@ hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* d, hx_mesh** out)
   if ((s = device_copy(mesh, &m.bc_serial, d->bc_serial, (long long)d->n_bc_con, true))) return bail(s);
   m.n_bc = d->n_bc;
   if ((s = device_copy(mesh, &m.bcs, d->bcs, (long long)d->n_bc, true))) return bail(s);
+  if ((s = device_copy(mesh, &m.bc_cache, d->bc_cache, (long long)d->n_bc_con * z.nv * z.nfq, true))) return bail(s);
   *out = mesh;
   return HX_OK;
 }

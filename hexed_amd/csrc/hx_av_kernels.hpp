@@ -493,6 +493,9 @@ __global__ __launch_bounds__(256) void k_av_bcs(Dev_mesh m, int mode)
     }
     for (int d = 0; d < ND; ++d) gh_f[d * NFQ + f] = in_f[d * NFQ + f] - 2 * dot * nrml[d * NFQ + f] / nsq;
     for (int a = 0; a < RS; ++a) gh_f[(ND + a) * NFQ + f] = in_f[(ND + a) * NFQ + f];
+  } else if (kind == HX_BC_NO_SLIP) {  // Boundary_condition.cpp:429-448: velocity reversed, scalars kept
+    for (int d = 0; d < ND; ++d) gh_f[d * NFQ + f] = -in_f[d * NFQ + f];
+    for (int a = 0; a < RS; ++a) gh_f[(ND + a) * NFQ + f] = in_f[(ND + a) * NFQ + f];
   } else {  // Flow_bc::apply_advection: velocity copied, advected scalars 2 - inside
     for (int d = 0; d < ND; ++d) gh_f[d * NFQ + f] = in_f[d * NFQ + f];
     for (int a = 0; a < RS; ++a) gh_f[(ND + a) * NFQ + f] = 2. - in_f[(ND + a) * NFQ + f];

@@ -50,6 +50,7 @@ struct Dev_mesh
   int* bc_serial;
   int n_bc;
   hx_bc* bcs;
+  double* bc_cache;  // [n_bc_con][nv][nfq] boundary-face state cache
 };
 
 // Operator tables of one row size, passed by value as a kernel argument.
@@ -173,6 +174,167 @@ __device__ inline void flux_conv(const T (&u)[ND + 2], const T (&n)[ND], T (&f)[
   const T vol_flux = mf * inv_mass;
   f[ND + 1] = (u[ND + 1] + pressure) * vol_flux;
   for (int j = 0; j < ND; ++j) f[j] = u[j] * vol_flux + pressure * n[j];
+}
+
+// ---- characteristic decomposition of the Euler flux (pde.hpp:182-260) ----------------------------------------
+// About a reference state and a flux direction: eigenvalues u_n - c, u_n + c, u_n of the flux Jacobian, and the
+// 1D eigenvectors (rows: normal momentum, mass, energy; one column per eigenvalue, unit mass perturbation).
+// decomp() splits a state into three parts, one per eigenvalue, that sum to it; the 1D coordinates come from a
+// 3x3 solve, here Gaussian elimination with complete pivoting that sets an unknown with a zero pivot to zero (the
+// reference's column-pivoting QR also returns such a basic solution for a singular system, e.g. zero pressure).
+__host__ __device__ inline void solve3(const double (&a_in)[3][3], const double (&b_in)[3], double (&x)[3])
+{
+  double a[3][3], b[3];
+  int col[3] = {0, 1, 2};
+  for (int i = 0; i < 3; ++i) {
+    b[i] = b_in[i];
+    for (int j = 0; j < 3; ++j) a[i][j] = a_in[i][j];
+  }
+  double amax = 0.;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) amax = fmax(amax, fabs(a[i][j]));
+  int rank = 0;
+  for (int k = 0; k < 3; ++k) {
+    int pi = k, pj = k;
+    for (int i = k; i < 3; ++i)
+      for (int j = k; j < 3; ++j)
+        if (fabs(a[i][j]) > fabs(a[pi][pj])) pi = i, pj = j;
+    if (!(fabs(a[pi][pj]) > 1e-14 * amax)) break;
+    for (int j = 0; j < 3; ++j) {
+      const double t = a[k][j];
+      a[k][j] = a[pi][j];
+      a[pi][j] = t;
+    }
+    const double tb = b[k];
+    b[k] = b[pi];
+    b[pi] = tb;
+    for (int i = 0; i < 3; ++i) {
+      const double t = a[i][k];
+      a[i][k] = a[i][pj];
+      a[i][pj] = t;
+    }
+    const int tc = col[k];
+    col[k] = col[pj];
+    col[pj] = tc;
+    for (int i = k + 1; i < 3; ++i) {
+      const double f = a[i][k] / a[k][k];
+      for (int j = k; j < 3; ++j) a[i][j] -= f * a[k][j];
+      b[i] -= f * b[k];
+    }
+    ++rank;
+  }
+  double y[3] = {0., 0., 0.};
+  for (int k = rank - 1; k >= 0; --k) {
+    double r = b[k];
+    for (int j = k + 1; j < rank; ++j) r -= a[k][j] * y[j];
+    y[k] = r / a[k][k];
+  }
+  for (int k = 0; k < 3; ++k) x[col[k]] = y[k];
+}
+
+template <int ND>
+struct Characteristics
+{
+  double vals[3];
+  double vecs[3][3];
+  double dir[ND];
+  double veloc[ND];
+  __host__ __device__ Characteristics(const double (&state)[ND + 2], const double (&normal)[ND])
+  {
+    double nn = 0.;
+    for (int d = 0; d < ND; ++d) nn += normal[d] * normal[d];
+    nn = sqrt(nn);
+    for (int d = 0; d < ND; ++d) dir[d] = normal[d] / nn;
+    const double mass = state[ND];
+    double vsq = 0., vn = 0.;
+    for (int d = 0; d < ND; ++d) {
+      veloc[d] = state[d] / mass;
+      vsq += veloc[d] * veloc[d];
+      vn += dir[d] * veloc[d];
+    }
+    const double pres = .4 * (state[ND + 1] - .5 * mass * vsq);
+    const double sound = sqrt(1.4 * fmax(pres, 0.) / mass);
+    vals[2] = vn;
+    vals[0] = vn - sound;
+    vals[1] = vn + sound;
+    for (int sign = 0; sign < 2; ++sign) {
+      const double d_veloc = (2 * sign - 1) * sound / mass, d_pres = 1.4 * pres / mass;
+      vecs[0][sign] = vals[2] + mass * d_veloc;
+      vecs[1][sign] = 1.;
+      vecs[2][sign] = d_pres / .4 + .5 * vsq + mass * vals[2] * d_veloc;
+    }
+    vecs[0][2] = vals[2];
+    vecs[1][2] = 1.;
+    vecs[2][2] = .5 * vsq;
+  }
+  // d[var][eigenvalue]
+  __host__ __device__ void decomp(const double (&state)[ND + 2], double (&d)[ND + 2][3]) const
+  {
+    double mn = 0., vtn = 0.;
+    for (int k = 0; k < ND; ++k) {
+      mn += dir[k] * state[k];
+      vtn += dir[k] * veloc[k];
+    }
+    double vt[ND], corr[ND], vcorr = 0.;
+    for (int k = 0; k < ND; ++k) {
+      vt[k] = veloc[k] - dir[k] * vtn;                           // tangential velocity
+      corr[k] = (state[k] - dir[k] * mn) - state[ND] * vt[k];    // tangential momentum not induced by mass
+      vcorr += veloc[k] * corr[k];
+    }
+    const double s1d[3] = {mn, state[ND], state[ND + 1] - vcorr};
+    double basis[3];
+    solve3(vecs, s1d, basis);
+    for (int c = 0; c < 3; ++c) {
+      d[ND][c] = vecs[1][c] * basis[c];
+      d[ND + 1][c] = vecs[2][c] * basis[c];
+      for (int k = 0; k < ND; ++k) d[k][c] = dir[k] * (vecs[0][c] * basis[c]) + vt[k] * basis[c];
+    }
+    for (int k = 0; k < ND; ++k) d[k][2] += corr[k];
+    d[ND + 1][2] += vcorr;
+  }
+};
+
+// Riemann_invariants' apply_char (Boundary_condition.cpp:83-95): decompose `inside` and `outside` about `state`
+// and take the outside part of each characteristic whose eigenvalue has the sign `sign`
+template <int ND>
+__host__ __device__ inline void apply_char(const double (&state)[ND + 2], const double (&normal)[ND], int sign,
+                                           const double (&inside)[ND + 2], const double (&outside)[ND + 2],
+                                           double (&out)[ND + 2])
+{
+  const Characteristics<ND> ch(state, normal);
+  double di[ND + 2][3], d_o[ND + 2][3];
+  ch.decomp(inside, di);
+  ch.decomp(outside, d_o);
+  for (int v = 0; v < ND + 2; ++v) {
+    double sum = 0.;
+    for (int c = 0; c < 3; ++c) sum += (sign * ch.vals[c] > 0) ? d_o[v][c] : di[v][c];
+    out[v] = sum;
+  }
+}
+
+// the Stefan-Boltzmann constant from its definition, as constants.hpp:52 computes it (math::pow multiplies from 1)
+__host__ __device__ constexpr double ipow_d(double b, int e)
+{
+  double r = 1.;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+constexpr double stefan_boltzmann = 2 * ipow_d(M_PI, 5) * ipow_d(1.380649e-23, 4) /
+                                    (15 * ipow_d(299792458., 2) * ipow_d(6.62607015e-34, 3));
+
+// No_slip's thermal condition (Boundary_condition.hpp:140-186, Boundary_condition.cpp:373-379)
+template <int NV>
+__host__ __device__ inline double ghost_energy(const hx_bc& bc, const double (&state)[NV])
+{
+  return bc.thermal == HX_THERMAL_ENERGY ? bc.thermal_value * state[NV - 2] : state[NV - 1];
+}
+template <int NV>
+__host__ __device__ inline double ghost_heat_flux(const hx_bc& bc, const double (&state)[NV], double heat_flux)
+{
+  if (bc.thermal == HX_THERMAL_ENERGY) return heat_flux;
+  if (bc.thermal == HX_THERMAL_HEAT_FLUX) return bc.thermal_value;
+  const double temp = state[NV - 1] * .4 / state[NV - 2] / specific_gas_air;
+  return bc.emissivity * stefan_boltzmann * ipow_d(temp, 4) + bc.heat_transfer_coef * (temp - bc.temperature);
 }
 
 // pressure (pde.hpp:97-104) in FP64

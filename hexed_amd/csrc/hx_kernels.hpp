@@ -1396,6 +1396,57 @@ __global__ __launch_bounds__(256) void k_state_bcs(Dev_mesh m)
       for (int v = 0; v < NV; ++v) gh_f[v * NFQ + f] = in_f[v * NFQ + f];
       reflect_normal<ND, RS>(gh_f, m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ, f);
       break;
+    case HX_BC_RIEMANN_INVARIANTS:
+    case HX_BC_PRESSURE_OUTFLOW:
+    case HX_BC_NO_SLIP: {
+      const double* nslot = m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ;
+      double* cache = m.bc_cache + size_t(ib) * NV * NFQ;
+      const Dir dir(m.con_dir[HX_DEFORMED][con]);
+      double in[NV], nrm[ND], gh[NV];
+      for (int v = 0; v < NV; ++v) in[v] = in_f[v * NFQ + f];
+      for (int d = 0; d < ND; ++d) nrm[d] = nslot[d * NFQ + f];
+      if (bc.kind == HX_BC_RIEMANN_INVARIANTS) {
+        // Boundary_condition.cpp:97-141: incoming characteristics from the freestream, outgoing from inside, then
+        // limited to stay thermodynamically admissible; the inside state primes the cache
+        double fs[NV];
+        for (int v = 0; v < NV; ++v) fs[v] = bc.state[v];
+        apply_char<ND>(in, nrm, 1 - 2 * int(dir.sign0), in, fs, gh);
+        gh[ND] = fmax(gh[ND], in[ND] / 2);
+        double ke = 0., ike = 0.;
+        for (int d = 0; d < ND; ++d) {
+          ke += gh[d] * gh[d];
+          ike += in[d] * in[d];
+        }
+        ke = .5 * ke / gh[ND];
+        ike = .5 * ike / in[ND];
+        gh[ND + 1] = fmax(ke + fmax(gh[ND + 1] - ke, (in[ND + 1] - ike) / 2), 0.);
+        for (int v = 0; v < NV; ++v) cache[v * NFQ + f] = in[v];
+      } else if (bc.kind == HX_BC_PRESSURE_OUTFLOW) {
+        // Boundary_condition.cpp:184-214: the specified pressure where the outflow is subsonic
+        double mn = 0., nsq = 0., msq = 0.;
+        for (int d = 0; d < ND; ++d) {
+          mn += in[d] * nrm[d];
+          nsq += nrm[d] * nrm[d];
+          msq += in[d] * in[d];
+        }
+        const double nrml_veloc = mn / in[ND] / sqrt(nsq);
+        const double kin = .5 * msq / in[ND];
+        const double pres = fmax(.4 * (in[ND + 1] - kin), 0.);
+        const double sound = sqrt(1.4 * pres / in[ND]);
+        for (int v = 0; v < NV; ++v) gh[v] = in[v];
+        if (nrml_veloc * (2 * int(dir.sign0) - 1) < sound) gh[ND + 1] = bc.pressure / .4 + kin;
+      } else {
+        // Boundary_condition.cpp:381-400: velocity reversed, energy from the thermal condition; the average of the
+        // ghost and inside states primes the cache
+        for (int d = 0; d < ND; ++d) gh[d] = -in[d];
+        gh[ND] = in[ND];
+        const double e = ghost_energy<NV>(bc, in);
+        gh[ND + 1] = e * e / in[ND + 1];
+        for (int v = 0; v < NV; ++v) cache[v * NFQ + f] = (gh[v] + in[v]) / 2;
+      }
+      for (int v = 0; v < NV; ++v) gh_f[v * NFQ + f] = gh[v];
+      break;
+    }
     default:
       break;
   }
@@ -1425,6 +1476,40 @@ __global__ __launch_bounds__(256) void k_flux_bcs(Dev_mesh m)
       for (int v = 0; v < NV; ++v) gh_f[(NV + v) * NFQ + f] = -in_f[(NV + v) * NFQ + f];
       reflect_normal<ND, RS>(gh_f + NV * NFQ, m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ, f);
       break;
+    case HX_BC_PRESSURE_OUTFLOW:  // Boundary_condition.cpp:216-223
+      for (int v = 0; v < NV; ++v) gh_f[(NV + v) * NFQ + f] = -in_f[(NV + v) * NFQ + f];
+      break;
+    case HX_BC_RIEMANN_INVARIANTS:
+    case HX_BC_NO_SLIP: {
+      const double* nslot = m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ;
+      const double* cache = m.bc_cache + size_t(ib) * NV * NFQ;
+      const Dir dir(m.con_dir[HX_DEFORMED][con]);
+      double fl[NV], cs[NV], nrm[ND];
+      for (int v = 0; v < NV; ++v) {
+        fl[v] = in_f[(NV + v) * NFQ + f];
+        cs[v] = cache[v * NFQ + f];
+      }
+      for (int d = 0; d < ND; ++d) nrm[d] = nslot[d * NFQ + f];
+      if (bc.kind == HX_BC_RIEMANN_INVARIANTS) {
+        // Boundary_condition.cpp:143-182: the viscous flux of the outgoing characteristics, zero for the incoming
+        double zero[NV] = {}, gh[NV];
+        apply_char<ND>(cs, nrm, 1 - 2 * int(dir.sign0), zero, fl, gh);
+        for (int v = 0; v < NV; ++v) gh_f[(NV + v) * NFQ + f] = gh[v];
+      } else {
+        // Boundary_condition.cpp:402-427: momentum flux kept, mass flux reversed, energy flux toward the thermal
+        // condition's heat flux
+        for (int d = 0; d < ND; ++d) gh_f[(NV + d) * NFQ + f] = fl[d];
+        gh_f[(NV + ND) * NFQ + f] = -fl[ND];
+        double nsq = 0.;
+        for (int d = 0; d < ND; ++d) nsq += nrm[d] * nrm[d];
+        const double normal = sqrt(nsq);
+        const int flux_sign = 2 * int(dir.sign0) - 1;
+        const double qe = fl[ND + 1];
+        gh_f[(NV + ND + 1) * NFQ + f] =
+            bc.coercion * (normal * flux_sign * ghost_heat_flux<NV>(bc, cs, qe * flux_sign / normal) - qe) + qe;
+      }
+      break;
+    }
     default:
       break;
   }

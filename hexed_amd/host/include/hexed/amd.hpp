@@ -94,5 +94,20 @@ class Mesh_lease
 //! Solver::update (Solver.cpp:834-886) on a registered mesh
 void update(Accelerated_mesh& mesh, const hx_update_options& opts, hx_update_status& status);
 
+// Device boundary conditions with the arguments and defaults of the reference's Flow_bc constructors
+// (Boundary_condition.hpp:61-225). `state` is the conserved freestream state: momentum[n_dim], density, energy.
+hx_bc freestream(const std::vector<double>& state);                // Freestream(Mat<>)
+hx_bc riemann_invariants(const std::vector<double>& state);        // Riemann_invariants(Mat<>)
+hx_bc pressure_outflow(double pressure);                           // Pressure_outflow(double)
+hx_bc nonpenetration();                                            // Nonpenetration()
+hx_bc copy();                                                      // Copy()
+hx_bc outflow();                                                   // Outflow()
+//! No_slip(Prescribed_heat_flux(heat_flux), coercion); the reference's default wall is adiabatic with coercion 2
+hx_bc no_slip_heat_flux(double heat_flux = 0., double coercion = 2.);
+//! No_slip(Prescribed_energy(energy_per_mass), coercion)
+hx_bc no_slip_energy(double energy_per_mass, double coercion = 2.);
+//! No_slip(Thermal_equilibrium{emissivity, heat_transfer_coef, temperature}, coercion)
+hx_bc no_slip_equilibrium(double emissivity, double heat_transfer_coef, double temperature, double coercion = 2.);
+
 }  // namespace hexed::amd
 #endif

@@ -208,9 +208,14 @@ void Accelerated_mesh::add_boundary(Kernel_connection& con, hx_bc bc)
   if (dev) throw std::runtime_error("hexed_amd: boundary conditions must be added before the first push");
   auto it = std::find(cons[1].begin(), cons[1].end(), &con);
   if (it == cons[1].end()) throw std::runtime_error("hexed_amd: boundary connection is not in def_cons");
-  int serial = int(std::find_if(bc_defs.begin(), bc_defs.end(), [&](const hx_bc& b) {
-                     return std::memcmp(&b, &bc, sizeof(hx_bc)) == 0;
-                   }) - bc_defs.begin());
+  // field by field: hx_bc has padding after its int members, which memcmp would compare too
+  auto same = [](const hx_bc& a, const hx_bc& b) {
+    return a.kind == b.kind && std::equal(a.state, a.state + HX_MAX_N_DIM + 2, b.state) && a.pressure == b.pressure &&
+           a.thermal == b.thermal && a.thermal_value == b.thermal_value && a.emissivity == b.emissivity &&
+           a.heat_transfer_coef == b.heat_transfer_coef && a.temperature == b.temperature && a.coercion == b.coercion;
+  };
+  int serial = int(std::find_if(bc_defs.begin(), bc_defs.end(), [&](const hx_bc& b) { return same(b, bc); }) -
+                   bc_defs.begin());
   if (serial == int(bc_defs.size())) bc_defs.push_back(bc);
   bc_con.push_back(int(it - cons[1].begin()));
   bc_serial.push_back(serial);
@@ -390,6 +395,66 @@ Mesh_lease::~Mesh_lease()
 void update(Accelerated_mesh& mesh, const hx_update_options& opts, hx_update_status& status)
 {
   check(hx_update(mesh.device(), &opts, &status), "Solver::update");
+}
+
+namespace
+{
+hx_bc bc_of(int kind)
+{
+  hx_bc bc{};
+  bc.kind = kind;
+  bc.coercion = 2.;
+  return bc;
+}
+
+hx_bc bc_with_state(int kind, const std::vector<double>& state)
+{
+  if (state.empty() || state.size() > HX_MAX_N_DIM + 2)
+    throw std::runtime_error("hexed_amd: freestream state must have n_dim + 2 entries");
+  hx_bc bc = bc_of(kind);
+  std::copy(state.begin(), state.end(), bc.state);
+  return bc;
+}
+}  // namespace
+
+hx_bc freestream(const std::vector<double>& state) { return bc_with_state(HX_BC_FREESTREAM, state); }
+hx_bc riemann_invariants(const std::vector<double>& state) { return bc_with_state(HX_BC_RIEMANN_INVARIANTS, state); }
+hx_bc nonpenetration() { return bc_of(HX_BC_NONPENETRATION); }
+hx_bc copy() { return bc_of(HX_BC_COPY); }
+hx_bc outflow() { return bc_of(HX_BC_OUTFLOW); }
+
+hx_bc pressure_outflow(double pressure)
+{
+  hx_bc bc = bc_of(HX_BC_PRESSURE_OUTFLOW);
+  bc.pressure = pressure;
+  return bc;
+}
+
+hx_bc no_slip_heat_flux(double heat_flux, double coercion)
+{
+  hx_bc bc = bc_of(HX_BC_NO_SLIP);
+  bc.thermal = HX_THERMAL_HEAT_FLUX;
+  bc.thermal_value = heat_flux;
+  bc.coercion = coercion;
+  return bc;
+}
+
+hx_bc no_slip_energy(double energy_per_mass, double coercion)
+{
+  hx_bc bc = no_slip_heat_flux(0., coercion);
+  bc.thermal = HX_THERMAL_ENERGY;
+  bc.thermal_value = energy_per_mass;
+  return bc;
+}
+
+hx_bc no_slip_equilibrium(double emissivity, double heat_transfer_coef, double temperature, double coercion)
+{
+  hx_bc bc = no_slip_heat_flux(0., coercion);
+  bc.thermal = HX_THERMAL_EQUILIBRIUM;
+  bc.emissivity = emissivity;
+  bc.heat_transfer_coef = heat_transfer_coef;
+  bc.temperature = temperature;
+  return bc;
 }
 
 }  // namespace hexed::amd

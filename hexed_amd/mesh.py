@@ -91,7 +91,10 @@ class HostMesh:
         self.ref_stretch = np.zeros((n_ref, 2), np.int32)
         self.bc_con = np.zeros(0, np.int32)
         self.bc_serial = np.zeros(0, np.int32)
+        # (kind, freestream state[, {pressure, thermal, thermal_value, emissivity, heat_transfer_coef, temperature,
+        # coercion}]) per boundary condition
         self.bcs = []
+        self.bc_cache = None  # [n_bc_con][nv][nfq], allocated by struct()
         # geometry kept for initial conditions: qpoint positions per class [n][nd][nq]
         self.qpos = [None, None]
 
@@ -151,12 +154,22 @@ class HostMesh:
         m.bc_con = abi.iptr(self.bc_con)
         m.bc_serial = abi.iptr(self.bc_serial)
         self._bc_arr = (abi.hx_bc * max(1, len(self.bcs)))()
-        for i, (kind, st) in enumerate(self.bcs):
-            self._bc_arr[i].kind = kind
+        for i, bc in enumerate(self.bcs):
+            kind, st = bc[0], bc[1]
+            extra = bc[2] if len(bc) > 2 else {}
+            b = self._bc_arr[i]
+            b.kind = kind
             for k, v in enumerate(st):
-                self._bc_arr[i].state[k] = v
+                b.state[k] = v
+            b.coercion = 2.0  # No_slip's default heat_flux_coercion (Boundary_condition.hpp:198)
+            for name, value in extra.items():
+                setattr(b, name, value)
         m.n_bc = len(self.bcs)
         m.bcs = ctypes.cast(self._bc_arr, ctypes.POINTER(abi.hx_bc))
+        n_bc_con = len(self.bc_con)
+        if self.bc_cache is None or self.bc_cache.shape[0] != n_bc_con:
+            self.bc_cache = np.zeros((n_bc_con, self.nv, self.nfq))
+        m.bc_cache = abi.dptr(self.bc_cache)
         return m
 
 
@@ -307,8 +320,8 @@ def box(nd, rs, n, lo=0.0, hi=1.0, deformed=False, jitter=0.0, seed=0, bc=(abi.H
     m.con_nslot[:] = np.arange(n_nslot, dtype=np.int32).reshape(-1, 2)
     m.bc_con = np.arange(off, off + n_bnd, dtype=np.int32)
     m.bc_serial = np.zeros(n_bnd, np.int32)
-    kind, fs_state = bc
-    m.bcs = [(kind, list(fs_state) if fs_state is not None else [0.0] * (nd + 2))]
+    kind, fs_state = bc[0], bc[1]
+    m.bcs = [(kind, list(fs_state) if fs_state is not None else [0.0] * (nd + 2)) + tuple(bc[2:])]
     m.nom_size[cls][:] = h
     # geometry
     rng = np.random.default_rng(seed)
@@ -489,8 +502,10 @@ class Builder:
         self.elems.append(dict(deformed=bool(deformed), level=int(ref_level), pos=list(pos) + [0] * (self.nd - len(pos))))
         return len(self.elems) - 1
 
-    def add_bc(self, kind, state):
-        self.bcs.append((kind, list(state)))
+    def add_bc(self, kind, state, **params):
+        """params: the hx_bc fields a condition takes (pressure, thermal, thermal_value, emissivity,
+        heat_transfer_coef, temperature, coercion)."""
+        self.bcs.append((kind, list(state), dict(params)) if params else (kind, list(state)))
         return len(self.bcs) - 1
 
     def connect_cartesian(self, a, b, i_dim):

@@ -85,12 +85,29 @@ typedef enum hx_bc_kind {
   HX_BC_FREESTREAM = 0,      /* Freestream      (Boundary_condition.cpp:62-76, 299)  */
   HX_BC_COPY = 1,            /* Copy            (Boundary_condition.cpp:450-463)     */
   HX_BC_NONPENETRATION = 2,  /* Nonpenetration  (Boundary_condition.cpp:302-345)     */
-  HX_BC_OUTFLOW = 3          /* Outflow         (Boundary_condition.cpp:465-477)     */
+  HX_BC_OUTFLOW = 3,         /* Outflow         (Boundary_condition.cpp:465-477)     */
+  HX_BC_RIEMANN_INVARIANTS = 4, /* Riemann_invariants (Boundary_condition.cpp:78-182), about `state`; uses
+                                   hx_host_mesh::bc_cache */
+  HX_BC_PRESSURE_OUTFLOW = 5,   /* Pressure_outflow (Boundary_condition.cpp:184-223), at `pressure`        */
+  HX_BC_NO_SLIP = 6             /* No_slip (Boundary_condition.cpp:371-448) with the thermal condition
+                                   `thermal`; uses hx_host_mesh::bc_cache */
 } hx_bc_kind;
+
+/* Thermal condition of a no-slip wall (Boundary_condition.hpp:140-186). */
+typedef enum hx_thermal_kind {
+  HX_THERMAL_HEAT_FLUX = 0,   /* Prescribed_heat_flux: wall heat flux `thermal_value` (0: adiabatic, the default) */
+  HX_THERMAL_ENERGY = 1,      /* Prescribed_energy: specific energy `thermal_value` at the wall (isothermal)      */
+  HX_THERMAL_EQUILIBRIUM = 2  /* Thermal_equilibrium: `emissivity`, `heat_transfer_coef`, `temperature`          */
+} hx_thermal_kind;
 
 typedef struct hx_bc {
   int kind;                  /* hx_bc_kind */
   double state[HX_MAX_N_DIM + 2];  /* freestream conserved state: momentum[n_dim], density, energy */
+  double pressure;           /* Pressure_outflow: the specified pressure */
+  int thermal;               /* No_slip: hx_thermal_kind */
+  double thermal_value;      /* No_slip: heat flux or specific energy of the thermal condition */
+  double emissivity, heat_transfer_coef, temperature;  /* No_slip with HX_THERMAL_EQUILIBRIUM */
+  double coercion;           /* No_slip heat_flux_coercion (Boundary_condition.hpp:198; the reference default is 2) */
 } hx_bc;
 
 /*
@@ -142,6 +159,9 @@ typedef struct hx_host_mesh {
   int* bc_serial;         /* [n_bc_con] index into bcs */
   int n_bc;
   hx_bc* bcs;
+  double* bc_cache;       /* [n_bc_con][nv][nfq] state cache of the boundary faces (Boundary_face::state_cache,
+                             written by the state conditions Riemann_invariants and No_slip, read by their flux
+                             conditions); may be NULL (the device then keeps its own, zero-initialised) */
 } hx_host_mesh;
 
 /* Fields that move between host and device (hx_mesh_upload / hx_mesh_download). */

@@ -39,6 +39,7 @@ def load():
         "oracle_restrict_refined": (I, [M, PDE, I, I]),
         "oracle_apply_state_bcs": (I, [M]),
         "oracle_apply_flux_bcs": (I, [M]),
+        "oracle_characteristics": (I, [I] + [ctypes.POINTER(ctypes.c_double)] * 5),
         "oracle_compute_euler": (I, [M, KO]),
         "oracle_compute_navier_stokes": (I, [M, KO, abi.hx_transport, abi.hx_transport]),
         "oracle_max_dt_euler": (I, [M, KO, D, D, I, ctypes.POINTER(D)]),

@@ -1395,6 +1395,10 @@ void apply_advection_bcs(hx_host_mesh* m)
           for (int a = 0; a < rs; ++a) gh_f[(nd + a) * nfq + q] = in_f[(nd + a) * nfq + q];
         }
         break;
+      case HX_BC_NO_SLIP:  // Boundary_condition.cpp:429-448
+        for (int k = 0; k < nd * nfq; ++k) gh_f[k] = -in_f[k];
+        for (int k = 0; k < rs * nfq; ++k) gh_f[nd * nfq + k] = in_f[nd * nfq + k];
+        break;
       default:  // Flow_bc::apply_advection (Boundary_condition.cpp:24-41)
         for (int k = 0; k < nd * nfq; ++k) gh_f[k] = in_f[k];
         for (int k = 0; k < rs * nfq; ++k) gh_f[nd * nfq + k] = 2. - in_f[nd * nfq + k];
@@ -1636,9 +1640,127 @@ struct Run_av_points
   }
 };
 
+// Characteristics of the Euler flux about `ref` in direction `n` (pde.hpp:182-260), for Riemann_invariants. The
+// 3x3 system of the 1D decomposition is solved by Cramer's rule (the reference uses column-pivoting QR; the systems
+// here are nonsingular whenever the pressure is positive).
+struct Char_decomp
+{
+  int nd;
+  double vals[3], vecs[3][3], dir[3], veloc[3];
+  Char_decomp(int nd_, const double* ref, const double* n) : nd{nd_}
+  {
+    double norm = 0.;
+    for (int d = 0; d < nd; ++d) norm += n[d] * n[d];
+    norm = std::sqrt(norm);
+    double vsq = 0., vn = 0.;
+    for (int d = 0; d < nd; ++d) {
+      dir[d] = n[d] / norm;
+      veloc[d] = ref[d] / ref[nd];
+    }
+    for (int d = 0; d < nd; ++d) {
+      vsq += veloc[d] * veloc[d];
+      vn += dir[d] * veloc[d];
+    }
+    const double mass = ref[nd];
+    const double pres = .4 * (ref[nd + 1] - .5 * mass * vsq);
+    const double c = std::sqrt(1.4 * std::max(pres, 0.) / mass);
+    vals[0] = vn - c;
+    vals[1] = vn + c;
+    vals[2] = vn;
+    for (int k = 0; k < 2; ++k) {
+      const double dv = (2 * k - 1) * c / mass;
+      vecs[0][k] = vn + mass * dv;
+      vecs[1][k] = 1.;
+      vecs[2][k] = 1.4 * pres / mass / .4 + .5 * vsq + mass * vn * dv;
+    }
+    vecs[0][2] = vn;
+    vecs[1][2] = 1.;
+    vecs[2][2] = .5 * vsq;
+  }
+  static double det3(const double a[3][3])
+  {
+    return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  }
+  // part[v][c]: the component of `u` in the eigenspace of vals[c]; the parts sum to u
+  void split(const double* u, double part[5][3]) const
+  {
+    double un = 0., vn = 0.;
+    for (int d = 0; d < nd; ++d) {
+      un += dir[d] * u[d];
+      vn += dir[d] * veloc[d];
+    }
+    double vt[3], corr[3], vc = 0.;
+    for (int d = 0; d < nd; ++d) {
+      vt[d] = veloc[d] - vn * dir[d];
+      corr[d] = u[d] - un * dir[d] - u[nd] * vt[d];
+      vc += veloc[d] * corr[d];
+    }
+    const double rhs[3] = {un, u[nd], u[nd + 1] - vc};
+    const double det = det3(vecs);
+    if (det == 0.) throw std::runtime_error("singular characteristic system");
+    double coef[3];
+    for (int c = 0; c < 3; ++c) {
+      double a[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) a[i][j] = j == c ? rhs[i] : vecs[i][j];
+      coef[c] = det3(a) / det;
+    }
+    for (int c = 0; c < 3; ++c) {
+      for (int d = 0; d < nd; ++d) part[d][c] = dir[d] * vecs[0][c] * coef[c] + vt[d] * coef[c];
+      part[nd][c] = vecs[1][c] * coef[c];
+      part[nd + 1][c] = vecs[2][c] * coef[c];
+    }
+    for (int d = 0; d < nd; ++d) part[d][2] += corr[d];
+    part[nd + 1][2] += vc;
+  }
+  // apply_char (Boundary_condition.cpp:83-95)
+  void choose(int sign, const double* inside, const double* outside, double* out) const
+  {
+    double pi[5][3], po[5][3];
+    split(inside, pi);
+    split(outside, po);
+    for (int v = 0; v < nd + 2; ++v) {
+      out[v] = 0.;
+      for (int c = 0; c < 3; ++c) out[v] += sign * vals[c] > 0 ? po[v][c] : pi[v][c];
+    }
+  }
+};
+
+// No_slip's thermal condition (Boundary_condition.hpp:140-186, Boundary_condition.cpp:373-379)
+constexpr double sigma_sb()
+{
+  double pi5 = 1., k4 = 1., c2 = 1., h3 = 1.;
+  for (int i = 0; i < 5; ++i) pi5 *= M_PI;
+  for (int i = 0; i < 4; ++i) k4 *= 1.380649e-23;
+  for (int i = 0; i < 2; ++i) c2 *= 299792458.;
+  for (int i = 0; i < 3; ++i) h3 *= 6.62607015e-34;
+  return 2 * pi5 * k4 / (15 * c2 * h3);  // constants.hpp:52
+}
+double wall_energy(const hx_bc& bc, const double* u, int nv)
+{
+  return bc.thermal == HX_THERMAL_ENERGY ? bc.thermal_value * u[nv - 2] : u[nv - 1];
+}
+double wall_heat_flux(const hx_bc& bc, const double* u, int nv, double q)
+{
+  switch (bc.thermal) {
+    case HX_THERMAL_ENERGY: return q;
+    case HX_THERMAL_HEAT_FLUX: return bc.thermal_value;
+    default: {
+      const double temp = u[nv - 1] * .4 / u[nv - 2] / 287.05287;
+      return bc.emissivity * sigma_sb() * (temp * temp * temp * temp) + bc.heat_transfer_coef * (temp - bc.temperature);
+    }
+  }
+}
+
+bool needs_cache(int kind) { return kind == HX_BC_RIEMANN_INVARIANTS || kind == HX_BC_NO_SLIP; }
+
 // Boundary conditions on the deformed-class boundary connections (Boundary_condition.cpp)
 void apply_state_bcs(hx_host_mesh* m)
 {
+  for (int i = 0; i < m->n_bc; ++i)
+    if (needs_cache(m->bcs[i].kind) && m->n_bc_con && !m->bc_cache)
+      throw std::runtime_error("Riemann_invariants and No_slip need hx_host_mesh::bc_cache");
   Mesh_view mv{m};
   const int nd = m->n_dim, nv = nd + 2, nfq = ipow(m->row_size, nd - 1);
   #pragma omp parallel for
@@ -1670,6 +1792,49 @@ void apply_state_bcs(hx_host_mesh* m)
           for (int d = 0; d < nd; ++d) gh_f[d * nfq + q] -= 2 * dot * nrml[d * nfq + q] / norm_sq;
         }
         break;
+      case HX_BC_RIEMANN_INVARIANTS:
+      case HX_BC_PRESSURE_OUTFLOW:
+      case HX_BC_NO_SLIP: {
+        const int face_sign = (m->con_dir[HX_DEFORMED][con] >> 4) & 1;  // inside_face_sign
+        double* sc = m->bc_cache + size_t(i) * nv * nfq;
+        for (int q = 0; q < nfq; ++q) {
+          double in[5], n[3], g[5];
+          for (int v = 0; v < nv; ++v) in[v] = in_f[v * nfq + q];
+          for (int d = 0; d < nd; ++d) n[d] = nrml[d * nfq + q];
+          if (bc.kind == HX_BC_RIEMANN_INVARIANTS) {  // Boundary_condition.cpp:97-141
+            Char_decomp(nd, in, n).choose(1 - 2 * face_sign, in, bc.state, g);
+            g[nd] = std::max(g[nd], in[nd] / 2);
+            double ke = 0., ike = 0.;
+            for (int d = 0; d < nd; ++d) {
+              ke += g[d] * g[d];
+              ike += in[d] * in[d];
+            }
+            ke = .5 * ke / g[nd];
+            ike = .5 * ike / in[nd];
+            g[nd + 1] = std::max(ke + std::max(g[nd + 1] - ke, (in[nd + 1] - ike) / 2), 0.);
+            for (int v = 0; v < nv; ++v) sc[v * nfq + q] = in[v];
+          } else if (bc.kind == HX_BC_PRESSURE_OUTFLOW) {  // Boundary_condition.cpp:184-214
+            double mn = 0., nn = 0., mm = 0.;
+            for (int d = 0; d < nd; ++d) {
+              mn += in[d] * n[d];
+              nn += n[d] * n[d];
+              mm += in[d] * in[d];
+            }
+            const double ke = .5 * mm / in[nd];
+            const double p = std::max(.4 * (in[nd + 1] - ke), 0.);
+            for (int v = 0; v < nv; ++v) g[v] = in[v];
+            if (mn / in[nd] / std::sqrt(nn) * (2 * face_sign - 1) < std::sqrt(1.4 * p / in[nd])) g[nd + 1] = bc.pressure / .4 + ke;
+          } else {  // Boundary_condition.cpp:381-400
+            for (int d = 0; d < nd; ++d) g[d] = -in[d];
+            g[nd] = in[nd];
+            const double e = wall_energy(bc, in, nv);
+            g[nd + 1] = e * e / in[nd + 1];
+            for (int v = 0; v < nv; ++v) sc[v * nfq + q] = (g[v] + in[v]) / 2;
+          }
+          for (int v = 0; v < nv; ++v) gh_f[v * nfq + q] = g[v];
+        }
+        break;
+      }
       default:
         break;
     }
@@ -1678,6 +1843,9 @@ void apply_state_bcs(hx_host_mesh* m)
 
 void apply_flux_bcs(hx_host_mesh* m)
 {
+  for (int i = 0; i < m->n_bc; ++i)
+    if (needs_cache(m->bcs[i].kind) && m->n_bc_con && !m->bc_cache)
+      throw std::runtime_error("Riemann_invariants and No_slip need hx_host_mesh::bc_cache");
   Mesh_view mv{m};
   const int nd = m->n_dim, nv = nd + 2, nfq = ipow(m->row_size, nd - 1);
   #pragma omp parallel for
@@ -1708,6 +1876,40 @@ void apply_flux_bcs(hx_host_mesh* m)
             norm_sq += n * n;
           }
           for (int d = 0; d < nd; ++d) g[d * nfq + q] -= 2 * dot * nrml[d * nfq + q] / norm_sq;
+        }
+        break;
+      }
+      case HX_BC_PRESSURE_OUTFLOW:  // Boundary_condition.cpp:216-223
+        for (int k = 0; k < nv * nfq; ++k) gh_f[nv * nfq + k] = -in_f[nv * nfq + k];
+        break;
+      case HX_BC_RIEMANN_INVARIANTS:
+      case HX_BC_NO_SLIP: {
+        const int face_sign = (m->con_dir[HX_DEFORMED][con] >> 4) & 1;
+        const double* sc = m->bc_cache + size_t(i) * nv * nfq;
+        const double* f = in_f + nv * nfq;
+        double* g = gh_f + nv * nfq;
+        for (int q = 0; q < nfq; ++q) {
+          double fl[5], cs[5], n[3];
+          for (int v = 0; v < nv; ++v) {
+            fl[v] = f[v * nfq + q];
+            cs[v] = sc[v * nfq + q];
+          }
+          for (int d = 0; d < nd; ++d) n[d] = nrml[d * nfq + q];
+          if (bc.kind == HX_BC_RIEMANN_INVARIANTS) {  // Boundary_condition.cpp:143-182
+            const double zero[5] = {0., 0., 0., 0., 0.};
+            double out[5];
+            Char_decomp(nd, cs, n).choose(1 - 2 * face_sign, zero, fl, out);
+            for (int v = 0; v < nv; ++v) g[v * nfq + q] = out[v];
+          } else {  // Boundary_condition.cpp:402-427
+            for (int d = 0; d < nd; ++d) g[d * nfq + q] = fl[d];
+            g[nd * nfq + q] = -fl[nd];
+            double nn = 0.;
+            for (int d = 0; d < nd; ++d) nn += n[d] * n[d];
+            const double normal = std::sqrt(nn);
+            const int fs = 2 * face_sign - 1;
+            const double qe = fl[nd + 1];
+            g[(nd + 1) * nfq + q] = bc.coercion * (normal * fs * wall_heat_flux(bc, cs, nv, qe * fs / normal) - qe) + qe;
+          }
         }
         break;
       }
@@ -1855,6 +2057,19 @@ int oracle_restrict_refined(hx_host_mesh* m, const hx_pde* pde, int scale, int o
 
 int oracle_apply_state_bcs(hx_host_mesh* m) { return guarded([&] { apply_state_bcs(m); }); }
 int oracle_apply_flux_bcs(hx_host_mesh* m) { return guarded([&] { apply_flux_bcs(m); }); }
+int oracle_characteristics(int n_dim, const double* ref, const double* normal, const double* state, double* vals,
+                           double* parts)
+{
+  return guarded([&] {
+    if (n_dim < 1 || n_dim > 3) throw std::runtime_error("invalid dimensionality");
+    const Char_decomp ch(n_dim, ref, normal);
+    double p[5][3];
+    ch.split(state, p);
+    for (int c = 0; c < 3; ++c) vals[c] = ch.vals[c];
+    for (int v = 0; v < n_dim + 2; ++v)
+      for (int c = 0; c < 3; ++c) parts[v * 3 + c] = p[v][c];
+  });
+}
 
 // compute_euler (kernels_convective.cpp:8-18)
 int oracle_compute_euler(hx_host_mesh* m, const hx_kernel_options* o)

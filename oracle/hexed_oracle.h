@@ -31,6 +31,10 @@ int oracle_prolong_refined(hx_host_mesh* m, const hx_pde* pde, int scale, int of
 int oracle_restrict_refined(hx_host_mesh* m, const hx_pde* pde, int scale, int offset);   /* Spatial.hpp:215-286 */
 int oracle_apply_state_bcs(hx_host_mesh* m);                                               /* Solver.cpp:56-67    */
 int oracle_apply_flux_bcs(hx_host_mesh* m);                                                /* Solver.cpp:69-81    */
+/* Characteristics of the Euler flux (pde.hpp:182-260) about `ref` in direction `normal`: vals[3], and parts[nv][3],
+ * the decomposition of `state` into the eigenspaces. */
+int oracle_characteristics(int n_dim, const double* ref, const double* normal, const double* state, double* vals,
+                           double* parts);
 int oracle_compute_euler(hx_host_mesh* m, const hx_kernel_options* o);                    /* kernels_convective.cpp:8-18 */
 int oracle_compute_navier_stokes(hx_host_mesh* m, const hx_kernel_options* o,             /* kernels_diffusive.cpp:8-29  */
                                  hx_transport visc, hx_transport cond);

@@ -199,7 +199,7 @@ struct Vec_seq : public Sequence<T&>
 struct Oracle_mesh
 {
   Params p;
-  std::vector<double> state, rcache, tss, av, nom, vtss, metrics, faces, normals;
+  std::vector<double> state, rcache, tss, av, nom, vtss, metrics, faces, normals, bc_cache;
   std::vector<int> face_slot, face_nslot, con_slot, con_dir, con_nslot, bc_con, bc_serial;
   std::vector<hx_bc> bcs;
   std::vector<int> car_slot, car_dir, def_slot, def_dir, def_nslot;
@@ -243,6 +243,7 @@ struct Oracle_mesh
         for (int q = 0; q < p.nfq; ++q) normals[(size_t(s + side) * p.nd + d) * p.nfq + q] = 1.;
     }
     bcs.push_back(bc);
+    bc_cache.assign(size_t(n_b) * p.nv * p.nfq, 0.);
     for (auto& e : b.elems) {
       state.insert(state.end(), e->data.begin(), e->data.begin() + p.nv * p.nq);
       rcache.insert(rcache.end(), e->rcache.begin(), e->rcache.end());
@@ -315,6 +316,7 @@ struct Oracle_mesh
     d.bc_serial = bc_serial.data();
     d.n_bc = 1;
     d.bcs = bcs.data();
+    d.bc_cache = bc_cache.data();
     return d;
   }
 };
@@ -352,7 +354,7 @@ static void host_only_checks()
 }
 
 // compute_* through the adapter on a registered mesh vs the oracle on the same data
-static void compare_update(int nd, int rs, int n, bool def, bool visc)
+static void compare_update(int nd, int rs, int n, bool def, bool visc, int bc_kind = HX_BC_FREESTREAM)
 {
   Box b(nd, rs, n, def);
   b.init(0.5, nd * 10 + rs);
@@ -361,10 +363,15 @@ static void compare_update(int nd, int rs, int n, bool def, bool visc)
   Vec_seq<Kernel_connection> car_c(b.car_c), def_c(b.def_c);
   Vec_seq<Refined_face> refs;
   Kernel_mesh km{nd, rs, basis, car_c, def_c, car, dfm, all, refs};
-  hx_bc fs{HX_BC_FREESTREAM, {}};
-  fs.state[0] = 0.5;
-  fs.state[nd] = 1.;
-  fs.state[nd + 1] = 1. / 1.4 / 0.4 + 0.125;
+  std::vector<double> fs_state(nd + 2, 0.);
+  fs_state[0] = 0.5;
+  fs_state[nd] = 1.;
+  fs_state[nd + 1] = 1. / 1.4 / 0.4 + 0.125;
+  // the reference's constructors with their defaults (Boundary_condition.hpp:66-198)
+  hx_bc fs = bc_kind == HX_BC_RIEMANN_INVARIANTS ? amd::riemann_invariants(fs_state)
+             : bc_kind == HX_BC_NO_SLIP          ? amd::no_slip_heat_flux()
+             : bc_kind == HX_BC_PRESSURE_OUTFLOW ? amd::pressure_outflow(1. / 1.4)
+                                                 : amd::freestream(fs_state);
   // initial faces from the element state (Solver::initialize -> compute_write_face)
   {
     amd::Accelerated_mesh am(km);
@@ -404,8 +411,8 @@ static void compare_update(int nd, int rs, int n, bool def, bool visc)
   std::vector<double> got;
   for (auto& e : b.elems) got.insert(got.end(), e->data.begin(), e->data.begin() + b.p.nv * b.p.nq);
   double err = rel_err(got, om.state);
-  std::printf("update nd %d rs %d %s %s: relative L2 vs oracle %.3e, dt %.6e vs %.6e\n", nd, rs, def ? "deformed" : "cartesian",
-              visc ? "navier-stokes" : "euler", err, st.time_step, so.time_step);
+  std::printf("update nd %d rs %d %s %s bc %d: relative L2 vs oracle %.3e, dt %.6e vs %.6e\n", nd, rs,
+              def ? "deformed" : "cartesian", visc ? "navier-stokes" : "euler", bc_kind, err, st.time_step, so.time_step);
   REQUIRE(err <= 1e-12);
   REQUIRE(std::fabs(st.time_step - so.time_step) <= 1e-13 * so.time_step);
 
@@ -415,6 +422,13 @@ static void compare_update(int nd, int rs, int n, bool def, bool visc)
   sw.children.emplace("local", Stopwatch_tree("(element*(time integration stage))"));
   double dt = 1e-3;
   am.push();
+  // after an update the face buffers hold Neighbor's fluxes: write the faces and apply the state conditions first,
+  // as a stage does (kernels_convective.cpp:8-18, Solver.cpp:56-67)
+  compute_write_face(km);
+  amd::check(hx_apply_state_bcs(am.device()), "apply_state_bcs");
+  REQUIRE(oracle_write_face(&od, 0, nullptr) == 0);
+  REQUIRE(oracle_write_face(&od, 1, nullptr) == 0);
+  REQUIRE(oracle_apply_state_bcs(&od) == 0);
   (*kernel_factory<Spatial<pde::Navier_stokes<false>::Pde, false>::Neighbor>(nd, rs, 0))(car_c, sw, "neighbor");
   (*kernel_factory<Spatial<pde::Navier_stokes<false>::Pde, true>::Neighbor>(nd, rs, 0))(def_c, sw, "neighbor");
   (*kernel_factory<Spatial<pde::Navier_stokes<false>::Pde, false>::Local>(nd, rs, basis, dt, 0, false, false))(car, sw, "local");
@@ -467,6 +481,9 @@ int main(int argc, char** argv)
     compare_update(3, 6, 2, false, true);
     compare_update(2, 5, 3, true, false);
     compare_update(3, 6, 2, true, true);
+    compare_update(3, 5, 2, true, true, HX_BC_NO_SLIP);
+    compare_update(2, 6, 3, true, true, HX_BC_RIEMANN_INVARIANTS);
+    compare_update(3, 4, 2, false, false, HX_BC_PRESSURE_OUTFLOW);
   }
   std::printf("%s: %d failure(s)\n", n_fail ? "FAIL" : "PASS", n_fail);
   return n_fail ? 1 : 0;
