@@ -84,18 +84,19 @@ class hx_kernel_options(ctypes.Structure):
 class hx_update_options(ctypes.Structure):
     _fields_ = [("max_safety", ctypes.c_double), ("max_time_step", ctypes.c_double), ("n_cheby_flow", ctypes.c_int),
                 ("flow_iters", ctypes.c_int), ("local_time", ctypes.c_int), ("use_filter", ctypes.c_int),
-                ("use_ldg", ctypes.c_int), ("visc", hx_transport), ("cond", hx_transport)]
+                ("use_ldg", ctypes.c_int), ("visc", hx_transport), ("cond", hx_transport),
+                ("fix_admis", ctypes.c_int), ("fix_admis_max_safety", ctypes.c_double)]
 
 
 def update_options(max_safety=0.7, max_time_step=1e300, n_cheby_flow=1, flow_iters=1, local_time=False, use_filter=False,
-                   visc=None, cond=None, use_ldg=None):
-    """Defaults of Solver (Solver.cpp:151-174); use_ldg follows Solver::use_ldg (Solver.cpp:117-120) unless given."""
+                   visc=None, cond=None, use_ldg=None, fix_admis=False, fix_admis_max_safety=0.7):
+    """Defaults of Solver (Solver.cpp:143-174); use_ldg follows Solver::use_ldg (Solver.cpp:117-120) unless given."""
     visc = visc if visc is not None else inviscid()
     cond = cond if cond is not None else inviscid()
     if use_ldg is None:
         use_ldg = bool(visc.is_viscous or cond.is_viscous)
     return hx_update_options(max_safety, max_time_step, n_cheby_flow, flow_iters, int(local_time), int(use_filter),
-                             int(use_ldg), visc, cond)
+                             int(use_ldg), visc, cond, int(fix_admis), fix_admis_max_safety)
 
 
 class hx_av_options(ctypes.Structure):
@@ -129,7 +130,8 @@ def smooth_av_pde(diff_time, cheby_step):
 
 
 class hx_update_status(ctypes.Structure):
-    _fields_ = [("time_step", ctypes.c_double), ("flow_time", ctypes.c_double), ("iteration", ctypes.c_int)]
+    _fields_ = [("time_step", ctypes.c_double), ("flow_time", ctypes.c_double), ("iteration", ctypes.c_int),
+                ("fix_admis_iters", ctypes.c_int)]
 
 
 def dptr(a):
@@ -222,6 +224,12 @@ def load_library(path=LIB_PATH):
         "hx_max_dt_smooth_av": (ctypes.c_int, [P, ctypes.POINTER(hx_kernel_options), ctypes.c_double, ctypes.c_double,
                                                ctypes.c_int, c_double_p]),
         "hx_update_art_visc_smoothness": (ctypes.c_int, [P, ctypes.POINTER(hx_av_options), c_double_p]),
+        "hx_is_admissible": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int)]),
+        "hx_fix_admissibility": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.c_double,
+                                                ctypes.POINTER(ctypes.c_int)]),
+        "hx_compute_fix_therm_admis": (ctypes.c_int, [P, ctypes.POINTER(hx_kernel_options)]),
+        "hx_max_dt_fix_therm_admis": (ctypes.c_int, [P, ctypes.POINTER(hx_kernel_options), ctypes.c_double,
+                                                     ctypes.c_double, ctypes.c_int, c_double_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -243,5 +251,6 @@ EXPORTED_SYMBOLS = [
     "hx_update", "hx_compute_residual", "hx_apply_advection_bcs", "hx_apply_avc_diff_bcs", "hx_apply_avc_diff_flux_bcs",
     "hx_compute_advection", "hx_compute_smooth_av", "hx_compute_write_face_advection", "hx_compute_write_face_smooth_av",
     "hx_compute_prolong_advection", "hx_max_dt_advection", "hx_max_dt_smooth_av", "hx_update_art_visc_smoothness",
-    "hx_mesh_set_cases", "hx_update_cases",
+    "hx_mesh_set_cases", "hx_update_cases", "hx_is_admissible", "hx_fix_admissibility", "hx_compute_fix_therm_admis",
+    "hx_max_dt_fix_therm_admis",
 ]

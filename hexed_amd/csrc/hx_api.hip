@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -64,6 +65,7 @@ struct hx_mesh
   double* dt_block_min = nullptr;  // one minimum per workgroup of the Euler Local, both classes
   long long n_dt_blocks = 0;
   Dt_fuse fuse{};                  // the fusion the next Euler Local launch performs (block_min == nullptr: none)
+  int* admis_flags = nullptr;      // [2] results of the admissibility check (hx_is_admissible)
   struct
   {
     bool valid = false;
@@ -304,11 +306,13 @@ Transport to_dev(const hx_transport& t)
 
 int pde_kind(const hx_pde* pde) { return pde ? pde->kind : HX_PDE_EULER; }
 bool is_av(int kind) { return kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV; }
+// PDEs served by the generator's kernel family (hx_av_kernels.hpp): the AV generator's two and Fix_therm_admis
+bool is_aux(int kind) { return is_av(kind) || kind == HX_PDE_FIX_ADMIS; }
 
 hx_status pde_check(hx_mesh* mesh, const hx_pde* pde, bool* visc, Launch* l)
 {
   int kind = pde_kind(pde);
-  if (kind != HX_PDE_EULER && kind != HX_PDE_NAVIER_STOKES && !is_av(kind))
+  if (kind != HX_PDE_EULER && kind != HX_PDE_NAVIER_STOKES && !is_aux(kind))
     return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "PDE kind not built into this library");
   if (is_av(kind)) {
     for (int c = 0; c < 2; ++c)
@@ -343,6 +347,7 @@ const char* pde_name(const hx_pde* pde)
     case HX_PDE_NAVIER_STOKES: return "navier_stokes";
     case HX_PDE_ADVECTION: return "advection";
     case HX_PDE_SMOOTH_AV: return "smooth_av";
+    case HX_PDE_FIX_ADMIS: return "fix_therm_admis";
     default: return "euler";
   }
 }
@@ -476,7 +481,7 @@ extern "C" {
 
 int hx_kernels_built(int n_dim, int row_size, int pde_kind)
 {
-  if (pde_kind != HX_PDE_EULER && pde_kind != HX_PDE_NAVIER_STOKES && !is_av(pde_kind)) return 0;
+  if (pde_kind != HX_PDE_EULER && pde_kind != HX_PDE_NAVIER_STOKES && !is_aux(pde_kind)) return 0;
   return kernel_set(n_dim, row_size) ? 1 : 0;
 }
 
@@ -786,7 +791,7 @@ hx_status hx_write_face(hx_mesh* mesh, int cls, const hx_pde* pde)
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
   Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/write face/" + pde_name(pde), mesh->dm.n_elem[cls]);
-  if (is_av(pde_kind(pde))) HX_TRY(mesh->ctx, mesh->ks->av_write_face(l, cls, pde->kind), "write_face");
+  if (is_aux(pde_kind(pde))) HX_TRY(mesh->ctx, mesh->ks->av_write_face(l, cls, pde->kind), "write_face");
   else HX_TRY(mesh->ctx, mesh->ks->write_face(l, cls), "write_face");
   return HX_OK;
 }
@@ -798,7 +803,7 @@ hx_status hx_neighbor(hx_mesh* mesh, int cls, const hx_pde* pde, int)
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
   Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor/" + pde_name(pde), mesh->dm.n_con[cls]);
-  if (is_av(pde_kind(pde))) HX_TRY(mesh->ctx, mesh->ks->av_neighbor(l, cls, pde->kind), "neighbor");
+  if (is_aux(pde_kind(pde))) HX_TRY(mesh->ctx, mesh->ks->av_neighbor(l, cls, pde->kind), "neighbor");
   else HX_TRY(mesh->ctx, mesh->ks->neighbor(l, cls, visc), "neighbor");
   return HX_OK;
 }
@@ -809,21 +814,22 @@ hx_status hx_local(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_s
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  const int kind = pde_kind(pde);
   // HEXED_ASSERTs of the Local constructor (Spatial.hpp:322-323)
-  if (visc && i_stage)
+  if ((visc || kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS) && i_stage)
     return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
   if (i_stage && compute_residual)
     return fail(mesh->ctx, HX_ERR_INVALID, "residual calculation is a single-stage operation");
   const double coef = i_stage ? dt * hx_basis_step_ratio(mesh->dm.rs) : dt;
-  const int kind = pde_kind(pde);
-  if (is_av(kind) && use_filter)
-    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the artificial-viscosity PDEs are built without the modal filter");
-  if (kind == HX_PDE_SMOOTH_AV && (i_stage || compute_residual))
-    return fail(mesh->ctx, HX_ERR_INVALID, "Smooth_art_visc runs single-stage without compute_residual");
+  if (is_aux(kind) && use_filter)
+    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the artificial-viscosity and admissibility PDEs are built without the modal filter");
+  if ((kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS) && compute_residual)
+    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the diffusion-only PDEs are built without compute_residual");
   Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/local/" + pde_name(pde), mesh->dm.n_elem[cls]);
   if (kind == HX_PDE_ADVECTION)
     HX_TRY(mesh->ctx, mesh->ks->adv_local(l, cls, coef, i_stage, compute_residual, pde->advect_length), "local");
-  else if (kind == HX_PDE_SMOOTH_AV) HX_TRY(mesh->ctx, mesh->ks->sav_local(l, cls, coef, pde->diff_time), "local");
+  else if (kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS)
+    HX_TRY(mesh->ctx, mesh->ks->diff_local(l, cls, kind, coef, pde->diff_time), "local");
   else HX_TRY(mesh->ctx, mesh->ks->local(l, cls, visc, coef, i_stage, compute_residual, use_filter), "local");
   return HX_OK;
 }
@@ -832,10 +838,11 @@ hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde* pde)
 {
   if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
-  const bool smooth = pde_kind(pde) == HX_PDE_SMOOTH_AV;
-  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor reconcile/" + (smooth ? "smooth_av" : "navier_stokes"),
+  const int kind = pde_kind(pde);
+  const bool diff_only = kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS;
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/neighbor reconcile/" + (diff_only ? pde_name(pde) : "navier_stokes"),
            mesh->dm.n_con[cls]);
-  if (smooth) HX_TRY(mesh->ctx, mesh->ks->sav_neighbor_reconcile(l, cls), "neighbor_reconcile");
+  if (diff_only) HX_TRY(mesh->ctx, mesh->ks->diff_neighbor_reconcile(l, cls, kind), "neighbor_reconcile");
   else HX_TRY(mesh->ctx, mesh->ks->neighbor_reconcile(l, cls), "neighbor_reconcile");
   return HX_OK;
 }
@@ -847,13 +854,16 @@ hx_status hx_reconcile_ldg_flux(hx_mesh* mesh, int cls, const hx_pde* pde, doubl
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
-  if ((visc || pde_kind(pde) == HX_PDE_SMOOTH_AV) && i_stage)
+  const int kind = pde_kind(pde);
+  if ((visc || kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS) && i_stage)
     return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
-  if (pde_kind(pde) == HX_PDE_SMOOTH_AV) {
+  if (kind == HX_PDE_ADVECTION)
+    return fail(mesh->ctx, HX_ERR_INVALID, "Reconcile_ldg_flux needs a PDE with diffusion");
+  if (kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS) {
     if (compute_residual || use_filter)
-      return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "Smooth_art_visc is built without compute_residual and the filter");
-    Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/reconcile LDG flux/smooth_av", mesh->dm.n_elem[cls]);
-    HX_TRY(mesh->ctx, mesh->ks->sav_reconcile(l, cls, dt, pde->diff_time, pde->cheby_step), "reconcile_ldg_flux");
+      return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the diffusion-only PDEs are built without compute_residual and the filter");
+    Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/reconcile LDG flux/" + pde_name(pde), mesh->dm.n_elem[cls]);
+    HX_TRY(mesh->ctx, mesh->ks->diff_reconcile(l, cls, kind, dt, pde->diff_time, pde->cheby_step), "reconcile_ldg_flux");
     return HX_OK;
   }
   Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/reconcile LDG flux/navier_stokes", mesh->dm.n_elem[cls]);
@@ -874,8 +884,9 @@ hx_status hx_max_dt(hx_mesh* mesh, int cls, const hx_pde* pde, int is_local, int
   HX_TRY(mesh->ctx, hipGetLastError(), "fill");
   {
     Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/compute time step/" + pde_name(pde), mesh->dm.n_elem[cls]);
-    if (is_av(pde_kind(pde)))
-      HX_TRY(mesh->ctx, mesh->ks->av_max_dt(l, cls, pde->kind == HX_PDE_SMOOTH_AV, is_local, max_cfl_c, max_cfl_d),
+    // Smooth_art_visc and Fix_therm_admis share Max_dt's diffusivity-1 form (pde.hpp:412-416, 486-490)
+    if (is_aux(pde_kind(pde)))
+      HX_TRY(mesh->ctx, mesh->ks->av_max_dt(l, cls, pde->kind != HX_PDE_ADVECTION, is_local, max_cfl_c, max_cfl_d),
              "max_dt");
     else HX_TRY(mesh->ctx, mesh->ks->max_dt(l, cls, visc, is_local, max_cfl_c, max_cfl_d), "max_dt");
   }
@@ -1007,7 +1018,7 @@ hx_status hx_compute_write_face(hx_mesh* mesh)
 hx_status hx_compute_prolong(hx_mesh* mesh, int scale, int offset) { return hx_prolong_refined(mesh, nullptr, scale, offset); }
 hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset) { return hx_restrict_refined(mesh, nullptr, scale, offset); }
 
-// Solver::update (Solver.cpp:834-886); fix_admissibility is off by default (Solver.cpp:143) and not on this path
+// Solver::update (Solver.cpp:834-886), with fix_admissibility after each stage when enabled (Solver.cpp:867)
 hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
 {
   if (!mesh || !o || !st) return HX_ERR_INVALID;
@@ -1066,6 +1077,7 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status*
         if (s) return s;
       }
       const double dt = std::min(mdt, o->max_time_step) * cheby(n_cheby, i_cheby);
+      bool fixed = false;
       for (int i = 0; i < 2; ++i) {
         hx_kernel_options opts{dt, i, 0, o->use_filter};
         if ((s = hx_apply_state_bcs(mesh))) return s;
@@ -1079,6 +1091,17 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status*
           mesh->fuse = Dt_fuse{};
         }
         if (s) return s;
+        if (o->fix_admis) {
+          int iters = 0;
+          if ((s = hx_fix_admissibility(mesh, o, o->fix_admis_max_safety, &iters))) return s;
+          st->fix_admis_iters += iters;
+          fixed = iters || fixed;
+        }
+      }
+      // a fix changed the state after the fused Max_dt read it, and ends this Chebyshev sequence (Solver.cpp:869)
+      if (fixed) {
+        pending = false;
+        break;
       }
       pending = fusable;
       st->time_step = dt;
@@ -1086,7 +1109,7 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status*
     }
   }
   ++st->iteration;
-  if (fusable) {
+  if (fusable && pending) {
     c.valid = true;
     c.gen = mesh->gen;
     c.msc = msc;
@@ -1205,6 +1228,7 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
   hx_ctx* ctx = mesh->ctx;
   const int nc = mesh->n_cases;
   if (nc < 1) return fail(ctx, HX_ERR_INVALID, "no cases registered (hx_mesh_set_cases)");
+  if (o->fix_admis) return fail(ctx, HX_ERR_UNSUPPORTED, "fix_admissibility is not built for batches of cases");
   if (o->local_time) {
     // every case takes the same step, min(1, max_time_step) * chebyshev_step
     hx_update_status one{};
@@ -1407,6 +1431,157 @@ hx_status hx_max_dt_smooth_av(hx_mesh* mesh, const hx_kernel_options* o, double 
   if ((s = hx_max_dt(mesh, HX_CARTESIAN, &p, local_time, uf, msc, msd, &a))) return s;
   if ((s = hx_max_dt(mesh, HX_DEFORMED, &p, local_time, uf, msc, msd, &b))) return s;
   *dt_out = std::min(a, b);
+  return HX_OK;
+}
+
+// The fix's boundary faces: mode 1 ghost state = inside (Solver.cpp:1066-1071), mode 2 ghost LDG = -inside
+// (Solver::apply_fta_flux_bcs, Solver.cpp:103-115); the same copies as the AV diffusion hooks
+static hx_status fta_bcs(hx_mesh* mesh, int mode)
+{
+  Timer tm(mesh->ctx, "fix admis./boundary conditions", mesh->dm.n_bc_con);
+  HX_TRY(mesh->ctx, mesh->ks->av_bcs(make_launch(mesh), mode), "fix admissibility bcs");
+  return HX_OK;
+}
+
+// compute_fix_therm_admis (kernels_diffusive.cpp:32): the COMPUTE_DIFFUSION sequence with pde::Fix_therm_admis; the
+// flux_bc callback is the fix's own (Solver::apply_fta_flux_bcs, Solver.cpp:103-115)
+hx_status hx_compute_fix_therm_admis(hx_mesh* mesh, const hx_kernel_options* o)
+{
+  if (!mesh || !o) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_FIX_ADMIS;
+  if (o->i_stage)
+    return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
+  hx_status s;
+  if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, 0))) return s;
+  if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, 0))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 0, 1))) return s;
+  if ((s = hx_local(mesh, HX_CARTESIAN, &p, o->dt, 0, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_local(mesh, HX_DEFORMED, &p, o->dt, 0, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_prolong_refined(mesh, &p, 1, 1))) return s;
+  if ((s = fta_bcs(mesh, 2))) return s;
+  if ((s = hx_neighbor_reconcile(mesh, HX_CARTESIAN, &p))) return s;
+  if ((s = hx_neighbor_reconcile(mesh, HX_DEFORMED, &p))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 1, 1))) return s;
+  if ((s = hx_reconcile_ldg_flux(mesh, HX_CARTESIAN, &p, o->dt, 0, o->compute_residual, o->use_filter))) return s;
+  if ((s = hx_reconcile_ldg_flux(mesh, HX_DEFORMED, &p, o->dt, 0, o->compute_residual, o->use_filter))) return s;
+  return hx_prolong_refined(mesh, &p, 0, 0);
+}
+
+hx_status hx_max_dt_fix_therm_admis(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time,
+                                    double* dt_out)
+{
+  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  hx_pde p{};
+  p.kind = HX_PDE_FIX_ADMIS;
+  double a, b;
+  hx_status s;
+  const int uf = o ? o->use_filter : 0;
+  if ((s = hx_max_dt(mesh, HX_CARTESIAN, &p, local_time, uf, msc, msd, &a))) return s;
+  if ((s = hx_max_dt(mesh, HX_DEFORMED, &p, local_time, uf, msc, msd, &b))) return s;
+  *dt_out = std::min(a, b);
+  return HX_OK;
+}
+
+// Solver::is_admissible (Solver.cpp:921-959)
+hx_status hx_is_admissible(hx_mesh* mesh, int* admissible)
+{
+  if (!mesh || !admissible) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  ++mesh->gen;
+  if (!mesh->admis_flags) {
+    void* p = nullptr;
+    HX_TRY(ctx, hipMalloc(&p, 2 * sizeof(int)), "hipMalloc(admissibility flags)");
+    mesh->allocs.push_back(p);
+    mesh->admis_flags = static_cast<int*>(p);
+  }
+  Launch l = make_launch(mesh);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<unsigned long long*>(mesh->admis_flags),
+                     0ull, 1);
+  HX_TRY(ctx, hipGetLastError(), "fill");
+  {
+    Timer tm(ctx, "check admis.", (long long)mesh->dm.n_elem[0] + mesh->dm.n_elem[1]);
+    HX_TRY(ctx, mesh->ks->admissible(l, mesh->admis_flags), "is_admissible");
+  }
+  int h[2] = {0, 0};
+  HX_TRY(ctx, hipMemcpyAsync(h, mesh->admis_flags, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "admissibility readback");
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "admissibility sync");
+  if (h[1]) return fail(ctx, HX_ERR_NOT_FINITE, "state is not finite");
+  *admissible = !h[0];
+  return HX_OK;
+}
+
+// Solver::fix_admissibility (Solver.cpp:961-1095). Each iteration of the reference writes a fix coefficient (the
+// inadmissible elements' flags, shared by maximum over vertices twice, matched at hanging vertices and interpolated to
+// the quadrature points) into the Laplacian artificial viscosity and swaps it with the bulk one around the fix's Max_dt,
+// its two diffusion steps and the flow's Max_dt; afterwards the Laplacian coefficient holds the fix coefficient. The
+// coefficient reaches no result: the fix's Max_dt and diffusion do not read artificial viscosity; the flow Max_dt's time
+// step scale is overwritten by the next iteration's fix Max_dt; and the last iteration always starts from an
+// admissible state (an inadmissible one restarts the count), so its coefficient is zero everywhere. Here every
+// iteration therefore swaps with a zero coefficient, which leaves the same state, time step scale and artificial
+// viscosity (Laplacian coefficient zero) as the reference.
+hx_status hx_fix_admissibility(hx_mesh* mesh, const hx_update_options* o, double stability_ratio, int* n_iters_out)
+{
+  if (!mesh || !o || !n_iters_out) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  Dev_mesh& dm = mesh->dm;
+  hx_status s;
+  int iter;
+  int n_iters = std::numeric_limits<int>::max();
+  for (iter = 0; iter < n_iters; ++iter) {
+    if (iter >= 100000)
+      return fail(ctx, HX_ERR_INVALID,
+                  "failed to fix thermodynamic admissability in " + std::to_string(iter) + " iterations");
+    int admissible = 0;
+    if ((s = hx_is_admissible(mesh, &admissible))) return s;
+    if (admissible) {
+      if (iter) n_iters = std::min(n_iters, 2 * iter);
+      else {
+        ++iter;
+        break;
+      }
+    } else {
+      n_iters = std::numeric_limits<int>::max();
+    }
+    // every element has artificial-viscosity storage in the reference (zero unless set)
+    for (int c = 0; c < 2; ++c)
+      if (dm.n_elem[c] && !dm.av[c]) {
+        void* p = nullptr;
+        const size_t bytes = size_t(dm.n_elem[c]) * 2 * dm.nq * sizeof(double);
+        HX_TRY(ctx, hipMalloc(&p, bytes), "hipMalloc(artificial viscosity)");
+        mesh->allocs.push_back(p);
+        HX_TRY(ctx, hipMemsetAsync(p, 0, bytes, ctx->stream), "hipMemsetAsync(artificial viscosity)");
+        dm.av[c] = static_cast<double*>(p);
+      }
+    Launch l = make_launch(mesh);
+    HX_TRY(ctx, mesh->ks->fix_av(l, 0), "fix_av");
+    hx_kernel_options ko{0., 0, 0, 0};
+    double dummy;
+    if ((s = hx_max_dt_fix_therm_admis(mesh, &ko, stability_ratio, stability_ratio, 1, &dummy))) return s;
+    // dt = 1: the two steps of a second-order stabilized update (Solver.cpp:1059-1064)
+    const double linear = 1., quadratic = 1. / 8 / 0.9;
+    double step[2];
+    step[1] = (linear + std::sqrt(linear * linear - 4 * quadratic)) / 2.;
+    step[0] = quadratic / step[1];
+    for (double st : step) {
+      if ((s = fta_bcs(mesh, 1))) return s;
+      ko.dt = st;
+      if ((s = hx_compute_fix_therm_admis(mesh, &ko))) return s;
+    }
+    // the flow's Max_dt (Solver.cpp:1076-1079); only its time step scale remains
+    auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
+    const double max_cheby = cheby(o->n_cheby_flow, o->n_cheby_flow - 1);
+    ko.use_filter = o->use_filter;
+    s = o->use_ldg ? hx_max_dt_navier_stokes(mesh, &ko, o->max_safety / max_cheby, o->max_safety, o->local_time, o->visc,
+                                             o->cond, &dummy)
+                   : hx_max_dt_euler(mesh, &ko, o->max_safety / max_cheby, o->max_safety, o->local_time, &dummy);
+    if (s) return s;
+    l = make_launch(mesh);
+    HX_TRY(ctx, mesh->ks->fix_av(l, 1), "fix_av");
+  }
+  --iter;
+  *n_iters_out = iter;
   return HX_OK;
 }
 

@@ -12,6 +12,7 @@ namespace hx {
 
 constexpr int av_kind_adv = HX_PDE_ADVECTION;
 constexpr int av_kind_sav = HX_PDE_SMOOTH_AV;
+constexpr int av_kind_fta = HX_PDE_FIX_ADMIS;  // pde::Fix_therm_admis: diffusion of the flow state (pde.hpp:437-493)
 
 // Point constants of the generator, passed by value
 template <int RS>
@@ -26,8 +27,18 @@ struct Av_consts
 template <int ND, int RS, int KIND>
 struct Av_vars
 {
-  static constexpr int N = KIND == av_kind_adv ? ND + RS : 3;  // n_extrap
+  static constexpr int N = KIND == av_kind_adv ? ND + RS : KIND == av_kind_sav ? 3 : ND + 2;  // n_extrap
 };
+
+// The variables a diffusion-only PDE works on, [N][nq] contiguous per element: forcing[1..3] for Smooth_art_visc
+// (pde.hpp:373-378), the conserved state for Fix_therm_admis (pde.hpp:448-453)
+template <int ND, int RS, int KIND>
+__device__ inline double* diff_vars(const Dev_mesh& m, int cls, size_t e)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ;
+  if constexpr (KIND == av_kind_sav) return m.forcing[cls] + (e * 4 + 1) * NQ;
+  else return m.state[cls] + e * (ND + 2) * NQ;
+}
 
 // Advection::fetch_extrap (pde.hpp:284-290) / Smooth_art_visc::fetch_extrap (pde.hpp:373-378) of variable v
 template <int ND, int RS, int KIND>
@@ -37,7 +48,7 @@ __device__ inline double av_extrap(const Dev_mesh& m, int cls, size_t e, int v, 
   if constexpr (KIND == av_kind_adv) {
     return v < ND ? m.state[cls][(e * (ND + 2) + v) * NQ + q] : m.advection[cls][(e * RS + (v - ND)) * NQ + q];
   } else {
-    return m.forcing[cls][(e * 4 + 1 + v) * NQ + q];
+    return diff_vars<ND, RS, KIND>(m, cls, e)[v * NQ + q];
   }
 }
 
@@ -57,10 +68,10 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_av_write_face(Dev_me
   write_faces_from<ND, RS, S::BLOCK, NVAR>(m, cls, o, lds, NVAR * NQP);
 }
 
-// ---- Neighbor<Advection / Smooth_art_visc> (Spatial.hpp:613-704) ----------------------------------------------
+// ---- Neighbor<Advection / Smooth_art_visc / Fix_therm_admis> (Spatial.hpp:613-704) -----------------------------
 // Advection: LLF flux of the rs advected scalars with flux node_i (v.n) a_i and characteristic speed max(1, |v|)
-// (pde.hpp:319-347), written over the first rs variables of block 0. Smooth_art_visc: diffusion only, the LDG
-// average of the 3 extrapolated variables.
+// (pde.hpp:319-347), written over the first rs variables of block 0. Smooth_art_visc and Fix_therm_admis: diffusion
+// only, the LDG average of the extrapolated variables.
 template <int ND, int RS, bool DEF, int KIND>
 __global__ __launch_bounds__(256) void k_av_neighbor(Dev_mesh m, Av_consts<RS> ac)
 {
@@ -73,7 +84,7 @@ __global__ __launch_bounds__(256) void k_av_neighbor(Dev_mesh m, Av_consts<RS> a
   double* f0 = slot_ptr(m, m.con_slot[cls][2 * con]);
   double* f1 = slot_ptr(m, m.con_slot[cls][2 * con + 1]);
   const int p1 = DEF ? face1_index<ND, RS>(dir, f) : f;
-  if constexpr (KIND == av_kind_sav) {
+  if constexpr (KIND != av_kind_adv) {
     for (int v = 0; v < NVAR; ++v) {
       const double avg = .5 * (f0[v * NFQ + f] + f1[v * NFQ + p1]);
       f0[NV * NFQ + v * NFQ + f] = avg;
@@ -214,33 +225,36 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_adv_local(Dev_mesh m
   write_faces_from<ND, RS, NT, ND + RS>(m, cls, o, lds + L::U, L::PER);
 }
 
-// ---- Local<Smooth_art_visc> (Spatial.hpp:326-509, stage 0) ----------------------------------------------------
-// LDG gradient of forcing[1..3], flux -gradient*normal (pde.hpp:405-410), the chained source
-// |f0|, sqrt|f1|, |f2| over diff_time (pde.hpp:419-427), and the additive update of write_update (pde.hpp:380-388).
-template <int ND, int RS, bool DEF>
+// ---- Local<Smooth_art_visc / Fix_therm_admis> (Spatial.hpp:326-509, stage 0) -----------------------------------
+// LDG gradient of the diffused variables, flux -gradient*normal (pde.hpp:405-410, 479-483), for Smooth_art_visc the
+// chained source |f0|, sqrt|f1|, |f2| over diff_time (pde.hpp:419-427), and the additive update of write_update
+// (pde.hpp:380-388, 455-458).
+template <int ND, int RS, bool DEF, int NS>
 struct Sav_lds
 {
   using S = Shape<ND, RS>;
-  static constexpr int NQP = S::NQP, NFQ = S::NFQ, NS = 3;
-  static constexpr int U = 0;                                   // [3][NQP] forcing[1..3]
-  static constexpr int G = U + NS * NQP;                        // [3][NQP] one gradient component
+  static constexpr int NQP = S::NQP, NFQ = S::NFQ;
+  static constexpr int U = 0;                                   // [NS][NQP] the diffused variables
+  static constexpr int G = U + NS * NQP;                        // [NS][NQP] one gradient component
   static constexpr int SN = G + NS * NQP;                       // [ND][ND][NQP] reference-level normals
-  static constexpr int SL = SN + (DEF ? ND * ND * NQP : 0);     // [2ND][3][NFQ] LDG face state
+  static constexpr int SL = SN + (DEF ? ND * ND * NQP : 0);     // [2ND][NS][NFQ] LDG face state
   static constexpr int SFN = SL + 2 * ND * NS * NFQ;            // [2ND][ND][NFQ] face normals
   static constexpr int P1 = SFN + (DEF ? 2 * ND * ND * NFQ : 0);
-  static constexpr int F = 0;                                   // [3][NQP] viscous flux along the current direction
-  static constexpr int RT = F + NS * NQP;                       // [3][NQP] rate
+  static constexpr int F = 0;                                   // [NS][NQP] viscous flux along the current direction
+  static constexpr int RT = F + NS * NQP;                       // [NS][NQP] rate
   static constexpr int P2 = RT + NS * NQP;
   static constexpr int PER = P1 > P2 ? P1 : P2;
 };
 
-template <int ND, int RS, bool DEF>
+template <int ND, int RS, bool DEF, int KIND>
 __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m, Eops<RS, double> o, double coef,
                                                                       double diff_time)
 {
   using S = Shape<ND, RS>;
-  using L = Sav_lds<ND, RS, DEF>;
-  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = S::BLOCK, NS = 3;
+  constexpr int NS = Av_vars<ND, RS, KIND>::N;
+  constexpr bool SAV = KIND == av_kind_sav;
+  using L = Sav_lds<ND, RS, DEF, NS>;
+  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = S::BLOCK;
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
   __shared__ double lds[EPB * L::PER];
   const int t = threadIdx.x, el = t / NQ, q = t % NQ, qp = S::pad(q);
@@ -250,9 +264,9 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
   double* my = lds + el * L::PER;
   const Contraction<RS, Fp64_form> con(o, nullptr);
 
-  double fc[4];
-  for (int v = 0; v < 4; ++v) fc[v] = valid ? m.forcing[cls][(size_t(e) * 4 + v) * NQ + q] : 0.;
-  for (int v = 0; v < NS; ++v) my[L::U + v * NQP + qp] = fc[1 + v];
+  double uv[NS];
+  for (int v = 0; v < NS; ++v) uv[v] = valid ? diff_vars<ND, RS, KIND>(m, cls, e)[v * NQ + q] : 0.;
+  for (int v = 0; v < NS; ++v) my[L::U + v * NQP + qp] = uv[v];
   double det = 1.;
   double met[ND][ND];
   if constexpr (DEF) {
@@ -314,11 +328,15 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
   if constexpr (DEF)
     for (int v = 0; v < NS; ++v)
       for (int j = 0; j < ND; ++j) grad[v][j] /= det;
-  double src_mult = d_pos;
-  if constexpr (DEF) src_mult *= det;
-  for (int v = 0; v < NS; ++v) {
-    const double f = fabs(fc[v]);
-    my[L::RT + v * NQP + qp] = src_mult * (((v == 1) ? sqrt(f) : f) / diff_time);
+  if constexpr (SAV) {
+    double src_mult = d_pos;
+    if constexpr (DEF) src_mult *= det;
+    for (int v = 0; v < NS; ++v) {
+      const double f = valid ? fabs(m.forcing[cls][(size_t(e) * 4 + v) * NQ + q]) : 0.;
+      my[L::RT + v * NQP + qp] = src_mult * (((v == 1) ? sqrt(f) : f) / diff_time);
+    }
+  } else {
+    for (int v = 0; v < NS; ++v) my[L::RT + v * NQP + qp] = 0.;
   }
   for (int i = 0; i < ND; ++i) {
     if (i) __syncthreads();
@@ -349,25 +367,27 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
   if (valid) {
     double mult = coef * m.tss[cls][size_t(e) * NQ + q] / d_pos;
     if constexpr (DEF) mult /= det;
+    double* out = diff_vars<ND, RS, KIND>(m, cls, e);
     for (int v = 0; v < NS; ++v) {
       double u = 0.;
       u += my[L::RT + v * NQP + qp];
       u *= mult;
-      m.forcing[cls][(size_t(e) * 4 + 1 + v) * NQ + q] = fc[1 + v] + u;
+      out[v * NQ + q] = uv[v] + u;
     }
   }
 }
 
-// ---- Reconcile_ldg_flux<Smooth_art_visc> (Spatial.hpp:543-594) ------------------------------------------------
-// -boundary_term(jump), the critical write_update (f += du; f /= 1 + tss cheby/diff_time, pde.hpp:380-388), then
-// Write_face of forcing[1..3].
-template <int ND, int RS, bool DEF>
+// ---- Reconcile_ldg_flux<Smooth_art_visc / Fix_therm_admis> (Spatial.hpp:543-594) --------------------------------
+// -boundary_term(jump), the critical write_update (Smooth_art_visc: f += du; f /= 1 + tss cheby/diff_time,
+// pde.hpp:380-388; Fix_therm_admis: u += du, pde.hpp:455-458), then Write_face of the diffused variables.
+template <int ND, int RS, bool DEF, int KIND>
 __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_reconcile(Dev_mesh m, Eops<RS, double> o, double dt,
                                                                           double diff_time, double cheby)
 {
   using S = Shape<ND, RS>;
   using R = Row<ND, RS>;
-  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NS = 3;
+  constexpr int NS = Av_vars<ND, RS, KIND>::N;
+  constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB;
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
   constexpr int PF = 2 * ND * NS * NFQ, PER = PF + NS * NQP;
   __shared__ double lds[EPB * PER];
@@ -389,16 +409,16 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_reconcile(Dev_me
       rate[v] -= o.lift[ki][0] * my[((2 * i) * NS + v) * NFQ + fq] + o.lift[ki][1] * my[((2 * i + 1) * NS + v) * NFQ + fq];
   }
   double fv[NS];
-  for (int v = 0; v < NS; ++v) fv[v] = valid ? m.forcing[cls][(size_t(e) * 4 + 1 + v) * NQ + q] : 0.;
+  for (int v = 0; v < NS; ++v) fv[v] = valid ? diff_vars<ND, RS, KIND>(m, cls, e)[v * NQ + q] : 0.;
   if (valid) {
     const double tss = m.tss[cls][size_t(e) * NQ + q];
     double mult = dt * tss / m.nom_size[cls][e];
     if constexpr (DEF) mult /= m.metrics[size_t(e) * (ND * ND + 1) * NQ + ND * ND * NQ + q];
-    const double pseudo = 1 + tss * cheby / diff_time;
+    double* out = diff_vars<ND, RS, KIND>(m, cls, e);
     for (int v = 0; v < NS; ++v) {
       fv[v] += rate[v] * mult;
-      fv[v] /= pseudo;
-      m.forcing[cls][(size_t(e) * 4 + 1 + v) * NQ + q] = fv[v];
+      if constexpr (KIND == av_kind_sav) fv[v] /= 1 + tss * cheby / diff_time;
+      out[v * NQ + q] = fv[v];
     }
   }
   for (int v = 0; v < NS; ++v) my[PF + v * NQP + S::pad(q)] = fv[v];
@@ -499,6 +519,90 @@ __global__ __launch_bounds__(256) void k_av_bcs(Dev_mesh m, int mode)
   } else {  // Flow_bc::apply_advection: velocity copied, advected scalars 2 - inside
     for (int d = 0; d < ND; ++d) gh_f[d * NFQ + f] = in_f[d * NFQ + f];
     for (int a = 0; a < RS; ++a) gh_f[(ND + a) * NFQ + f] = 2. - in_f[(ND + a) * NFQ + f];
+  }
+}
+
+// ---- Solver::is_admissible (Solver.cpp:921-959) with thermo::admissible (thermo.cpp:6-18) ----------------------
+// Density and energy must be positive at every quadrature point of the element and of its faces (state block);
+// flags[0] |= 1 when some element is not, flags[1] |= 1 when a checked value is not finite (the reference's "state is
+// not finite" assertion). Per element the reference stops at the first inadmissible array: the state, then the faces
+// in order; a face after that one is not checked for finiteness.
+template <int ND, int RS>
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_admissible(Dev_mesh m, int cls, int* flags)
+{
+  using S = Shape<ND, RS>;
+  constexpr int NQ = S::NQ, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NF = 2 * ND;
+  __shared__ int bad[EPB], nonfinite[EPB], face_bad[EPB], face_nonfinite[EPB];
+  const int t = threadIdx.x, el = t / NQ, q = t % NQ;
+  const int e = blockIdx.x * EPB + el;
+  const bool valid = e < m.n_elem[cls];
+  if (q == 0) bad[el] = nonfinite[el] = face_bad[el] = face_nonfinite[el] = 0;
+  __syncthreads();
+  if (valid) {
+    const double* st = m.state[cls] + size_t(e) * NV * NQ;
+    bool fin = true;
+    for (int v = 0; v < NV; ++v) fin = fin && isfinite(st[v * NQ + q]);
+    if (!(st[ND * NQ + q] > 0.) || !(st[(ND + 1) * NQ + q] > 0.)) atomicOr(&bad[el], 1);
+    if (!fin) atomicOr(&nonfinite[el], 1);
+    for (int k = q; k < NF * NFQ; k += NQ) {
+      const int f = k / NFQ, p = k % NFQ;
+      const double* fc = slot_ptr(m, m.face_slot[cls][e * NF + f]);
+      bool ffin = true;
+      for (int v = 0; v < NV; ++v) ffin = ffin && isfinite(fc[v * NFQ + p]);
+      if (!(fc[ND * NFQ + p] > 0.) || !(fc[(ND + 1) * NFQ + p] > 0.)) atomicOr(&face_bad[el], 1 << f);
+      if (!ffin) atomicOr(&face_nonfinite[el], 1 << f);
+    }
+  }
+  __syncthreads();
+  if (valid && q == 0) {
+    bool admis = !bad[el], fin = !nonfinite[el];
+    for (int f = 0; f < NF && admis; ++f) {
+      if ((face_nonfinite[el] >> f) & 1) fin = false;
+      if ((face_bad[el] >> f) & 1) admis = false;
+    }
+    if (!admis) atomicOr(flags, 1);
+    if (!fin) atomicOr(flags + 1, 1);
+  }
+}
+
+// The fine faces of every hanging face (Solver.cpp:946-957). The reference's short-circuit there depends on how
+// OpenMP splits the loop; here every fine face is checked for finiteness.
+template <int ND, int RS>
+__global__ __launch_bounds__(256) void k_admissible_ref(Dev_mesh m, int* flags)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2, NH = 1 << (ND - 1);
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid / (NH * NFQ), r = gid % (NH * NFQ), f = r / NFQ, p = r % NFQ;
+  if (i >= m.n_ref) return;
+  int n_fine = NH;
+  for (int d = 0; d < ND - 1; ++d)
+    if (m.ref_stretch[2 * i + d]) n_fine /= 2;
+  if (f >= n_fine) return;
+  const double* fc = slot_ptr(m, m.ref_slot[5 * i + 1 + f]);
+  bool fin = true;
+  for (int v = 0; v < NV; ++v) fin = fin && isfinite(fc[v * NFQ + p]);
+  if (!(fc[ND * NFQ + p] > 0.) || !(fc[(ND + 1) * NFQ + p] > 0.)) atomicOr(flags, 1);
+  if (!fin) atomicOr(flags + 1, 1);
+}
+
+// The artificial-viscosity swap around one iteration of Solver::fix_admissibility (Solver.cpp:1040-1047,
+// 1080-1086) with the fix coefficient taken as zero. mode 0: laplacian <- bulk, bulk <- 0; mode 1: bulk <- laplacian,
+// laplacian <- 0.
+template <int ND, int RS>
+__global__ __launch_bounds__(256) void k_fix_av(Dev_mesh m, int cls, int mode)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ;
+  const size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (gid >= size_t(m.n_elem[cls]) * NQ) return;
+  const size_t e = gid / NQ, q = gid % NQ;
+  double* bulk = m.av[cls] + e * 2 * NQ + q;
+  double* lap = bulk + NQ;
+  if (mode == 0) {
+    *lap = *bulk;
+    *bulk = 0.;
+  } else {
+    *bulk = *lap;
+    *lap = 0.;
   }
 }
 

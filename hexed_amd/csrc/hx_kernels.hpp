@@ -1667,7 +1667,8 @@ struct Launchers
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
     dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
     if (kind == av_kind_adv) hipLaunchKernelGGL((k_av_write_face<ND, RS, av_kind_adv>), g, b, 0, l.stream, l.m, o, cls);
-    else hipLaunchKernelGGL((k_av_write_face<ND, RS, av_kind_sav>), g, b, 0, l.stream, l.m, o, cls);
+    else if (kind == av_kind_sav) hipLaunchKernelGGL((k_av_write_face<ND, RS, av_kind_sav>), g, b, 0, l.stream, l.m, o, cls);
+    else hipLaunchKernelGGL((k_av_write_face<ND, RS, av_kind_fta>), g, b, 0, l.stream, l.m, o, cls);
     return hipGetLastError();
   }
   static hipError_t av_neighbor(const Launch& l, int cls, int kind)
@@ -1678,10 +1679,12 @@ struct Launchers
     const Av_consts<RS> c = av_consts(l, RS);
     if (cls == HX_DEFORMED) {
       if (kind == av_kind_adv) hipLaunchKernelGGL((k_av_neighbor<ND, RS, true, av_kind_adv>), g, b, 0, l.stream, l.m, c);
-      else hipLaunchKernelGGL((k_av_neighbor<ND, RS, true, av_kind_sav>), g, b, 0, l.stream, l.m, c);
+      else if (kind == av_kind_sav) hipLaunchKernelGGL((k_av_neighbor<ND, RS, true, av_kind_sav>), g, b, 0, l.stream, l.m, c);
+      else hipLaunchKernelGGL((k_av_neighbor<ND, RS, true, av_kind_fta>), g, b, 0, l.stream, l.m, c);
     } else {
       if (kind == av_kind_adv) hipLaunchKernelGGL((k_av_neighbor<ND, RS, false, av_kind_adv>), g, b, 0, l.stream, l.m, c);
-      else hipLaunchKernelGGL((k_av_neighbor<ND, RS, false, av_kind_sav>), g, b, 0, l.stream, l.m, c);
+      else if (kind == av_kind_sav) hipLaunchKernelGGL((k_av_neighbor<ND, RS, false, av_kind_sav>), g, b, 0, l.stream, l.m, c);
+      else hipLaunchKernelGGL((k_av_neighbor<ND, RS, false, av_kind_fta>), g, b, 0, l.stream, l.m, c);
     }
     return hipGetLastError();
   }
@@ -1697,34 +1700,70 @@ struct Launchers
     else hipLaunchKernelGGL((k_adv_local<ND, RS, false>), g, b, 0, l.stream, l.m, o, c, coef, stage, cr, advect_length);
     return hipGetLastError();
   }
-  static hipError_t sav_local(const Launch& l, int cls, double dt, double diff_time)
+  template <int KIND>
+  static void diff_local_as(const Launch& l, int cls, double dt, double diff_time)
   {
-    const int n = l.m.n_elem[cls];
-    if (n == 0) return hipSuccess;
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
-    dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
-    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_sav_local<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, diff_time);
-    else hipLaunchKernelGGL((k_sav_local<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, diff_time);
+    dim3 g(blocks_for(l.m.n_elem[cls], S::EPB)), b(S::BLOCK);
+    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_sav_local<ND, RS, true, KIND>), g, b, 0, l.stream, l.m, o, dt, diff_time);
+    else hipLaunchKernelGGL((k_sav_local<ND, RS, false, KIND>), g, b, 0, l.stream, l.m, o, dt, diff_time);
+  }
+  // Local of a diffusion-only PDE: Smooth_art_visc or Fix_therm_admis (kind)
+  static hipError_t diff_local(const Launch& l, int cls, int kind, double dt, double diff_time)
+  {
+    if (l.m.n_elem[cls] == 0) return hipSuccess;
+    if (kind == av_kind_sav) diff_local_as<av_kind_sav>(l, cls, dt, diff_time);
+    else diff_local_as<av_kind_fta>(l, cls, dt, diff_time);
     return hipGetLastError();
   }
-  static hipError_t sav_neighbor_reconcile(const Launch& l, int cls)
+  static hipError_t diff_neighbor_reconcile(const Launch& l, int cls, int kind)
   {
     const int n = l.m.n_con[cls];
     if (n == 0) return hipSuccess;
     dim3 g(blocks_for(size_t(n) * S::NFQ, 256)), b(256);
-    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_neighbor_reconcile<ND, RS, true, 3>), g, b, 0, l.stream, l.m);
-    else hipLaunchKernelGGL((k_neighbor_reconcile<ND, RS, false, 3>), g, b, 0, l.stream, l.m);
+    if (kind == av_kind_sav) {
+      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_neighbor_reconcile<ND, RS, true, 3>), g, b, 0, l.stream, l.m);
+      else hipLaunchKernelGGL((k_neighbor_reconcile<ND, RS, false, 3>), g, b, 0, l.stream, l.m);
+    } else {
+      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_neighbor_reconcile<ND, RS, true>), g, b, 0, l.stream, l.m);
+      else hipLaunchKernelGGL((k_neighbor_reconcile<ND, RS, false>), g, b, 0, l.stream, l.m);
+    }
     return hipGetLastError();
   }
-  static hipError_t sav_reconcile(const Launch& l, int cls, double dt, double diff_time, double cheby)
+  template <int KIND>
+  static void diff_reconcile_as(const Launch& l, int cls, double dt, double diff_time, double cheby)
   {
-    const int n = l.m.n_elem[cls];
-    if (n == 0) return hipSuccess;
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
-    dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
+    dim3 g(blocks_for(l.m.n_elem[cls], S::EPB)), b(S::BLOCK);
     if (cls == HX_DEFORMED)
-      hipLaunchKernelGGL((k_sav_reconcile<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, diff_time, cheby);
-    else hipLaunchKernelGGL((k_sav_reconcile<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, diff_time, cheby);
+      hipLaunchKernelGGL((k_sav_reconcile<ND, RS, true, KIND>), g, b, 0, l.stream, l.m, o, dt, diff_time, cheby);
+    else hipLaunchKernelGGL((k_sav_reconcile<ND, RS, false, KIND>), g, b, 0, l.stream, l.m, o, dt, diff_time, cheby);
+  }
+  static hipError_t diff_reconcile(const Launch& l, int cls, int kind, double dt, double diff_time, double cheby)
+  {
+    if (l.m.n_elem[cls] == 0) return hipSuccess;
+    if (kind == av_kind_sav) diff_reconcile_as<av_kind_sav>(l, cls, dt, diff_time, cheby);
+    else diff_reconcile_as<av_kind_fta>(l, cls, dt, diff_time, cheby);
+    return hipGetLastError();
+  }
+  // Solver::is_admissible over both classes and the hanging faces; flags[2] must be zero beforehand
+  static hipError_t admissible(const Launch& l, int* flags)
+  {
+    for (int cls = 0; cls < 2; ++cls)
+      if (l.m.n_elem[cls])
+        hipLaunchKernelGGL((k_admissible<ND, RS>), dim3(blocks_for(l.m.n_elem[cls], S::EPB)), dim3(S::BLOCK), 0,
+                           l.stream, l.m, cls, flags);
+    if (l.m.n_ref)
+      hipLaunchKernelGGL((k_admissible_ref<ND, RS>), dim3(blocks_for(size_t(l.m.n_ref) * (1 << (ND - 1)) * S::NFQ, 256)),
+                         dim3(256), 0, l.stream, l.m, flags);
+    return hipGetLastError();
+  }
+  static hipError_t fix_av(const Launch& l, int mode)
+  {
+    for (int cls = 0; cls < 2; ++cls)
+      if (l.m.n_elem[cls])
+        hipLaunchKernelGGL((k_fix_av<ND, RS>), dim3(blocks_for(size_t(l.m.n_elem[cls]) * S::NQ, 256)), dim3(256), 0,
+                           l.stream, l.m, cls, mode);
     return hipGetLastError();
   }
   static hipError_t av_max_dt(const Launch& l, int cls, int smooth, int is_local, double cc, double cd)
@@ -1757,8 +1796,9 @@ struct Launchers
   {
     static const Kernel_set s{&write_face,  &neighbor,      &local,     &neighbor_reconcile, &reconcile_ldg_flux,
                               &max_dt,      &prolong,       &restrict_, &state_bcs,          &flux_bcs,
-                              &av_write_face, &av_neighbor, &adv_local, &sav_local,          &sav_neighbor_reconcile,
-                              &sav_reconcile, &av_max_dt,   &av_bcs,    &av_points,          &max_dt_cases};
+                              &av_write_face, &av_neighbor, &adv_local, &diff_local,         &diff_neighbor_reconcile,
+                              &diff_reconcile, &av_max_dt,  &av_bcs,    &av_points,          &max_dt_cases,
+                              &admissible,    &fix_av};
     return &s;
   }
 };

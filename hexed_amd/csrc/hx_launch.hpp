@@ -33,19 +33,24 @@ struct Kernel_set
   hipError_t (*restrict_)(const Launch&, int n_var, int scale, int offset);
   hipError_t (*state_bcs)(const Launch&);
   hipError_t (*flux_bcs)(const Launch&);
-  // artificial-viscosity generator (kind: HX_PDE_ADVECTION or HX_PDE_SMOOTH_AV)
+  // artificial-viscosity generator and the admissibility fix (kind: HX_PDE_ADVECTION, HX_PDE_SMOOTH_AV or
+  // HX_PDE_FIX_ADMIS; the diff_* entries take the two diffusion-only kinds)
   hipError_t (*av_write_face)(const Launch&, int cls, int kind);
   hipError_t (*av_neighbor)(const Launch&, int cls, int kind);
   hipError_t (*adv_local)(const Launch&, int cls, double coef, int stage, int compute_residual, double advect_length);
-  hipError_t (*sav_local)(const Launch&, int cls, double dt, double diff_time);
-  hipError_t (*sav_neighbor_reconcile)(const Launch&, int cls);
-  hipError_t (*sav_reconcile)(const Launch&, int cls, double dt, double diff_time, double cheby);
+  hipError_t (*diff_local)(const Launch&, int cls, int kind, double dt, double diff_time);
+  hipError_t (*diff_neighbor_reconcile)(const Launch&, int cls, int kind);
+  hipError_t (*diff_reconcile)(const Launch&, int cls, int kind, double dt, double diff_time, double cheby);
   hipError_t (*av_max_dt)(const Launch&, int cls, int smooth, int is_local, double max_cfl_c, double max_cfl_d);
   hipError_t (*av_bcs)(const Launch&, int mode);
   hipError_t (*av_points)(const Launch&, int cls, int what, int av_rs, double mult, double us_max, double* resid);
   // batch of independent cases: global-time-step Max_dt with a minimum per case
   hipError_t (*max_dt_cases)(const Launch&, int cls, bool visc, double max_cfl_c, double max_cfl_d, const int* case_id,
                              unsigned long long* case_min);
+  // Solver::is_admissible (flags[0]: some state inadmissible, flags[1]: some value not finite; zero them first) and
+  // the artificial-viscosity swap of Solver::fix_admissibility
+  hipError_t (*admissible)(const Launch&, int* flags);
+  hipError_t (*fix_av)(const Launch&, int mode);
 };
 
 const Kernel_set* kernel_set_nd1(int rs);

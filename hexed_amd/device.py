@@ -200,6 +200,30 @@ class Device_mesh:
     def apply_avc_diff_bcs(self):
         self.ctx.check(self.lib.hx_apply_avc_diff_bcs(self.ptr), "avc diff bcs")
 
+    # thermodynamic admissibility ------------------------------------------------------------------------------
+    def is_admissible(self):
+        out = ctypes.c_int()
+        self.ctx.check(self.lib.hx_is_admissible(self.ptr, ctypes.byref(out)), "is_admissible")
+        return bool(out.value)
+
+    def fix_admissibility(self, opts, stability_ratio=0.7):
+        """Solver::fix_admissibility with the flow settings `opts`; returns its iteration count."""
+        out = ctypes.c_int()
+        self.ctx.check(self.lib.hx_fix_admissibility(self.ptr, ctypes.byref(opts), stability_ratio, ctypes.byref(out)),
+                       "fix_admissibility")
+        return out.value
+
+    def compute_fix_therm_admis(self, dt):
+        o = abi.hx_kernel_options(dt, 0, 0, 0)
+        self.ctx.check(self.lib.hx_compute_fix_therm_admis(self.ptr, ctypes.byref(o)), "compute_fix_therm_admis")
+
+    def max_dt_fix_therm_admis(self, msc, msd, local_time):
+        out = ctypes.c_double()
+        o = abi.hx_kernel_options(0., 0, 0, 0)
+        self.ctx.check(self.lib.hx_max_dt_fix_therm_admis(self.ptr, ctypes.byref(o), msc, msd, int(local_time),
+                                                          ctypes.byref(out)), "max_dt_fix_therm_admis")
+        return out.value
+
     def update(self, opts, status=None):
         status = status if status is not None else abi.hx_update_status()
         self.ctx.check(self.lib.hx_update(self.ptr, ctypes.byref(opts), ctypes.byref(status)), "update")

@@ -97,6 +97,12 @@ class HostMesh:
         self.bc_cache = None  # [n_bc_con][nv][nfq], allocated by struct()
         # geometry kept for initial conditions: qpoint positions per class [n][nd][nq]
         self.qpos = [None, None]
+        # vertex connectivity (not part of hx_host_mesh; the oracle's fix_admissibility needs it): the shared vertex of
+        # every element vertex per class [n][nvert], and per hanging face [12]: i_dim, is_positive, stretch[2], then
+        # (class, element) of the fine elements in order (Hanging_vertex_matcher.hpp:25)
+        self.n_vertex = 0
+        self.vertex_ids = [np.zeros((n, self.nvert), np.int32) for n in self.n_elem]
+        self.hanging = np.zeros((0, 12), np.int32)
 
     def enable_av_generator(self):
         """Allocate the arrays Solver::update_art_visc_smoothness works on (zeros, as Element.cpp:134-142 does)."""
@@ -368,6 +374,8 @@ def box(nd, rs, n, lo=0.0, hi=1.0, deformed=False, jitter=0.0, seed=0, bc=(abi.H
     vmin = np.full((n + 1) ** nd, np.inf)
     np.minimum.at(vmin, vid.reshape(-1), elem_vtss.reshape(-1))
     m.vertex_tss[cls][:] = vmin[vid]
+    m.n_vertex = (n + 1) ** nd
+    m.vertex_ids[cls][:] = vid
     return m
 
 
@@ -406,8 +414,8 @@ def concat(meshes):
         a[a >= 0] += off
         return a
     e_off, c_off = [0, 0], [0, 0]
-    s_off = ns_off = r_off = b_off = 0
-    bc_con, bc_serial, qpos = [], [], [[], []]
+    s_off = ns_off = r_off = b_off = v_off = 0
+    bc_con, bc_serial, qpos, hanging = [], [], [[], []], []
     for k, m in enumerate(meshes):
         for c in range(2):
             n, lo = m.n_elem[c], e_off[c]
@@ -419,6 +427,7 @@ def concat(meshes):
                 out.av[c][sl] = m.av[c]
             out.nom_size[c][sl] = m.nom_size[c]
             out.vertex_tss[c][sl] = m.vertex_tss[c]
+            out.vertex_ids[c][sl] = m.vertex_ids[c] + v_off
             out.face_slot[c][sl] = m.face_slot[c] + s_off
             out.case_id[c][sl] = k
             if n:
@@ -438,6 +447,11 @@ def concat(meshes):
         bc_con.append(m.bc_con + c_off[1])
         bc_serial.append(m.bc_serial + b_off)
         out.bcs += list(m.bcs)
+        h = m.hanging.copy()
+        for f in range(4):
+            h[:, 5 + 2 * f] += np.where(h[:, 4 + 2 * f] == 1, e_off[1], e_off[0])
+        hanging.append(h)
+        v_off += m.n_vertex
         for c in range(2):
             e_off[c] += m.n_elem[c]
             c_off[c] += len(m.con_dir[c])
@@ -448,6 +462,8 @@ def concat(meshes):
     out.bc_con = np.concatenate(bc_con).astype(np.int32)
     out.bc_serial = np.concatenate(bc_serial).astype(np.int32)
     out.qpos = [np.concatenate(q) if q else None for q in qpos]
+    out.n_vertex = v_off
+    out.hanging = np.concatenate(hanging).astype(np.int32)
     return out
 
 
@@ -687,4 +703,17 @@ class Builder:
             m.vertex_tss[1][j] = vmin[vid[i]]
         self.vertex_positions = vpos
         self.vertex_ids = vid
+        m.n_vertex = len(uniq)
+        for j, i in enumerate(car_ids):
+            m.vertex_ids[0][j] = vid[i]
+        for j, i in enumerate(def_ids):
+            m.vertex_ids[1][j] = vid[i]
+        # one hanging-vertex matcher per hanging face: the fine elements' face is on the side face_sign[!rev] of the
+        # Cartesian direction {{d, d}, {1, 0}} (connection.hpp:35, 231)
+        hang = np.zeros((len(self.ref_cons), 12), np.int32)
+        for k, (c, fine, d, rev) in enumerate(self.ref_cons):
+            hang[k, 0], hang[k, 1] = d, int(bool(rev))
+            for f, e in enumerate(fine):
+                hang[k, 4 + 2 * f], hang[k, 5 + 2 * f] = cls_of[e], idx_of[e]
+        m.hanging = hang
         return m

@@ -197,6 +197,8 @@ typedef struct hx_update_options {
   int use_ldg;            /* Solver::use_ldg(): viscous transport or artificial viscosity (Solver.cpp:117-120) */
   hx_transport visc;
   hx_transport cond;
+  int fix_admis;               /* Solver::set_fix_admissibility, default 0 (Solver.cpp:143, 667-670) */
+  double fix_admis_max_safety; /* stability ratio of the fix, default 0.7 (Solver.cpp:153)              */
 } hx_update_options;
 
 /* Settings of Solver::update_art_visc_smoothness (Solver.cpp:457-582); the reference's defaults are at
@@ -221,6 +223,7 @@ typedef struct hx_update_status {
   double time_step;
   double flow_time;
   int iteration;
+  int fix_admis_iters;  /* iterations of fix_admissibility (Iteration_status::fix_admis_iters, Solver.cpp:1090) */
 } hx_update_status;
 
 typedef struct hx_ctx hx_ctx;
@@ -328,10 +331,30 @@ hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset);
 
 /* ---- time step -------------------------------------------------------------------------------------- */
 /* Solver::update (Solver.cpp:834-886): flow_iters x n_cheby_flow x [dt <- max_dt; state BCs + stage 0;
- * state BCs + stage 1]. `status` is advanced in place. */
+ * state BCs + stage 1], each stage followed by fix_admissibility when opts->fix_admis is set (a fix ends the Chebyshev
+ * sequence without advancing the time, Solver.cpp:867-870). `status` is advanced in place. */
 hx_status hx_update(hx_mesh* mesh, const hx_update_options* opts, hx_update_status* status);
 /* Solver::compute_residual (Solver.cpp:899-913): writes the scaled residual into the residual cache. */
 hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* opts);
+
+/* ---- thermodynamic admissibility (Solver.cpp:921-1095; HX_PDE_FIX_ADMIS also runs through the per-kernel entry
+ * points above, without compute_residual and the filter) --------------------------------------------------------- */
+/* Solver::is_admissible (Solver.cpp:921-959): density and energy positive at every quadrature point of every element,
+ * its faces and the fine faces of hanging faces (thermo::admissible, thermo.cpp:6-18). HX_ERR_NOT_FINITE ("state is
+ * not finite") when a checked value is not finite. */
+hx_status hx_is_admissible(hx_mesh* mesh, int* admissible);
+/* Solver::fix_admissibility (Solver.cpp:961-1095) with the flow settings `opts` (its closing Max_dt): while states are
+ * inadmissible, and as many iterations again once they are not, two local-time-step diffusion steps of
+ * pde::Fix_therm_admis. *n_iters is the reference's return value (0: admissible at once). Afterwards the Laplacian
+ * artificial viscosity is zero (the fix coefficient of the last iteration); the mesh's artificial-viscosity field is
+ * allocated on the device if it was NULL. */
+hx_status hx_fix_admissibility(hx_mesh* mesh, const hx_update_options* opts, double stability_ratio, int* n_iters);
+/* compute_fix_therm_admis (kernels_diffusive.cpp:32); the flux_bc callback is the fix's own (ghost LDG = -inside,
+ * Solver::apply_fta_flux_bcs, Solver.cpp:103-115). */
+hx_status hx_compute_fix_therm_admis(hx_mesh* mesh, const hx_kernel_options* opts);
+/* max_dt_fix_therm_admis (kernels_max_dt.cpp:21). */
+hx_status hx_max_dt_fix_therm_admis(hx_mesh* mesh, const hx_kernel_options* opts, double msc, double msd,
+                                    int local_time, double* dt_out);
 
 /* ---- batch of independent cases ------------------------------------------------------------------------------
  * No counterpart in the reference, which runs one Solver per case: many small cases of one (n_dim, row_size) share

@@ -52,6 +52,12 @@ def load():
         "oracle_compute_advection": (I, [M, KO, D]),
         "oracle_compute_smooth_av": (I, [M, KO, D, D]),
         "oracle_update_art_visc_smoothness": (I, [M, ctypes.POINTER(abi.hx_av_options), ctypes.POINTER(D)]),
+        "oracle_set_vertices": (I, [M, I, ctypes.POINTER(I), ctypes.POINTER(I), I, ctypes.POINTER(I)]),
+        "oracle_is_admissible": (I, [M, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "oracle_fix_coefficient": (I, [M, ctypes.POINTER(I), ctypes.POINTER(D)]),
+        "oracle_compute_fix_therm_admis": (I, [M, KO]),
+        "oracle_max_dt_fix_therm_admis": (I, [M, KO, D, D, I, ctypes.POINTER(D)]),
+        "oracle_fix_admissibility": (I, [M, ctypes.POINTER(abi.hx_update_options), D, ctypes.POINTER(I)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -15,10 +15,14 @@
  *   src/kernels_convective.cpp, src/kernels_diffusive.cpp, src/kernels_max_dt.cpp   kernel sequences
  *   src/Solver.cpp:56-81,117-133,834-913   BC application, max_dt, update, compute_residual
  *   src/Solver.cpp:83-101,428-582          AV boundary conditions, diffuse_art_visc, update_art_visc_smoothness
- *   src/Boundary_condition.cpp   Freestream, Copy, Nonpenetration, Outflow
+ *   src/Boundary_condition.cpp   Freestream, Copy, Nonpenetration, Outflow, Riemann_invariants, Pressure_outflow,
+ *                                No_slip with its thermal conditions; include/pde.hpp:182-260 Characteristics
+ *   include/pde.hpp:437-493      Fix_therm_admis; src/thermo.cpp:6-18 admissible
+ *   src/Solver.cpp:40-54,921-1095   vertex sharing, is_admissible, fix_admissibility; src/Hanging_vertex_matcher.cpp,
+ *                                src/math.cpp:19-56 (hypercube_matvec), include/math.hpp:188-199 (stretched_ind)
  *
  * What pins each part (closed forms, invariants, the reference's own tests restated) is in the tests under tests/:
- * test_oracle.py, test_basis.py, test_tree.py, test_av.py, test_batch.py.
+ * test_oracle.py, test_basis.py, test_tree.py, test_av.py, test_batch.py, test_bcs.py, test_admissibility.py.
  */
 #include "hexed_oracle.h"
 
@@ -1039,7 +1043,9 @@ struct Av
   }
 
   // Neighbor<Smooth_art_visc>: diffusion only, the LDG average of the 3 extrapolated variables
-  static void sav_neighbor(hx_host_mesh* m, int cls)
+  // Neighbor of a diffusion-only PDE (Smooth_art_visc: nvar 3; Fix_therm_admis: nvar nv): the LDG average only
+  template <int nvar>
+  static void diff_neighbor(hx_host_mesh* m, int cls)
   {
     const bool is_deformed = cls == HX_DEFORMED;
     Mesh_view mv{m};
@@ -1047,24 +1053,25 @@ struct Av
     for (int i_con = 0; i_con < m->n_con[cls]; ++i_con) {
       Dir dir = unpack_dir(m->con_dir[cls][i_con]);
       double* side[2] = {mv.slot(m->con_slot[cls][2 * i_con]), mv.slot(m->con_slot[cls][2 * i_con + 1])};
-      double face[4][n_sav * nfq];
+      double face[4][nvar * nfq];
       for (int s = 0; s < 2; ++s)
-        for (int i = 0; i < n_sav * nfq; ++i) face[s][i] = side[s][i];
-      Face_permutation<nd, rs> perm{dir, face[1], n_sav};
+        for (int i = 0; i < nvar * nfq; ++i) face[s][i] = side[s][i];
+      Face_permutation<nd, rs> perm{dir, face[1], nvar};
       if (is_deformed) perm.match_faces();
       for (int q = 0; q < nfq; ++q)
-        for (int v = 0; v < n_sav; ++v) {
+        for (int v = 0; v < nvar; ++v) {
           const double avg = .5 * (face[0][v * nfq + q] + face[1][v * nfq + q]);
           for (int s = 0; s < 2; ++s) face[2 + s][v * nfq + q] = avg;
         }
       if (is_deformed) {
         perm.restore();
-        Face_permutation<nd, rs>{dir, face[3], n_sav}.restore();
+        Face_permutation<nd, rs>{dir, face[3], nvar}.restore();
       }
       for (int s = 0; s < 2; ++s)
-        for (int i = 0; i < n_sav * nfq; ++i) side[s][nv * nfq + i] = face[2 + s][i];
+        for (int i = 0; i < nvar * nfq; ++i) side[s][nv * nfq + i] = face[2 + s][i];
     }
   }
+  static void sav_neighbor(hx_host_mesh* m, int cls) { diff_neighbor<n_sav>(m, cls); }
 
   // Neighbor_reconcile<Smooth_art_visc> (Spatial.hpp:716-759) on the 3 LDG variables
   static void sav_neighbor_reconcile(hx_host_mesh* m, int cls)
@@ -1178,7 +1185,10 @@ struct Av
 
   // Local<Smooth_art_visc> (Spatial.hpp:326-509, stage 0): LDG gradient of forcing[1..3], flux -gradient*normal
   // (pde.hpp:405-410), the chained source of pde.hpp:419-427, and an additive update (write_update, pde.hpp:380-388)
-  static void sav_local(hx_host_mesh* m, int cls, double dt, double diff_time)
+  // Local (Spatial.hpp:326-509, stage 0) of a diffusion-only PDE on nvar contiguous variables: Smooth_art_visc
+  // (forcing[1..3], with its source, pde.hpp:355-431) or Fix_therm_admis (the conserved state, pde.hpp:437-493)
+  template <int n_sav, bool sav>
+  static void diff_local(hx_host_mesh* m, int cls, double dt, double diff_time)
   {
     const bool is_deformed = cls == HX_DEFORMED;
     Basis basis(rs);
@@ -1190,7 +1200,8 @@ struct Av
         for (int q = 0; q < nfq; ++q) cartesian_normal[i][j][q] = i == j;
     #pragma omp parallel for
     for (int e = 0; e < m->n_elem[cls]; ++e) {
-      double* forcing = m->forcing[cls] + size_t(e) * 4 * nq;
+      double* forcing = sav ? m->forcing[cls] + size_t(e) * 4 * nq : nullptr;
+      double* vars = sav ? forcing + nq : m->state[cls] + size_t(e) * nv * nq;
       double* faces[6];
       double* visc_faces[6];
       element_faces(m, mv, cls, e, faces);
@@ -1208,7 +1219,7 @@ struct Av
       double time_rate[2][n_sav][nq] = {};
       double visc_storage[nd][n_sav][nq] = {};
       for (int q = 0; q < nq; ++q)
-        for (int v = 0; v < n_sav; ++v) time_rate[0][v][q] = forcing[(1 + v) * nq + q];
+        for (int v = 0; v < n_sav; ++v) time_rate[0][v][q] = vars[v * nq + q];
       for (int i_dim = 0; i_dim < nd; ++i_dim)
         for (int i_fq = 0; i_fq < nfq; ++i_fq) {
           double row_r[rs][n_sav], face_state[2][n_sav];
@@ -1250,11 +1261,13 @@ struct Av
               acc += -grad[v][j] * (is_deformed ? nrml[(i_dim * nd + j) * nq + q] : double(j == i_dim));
             visc_storage[i_dim][v][q] = acc;
           }
-        double mult = d_pos;
-        if (is_deformed) mult *= elem_det[q];
-        for (int v = 0; v < n_sav; ++v) {
-          const double f = std::abs(forcing[v * nq + q]);
-          time_rate[1][v][q] = mult * (((v == 1) ? std::sqrt(f) : f) / diff_time);
+        if (sav) {  // Smooth_art_visc::compute_source (pde.hpp:419-427)
+          double mult = d_pos;
+          if (is_deformed) mult *= elem_det[q];
+          for (int v = 0; v < n_sav; ++v) {
+            const double f = std::abs(forcing[v * nq + q]);
+            time_rate[1][v][q] = mult * (((v == 1) ? std::sqrt(f) : f) / diff_time);
+          }
         }
       }
       for (int i_dim = 0; i_dim < nd; ++i_dim)
@@ -1281,15 +1294,19 @@ struct Av
           double u = time_rate[0][v][q];
           u += time_rate[1][v][q];
           u *= mult;
-          forcing[(1 + v) * nq + q] += u;
+          vars[v * nq + q] += u;
         }
       }
     }
   }
+  static void sav_local(hx_host_mesh* m, int cls, double dt, double diff_time) { diff_local<n_sav, true>(m, cls, dt, diff_time); }
+  static void fta_local(hx_host_mesh* m, int cls, double dt) { diff_local<nv, false>(m, cls, dt, 1.); }
 
-  // Reconcile_ldg_flux<Smooth_art_visc> (Spatial.hpp:543-594): lift the flux jump, update with the pseudo-time
+  // Reconcile_ldg_flux<Smooth_art_visc / Fix_therm_admis> (Spatial.hpp:543-594): lift the flux jump, update (for
+  // Smooth_art_visc additionally divided by the pseudo-time factor), extrapolate to the faces. Smooth_art_visc:
   // division (critical write_update), extrapolate forcing[1..3] to the faces
-  static void sav_reconcile(hx_host_mesh* m, int cls, double dt, double diff_time, double cheby)
+  template <int n_sav, bool sav>
+  static void diff_reconcile(hx_host_mesh* m, int cls, double dt, double diff_time, double cheby)
   {
     const bool is_deformed = cls == HX_DEFORMED;
     Basis basis(rs);
@@ -1297,7 +1314,7 @@ struct Av
     Mesh_view mv{m};
     #pragma omp parallel for
     for (int e = 0; e < m->n_elem[cls]; ++e) {
-      double* forcing = m->forcing[cls] + size_t(e) * 4 * nq;
+      double* vars = sav ? m->forcing[cls] + size_t(e) * 4 * nq + nq : m->state[cls] + size_t(e) * nv * nq;
       double* faces[6];
       double* visc_faces[6];
       element_faces(m, mv, cls, e, faces);
@@ -1318,18 +1335,23 @@ struct Av
       for (int q = 0; q < nq; ++q) {
         double mult = dt * tss[q] / d_pos;
         if (is_deformed) mult /= elem_det[q];
-        const double pseudo = 1 + tss[q] * cheby / diff_time;
         for (int v = 0; v < n_sav; ++v) {
-          double& f = forcing[(1 + v) * nq + q];
+          double& f = vars[v * nq + q];
           f += time_rate[v][q] * mult;
-          f /= pseudo;
+          if (sav) f /= 1 + tss[q] * cheby / diff_time;  // the critical write_update (pde.hpp:380-388)
         }
       }
       double ext[n_sav][nq];
-      sav_gather(m, cls, e, ext);
+      for (int v = 0; v < n_sav; ++v)
+        for (int q = 0; q < nq; ++q) ext[v][q] = vars[v * nq + q];
       write_face_vars<n_sav>(basis, ext, faces);
     }
   }
+  static void sav_reconcile(hx_host_mesh* m, int cls, double dt, double diff_time, double cheby)
+  {
+    diff_reconcile<n_sav, true>(m, cls, dt, diff_time, cheby);
+  }
+  static void fta_reconcile(hx_host_mesh* m, int cls, double dt) { diff_reconcile<nv, false>(m, cls, dt, 1., 0.); }
 
   // Max_dt (Spatial.hpp:784-828) for Advection (char_speed max(1, |v|), pde.hpp:341-345) or Smooth_art_visc
   // (diffusivity 1, pde.hpp:412-416)
@@ -1526,6 +1548,7 @@ int n_extrap(const hx_host_mesh* m, const hx_pde* pde)
   if (kind == HX_PDE_EULER || kind == HX_PDE_NAVIER_STOKES) return m->n_dim + 2;
   if (kind == HX_PDE_ADVECTION) return m->n_dim + m->row_size;
   if (kind == HX_PDE_SMOOTH_AV) return 3;
+  if (kind == HX_PDE_FIX_ADMIS) return m->n_dim + 2;
   throw std::runtime_error("oracle: PDE kind not restated");
 }
 
@@ -1535,7 +1558,8 @@ struct Run_av_write_face
   static int run(hx_host_mesh* m, int cls, int kind)
   {
     if (kind == HX_PDE_ADVECTION) Av<nd, rs>::adv_write_face(m, cls);
-    else Av<nd, rs>::sav_write_face(m, cls);
+    else if (kind == HX_PDE_SMOOTH_AV) Av<nd, rs>::sav_write_face(m, cls);
+    else Ops<nd, rs>::write_face(m, cls);  // Write_face<Fix_therm_admis>: the nv state variables
     return 0;
   }
 };
@@ -1545,7 +1569,8 @@ struct Run_av_neighbor
   static int run(hx_host_mesh* m, int cls, int kind)
   {
     if (kind == HX_PDE_ADVECTION) Av<nd, rs>::adv_neighbor(m, cls);
-    else Av<nd, rs>::sav_neighbor(m, cls);
+    else if (kind == HX_PDE_SMOOTH_AV) Av<nd, rs>::sav_neighbor(m, cls);
+    else Av<nd, rs>::template diff_neighbor<nd + 2>(m, cls);
     return 0;
   }
 };
@@ -1574,6 +1599,10 @@ struct Run_sav_reconcile
     return 0;
   }
 };
+template <int nd, int rs>
+struct Run_fta_local { static int run(hx_host_mesh* m, int cls, double dt) { Av<nd, rs>::fta_local(m, cls, dt); return 0; } };
+template <int nd, int rs>
+struct Run_fta_reconcile { static int run(hx_host_mesh* m, int cls, double dt) { Av<nd, rs>::fta_reconcile(m, cls, dt); return 0; } };
 template <int nd, int rs>
 struct Run_av_max_dt
 {
@@ -1947,6 +1976,208 @@ hx_pde ns_pde(hx_transport visc, hx_transport cond)
   return p;
 }
 
+// ---- thermodynamic admissibility (Solver.cpp:921-1095) -----------------------------------------------------------
+
+// thermo::admissible (thermo.cpp:6-18): density and energy positive at every point; a value that is not finite throws
+bool admissible(const double* data, int nd, int n_qpoint)
+{
+  bool admiss = true;
+  for (int q = 0; q < n_qpoint; ++q) {
+    admiss = admiss && (data[nd * n_qpoint + q] > 0.) && (data[(nd + 1) * n_qpoint + q] > 0.);
+    for (int v = 0; v < nd + 2; ++v)
+      if (!std::isfinite(data[v * n_qpoint + q])) throw std::runtime_error("state is not finite");
+  }
+  return admiss;
+}
+
+// Solver::is_admissible (Solver.cpp:921-959); record[E] over the elements of both classes, Cartesian first. Per element
+// the faces are checked in order until one is inadmissible, as the reference's && does. Every fine face of the hanging
+// faces is checked (the reference's short-circuit over them depends on how OpenMP splits that loop).
+bool is_admissible(hx_host_mesh* m, std::vector<int>& record)
+{
+  const int nd = m->n_dim, nq = ipow(m->row_size, nd), nfq = nq / m->row_size;
+  Mesh_view mv{m};
+  record.assign(size_t(m->n_elem[0]) + m->n_elem[1], 0);
+  bool admiss = true;
+  for (int c = 0; c < 2; ++c)
+    for (int e = 0; e < m->n_elem[c]; ++e) {
+      bool elem_admis = admissible(m->state[c] + size_t(e) * (nd + 2) * nq, nd, nq);
+      for (int f = 0; f < 2 * nd; ++f)
+        elem_admis = elem_admis && admissible(mv.slot(m->face_slot[c][e * 2 * nd + f]), nd, nfq);
+      if (!elem_admis) record[(c ? m->n_elem[0] : 0) + e] = 1;
+      admiss = admiss && elem_admis;
+    }
+  bool refined_admiss = true;
+  for (int i = 0; i < m->n_ref; ++i) {
+    int n_fine = 1 << (nd - 1);
+    for (int d = 0; d < nd - 1; ++d)
+      if (m->ref_stretch[2 * i + d]) n_fine /= 2;
+    for (int f = 0; f < n_fine; ++f) {
+      const bool a = admissible(mv.slot(m->ref_slot[5 * i + 1 + f]), nd, nfq);
+      refined_admiss = refined_admiss && a;
+    }
+  }
+  return admiss && refined_admiss;
+}
+
+// Vertex connectivity, which hx_host_mesh does not carry: the shared vertex of every element vertex and the
+// hanging-vertex matchers of the hanging faces (set by oracle_set_vertices)
+struct Vertex_data
+{
+  bool set = false;
+  int n_elem[2] = {0, 0};
+  int n_vertex = 0;
+  std::vector<int> vid;      // [n_elem[0] + n_elem[1]][nvert]
+  std::vector<int> hanging;  // [n][12]: i_dim, is_positive, stretch[2], then (class, element) of up to 4 fine elements
+};
+Vertex_data g_vertices;
+
+// math::hypercube_matvec (math.cpp:19-56): `mat` (rows x cols, row-major) applied along every dimension of a hypercubic
+// array, the last dimension first
+std::vector<double> hypercube_matvec(const std::vector<double>& mat, int rows, int cols, const std::vector<double>& vec)
+{
+  const int vec_row_size = int(vec.size()) / cols;
+  if (vec_row_size == 1) {
+    std::vector<double> out(rows);
+    for (int r = 0; r < rows; ++r) {
+      double acc = 0.;
+      for (int c = 0; c < cols; ++c) acc += mat[r * cols + c] * vec[c];
+      out[r] = acc;
+    }
+    return out;
+  }
+  int prod_size = 1;
+  for (int vs = int(vec.size()); vs > 1;) {
+    prod_size *= rows;
+    vs /= cols;
+  }
+  const int row_size = prod_size / rows;
+  std::vector<double> fact(size_t(cols) * row_size);
+  for (int i_row = 0; i_row < cols; ++i_row) {
+    std::vector<double> sub(vec.begin() + i_row * vec_row_size, vec.begin() + (i_row + 1) * vec_row_size);
+    std::vector<double> r = hypercube_matvec(mat, rows, cols, sub);
+    std::copy(r.begin(), r.end(), fact.begin() + i_row * row_size);
+  }
+  std::vector<double> prod(prod_size);
+  for (int i_col = 0; i_col < row_size; ++i_col)
+    for (int r = 0; r < rows; ++r) {
+      double acc = 0.;
+      for (int c = 0; c < cols; ++c) acc += mat[r * cols + c] * fact[c * row_size + i_col];
+      prod[r * row_size + i_col] = acc;
+    }
+  return prod;
+}
+
+// math::stretched_ind (math.hpp:188-199)
+int stretched_ind(int nd, int ind, const int* stretch)
+{
+  int stride = 1, stretched = 0;
+  for (int i_dim = nd - 2; i_dim >= 0; --i_dim)
+    if (!stretch[i_dim]) {
+      stretched += ((ind / ipow(2, nd - 2 - i_dim)) % 2) * stride;
+      stride *= 2;
+    }
+  return stretched;
+}
+
+// Solver::share_vertex_data with Vertex::vector_max (Solver.cpp:40-54, Vertex.cpp:6-9): every element's copy of a
+// vertex value becomes the maximum over the elements sharing the vertex; then Hanging_vertex_matcher::match
+// (Hanging_vertex_matcher.cpp:13-41) interpolates the hanging vertices of every hanging face from its corners.
+void share_vertex_max(const hx_host_mesh* m, std::vector<double>& vc)
+{
+  const int nd = m->n_dim, nvert = 1 << nd;
+  const Vertex_data& V = g_vertices;
+  std::vector<double> shared(V.n_vertex, -std::numeric_limits<double>::infinity());
+  for (size_t k = 0; k < V.vid.size(); ++k) shared[V.vid[k]] = std::max(shared[V.vid[k]], vc[k]);
+  for (size_t k = 0; k < V.vid.size(); ++k) vc[k] = shared[V.vid[k]];
+  const std::vector<double> interp_mat{1., 0., .5, .5, 0., 1.};  // 3 x 2
+  const size_t n_match = V.hanging.size() / 12;
+  for (size_t i = 0; i < n_match; ++i) {
+    const int* h = &V.hanging[12 * i];
+    const int id = h[0], isp = h[1];
+    const int str[2] = {h[2], h[3]};
+    int n_fine = 1 << (nd - 1);
+    for (int d = 0; d < nd - 1; ++d)
+      if (str[d]) n_fine /= 2;
+    int elem[4];
+    for (int f = 0; f < n_fine; ++f) elem[f] = (h[4 + 2 * f] ? m->n_elem[0] : 0) + h[5 + 2 * f];
+    const int n_vert = 1 << (nd - 1), stride = 1 << (nd - 1 - id);
+    int inds[4];
+    std::vector<double> values(n_vert);
+    for (int iv = 0; iv < n_vert; ++iv) {
+      inds[iv] = iv / stride * stride * 2 + iv % stride + isp * stride;
+      values[iv] = vc[size_t(elem[stretched_ind(nd, iv, str)]) * nvert + inds[iv]];
+    }
+    const std::vector<double> interpolated = hypercube_matvec(interp_mat, 3, 2, values);
+    for (int ie = 0; ie < n_fine; ++ie)
+      for (int iv = 0; iv < n_vert; ++iv) {
+        int interp_ind = (ie * !str[nd - 2]) % 2 + (iv % 2) * (1 + str[nd - 2]);
+        if (nd == 3) interp_ind += ((ie * !str[0]) / (1 + !str[1]) + iv / 2 * (1 + str[0])) * 3;
+        vc[size_t(elem[ie]) * nvert + inds[iv]] = interpolated[interp_ind];
+      }
+  }
+}
+
+// The fix coefficient of Solver::fix_admissibility (Solver.cpp:999-1030): element flags on the vertices, shared by
+// maximum, the element maximum on all its vertices, shared again, interpolated n-linearly into the Laplacian
+// artificial viscosity; returns the vertex values [n_elem[0] + n_elem[1]][nvert]
+std::vector<double> fix_coefficient(hx_host_mesh* m, const std::vector<int>& record)
+{
+  const int nd = m->n_dim, rs = m->row_size, nq = ipow(rs, nd), nvert = 1 << nd;
+  const Vertex_data& V = g_vertices;
+  if (!V.set || V.n_elem[0] != m->n_elem[0] || V.n_elem[1] != m->n_elem[1])
+    throw std::runtime_error("oracle: fix_admissibility needs the mesh's vertices (oracle_set_vertices)");
+  const size_t n_tot = size_t(m->n_elem[0]) + m->n_elem[1];
+  std::vector<double> vc(n_tot * nvert);
+  for (size_t E = 0; E < n_tot; ++E)
+    for (int v = 0; v < nvert; ++v) vc[E * nvert + v] = record[E];
+  share_vertex_max(m, vc);
+  for (size_t E = 0; E < n_tot; ++E) {
+    double max_fac = 0;
+    for (int v = 0; v < nvert; ++v) max_fac = std::max(max_fac, vc[E * nvert + v]);
+    for (int v = 0; v < nvert; ++v) vc[E * nvert + v] = max_fac;
+  }
+  share_vertex_max(m, vc);
+  Basis basis(rs);
+  std::vector<double> interp(size_t(rs) * 2);
+  for (int i = 0; i < rs; ++i) {
+    interp[2 * i] = 1. - basis.node[i];
+    interp[2 * i + 1] = basis.node[i];
+  }
+  for (int c = 0; c < 2; ++c)
+    for (int e = 0; e < m->n_elem[c]; ++e) {
+      const size_t E = (c ? m->n_elem[0] : 0) + e;
+      std::vector<double> vf(vc.begin() + E * nvert, vc.begin() + (E + 1) * nvert);
+      const std::vector<double> lap = hypercube_matvec(interp, rs, 2, vf);
+      std::copy(lap.begin(), lap.end(), m->av[c] + (size_t(e) * 2 + 1) * nq);
+    }
+  return vc;
+}
+
+void swap_av(hx_host_mesh* m)
+{
+  const int nq = ipow(m->row_size, m->n_dim);
+  for (int c = 0; c < 2; ++c)
+    for (int e = 0; e < m->n_elem[c]; ++e)
+      for (int q = 0; q < nq; ++q) std::swap(m->av[c][(size_t(e) * 2 + 1) * nq + q], m->av[c][size_t(e) * 2 * nq + q]);
+}
+
+// the fix's boundary faces: ghost state = inside (Solver.cpp:1066-1071), ghost LDG = -inside (Solver.cpp:103-115)
+void fta_bcs(hx_host_mesh* m, bool flux)
+{
+  Mesh_view mv{m};
+  const int nfq = ipow(m->row_size, m->n_dim - 1), nv = m->n_dim + 2;
+  for (int i = 0; i < m->n_bc_con; ++i) {
+    const int con = m->bc_con[i];
+    const double* in_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con]);
+    double* gh_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con + 1]);
+    for (int k = 0; k < nv * nfq; ++k) {
+      if (flux) gh_f[nv * nfq + k] = -in_f[nv * nfq + k];
+      else gh_f[k] = in_f[k];
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1961,7 +2192,8 @@ int oracle_write_face(hx_host_mesh* m, int cls, const hx_pde* pde)
     check_mesh(m);
     n_extrap(m, pde);
     int kind = pde_kind(pde);
-    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV) dispatch<Run_av_write_face>(m->n_dim, m->row_size, m, cls, kind);
+    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS)
+      dispatch<Run_av_write_face>(m->n_dim, m->row_size, m, cls, kind);
     else dispatch<Run_write_face>(m->n_dim, m->row_size, m, cls);
   });
 }
@@ -1971,7 +2203,8 @@ int oracle_neighbor(hx_host_mesh* m, int cls, const hx_pde* pde, int)
   return guarded([&] {
     check_mesh(m);
     int kind = pde_kind(pde);
-    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV) dispatch<Run_av_neighbor>(m->n_dim, m->row_size, m, cls, kind);
+    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS)
+      dispatch<Run_av_neighbor>(m->n_dim, m->row_size, m, cls, kind);
     else dispatch<Run_neighbor>(m->n_dim, m->row_size, m, cls, pde);
   });
 }
@@ -1982,9 +2215,15 @@ int oracle_local(hx_host_mesh* m, int cls, const hx_pde* pde, double dt, int i_s
     check_mesh(m);
     // HEXED_ASSERTs of the Local constructor (Spatial.hpp:322-323)
     int kind = pde_kind(pde);
-    if (kind == HX_PDE_NAVIER_STOKES && i_stage)
+    if ((kind == HX_PDE_NAVIER_STOKES || kind == HX_PDE_FIX_ADMIS) && i_stage)
       throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
     if (i_stage && compute_residual) throw std::runtime_error("residual calculation is a single-stage operation");
+    if (kind == HX_PDE_FIX_ADMIS) {
+      if (compute_residual || use_filter)
+        throw std::runtime_error("oracle: Fix_therm_admis is restated without compute_residual and filter");
+      dispatch<Run_fta_local>(m->n_dim, m->row_size, m, cls, dt);
+      return;
+    }
     if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV) {
       if (use_filter) throw std::runtime_error("oracle: the artificial-viscosity PDEs are restated without the filter");
       if (kind == HX_PDE_ADVECTION)
@@ -2005,7 +2244,7 @@ int oracle_neighbor_reconcile(hx_host_mesh* m, int cls, const hx_pde* pde)
   return guarded([&] {
     check_mesh(m);
     if (pde_kind(pde) == HX_PDE_SMOOTH_AV) dispatch<Run_sav_neighbor_reconcile>(m->n_dim, m->row_size, m, cls);
-    else dispatch<Run_neighbor_reconcile>(m->n_dim, m->row_size, m, cls);
+    else dispatch<Run_neighbor_reconcile>(m->n_dim, m->row_size, m, cls);  // the nv LDG variables
   });
 }
 
@@ -2021,6 +2260,13 @@ int oracle_reconcile_ldg_flux(hx_host_mesh* m, int cls, const hx_pde* pde, doubl
       dispatch<Run_sav_reconcile>(m->n_dim, m->row_size, m, cls, dt, pde->diff_time, pde->cheby_step);
       return;
     }
+    if (pde_kind(pde) == HX_PDE_FIX_ADMIS) {
+      if (i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+      if (compute_residual || use_filter)
+        throw std::runtime_error("oracle: Fix_therm_admis is restated without compute_residual and filter");
+      dispatch<Run_fta_reconcile>(m->n_dim, m->row_size, m, cls, dt);
+      return;
+    }
     if (is_ns(pde) && i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
     dispatch<Run_reconcile>(m->n_dim, m->row_size, m, cls, dt, compute_residual, use_filter);
   });
@@ -2032,8 +2278,8 @@ int oracle_max_dt(hx_host_mesh* m, int cls, const hx_pde* pde, int is_local, int
   return guarded([&] {
     check_mesh(m);
     int kind = pde_kind(pde);
-    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV)
-      *dt_out = dispatch<Run_av_max_dt>(m->n_dim, m->row_size, m, cls, kind == HX_PDE_SMOOTH_AV, is_local, safety_conv,
+    if (kind == HX_PDE_ADVECTION || kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS)
+      *dt_out = dispatch<Run_av_max_dt>(m->n_dim, m->row_size, m, cls, kind != HX_PDE_ADVECTION, is_local, safety_conv,
                                         safety_diff);
     else *dt_out = dispatch<Run_max_dt>(m->n_dim, m->row_size, m, cls, pde, is_local, safety_conv, safety_diff);
   });
@@ -2154,19 +2400,179 @@ int oracle_update(hx_host_mesh* m, const hx_update_options* o, hx_update_status*
       if (rc) return rc;
       double nominal_dt = std::min(mdt, o->max_time_step);
       double dt = nominal_dt * chebyshev_step(n_cheby, i_cheby);
+      bool fixed = false;
       for (int i = 0; i < 2; ++i) {
         hx_kernel_options opts{dt, i, 0, o->use_filter};
         if ((rc = oracle_apply_state_bcs(m))) return rc;
         if (o->use_ldg && !i) rc = oracle_compute_navier_stokes(m, &opts, o->visc, o->cond);
         else rc = oracle_compute_euler(m, &opts);
         if (rc) return rc;
+        if (o->fix_admis) {  // Solver.cpp:867
+          int iters = 0;
+          if ((rc = oracle_fix_admissibility(m, o, o->fix_admis_max_safety, &iters))) return rc;
+          st->fix_admis_iters += iters;
+          fixed = iters || fixed;
+        }
       }
+      if (fixed) break;
       st->time_step = dt;
       st->flow_time += dt;
     }
   }
   ++st->iteration;
   return 0;
+}
+
+// ---- thermodynamic admissibility ------------------------------------------------------------------------------
+
+int oracle_set_vertices(const hx_host_mesh* m, int n_vertex, const int* vid_cartesian, const int* vid_deformed,
+                        int n_hanging, const int* hanging)
+{
+  return guarded([&] {
+    const int nd = m->n_dim, nvert = 1 << nd;
+    Vertex_data v;
+    v.n_elem[0] = m->n_elem[0];
+    v.n_elem[1] = m->n_elem[1];
+    v.n_vertex = n_vertex;
+    for (int c = 0; c < 2; ++c) {
+      const int* src = c ? vid_deformed : vid_cartesian;
+      for (long long k = 0; k < (long long)m->n_elem[c] * nvert; ++k) {
+        if (src[k] < 0 || src[k] >= n_vertex) throw std::runtime_error("oracle: vertex id out of range");
+        v.vid.push_back(src[k]);
+      }
+    }
+    if (n_hanging && nd < 2) throw std::runtime_error("oracle: hanging vertices need n_dim >= 2");
+    for (long long k = 0; k < 12LL * n_hanging; ++k) v.hanging.push_back(hanging[k]);
+    for (int i = 0; i < n_hanging; ++i) {
+      const int* h = hanging + 12 * i;
+      int n_fine = 1 << (nd - 1);
+      for (int d = 0; d < nd - 1; ++d)
+        if (h[2 + d]) n_fine /= 2;
+      if (h[0] < 0 || h[0] >= nd) throw std::runtime_error("oracle: hanging-vertex dimension out of range");
+      for (int f = 0; f < n_fine; ++f)
+        if (h[4 + 2 * f] < 0 || h[4 + 2 * f] > 1 || h[5 + 2 * f] < 0 || h[5 + 2 * f] >= m->n_elem[h[4 + 2 * f]])
+          throw std::runtime_error("oracle: hanging-vertex element out of range");
+    }
+    v.set = true;
+    g_vertices = std::move(v);
+  });
+}
+
+int oracle_is_admissible(hx_host_mesh* m, int* admissible_out, int* record)
+{
+  return guarded([&] {
+    check_mesh(m);
+    std::vector<int> rec;
+    *admissible_out = is_admissible(m, rec);
+    if (record) std::copy(rec.begin(), rec.end(), record);
+  });
+}
+
+// the fix coefficient alone (Solver.cpp:999-1030), from the given element flags, into the Laplacian coefficient;
+// vertex_out (may be NULL) receives the vertex values [n_elem[0] + n_elem[1]][2^n_dim]
+int oracle_fix_coefficient(hx_host_mesh* m, const int* record, double* vertex_out)
+{
+  return guarded([&] {
+    check_mesh(m);
+    for (int c = 0; c < 2; ++c)
+      if (m->n_elem[c] && !m->av[c]) throw std::runtime_error("oracle: the fix coefficient needs the av arrays");
+    std::vector<int> rec(record, record + m->n_elem[0] + m->n_elem[1]);
+    const std::vector<double> vc = fix_coefficient(m, rec);
+    if (vertex_out) std::copy(vc.begin(), vc.end(), vertex_out);
+  });
+}
+
+// compute_fix_therm_admis (kernels_diffusive.cpp:32), flux_bc = Solver::apply_fta_flux_bcs (Solver.cpp:103-115)
+int oracle_compute_fix_therm_admis(hx_host_mesh* m, const hx_kernel_options* o)
+{
+  return guarded([&] {
+    check_mesh(m);
+    const int nd = m->n_dim, rs = m->row_size, nv = nd + 2;
+    if (o->i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+    dispatch<Run_av_neighbor>(nd, rs, m, HX_CARTESIAN, int(HX_PDE_FIX_ADMIS));
+    dispatch<Run_av_neighbor>(nd, rs, m, HX_DEFORMED, int(HX_PDE_FIX_ADMIS));
+    dispatch<Run_restrict>(nd, rs, m, nv, 1, 0);
+    dispatch<Run_restrict>(nd, rs, m, nv, 0, 1);
+    dispatch<Run_fta_local>(nd, rs, m, HX_CARTESIAN, o->dt);
+    dispatch<Run_fta_local>(nd, rs, m, HX_DEFORMED, o->dt);
+    dispatch<Run_prolong>(nd, rs, m, nv, 1, 1);
+    fta_bcs(m, true);
+    dispatch<Run_neighbor_reconcile>(nd, rs, m, HX_CARTESIAN);
+    dispatch<Run_neighbor_reconcile>(nd, rs, m, HX_DEFORMED);
+    dispatch<Run_restrict>(nd, rs, m, nv, 1, 1);
+    dispatch<Run_fta_reconcile>(nd, rs, m, HX_CARTESIAN, o->dt);
+    dispatch<Run_fta_reconcile>(nd, rs, m, HX_DEFORMED, o->dt);
+    dispatch<Run_prolong>(nd, rs, m, nv, 0, 0);
+  });
+}
+
+// max_dt_fix_therm_admis (kernels_max_dt.cpp:21)
+int oracle_max_dt_fix_therm_admis(hx_host_mesh* m, const hx_kernel_options*, double msc, double msd, int local_time,
+                                  double* dt_out)
+{
+  return guarded([&] {
+    check_mesh(m);
+    const int nd = m->n_dim, rs = m->row_size;
+    const double a = dispatch<Run_av_max_dt>(nd, rs, m, HX_CARTESIAN, true, local_time, msc, msd);
+    const double b = dispatch<Run_av_max_dt>(nd, rs, m, HX_DEFORMED, true, local_time, msc, msd);
+    *dt_out = std::min(a, b);
+  });
+}
+
+// Solver::fix_admissibility (Solver.cpp:961-1095), step for step; the flow's Max_dt uses the settings `o`
+int fix_admissibility(hx_host_mesh* m, const hx_update_options* o, double stability_ratio)
+{
+  for (int c = 0; c < 2; ++c)
+    if (m->n_elem[c] && !m->av[c]) throw std::runtime_error("oracle: fix_admissibility needs the av arrays");
+  std::vector<int> record;
+  int iter;
+  int n_iters = std::numeric_limits<int>::max();
+  for (iter = 0; iter < n_iters; ++iter) {
+    if (iter >= 100000)
+      throw std::runtime_error("failed to fix thermodynamic admissability in " + std::to_string(iter) + " iterations");
+    if (is_admissible(m, record)) {
+      if (iter) n_iters = std::min(n_iters, 2 * iter);
+      else {
+        ++iter;
+        break;
+      }
+    } else {
+      n_iters = std::numeric_limits<int>::max();
+    }
+    fix_coefficient(m, record);
+    swap_av(m);
+    hx_kernel_options ko{0., 0, 0, 0};
+    double dt = stability_ratio;
+    if (oracle_max_dt_fix_therm_admis(m, &ko, dt, dt, 1, &dt)) throw std::runtime_error(g_err);
+    dt = 1.;
+    const double linear = dt, quadratic = dt * dt / 8 / 0.9;
+    double step[2];
+    step[1] = (linear + std::sqrt(linear * linear - 4 * quadratic)) / 2.;
+    step[0] = quadratic / step[1];
+    for (double st : step) {
+      fta_bcs(m, false);
+      ko.dt = st;
+      if (oracle_compute_fix_therm_admis(m, &ko)) throw std::runtime_error(g_err);
+    }
+    const double n_cheby = o->n_cheby_flow, max_cheby = chebyshev_step(n_cheby, n_cheby - 1);
+    hx_kernel_options fo{0., 0, 0, o->use_filter};
+    double mdt;
+    const int rc = o->use_ldg ? oracle_max_dt_navier_stokes(m, &fo, o->max_safety / max_cheby, o->max_safety, o->local_time,
+                                                            o->visc, o->cond, &mdt)
+                              : oracle_max_dt_euler(m, &fo, o->max_safety / max_cheby, o->max_safety, o->local_time, &mdt);
+    if (rc) throw std::runtime_error(g_err);
+    swap_av(m);
+  }
+  --iter;
+  return iter;
+}
+
+int oracle_fix_admissibility(hx_host_mesh* m, const hx_update_options* o, double stability_ratio, int* n_iters)
+{
+  return guarded([&] {
+    check_mesh(m);
+    *n_iters = fix_admissibility(m, o, stability_ratio);
+  });
 }
 
 // ---- artificial-viscosity generator: kernel sequences and Solver::update_art_visc_smoothness ------------------

@@ -54,6 +54,20 @@ int oracle_compute_advection(hx_host_mesh* m, const hx_kernel_options* o, double
 int oracle_compute_smooth_av(hx_host_mesh* m, const hx_kernel_options* o, double diff_time, double cheby_step);
 int oracle_update_art_visc_smoothness(hx_host_mesh* m, const hx_av_options* o, double* residual);
 
+/* thermodynamic admissibility (Solver.cpp:921-1095, thermo.cpp:6-18, pde.hpp:437-493); the per-kernel entry points
+ * above also take HX_PDE_FIX_ADMIS. hx_host_mesh carries no vertex connectivity, so the fix needs
+ * oracle_set_vertices first: vid_* [n_elem][2^n_dim] shared vertex of each element vertex; hanging [n_hanging][12]:
+ * i_dim, is_positive, stretch[2], then (class, element) of each fine element in the hanging face's order. */
+int oracle_set_vertices(const hx_host_mesh* m, int n_vertex, const int* vid_cartesian, const int* vid_deformed,
+                        int n_hanging, const int* hanging);
+int oracle_is_admissible(hx_host_mesh* m, int* admissible, int* record);                   /* Solver.cpp:921-959  */
+int oracle_fix_coefficient(hx_host_mesh* m, const int* record, double* vertex_out);        /* Solver.cpp:999-1030 */
+int oracle_compute_fix_therm_admis(hx_host_mesh* m, const hx_kernel_options* o);          /* kernels_diffusive.cpp:32 */
+int oracle_max_dt_fix_therm_admis(hx_host_mesh* m, const hx_kernel_options* o, double msc, double msd,
+                                  int local_time, double* dt_out);                         /* kernels_max_dt.cpp:21 */
+int oracle_fix_admissibility(hx_host_mesh* m, const hx_update_options* o, double stability_ratio,
+                             int* n_iters);                                                /* Solver.cpp:961-1095 */
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,3 +73,14 @@ def per_var_rel_l2(a, b):
 def say(*msg):
     print(*msg, flush=True)
     sys.stdout.flush()
+
+
+def set_vertices(m):
+    """Hand the oracle the vertex connectivity of `m` (shared vertex ids and hanging-vertex matchers), which
+    fix_admissibility needs and hx_host_mesh does not carry."""
+    s = m.struct()
+    vid = [np.ascontiguousarray(m.vertex_ids[c], dtype=np.int32) for c in range(2)]
+    hang = np.ascontiguousarray(m.hanging, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    o_call("oracle_set_vertices", ctypes.byref(s), int(m.n_vertex), vid[0].ctypes.data_as(ip),
+           vid[1].ctypes.data_as(ip), int(hang.shape[0]), hang.ctypes.data_as(ip))
