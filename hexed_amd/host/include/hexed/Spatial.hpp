@@ -11,6 +11,7 @@
 #include "Refined_face.hpp"
 #include "amd.hpp"
 #include "kernel_factory.hpp"
+#include "pde.hpp"
 
 namespace hexed
 {
@@ -108,7 +109,7 @@ class Spatial
     hx_pde pde;
 
     public:
-    Neighbor_reconcile() : pde{Pde_templ<n_dim, row_size>().abi} {}
+    Neighbor_reconcile() : pde{pde::kind_of<Pde_templ<n_dim, row_size>>()} {}
     void operator()(Sequence<Kernel_connection&>& connections) override
     {
       amd::Mesh_lease m(connections, n_dim, row_size, is_deformed);
@@ -147,7 +148,7 @@ class Spatial
     void operator()(Sequence<Refined_face&>& ref_faces) override
     {
       amd::Mesh_lease m(ref_faces, n_dim, row_size);
-      hx_pde pde = Pde_templ<n_dim, row_size>().abi;
+      hx_pde pde = pde::kind_of<Pde_templ<n_dim, row_size>>();
       amd::check(hx_prolong_refined(m.device(), &pde, scl, off), "Prolong_refined");
     }
   };
@@ -162,7 +163,7 @@ class Spatial
     void operator()(Sequence<Refined_face&>& ref_faces) override
     {
       amd::Mesh_lease m(ref_faces, n_dim, row_size);
-      hx_pde pde = Pde_templ<n_dim, row_size>().abi;
+      hx_pde pde = pde::kind_of<Pde_templ<n_dim, row_size>>();
       amd::check(hx_restrict_refined(m.device(), &pde, scl, off), "Restrict_refined");
     }
   };

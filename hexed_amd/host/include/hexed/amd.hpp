@@ -30,7 +30,8 @@ class Accelerated_mesh
 {
   public:
   //! `reference_layout`: element storage follows the reference's Element (time step scale right after the state,
-  //! then the bulk and Laplacian artificial viscosity coefficients), so the AV fields can be read from it
+  //! then the bulk and Laplacian artificial viscosity coefficients, the 4 forcing and the row_size advection arrays,
+  //! Element.cpp:114-142), so the artificial-viscosity fields are read from it and written back to it
   Accelerated_mesh(Kernel_mesh mesh, bool reference_layout = true);
   ~Accelerated_mesh();
   Accelerated_mesh(const Accelerated_mesh&) = delete;
@@ -64,6 +65,7 @@ class Accelerated_mesh
   hx_mesh* dev = nullptr;
   // host mirrors in the layout of hx_host_mesh
   std::vector<double> state[2], rcache[2], tss[2], av[2], nom[2], vtss[2], metrics, faces, normals;
+  std::vector<double> forcing[2], advection[2];  // reference layout only (Element.cpp:134-142)
   std::vector<int> face_slot[2], face_nslot, con_slot[2], con_dir[2], con_nslot, ref_slot, ref_stretch, bc_con, bc_serial;
   std::vector<hx_bc> bc_defs;
   void build_topology();

@@ -4,7 +4,10 @@
 #define HEXED_AMD_KERNELS_HPP_
 
 #include <functional>
+#include <memory>
 
+#include "Face_permutation.hpp"
+#include "Kernel_connection.hpp"
 #include "Kernel_mesh.hpp"
 #include "Stopwatch_tree.hpp"
 #include "Transport_model.hpp"
@@ -34,6 +37,26 @@ double max_dt_navier_stokes(Kernel_mesh, Kernel_options, double convective_safet
 void compute_prolong(Kernel_mesh, bool scale = false, bool offset = false);
 void compute_restrict(Kernel_mesh, bool scale = true, bool offset = false);
 void compute_write_face(Kernel_mesh);
+
+// the artificial-viscosity generator's PDEs (pde::Advection, pde::Smooth_art_visc) and the admissibility fix's
+// (pde::Fix_therm_admis); on a registered mesh with the reference element layout, which holds their fields.
+// `flux_bc` as for compute_navier_stokes: empty applies the device's own hooks (Solver::apply_avc_diff_flux_bcs,
+// Solver::apply_fta_flux_bcs)
+void compute_advection(Kernel_mesh, Kernel_options, double advect_length);
+void compute_smooth_av(Kernel_mesh, Kernel_options, std::function<void()> flux_bc, double diff_time,
+                       double chebyshev_step);
+void compute_fix_therm_admis(Kernel_mesh, Kernel_options, std::function<void()> flux_bc);
+double max_dt_advection(Kernel_mesh, Kernel_options, double convective_safety, double diffusive_safety,
+                        bool local_time, double advect_length);
+double max_dt_smooth_av(Kernel_mesh, Kernel_options, double convective_safety, double diffusive_safety,
+                        bool local_time);
+double max_dt_fix_therm_admis(Kernel_mesh, Kernel_options, double convective_safety, double diffusive_safety,
+                              bool local_time);
+void compute_prolong_advection(Kernel_mesh);
+void compute_write_face_advection(Kernel_mesh);
+void compute_write_face_smooth_av(Kernel_mesh);
+//! host reordering of one connection's face-1 data (nd + 2 variables) to match face 0, as Neighbor does on the device
+std::unique_ptr<Face_permutation_dynamic> face_permutation(int n_dim, int row_size, Connection_direction, double* data);
 
 }  // namespace hexed
 #endif

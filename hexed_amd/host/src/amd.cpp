@@ -191,6 +191,8 @@ void Accelerated_mesh::build_topology()
     rcache[c].assign(n * std::max(nv, rs) * nq, 0.);
     tss[c].assign(n * nq, 1.);
     av[c].assign(ref_layout ? n * 2 * nq : 0, 0.);
+    forcing[c].assign(ref_layout ? n * 4 * nq : 0, 0.);
+    advection[c].assign(ref_layout ? n * rs * nq : 0, 0.);
     nom[c].resize(n);
     vtss[c].resize(n << nd);
     for (size_t e = 0; e < n; ++e) {
@@ -233,7 +235,11 @@ void Accelerated_mesh::gather()
       std::copy(rc, rc + nrc, rcache[c].begin() + e * nrc);
       const double* ts = el.time_step_scale();
       std::copy(ts, ts + nq, tss[c].begin() + e * nq);
-      if (ref_layout) std::copy(st + (nv + 1) * nq, st + (nv + 3) * nq, av[c].begin() + e * 2 * nq);
+      if (ref_layout) {
+        std::copy(st + (nv + 1) * nq, st + (nv + 3) * nq, av[c].begin() + e * 2 * nq);
+        std::copy(st + (nv + 3) * nq, st + (nv + 7) * nq, forcing[c].begin() + e * 4 * nq);
+        std::copy(st + (nv + 7) * nq, st + (nv + 7 + rs) * nq, advection[c].begin() + e * rs * nq);
+      }
       if (c == HX_DEFORMED) {
         const double* n = el.reference_level_normals();
         const double* d = el.jacobian_determinant();
@@ -257,6 +263,12 @@ void Accelerated_mesh::scatter()
       const size_t nrc = std::max(nv, rs) * nq;
       std::copy(rcache[c].begin() + e * nrc, rcache[c].begin() + (e + 1) * nrc, el.residual_cache());
       std::copy(tss[c].begin() + e * nq, tss[c].begin() + (e + 1) * nq, el.time_step_scale());
+      if (ref_layout) {
+        double* st = el.state();
+        std::copy(av[c].begin() + e * 2 * nq, av[c].begin() + (e + 1) * 2 * nq, st + (nv + 1) * nq);
+        std::copy(forcing[c].begin() + e * 4 * nq, forcing[c].begin() + (e + 1) * 4 * nq, st + (nv + 3) * nq);
+        std::copy(advection[c].begin() + e * rs * nq, advection[c].begin() + (e + 1) * rs * nq, st + (nv + 7) * nq);
+      }
     }
   }
   for (size_t s = 0; s < slot_ptr.size(); ++s)
@@ -274,6 +286,8 @@ hx_host_mesh Accelerated_mesh::description()
     d.rcache[c] = rcache[c].data();
     d.tss[c] = tss[c].data();
     d.av[c] = av[c].empty() ? nullptr : av[c].data();
+    d.forcing[c] = forcing[c].empty() ? nullptr : forcing[c].data();
+    d.advection[c] = advection[c].empty() ? nullptr : advection[c].data();
     d.nom_size[c] = nom[c].data();
     d.vertex_tss[c] = vtss[c].data();
     d.face_slot[c] = face_slot[c].data();
@@ -319,6 +333,9 @@ void Accelerated_mesh::push()
     check(hx_mesh_upload(dev, HX_FIELD_RCACHE, c, rcache[c].data(), rcache[c].size()), "upload");
     check(hx_mesh_upload(dev, HX_FIELD_TSS, c, tss[c].data(), tss[c].size()), "upload");
     if (!av[c].empty()) check(hx_mesh_upload(dev, HX_FIELD_AV, c, av[c].data(), av[c].size()), "upload");
+    if (!forcing[c].empty()) check(hx_mesh_upload(dev, HX_FIELD_FORCING, c, forcing[c].data(), forcing[c].size()), "upload");
+    if (!advection[c].empty())
+      check(hx_mesh_upload(dev, HX_FIELD_ADVECTION, c, advection[c].data(), advection[c].size()), "upload");
   }
   if (!elems[1].empty()) check(hx_mesh_upload(dev, HX_FIELD_METRICS, 1, metrics.data(), metrics.size()), "upload");
   check(hx_mesh_upload(dev, HX_FIELD_FACES, 0, faces.data(), faces.size()), "upload");
@@ -333,6 +350,11 @@ void Accelerated_mesh::pull()
     check(hx_mesh_download(dev, HX_FIELD_STATE, c, state[c].data(), state[c].size()), "download");
     check(hx_mesh_download(dev, HX_FIELD_RCACHE, c, rcache[c].data(), rcache[c].size()), "download");
     check(hx_mesh_download(dev, HX_FIELD_TSS, c, tss[c].data(), tss[c].size()), "download");
+    if (!av[c].empty()) check(hx_mesh_download(dev, HX_FIELD_AV, c, av[c].data(), av[c].size()), "download");
+    if (!forcing[c].empty())
+      check(hx_mesh_download(dev, HX_FIELD_FORCING, c, forcing[c].data(), forcing[c].size()), "download");
+    if (!advection[c].empty())
+      check(hx_mesh_download(dev, HX_FIELD_ADVECTION, c, advection[c].data(), advection[c].size()), "download");
   }
   check(hx_mesh_download(dev, HX_FIELD_FACES, 0, faces.data(), faces.size()), "download");
   scatter();
@@ -495,6 +517,78 @@ void compute_euler(Kernel_mesh mesh, Kernel_options opts)
   count(mesh, opts, true);
 }
 
+namespace
+{
+// COMPUTE_DIFFUSION (kernels_diffusive.cpp:8-30) with a host flux_bc callback, which needs the face data on the host:
+// the device sequence is split around it
+void diffusion_with_host_bc(Kernel_mesh& mesh, amd::Mesh_lease& m, const hx_kernel_options& o, const hx_pde& p,
+                            const std::function<void()>& flux_bc)
+{
+  hx_mesh* d = m.device();
+  amd::check(hx_neighbor(d, HX_CARTESIAN, &p, 0), "neighbor");
+  amd::check(hx_neighbor(d, HX_DEFORMED, &p, 0), "neighbor");
+  amd::check(hx_restrict_refined(d, &p, 1, 0), "restrict");
+  amd::check(hx_restrict_refined(d, &p, 0, 1), "restrict");
+  amd::check(hx_local(d, HX_CARTESIAN, &p, o.dt, 0, o.compute_residual, o.use_filter), "local");
+  amd::check(hx_local(d, HX_DEFORMED, &p, o.dt, 0, o.compute_residual, o.use_filter), "local");
+  amd::check(hx_prolong_refined(d, &p, 1, 1), "prolong");
+  int cls;
+  amd::Accelerated_mesh* am = amd::Accelerated_mesh::find(&mesh.car_elems, &cls);
+  if (am) am->pull();
+  flux_bc();
+  if (am) am->push();
+  amd::check(hx_neighbor_reconcile(d, HX_CARTESIAN, &p), "neighbor_reconcile");
+  amd::check(hx_neighbor_reconcile(d, HX_DEFORMED, &p), "neighbor_reconcile");
+  amd::check(hx_restrict_refined(d, &p, 1, 1), "restrict");
+  amd::check(hx_reconcile_ldg_flux(d, HX_CARTESIAN, &p, o.dt, 0, o.compute_residual, o.use_filter), "reconcile");
+  amd::check(hx_reconcile_ldg_flux(d, HX_DEFORMED, &p, o.dt, 0, o.compute_residual, o.use_filter), "reconcile");
+  amd::check(hx_prolong_refined(d, &p, 0, 0), "prolong");
+}
+
+// the face-1 index matching face-0 index i (Spatial.hpp:85-129: transpose, then flip)
+int face1_index(int nd, int rs, Connection_direction dir, int i)
+{
+  if (nd == 3) {
+    int c = i / rs, r = i % rs;
+    const bool tr = dir.transpose();
+    if (dir.flip_tangential()) {
+      const bool colwise = (dir.i_dim[0] > 3 - dir.i_dim[0] - dir.i_dim[1]) != tr;
+      if (colwise) r = rs - 1 - r;
+      else c = rs - 1 - c;
+    }
+    return tr ? r * rs + c : c * rs + r;
+  }
+  if (nd == 2) return dir.flip_tangential() ? rs - 1 - i : i;
+  return i;
+}
+
+class Face_perm : public Face_permutation_dynamic
+{
+  int nd, rs, nfq;
+  Connection_direction dir;
+  double* data;
+
+  public:
+  Face_perm(int n_dim, int row_size, Connection_direction d, double* target)
+  : nd{n_dim}, rs{row_size}, nfq{1}, dir{d}, data{target}
+  {
+    for (int i = 1; i < nd; ++i) nfq *= rs;
+  }
+  void match_faces() override
+  {
+    std::vector<double> old(data, data + (nd + 2) * nfq);
+    for (int v = 0; v < nd + 2; ++v)
+      for (int i = 0; i < nfq; ++i) data[v * nfq + i] = old[v * nfq + face1_index(nd, rs, dir, i)];
+  }
+  void restore() override
+  {
+    std::vector<double> cur(data, data + (nd + 2) * nfq);
+    for (int v = 0; v < nd + 2; ++v)
+      for (int i = 0; i < nfq; ++i) data[v * nfq + face1_index(nd, rs, dir, i)] = cur[v * nfq + i];
+  }
+};
+}  // namespace
+
 // compute_navier_stokes (kernels_diffusive.cpp:8-29). A host flux_bc callback needs the face data on the host, so
 // with one the device sequence is split around it; without one, the mesh's device boundary conditions apply.
 void compute_navier_stokes(Kernel_mesh mesh, Kernel_options opts, std::function<void()> flux_bc, Transport_model visc,
@@ -506,28 +600,98 @@ void compute_navier_stokes(Kernel_mesh mesh, Kernel_options opts, std::function<
   if (!flux_bc) {
     amd::check(hx_compute_navier_stokes(m.device(), &o, visc.abi(), therm_cond.abi()), "compute_navier_stokes");
   } else {
-    hx_pde p{HX_PDE_NAVIER_STOKES, visc.abi(), therm_cond.abi(), 0., 0., 0.};
-    hx_mesh* d = m.device();
-    amd::check(hx_neighbor(d, HX_CARTESIAN, &p, 0), "neighbor");
-    amd::check(hx_neighbor(d, HX_DEFORMED, &p, 0), "neighbor");
-    amd::check(hx_restrict_refined(d, &p, 1, 0), "restrict");
-    amd::check(hx_restrict_refined(d, &p, 0, 1), "restrict");
-    amd::check(hx_local(d, HX_CARTESIAN, &p, o.dt, 0, o.compute_residual, o.use_filter), "local");
-    amd::check(hx_local(d, HX_DEFORMED, &p, o.dt, 0, o.compute_residual, o.use_filter), "local");
-    amd::check(hx_prolong_refined(d, &p, 1, 1), "prolong");
-    int cls;
-    amd::Accelerated_mesh* am = amd::Accelerated_mesh::find(&mesh.car_elems, &cls);
-    if (am) am->pull();
-    flux_bc();
-    if (am) am->push();
-    amd::check(hx_neighbor_reconcile(d, HX_CARTESIAN, &p), "neighbor_reconcile");
-    amd::check(hx_neighbor_reconcile(d, HX_DEFORMED, &p), "neighbor_reconcile");
-    amd::check(hx_restrict_refined(d, &p, 1, 1), "restrict");
-    amd::check(hx_reconcile_ldg_flux(d, HX_CARTESIAN, &p, o.dt, 0, o.compute_residual, o.use_filter), "reconcile");
-    amd::check(hx_reconcile_ldg_flux(d, HX_DEFORMED, &p, o.dt, 0, o.compute_residual, o.use_filter), "reconcile");
-    amd::check(hx_prolong_refined(d, &p, 0, 0), "prolong");
+    diffusion_with_host_bc(mesh, m, o, hx_pde{HX_PDE_NAVIER_STOKES, visc.abi(), therm_cond.abi(), 0., 0., 0.}, flux_bc);
   }
   count(mesh, opts, true);
+}
+
+// compute_advection (kernels_convective.cpp:8-19 with pde::Advection)
+void compute_advection(Kernel_mesh mesh, Kernel_options opts, double advect_length)
+{
+  if (opts.i_stage && opts.compute_residual) throw std::runtime_error("residual calculation is a single-stage operation");
+  amd::Mesh_lease m(mesh);
+  hx_kernel_options o = abi_opts(opts);
+  amd::check(hx_compute_advection(m.device(), &o, advect_length), "compute_advection");
+  count(mesh, opts, true);
+}
+
+// compute_smooth_av (kernels_diffusive.cpp:31)
+void compute_smooth_av(Kernel_mesh mesh, Kernel_options opts, std::function<void()> flux_bc, double diff_time,
+                       double chebyshev_step)
+{
+  if (opts.i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+  amd::Mesh_lease m(mesh);
+  hx_kernel_options o = abi_opts(opts);
+  if (!flux_bc) amd::check(hx_compute_smooth_av(m.device(), &o, diff_time, chebyshev_step), "compute_smooth_av");
+  else diffusion_with_host_bc(mesh, m, o, hx_pde{HX_PDE_SMOOTH_AV, inviscid.abi(), inviscid.abi(), 0., diff_time, chebyshev_step},
+                              flux_bc);
+  count(mesh, opts, true);
+}
+
+// compute_fix_therm_admis (kernels_diffusive.cpp:32)
+void compute_fix_therm_admis(Kernel_mesh mesh, Kernel_options opts, std::function<void()> flux_bc)
+{
+  if (opts.i_stage) throw std::runtime_error("two-stage stabilization is not applicable to diffusion equations");
+  amd::Mesh_lease m(mesh);
+  hx_kernel_options o = abi_opts(opts);
+  if (!flux_bc) amd::check(hx_compute_fix_therm_admis(m.device(), &o), "compute_fix_therm_admis");
+  else diffusion_with_host_bc(mesh, m, o, hx_pde{HX_PDE_FIX_ADMIS, inviscid.abi(), inviscid.abi(), 0., 0., 0.}, flux_bc);
+  count(mesh, opts, true);
+}
+
+// max_dt_advection / max_dt_smooth_av / max_dt_fix_therm_admis (kernels_max_dt.cpp:13-21)
+double max_dt_advection(Kernel_mesh mesh, Kernel_options opts, double msc, double msd, bool local_time,
+                        double advect_length)
+{
+  amd::Mesh_lease m(mesh);
+  hx_kernel_options o = abi_opts(opts);
+  double dt = 0.;
+  amd::check(hx_max_dt_advection(m.device(), &o, msc, msd, local_time, advect_length, &dt), "max_dt_advection");
+  return dt;
+}
+
+double max_dt_smooth_av(Kernel_mesh mesh, Kernel_options opts, double msc, double msd, bool local_time)
+{
+  amd::Mesh_lease m(mesh);
+  hx_kernel_options o = abi_opts(opts);
+  double dt = 0.;
+  amd::check(hx_max_dt_smooth_av(m.device(), &o, msc, msd, local_time, &dt), "max_dt_smooth_av");
+  return dt;
+}
+
+double max_dt_fix_therm_admis(Kernel_mesh mesh, Kernel_options opts, double msc, double msd, bool local_time)
+{
+  amd::Mesh_lease m(mesh);
+  hx_kernel_options o = abi_opts(opts);
+  double dt = 0.;
+  amd::check(hx_max_dt_fix_therm_admis(m.device(), &o, msc, msd, local_time, &dt), "max_dt_fix_therm_admis");
+  return dt;
+}
+
+// kernels_convective.cpp:38-56; Write_face<Advection>(basis, 1.) and Write_face<Smooth_art_visc>(basis, 1., 1.)
+void compute_prolong_advection(Kernel_mesh mesh)
+{
+  amd::Mesh_lease m(mesh);
+  amd::check(hx_compute_prolong_advection(m.device(), 1.), "compute_prolong_advection");
+}
+
+void compute_write_face_advection(Kernel_mesh mesh)
+{
+  amd::Mesh_lease m(mesh);
+  amd::check(hx_compute_write_face_advection(m.device(), 1.), "compute_write_face_advection");
+}
+
+void compute_write_face_smooth_av(Kernel_mesh mesh)
+{
+  amd::Mesh_lease m(mesh);
+  amd::check(hx_compute_write_face_smooth_av(m.device()), "compute_write_face_smooth_av");
+}
+
+std::unique_ptr<Face_permutation_dynamic> face_permutation(int n_dim, int row_size, Connection_direction dir, double* data)
+{
+  if (n_dim < 1 || n_dim > HX_MAX_N_DIM || row_size < 2 || row_size > HX_MAX_ROW_SIZE)
+    throw std::runtime_error("demand for invalid kernel");
+  return std::unique_ptr<Face_permutation_dynamic>(new Face_perm(n_dim, row_size, dir, data));
 }
 
 // max_dt_euler / max_dt_navier_stokes (kernels_max_dt.cpp:8-17)

@@ -60,7 +60,7 @@ class Test_element : public Kernel_element
 {
   public:
   Params p;
-  std::vector<double> data;   // state, time step scale, bulk AV, Laplacian AV (Element.cpp:114-142 ordering)
+  std::vector<double> data;   // state, time step scale, bulk AV, Laplacian AV, 4 forcing, rs advection (Element.cpp:114-142)
   std::vector<double> rcache;
   std::vector<double> vertex;
   std::vector<double> jac;    // deformed: reference-level normals then determinant
@@ -70,7 +70,7 @@ class Test_element : public Kernel_element
   double* fnrml[6] = {};
   double unc = 0;
   Test_element(Params pp, double size, bool deformed)
-  : p{pp}, data((pp.nv + 3) * pp.nq, 0.), rcache(std::max(pp.nv, pp.rs) * pp.nq, 0.), vertex(1 << pp.nd, size / pp.nd), h{size}, def{deformed}
+  : p{pp}, data((pp.nv + 7 + pp.rs) * pp.nq, 0.), rcache(std::max(pp.nv, pp.rs) * pp.nq, 0.), vertex(1 << pp.nd, size / pp.nd), h{size}, def{deformed}
   {
     for (int q = 0; q < p.nq; ++q) data[p.nv * p.nq + q] = 1.;
     if (def) {
@@ -331,8 +331,52 @@ static double rel_err(const std::vector<double>& a, const std::vector<double>& b
   return std::sqrt(num / den);
 }
 
+// face_permutation (kernels_convective.cpp:38-41; Spatial.hpp:73-131) on one face of nd + 2 variables
+static void face_permutation_checks()
+{
+  const int rs = 3, nfq = rs * rs, nv = 5;
+  std::vector<double> orig(nv * nfq);
+  for (int k = 0; k < nv * nfq; ++k) orig[k] = k;
+  // transpose only: dimensions {0, 2} with face signs {1, 1} flip no tangential axis
+  std::vector<double> d = orig;
+  auto fp = face_permutation(3, rs, Connection_direction{{0, 2}, {true, true}}, d.data());
+  fp->match_faces();
+  for (int v = 0; v < nv; ++v)
+    for (int a = 0; a < rs; ++a)
+      for (int b = 0; b < rs; ++b) REQUIRE(d[v * nfq + a * rs + b] == orig[v * nfq + b * rs + a]);
+  fp->restore();
+  REQUIRE(d == orig);
+  // every one of the 3D directions permutes each variable's face points and restores them
+  for (int i0 = 0; i0 < 3; ++i0)
+    for (int i1 = 0; i1 < 3; ++i1)
+      for (int s0 = 0; s0 < 2; ++s0)
+        for (int s1 = 0; s1 < 2; ++s1) {
+          if (i0 == i1 && s0 == s1) continue;
+          d = orig;
+          auto p = face_permutation(3, rs, Connection_direction{{i0, i1}, {bool(s0), bool(s1)}}, d.data());
+          p->match_faces();
+          for (int v = 0; v < nv; ++v) {
+            std::vector<double> got(d.begin() + v * nfq, d.begin() + (v + 1) * nfq), want(orig.begin() + v * nfq,
+                                                                                          orig.begin() + (v + 1) * nfq);
+            std::sort(got.begin(), got.end());
+            REQUIRE(got == want);
+          }
+          p->restore();
+          REQUIRE(d == orig);
+        }
+  // 2D: a flipped tangential axis reverses each variable's row (dimensions {0, 1}, signs {1, 0}: neither normal flips)
+  std::vector<double> e(4 * rs);
+  for (int k = 0; k < 4 * rs; ++k) e[k] = k;
+  face_permutation(2, rs, Connection_direction{{0, 1}, {true, false}}, e.data())->match_faces();
+  for (int v = 0; v < 4; ++v)
+    for (int i = 0; i < rs; ++i) REQUIRE(e[v * rs + i] == v * rs + rs - 1 - i);
+  REQUIRE(throws([&] { face_permutation(4, rs, Connection_direction{{0, 1}, {true, false}}, e.data()); }) ==
+          "demand for invalid kernel");
+}
+
 static void host_only_checks()
 {
+  face_permutation_checks();
   Gauss_legendre basis(4);
   // kernel_factory range check (kernel_factory.hpp:114-116)
   REQUIRE(throws([&] { kernel_factory<Spatial<pde::Navier_stokes<false>::Pde, false>::Local>(4, 4, basis, 1., 0, false, false); }) ==
@@ -445,6 +489,27 @@ static void compare_update(int nd, int rs, int n, bool def, bool visc, int bc_ki
               sw.children.at("neighbor").work_units_completed, sw.children.at("local").work_units_completed);
   REQUIRE(err <= 1e-12);
   REQUIRE(sw.children.at("local").work_units_completed == (long long)b.elems.size());
+
+  // the admissibility fix's diffusion through the free functions (kernels_max_dt.cpp:21, kernels_diffusive.cpp:32):
+  // local time step, ghost states copied, then one step with the device's own flux hook
+  am.push();
+  Kernel_options fo{sw, sw, sw, 0., 0};
+  const double fdt = max_dt_fix_therm_admis(km, fo, 0.7, 0.7, true);
+  hx_kernel_options ko{0.4, 0, 0, 0};
+  double odt = 0.;
+  REQUIRE(oracle_max_dt_fix_therm_admis(&od, &ko, 0.7, 0.7, 1, &odt) == 0);
+  REQUIRE(fdt == 1. && odt == 1.);
+  amd::check(hx_apply_avc_diff_bcs(am.device()), "apply_avc_diff_bcs");
+  REQUIRE(oracle_apply_avc_diff_bcs(&od) == 0);
+  fo.dt = 0.4;
+  compute_fix_therm_admis(km, fo, {});
+  REQUIRE(oracle_compute_fix_therm_admis(&od, &ko) == 0);
+  am.pull();
+  got.clear();
+  for (auto& e : b.elems) got.insert(got.end(), e->data.begin(), e->data.begin() + b.p.nv * b.p.nq);
+  err = rel_err(got, om.state);
+  std::printf("  compute_fix_therm_admis: relative L2 vs oracle %.3e\n", err);
+  REQUIRE(err <= 1e-12);
 }
 
 // Max_dt on sequences that were never registered (test_Max_dt.cpp:13-38)
