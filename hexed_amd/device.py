@@ -200,6 +200,34 @@ class Device_mesh:
     def apply_avc_diff_bcs(self):
         self.ctx.check(self.lib.hx_apply_avc_diff_bcs(self.ptr), "avc diff bcs")
 
+    # state files (Solver::write_state / read_state, Solver.cpp:232-271) -----------------------------------------
+    def _field_present(self, name, cls):
+        n, p = ctypes.c_size_t(), ctypes.c_void_p()
+        self.ctx.check(self.lib.hx_mesh_device_ptr(self.ptr, FIELDS[name], cls, ctypes.byref(p), ctypes.byref(n)),
+                       "device_ptr")
+        return bool(p.value)
+
+    def write_state(self, name):
+        """`name`.state.h5 from the device fields, in the reference's element storage layout."""
+        from hexed_amd import state_file
+        h = self.host
+        per_elem = dict(state=h.nv, tss=1, av=2, forcing=4, advection=h.row_size, rcache=h.nrc)
+
+        def get(field, c):
+            return self.download(field, c, out=np.zeros((h.n_elem[c], per_elem[field], h.nq)))
+        state_file.write(name, state_file.elements_of(h, get))
+
+    def read_state(self, name):
+        """The fields of `name`.state.h5 onto the device. An optional field (artificial viscosity, forcing, advection)
+        that the device does not hold and the file holds as zero stays absent."""
+        from hexed_amd import state_file
+
+        def store(field, c, arr):
+            if field in ("av", "forcing", "advection") and not arr.any() and not self._field_present(field, c):
+                return
+            self.upload(field, c, arr)
+        state_file.set_elements(self.host, state_file.read(name), store)
+
     # thermodynamic admissibility ------------------------------------------------------------------------------
     def is_admissible(self):
         out = ctypes.c_int()
