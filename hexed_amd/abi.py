@@ -129,6 +129,12 @@ def smooth_av_pde(diff_time, cheby_step):
     return p
 
 
+class hx_vertex_data(ctypes.Structure):
+    _fields_ = [("n_vertex", ctypes.c_int), ("vertex_id", ctypes.POINTER(ctypes.c_int) * 2),
+                ("position", ctypes.POINTER(ctypes.c_double)), ("node_adj", ctypes.POINTER(ctypes.c_double)),
+                ("n_hanging", ctypes.c_int), ("hanging", ctypes.POINTER(ctypes.c_int))]
+
+
 class hx_update_status(ctypes.Structure):
     _fields_ = [("time_step", ctypes.c_double), ("flow_time", ctypes.c_double), ("iteration", ctypes.c_int),
                 ("fix_admis_iters", ctypes.c_int)]
@@ -224,6 +230,7 @@ def load_library(path=LIB_PATH):
         "hx_max_dt_smooth_av": (ctypes.c_int, [P, ctypes.POINTER(hx_kernel_options), ctypes.c_double, ctypes.c_double,
                                                ctypes.c_int, c_double_p]),
         "hx_update_art_visc_smoothness": (ctypes.c_int, [P, ctypes.POINTER(hx_av_options), c_double_p]),
+        "hx_calc_jacobian": (ctypes.c_int, [P, ctypes.POINTER(hx_vertex_data)]),
         "hx_is_admissible": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int)]),
         "hx_fix_admissibility": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.c_double,
                                                 ctypes.POINTER(ctypes.c_int)]),
@@ -252,5 +259,5 @@ EXPORTED_SYMBOLS = [
     "hx_compute_advection", "hx_compute_smooth_av", "hx_compute_write_face_advection", "hx_compute_write_face_smooth_av",
     "hx_compute_prolong_advection", "hx_max_dt_advection", "hx_max_dt_smooth_av", "hx_update_art_visc_smoothness",
     "hx_mesh_set_cases", "hx_update_cases", "hx_is_admissible", "hx_fix_admissibility", "hx_compute_fix_therm_admis",
-    "hx_max_dt_fix_therm_admis",
+    "hx_max_dt_fix_therm_admis", "hx_calc_jacobian",
 ]

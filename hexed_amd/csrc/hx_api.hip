@@ -477,6 +477,65 @@ __global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__
 
 }  // namespace
 
+namespace {
+
+// vertex sharing by minimum (Solver.cpp:40-54 with Vertex::vector_min): each element copy pushes into its shared
+// vertex with a compare-and-swap minimum on the double, then every copy fetches the shared value
+__device__ inline void atomic_min_double(double* addr, double v)
+{
+  unsigned long long* a = reinterpret_cast<unsigned long long*>(addr);
+  unsigned long long old = *a;
+  while (v < __longlong_as_double(old)) {
+    const unsigned long long prev = atomicCAS(a, old, (unsigned long long)__double_as_longlong(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_vertex_push_min(const int* vid, const double* vtss, long long n, double* shared)
+{
+  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (gid < n) atomic_min_double(shared + vid[gid], vtss[gid]);
+}
+
+__global__ __launch_bounds__(256) void k_vertex_fetch(const int* vid, double* vtss, long long n, const double* shared)
+{
+  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (gid < n) vtss[gid] = shared[vid[gid]];
+}
+
+__global__ __launch_bounds__(256) void k_fill_n(double* p, double v, long long n)
+{
+  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (gid < n) p[gid] = v;
+}
+
+// device copies of a call's inputs, freed when the call returns
+struct Scratch
+{
+  std::vector<void*> ptrs;
+  ~Scratch()
+  {
+    // kernels queued before an early error return may still read these buffers
+    if (!ptrs.empty()) (void)hipDeviceSynchronize();
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t put(T** dst, const T* src, size_t count)
+  {
+    *dst = nullptr;
+    if (!count) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) return e;
+    ptrs.push_back(p);
+    *dst = static_cast<T*>(p);
+    return src ? hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice) : hipSuccess;
+  }
+};
+
+}  // namespace
+
 extern "C" {
 
 int hx_kernels_built(int n_dim, int row_size, int pde_kind)
@@ -1440,6 +1499,86 @@ static hx_status fta_bcs(hx_mesh* mesh, int mode)
 {
   Timer tm(mesh->ctx, "fix admis./boundary conditions", mesh->dm.n_bc_con);
   HX_TRY(mesh->ctx, mesh->ks->av_bcs(make_launch(mesh), mode), "fix admissibility bcs");
+  return HX_OK;
+}
+
+// ---- Solver::calc_jacobian (Solver.cpp:273-381) ---------------------------------------------------------------------
+
+hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
+{
+  if (!mesh || !v) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  Dev_mesh& dm = mesh->dm;
+  const int nd = dm.nd, rs = dm.rs, nvert = 1 << nd, nfq = dm.nfq;
+  const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1];
+  // every index the kernels follow is checked here
+  if (v->n_vertex < 0 || (n_tot && v->n_vertex == 0)) return fail(ctx, HX_ERR_INVALID, "vertex count missing");
+  for (int c = 0; c < 2; ++c) {
+    if (dm.n_elem[c] && !v->vertex_id[c]) return fail(ctx, HX_ERR_INVALID, "vertex ids missing");
+    for (long long k = 0; k < (long long)dm.n_elem[c] * nvert; ++k)
+      if (v->vertex_id[c][k] < 0 || v->vertex_id[c][k] >= v->n_vertex)
+        return fail(ctx, HX_ERR_INVALID, "vertex id out of range");
+  }
+  if (dm.n_elem[HX_DEFORMED] && !v->position) return fail(ctx, HX_ERR_INVALID, "vertex positions missing");
+  if (v->n_hanging < 0 || (v->n_hanging && !v->hanging)) return fail(ctx, HX_ERR_INVALID, "hanging vertices missing");
+  if (v->n_hanging && nd < 2) return fail(ctx, HX_ERR_INVALID, "hanging vertices need n_dim >= 2");
+  for (int i = 0; i < v->n_hanging; ++i) {
+    const int* h = v->hanging + 12 * i;
+    if (h[0] < 0 || h[0] >= nd || (h[1] != 0 && h[1] != 1)) return fail(ctx, HX_ERR_INVALID, "bad hanging-vertex face");
+    int n_fine = 1 << (nd - 1);
+    for (int d = 0; d < nd - 1; ++d)
+      if (h[2 + d]) n_fine /= 2;
+    for (int f = 0; f < n_fine; ++f) {
+      const int c = h[4 + 2 * f], e = h[5 + 2 * f];
+      if (c < 0 || c > 1 || e < 0 || e >= dm.n_elem[c]) return fail(ctx, HX_ERR_INVALID, "hanging-vertex element out of range");
+    }
+  }
+  ++mesh->gen;
+  Scratch sc;
+  int* vid = nullptr;
+  int* hang = nullptr;
+  double *pos = nullptr, *adj = nullptr, *vt = nullptr, *shared = nullptr, *vals = nullptr;
+  const long long n_def = dm.n_elem[HX_DEFORMED];
+  HX_TRY(ctx, sc.put(&vid, (const int*)nullptr, size_t(n_tot * nvert)), "hipMalloc(vertex ids)");
+  for (int c = 0, off = 0; c < 2; off += dm.n_elem[c] * nvert, ++c)
+    if (dm.n_elem[c])
+      HX_TRY(ctx, hipMemcpy(vid + off, v->vertex_id[c], size_t(dm.n_elem[c]) * nvert * sizeof(int), hipMemcpyHostToDevice),
+             "upload vertex ids");
+  HX_TRY(ctx, sc.put(&pos, v->position, size_t(n_def * nvert * nd)), "upload vertex positions");
+  if (v->node_adj) HX_TRY(ctx, sc.put(&adj, v->node_adj, size_t(n_def * 2 * nd * nfq)), "upload node adjustments");
+  HX_TRY(ctx, sc.put(&hang, v->hanging, size_t(12LL * v->n_hanging)), "upload hanging vertices");
+  HX_TRY(ctx, sc.put(&vt, (const double*)nullptr, size_t(n_tot * nvert)), "hipMalloc(vertex scale)");
+  HX_TRY(ctx, sc.put(&shared, (const double*)nullptr, size_t(v->n_vertex)), "hipMalloc(shared vertices)");
+  HX_TRY(ctx, sc.put(&vals, (const double*)nullptr, size_t(9LL * v->n_hanging)), "hipMalloc(hanging values)");
+  Launch l = make_launch(mesh);
+  {
+    Timer tm(ctx, "compute jacobian/elements", n_tot);
+    const int k = rs - 2;
+    HX_TRY(ctx, mesh->ks->set_jacobian(l, pos, adj, vt, hx_basis_tables::nodes[k], hx_basis_tables::diff_mats[k],
+                                       hx_basis_tables::boundarys[k]), "set_jacobian");
+  }
+  {
+    Timer tm(ctx, "compute jacobian/normals", dm.n_con[HX_DEFORMED]);
+    HX_TRY(ctx, mesh->ks->connection_normals(l), "connection normals");
+  }
+  {
+    Timer tm(ctx, "compute jacobian/vertex time step scale", n_tot);
+    const long long n = n_tot * nvert;
+    if (n) {
+      hipLaunchKernelGGL(k_fill_n, dim3(unsigned((v->n_vertex + 255) / 256)), dim3(256), 0, ctx->stream, shared,
+                         std::numeric_limits<double>::infinity(), (long long)v->n_vertex);
+      hipLaunchKernelGGL(k_vertex_push_min, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vt, n, shared);
+      hipLaunchKernelGGL(k_vertex_fetch, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vt, n, shared);
+      HX_TRY(ctx, hipGetLastError(), "vertex sharing");
+    }
+    HX_TRY(ctx, mesh->ks->hanging_match(l, hang, v->n_hanging, vt, vals, 0), "hanging vertices");
+    HX_TRY(ctx, mesh->ks->hanging_match(l, hang, v->n_hanging, vt, vals, 1), "hanging vertices");
+  }
+  for (int c = 0, off = 0; c < 2; off += dm.n_elem[c] * nvert, ++c)
+    if (dm.n_elem[c])
+      HX_TRY(ctx, hipMemcpyAsync(dm.vertex_tss[c], vt + off, size_t(dm.n_elem[c]) * nvert * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ctx->stream), "vertex scale copy");
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "calc_jacobian sync");
   return HX_OK;
 }
 

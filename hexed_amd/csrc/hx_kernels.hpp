@@ -1518,6 +1518,7 @@ __global__ __launch_bounds__(256) void k_flux_bcs(Dev_mesh m)
 }  // namespace hx
 
 #include "hx_av_kernels.hpp"
+#include "hx_metrics_kernels.hpp"
 
 namespace hx {
 
@@ -1758,6 +1759,42 @@ struct Launchers
                          dim3(256), 0, l.stream, l.m, flags);
     return hipGetLastError();
   }
+  // Solver::calc_jacobian pieces (hx_metrics_kernels.hpp)
+  static hipError_t set_jacobian(const Launch& l, const double* vert, const double* adj, double* vtss,
+                                 const double* node, const double* diff, const double* bound)
+  {
+    Metric_consts<RS> mc;
+    for (int i = 0; i < RS; ++i) {
+      mc.node[i] = node[i];
+      for (int j = 0; j < RS; ++j) mc.diff[i][j] = diff[i * RS + j];
+      mc.bound[0][i] = bound[i];
+      mc.bound[1][i] = bound[RS + i];
+    }
+    if (l.m.n_elem[HX_DEFORMED])
+      hipLaunchKernelGGL((k_set_jacobian<ND, RS>), dim3(blocks_for(l.m.n_elem[HX_DEFORMED], S::EPB)), dim3(S::BLOCK), 0,
+                         l.stream, l.m, mc, vert, adj, vtss);
+    if (l.m.n_elem[HX_CARTESIAN])
+      hipLaunchKernelGGL((k_cartesian_jacobian<ND, RS>), dim3(blocks_for(size_t(l.m.n_elem[HX_CARTESIAN]) * 2 * ND * S::NFQ, 256)),
+                         dim3(256), 0, l.stream, l.m, vtss);
+    return hipGetLastError();
+  }
+  static hipError_t connection_normals(const Launch& l)
+  {
+    if (l.m.n_bc_con)
+      hipLaunchKernelGGL((k_bc_copy_normal<ND, RS>), dim3(blocks_for(size_t(l.m.n_bc_con) * S::NFQ, 256)), dim3(256), 0,
+                         l.stream, l.m);
+    if (l.m.n_con[HX_DEFORMED])
+      hipLaunchKernelGGL((k_connection_normals<ND, RS>), dim3(blocks_for(size_t(l.m.n_con[HX_DEFORMED]) * S::NFQ, 256)),
+                         dim3(256), 0, l.stream, l.m);
+    return hipGetLastError();
+  }
+  static hipError_t hanging_match(const Launch& l, const int* hanging, int n_hanging, double* vtss, double* vals, int mode)
+  {
+    if (n_hanging == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_hanging_match<ND>), dim3(blocks_for(n_hanging, 256)), dim3(256), 0, l.stream, hanging,
+                       n_hanging, l.m.n_elem[HX_CARTESIAN], vtss, vals, mode);
+    return hipGetLastError();
+  }
   static hipError_t fix_av(const Launch& l, int mode)
   {
     for (int cls = 0; cls < 2; ++cls)
@@ -1798,7 +1835,7 @@ struct Launchers
                               &max_dt,      &prolong,       &restrict_, &state_bcs,          &flux_bcs,
                               &av_write_face, &av_neighbor, &adv_local, &diff_local,         &diff_neighbor_reconcile,
                               &diff_reconcile, &av_max_dt,  &av_bcs,    &av_points,          &max_dt_cases,
-                              &admissible,    &fix_av};
+                              &admissible,    &fix_av,      &set_jacobian, &connection_normals, &hanging_match};
     return &s;
   }
 };

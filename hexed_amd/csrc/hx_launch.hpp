@@ -51,6 +51,12 @@ struct Kernel_set
   // the artificial-viscosity swap of Solver::fix_admissibility
   hipError_t (*admissible)(const Launch&, int* flags);
   hipError_t (*fix_av)(const Launch&, int mode);
+  // Solver::calc_jacobian: element jacobians (vtss [n_cartesian + n_deformed][2^nd] receives each element's vertex
+  // time step scale), boundary and connection normals, hanging-vertex matching of vtss (mode 0 read, 1 write)
+  hipError_t (*set_jacobian)(const Launch&, const double* vert, const double* adj, double* vtss, const double* node,
+                             const double* diff, const double* bound);
+  hipError_t (*connection_normals)(const Launch&);
+  hipError_t (*hanging_match)(const Launch&, const int* hanging, int n_hanging, double* vtss, double* vals, int mode);
 };
 
 const Kernel_set* kernel_set_nd1(int rs);

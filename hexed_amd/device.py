@@ -200,6 +200,12 @@ class Device_mesh:
     def apply_avc_diff_bcs(self):
         self.ctx.check(self.lib.hx_apply_avc_diff_bcs(self.ptr), "avc diff bcs")
 
+    # mesh metrics (Solver::calc_jacobian, Solver.cpp:273-381) ---------------------------------------------------
+    def calc_jacobian(self, vertex_data=None):
+        """Metrics, normals and vertex time step scale on the device from the host mesh's vertex data."""
+        vd = vertex_data if vertex_data is not None else self.host.vertex_data()
+        self.ctx.check(self.lib.hx_calc_jacobian(self.ptr, ctypes.byref(vd)), "calc_jacobian")
+
     # state files (Solver::write_state / read_state, Solver.cpp:232-271) -----------------------------------------
     def _field_present(self, name, cls):
         n, p = ctypes.c_size_t(), ctypes.c_void_p()

@@ -103,6 +103,26 @@ class HostMesh:
         self.n_vertex = 0
         self.vertex_ids = [np.zeros((n, self.nvert), np.int32) for n in self.n_elem]
         self.hanging = np.zeros((0, 12), np.int32)
+        # vertex positions of the deformed elements [n_def][nvert][nd] (Vertex::pos) and their face node adjustments
+        # [n_def][2 nd][nfq] (Deformed_element::node_adjustments; zero: unwarped faces)
+        self.def_vertex_pos = np.zeros((n_def, self.nvert, nd))
+        self.node_adj = None
+
+    def vertex_data(self):
+        """hx_vertex_data over this mesh's arrays (the arrays must outlive the struct)."""
+        self._vd_arrays = [np.ascontiguousarray(self.vertex_ids[c], dtype=np.int32) for c in range(2)]
+        self._vd_hang = np.ascontiguousarray(self.hanging, dtype=np.int32)
+        self._vd_pos = np.ascontiguousarray(self.def_vertex_pos, dtype=np.float64)
+        self._vd_adj = None if self.node_adj is None else np.ascontiguousarray(self.node_adj, dtype=np.float64)
+        v = abi.hx_vertex_data()
+        v.n_vertex = int(self.n_vertex)
+        for c in range(2):
+            v.vertex_id[c] = abi.iptr(self._vd_arrays[c])
+        v.position = abi.dptr(self._vd_pos)
+        v.node_adj = abi.dptr(self._vd_adj) if self._vd_adj is not None else None
+        v.n_hanging = int(self._vd_hang.shape[0])
+        v.hanging = abi.iptr(self._vd_hang) if self._vd_hang.shape[0] else None
+        return v
 
     def enable_av_generator(self):
         """Allocate the arrays Solver::update_art_visc_smoothness works on (zeros, as Element.cpp:134-142 does)."""
@@ -216,9 +236,9 @@ def _cofactor(J, r, c):
     return J[r1][c1] * J[r2][c2] - J[r1][c2] * J[r2][c1]
 
 
-def deformed_metrics(nd, rs, vert_pos, nom_size):
-    """Deformed_element::set_jacobian restated (Deformed_element.cpp:60-136) for elements whose faces are not
-    warped (node adjustments zero), vectorised over elements.
+def deformed_metrics(nd, rs, vert_pos, nom_size, qpos=None):
+    """Deformed_element::set_jacobian restated (Deformed_element.cpp:60-136), vectorised over elements. The quadrature
+    point positions are the n-linear interpolation of the vertices (unwarped faces) unless `qpos` gives them.
 
     vert_pos: [n][nvert][nd]. Returns (metrics [n][nd*nd+1][nq], face_normals [n][2nd][nd][nfq],
     vertex_tss [n][nvert], qpos [n][nd][nq]).
@@ -226,8 +246,9 @@ def deformed_metrics(nd, rs, vert_pos, nom_size):
     b = basis(rs)
     n = vert_pos.shape[0]
     nq = rs ** nd
-    W = vertex_weights(nd, rs)                                    # [nq][nvert]
-    qpos = np.einsum("qv,nvd->ndq", W, vert_pos)                   # [n][nd][nq]
+    if qpos is None:
+        W = vertex_weights(nd, rs)                                # [nq][nvert]
+        qpos = np.einsum("qv,nvd->ndq", W, vert_pos)               # [n][nd][nq]
     shape = (n,) + (rs,) * nd
     J = [[None] * nd for _ in range(nd)]
     for i in range(nd):
@@ -340,6 +361,7 @@ def box(nd, rs, n, lo=0.0, hi=1.0, deformed=False, jitter=0.0, seed=0, bc=(abi.H
     corner = np.stack(np.unravel_index(np.arange(2 ** nd), (2,) * nd), axis=1)  # vertex index -> offsets
     vid = (grid[:, None, :] + corner[None, :, :]) @ vstrides                 # [n_e][nvert] global vertex ids
     if deformed:
+        m.def_vertex_pos[:] = vpos[vid]
         metrics, fn, vtss, qpos = deformed_metrics(nd, rs, vpos[vid], m.nom_size[1])
         m.metrics[:] = metrics
         m.qpos[1] = qpos
@@ -438,6 +460,7 @@ def concat(meshes):
             if c == 1:
                 out.con_nslot[c_off[1]:c_off[1] + nc] = shifted(m.con_nslot, ns_off)
                 out.metrics[sl] = m.metrics
+                out.def_vertex_pos[sl] = m.def_vertex_pos
                 out.face_nslot[sl] = shifted(m.face_nslot, ns_off)
         out.faces[s_off:s_off + m.faces.shape[0]] = m.faces
         out.normals[ns_off:ns_off + m.normals.shape[0]] = m.normals
@@ -613,6 +636,7 @@ class Builder:
         car_ids = [i for i in range(len(self.elems)) if cls_of[i] == 0]
         fn = None
         if def_ids:
+            m.def_vertex_pos[:] = vpos[vid[def_ids]]
             metrics, fn, vtss_def, qpos = deformed_metrics(nd, rs, vpos[vid[def_ids]], m.nom_size[1])
             m.metrics[:] = metrics
             m.qpos[1] = qpos

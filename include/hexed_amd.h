@@ -164,6 +164,19 @@ typedef struct hx_host_mesh {
                              conditions); may be NULL (the device then keeps its own, zero-initialised) */
 } hx_host_mesh;
 
+/* Vertex data of a mesh, which hx_host_mesh does not carry: what Solver::calc_jacobian needs besides the mesh
+ * (Solver.cpp:273-381, Deformed_element.cpp:15-136, Hanging_vertex_matcher.hpp:14-33). */
+typedef struct hx_vertex_data {
+  int n_vertex;                /* number of shared vertices                                                     */
+  const int* vertex_id[2];     /* per class [n][2^nd]: the shared vertex of each element vertex, in [0, n_vertex) */
+  const double* position;      /* [n_deformed][2^nd][nd] vertex positions of the deformed elements (Vertex::pos)  */
+  const double* node_adj;      /* [n_deformed][2 nd][nfq] face node adjustments (Deformed_element::node_adjustments),
+                                  [face 2 i_dim + sign][face point]; NULL: all zero (unwarped faces)          */
+  int n_hanging;               /* hanging-vertex matchers, one per hanging face                                 */
+  const int* hanging;          /* [n_hanging][12]: i_dim, is_positive, stretch[2], then (class, element) of the
+                                  2^(nd-1) / (1 + stretch...) fine elements in the matcher's order              */
+} hx_vertex_data;
+
 /* Fields that move between host and device (hx_mesh_upload / hx_mesh_download). */
 typedef enum hx_field {
   HX_FIELD_STATE = 0,
@@ -336,6 +349,18 @@ hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset);
 hx_status hx_update(hx_mesh* mesh, const hx_update_options* opts, hx_update_status* status);
 /* Solver::compute_residual (Solver.cpp:899-913): writes the scaled residual into the residual cache. */
 hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* opts);
+
+/* ---- mesh metrics ------------------------------------------------------------------------------------------------ */
+/* Solver::calc_jacobian (Solver.cpp:273-381) without snapping: Element::set_jacobian (Element.cpp:99-112) writes unit
+ * face normals of the Cartesian elements and Deformed_element::set_jacobian (Deformed_element.cpp:60-136) the metrics,
+ * face normals and vertex time step scale of the deformed ones, from vertex positions (Deformed_element::position,
+ * Deformed_element.cpp:15-58); boundary ghost faces copy the inside normal; every deformed connection gets the
+ * sign-corrected average of its two element face normals (in normal slots con_nslot, side 1 only when it has a slot
+ * of its own); the vertex time step scale is shared by minimum over the elements of each vertex, hanging vertices
+ * matched. Cartesian elements start from nominal size / n_dim (Element.cpp:17). The face state blocks hold normals
+ * afterwards, as in the reference: write the faces (hx_compute_write_face) before stepping. Hanging faces between
+ * deformed elements are not supported (their normals are left as they are). */
+hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* vertices);
 
 /* ---- thermodynamic admissibility (Solver.cpp:921-1095; HX_PDE_FIX_ADMIS also runs through the per-kernel entry
  * points above, without compute_residual and the filter) --------------------------------------------------------- */

@@ -53,6 +53,7 @@ def load():
         "oracle_compute_smooth_av": (I, [M, KO, D, D]),
         "oracle_update_art_visc_smoothness": (I, [M, ctypes.POINTER(abi.hx_av_options), ctypes.POINTER(D)]),
         "oracle_set_vertices": (I, [M, I, ctypes.POINTER(I), ctypes.POINTER(I), I, ctypes.POINTER(I)]),
+        "oracle_calc_jacobian": (I, [M, ctypes.POINTER(abi.hx_vertex_data)]),
         "oracle_is_admissible": (I, [M, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "oracle_fix_coefficient": (I, [M, ctypes.POINTER(I), ctypes.POINTER(D)]),
         "oracle_compute_fix_therm_admis": (I, [M, KO]),

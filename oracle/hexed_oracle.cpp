@@ -1600,6 +1600,17 @@ struct Run_sav_reconcile
   }
 };
 template <int nd, int rs>
+struct Run_face_permutation
+{
+  static int run(Dir dir, double* data, int n_var, bool match)
+  {
+    Face_permutation<nd, rs> fp{dir, data, n_var};
+    if (match) fp.match_faces();
+    else fp.restore();
+    return 0;
+  }
+};
+template <int nd, int rs>
 struct Run_fta_local { static int run(hx_host_mesh* m, int cls, double dt) { Av<nd, rs>::fta_local(m, cls, dt); return 0; } };
 template <int nd, int rs>
 struct Run_fta_reconcile { static int run(hx_host_mesh* m, int cls, double dt) { Av<nd, rs>::fta_reconcile(m, cls, dt); return 0; } };
@@ -2080,15 +2091,17 @@ int stretched_ind(int nd, int ind, const int* stretch)
   return stretched;
 }
 
-// Solver::share_vertex_data with Vertex::vector_max (Solver.cpp:40-54, Vertex.cpp:6-9): every element's copy of a
-// vertex value becomes the maximum over the elements sharing the vertex; then Hanging_vertex_matcher::match
-// (Hanging_vertex_matcher.cpp:13-41) interpolates the hanging vertices of every hanging face from its corners.
-void share_vertex_max(const hx_host_mesh* m, std::vector<double>& vc)
+// Solver::share_vertex_data with Vertex::vector_max or vector_min (Solver.cpp:40-54, Vertex.cpp:6-14): every
+// element's copy of a vertex value becomes the maximum (minimum) over the elements sharing the vertex; then
+// Hanging_vertex_matcher::match (Hanging_vertex_matcher.cpp:13-41) interpolates the hanging vertices of every hanging
+// face from its corners.
+void share_vertex(const hx_host_mesh* m, const Vertex_data& V, std::vector<double>& vc, bool minimum)
 {
   const int nd = m->n_dim, nvert = 1 << nd;
-  const Vertex_data& V = g_vertices;
-  std::vector<double> shared(V.n_vertex, -std::numeric_limits<double>::infinity());
-  for (size_t k = 0; k < V.vid.size(); ++k) shared[V.vid[k]] = std::max(shared[V.vid[k]], vc[k]);
+  const double inf = std::numeric_limits<double>::infinity();
+  std::vector<double> shared(V.n_vertex, minimum ? inf : -inf);
+  for (size_t k = 0; k < V.vid.size(); ++k)
+    shared[V.vid[k]] = minimum ? std::min(shared[V.vid[k]], vc[k]) : std::max(shared[V.vid[k]], vc[k]);
   for (size_t k = 0; k < V.vid.size(); ++k) vc[k] = shared[V.vid[k]];
   const std::vector<double> interp_mat{1., 0., .5, .5, 0., 1.};  // 3 x 2
   const size_t n_match = V.hanging.size() / 12;
@@ -2131,13 +2144,13 @@ std::vector<double> fix_coefficient(hx_host_mesh* m, const std::vector<int>& rec
   std::vector<double> vc(n_tot * nvert);
   for (size_t E = 0; E < n_tot; ++E)
     for (int v = 0; v < nvert; ++v) vc[E * nvert + v] = record[E];
-  share_vertex_max(m, vc);
+  share_vertex(m, V, vc, false);
   for (size_t E = 0; E < n_tot; ++E) {
     double max_fac = 0;
     for (int v = 0; v < nvert; ++v) max_fac = std::max(max_fac, vc[E * nvert + v]);
     for (int v = 0; v < nvert; ++v) vc[E * nvert + v] = max_fac;
   }
-  share_vertex_max(m, vc);
+  share_vertex(m, V, vc, false);
   Basis basis(rs);
   std::vector<double> interp(size_t(rs) * 2);
   for (int i = 0; i < rs; ++i) {
@@ -2152,6 +2165,203 @@ std::vector<double> fix_coefficient(hx_host_mesh* m, const std::vector<int>& rec
       std::copy(lap.begin(), lap.end(), m->av[c] + (size_t(e) * 2 + 1) * nq);
     }
   return vc;
+}
+
+// ---- mesh metrics (Solver::calc_jacobian, Solver.cpp:273-381) ------------------------------------------------------
+
+// math::dimension_matvec (math.cpp:58-84): `mat` (rows x cols) applied along dimension i_dim (0 slowest) of a
+// hypercubic array of side cols
+std::vector<double> dimension_matvec(const double* mat, int rows, int cols, const std::vector<double>& vec, int i_dim)
+{
+  const int n_rows = ipow(cols, i_dim + 1);
+  const int n_cols = int(vec.size()) / n_rows;
+  std::vector<double> prod(vec.size() * rows / cols);
+  for (int i_outer = 0; i_outer < n_rows / cols; ++i_outer)
+    for (int r = 0; r < rows; ++r)
+      for (int j = 0; j < n_cols; ++j) {
+        double acc = 0.;
+        for (int c = 0; c < cols; ++c) acc += mat[r * cols + c] * vec[(size_t(i_outer) * cols + c) * n_cols + j];
+        prod[(size_t(i_outer) * rows + r) * n_cols + j] = acc;
+      }
+  return prod;
+}
+
+// determinant of an n x n row-major matrix, n <= 3 (cofactor expansion along the first row)
+double determinant(const double* a, int n)
+{
+  if (n == 1) return a[0];
+  if (n == 2) return a[0] * a[3] - a[1] * a[2];
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+// Deformed_element::position (Deformed_element.cpp:15-58): n-linear interpolation of the vertex positions plus, along
+// each reference direction, the face node adjustments interpolated between the two faces times the element's edge
+// vector in that direction. vert [nvert][nd]; adj [2 nd][nfq] or null
+void deformed_position(int nd, int rs, const Basis& basis, const double* vert, const double* adj, int q, double* pos)
+{
+  const int nvert = 1 << nd, nq = ipow(rs, nd), nfq = nq / rs;
+  for (int i_dim = 0; i_dim < nd; ++i_dim) {
+    std::vector<double> vp(nvert);
+    for (int v = 0; v < nvert; ++v) vp[v] = vert[v * nd + i_dim];
+    std::vector<std::vector<double>> adjustments(nd, vp);
+    for (int j_dim = nd - 1; j_dim >= 0; --j_dim) {
+      const int stride = ipow(rs, nd - j_dim - 1);
+      const double node = basis.node[(q / stride) % rs];
+      const double interp[2] = {1. - node, node};
+      vp = dimension_matvec(interp, 1, 2, vp, j_dim);
+      int i_face_qpoint = 0;
+      for (int k_dim = 0, face_stride = nq / rs / rs; k_dim < nd; ++k_dim)
+        if (k_dim != j_dim) {
+          i_face_qpoint += ((q / ipow(rs, nd - k_dim - 1)) % rs) * face_stride;
+          face_stride /= rs;
+        }
+      for (int k_dim = 0; k_dim < nd; ++k_dim) {
+        double contract[2] = {interp[0], interp[1]};
+        if (j_dim == k_dim) {
+          // adjust.cwiseProduct(interp) * [[-1, 1], [-1, 1]]
+          const int i_adjust = 2 * j_dim * nfq + i_face_qpoint;
+          const double p0 = (adj ? adj[i_adjust] : 0.) * interp[0], p1 = (adj ? adj[i_adjust + nfq] : 0.) * interp[1];
+          contract[0] = -p0 - p1;
+          contract[1] = p0 + p1;
+        }
+        adjustments[k_dim] = dimension_matvec(contract, 1, 2, adjustments[k_dim], j_dim);
+      }
+    }
+    pos[i_dim] = vp[0];
+    for (int j_dim = 0; j_dim < nd; ++j_dim) pos[i_dim] += adjustments[j_dim][0];
+  }
+}
+
+// Deformed_element::set_jacobian (Deformed_element.cpp:60-136) of deformed element e: metrics, the element's face
+// normals (into its face state blocks) and its vertex time step scale (vtss [nvert])
+void deformed_set_jacobian(hx_host_mesh* m, const Mesh_view& mv, const Basis& basis, int e, const double* vert,
+                           const double* adj, double* vtss)
+{
+  const int nd = m->n_dim, rs = m->row_size, nq = ipow(rs, nd), nfq = nq / rs, nvert = 1 << nd;
+  const double nom_sz = m->nom_size[HX_DEFORMED][e];
+  std::vector<double> jac(size_t(nd) * nd * nq);
+  std::vector<std::vector<double>> pos(nd, std::vector<double>(nq));
+  for (int q = 0; q < nq; ++q) {
+    double p[3];
+    deformed_position(nd, rs, basis, vert, adj, q, p);
+    for (int i = 0; i < nd; ++i) pos[i][q] = p[i];
+  }
+  for (int i = 0; i < nd; ++i)
+    for (int j = 0; j < nd; ++j) {
+      const std::vector<double> d = dimension_matvec(basis.diff, rs, rs, pos[i], j);
+      for (int q = 0; q < nq; ++q) jac[(size_t(i) * nd + j) * nq + q] = d[q] / nom_sz;
+    }
+  double* jac_dat = m->metrics + size_t(e) * (nd * nd + 1) * nq;
+  for (int q = 0; q < nq; ++q) {
+    double qj[9];
+    for (int i = 0; i < nd; ++i)
+      for (int j = 0; j < nd; ++j) qj[i * nd + j] = jac[(size_t(i) * nd + j) * nq + q];
+    jac_dat[size_t(nd) * nd * nq + q] = determinant(qj, nd);
+    for (int i = 0; i < nd; ++i)
+      for (int j = 0; j < nd; ++j) {
+        double copy[9];
+        std::copy(qj, qj + nd * nd, copy);
+        for (int r = 0; r < nd; ++r) copy[r * nd + i] = r == j;
+        jac_dat[(size_t(i) * nd + j) * nq + q] = determinant(copy, nd);
+      }
+  }
+  for (int i_dim = 0; i_dim < nd; ++i_dim)
+    for (int sign = 0; sign < 2; ++sign) {
+      double* face_data = mv.slot(m->face_slot[HX_DEFORMED][e * 2 * nd + 2 * i_dim + sign]);
+      std::vector<std::vector<double>> face_jac(nd * nd);
+      for (int i_jac = 0; i_jac < nd * nd; ++i_jac)
+        face_jac[i_jac] = dimension_matvec(basis.bound + sign * rs, 1, rs,
+                                           std::vector<double>(jac.begin() + size_t(i_jac) * nq,
+                                                               jac.begin() + size_t(i_jac + 1) * nq), i_dim);
+      for (int fq = 0; fq < nfq; ++fq) {
+        double qj[9];
+        for (int j = 0; j < nd; ++j)
+          for (int k = 0; k < nd; ++k) qj[j * nd + k] = face_jac[j * nd + k][fq];
+        for (int j = 0; j < nd; ++j) {
+          for (int r = 0; r < nd; ++r) qj[r * nd + i_dim] = r == j;
+          face_data[j * nfq + fq] = determinant(qj, nd);
+        }
+      }
+    }
+  std::vector<double> bound2(basis.bound, basis.bound + 2 * rs);
+  std::vector<std::vector<double>> vertex_nrml(nd * nd);
+  for (int i_jac = 0; i_jac < nd * nd; ++i_jac)
+    vertex_nrml[i_jac] = hypercube_matvec(bound2, 2, rs, std::vector<double>(jac_dat + size_t(i_jac) * nq,
+                                                                              jac_dat + size_t(i_jac + 1) * nq));
+  const std::vector<double> vertex_det =
+      hypercube_matvec(bound2, 2, rs, std::vector<double>(jac_dat + size_t(nd) * nd * nq, jac_dat + size_t(nd * nd + 1) * nq));
+  for (int v = 0; v < nvert; ++v) {
+    double norm_sum = 0.;
+    for (int i = 0; i < nd; ++i) {
+      double norm_sq = 0.;
+      for (int j = 0; j < nd; ++j) norm_sq += vertex_nrml[i * nd + j][v] * vertex_nrml[i * nd + j][v];
+      norm_sum += std::sqrt(norm_sq);
+    }
+    vtss[v] = nom_sz * vertex_det[v] / norm_sum;
+  }
+}
+
+// Solver::calc_jacobian (Solver.cpp:273-381) on hx_host_mesh, without snapping or deformed hanging faces
+void calc_jacobian(hx_host_mesh* m, const hx_vertex_data* vd)
+{
+  const int nd = m->n_dim, rs = m->row_size, nq = ipow(rs, nd), nfq = nq / rs, nvert = 1 << nd;
+  Basis basis(rs);
+  Mesh_view mv{m};
+  Vertex_data V;
+  V.n_vertex = vd->n_vertex;
+  V.n_elem[0] = m->n_elem[0];
+  V.n_elem[1] = m->n_elem[1];
+  for (int c = 0; c < 2; ++c)
+    for (long long k = 0; k < (long long)m->n_elem[c] * nvert; ++k) {
+      const int id = vd->vertex_id[c][k];
+      if (id < 0 || id >= vd->n_vertex) throw std::runtime_error("oracle: vertex id out of range");
+      V.vid.push_back(id);
+    }
+  V.hanging.assign(vd->hanging, vd->hanging + 12LL * vd->n_hanging);
+  // element jacobians, Cartesian then deformed (Element.cpp:99-112, Deformed_element.cpp:60-136)
+  std::vector<double> vtss(size_t(m->n_elem[0] + m->n_elem[1]) * nvert);
+  for (int e = 0; e < m->n_elem[HX_CARTESIAN]; ++e) {
+    for (int f = 0; f < 2 * nd; ++f) {
+      double* face = mv.slot(m->face_slot[HX_CARTESIAN][e * 2 * nd + f]);
+      for (int j = 0; j < nd; ++j)
+        for (int q = 0; q < nfq; ++q) face[j * nfq + q] = f / 2 == j;
+    }
+    for (int v = 0; v < nvert; ++v) vtss[size_t(e) * nvert + v] = m->nom_size[HX_CARTESIAN][e] / nd;  // Element.cpp:17
+  }
+  for (int e = 0; e < m->n_elem[HX_DEFORMED]; ++e)
+    deformed_set_jacobian(m, mv, basis, e, vd->position + size_t(e) * nvert * nd,
+                          vd->node_adj ? vd->node_adj + size_t(e) * 2 * nd * nfq : nullptr,
+                          vtss.data() + (size_t(m->n_elem[0]) + e) * nvert);
+  // boundary ghost faces copy the inside normal
+  for (int i = 0; i < m->n_bc_con; ++i) {
+    const int con = m->bc_con[i];
+    const double* in_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con]);
+    double* gh_f = mv.slot(m->con_slot[HX_DEFORMED][2 * con + 1]);
+    for (int k = 0; k < nd * nfq; ++k) gh_f[k] = in_f[k];
+  }
+  // the shared face normal of every deformed connection (boundary connections included)
+  for (int con = 0; con < m->n_con[HX_DEFORMED]; ++con) {
+    Dir dir = unpack_dir(m->con_dir[HX_DEFORMED][con]);
+    double* elem_nrml[2] = {mv.slot(m->con_slot[HX_DEFORMED][2 * con]), mv.slot(m->con_slot[HX_DEFORMED][2 * con + 1])};
+    // face_permutation (kernels_convective.cpp:38-41) of the nd normal components of face 1
+    dispatch<Run_face_permutation>(nd, rs, dir, elem_nrml[1], nd, true);
+    int sign[2];
+    for (int s_ = 0; s_ < 2; ++s_) sign[s_] = 1 - 2 * dir.flip_normal(s_);
+    for (int k = 0; k < nd * nfq; ++k) {
+      double n = 0.;
+      for (int s_ = 0; s_ < 2; ++s_) n += .5 * sign[s_] * elem_nrml[s_][k];
+      for (int s_ = 0; s_ < 2; ++s_) elem_nrml[s_][k] = sign[s_] * n;
+    }
+    dispatch<Run_face_permutation>(nd, rs, dir, elem_nrml[1], nd, false);
+    const int ns0 = m->con_nslot[2 * con], ns1 = m->con_nslot[2 * con + 1];
+    std::copy(elem_nrml[0], elem_nrml[0] + nd * nfq, mv.nslot(ns0));
+    if (ns1 != ns0) std::copy(elem_nrml[1], elem_nrml[1] + nd * nfq, mv.nslot(ns1));
+  }
+  share_vertex(m, V, vtss, true);
+  for (int c = 0; c < 2; ++c)
+    for (int e = 0; e < m->n_elem[c]; ++e)
+      for (int v = 0; v < nvert; ++v)
+        m->vertex_tss[c][size_t(e) * nvert + v] = vtss[((c ? m->n_elem[0] : 0) + size_t(e)) * nvert + v];
 }
 
 void swap_av(hx_host_mesh* m)
@@ -2455,6 +2665,15 @@ int oracle_set_vertices(const hx_host_mesh* m, int n_vertex, const int* vid_cart
     }
     v.set = true;
     g_vertices = std::move(v);
+  });
+}
+
+int oracle_calc_jacobian(hx_host_mesh* m, const hx_vertex_data* vertices)
+{
+  return guarded([&] {
+    check_mesh(m);
+    if (m->n_elem[HX_DEFORMED] && !vertices->position) throw std::runtime_error("oracle: vertex positions missing");
+    calc_jacobian(m, vertices);
   });
 }
 

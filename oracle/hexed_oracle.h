@@ -60,6 +60,7 @@ int oracle_update_art_visc_smoothness(hx_host_mesh* m, const hx_av_options* o, d
  * i_dim, is_positive, stretch[2], then (class, element) of each fine element in the hanging face's order. */
 int oracle_set_vertices(const hx_host_mesh* m, int n_vertex, const int* vid_cartesian, const int* vid_deformed,
                         int n_hanging, const int* hanging);
+int oracle_calc_jacobian(hx_host_mesh* m, const hx_vertex_data* vertices);              /* Solver.cpp:273-381  */
 int oracle_is_admissible(hx_host_mesh* m, int* admissible, int* record);                   /* Solver.cpp:921-959  */
 int oracle_fix_coefficient(hx_host_mesh* m, const int* record, double* vertex_out);        /* Solver.cpp:999-1030 */
 int oracle_compute_fix_therm_admis(hx_host_mesh* m, const hx_kernel_options* o);          /* kernels_diffusive.cpp:32 */
