@@ -22,6 +22,22 @@ class Spatial
   public:
   Spatial() = delete;
 
+  //! reorders a connection's face-1 data (Pde::n_extrap variables, host memory) to match face 0 (Spatial.hpp:73-131)
+  template <int n_dim, int row_size>
+  class Face_permutation : public Face_permutation_dynamic
+  {
+    std::unique_ptr<Face_permutation_dynamic> impl;
+
+    public:
+    typedef std::unique_ptr<Face_permutation_dynamic> ptr_t;
+    Face_permutation(Connection_direction direction, double* target)
+    : impl{row_size < 2 ? nullptr
+                        : amd::face_permutation(n_dim, row_size, direction, target, Pde_templ<n_dim, row_size>::n_extrap)}
+    {}
+    void match_faces() override { impl->match_faces(); }
+    void restore() override { impl->restore(); }
+  };
+
   template <int n_dim, int row_size>
   class Write_face : public Kernel<Kernel_element&>
   {

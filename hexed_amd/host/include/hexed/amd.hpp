@@ -17,6 +17,8 @@
 #include <vector>
 
 #include "../../../../include/hexed_amd.h"
+#include "Face_permutation.hpp"
+#include "Kernel_connection.hpp"
 #include "Kernel_mesh.hpp"
 
 namespace hexed::amd
@@ -92,6 +94,11 @@ class Mesh_lease
   hx_mesh* device() { return am->device(); }
   int cls() const { return c; }
 };
+
+//! host reordering of `n_var` variables of face-1 data to match face 0 (Spatial.hpp:73-131), the index map the
+//! device's Neighbor uses
+std::unique_ptr<Face_permutation_dynamic> face_permutation(int n_dim, int row_size, Connection_direction dir,
+                                                           double* data, int n_var);
 
 //! Solver::update (Solver.cpp:834-886) on a registered mesh
 void update(Accelerated_mesh& mesh, const hx_update_options& opts, hx_update_status& status);

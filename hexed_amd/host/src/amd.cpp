@@ -414,6 +414,60 @@ Mesh_lease::~Mesh_lease()
   }
 }
 
+namespace
+{
+// the face-1 index matching face-0 index i (Spatial.hpp:85-129: transpose, then flip)
+int face1_index(int nd, int rs, Connection_direction dir, int i)
+{
+  if (nd == 3) {
+    int c = i / rs, r = i % rs;
+    const bool tr = dir.transpose();
+    if (dir.flip_tangential()) {
+      const bool colwise = (dir.i_dim[0] > 3 - dir.i_dim[0] - dir.i_dim[1]) != tr;
+      if (colwise) r = rs - 1 - r;
+      else c = rs - 1 - c;
+    }
+    return tr ? r * rs + c : c * rs + r;
+  }
+  if (nd == 2) return dir.flip_tangential() ? rs - 1 - i : i;
+  return i;
+}
+
+class Face_perm : public Face_permutation_dynamic
+{
+  int nd, rs, nfq, n_var;
+  Connection_direction dir;
+  double* data;
+
+  public:
+  Face_perm(int n_dim, int row_size, Connection_direction d, double* target, int nv)
+  : nd{n_dim}, rs{row_size}, nfq{1}, n_var{nv}, dir{d}, data{target}
+  {
+    for (int i = 1; i < nd; ++i) nfq *= rs;
+  }
+  void match_faces() override
+  {
+    std::vector<double> old(data, data + n_var * nfq);
+    for (int v = 0; v < n_var; ++v)
+      for (int i = 0; i < nfq; ++i) data[v * nfq + i] = old[v * nfq + face1_index(nd, rs, dir, i)];
+  }
+  void restore() override
+  {
+    std::vector<double> cur(data, data + n_var * nfq);
+    for (int v = 0; v < n_var; ++v)
+      for (int i = 0; i < nfq; ++i) data[v * nfq + face1_index(nd, rs, dir, i)] = cur[v * nfq + i];
+  }
+};
+}  // namespace
+
+std::unique_ptr<Face_permutation_dynamic> face_permutation(int n_dim, int row_size, Connection_direction dir,
+                                                           double* data, int n_var)
+{
+  if (n_dim < 1 || n_dim > HX_MAX_N_DIM || row_size < 2 || row_size > HX_MAX_ROW_SIZE)
+    throw std::runtime_error("demand for invalid kernel");
+  return std::unique_ptr<Face_permutation_dynamic>(new Face_perm(n_dim, row_size, dir, data, n_var));
+}
+
 void update(Accelerated_mesh& mesh, const hx_update_options& opts, hx_update_status& status)
 {
   check(hx_update(mesh.device(), &opts, &status), "Solver::update");
@@ -545,48 +599,6 @@ void diffusion_with_host_bc(Kernel_mesh& mesh, amd::Mesh_lease& m, const hx_kern
   amd::check(hx_prolong_refined(d, &p, 0, 0), "prolong");
 }
 
-// the face-1 index matching face-0 index i (Spatial.hpp:85-129: transpose, then flip)
-int face1_index(int nd, int rs, Connection_direction dir, int i)
-{
-  if (nd == 3) {
-    int c = i / rs, r = i % rs;
-    const bool tr = dir.transpose();
-    if (dir.flip_tangential()) {
-      const bool colwise = (dir.i_dim[0] > 3 - dir.i_dim[0] - dir.i_dim[1]) != tr;
-      if (colwise) r = rs - 1 - r;
-      else c = rs - 1 - c;
-    }
-    return tr ? r * rs + c : c * rs + r;
-  }
-  if (nd == 2) return dir.flip_tangential() ? rs - 1 - i : i;
-  return i;
-}
-
-class Face_perm : public Face_permutation_dynamic
-{
-  int nd, rs, nfq;
-  Connection_direction dir;
-  double* data;
-
-  public:
-  Face_perm(int n_dim, int row_size, Connection_direction d, double* target)
-  : nd{n_dim}, rs{row_size}, nfq{1}, dir{d}, data{target}
-  {
-    for (int i = 1; i < nd; ++i) nfq *= rs;
-  }
-  void match_faces() override
-  {
-    std::vector<double> old(data, data + (nd + 2) * nfq);
-    for (int v = 0; v < nd + 2; ++v)
-      for (int i = 0; i < nfq; ++i) data[v * nfq + i] = old[v * nfq + face1_index(nd, rs, dir, i)];
-  }
-  void restore() override
-  {
-    std::vector<double> cur(data, data + (nd + 2) * nfq);
-    for (int v = 0; v < nd + 2; ++v)
-      for (int i = 0; i < nfq; ++i) data[v * nfq + face1_index(nd, rs, dir, i)] = cur[v * nfq + i];
-  }
-};
 }  // namespace
 
 // compute_navier_stokes (kernels_diffusive.cpp:8-29). A host flux_bc callback needs the face data on the host, so
@@ -689,9 +701,7 @@ void compute_write_face_smooth_av(Kernel_mesh mesh)
 
 std::unique_ptr<Face_permutation_dynamic> face_permutation(int n_dim, int row_size, Connection_direction dir, double* data)
 {
-  if (n_dim < 1 || n_dim > HX_MAX_N_DIM || row_size < 2 || row_size > HX_MAX_ROW_SIZE)
-    throw std::runtime_error("demand for invalid kernel");
-  return std::unique_ptr<Face_permutation_dynamic>(new Face_perm(n_dim, row_size, dir, data));
+  return amd::face_permutation(n_dim, row_size, dir, data, n_dim + 2);  // Navier_stokes<false>'s n_extrap
 }
 
 // max_dt_euler / max_dt_navier_stokes (kernels_max_dt.cpp:8-17)

@@ -372,6 +372,21 @@ static void face_permutation_checks()
     for (int i = 0; i < rs; ++i) REQUIRE(e[v * rs + i] == v * rs + rs - 1 - i);
   REQUIRE(throws([&] { face_permutation(4, rs, Connection_direction{{0, 1}, {true, false}}, e.data()); }) ==
           "demand for invalid kernel");
+  // the kernel class behind it (kernels_convective.cpp:38-41): the same reordering of Pde::n_extrap variables
+  const Connection_direction dir{{2, 0}, {false, false}};
+  std::vector<double> a = orig, c = orig;
+  face_permutation(3, rs, dir, a.data())->match_faces();
+  kernel_factory<Spatial<pde::Navier_stokes<false>::Pde, true>::Face_permutation>(3, rs, dir, c.data())->match_faces();
+  REQUIRE(a == c);
+  std::vector<double> adv(size_t(3 + rs) * nfq), adv0;
+  for (size_t k = 0; k < adv.size(); ++k) adv[k] = double(k);
+  adv0 = adv;
+  auto fa = kernel_factory<Spatial<pde::Advection, true>::Face_permutation>(3, rs, dir, adv.data());
+  fa->match_faces();
+  for (int v = 0; v < 3 + rs; ++v)
+    for (int i = 0; i < nfq; ++i) REQUIRE(adv[v * nfq + i] - v * nfq == c[i] - 0.);  // every variable alike
+  fa->restore();
+  REQUIRE(adv == adv0);
 }
 
 static void host_only_checks()
