@@ -284,9 +284,10 @@ std::string validate(const hx_host_mesh* d, const Sizes& z)
   return "";
 }
 
-Launch make_launch(hx_mesh* mesh)
+// `writes`: the launch may change mesh data (every launch except read-only checks), which retires a fused Max_dt
+Launch make_launch(hx_mesh* mesh, bool writes = true)
 {
-  ++mesh->gen;
+  if (writes) ++mesh->gen;
   Launch l{};
   l.stream = mesh->ctx->stream;
   l.m = mesh->dm;
@@ -1628,14 +1629,14 @@ hx_status hx_is_admissible(hx_mesh* mesh, int* admissible)
 {
   if (!mesh || !admissible) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
-  ++mesh->gen;
+  // reads the fields only, so a fused Max_dt stays valid
   if (!mesh->admis_flags) {
     void* p = nullptr;
     HX_TRY(ctx, hipMalloc(&p, 2 * sizeof(int)), "hipMalloc(admissibility flags)");
     mesh->allocs.push_back(p);
     mesh->admis_flags = static_cast<int*>(p);
   }
-  Launch l = make_launch(mesh);
+  Launch l = make_launch(mesh, false);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, ctx->stream, reinterpret_cast<unsigned long long*>(mesh->admis_flags),
                      0ull, 1);
   HX_TRY(ctx, hipGetLastError(), "fill");
