@@ -102,6 +102,12 @@ std::unique_ptr<Face_permutation_dynamic> face_permutation(int n_dim, int row_si
 
 //! Solver::update (Solver.cpp:834-886) on a registered mesh
 void update(Accelerated_mesh& mesh, const hx_update_options& opts, hx_update_status& status);
+//! Solver::is_admissible (Solver.cpp:921-959); throws std::runtime_error("state is not finite") as the reference does
+bool is_admissible(Accelerated_mesh& mesh);
+//! Solver::fix_admissibility (Solver.cpp:961-1095) with the flow settings `opts`; returns its iteration count
+int fix_admissibility(Accelerated_mesh& mesh, const hx_update_options& opts, double stability_ratio);
+//! Solver::calc_jacobian (Solver.cpp:273-381) on the device from the mesh's vertex data
+void calc_jacobian(Accelerated_mesh& mesh, const hx_vertex_data& vertices);
 
 // Device boundary conditions with the arguments and defaults of the reference's Flow_bc constructors
 // (Boundary_condition.hpp:61-225). `state` is the conserved freestream state: momentum[n_dim], density, energy.

@@ -473,6 +473,25 @@ void update(Accelerated_mesh& mesh, const hx_update_options& opts, hx_update_sta
   check(hx_update(mesh.device(), &opts, &status), "Solver::update");
 }
 
+bool is_admissible(Accelerated_mesh& mesh)
+{
+  int admissible = 0;
+  check(hx_is_admissible(mesh.device(), &admissible), "Solver::is_admissible");
+  return admissible;
+}
+
+int fix_admissibility(Accelerated_mesh& mesh, const hx_update_options& opts, double stability_ratio)
+{
+  int iters = 0;
+  check(hx_fix_admissibility(mesh.device(), &opts, stability_ratio, &iters), "Solver::fix_admissibility");
+  return iters;
+}
+
+void calc_jacobian(Accelerated_mesh& mesh, const hx_vertex_data& vertices)
+{
+  check(hx_calc_jacobian(mesh.device(), &vertices), "Solver::calc_jacobian");
+}
+
 namespace
 {
 hx_bc bc_of(int kind)

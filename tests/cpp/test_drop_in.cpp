@@ -505,6 +505,19 @@ static void compare_update(int nd, int rs, int n, bool def, bool visc, int bc_ki
   REQUIRE(err <= 1e-12);
   REQUIRE(sw.children.at("local").work_units_completed == (long long)b.elems.size());
 
+  // Solver::is_admissible through the drop-in: admissible, then not after a negative density, then the error text of a
+  // non-finite value (thermo.cpp:14)
+  am.push();
+  REQUIRE(amd::is_admissible(am));
+  const double keep = b.elems[0]->data[b.p.nd * b.p.nq + 1];
+  b.elems[0]->data[b.p.nd * b.p.nq + 1] = -1.;
+  am.push();
+  REQUIRE(!amd::is_admissible(am));
+  b.elems[0]->data[b.p.nd * b.p.nq + 1] = std::nan("");
+  am.push();
+  REQUIRE(throws([&] { amd::is_admissible(am); }) == "state is not finite");
+  b.elems[0]->data[b.p.nd * b.p.nq + 1] = keep;
+
   // the admissibility fix's diffusion through the free functions (kernels_max_dt.cpp:21, kernels_diffusive.cpp:32):
   // local time step, ghost states copied, then one step with the device's own flux hook
   am.push();
