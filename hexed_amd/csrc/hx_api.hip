@@ -24,6 +24,7 @@ struct hx_ctx
   std::vector<char> host_ops[HX_MAX_ROW_SIZE + 1];  // host copies, for kernels taking constants by value
   void* eops[HX_MAX_ROW_SIZE + 1] = {};             // device Eops<rs, double> tables
   double* scalar = nullptr;             // device scratch for reductions
+  double* host_scalar = nullptr;        // pinned host memory the per-step reductions are read back into
   // per-category event timing
   struct Pending
   {
@@ -555,7 +556,9 @@ hx_status hx_ctx_create(int device, hx_ctx** out)
   auto* ctx = new hx_ctx;
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&ctx->scalar, 8 * sizeof(double)) != hipSuccess) {
+      hipMalloc(&ctx->scalar, 8 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&ctx->host_scalar, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (ctx->scalar) (void)hipFree(ctx->scalar);
     delete ctx;
     return HX_ERR_HIP;
   }
@@ -578,6 +581,7 @@ hx_status hx_ctx_destroy(hx_ctx* ctx)
   }
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   (void)hipFree(ctx->scalar);
+  (void)hipHostFree(ctx->host_scalar);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return HX_OK;
@@ -950,9 +954,10 @@ hx_status hx_max_dt(hx_mesh* mesh, int cls, const hx_pde* pde, int is_local, int
              "max_dt");
     else HX_TRY(mesh->ctx, mesh->ks->max_dt(l, cls, visc, is_local, max_cfl_c, max_cfl_d), "max_dt");
   }
-  double dt = DBL_MAX;
-  HX_TRY(mesh->ctx, hipMemcpyAsync(&dt, l.scalar, sizeof(double), hipMemcpyDeviceToHost, l.stream), "max_dt readback");
+  HX_TRY(mesh->ctx, hipMemcpyAsync(mesh->ctx->host_scalar, l.scalar, sizeof(double), hipMemcpyDeviceToHost, l.stream),
+         "max_dt readback");
   HX_TRY(mesh->ctx, hipStreamSynchronize(l.stream), "max_dt sync");
+  const double dt = *mesh->ctx->host_scalar;
   *dt_out = is_local ? 1. : dt;
   return HX_OK;
 }
@@ -1129,8 +1134,10 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status*
           if (g) hipLaunchKernelGGL(k_min_reduce, dim3(g), dim3(256), 0, ctx->stream, mesh->dt_block_min, mesh->n_dt_blocks, ctx->scalar);
           HX_TRY(ctx, hipGetLastError(), "time step reduction");
         }
-        HX_TRY(ctx, hipMemcpyAsync(&mdt, ctx->scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "max_dt readback");
+        HX_TRY(ctx, hipMemcpyAsync(ctx->host_scalar, ctx->scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+               "max_dt readback");
         HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "max_dt sync");
+        mdt = *ctx->host_scalar;
       } else {
         s = o->use_ldg ? hx_max_dt_navier_stokes(mesh, &ko, msc, msd, o->local_time, o->visc, o->cond, &mdt)
                        : hx_max_dt_euler(mesh, &ko, msc, msd, o->local_time, &mdt);
