@@ -691,8 +691,20 @@ struct Ns_lds
   static constexpr int PER_D = P1_D > P2_D ? P1_D : P2_D;
 };
 
+// waves per SIMD that the Navier-Stokes Local's LDS allows, at most 4 (3 for row sizes above 6, which spill at 4):
+// the register target follows it
 template <int ND, int RS, bool DEF, class P>
-__global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh m, Eops<RS, double> o,
+constexpr int ns_waves()
+{
+  using L = Ns_lds<ND, RS, DEF, P>;
+  constexpr int lds = Shape<ND, RS>::EPB * (int(sizeof(typename P::T)) * L::PER_T + 8 * L::PER_D);
+  constexpr int w = (163840 / lds) * (form_block<ND, RS, P>() / 64) / 4, cap = RS > 6 ? 3 : 4;
+  return w < 1 ? 1 : (w > cap ? cap : w);
+}
+
+template <int ND, int RS, bool DEF, class P>
+__global__ __launch_bounds__((form_block<ND, RS, P>())) __attribute__((amdgpu_waves_per_eu(ns_waves<ND, RS, DEF, P>())))
+void k_local_ns(Dev_mesh m, Eops<RS, double> o,
                                                                         const Eops<RS, double>* od, Ns_params ns,
                                                                         double coef, int compute_residual,
                                                                         int use_filter)
@@ -725,12 +737,10 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
   }
 
   double u[NV];
-  T ut[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) {
     u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
-    ut[v] = T(u[v]);
-    if (!P::mfma && pt) my[L::U + v * NQP + qp] = ut[v];
+    if (!P::mfma && pt) my[L::U + v * NQP + qp] = T(u[v]);
   }
   T bulk = 0, lap = 0;
   if (valid && m.av[cls]) {
@@ -738,18 +748,20 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
     lap = T(m.av[cls][size_t(e) * 2 * NQ + NQ + q]);
   }
   double det = 1.;
-  double metd[ND][ND];
-  T met[ND][ND];
+  double metd[DEF ? ND : 1][DEF ? ND : 1];
   if constexpr (DEF) {
     const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
     for (int i = 0; i < ND; ++i)
       for (int j = 0; j < ND; ++j) {
         metd[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
-        met[i][j] = T(metd[i][j]);
-        if (!P::mfma && pt) my[L::SN + (i * ND + j) * NQP + qp] = met[i][j];
+        if (!P::mfma && pt) my[L::SN + (i * ND + j) * NQP + qp] = T(metd[i][j]);
       }
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
+  auto metric = [&](int i, int j) -> double {
+    if constexpr (DEF) return metd[i][j];
+    else return double(i == j);
+  };
   load_faces<ND, RS, NT>(m, cls, 1, ld + L::SL, L::PER_D);
   if constexpr (DEF) {
     for (int k = t; k < EPB * 2 * ND * ND * NFQ; k += NT) {
@@ -804,7 +816,7 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
         for (int ii = 0; ii < L::NI; ++ii) {
           const int i = DEF ? ii : j, r = Row<ND, RS>::fq(q, i);
           const double nf = DEF ? db[L::SFN + ((2 * i) * ND + j) * NFQ + r] : 1.;
-          const double nq = DEF ? metd[i][j] : 1.;
+          const double nq = metric(i, j);
           for (int v = 0; v < NV; ++v)
             my[L::X + (ii * NV + v) * NQP + qp] = T(u[v] * nq - db[L::SL + ((2 * i) * NV + v) * NFQ + r] * nf);
         }
@@ -846,47 +858,29 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
   }
   // physical viscous flux (Spatial.hpp:420-435); its product with the normals is taken per direction below
   T fvis[NV][ND];
-  flux_visc<ND>(ns, ut, bulk, lap, grad, fvis);
+  {
+    T ut[NV];
+    for (int v = 0; v < NV; ++v) ut[v] = T(u[v]);
+    flux_visc<ND>(ns, ut, bulk, lap, grad, fvis);
+  }
 #pragma unroll
   for (int k = 0; k < NSB; ++k)
     if (t + k * NT < EPB * PF) {
       const int kk = t + k * NT;
       ld[(kk / PF) * L::PER_D + L::SB + kk % PF] = sb[k];
     }
-  // residual along each direction (Spatial.hpp:444-470): the convective flux with its numerical face fluxes, then
-  // the viscous flux, which is also extrapolated to the LDG blocks of the faces
-  const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
+  // residual along each direction (Spatial.hpp:444-470). The viscous flux goes first (it is also extrapolated to the
+  // LDG blocks of the faces), then the convective flux with its numerical face fluxes. The two rates accumulate in
+  // separate arrays, each over the directions in order, so the order of the two passes leaves the result unchanged;
+  // it frees the viscous flux's registers before the convective pass.
   for (int i = 0; i < ND; ++i) {
-    T nrm[ND];
-    double nrmd[ND];
-    for (int j = 0; j < ND; ++j) {
-      if constexpr (DEF) nrm[j] = met[i][j], nrmd[j] = metd[i][j];
-      else nrm[j] = T(i == j), nrmd[j] = double(i == j);
-    }
-    // (FP32-MFMA: the face fluxes stored above and the gradient operands they alias need the barrier at i = 0 too)
-    if (i || P::mfma) __syncthreads();
-    if (pt)
-      conv_point<ND, DEF, P>(u, p64, nrmd, ld + el * L::PER_D + L::SB + (2 * i) * NV * NFQ + Row<ND, RS>::fq(q, i), NFQ,
-                             my + L::F + qp, ld + el * L::PER_D + L::PD + qp, NQP);
-    __syncthreads();
-    for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
-      T* b = lt + el2 * L::PER_T;
-      double* db = ld + el2 * L::PER_D;
-      T y[RS];
-      conv_row<ND, RS, DEF, P>(con, o, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
-                               db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
-      if (ok) {
-        sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
-        pressure_row<ND, RS, DEF, P>(o, i, v, db + L::PD + (v % ND) * NQP + base, db + L::RP + (v % ND) * NQP + base, sd);
-      }
-    });
     __syncthreads();
     if (pt)
       for (int v = 0; v < NV; ++v) {
         T fv;
         if constexpr (DEF) {
           fv = 0;
-          for (int j = 0; j < ND; ++j) fv += fvis[v][j] * nrm[j];
+          for (int j = 0; j < ND; ++j) fv += fvis[v][j] * T(metric(i, j));
         } else {
           fv = fvis[v][i];
         }
@@ -905,6 +899,27 @@ __global__ __launch_bounds__((form_block<ND, RS, P>())) void k_local_ns(Dev_mesh
         slot_ptr(m, fs[1])[(NV + v) * NFQ + r] = double(bx[1]);
       }
       if (ok) sub_row(b + L::RT + (NV + v) * NQP + base, sd, y, i == 0);
+    });
+  }
+  const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
+  for (int i = 0; i < ND; ++i) {
+    double nrmd[ND];
+    for (int j = 0; j < ND; ++j) nrmd[j] = metric(i, j);
+    __syncthreads();
+    if (pt)
+      conv_point<ND, DEF, P>(u, p64, nrmd, ld + el * L::PER_D + L::SB + (2 * i) * NV * NFQ + Row<ND, RS>::fq(q, i), NFQ,
+                             my + L::F + qp, ld + el * L::PER_D + L::PD + qp, NQP);
+    __syncthreads();
+    for_columns<ND, RS, NV, NT, P::mfma>(i, [&](bool ok, int el2, int v, int r, int base, int sd) {
+      T* b = lt + el2 * L::PER_T;
+      double* db = ld + el2 * L::PER_D;
+      T y[RS];
+      conv_row<ND, RS, DEF, P>(con, o, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
+                               db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
+      if (ok) {
+        sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
+        pressure_row<ND, RS, DEF, P>(o, i, v, db + L::PD + (v % ND) * NQP + base, db + L::RP + (v % ND) * NQP + base, sd);
+      }
     });
   }
   if (use_filter) {
