@@ -672,13 +672,15 @@ struct Ns_lds
   static constexpr int NI = DEF ? ND : 1;                         // reference directions per physical component
   static constexpr int U = 0;                                     // [NV][NQP] state (FP64 form)
   static constexpr int G = U + NV * NQP;                          // [NV][NQP] one gradient component (FP64 form)
-  static constexpr int SN = G + NV * NQP;                         // [ND][ND][NQP] reference-level normals (FP64 form)
+  static constexpr int SN = G + NV * NQP;                         // [ND][NQP] column j of the reference-level normals (FP64 form)
   static constexpr int X = 0;                                     // [NI][NV][NQP] row operands (FP32-MFMA form)
-  static constexpr int P1_T = P::mfma ? NI * NV * NQP : SN + (DEF ? ND * ND * NQP : 0);
-  // arrays of T (padded), residual phase
-  static constexpr int F = 0;                                     // [NV][NQP] convective, then viscous flux
-  static constexpr int RT = F + NV * NQP;                         // [2][NV][NQP] convective, viscous rate
-  static constexpr int P2_T = RT + 2 * NV * NQP;
+  static constexpr int P1_T = P::mfma ? NI * NV * NQP : SN + (DEF ? ND * NQP : 0);
+  // arrays of T (padded), residual phase. FP64 form: one rate array, the viscous rate moving to registers before the
+  // convective pass (so that four workgroups of 3D row size 6 fit a compute unit's LDS); FP32-MFMA form: both rates.
+  static constexpr int F = 0;                                     // [NV][NQP] viscous, then convective flux
+  static constexpr int RT = F + NV * NQP;                         // [NV][NQP] convective rate
+  static constexpr int RV = P::mfma ? RT + NV * NQP : RT;         // [NV][NQP] viscous rate
+  static constexpr int P2_T = RV + NV * NQP;
   static constexpr int PER_T = P1_T > P2_T ? P1_T : P2_T;
   // arrays of double
   static constexpr int SL = 0;                                    // [2ND][NV][NFQ] LDG face state
@@ -698,7 +700,7 @@ constexpr int ns_waves()
 {
   using L = Ns_lds<ND, RS, DEF, P>;
   constexpr int lds = Shape<ND, RS>::EPB * (int(sizeof(typename P::T)) * L::PER_T + 8 * L::PER_D);
-  constexpr int w = (163840 / lds) * (form_block<ND, RS, P>() / 64) / 4, cap = RS > 6 ? 3 : 4;
+  constexpr int w = (163840 / lds) * ((form_block<ND, RS, P>() + 63) / 64) / 4, cap = RS > 6 ? 3 : 4;
   return w < 1 ? 1 : (w > cap ? cap : w);
 }
 
@@ -754,7 +756,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
     for (int i = 0; i < ND; ++i)
       for (int j = 0; j < ND; ++j) {
         metd[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
-        if (!P::mfma && pt) my[L::SN + (i * ND + j) * NQP + qp] = T(metd[i][j]);
+        if (!P::mfma && pt && j == 0) my[L::SN + i * NQP + qp] = T(metd[i][j]);
       }
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
@@ -792,7 +794,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
           double b0 = db[L::SL + ((2 * i) * NV + v) * NFQ + r], b1 = db[L::SL + ((2 * i + 1) * NV + v) * NFQ + r];
           if constexpr (DEF) {
             T nk[RS];
-            load_row(b + L::SN + (i * ND + j) * NQP + base, sd, nk);
+            load_row(b + L::SN + i * NQP + base, sd, nk);
 #pragma unroll
             for (int k = 0; k < RS; ++k) x[k] *= nk[k];
             b0 *= db[L::SFN + ((2 * i) * ND + j) * NFQ + r];
@@ -806,6 +808,10 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
         __syncthreads();
       }
       for (int v = 0; v < NV; ++v) grad[v][j] = my[L::G + v * NQP + qp] * T(inv_h);
+      // the next component's column of the normals (every row of this one was read before the barrier above)
+      if constexpr (DEF)
+        if (pt && j + 1 < ND)
+          for (int i = 0; i < ND; ++i) my[L::SN + i * NQP + qp] = T(metric(i, j + 1));
       __syncthreads();
     }
   } else {
@@ -898,8 +904,15 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
         slot_ptr(m, fs[0])[(NV + v) * NFQ + r] = double(bx[0]);
         slot_ptr(m, fs[1])[(NV + v) * NFQ + r] = double(bx[1]);
       }
-      if (ok) sub_row(b + L::RT + (NV + v) * NQP + base, sd, y, i == 0);
+      if (ok) sub_row(b + L::RV + v * NQP + base, sd, y, i == 0);
     });
+  }
+  // FP64 form: the viscous rate (filtered) moves to registers, its arrays take the convective rate
+  T vrate[P::mfma ? 1 : NV];
+  if constexpr (!P::mfma) {
+    if (use_filter) filter_columns<ND, RS, NV, NT>(o, lt + L::RV, L::PER_T);
+    else __syncthreads();
+    for (int v = 0; v < NV; ++v) vrate[v] = my[L::RV + v * NQP + qp];
   }
   const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
   for (int i = 0; i < ND; ++i) {
@@ -923,7 +936,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
     });
   }
   if (use_filter) {
-    filter_columns<ND, RS, 2 * NV, NT>(o, lt + L::RT, L::PER_T);
+    filter_columns<ND, RS, (P::mfma ? 2 : 1) * NV, NT>(o, lt + L::RT, L::PER_T);
     if constexpr (P::mfma && DEF) filter_columns<ND, RS, ND, NT>(o, ld + L::RP, L::PER_D);
   } else {
     __syncthreads();
@@ -938,7 +951,8 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
       double r = double(my[L::RT + v * NQP + qp]);
       if (has_pressure<ND, DEF>(v) && P::mfma) r -= ld[el * L::PER_D + L::RP + v * NQP + qp];
       rc[v * NQ + q] = r;
-      r += double(my[L::RT + (NV + v) * NQP + qp]);
+      if constexpr (P::mfma) r += double(my[L::RV + v * NQP + qp]);
+      else r += vrate[v];
       r *= mult;
       if (compute_residual) rc[v * NQ + q] = r;
       else sw[v * NQ + q] = u[v] + r;
