@@ -7,6 +7,7 @@
 #pragma once
 
 #include <cfloat>
+#include <type_traits>
 
 #include "hx_launch.hpp"
 
@@ -174,14 +175,29 @@ __device__ inline void sub_row(T* out, int sd, const T (&y)[RS], bool first)
 }
 
 // boundary(s) . x in FP64 (Spatial.hpp:29-71)
-template <int RS, class T>
-__device__ inline double row_bound(const Eops<RS, double>& o, int s, const T (&x)[RS])
+template <int RS, class T, class O>
+__device__ inline double row_bound(const O& o, int s, const T (&x)[RS])
 {
   double a = 0;
 #pragma unroll
   for (int k = 0; k < RS; ++k) a += o.bound[s][k] * double(x[k]);
   return a;
 }
+
+// The operator constants read through a constant-address-space pointer that the compiler cannot follow across the
+// empty asm: each call is reloaded with scalar loads where it is used, instead of being held (and spilled to vector
+// register lanes) in scalar registers across a kernel's phases.
+template <int RS>
+struct Ops_reload
+{
+  const Eops<RS, double>* g;
+  __device__ const auto& operator()() const
+  {
+    auto r = (const __attribute__((address_space(4))) Eops<RS, double>*)g;
+    asm volatile("" : "+s"(r));
+    return *r;
+  }
+};
 
 // Row derivatives, Derivative.hpp:51-60: faces(x, b0, b1) = D x + L (b - B x), interior(x) = D x.
 template <int RS, class P, bool RELOAD = false>
@@ -449,9 +465,9 @@ __device__ inline void conv_point(const double (&u)[ND + 2], double p, const dou
 // row phase: y = D x + L (b - B x) of the flux row of variable v from the operands conv_point wrote (frow; prow the
 // pressure row), without the FP32-MFMA form's FP64 pressure derivative (pressure_row). The matrix instructions run
 // outside any branch that diverges by variable.
-template <int ND, int RS, bool DEF, class P, class C>
-__device__ inline void conv_row(const C& con, const Eops<RS, double>& o, int v, const typename P::T* frow,
-                                const double* prow, int sd, double b0, double b1, typename P::T (&y)[RS])
+template <int ND, int RS, bool DEF, class P, class C, class G>
+__device__ inline void conv_row(const C& con, G&& ops, int v, const typename P::T* frow, const double* prow, int sd,
+                                double b0, double b1, typename P::T (&y)[RS])
 {
   using T = typename P::T;
   T x[RS];
@@ -463,6 +479,7 @@ __device__ inline void conv_row(const C& con, const Eops<RS, double>& o, int v, 
     if (has_pressure<ND, DEF>(v)) {
       double xp[RS];
       load_row(prow, sd, xp);
+      const auto& o = ops();
       s0 = b0 - row_bound(o, 0, xp);
       s1 = b1 - row_bound(o, 1, xp);
     }
@@ -470,17 +487,24 @@ __device__ inline void conv_row(const C& con, const Eops<RS, double>& o, int v, 
   }
 }
 
-// FP32-MFMA form on deformed elements: acc = D xp (first direction) or acc += D xp, in FP64, for a momentum row
-template <int ND, int RS, bool DEF, class P>
-__device__ inline void pressure_row(const Eops<RS, double>& o, int i, int v, const double* prow, double* acc, int sd)
+// FP32-MFMA form on deformed elements: acc = D xp (first direction) or acc += D xp, in FP64, for a momentum row. The
+// operator comes from `ops()` one row at a time (behind a scheduling barrier where `ops` reloads it).
+template <int ND, int RS, bool DEF, class P, class G>
+__device__ inline void pressure_row(G&& ops, int i, int v, const double* prow, double* acc, int sd)
 {
   if constexpr (P::mfma && DEF) {
     if (v < ND) {
-      double xp[RS], yd[RS];
+      double xp[RS];
       load_row(prow, sd, xp);
-      row_matvec(o.diff, xp, yd);
 #pragma unroll
-      for (int k = 0; k < RS; ++k) acc[k * sd] = i == 0 ? yd[k] : acc[k * sd] + yd[k];
+      for (int r = 0; r < RS; ++r) {
+        if constexpr (std::is_same_v<std::decay_t<G>, Ops_reload<RS>>) __builtin_amdgcn_sched_barrier(0);
+        const auto& o = ops();
+        double a = 0;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) a += o.diff[r][k] * xp[k];
+        acc[r * sd] = i == 0 ? a : acc[r * sd] + a;
+      }
     }
   }
 }
@@ -599,11 +623,11 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
       T* b = lt + el2 * L::PER_T;
       double* db = ld + el2 * L::PER_D;
       T y[RS];
-      conv_row<ND, RS, DEF, P>(con, o, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
+      conv_row<ND, RS, DEF, P>(con, [&]() -> const Eops<RS, double>& { return o; }, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
                                db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
       if (ok) {
         sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
-        pressure_row<ND, RS, DEF, P>(o, i, v, db + L::PD + (v % ND) * NQP + base, db + L::RP + (v % ND) * NQP + base, sd);
+        pressure_row<ND, RS, DEF, P>([&]() -> const Eops<RS, double>& { return o; }, i, v, db + L::PD + (v % ND) * NQP + base, db + L::RP + (v % ND) * NQP + base, sd);
       }
     });
   }
@@ -927,11 +951,11 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
       T* b = lt + el2 * L::PER_T;
       double* db = ld + el2 * L::PER_D;
       T y[RS];
-      conv_row<ND, RS, DEF, P>(con, o, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
+      conv_row<ND, RS, DEF, P>(con, Ops_reload<RS>{od}, v, b + L::F + v * NQP + base, db + L::PD + (v % ND) * NQP + base, sd,
                                db[L::SB + ((2 * i) * NV + v) * NFQ + r], db[L::SB + ((2 * i + 1) * NV + v) * NFQ + r], y);
       if (ok) {
         sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
-        pressure_row<ND, RS, DEF, P>(o, i, v, db + L::PD + (v % ND) * NQP + base, db + L::RP + (v % ND) * NQP + base, sd);
+        pressure_row<ND, RS, DEF, P>(Ops_reload<RS>{od}, i, v, db + L::PD + (v % ND) * NQP + base, db + L::RP + (v % ND) * NQP + base, sd);
       }
     });
   }
