@@ -59,10 +59,14 @@ def run(ctx, host, opts, precision, steps):
     return out
 
 
-@pytest.mark.parametrize("cfg", CONFIGS, ids=ids)
-def test_fp32_mfma_vs_fp64(ctx, cfg):
+# every configuration without the modal filter, the first four with it
+CASES = [(c, False) for c in CONFIGS] + [(c, True) for c in CONFIGS[:4]]
+
+
+@pytest.mark.parametrize("cfg,use_filter", CASES, ids=[ids(c) + ("-filter" if f else "") for c, f in CASES])
+def test_fp32_mfma_vs_fp64(ctx, cfg, use_filter):
     nd, rs, deformed, n, ns = cfg
-    say(f"[gpu] fp32-mfma vs fp64 {ids(cfg)}")
+    say(f"[gpu] fp32-mfma vs fp64 {ids(cfg)} filter={use_filter}")
     host = mesh.box(nd, rs, n, deformed=deformed, jitter=0.1 if deformed else 0.0, seed=3,
                     bc=(abi.HX_BC_FREESTREAM, freestream(nd)), with_av=ns)
     smooth_state(host, seed=3)
@@ -73,9 +77,9 @@ def test_fp32_mfma_vs_fp64(ctx, cfg):
     init_faces(host)
     if ns:
         visc, cond = sutherland_air()
-        opts = abi.update_options(max_safety=0.5, visc=visc, cond=cond)
+        opts = abi.update_options(max_safety=0.5, visc=visc, cond=cond, use_filter=use_filter)
     else:
-        opts = abi.update_options(max_safety=0.5)
+        opts = abi.update_options(max_safety=0.5, use_filter=use_filter)
     ref = run(ctx, host, opts, abi.HX_FP64, STEPS)
     got = run(ctx, host, opts, abi.HX_FP32_MFMA, STEPS)
     for c in (0, 1):

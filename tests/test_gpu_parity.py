@@ -159,6 +159,23 @@ def sutherland_air():
     return abi.sutherland(1.716e-5 * 2e3, 273., 111.), abi.sutherland(0.0241 * 2e3, 273., 194.)
 
 
+@pytest.mark.parametrize("cfg", [(3, 6, True, 3), (3, 6, False, 3), (2, 5, True, 4), (3, 7, True, 2)], ids=ids)
+def test_update_navier_stokes_filter(ctx, cfg):
+    """Modal filter on the LDG stage 0 residual (Spatial.hpp:472-481): the convective and viscous rates are each
+    filtered along every axis before the update."""
+    say(f"[gpu] update navier-stokes filter {ids(cfg)}")
+    host = build(cfg, with_av=True, seed=6)
+    dev = Device_mesh(ctx, host.copy())
+    s = host.struct()
+    visc, cond = sutherland_air()
+    opts = abi.update_options(max_safety=0.3, visc=visc, cond=cond, use_filter=True)
+    st_o, st_d = abi.hx_update_status(), abi.hx_update_status()
+    for it in range(2):
+        o_call("oracle_update", ctypes.byref(s), ctypes.byref(opts), ctypes.byref(st_o))
+        dev.update(opts, st_d)
+        compare_mesh(f"update {it}", host, dev)
+
+
 @pytest.mark.parametrize("cfg", NS_CONFIGS, ids=ids)
 @pytest.mark.parametrize("bc_kind", [abi.HX_BC_FREESTREAM, abi.HX_BC_NONPENETRATION, abi.HX_BC_OUTFLOW])
 def test_update_navier_stokes(ctx, cfg, bc_kind):
