@@ -994,8 +994,18 @@ struct Reconcile_lds
   static constexpr int PER = U + S::NV * S::NQP;
 };
 
+// waves per SIMD that Reconcile_ldg_flux's LDS allows, at most 8: the register target follows it
+template <int ND, int RS>
+constexpr int reconcile_waves()
+{
+  constexpr int lds = Shape<ND, RS>::EPB * 8 * Reconcile_lds<ND, RS>::PER;
+  constexpr int w = (163840 / lds) * ((Shape<ND, RS>::BLOCK + 63) / 64) / 4;
+  return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
 template <int ND, int RS, bool DEF>
-__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt,
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) __attribute__((amdgpu_waves_per_eu(reconcile_waves<ND, RS>())))
+void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt,
                                                                         int compute_residual, int use_filter)
 {
   using S = Shape<ND, RS>;
