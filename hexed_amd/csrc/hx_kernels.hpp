@@ -690,13 +690,13 @@ struct Ns_lds
 {
   using S = Shape<ND, RS>;
   static constexpr int NQP = S::NQP, NFQ = S::NFQ, NV = S::NV;
-  // arrays of T (padded), gradient phase. FP64 form: the state, one gradient component and the reference-level
-  // normals, contracted row by row. FP32-MFMA form: the face-referenced products u n_ij of the directions i that
-  // contribute to one physical component j, contracted in place.
+  // arrays of T (padded), gradient phase. FP64 form: the state, one gradient component and the column of the
+  // reference-level normals that component j needs, contracted row by row. FP32-MFMA form: the face-referenced
+  // products u n_ij of the directions i that contribute to one physical component j, contracted in place.
   static constexpr int NI = DEF ? ND : 1;                         // reference directions per physical component
   static constexpr int U = 0;                                     // [NV][NQP] state (FP64 form)
   static constexpr int G = U + NV * NQP;                          // [NV][NQP] one gradient component (FP64 form)
-  static constexpr int SN = G + NV * NQP;                         // [ND][NQP] column j of the reference-level normals (FP64 form)
+  static constexpr int SN = G + NV * NQP;                         // [ND][NQP] normals n_ij of one j (FP64 form)
   static constexpr int X = 0;                                     // [NI][NV][NQP] row operands (FP32-MFMA form)
   static constexpr int P1_T = P::mfma ? NI * NV * NQP : SN + (DEF ? ND * NQP : 0);
   // arrays of T (padded), residual phase. FP64 form: one rate array, the viscous rate moving to registers before the
