@@ -287,8 +287,9 @@ def main():
     if cpu_steps > 0:
         import oracle
         lib = oracle.load()
-        cpu_s, cpu_dof, threads, got, ref = 0.0, 0, 1, [], []
+        cpu_s, cpu_dof, threads, got, ref, samples = 0.0, 0, 1, [], [], []
         for sample in (x for (_, _, make_sample) in hosts for x in make_sample()):
+            samples.append(sample)
             init_state = [a.copy() for a in sample.state]
             s = sample.struct()
             for c in (0, 1):
@@ -314,6 +315,20 @@ def main():
                "kind": "port",
                "sample": f"{cpu_steps} time steps ({2 * cpu_steps} stages) of {what}, FP64, OpenMP over {threads} "
                          f"threads, {cpu_s:.2f} s"}
+        # the same oracle on one thread, over a shorter sample of the same meshes (timing only)
+        steps_1t = max(1, cpu_steps // 20)
+        prev_threads = lib.oracle_get_threads()
+        lib.oracle_set_threads(1)
+        try:
+            t_1t = 0.0
+            for sample in samples:
+                t, _ = run_oracle_steps(sample, opts, steps_1t)
+                t_1t += t
+        finally:
+            lib.oracle_set_threads(prev_threads)
+        print(f"[bench] CPU oracle: {steps_1t} step(s) in {t_1t:.2f} s on 1 thread", file=sys.stderr, flush=True)
+        cpu["value_1_thread"] = round(cpu_dof * 2 * steps_1t / t_1t, 1)
+        cpu["sample_1_thread"] = f"{steps_1t} time step(s) ({2 * steps_1t} stages) of the same sample on 1 thread, {t_1t:.2f} s"
         errs = rel_l2_per_var(got, ref, nv)
         parity = {"steps": cpu_steps, "form": "fp64", "rel_l2_per_var": [float(f"{e:.3e}") for e in errs],
                   "max_rel_l2": max(errs)}
