@@ -75,6 +75,17 @@ struct hx_mesh
     int use_ldg = 0;
     hx_transport visc{}, cond{};
   } dt_cache;
+  // the same fusion for hx_update_cases (cases listed in order): stage 1's Local writes each point's 1/scale into
+  // case_scale, which the next call's per-case minimum reads instead of running Max_dt
+  double* case_scale[2] = {};
+  struct
+  {
+    bool valid = false;
+    long long gen = 0;
+    double cc = 0, cd = 0;
+    int use_ldg = 0;
+    hx_transport visc{}, cond{};
+  } case_cache;
 };
 
 
@@ -1308,6 +1319,7 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
     }
     return HX_OK;
   }
+  const long long gen_at_entry = mesh->gen;  // (make_launch below advances it)
   hx_pde p{};
   p.kind = o->use_ldg ? HX_PDE_NAVIER_STOKES : HX_PDE_EULER;
   p.visc = o->visc;
@@ -1318,15 +1330,32 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
   const int rs = mesh->dm.rs;
   const char* name = pde_name(o->use_ldg != 0);
   std::vector<double> dts(nc);
+  const double safety = o->max_safety;
+  const double n_cheby = o->n_cheby_flow;
+  auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
+  const double max_cheby = cheby(n_cheby, n_cheby - 1);
+  // the constants of hx_max_dt with safety_conv = safety / max_cheby, safety_diff = safety
+  const double cc = hx_basis_max_cfl(rs) * (safety / max_cheby);
+  const double cd = -2 / hx_basis_tables::eigenvalss[rs - 2][1] * safety;
+  // with every class's cases in order, stage 1's Local also computes the next step's per-point 1/scale (the state it
+  // writes is the state the next Max_dt reads); the next call uses it if nothing changed the mesh in between and the
+  // time-step settings are the same
+  bool fusable = true;
+  for (int c = 0; c < 2; ++c) fusable = fusable && (!mesh->dm.n_elem[c] || mesh->case_start[c]);
+  for (int c = 0; c < 2 && fusable; ++c)
+    if (mesh->dm.n_elem[c] && !mesh->case_scale[c]) {
+      void* q = nullptr;
+      HX_TRY(ctx, hipMalloc(&q, size_t(mesh->dm.n_elem[c]) * mesh->dm.nq * sizeof(double)), "hipMalloc(case scale)");
+      mesh->allocs.push_back(q);
+      mesh->case_scale[c] = static_cast<double*>(q);
+    }
+  auto& cache = mesh->case_cache;
+  bool pending = fusable && cache.valid && cache.gen == gen_at_entry && cache.cc == cc && cache.cd == cd &&
+                 cache.use_ldg == o->use_ldg && !std::memcmp(&cache.visc, &o->visc, sizeof(hx_transport)) &&
+                 !std::memcmp(&cache.cond, &o->cond, sizeof(hx_transport));
+  cache.valid = false;
   for (int i_flow = 0; i_flow < o->flow_iters; ++i_flow) {
-    const double safety = o->max_safety;
-    const double n_cheby = o->n_cheby_flow;
-    auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
-    const double max_cheby = cheby(n_cheby, n_cheby - 1);
     for (int i_cheby = 0; i_cheby < n_cheby; ++i_cheby) {
-      // the constants of hx_max_dt with safety_conv = safety / max_cheby, safety_diff = safety
-      const double cc = hx_basis_max_cfl(rs) * (safety / max_cheby);
-      const double cd = -2 / hx_basis_tables::eigenvalss[rs - 2][1] * safety;
       hipLaunchKernelGGL(k_fill_u64, dim3((nc + 255) / 256), dim3(256), 0, l.stream, mesh->case_min,
                          (unsigned long long)__builtin_bit_cast(long long, DBL_MAX), nc);
       HX_TRY(ctx, hipGetLastError(), "fill");
@@ -1334,10 +1363,12 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
         if (!mesh->dm.n_elem[c]) continue;
         Timer tm(ctx, std::string(cls_name(c)) + "/compute time step/" + name, mesh->dm.n_elem[c]);
         if (mesh->case_start[c]) {
-          // each point's 1/scale into the time step scale (Max_dt in local mode), then a streaming minimum per case
-          HX_TRY(ctx, mesh->ks->max_dt(l, c, visc, 1, cc, cd), "max_dt");
-          hipLaunchKernelGGL(k_case_min_tss, dim3(nc * mesh->case_split[c]), dim3(256), 0, l.stream, mesh->dm.tss[c],
-                             mesh->case_start[c], mesh->dm.nq, mesh->case_split[c], mesh->case_min);
+          // each point's 1/scale into the time step scale (Max_dt in local mode), or the fused values of the previous
+          // stage 1, then a streaming minimum per case
+          if (!pending) HX_TRY(ctx, mesh->ks->max_dt(l, c, visc, 1, cc, cd), "max_dt");
+          hipLaunchKernelGGL(k_case_min_tss, dim3(nc * mesh->case_split[c]), dim3(256), 0, l.stream,
+                             pending ? mesh->case_scale[c] : mesh->dm.tss[c], mesh->case_start[c], mesh->dm.nq,
+                             mesh->case_split[c], mesh->case_min);
           HX_TRY(ctx, hipGetLastError(), "case minimum");
         } else {
           HX_TRY(ctx, mesh->ks->max_dt_cases(l, c, visc, cc, cd, mesh->case_id[c], mesh->case_min), "max_dt_cases");
@@ -1357,10 +1388,24 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
         hx_kernel_options opts{1., i, 0, o->use_filter};
         hx_status s;
         if ((s = hx_apply_state_bcs(mesh))) return s;
-        if (o->use_ldg && !i) s = hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
-        else s = hx_compute_euler(mesh, &opts);
+        if (o->use_ldg && !i) {
+          s = hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
+        } else {
+          if (i == 1 && fusable) {
+            mesh->fuse = Dt_fuse{};
+            mesh->fuse.visc = o->use_ldg ? 1 : 0;
+            mesh->fuse.max_cfl_c = cc;
+            mesh->fuse.max_cfl_d = cd;
+            mesh->fuse.ns = l.ns;
+            mesh->fuse.point_scale[0] = mesh->case_scale[0];
+            mesh->fuse.point_scale[1] = mesh->case_scale[1];
+          }
+          s = hx_compute_euler(mesh, &opts);
+          mesh->fuse = Dt_fuse{};
+        }
         if (s) return s;
       }
+      pending = fusable;
       HX_TRY(ctx, hipMemcpyAsync(dts.data(), mesh->case_dt, nc * sizeof(double), hipMemcpyDeviceToHost, l.stream),
              "case dt readback");
       HX_TRY(ctx, hipStreamSynchronize(l.stream), "case dt sync");
@@ -1371,6 +1416,15 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
     }
   }
   for (int k = 0; k < nc; ++k) ++st[k].iteration;
+  if (fusable && pending) {
+    cache.valid = true;
+    cache.gen = mesh->gen;
+    cache.cc = cc;
+    cache.cd = cd;
+    cache.use_ldg = o->use_ldg;
+    cache.visc = o->visc;
+    cache.cond = o->cond;
+  }
   return HX_OK;
 }
 

@@ -118,11 +118,12 @@ struct Ns_params
 // minimum over the workgroups is taken by a separate small reduction). block_min == nullptr: not fused.
 struct Dt_fuse
 {
-  double* block_min;
+  double* block_min;  // one minimum of 1/scale per workgroup (global time step), or null
   long long block_offset;
   int visc;  // include the diffusive scale (Max_dt of Navier-Stokes)
   double max_cfl_c, max_cfl_d;
   Ns_params ns;
+  double* point_scale[2];  // per class, each point's 1/scale (the batch's per-case time steps), or null
 };
 
 // Connection_direction (Kernel_connection.hpp:7-37) from the packed form of hx_host_mesh::con_dir

@@ -658,10 +658,11 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
   // the next time step's Max_dt (Spatial.hpp:790-818, global time step) on the state just written: each point's
   // 1/scale here (its loads then overlap the face writes below), the workgroup minimum after them
   double c = DBL_MAX;
-  if (fuse.block_min && valid && !compute_residual) {
+  if ((fuse.block_min || fuse.point_scale[cls]) && valid && !compute_residual) {
     const double scale = fuse.visc ? max_dt_scale_u<ND, RS, true>(m, gops, fuse.ns, cls, e, q, u, fuse.max_cfl_c, fuse.max_cfl_d)
                                    : max_dt_scale_u<ND, RS, false>(m, gops, fuse.ns, cls, e, q, u, fuse.max_cfl_c, fuse.max_cfl_d);
     c = 1. / scale;
+    if (fuse.point_scale[cls]) fuse.point_scale[cls][size_t(e) * NQ + q] = c;
   }
   // Write_face of the updated state (Spatial.hpp:507)
   if (pt)

@@ -93,3 +93,44 @@ def test_euler_then_navier_stokes(ctx):
     st_o, st_d = abi.hx_update_status(), abi.hx_update_status()
     for k, opts in enumerate([eu, ns, ns, eu, eu, ns]):
         step(host, dev, opts, st_o, st_d, f"step {k} {'ns' if opts.use_ldg else 'euler'}")
+
+
+def test_batch_state_upload_between_updates(ctx):
+    """hx_update_cases takes each case's time step from its previous stage 1 only while nothing changed the mesh: after
+    an upload, the per-case steps are those of a fresh batch mesh holding the uploaded state (Max_dt run anew)."""
+    import numpy as np
+    from hexed_amd import configs, mesh
+    from helpers import init_faces
+    say("[gpu] fused max_dt: batch, state upload between updates")
+    cases = [configs.euler_box(2, 4, mach=0.3, seed=1), configs.euler_box(3, 4, mach=0.7, seed=2),
+             configs.euler_box(2, 4, mach=0.5, seed=3, deformed=True, jitter=0.1)]
+    for m in cases:
+        init_faces(m)
+    b = mesh.concat(cases)
+    opts = abi.update_options(max_safety=0.5)
+
+    def batch_dev(host):
+        d = Device_mesh(ctx, host.copy())
+        d.set_cases(len(cases), [c.astype(np.int32) for c in host.case_id])
+        return d
+    dev = batch_dev(b)
+    st = (abi.hx_update_status * len(cases))()
+    dev.update_cases(opts, st)
+    dev.update_cases(opts, st)  # (per-case steps from the fused values)
+    # a faster flow in every case
+    after = b.copy()
+    for c in (0, 1):
+        if after.n_elem[c]:
+            u = dev.download("state", c, out=np.zeros_like(after.state[c]))
+            u[:, 0] *= 3.0
+            u[:, -1] += 4.0 * u[:, 0] ** 2 / u[:, after.n_dim]
+            after.state[c][:] = u
+            dev.upload("state", c, u)
+    dev.update_cases(opts, st)
+    fresh = batch_dev(after)
+    st2 = (abi.hx_update_status * len(cases))()
+    fresh.update_cases(opts, st2)
+    for k in range(len(cases)):
+        assert st[k].time_step == st2[k].time_step, f"case {k}: dt {st[k].time_step!r} vs {st2[k].time_step!r}"
+    dev.close()
+    fresh.close()
