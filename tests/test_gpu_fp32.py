@@ -103,3 +103,23 @@ def test_fp32_mfma_freestream(ctx, nd, rs):
     devs = [float(np.abs(got[1][:, v] - fs[v]).max()) / scale for v in range(nd + 2)]
     say("    max deviation / freestream scale per variable " + " ".join(f"{d:.2e}" for d in devs))
     assert max(devs) <= 5e-8, f"max deviations {devs}"  # measured up to 4.7e-9
+
+
+@pytest.mark.parametrize("nd,ns", [(3, False), (3, True), (2, True)], ids=["3d-euler", "3d-ns", "2d-ns"])
+def test_fp32_mfma_vs_fp64_tree(ctx, nd, ns):
+    """Seeded 2:1 tree meshes: the hanging faces' prolongation and restriction stay FP64 in both forms, the elements
+    next to them run the FP32-MFMA contractions; same bar as the box meshes."""
+    from hexed_amd import configs
+    say(f"[gpu] fp32-mfma vs fp64 tree nd{nd} {'ns' if ns else 'euler'}")
+    host = configs.euler_tree(base=3, rs=6, levels=2, frac=0.3, seed=5, nd=nd)
+    if ns:
+        host.av = [np.full((host.n_elem[0], 2, host.nq), 1e-3), None]
+        opts = abi.update_options(max_safety=0.3, visc=abi.constant(2e-3), cond=abi.constant(2.8e-3))
+    else:
+        opts = abi.update_options(max_safety=0.5)
+    init_faces(host)
+    ref = run(ctx, host, opts, abi.HX_FP64, STEPS)
+    got = run(ctx, host, opts, abi.HX_FP32_MFMA, STEPS)
+    errs = per_var_rel_l2(got[0], ref[0])
+    say("    rel L2 per variable " + " ".join(f"{e:.2e}" for e in errs))
+    assert max(errs) <= TOL32, errs
