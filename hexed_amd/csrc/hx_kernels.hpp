@@ -570,9 +570,7 @@ template <int ND, int RS, bool DEF, class P>
 constexpr int euler_waves()
 {
   using L = Euler_lds<ND, RS, DEF, P>;
-  // (plus the time-step reduction's one double per wavefront)
-  constexpr int lds = Shape<ND, RS>::EPB * (int(sizeof(typename P::T)) * L::PER_T + 8 * L::PER_D) +
-                      8 * (euler_block<ND, RS>() / 64);
+  constexpr int lds = Shape<ND, RS>::EPB * (int(sizeof(typename P::T)) * L::PER_T + 8 * L::PER_D);
   constexpr int w = (163840 / lds) * (euler_block<ND, RS>() / 64) / 4;
   return w < 1 ? 1 : (w > 5 ? 5 : w);
 }
@@ -670,11 +668,14 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
   __syncthreads();
   write_faces_from<ND, RS, NT>(m, cls, o, ld + L::U, L::PER_D);
   if (fuse.block_min) {
-    __shared__ double wmin[NT / 64];
     for (int off = 32; off > 0; off /= 2) {
       const double x = __shfl_xor(c, off, 64);
       c = (x < c) ? x : c;
     }
+    // the per-wavefront minima reuse the start of the Write_face staging array, once every thread has read it (no LDS
+    // of its own: at 3D row size 4 in FP64 that keeps four workgroups per compute unit)
+    __syncthreads();
+    double* wmin = ld;
     if ((t & 63) == 0) wmin[t / 64] = c;
     __syncthreads();
     if (t == 0) {
