@@ -41,6 +41,7 @@ struct hx_ctx
   std::vector<hipEvent_t> event_pool;
   std::vector<Pending> pending;
   std::map<std::string, Total> totals;
+  std::vector<hx_mesh*> meshes;  // live meshes of this context, detached by hx_ctx_destroy
 };
 
 struct hx_mesh
@@ -90,6 +91,27 @@ struct hx_mesh
 
 
 namespace {
+
+// a mesh that can run work: not NULL and its context not destroyed
+bool live(const hx_mesh* mesh) { return mesh && mesh->ctx; }
+
+// frees a mesh's device memory; the struct stays valid and holds no device pointer afterwards
+void release_mesh_memory(hx_mesh* mesh)
+{
+  for (void* p : mesh->allocs) (void)hipFree(p);
+  mesh->allocs.clear();
+  mesh->dm = Dev_mesh{};
+  mesh->n_cases = 0;
+  mesh->case_id[0] = mesh->case_id[1] = nullptr;
+  mesh->case_start[0] = mesh->case_start[1] = nullptr;
+  mesh->case_scale[0] = mesh->case_scale[1] = nullptr;
+  mesh->case_min = nullptr;
+  mesh->case_dt = nullptr;
+  mesh->dt_block_min = nullptr;
+  mesh->admis_flags = nullptr;
+  mesh->fuse = Dt_fuse{};
+  mesh->dt_cache.valid = mesh->case_cache.valid = false;
+}
 
 hx_status fail(hx_ctx* ctx, hx_status code, const std::string& msg)
 {
@@ -582,6 +604,13 @@ hx_status hx_ctx_destroy(hx_ctx* ctx)
   if (!ctx) return HX_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  // meshes still alive are detached: their device memory is freed now, their handles stay valid for
+  // hx_mesh_destroy, and every other call on them fails with HX_ERR_INVALID
+  for (hx_mesh* mesh : ctx->meshes) {
+    release_mesh_memory(mesh);
+    mesh->ctx = nullptr;
+  }
+  ctx->meshes.clear();
   for (void* p : ctx->ops)
     if (p) (void)hipFree(p);
   for (void* p : ctx->eops)
@@ -772,6 +801,7 @@ hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* d, hx_mesh** out)
   m.n_bc = d->n_bc;
   if ((s = device_copy(mesh, &m.bcs, d->bcs, (long long)d->n_bc, true))) return bail(s);
   if ((s = device_copy(mesh, &m.bc_cache, d->bc_cache, (long long)d->n_bc_con * z.nv * z.nfq, true))) return bail(s);
+  ctx->meshes.push_back(mesh);
   *out = mesh;
   return HX_OK;
 }
@@ -793,16 +823,20 @@ hx_status hx_mesh_validate(const hx_host_mesh* d, char* msg, size_t capacity)
 hx_status hx_mesh_destroy(hx_mesh* mesh)
 {
   if (!mesh) return HX_OK;
-  (void)hipSetDevice(mesh->ctx->device);
-  (void)hipStreamSynchronize(mesh->ctx->stream);
-  for (void* p : mesh->allocs) (void)hipFree(p);
+  if (hx_ctx* ctx = mesh->ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    release_mesh_memory(mesh);
+    ctx->meshes.erase(std::remove(ctx->meshes.begin(), ctx->meshes.end(), mesh), ctx->meshes.end());
+  }
+  // a mesh detached by hx_ctx_destroy holds no device memory any more
   delete mesh;
   return HX_OK;
 }
 
 hx_status hx_mesh_upload(hx_mesh* mesh, int field, int cls, const double* host, size_t n)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   ++mesh->gen;
   double** p = field_ptr(mesh->dm, field, cls);
   long long sz = field_size(mesh->dm, field, cls);
@@ -823,7 +857,7 @@ hx_status hx_mesh_upload(hx_mesh* mesh, int field, int cls, const double* host, 
 
 hx_status hx_mesh_download(hx_mesh* mesh, int field, int cls, double* host, size_t n)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   double** p = field_ptr(mesh->dm, field, cls);
   long long sz = field_size(mesh->dm, field, cls);
   if (!p || sz < 0) return fail(mesh->ctx, HX_ERR_INVALID, "unknown field or class");
@@ -840,7 +874,7 @@ hx_status hx_mesh_download(hx_mesh* mesh, int field, int cls, double* host, size
 
 hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   if (precision != HX_FP64 && precision != HX_FP32_MFMA) return fail(mesh->ctx, HX_ERR_INVALID, "unknown precision");
   ++mesh->gen;
   mesh->precision = precision;
@@ -849,7 +883,7 @@ hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision)
 
 hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size_t* n)
 {
-  if (!mesh || !out || !n) return HX_ERR_INVALID;
+  if (!live(mesh) || !out || !n) return HX_ERR_INVALID;
   ++mesh->gen;  // the caller may write through the pointer
   double** p = field_ptr(mesh->dm, field, cls);
   long long sz = field_size(mesh->dm, field, cls);
@@ -859,9 +893,16 @@ hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size
   return HX_OK;
 }
 
+hx_status hx_mesh_mark_changed(hx_mesh* mesh)
+{
+  if (!live(mesh)) return HX_ERR_INVALID;
+  ++mesh->gen;
+  return HX_OK;
+}
+
 hx_status hx_write_face(hx_mesh* mesh, int cls, const hx_pde* pde)
 {
-  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  if (!live(mesh) || !valid_cls(cls)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
@@ -873,7 +914,7 @@ hx_status hx_write_face(hx_mesh* mesh, int cls, const hx_pde* pde)
 
 hx_status hx_neighbor(hx_mesh* mesh, int cls, const hx_pde* pde, int)
 {
-  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  if (!live(mesh) || !valid_cls(cls)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
@@ -885,7 +926,7 @@ hx_status hx_neighbor(hx_mesh* mesh, int cls, const hx_pde* pde, int)
 
 hx_status hx_local(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual, int use_filter)
 {
-  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  if (!live(mesh) || !valid_cls(cls)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
@@ -911,7 +952,7 @@ hx_status hx_local(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_s
 
 hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde* pde)
 {
-  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  if (!live(mesh) || !valid_cls(cls)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
   const int kind = pde_kind(pde);
   const bool diff_only = kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS;
@@ -925,7 +966,7 @@ hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde* pde)
 hx_status hx_reconcile_ldg_flux(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual,
                                 int use_filter)
 {
-  if (!mesh || !valid_cls(cls)) return HX_ERR_INVALID;
+  if (!live(mesh) || !valid_cls(cls)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
@@ -949,7 +990,7 @@ hx_status hx_reconcile_ldg_flux(hx_mesh* mesh, int cls, const hx_pde* pde, doubl
 hx_status hx_max_dt(hx_mesh* mesh, int cls, const hx_pde* pde, int is_local, int, double safety_conv, double safety_diff,
                     double* dt_out)
 {
-  if (!mesh || !valid_cls(cls) || !dt_out) return HX_ERR_INVALID;
+  if (!live(mesh) || !valid_cls(cls) || !dt_out) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
@@ -975,7 +1016,7 @@ hx_status hx_max_dt(hx_mesh* mesh, int cls, const hx_pde* pde, int is_local, int
 
 hx_status hx_prolong_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int offset)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
@@ -986,7 +1027,7 @@ hx_status hx_prolong_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int of
 
 hx_status hx_restrict_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int offset)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   Launch l = make_launch(mesh);
   bool visc;
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
@@ -997,7 +1038,7 @@ hx_status hx_restrict_refined(hx_mesh* mesh, const hx_pde* pde, int scale, int o
 
 hx_status hx_apply_state_bcs(hx_mesh* mesh)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   Timer tm(mesh->ctx, "boundary conditions/state", mesh->dm.n_bc_con);
   HX_TRY(mesh->ctx, mesh->ks->state_bcs(make_launch(mesh)), "state_bcs");
   return HX_OK;
@@ -1005,7 +1046,7 @@ hx_status hx_apply_state_bcs(hx_mesh* mesh)
 
 hx_status hx_apply_flux_bcs(hx_mesh* mesh)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   Timer tm(mesh->ctx, "boundary conditions/flux", mesh->dm.n_bc_con);
   HX_TRY(mesh->ctx, mesh->ks->flux_bcs(make_launch(mesh)), "flux_bcs");
   return HX_OK;
@@ -1013,7 +1054,7 @@ hx_status hx_apply_flux_bcs(hx_mesh* mesh)
 
 hx_status hx_compute_euler(hx_mesh* mesh, const hx_kernel_options* o)
 {
-  if (!mesh || !o) return HX_ERR_INVALID;
+  if (!live(mesh) || !o) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_EULER;
   hx_status s;
@@ -1027,7 +1068,7 @@ hx_status hx_compute_euler(hx_mesh* mesh, const hx_kernel_options* o)
 
 hx_status hx_compute_navier_stokes(hx_mesh* mesh, const hx_kernel_options* o, hx_transport visc, hx_transport cond)
 {
-  if (!mesh || !o) return HX_ERR_INVALID;
+  if (!live(mesh) || !o) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_NAVIER_STOKES;
   p.visc = visc;
@@ -1055,7 +1096,7 @@ hx_status hx_compute_navier_stokes(hx_mesh* mesh, const hx_kernel_options* o, hx
 
 hx_status hx_max_dt_euler(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time, double* dt_out)
 {
-  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  if (!live(mesh) || !dt_out) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_EULER;
   double a, b;
@@ -1070,7 +1111,7 @@ hx_status hx_max_dt_euler(hx_mesh* mesh, const hx_kernel_options* o, double msc,
 hx_status hx_max_dt_navier_stokes(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time,
                                   hx_transport visc, hx_transport cond, double* dt_out)
 {
-  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  if (!live(mesh) || !dt_out) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_NAVIER_STOKES;
   p.visc = visc;
@@ -1097,7 +1138,7 @@ hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset) { return hx_
 // Solver::update (Solver.cpp:834-886), with fix_admissibility after each stage when enabled (Solver.cpp:867)
 hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
 {
-  if (!mesh || !o || !st) return HX_ERR_INVALID;
+  if (!live(mesh) || !o || !st) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
   // with a global time step, stage 1's Local also computes the next step's Max_dt (the state it writes is the state
   // that Max_dt would read next); local time stepping writes the time step scale in Max_dt, so it is not fused
@@ -1202,7 +1243,7 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status*
 // Solver::compute_residual (Solver.cpp:899-913)
 hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* o)
 {
-  if (!mesh || !o) return HX_ERR_INVALID;
+  if (!live(mesh) || !o) return HX_ERR_INVALID;
   hx_status s = hx_apply_state_bcs(mesh);
   if (s) return s;
   hx_kernel_options opts{1., 0, 1, o->use_filter};
@@ -1214,7 +1255,7 @@ hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* o)
 
 hx_status hx_mesh_set_cases(hx_mesh* mesh, int n_cases, const int* case_cartesian, const int* case_deformed)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
   const Dev_mesh& m = mesh->dm;
   if (n_cases < 1) return fail(ctx, HX_ERR_INVALID, "n_cases must be positive");
@@ -1302,7 +1343,7 @@ hx_status hx_mesh_set_cases(hx_mesh* mesh, int n_cases, const int* case_cartesia
 // case's dt times a scale of 1.
 hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
 {
-  if (!mesh || !o || !st) return HX_ERR_INVALID;
+  if (!live(mesh) || !o || !st) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
   const int nc = mesh->n_cases;
   if (nc < 1) return fail(ctx, HX_ERR_INVALID, "no cases registered (hx_mesh_set_cases)");
@@ -1432,7 +1473,7 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
 
 hx_status hx_apply_advection_bcs(hx_mesh* mesh)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   Timer tm(mesh->ctx, "boundary conditions/advection", mesh->dm.n_bc_con);
   HX_TRY(mesh->ctx, mesh->ks->av_bcs(make_launch(mesh), 0), "advection bcs");
   return HX_OK;
@@ -1440,7 +1481,7 @@ hx_status hx_apply_advection_bcs(hx_mesh* mesh)
 
 hx_status hx_apply_avc_diff_bcs(hx_mesh* mesh)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   Timer tm(mesh->ctx, "boundary conditions/av diffusion", mesh->dm.n_bc_con);
   HX_TRY(mesh->ctx, mesh->ks->av_bcs(make_launch(mesh), 1), "avc diff bcs");
   return HX_OK;
@@ -1448,7 +1489,7 @@ hx_status hx_apply_avc_diff_bcs(hx_mesh* mesh)
 
 hx_status hx_apply_avc_diff_flux_bcs(hx_mesh* mesh)
 {
-  if (!mesh) return HX_ERR_INVALID;
+  if (!live(mesh)) return HX_ERR_INVALID;
   Timer tm(mesh->ctx, "boundary conditions/av diffusion flux", mesh->dm.n_bc_con);
   HX_TRY(mesh->ctx, mesh->ks->av_bcs(make_launch(mesh), 2), "avc diff flux bcs");
   return HX_OK;
@@ -1456,7 +1497,7 @@ hx_status hx_apply_avc_diff_flux_bcs(hx_mesh* mesh)
 
 hx_status hx_compute_advection(hx_mesh* mesh, const hx_kernel_options* o, double advect_length)
 {
-  if (!mesh || !o) return HX_ERR_INVALID;
+  if (!live(mesh) || !o) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_ADVECTION;
   p.advect_length = advect_length;
@@ -1471,7 +1512,7 @@ hx_status hx_compute_advection(hx_mesh* mesh, const hx_kernel_options* o, double
 
 hx_status hx_compute_smooth_av(hx_mesh* mesh, const hx_kernel_options* o, double diff_time, double cheby_step)
 {
-  if (!mesh || !o) return HX_ERR_INVALID;
+  if (!live(mesh) || !o) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_SMOOTH_AV;
   p.diff_time = diff_time;
@@ -1526,7 +1567,7 @@ hx_status hx_compute_prolong_advection(hx_mesh* mesh, double advect_length)
 hx_status hx_max_dt_advection(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time,
                               double advect_length, double* dt_out)
 {
-  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  if (!live(mesh) || !dt_out) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_ADVECTION;
   p.advect_length = advect_length;
@@ -1542,7 +1583,7 @@ hx_status hx_max_dt_advection(hx_mesh* mesh, const hx_kernel_options* o, double 
 hx_status hx_max_dt_smooth_av(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time,
                               double* dt_out)
 {
-  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  if (!live(mesh) || !dt_out) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_SMOOTH_AV;
   p.diff_time = p.cheby_step = 1.;  // Max_dt<Smooth_art_visc>(..., 1., 1.) (kernels_max_dt.cpp:15)
@@ -1568,7 +1609,7 @@ static hx_status fta_bcs(hx_mesh* mesh, int mode)
 
 hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
 {
-  if (!mesh || !v) return HX_ERR_INVALID;
+  if (!live(mesh) || !v) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
   Dev_mesh& dm = mesh->dm;
   const int nd = dm.nd, rs = dm.rs, nvert = 1 << nd, nfq = dm.nfq;
@@ -1648,7 +1689,7 @@ hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
 // flux_bc callback is the fix's own (Solver::apply_fta_flux_bcs, Solver.cpp:103-115)
 hx_status hx_compute_fix_therm_admis(hx_mesh* mesh, const hx_kernel_options* o)
 {
-  if (!mesh || !o) return HX_ERR_INVALID;
+  if (!live(mesh) || !o) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_FIX_ADMIS;
   if (o->i_stage)
@@ -1673,7 +1714,7 @@ hx_status hx_compute_fix_therm_admis(hx_mesh* mesh, const hx_kernel_options* o)
 hx_status hx_max_dt_fix_therm_admis(hx_mesh* mesh, const hx_kernel_options* o, double msc, double msd, int local_time,
                                     double* dt_out)
 {
-  if (!mesh || !dt_out) return HX_ERR_INVALID;
+  if (!live(mesh) || !dt_out) return HX_ERR_INVALID;
   hx_pde p{};
   p.kind = HX_PDE_FIX_ADMIS;
   double a, b;
@@ -1688,7 +1729,7 @@ hx_status hx_max_dt_fix_therm_admis(hx_mesh* mesh, const hx_kernel_options* o, d
 // Solver::is_admissible (Solver.cpp:921-959)
 hx_status hx_is_admissible(hx_mesh* mesh, int* admissible)
 {
-  if (!mesh || !admissible) return HX_ERR_INVALID;
+  if (!live(mesh) || !admissible) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
   // reads the fields only, so a fused Max_dt stays valid
   if (!mesh->admis_flags) {
@@ -1724,7 +1765,7 @@ hx_status hx_is_admissible(hx_mesh* mesh, int* admissible)
 // viscosity (Laplacian coefficient zero) as the reference.
 hx_status hx_fix_admissibility(hx_mesh* mesh, const hx_update_options* o, double stability_ratio, int* n_iters_out)
 {
-  if (!mesh || !o || !n_iters_out) return HX_ERR_INVALID;
+  if (!live(mesh) || !o || !n_iters_out) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
   Dev_mesh& dm = mesh->dm;
   hx_status s;
@@ -1789,7 +1830,7 @@ hx_status hx_fix_admissibility(hx_mesh* mesh, const hx_update_options* o, double
 // Solver::update_art_visc_smoothness (Solver.cpp:457-582) with diffuse_art_visc (Solver.cpp:428-455)
 hx_status hx_update_art_visc_smoothness(hx_mesh* mesh, const hx_av_options* o, double* residual)
 {
-  if (!mesh || !o) return HX_ERR_INVALID;
+  if (!live(mesh) || !o) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
   const Dev_mesh& dm = mesh->dm;
   for (int c = 0; c < 2; ++c)

@@ -659,8 +659,10 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
   if ((fuse.block_min || fuse.point_scale[cls]) && valid && !compute_residual) {
     const double scale = fuse.visc ? max_dt_scale_u<ND, RS, true>(m, gops, fuse.ns, cls, e, q, u, fuse.max_cfl_c, fuse.max_cfl_d)
                                    : max_dt_scale_u<ND, RS, false>(m, gops, fuse.ns, cls, e, q, u, fuse.max_cfl_c, fuse.max_cfl_d);
-    c = 1. / scale;
-    if (fuse.point_scale[cls]) fuse.point_scale[cls][size_t(e) * NQ + q] = c;
+    // folded as k_max_dt does: a NaN of this point never wins (std::min(dt, NaN) keeps dt in the reference)
+    const double inv = 1. / scale;
+    c = (inv < c) ? inv : c;
+    if (fuse.point_scale[cls]) fuse.point_scale[cls][size_t(e) * NQ + q] = inv;
   }
   // Write_face of the updated state (Spatial.hpp:507)
   if (pt)

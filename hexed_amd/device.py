@@ -94,8 +94,9 @@ class Device_mesh:
         ctx._meshes.add(self)
 
     def close(self):
-        # a mesh whose context is already destroyed has been released with it (Context.close)
-        if self.ptr and self.ctx.ptr:
+        # valid in either order: a mesh whose context was destroyed first holds no device memory any more, and
+        # hx_mesh_destroy then releases only its handle
+        if self.ptr:
             self.lib.hx_mesh_destroy(self.ptr)
         self.ptr = ctypes.c_void_p()
 

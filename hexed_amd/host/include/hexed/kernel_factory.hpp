@@ -7,6 +7,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 #include "Sequence.hpp"
 #include "Stopwatch_tree.hpp"
@@ -21,15 +23,23 @@ void synchronize();  // waits for the device work of the process-wide context
 
 constexpr int max_row_size = 8;  // config::max_row_size of the reference build (CMakeLists.txt:74-77)
 
-//! waits for the device in its destructor, so a stopwatch declared before it stops after the device work finished
-struct Device_sync_guard
-{
-  ~Device_sync_guard() { amd::synchronize(); }
-};
-
 template <typename T, typename U = void>
 class Kernel
 {
+  // runs the kernel, then waits for its device work, so that a stopwatch running around this call stops after the
+  // device finished; an error of the device work is thrown here, never from a destructor
+  U timed_call(Sequence<T>& sequence)
+  {
+    if constexpr (std::is_void_v<U>) {
+      (*this)(sequence);
+      amd::synchronize();
+    } else {
+      U result = (*this)(sequence);
+      amd::synchronize();
+      return result;
+    }
+  }
+
   public:
   virtual ~Kernel() = default;
   virtual U operator()(Sequence<T>&) = 0;
@@ -37,8 +47,7 @@ class Kernel
   {
     tree.work_units_completed += sequence.size();
     Stopwatch::Operator oper(tree.stopwatch);
-    Device_sync_guard sync;
-    return (*this)(sequence);
+    return timed_call(sequence);
   }
   virtual U operator()(Sequence<T>& sequence, Stopwatch_tree& category_tree, std::string name)
   {
@@ -46,8 +55,7 @@ class Kernel
     specific_tree.work_units_completed += sequence.size();
     Stopwatch::Operator cat_oper(category_tree.stopwatch);
     Stopwatch::Operator spc_oper(specific_tree.stopwatch);
-    Device_sync_guard sync;
-    return (*this)(sequence);
+    return timed_call(sequence);
   }
   typedef std::unique_ptr<Kernel<T, U>> ptr_t;
 };
@@ -58,45 +66,22 @@ namespace kernel_lookup
 template <template <int, int> typename kernel>
 using ptr_t = typename kernel<1, 2>::ptr_t;
 
-// compile-time list of every (n_dim, row_size) instance, searched from (max_n_dim, max_row_size) downwards
-template <template <int, int> typename kernel, int max_n_dim, int max_row_size_>
-class Kernel_lookup
+// Every valid (n_dim, row_size) is one index of a flat compile-time range, n_dim-major; a fold over that range
+// constructs the one instance whose index matches the requested pair.
+constexpr int n_row_sizes = max_row_size - 1;  // row sizes 2 .. max_row_size
+constexpr int n_instances = 3 * n_row_sizes;
+constexpr int instance_n_dim(int index) { return index / n_row_sizes + 1; }
+constexpr int instance_row_size(int index) { return index % n_row_sizes + 2; }
+
+template <template <int, int> typename kernel, int... index, typename... A>
+ptr_t<kernel> construct(std::integer_sequence<int, index...>, int n_dim, int row_size, A&... args)
 {
-  Kernel_lookup<kernel, max_n_dim, max_row_size_ - 1> next;
-
-  public:
-  template <typename... A>
-  ptr_t<kernel> get(int n_dim, int row_size, A&&... args)
-  {
-    if (n_dim == max_n_dim && row_size == max_row_size_) return ptr_t<kernel>{new kernel<max_n_dim, max_row_size_>(args...)};
-    return next.get(n_dim, row_size, args...);
-  }
-};
-
-template <template <int, int> typename kernel, int max_n_dim>
-class Kernel_lookup<kernel, max_n_dim, 1>
-{
-  Kernel_lookup<kernel, max_n_dim - 1, max_row_size> next;
-
-  public:
-  template <typename... A>
-  ptr_t<kernel> get(int n_dim, int row_size, A&&... args)
-  {
-    if (n_dim == max_n_dim) return ptr_t<kernel>{new kernel<max_n_dim, 1>(args...)};
-    return next.get(n_dim, row_size, args...);
-  }
-};
-
-template <template <int, int> typename kernel>
-class Kernel_lookup<kernel, 1, 1>
-{
-  public:
-  template <typename... A>
-  ptr_t<kernel> get(int, int, A&&... args)
-  {
-    return ptr_t<kernel>{new kernel<1, 1>(args...)};
-  }
-};
+  const int wanted = (n_dim - 1) * n_row_sizes + (row_size - 2);
+  ptr_t<kernel> made;
+  ((index == wanted ? (void)made.reset(new kernel<instance_n_dim(index), instance_row_size(index)>(args...)) : (void)0),
+   ...);
+  return made;
+}
 
 }  // namespace kernel_lookup
 
@@ -107,8 +92,8 @@ kernel_lookup::ptr_t<kernel> kernel_factory(int n_dim, int row_size, A&&... args
   if ((n_dim < 1) || (n_dim > 3) || (row_size < 2) || (row_size > max_row_size)) {
     throw std::runtime_error("demand for invalid kernel");
   }
-  kernel_lookup::Kernel_lookup<kernel, 3, max_row_size> lookup;
-  return lookup.get(n_dim, row_size, args...);
+  return kernel_lookup::construct<kernel>(std::make_integer_sequence<int, kernel_lookup::n_instances>{}, n_dim,
+                                          row_size, args...);
 }
 
 }  // namespace hexed

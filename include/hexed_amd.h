@@ -244,6 +244,9 @@ typedef struct hx_mesh hx_mesh;
 
 /* ---- context ---------------------------------------------------------------------------------------- */
 hx_status hx_ctx_create(int device, hx_ctx** out);
+/* Destroys a context. Meshes of the context that are still alive are detached first: their device memory is freed,
+ * their handles stay valid, hx_mesh_destroy on them then only releases the handle, and every other call on them
+ * returns HX_ERR_INVALID. Either destruction order is therefore safe; destroying meshes first is the usual one. */
 hx_status hx_ctx_destroy(hx_ctx* ctx);
 const char* hx_last_error(const hx_ctx* ctx);
 hx_status hx_ctx_synchronize(hx_ctx* ctx);
@@ -283,6 +286,7 @@ double hx_basis_step_ratio(int row_size);
  * Kernel_mesh sequences (Kernel_mesh.hpp:14-25) with one registration. Fails with the reference's message
  * "demand for invalid kernel" unless 1 <= n_dim <= 3 and 2 <= row_size <= 8 (kernel_factory.hpp:114-116). */
 hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* desc, hx_mesh** out);
+/* Waits for the mesh's pending work and frees it; valid before or after hx_ctx_destroy of its context. */
 hx_status hx_mesh_destroy(hx_mesh* mesh);
 /* The checks hx_mesh_create applies before touching the device (sizes, every slot / normal / connection index in
  * range, required arrays present); host only. Returns HX_OK or HX_ERR_INVALID with the reason in `msg`. */
@@ -299,6 +303,10 @@ hx_status hx_mesh_set_precision(hx_mesh* mesh, int precision);
  * it obtained before that call must call hx_mesh_device_ptr again (which marks the mesh changed) before the next
  * hx_update. */
 hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size_t* n_doubles);
+/* Marks the mesh's device data as changed by the caller (no counterpart in the reference, which has no cached state):
+ * call it after writing through a pointer from hx_mesh_device_ptr, so that the next hx_update computes Max_dt afresh
+ * instead of reusing the minimum its previous call fused into the last stage. */
+hx_status hx_mesh_mark_changed(hx_mesh* mesh);
 
 /* ---- kernels: one per reference kernel class (Spatial.hpp) ------------------------------------------ */
 /* Spatial<Pde, def>::Write_face (Spatial.hpp:29-71): extrapolate element state to its faces. */

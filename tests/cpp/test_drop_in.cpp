@@ -504,6 +504,15 @@ static void compare_update(int nd, int rs, int n, bool def, bool visc, int bc_ki
               sw.children.at("neighbor").work_units_completed, sw.children.at("local").work_units_completed);
   REQUIRE(err <= 1e-12);
   REQUIRE(sw.children.at("local").work_units_completed == (long long)b.elems.size());
+  // an error inside a timed kernel call reaches the caller as std::runtime_error and leaves the stopwatches stopped
+  // (the Smooth_art_visc PDE needs the AV generator's fields, which this mesh does not have)
+  {
+    auto bad = kernel_factory<Spatial<pde::Smooth_art_visc, false>::Local>(nd, rs, basis, dt, 0, false, false, 1., 1.);
+    const std::string what = throws([&] { (*bad)(car, sw, "local"); });
+    std::printf("  timed Local with a failing status: \"%s\"\n", what.c_str());
+    REQUIRE(!what.empty());
+    REQUIRE(!sw.stopwatch.running() && !sw.children.at("local").stopwatch.running());
+  }
 
   // Solver::is_admissible through the drop-in: admissible, then not after a negative density, then the error text of a
   // non-finite value (thermo.cpp:14)
