@@ -56,6 +56,8 @@ def load():
         "oracle_calc_jacobian": (I, [M, ctypes.POINTER(abi.hx_vertex_data)]),
         "oracle_is_admissible": (I, [M, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "oracle_fix_coefficient": (I, [M, ctypes.POINTER(I), ctypes.POINTER(D)]),
+        "oracle_stab_art_visc": (I, [M, D, ctypes.POINTER(D)]),
+        "oracle_set_art_visc_admis": (I, [M, D, ctypes.POINTER(D), ctypes.POINTER(D)]),
         "oracle_compute_fix_therm_admis": (I, [M, KO]),
         "oracle_max_dt_fix_therm_admis": (I, [M, KO, D, D, I, ctypes.POINTER(D)]),
         "oracle_fix_admissibility": (I, [M, ctypes.POINTER(abi.hx_update_options), D, ctypes.POINTER(I)]),

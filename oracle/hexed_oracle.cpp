@@ -2167,6 +2167,92 @@ std::vector<double> fix_coefficient(hx_host_mesh* m, const std::vector<int>& rec
   return vc;
 }
 
+// Stab_art_visc (stabilizing_art_visc.cpp:8-61): per element, the specific volume's share of energy in the top
+// Legendre mode along each dimension, (sum over rows of (row . orth_{rs-1} w)^2 face_weight) / (n_dim ||1/rho||^2),
+// mapped onto [0, 1] by a sine ramp of half width 0.5 centred at log10 = -4.25 log10(rs - 1), times
+// (rs - 1) char_speed h. Returns [n_elem[0] + n_elem[1]] (Cartesian first).
+std::vector<double> stab_art_visc(const hx_host_mesh* m, double char_speed)
+{
+  const int nd = m->n_dim, rs = m->row_size, nq = ipow(rs, nd), nfq = nq / rs;
+  const Basis basis(rs);
+  const double ramp_center = -4.25 * std::log(rs - 1) / std::log(10), half_width = 0.5;
+  std::vector<double> proj(rs), qw(nq), fw(nfq);
+  for (int i = 0; i < rs; ++i) proj[i] = basis.ortho[(rs - 1) * rs + i] * basis.weight[i];
+  for (int q = 0; q < nq; ++q) {
+    qw[q] = 1.;
+    for (int d = 0; d < nd; ++d) qw[q] *= basis.weight[(q / ipow(rs, nd - 1 - d)) % rs];
+  }
+  for (int f = 0; f < nfq; ++f) {
+    fw[f] = 1.;
+    for (int d = 0; d < nd - 1; ++d) fw[f] *= basis.weight[(f / ipow(rs, nd - 2 - d)) % rs];
+  }
+  std::vector<double> out(size_t(m->n_elem[0]) + m->n_elem[1]);
+  for (int c = 0; c < 2; ++c) {
+#pragma omp parallel for
+    for (int e = 0; e < m->n_elem[c]; ++e) {
+      const double* state = m->state[c] + size_t(e) * (nd + 2) * nq;
+      std::vector<double> iv(nq);
+      double norm_sq = 0;
+      for (int q = 0; q < nq; ++q) {
+        iv[q] = 1. / state[nd * nq + q];
+        norm_sq += iv[q] * iv[q] * qw[q];
+      }
+      double nonsmooth = 0;
+      for (int i_dim = 0; i_dim < nd; ++i_dim) {
+        const int stride = ipow(rs, nd - 1 - i_dim);
+        for (int f = 0; f < nfq; ++f) {
+          // Row_index (Row_index.hpp:14-64): row f along i_dim starts at (f / stride) * stride * rs + f % stride
+          const int base = (f / stride) * stride * rs + f % stride;
+          double dot = 0;
+          for (int k = 0; k < rs; ++k) dot += iv[base + k * stride] * proj[k];
+          nonsmooth += dot * dot * fw[f];
+        }
+      }
+      nonsmooth /= norm_sq * nd;
+      double indicator = std::log(nonsmooth) / std::log(10);
+      if (indicator <= ramp_center - half_width) indicator = 0;
+      else if (indicator < ramp_center + half_width)
+        indicator = .5 * (1 + std::sin(M_PI * (indicator - ramp_center) / 2 / half_width));
+      else indicator = 1;
+      out[(c ? m->n_elem[0] : 0) + e] = (rs - 1) * char_speed * m->nom_size[c][e] * indicator;
+    }
+  }
+  return out;
+}
+
+// Solver::set_art_visc_admis (Solver.cpp:636-659): Stab_art_visc's element values on every vertex, shared by maximum
+// (hanging vertices matched), interpolated n-linearly into the Laplacian artificial viscosity; returns the vertex values
+std::vector<double> set_art_visc_admis(hx_host_mesh* m, double char_speed, std::vector<double>* uncert)
+{
+  const int nd = m->n_dim, rs = m->row_size, nq = ipow(rs, nd), nvert = 1 << nd;
+  const Vertex_data& V = g_vertices;
+  if (!V.set || V.n_elem[0] != m->n_elem[0] || V.n_elem[1] != m->n_elem[1])
+    throw std::runtime_error("oracle: set_art_visc_admis needs the mesh's vertices (oracle_set_vertices)");
+  for (int c = 0; c < 2; ++c)
+    if (m->n_elem[c] && !m->av[c]) throw std::runtime_error("oracle: set_art_visc_admis needs the av arrays");
+  const std::vector<double> u = stab_art_visc(m, char_speed);
+  if (uncert) *uncert = u;
+  std::vector<double> vc(u.size() * nvert);
+  for (size_t E = 0; E < u.size(); ++E)
+    for (int v = 0; v < nvert; ++v) vc[E * nvert + v] = u[E];
+  share_vertex(m, V, vc, false);
+  // Gauss_lobatto(2).interpolate(nodes): the linear interpolant of the two end values (Gauss_lobatto.cpp)
+  const Basis basis(rs);
+  std::vector<double> interp(size_t(rs) * 2);
+  for (int i = 0; i < rs; ++i) {
+    interp[2 * i] = 1. - basis.node[i];
+    interp[2 * i + 1] = basis.node[i];
+  }
+  for (int c = 0; c < 2; ++c)
+    for (int e = 0; e < m->n_elem[c]; ++e) {
+      const size_t E = (c ? m->n_elem[0] : 0) + e;
+      const std::vector<double> vf(vc.begin() + E * nvert, vc.begin() + (E + 1) * nvert);
+      const std::vector<double> lap = hypercube_matvec(interp, rs, 2, vf);
+      std::copy(lap.begin(), lap.end(), m->av[c] + (size_t(e) * 2 + 1) * nq);
+    }
+  return vc;
+}
+
 // ---- mesh metrics (Solver::calc_jacobian, Solver.cpp:273-381) ------------------------------------------------------
 
 // math::dimension_matvec (math.cpp:58-84): `mat` (rows x cols) applied along dimension i_dim (0 slowest) of a
@@ -2697,6 +2783,26 @@ int oracle_fix_coefficient(hx_host_mesh* m, const int* record, double* vertex_ou
       if (m->n_elem[c] && !m->av[c]) throw std::runtime_error("oracle: the fix coefficient needs the av arrays");
     std::vector<int> rec(record, record + m->n_elem[0] + m->n_elem[1]);
     const std::vector<double> vc = fix_coefficient(m, rec);
+    if (vertex_out) std::copy(vc.begin(), vc.end(), vertex_out);
+  });
+}
+
+int oracle_stab_art_visc(hx_host_mesh* m, double char_speed, double* uncert)
+{
+  return guarded([&] {
+    check_mesh(m);
+    const std::vector<double> u = stab_art_visc(m, char_speed);
+    std::copy(u.begin(), u.end(), uncert);
+  });
+}
+
+int oracle_set_art_visc_admis(hx_host_mesh* m, double char_speed, double* uncert, double* vertex_out)
+{
+  return guarded([&] {
+    check_mesh(m);
+    std::vector<double> u;
+    const std::vector<double> vc = set_art_visc_admis(m, char_speed, &u);
+    if (uncert) std::copy(u.begin(), u.end(), uncert);
     if (vertex_out) std::copy(vc.begin(), vc.end(), vertex_out);
   });
 }

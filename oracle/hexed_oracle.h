@@ -63,6 +63,11 @@ int oracle_set_vertices(const hx_host_mesh* m, int n_vertex, const int* vid_cart
 int oracle_calc_jacobian(hx_host_mesh* m, const hx_vertex_data* vertices);              /* Solver.cpp:273-381  */
 int oracle_is_admissible(hx_host_mesh* m, int* admissible, int* record);                   /* Solver.cpp:921-959  */
 int oracle_fix_coefficient(hx_host_mesh* m, const int* record, double* vertex_out);        /* Solver.cpp:999-1030 */
+/* Stab_art_visc (stabilizing_art_visc.cpp:8-61): uncert[n_elem[0] + n_elem[1]], Cartesian first */
+int oracle_stab_art_visc(hx_host_mesh* m, double char_speed, double* uncert);
+/* Solver::set_art_visc_admis (Solver.cpp:636-659) into the Laplacian artificial viscosity (needs oracle_set_vertices);
+ * uncert and vertex_out (may be NULL) receive the element values and the shared vertex values */
+int oracle_set_art_visc_admis(hx_host_mesh* m, double char_speed, double* uncert, double* vertex_out);
 int oracle_compute_fix_therm_admis(hx_host_mesh* m, const hx_kernel_options* o);          /* kernels_diffusive.cpp:32 */
 int oracle_max_dt_fix_therm_admis(hx_host_mesh* m, const hx_kernel_options* o, double msc, double msd,
                                   int local_time, double* dt_out);                         /* kernels_max_dt.cpp:21 */
