@@ -12,17 +12,26 @@ face fluxes and jumps, metrics scaling and max_dt in FP64). `value` is the form 
 fp32_mfma, the headline); the line also carries the other form's rate, the speedup, and the FP32-MFMA form's relative
 L2 error per conserved variable against the FP64 form after the same number of steps.
 
+The timed region enqueues the steps with hx_update_async: each step's time step stays on the device and the
+steady-state update is one launch of a captured HIP graph, so the host never waits inside the region. The same
+steps are then run again through hx_update (the time step read back every step) with HIP events around every
+kernel launch; that eager region gives the kernel breakdown and the roofline's launch time, and its rate is
+reported as `eager`.
+
 The JSON line also carries
   roofline:     the chosen form's dominant kernel: algorithmic HBM bytes per launch / its average launch time,
-                measured with HIP events on the library's stream over the timed region, against the 8 TB/s HBM peak;
+                measured with HIP events on the library's stream over the eager timed region, against the 8 TB/s
+                HBM peak;
   cpu_baseline: the C++/OpenMP oracle (a loop-for-loop restatement of the reference CPU path) timed on this
                 host's cores for a few steps of the same mesh;
   parity_vs_cpu: relative L2 error per conserved variable between the FP64 GPU form and the oracle after those steps.
 
-The batch workload (batch64) holds 64 independent cases in one mesh per row size; a step is one hx_update_cases
-per mesh, every case with its own time step.
+The batch workload (batch64) holds 64 independent cases in one mesh per row size; the three meshes form one group
+(hx_group), so a step is one hx_group_update: every kernel of an Euler stage launched once for all meshes, classes
+and row sizes, every case with its own time step, with the fused Local's launch shape picked by the autotuner
+(hx_group_autotune, run for each form before its timed region; its table is in the JSON line).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32|amr16|batch64]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config ns48|euler32|amr16|amr16_rs4|batch64]
                        [--precision fp32_mfma|fp64] [--cpu-steps N] [--err-steps N]
 """
 import argparse
@@ -38,7 +47,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 from hexed_amd import abi, configs  # noqa: E402
-from hexed_amd.device import Context, Device_mesh  # noqa: E402
+from hexed_amd.device import Context, Device_group, Device_mesh  # noqa: E402
 
 METRIC = "DOF-stage/s, 3D NS row_size 6: FP32-MFMA vs FP64 GPU at stated L2 err"
 HBM_PEAK_GBS = 8000.0
@@ -55,6 +64,11 @@ WORKLOADS = {
              "Prolong/Restrict_refined), row_size 6, Mach 0.8 + 1% seeded Fourier perturbations, freestream "
              "boundaries, global time step at max_safety 0.7",
         build=lambda: configs.euler_tree(), visc=False, cpu_steps=200, bytes_per_dof_stage=128),
+    "amr16_rs4": dict(
+        desc="3D Euler on the same seeded 2:1-balanced tree (base 16^3, 19,461 leaves over 4 levels, 7,691 hanging "
+             "faces), row_size 4, Mach 0.8 + 1% seeded Fourier perturbations, freestream boundaries, global time "
+             "step at max_safety 0.7",
+        build=lambda: configs.euler_tree(rs=4), visc=False, cpu_steps=400, bytes_per_dof_stage=128),
     "euler32": dict(
         desc="3D Euler, 32^3 Cartesian hexes, row_size 6, Mach 0.8 + 1% seeded Fourier perturbations, "
              "freestream boundaries, global time step at max_safety 0.7",
@@ -123,6 +137,58 @@ def measured_traffic(config, symbol):
     return table.get("bytes_per_launch", {}).get(symbol)
 
 
+def time_artificial_viscosity(ctx, m, reps=3):
+    """The Navier-Stokes workload's artificial-viscosity kernels on its own mesh, outside the time step (the workload
+    itself runs a constant artificial viscosity): Solver::update_art_visc_smoothness (advection, Legendre projection,
+    Chebyshev diffusion of the forcing, Solver.cpp:457-582) with the reference's defaults and advection length 0.1, and
+    Solver::set_art_visc_admis (Stab_art_visc, vertex sharing, interpolation; Solver.cpp:636-659). Each is run once,
+    then `reps` times with HIP events around every kernel launch. Returns milliseconds per call and the per-kernel
+    breakdown."""
+    rho, u, p, _ = configs.ns_freestream()
+    host = m.copy().enable_av_generator()
+    dev = Device_mesh(ctx, host)
+    reset(dev, host)
+    opts = abi.av_options(0.1, host.row_size, rho, p / 0.4 + 0.5 * rho * u * u)
+    vd = host.vertex_data()
+    char_speed = u + (1.4 * p / rho) ** 0.5
+    out = {}
+    for name, call in (("update_art_visc_smoothness", lambda: dev.update_art_visc_smoothness(opts)),
+                       ("set_art_visc_admis", lambda: dev.set_art_visc_admis(char_speed, vd))):
+        call()
+        ctx.synchronize()
+        ctx.set_timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        t = ctx.timings()
+        ctx.set_timing(False)
+        out[name] = {"ms_per_call": round(1e3 * wall, 3),
+                     "kernels": {k: {"ms_per_call": round(v[0] / reps, 4), "launches": v[1] // reps}
+                                 for k, v in sorted(t.items())}}
+    dev.close()
+    return out
+
+
+def host_cpu():
+    """The host's CPU model (lscpu), its logical CPU count (nproc) and the CPUs this process may run on."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith("Model name")), None)
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc", "--all"], capture_output=True, text=True, timeout=10).stdout)
+    except (OSError, subprocess.SubprocessError, ValueError):
+        nproc = os.cpu_count()
+    return {"model": model, "nproc_all": nproc, "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def run_oracle_steps(m, opts, steps):
     import oracle
     lib = oracle.load()
@@ -162,36 +228,75 @@ def rel_l2_per_var(got, ref, nv):
     return errs
 
 
-def step(groups, opts):
+def step(groups, opts, n=1, fused=None):
+    """n time steps of every mesh: hx_update, hx_update_cases for a batch mesh, or one hx_group_update for all of them
+    when they form a group (`fused`: (Device_group, status array))."""
+    if fused is not None:
+        for _ in range(n):
+            fused[0].update(opts, fused[1])
+        return
     for m, dev, nc, status in groups:
-        if nc is None:
-            dev.update(opts, status)
-        else:
-            dev.update_cases(opts, status)
+        for _ in range(n):
+            if nc is None:
+                dev.update(opts, status)
+            else:
+                dev.update_cases(opts, status)
+
+
+def step_async(groups, opts, n, fused=None):
+    """n time steps of every mesh enqueued with hx_update_async (batch meshes step as in `step`, reading their per-case
+    time steps back)."""
+    if fused is not None or any(nc is not None for _, _, nc, _ in groups):
+        return step(groups, opts, n, fused)
+    for m, dev, nc, status in groups:
+        dev.update_async(opts, n)
 
 
 def fresh_status(nc):
     return abi.hx_update_status() if nc is None else (abi.hx_update_status * nc)()
 
 
-def timed_run(ctx, groups, opts, form, warmup, steps):
+def timed_run(ctx, groups, opts, form, warmup, steps, group=None):
+    """The timed region on the graph path (from the initial state, after `warmup` steps), then the same steps again
+    from the initial state through hx_update with every kernel launch timed. A batch workload's group is autotuned
+    first. Returns (graph seconds, eager seconds, per-kernel timings of the eager region, final status, autotune
+    table or None)."""
     groups = [(m, dev, nc, fresh_status(nc)) for m, dev, nc, _ in groups]
     for m, dev, nc, _ in groups:
         dev.set_precision(FORMS[form][0])
         reset(dev, m)
-    for _ in range(warmup):
-        step(groups, opts)
+    fused, table = None, None
+    if group is not None:
+        table = group.autotune(use_filter=bool(opts.use_filter), reps=5)
+        for m, dev, nc, _ in groups:
+            reset(dev, m)
+        fused = (group, (abi.hx_update_status * group.n_cases)())
+    step_async(groups, opts, warmup, fused)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    step_async(groups, opts, steps, fused)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    for m, dev, nc, status in groups:
+        if nc is None:
+            dev.status_sync(status)
+    st = fused[1] if fused else groups[0][3]
+    status = st if (groups[0][2] is None and not fused) else st[0]
+    groups = [(m, dev, nc, fresh_status(nc)) for m, dev, nc, _ in groups]
+    for m, dev, nc, _ in groups:
+        reset(dev, m)
+    if fused:
+        fused = (group, (abi.hx_update_status * group.n_cases)())
+    step(groups, opts, warmup, fused)
     ctx.synchronize()
     ctx.set_timing(True)
     ctx.timing_reset()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        step(groups, opts)
+    step(groups, opts, steps, fused)
     ctx.synchronize()
-    elapsed = time.perf_counter() - t0
+    eager = time.perf_counter() - t0
     ctx.set_timing(False)
-    st = groups[0][3]
-    return elapsed, ctx.timings(), (st if groups[0][2] is None else st[0])
+    return elapsed, eager, ctx.timings(), status, table
 
 
 def main():
@@ -203,6 +308,7 @@ def main():
     ap.add_argument("--precision", default="fp32_mfma", choices=sorted(FORMS), help="form reported as `value`")
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps for the CPU baseline (0 to skip)")
     ap.add_argument("--err-steps", type=int, default=10, help="steps after which FP32-MFMA is compared with FP64")
+    ap.add_argument("--av", type=int, default=1, help="ns48: also time the artificial-viscosity kernels (0 to skip)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -233,12 +339,14 @@ def main():
         if nc is not None:
             dev.set_cases(nc, m.case_id)
         groups.append((m, dev, nc, None))
+    group = Device_group(ctx, [dev for _, dev, _, _ in groups]) if args.config == "batch64" else None
     runs = {}
     for form in FORMS:
-        elapsed, timings, status = timed_run(ctx, groups, opts, form, args.warmup, args.steps)
-        runs[form] = dict(elapsed=elapsed, timings=timings, value=n_dof * 2 * args.steps / elapsed)
-        print(f"[bench] GPU {form}: {args.steps} steps in {elapsed:.3f} s, dt {status.time_step:.3e}",
-              file=sys.stderr, flush=True)
+        elapsed, eager, timings, status, table = timed_run(ctx, groups, opts, form, args.warmup, args.steps, group)
+        runs[form] = dict(elapsed=elapsed, eager=eager, timings=timings, value=n_dof * 2 * args.steps / elapsed,
+                          autotune=table)
+        print(f"[bench] GPU {form}: {args.steps} steps in {elapsed:.3f} s (graph), {eager:.3f} s (eager, "
+              f"events), dt {status.time_step:.3e}", file=sys.stderr, flush=True)
     chosen = runs[args.precision]
     value, elapsed, timings = chosen["value"], chosen["elapsed"], chosen["timings"]
 
@@ -249,8 +357,7 @@ def main():
         for m, dev, nc, _ in gs:
             dev.set_precision(FORMS[form][0])
             reset(dev, m)
-        for _ in range(args.err_steps):
-            step(gs, opts)
+        step(gs, opts, args.err_steps, (group, (abi.hx_update_status * group.n_cases)()) if group else None)
         finals[form] = [a for m, dev, _, _ in gs for a in download_state(dev, m)]
     fp32_err = rel_l2_per_var(finals["fp32_mfma"], finals["fp64"], nv)
     print(f"[bench] FP32-MFMA vs FP64 after {args.err_steps} steps: rel L2 per variable "
@@ -258,20 +365,34 @@ def main():
     for m, dev, _, _ in groups:
         dev.set_precision(abi.HX_FP64)
 
+    aux = None
+    if args.config == "ns48" and args.av:
+        aux = time_artificial_viscosity(ctx, hosts[0][0])
+        print("[bench] artificial viscosity: " + ", ".join(f"{k} {v['ms_per_call']} ms" for k, v in aux.items()),
+              file=sys.stderr, flush=True)
+
     # dominant kernel of the chosen form and its roofline: algorithmic bytes of every launch in its category over the
     # time spent in them (for one mesh: bytes per launch over the average launch time)
     dominant = max(timings, key=lambda k: timings[k][0])
     ms, launches, units = timings[dominant]
-    cls = abi.HX_DEFORMED if dominant.startswith("deformed") else abi.HX_CARTESIAN
-    per_round = [algorithmic_bytes(dominant, nd, m.nq, m.n_elem[cls]) for m, _, _ in hosts]
+    if dominant.startswith("fused/"):
+        # one fused launch covers every mesh and class: its bytes are those of all of them
+        per_round = [sum(algorithmic_bytes(f"{('cartesian', 'deformed')[c]}/local/euler", nd, m.nq, m.n_elem[c])
+                         for c in (0, 1)) for m, _, _ in hosts]
+        per_round = [sum(per_round)]
+        launches_per_mesh = launches
+    else:
+        cls = abi.HX_DEFORMED if dominant.startswith("deformed") else abi.HX_CARTESIAN
+        per_round = [algorithmic_bytes(dominant, nd, m.nq, m.n_elem[cls]) for m, _, _ in hosts]
+        launches_per_mesh = launches / len(hosts)
     symbol = kernel_symbol(dominant, nd, hosts[0][0].row_size, args.precision) if len(hosts) == 1 else None
     roofline = {"bound": "hbm", "kernel": dominant, "symbol": symbol, "launch_ms": round(ms / launches, 5),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "traffic": measured_traffic(args.config, symbol) if symbol else None,
                 "stream_copy_gbs": round(copy_gbs, 1)}
     if None not in per_round:
-        alg = sum(per_round)  # one launch per mesh
-        achieved = alg * (launches / len(hosts)) / (ms * 1e-3) / 1e9
+        alg = sum(per_round)  # one launch per mesh, or one fused launch for all
+        achieved = alg * launches_per_mesh / (ms * 1e-3) / 1e9
         roofline.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "algorithmic_bytes": alg})
     step_gbs = value * wl["bytes_per_dof_stage"] / 1e9
@@ -312,7 +433,7 @@ def main():
         what = (f"the same {n_elem}-element mesh" if len(groups) == 1 else
                 f"every case of the batch, one at a time ({cpu_dof} DOF)")
         cpu = {"value": round(cpu_dof * 2 * cpu_steps / cpu_s, 1), "unit": "DOF-stage/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "host": host_cpu(),
                "sample": f"{cpu_steps} time steps ({2 * cpu_steps} stages) of {what}, FP64, OpenMP over {threads} "
                          f"threads, {cpu_s:.2f} s"}
         # the same oracle on one thread, over a shorter sample of the same meshes (timing only)
@@ -342,18 +463,27 @@ def main():
         "config": {"workload": wl["desc"], "name": args.config, "n_elem": n_elem, "dof": n_dof,
                    "row_size": sorted({m.row_size for m, _, _ in hosts}), "precision": args.precision},
         "forms": {
-            args.precision: {"value": round(value, 1), "ms_per_step": round(1e3 * elapsed / args.steps, 4)},
+            args.precision: {"value": round(value, 1), "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+                             "eager": {"value": round(n_dof * 2 * args.steps / chosen["eager"], 1),
+                                       "ms_per_step": round(1e3 * chosen["eager"] / args.steps, 4)}},
             other: {"value": round(runs[other]["value"], 1),
-                    "ms_per_step": round(1e3 * runs[other]["elapsed"] / args.steps, 4)},
+                    "ms_per_step": round(1e3 * runs[other]["elapsed"] / args.steps, 4),
+                    "eager": {"value": round(n_dof * 2 * args.steps / runs[other]["eager"], 1),
+                              "ms_per_step": round(1e3 * runs[other]["eager"] / args.steps, 4)}},
             "fp32_mfma_speedup_vs_fp64": round(runs["fp32_mfma"]["value"] / runs["fp64"]["value"], 3),
+            "autotune": ({f: [{"order": o, "workgroups_per_cu": w, "ms": round(t, 4)} for o, w, t in runs[f]["autotune"]]
+                          for f in FORMS} if group else None),
             "fp32_mfma_vs_fp64_rel_l2_per_var": {"steps": args.err_steps,
                                                   "vars": [f"momentum{i}" for i in range(nd)] + ["mass", "energy"],
                                                   "rel_l2": [float(f"{e:.3e}") for e in fp32_err],
                                                   "max": float(f"{max(fp32_err):.3e}")},
         },
         "roofline": roofline, "cpu_baseline": cpu, "parity_vs_cpu": parity, "kernel_breakdown": breakdown,
+        "artificial_viscosity": aux,
     }
     print(json.dumps(line))
+    if group is not None:
+        group.close()
     for m, dev, _, _ in groups:
         dev.close()
     ctx.close()

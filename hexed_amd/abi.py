@@ -99,6 +99,10 @@ def update_options(max_safety=0.7, max_time_step=1e300, n_cheby_flow=1, flow_ite
                              int(use_ldg), visc, cond, int(fix_admis), fix_admis_max_safety)
 
 
+class hx_launch_shape(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int), ("workgroups_per_cu", ctypes.c_int), ("ms", ctypes.c_double)]
+
+
 class hx_av_options(ctypes.Structure):
     _fields_ = [("advect_length", ctypes.c_double), ("advect_iters", ctypes.c_int), ("advect_max_safety", ctypes.c_double),
                 ("diff_ratio", ctypes.c_double), ("diff_max_safety", ctypes.c_double), ("n_cheby", ctypes.c_int),
@@ -215,6 +219,16 @@ def load_library(path=LIB_PATH):
         "hx_compute_prolong": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "hx_compute_restrict": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "hx_update": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.POINTER(hx_update_status)]),
+        "hx_update_async": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.c_int,
+                                           ctypes.POINTER(hx_update_status)]),
+        "hx_update_status_sync": (ctypes.c_int, [P, ctypes.POINTER(hx_update_status)]),
+        "hx_group_create": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)]),
+        "hx_group_destroy": (ctypes.c_int, [P]),
+        "hx_group_fused": (ctypes.c_int, [P]),
+        "hx_group_set_shape": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+        "hx_group_update": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.POINTER(hx_update_status)]),
+        "hx_group_autotune": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(hx_launch_shape), ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_int)]),
         "hx_mesh_set_cases": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "hx_update_cases": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.POINTER(hx_update_status)]),
         "hx_compute_residual": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options)]),
@@ -232,6 +246,8 @@ def load_library(path=LIB_PATH):
                                                ctypes.c_int, c_double_p]),
         "hx_update_art_visc_smoothness": (ctypes.c_int, [P, ctypes.POINTER(hx_av_options), c_double_p]),
         "hx_calc_jacobian": (ctypes.c_int, [P, ctypes.POINTER(hx_vertex_data)]),
+        "hx_stab_art_visc": (ctypes.c_int, [P, ctypes.c_double, c_double_p]),
+        "hx_set_art_visc_admis": (ctypes.c_int, [P, ctypes.POINTER(hx_vertex_data), ctypes.c_double, c_double_p]),
         "hx_is_admissible": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int)]),
         "hx_fix_admissibility": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.c_double,
                                                 ctypes.POINTER(ctypes.c_int)]),
@@ -257,9 +273,10 @@ EXPORTED_SYMBOLS = [
     "hx_neighbor", "hx_local", "hx_neighbor_reconcile", "hx_reconcile_ldg_flux", "hx_max_dt", "hx_prolong_refined",
     "hx_restrict_refined", "hx_apply_state_bcs", "hx_apply_flux_bcs", "hx_compute_euler", "hx_compute_navier_stokes",
     "hx_max_dt_euler", "hx_max_dt_navier_stokes", "hx_compute_write_face", "hx_compute_prolong", "hx_compute_restrict",
-    "hx_update", "hx_compute_residual", "hx_apply_advection_bcs", "hx_apply_avc_diff_bcs", "hx_apply_avc_diff_flux_bcs",
+    "hx_update", "hx_update_async", "hx_update_status_sync", "hx_group_create", "hx_group_destroy",
+    "hx_group_fused", "hx_group_set_shape", "hx_group_update", "hx_group_autotune", "hx_compute_residual", "hx_apply_advection_bcs", "hx_apply_avc_diff_bcs", "hx_apply_avc_diff_flux_bcs",
     "hx_compute_advection", "hx_compute_smooth_av", "hx_compute_write_face_advection", "hx_compute_write_face_smooth_av",
     "hx_compute_prolong_advection", "hx_max_dt_advection", "hx_max_dt_smooth_av", "hx_update_art_visc_smoothness",
     "hx_mesh_set_cases", "hx_update_cases", "hx_is_admissible", "hx_fix_admissibility", "hx_compute_fix_therm_admis",
-    "hx_max_dt_fix_therm_admis", "hx_calc_jacobian",
+    "hx_max_dt_fix_therm_admis", "hx_calc_jacobian", "hx_stab_art_visc", "hx_set_art_visc_admis",
 ]

@@ -42,6 +42,17 @@ struct hx_ctx
   std::vector<Pending> pending;
   std::map<std::string, Total> totals;
   std::vector<hx_mesh*> meshes;  // live meshes of this context, detached by hx_ctx_destroy
+  std::vector<hx_group*> groups; // live groups of this context, detached by hx_ctx_destroy
+  bool events_off = false;       // no per-launch events (a graph capture, or a call that may not create any)
+};
+
+// status of the updates enqueued by hx_update_async, kept on the device
+struct Step_dev
+{
+  double dt;          // the time step the stage kernels of the current step read
+  double time_step;   // hx_update_status::time_step
+  double flow_time;   // hx_update_status::flow_time
+  long long iteration;
 };
 
 struct hx_mesh
@@ -87,8 +98,36 @@ struct hx_mesh
     int use_ldg = 0;
     hx_transport visc{}, cond{};
   } case_cache;
+  // hx_update_async: the device status, the time step the stage kernels read while an update is enqueued with it
+  // (null otherwise), and the captured steady-state update with the options and form it was captured for
+  Step_dev* step = nullptr;
+  const double* dt_dev = nullptr;
+  int n_groups = 0;  // groups (hx_group) holding the mesh, which must be destroyed first
+  hipGraphExec_t graph = nullptr;
+  hx_update_options graph_opts{};
+  int graph_precision = -1;
 };
 
+
+// Meshes stepped together (hx_group_update): each kernel of an Euler stage launched once for all of them, both classes
+// and every row size, when every mesh is 3D with row size 4, 5 or 6 (the fused set); otherwise the meshes step one
+// after the other.
+struct hx_group
+{
+  hx_ctx* ctx = nullptr;
+  std::vector<hx_mesh*> meshes;
+  const hx::Fused_set* fs = nullptr;    // null: no fused launches
+  bool with_def = false;                // some mesh has deformed elements
+  int n_cases = 0;                      // over all meshes
+  hx::Fused_member* mem = nullptr;      // device [2][n]: stage 0 (no fusion), stage 1 (the batch's fused time step)
+  hx::Fused_plan* plans = nullptr;      // device [3]: state BCs, Neighbor, Local
+  hx::Fused_plan hplan[3]{};
+  std::vector<hx::Fused_member> hmem;   // host copy of `mem`
+  double* dts = nullptr;                // pinned host: every case's time step
+  int order = 2, per_cu = 0;            // launch shape of the fused Local (hx_group_set_shape)
+  int n_cu = 0;
+  const void* hops[HX_MAX_ROW_SIZE + 1] = {};  // the context's host Ops<rs> tables, by row size
+};
 
 namespace {
 
@@ -96,8 +135,16 @@ namespace {
 bool live(const hx_mesh* mesh) { return mesh && mesh->ctx; }
 
 // frees a mesh's device memory; the struct stays valid and holds no device pointer afterwards
+void release_graph(hx_mesh* mesh)
+{
+  if (mesh->graph) (void)hipGraphExecDestroy(mesh->graph);
+  mesh->graph = nullptr;
+  mesh->graph_precision = -1;
+}
+
 void release_mesh_memory(hx_mesh* mesh)
 {
+  release_graph(mesh);
   for (void* p : mesh->allocs) (void)hipFree(p);
   mesh->allocs.clear();
   mesh->dm = Dev_mesh{};
@@ -109,8 +156,24 @@ void release_mesh_memory(hx_mesh* mesh)
   mesh->case_dt = nullptr;
   mesh->dt_block_min = nullptr;
   mesh->admis_flags = nullptr;
+  mesh->step = nullptr;
+  mesh->dt_dev = nullptr;
   mesh->fuse = Dt_fuse{};
   mesh->dt_cache.valid = mesh->case_cache.valid = false;
+}
+
+// frees a group's memory and releases its meshes; the struct stays valid for hx_group_destroy
+void release_group(hx_group* g)
+{
+  if (g->mem) (void)hipFree(g->mem);
+  if (g->plans) (void)hipFree(g->plans);
+  if (g->dts) (void)hipHostFree(g->dts);
+  g->mem = nullptr;
+  g->plans = nullptr;
+  g->dts = nullptr;
+  for (hx_mesh* m : g->meshes) --m->n_groups;
+  g->meshes.clear();
+  g->ctx = nullptr;
 }
 
 hx_status fail(hx_ctx* ctx, hx_status code, const std::string& msg)
@@ -331,6 +394,7 @@ Launch make_launch(hx_mesh* mesh, bool writes = true)
   l.scalar = mesh->ctx->scalar;
   l.precision = mesh->precision;
   l.fuse = mesh->fuse;
+  l.dtp = mesh->dt_dev;
   return l;
 }
 
@@ -408,14 +472,14 @@ struct Timer
   long long units;
   Timer(hx_ctx* c, std::string cat, long long u) : ctx{c}, category{std::move(cat)}, units{u}
   {
-    if (!ctx->timing) return;
+    if (!ctx->timing || ctx->events_off) return;
     a = pool_event(ctx);
     b = pool_event(ctx);
     if (a && b && hipEventRecord(a, ctx->stream) != hipSuccess) a = nullptr;
   }
   ~Timer()
   {
-    if (!ctx->timing || !a || !b) return;
+    if (!ctx->timing || ctx->events_off || !a || !b) return;
     if (hipEventRecord(b, ctx->stream) == hipSuccess) ctx->pending.push_back({a, b, category, units});
   }
 };
@@ -437,6 +501,28 @@ void resolve_timing(hx_ctx* ctx)
 }
 
 __global__ void k_fill(double* p, double v) { p[threadIdx.x] = v; }
+
+// one step's time step on the device, min(Max_dt, max_time_step) * chebyshev_step (Solver.cpp:843-848), with
+// Max_dt from *mdt (or 1 with local time stepping), and the status it advances (Solver.cpp:872-876)
+__global__ void k_step_dt(const double* mdt, int unit, double max_time_step, double cheby, Step_dev* s, int count)
+{
+  const double m = unit ? 1. : *mdt;
+  double dt = (max_time_step < m ? max_time_step : m) * cheby;
+  // the product is rounded before it is accumulated, as the host's flow_time += dt does (no fused multiply-add)
+  asm volatile("" : "+v"(dt));
+  s->dt = dt;
+  s->time_step = dt;
+  s->flow_time += dt;
+  if (count) ++s->iteration;
+}
+
+__global__ void k_set_status(Step_dev* s, double time_step, double flow_time, long long iteration)
+{
+  s->dt = time_step;
+  s->time_step = time_step;
+  s->flow_time = flow_time;
+  s->iteration = iteration;
+}
 
 // batch of cases: per-case time step min(Max_dt, max_time_step) * chebyshev_step (Solver.cpp:843-848), and the time
 // step scale of every point set to its case's time step
@@ -504,10 +590,25 @@ __global__ __launch_bounds__(256) void k_min_reduce(const double* __restrict__ v
   }
 }
 
-// plain streaming copy, 16 bytes per thread per iteration
+// plain streaming copy: each workgroup moves chunks of 4 x 256 16-byte items, every lane with its four loads in flight
+// before its four stores
+constexpr int copy_unroll = 4;
 __global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n)
 {
-  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) dst[i] = src[i];
+  const size_t step = size_t(gridDim.x) * 256 * copy_unroll;
+  for (size_t base = size_t(blockIdx.x) * 256 * copy_unroll + threadIdx.x; base < n; base += step) {
+    double2 v[copy_unroll];
+#pragma unroll
+    for (int k = 0; k < copy_unroll; ++k) {
+      const size_t i = base + size_t(k) * 256;
+      if (i < n) v[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < copy_unroll; ++k) {
+      const size_t i = base + size_t(k) * 256;
+      if (i < n) dst[i] = v[k];
+    }
+  }
 }
 
 }  // namespace
@@ -531,6 +632,24 @@ __global__ __launch_bounds__(256) void k_vertex_push_min(const int* vid, const d
 {
   const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
   if (gid < n) atomic_min_double(shared + vid[gid], vtss[gid]);
+}
+
+// vertex sharing by maximum (Vertex::vector_max): std::max(shared, v) keeps the shared value unless it is below v
+__device__ inline void atomic_max_double(double* addr, double v)
+{
+  unsigned long long* a = reinterpret_cast<unsigned long long*>(addr);
+  unsigned long long old = *a;
+  while (__longlong_as_double(old) < v) {
+    const unsigned long long prev = atomicCAS(a, old, (unsigned long long)__double_as_longlong(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_vertex_push_max(const int* vid, const double* vals, long long n, double* shared)
+{
+  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (gid < n) atomic_max_double(shared + vid[gid], vals[gid]);
 }
 
 __global__ __launch_bounds__(256) void k_vertex_fetch(const int* vid, double* vtss, long long n, const double* shared)
@@ -606,6 +725,8 @@ hx_status hx_ctx_destroy(hx_ctx* ctx)
   (void)hipStreamSynchronize(ctx->stream);
   // meshes still alive are detached: their device memory is freed now, their handles stay valid for
   // hx_mesh_destroy, and every other call on them fails with HX_ERR_INVALID
+  for (hx_group* g : ctx->groups) release_group(g);
+  ctx->groups.clear();
   for (hx_mesh* mesh : ctx->meshes) {
     release_mesh_memory(mesh);
     mesh->ctx = nullptr;
@@ -658,19 +779,25 @@ hx_status hx_stream_copy(hx_ctx* ctx, size_t bytes, int reps, double* gbs)
     cleanup();
     return st;
   }
-  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, 256 * 64));
-  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, (const double2*)a, (double2*)b, n);
-  (void)hipEventRecord(e0, ctx->stream);
-  for (int r = 0; r < reps; ++r)
+  // the best of several grid sizes: 8, 32 and 128 workgroups per compute unit, and one workgroup per chunk
+  const size_t chunks = (n + 256 * copy_unroll - 1) / (256 * copy_unroll);
+  double best = 0;
+  for (size_t cap : {size_t(256) * 8, size_t(256) * 32, size_t(256) * 128, chunks}) {
+    const unsigned grid = unsigned(std::max<size_t>(1, std::min(chunks, cap)));
     hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, (const double2*)a, (double2*)b, n);
-  (void)hipEventRecord(e1, ctx->stream);
-  if ((st = hip_check(ctx, hipEventSynchronize(e1), "copy sync"))) {
-    cleanup();
-    return st;
+    (void)hipEventRecord(e0, ctx->stream);
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, (const double2*)a, (double2*)b, n);
+    (void)hipEventRecord(e1, ctx->stream);
+    if ((st = hip_check(ctx, hipEventSynchronize(e1), "copy sync"))) {
+      cleanup();
+      return st;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    best = std::max(best, 2.0 * double(n * 16) * reps / (ms * 1e-3) / 1e9);
   }
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  *gbs = 2.0 * double(n * 16) * reps / (ms * 1e-3) / 1e9;
+  *gbs = best;
   cleanup();
   return HX_OK;
 }
@@ -823,6 +950,7 @@ hx_status hx_mesh_validate(const hx_host_mesh* d, char* msg, size_t capacity)
 hx_status hx_mesh_destroy(hx_mesh* mesh)
 {
   if (!mesh) return HX_OK;
+  if (mesh->n_groups) return fail(mesh->ctx, HX_ERR_INVALID, "the mesh belongs to a group: destroy the group first");
   if (hx_ctx* ctx = mesh->ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
@@ -1135,34 +1263,67 @@ hx_status hx_compute_write_face(hx_mesh* mesh)
 hx_status hx_compute_prolong(hx_mesh* mesh, int scale, int offset) { return hx_prolong_refined(mesh, nullptr, scale, offset); }
 hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset) { return hx_restrict_refined(mesh, nullptr, scale, offset); }
 
-// Solver::update (Solver.cpp:834-886), with fix_admissibility after each stage when enabled (Solver.cpp:867)
-hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
+}  // extern "C"
+
+namespace {
+
+// member-wise equality (structure padding is not compared)
+bool same_transport(const hx_transport& a, const hx_transport& b)
 {
-  if (!live(mesh) || !o || !st) return HX_ERR_INVALID;
+  return a.const_val == b.const_val && a.ref_val == b.ref_val && a.ref_temp == b.ref_temp &&
+         a.temp_offset == b.temp_offset && a.is_viscous == b.is_viscous;
+}
+bool same_options(const hx_update_options& a, const hx_update_options& b)
+{
+  return a.max_safety == b.max_safety && a.max_time_step == b.max_time_step && a.n_cheby_flow == b.n_cheby_flow &&
+         a.flow_iters == b.flow_iters && a.local_time == b.local_time && a.use_filter == b.use_filter &&
+         a.use_ldg == b.use_ldg && same_transport(a.visc, b.visc) && same_transport(a.cond, b.cond) &&
+         a.fix_admis == b.fix_admis && a.fix_admis_max_safety == b.fix_admis_max_safety;
+}
+
+// math::chebyshev_step (math.cpp:104-107)
+double cheby_step(double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); }
+
+// the time-step settings of Solver::update (Solver.cpp:838-849), shared by its host and device forms
+struct Update_setup
+{
+  double msc, msd;   // safety factors of Max_dt (convective over the largest Chebyshev step, diffusive)
+  int n_cheby;
+  bool fusable;      // with a global time step, stage 1's Local also computes the next step's Max_dt
+  Ns_params ns;
+  double* block_min; // the fused per-workgroup minima
+  long long n_blocks;
+  double* scalar;    // the reduction scalar
+};
+
+// workgroups of the Euler Local over both classes, each storing one fused minimum
+long long euler_local_blocks(const hx_mesh* mesh)
+{
+  const long long epb = std::max(1, 256 / mesh->dm.nq);  // elements per workgroup of the Euler Local
+  return (mesh->dm.n_elem[0] + epb - 1) / epb + (mesh->dm.n_elem[1] + epb - 1) / epb;
+}
+
+hx_status update_setup(hx_mesh* mesh, const hx_update_options* o, Update_setup* u)
+{
   hx_ctx* ctx = mesh->ctx;
-  // with a global time step, stage 1's Local also computes the next step's Max_dt (the state it writes is the state
-  // that Max_dt would read next); local time stepping writes the time step scale in Max_dt, so it is not fused
-  const bool fusable = !o->local_time;
-  if (fusable && !mesh->dt_block_min) {
-    const long long epb = std::max(1, 256 / mesh->dm.nq);  // elements per workgroup of the Euler Local
-    mesh->n_dt_blocks = (mesh->dm.n_elem[0] + epb - 1) / epb + (mesh->dm.n_elem[1] + epb - 1) / epb;
+  // the state that stage 1's Local writes is the state that Max_dt would read next; local time stepping writes the
+  // time step scale in Max_dt, so it is not fused
+  u->fusable = !o->local_time;
+  u->n_blocks = euler_local_blocks(mesh);
+  if (u->fusable && !mesh->dt_block_min) {
+    mesh->n_dt_blocks = u->n_blocks;
     void* p = nullptr;
     HX_TRY(ctx, hipMalloc(&p, std::max<long long>(mesh->n_dt_blocks, 1) * sizeof(double)), "hipMalloc(time step minima)");
     mesh->allocs.push_back(p);
     mesh->dt_block_min = static_cast<double*>(p);
   }
-  const double safety = o->max_safety;
-  const double n_cheby = o->n_cheby_flow;
-  // math::chebyshev_step (math.cpp:104-107)
-  auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
-  const double max_cheby = cheby(n_cheby, n_cheby - 1);
-  const double msc = safety / max_cheby, msd = safety;
-  auto& c = mesh->dt_cache;
-  bool pending = fusable && c.valid && c.gen == mesh->gen && c.msc == msc && c.msd == msd &&
-                 c.use_ldg == o->use_ldg && !std::memcmp(&c.visc, &o->visc, sizeof(hx_transport)) &&
-                 !std::memcmp(&c.cond, &o->cond, sizeof(hx_transport));
-  c.valid = false;
-  Ns_params ns{};
+  u->block_min = mesh->dt_block_min;
+  u->scalar = ctx->scalar;
+  u->n_cheby = o->n_cheby_flow;
+  const double max_cheby = cheby_step(u->n_cheby, u->n_cheby - 1);
+  u->msc = o->max_safety / max_cheby;
+  u->msd = o->max_safety;
+  u->ns = Ns_params{};
   if (o->use_ldg) {
     hx_pde p{};
     p.kind = HX_PDE_NAVIER_STOKES;
@@ -1171,31 +1332,94 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status*
     Launch lp{};
     bool visc;
     if (hx_status s = pde_check(mesh, &p, &visc, &lp)) return s;
-    ns = lp.ns;
+    u->ns = lp.ns;
   }
+  return HX_OK;
+}
+
+// whether the previous update's fused Max_dt minima are still those of the current state and settings
+bool fused_pending(hx_mesh* mesh, const hx_update_options* o, const Update_setup& u)
+{
+  const auto& c = mesh->dt_cache;
+  return u.fusable && c.valid && c.gen == mesh->gen && c.msc == u.msc && c.msd == u.msd && c.use_ldg == o->use_ldg &&
+         same_transport(c.visc, o->visc) && same_transport(c.cond, o->cond);
+}
+
+void remember_fused(hx_mesh* mesh, const hx_update_options* o, const Update_setup& u)
+{
+  auto& c = mesh->dt_cache;
+  c.valid = true;
+  c.gen = mesh->gen;
+  c.msc = u.msc;
+  c.msd = u.msd;
+  c.use_ldg = o->use_ldg;
+  c.visc = o->visc;
+  c.cond = o->cond;
+}
+
+// Max_dt of both classes into the context's reduction scalar, without reading it back (kernels_max_dt.cpp:8-17)
+hx_status enqueue_max_dt(hx_mesh* mesh, const hx_update_options* o, const Update_setup& u)
+{
+  hx_ctx* ctx = mesh->ctx;
+  const double max_cfl_c = hx_basis_max_cfl(mesh->dm.rs) * u.msc;
+  const double max_cfl_d = -2 / hx_basis_tables::eigenvalss[mesh->dm.rs - 2][1] * u.msd;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, ctx->stream, u.scalar, DBL_MAX);
+  HX_TRY(ctx, hipGetLastError(), "fill");
+  for (int cls = 0; cls < 2; ++cls) {
+    Launch l = make_launch(mesh);
+    l.ns = u.ns;
+    Timer tm(ctx, std::string(cls_name(cls)) + "/compute time step/" + pde_name(o->use_ldg != 0), mesh->dm.n_elem[cls]);
+    HX_TRY(ctx, mesh->ks->max_dt(l, cls, o->use_ldg != 0, o->local_time, max_cfl_c, max_cfl_d), "max_dt");
+  }
+  return HX_OK;
+}
+
+// the fused minima of the previous stage 1 reduced into the context's reduction scalar
+hx_status enqueue_fused_reduce(hx_mesh* mesh, const Update_setup& u)
+{
+  hx_ctx* ctx = mesh->ctx;
+  Timer tm(ctx, "fused/compute time step/reduce", u.n_blocks);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, ctx->stream, u.scalar, DBL_MAX);
+  const unsigned g = unsigned(std::min<long long>(256, (u.n_blocks + 255) / 256));
+  if (g) hipLaunchKernelGGL(k_min_reduce, dim3(g), dim3(256), 0, ctx->stream, u.block_min, u.n_blocks, u.scalar);
+  HX_TRY(ctx, hipGetLastError(), "time step reduction");
+  return HX_OK;
+}
+
+// The kernels of one Solver::update (Solver.cpp:834-886), with fix_admissibility after each stage when enabled
+// (Solver.cpp:867). Host form (st not null): each step's Max_dt is read back and the time step passed by value.
+// Device form (st null): the time step stays in mesh->step, formed by k_step_dt, and the stage kernels read it; nothing
+// here waits for the device. `pending`: the previous stage 1's fused Max_dt minima hold the current state's.
+hx_status update_once(hx_mesh* mesh, const hx_update_options* o, const Update_setup& u, hx_update_status* st, bool& pending)
+{
+  hx_ctx* ctx = mesh->ctx;
+  const bool device_dt = st == nullptr;
   for (int i_flow = 0; i_flow < o->flow_iters; ++i_flow) {
-    for (int i_cheby = 0; i_cheby < n_cheby; ++i_cheby) {
-      hx_kernel_options ko{0., 0, 0, o->use_filter};
-      double mdt;
+    for (int i_cheby = 0; i_cheby < u.n_cheby; ++i_cheby) {
       hx_status s;
+      double dt = 1.;
       if (pending) {
-        {
-          Timer tm(ctx, "fused/compute time step/reduce", mesh->n_dt_blocks);
-          hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, ctx->stream, ctx->scalar, DBL_MAX);
-          const unsigned g = unsigned(std::min<long long>(256, (mesh->n_dt_blocks + 255) / 256));
-          if (g) hipLaunchKernelGGL(k_min_reduce, dim3(g), dim3(256), 0, ctx->stream, mesh->dt_block_min, mesh->n_dt_blocks, ctx->scalar);
-          HX_TRY(ctx, hipGetLastError(), "time step reduction");
-        }
-        HX_TRY(ctx, hipMemcpyAsync(ctx->host_scalar, ctx->scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+        if ((s = enqueue_fused_reduce(mesh, u))) return s;
+      } else if (device_dt) {
+        if ((s = enqueue_max_dt(mesh, o, u))) return s;
+      } else {
+        hx_kernel_options ko{0., 0, 0, o->use_filter};
+        double mdt;
+        s = o->use_ldg ? hx_max_dt_navier_stokes(mesh, &ko, u.msc, u.msd, o->local_time, o->visc, o->cond, &mdt)
+                       : hx_max_dt_euler(mesh, &ko, u.msc, u.msd, o->local_time, &mdt);
+        if (s) return s;
+        dt = std::min(mdt, o->max_time_step) * cheby_step(u.n_cheby, i_cheby);
+      }
+      if (device_dt) {
+        hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(1), 0, ctx->stream, u.scalar, pending ? 0 : o->local_time,
+                           o->max_time_step, cheby_step(u.n_cheby, i_cheby), mesh->step, int(i_flow == 0 && i_cheby == 0));
+        HX_TRY(ctx, hipGetLastError(), "time step");
+      } else if (pending) {
+        HX_TRY(ctx, hipMemcpyAsync(ctx->host_scalar, u.scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
                "max_dt readback");
         HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "max_dt sync");
-        mdt = *ctx->host_scalar;
-      } else {
-        s = o->use_ldg ? hx_max_dt_navier_stokes(mesh, &ko, msc, msd, o->local_time, o->visc, o->cond, &mdt)
-                       : hx_max_dt_euler(mesh, &ko, msc, msd, o->local_time, &mdt);
-        if (s) return s;
+        dt = std::min(*ctx->host_scalar, o->max_time_step) * cheby_step(u.n_cheby, i_cheby);
       }
-      const double dt = std::min(mdt, o->max_time_step) * cheby(n_cheby, i_cheby);
       bool fixed = false;
       for (int i = 0; i < 2; ++i) {
         hx_kernel_options opts{dt, i, 0, o->use_filter};
@@ -1203,9 +1427,9 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status*
         if (o->use_ldg && !i) {
           s = hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
         } else {
-          if (i == 1 && fusable)
-            mesh->fuse = Dt_fuse{mesh->dt_block_min, 0, o->use_ldg ? 1 : 0, hx_basis_max_cfl(mesh->dm.rs) * msc,
-                                 -2 / hx_basis_tables::eigenvalss[mesh->dm.rs - 2][1] * msd, ns};
+          if (i == 1 && u.fusable)
+            mesh->fuse = Dt_fuse{u.block_min, 0, o->use_ldg ? 1 : 0, hx_basis_max_cfl(mesh->dm.rs) * u.msc,
+                                 -2 / hx_basis_tables::eigenvalss[mesh->dm.rs - 2][1] * u.msd, u.ns};
           s = hx_compute_euler(mesh, &opts);
           mesh->fuse = Dt_fuse{};
         }
@@ -1222,21 +1446,127 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status*
         pending = false;
         break;
       }
-      pending = fusable;
-      st->time_step = dt;
-      st->flow_time += dt;
+      pending = u.fusable;
+      if (!device_dt) {
+        st->time_step = dt;
+        st->flow_time += dt;
+      }
     }
   }
-  ++st->iteration;
-  if (fusable && pending) {
-    c.valid = true;
-    c.gen = mesh->gen;
-    c.msc = msc;
-    c.msd = msd;
-    c.use_ldg = o->use_ldg;
-    c.visc = o->visc;
-    c.cond = o->cond;
+  if (!device_dt) ++st->iteration;
+  return HX_OK;
+}
+
+// sets mesh->dt_dev for the duration of a device-form update
+struct Device_dt
+{
+  hx_mesh* mesh;
+  Device_dt(hx_mesh* m) : mesh{m} { mesh->dt_dev = &mesh->step->dt; }
+  ~Device_dt() { mesh->dt_dev = nullptr; }
+};
+
+// the steady-state update (the fused minima pending when it starts, or local time stepping) captured into a graph
+hx_status capture_update(hx_mesh* mesh, const hx_update_options* o, const Update_setup& u)
+{
+  hx_ctx* ctx = mesh->ctx;
+  release_graph(mesh);
+  HX_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  ctx->events_off = true;
+  bool pending = u.fusable;
+  hx_status s;
+  {
+    Device_dt d(mesh);
+    s = update_once(mesh, o, u, nullptr, pending);
   }
+  ctx->events_off = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (s) {
+    if (g) (void)hipGraphDestroy(g);
+    return s;
+  }
+  HX_TRY(ctx, e, "hipStreamEndCapture");
+  const hipError_t ei = hipGraphInstantiate(&mesh->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) mesh->graph = nullptr;
+  HX_TRY(ctx, ei, "hipGraphInstantiate");
+  mesh->graph_opts = *o;
+  mesh->graph_precision = mesh->precision;
+  return HX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hx_status hx_update(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
+{
+  if (!live(mesh) || !o || !st) return HX_ERR_INVALID;
+  Update_setup u;
+  if (hx_status s = update_setup(mesh, o, &u)) return s;
+  bool pending = fused_pending(mesh, o, u);
+  mesh->dt_cache.valid = false;
+  if (hx_status s = update_once(mesh, o, u, st, pending)) return s;
+  if (u.fusable && pending) remember_fused(mesh, o, u);
+  return HX_OK;
+}
+
+hx_status hx_update_async(hx_mesh* mesh, const hx_update_options* o, int n_updates, const hx_update_status* from)
+{
+  if (!live(mesh) || !o || n_updates < 0) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  if (o->fix_admis)
+    return fail(ctx, HX_ERR_UNSUPPORTED, "the admissibility fix reads its verdict on the host; use hx_update");
+  if (mesh->n_cases) return fail(ctx, HX_ERR_INVALID, "the mesh holds a batch of cases; use hx_update_cases");
+  Update_setup u;
+  if (hx_status s = update_setup(mesh, o, &u)) return s;
+  if (!mesh->step) {
+    void* p = nullptr;
+    HX_TRY(ctx, hipMalloc(&p, sizeof(Step_dev)), "hipMalloc(update status)");
+    mesh->allocs.push_back(p);
+    mesh->step = static_cast<Step_dev*>(p);
+    hipLaunchKernelGGL(k_set_status, dim3(1), dim3(1), 0, ctx->stream, mesh->step, 0., 0., 0LL);
+    HX_TRY(ctx, hipGetLastError(), "status");
+  }
+  if (from) {
+    hipLaunchKernelGGL(k_set_status, dim3(1), dim3(1), 0, ctx->stream, mesh->step, from->time_step, from->flow_time,
+                       (long long)from->iteration);
+    HX_TRY(ctx, hipGetLastError(), "status");
+  }
+  bool pending = fused_pending(mesh, o, u);
+  mesh->dt_cache.valid = false;
+  for (int k = 0; k < n_updates; ++k) {
+    if (u.fusable && !pending) {
+      // the first update of a sequence computes Max_dt itself; every later one starts from the fused minima
+      Device_dt d(mesh);
+      if (hx_status s = update_once(mesh, o, u, nullptr, pending)) return s;
+      continue;
+    }
+    if (!mesh->graph || mesh->graph_precision != mesh->precision || !same_options(mesh->graph_opts, *o))
+      if (hx_status s = capture_update(mesh, o, u)) return s;
+    Timer tm(ctx, "graph/update", mesh->dm.n_elem[0] + mesh->dm.n_elem[1]);
+    HX_TRY(ctx, hipGraphLaunch(mesh->graph, ctx->stream), "hipGraphLaunch");
+    ++mesh->gen;
+    pending = u.fusable;
+  }
+  if (u.fusable && pending) remember_fused(mesh, o, u);
+  return HX_OK;
+}
+
+hx_status hx_update_status_sync(hx_mesh* mesh, hx_update_status* st)
+{
+  if (!live(mesh) || !st) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  if (!mesh->step) return fail(ctx, HX_ERR_INVALID, "no asynchronous update has run on this mesh");
+  static_assert(sizeof(Step_dev) <= 8 * sizeof(double), "the status fits the pinned readback buffer");
+  HX_TRY(ctx, hipMemcpyAsync(ctx->host_scalar, mesh->step, sizeof(Step_dev), hipMemcpyDeviceToHost, ctx->stream),
+         "status readback");
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "status sync");
+  Step_dev d;
+  std::memcpy(&d, ctx->host_scalar, sizeof(Step_dev));
+  st->time_step = d.time_step;
+  st->flow_time = d.flow_time;
+  st->iteration = int(d.iteration);
   return HX_OK;
 }
 
@@ -1341,6 +1671,150 @@ hx_status hx_mesh_set_cases(hx_mesh* mesh, int n_cases, const int* case_cartesia
 // the Max_dt minimum is reduced per case, each point's time step scale is set to its case's time step, and the stage
 // kernels run with dt = 1, so that their coefficient times the time step scale is the same product as a single
 // case's dt times a scale of 1.
+}  // extern "C"
+
+namespace {
+
+// One time step of a batch of cases (hx_update_cases), split so that a group of batch meshes (hx_group) can run the
+// per-mesh parts around its fused stage launches: Cases_step holds what one call needs; cases_setup prepares the mesh,
+// cases_dt forms every case's time step on the device, cases_stage runs one stage on the mesh alone, cases_status
+// advances the host status from the time steps read back, cases_end records the fused Max_dt for the next call.
+struct Cases_step
+{
+  Launch l;
+  bool visc;
+  const char* name;
+  double cc, cd;       // the constants of Max_dt with safety_conv = max_safety / max Chebyshev step, safety_diff = max_safety
+  double n_cheby;
+  bool fusable;        // every class lists its cases in order: stage 1's Local computes the next step's 1/scale per point
+  bool pending;        // the previous call's per-point 1/scale is still that of the current state
+  long long gen_at_entry;
+};
+
+hx_status cases_setup(hx_mesh* mesh, const hx_update_options* o, Cases_step& c)
+{
+  hx_ctx* ctx = mesh->ctx;
+  c.gen_at_entry = mesh->gen;  // (make_launch below advances it)
+  hx_pde p{};
+  p.kind = o->use_ldg ? HX_PDE_NAVIER_STOKES : HX_PDE_EULER;
+  p.visc = o->visc;
+  p.cond = o->cond;
+  c.l = make_launch(mesh);
+  if (hx_status s = pde_check(mesh, &p, &c.visc, &c.l)) return s;
+  const int rs = mesh->dm.rs;
+  c.name = pde_name(o->use_ldg != 0);
+  c.n_cheby = o->n_cheby_flow;
+  const double max_cheby = cheby_step(c.n_cheby, c.n_cheby - 1);
+  c.cc = hx_basis_max_cfl(rs) * (o->max_safety / max_cheby);
+  c.cd = -2 / hx_basis_tables::eigenvalss[rs - 2][1] * o->max_safety;
+  // with every class's cases in order, stage 1's Local also computes the next step's per-point 1/scale (the state it
+  // writes is the state the next Max_dt reads); the next call uses it if nothing changed the mesh in between and the
+  // time-step settings are the same
+  c.fusable = true;
+  for (int k = 0; k < 2; ++k) c.fusable = c.fusable && (!mesh->dm.n_elem[k] || mesh->case_start[k]);
+  for (int k = 0; k < 2 && c.fusable; ++k)
+    if (mesh->dm.n_elem[k] && !mesh->case_scale[k]) {
+      void* q = nullptr;
+      HX_TRY(ctx, hipMalloc(&q, size_t(mesh->dm.n_elem[k]) * mesh->dm.nq * sizeof(double)), "hipMalloc(case scale)");
+      mesh->allocs.push_back(q);
+      mesh->case_scale[k] = static_cast<double*>(q);
+    }
+  auto& cache = mesh->case_cache;
+  c.pending = c.fusable && cache.valid && cache.gen == c.gen_at_entry && cache.cc == c.cc && cache.cd == c.cd &&
+              cache.use_ldg == o->use_ldg && same_transport(cache.visc, o->visc) && same_transport(cache.cond, o->cond);
+  cache.valid = false;
+  return HX_OK;
+}
+
+// every case's time step min(Max_dt, max_time_step) * chebyshev_step into mesh->case_dt, and into the time step scale
+hx_status cases_dt(hx_mesh* mesh, const hx_update_options* o, const Cases_step& c, int i_cheby)
+{
+  hx_ctx* ctx = mesh->ctx;
+  const int nc = mesh->n_cases;
+  hipLaunchKernelGGL(k_fill_u64, dim3((nc + 255) / 256), dim3(256), 0, c.l.stream, mesh->case_min,
+                     (unsigned long long)__builtin_bit_cast(long long, DBL_MAX), nc);
+  HX_TRY(ctx, hipGetLastError(), "fill");
+  for (int k = 0; k < 2; ++k) {
+    if (!mesh->dm.n_elem[k]) continue;
+    Timer tm(ctx, std::string(cls_name(k)) + "/compute time step/" + c.name, mesh->dm.n_elem[k]);
+    if (mesh->case_start[k]) {
+      // each point's 1/scale into the time step scale (Max_dt in local mode), or the fused values of the previous
+      // stage 1, then a streaming minimum per case
+      if (!c.pending) HX_TRY(ctx, mesh->ks->max_dt(c.l, k, c.visc, 1, c.cc, c.cd), "max_dt");
+      hipLaunchKernelGGL(k_case_min_tss, dim3(nc * mesh->case_split[k]), dim3(256), 0, c.l.stream,
+                         c.pending ? mesh->case_scale[k] : mesh->dm.tss[k], mesh->case_start[k], mesh->dm.nq,
+                         mesh->case_split[k], mesh->case_min);
+      HX_TRY(ctx, hipGetLastError(), "case minimum");
+    } else {
+      HX_TRY(ctx, mesh->ks->max_dt_cases(c.l, k, c.visc, c.cc, c.cd, mesh->case_id[k], mesh->case_min), "max_dt_cases");
+    }
+  }
+  hipLaunchKernelGGL(k_case_dt, dim3((nc + 255) / 256), dim3(256), 0, c.l.stream, mesh->case_min, mesh->case_dt, nc,
+                     o->max_time_step, cheby_step(c.n_cheby, i_cheby));
+  HX_TRY(ctx, hipGetLastError(), "case dt");
+  for (int k = 0; k < 2; ++k) {
+    const size_t n = size_t(mesh->dm.n_elem[k]) * mesh->dm.nq;
+    if (!n) continue;
+    hipLaunchKernelGGL(k_case_tss, dim3(unsigned(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, c.l.stream,
+                       mesh->dm.tss[k], mesh->case_id[k], mesh->case_dt, mesh->dm.nq, n);
+    HX_TRY(ctx, hipGetLastError(), "case tss");
+  }
+  return HX_OK;
+}
+
+// the stage-1 fusion of the Euler Local in a batch step: each point's next-step 1/scale into case_scale
+Dt_fuse cases_fuse(const hx_mesh* mesh, const hx_update_options* o, const Cases_step& c)
+{
+  Dt_fuse f{};
+  f.visc = o->use_ldg ? 1 : 0;
+  f.max_cfl_c = c.cc;
+  f.max_cfl_d = c.cd;
+  f.ns = c.l.ns;
+  f.point_scale[0] = mesh->case_scale[0];
+  f.point_scale[1] = mesh->case_scale[1];
+  return f;
+}
+
+// stage i of the mesh alone, each case's time step already in the time step scale (dt = 1)
+hx_status cases_stage(hx_mesh* mesh, const hx_update_options* o, const Cases_step& c, int i)
+{
+  hx_kernel_options opts{1., i, 0, o->use_filter};
+  hx_status s;
+  if ((s = hx_apply_state_bcs(mesh))) return s;
+  if (o->use_ldg && !i) return hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
+  if (i == 1 && c.fusable) mesh->fuse = cases_fuse(mesh, o, c);
+  s = hx_compute_euler(mesh, &opts);
+  mesh->fuse = Dt_fuse{};
+  return s;
+}
+
+void cases_status(const hx_mesh* mesh, const double* dts, hx_update_status* st)
+{
+  for (int k = 0; k < mesh->n_cases; ++k) {
+    st[k].time_step = dts[k];
+    st[k].flow_time += dts[k];
+  }
+}
+
+void cases_end(hx_mesh* mesh, const hx_update_options* o, const Cases_step& c, bool pending, hx_update_status* st)
+{
+  for (int k = 0; k < mesh->n_cases; ++k) ++st[k].iteration;
+  if (c.fusable && pending) {
+    auto& cache = mesh->case_cache;
+    cache.valid = true;
+    cache.gen = mesh->gen;
+    cache.cc = c.cc;
+    cache.cd = c.cd;
+    cache.use_ldg = o->use_ldg;
+    cache.visc = o->visc;
+    cache.cond = o->cond;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
 {
   if (!live(mesh) || !o || !st) return HX_ERR_INVALID;
@@ -1360,112 +1834,306 @@ hx_status hx_update_cases(hx_mesh* mesh, const hx_update_options* o, hx_update_s
     }
     return HX_OK;
   }
-  const long long gen_at_entry = mesh->gen;  // (make_launch below advances it)
-  hx_pde p{};
-  p.kind = o->use_ldg ? HX_PDE_NAVIER_STOKES : HX_PDE_EULER;
-  p.visc = o->visc;
-  p.cond = o->cond;
-  Launch l = make_launch(mesh);
-  bool visc;
-  if (hx_status s = pde_check(mesh, &p, &visc, &l)) return s;
-  const int rs = mesh->dm.rs;
-  const char* name = pde_name(o->use_ldg != 0);
+  Cases_step c;
+  if (hx_status s = cases_setup(mesh, o, c)) return s;
   std::vector<double> dts(nc);
-  const double safety = o->max_safety;
-  const double n_cheby = o->n_cheby_flow;
-  auto cheby = [](double n, double i) { return 1 / (1 - std::cos((n - i - 0.5) * M_PI / n)); };
-  const double max_cheby = cheby(n_cheby, n_cheby - 1);
-  // the constants of hx_max_dt with safety_conv = safety / max_cheby, safety_diff = safety
-  const double cc = hx_basis_max_cfl(rs) * (safety / max_cheby);
-  const double cd = -2 / hx_basis_tables::eigenvalss[rs - 2][1] * safety;
-  // with every class's cases in order, stage 1's Local also computes the next step's per-point 1/scale (the state it
-  // writes is the state the next Max_dt reads); the next call uses it if nothing changed the mesh in between and the
-  // time-step settings are the same
-  bool fusable = true;
-  for (int c = 0; c < 2; ++c) fusable = fusable && (!mesh->dm.n_elem[c] || mesh->case_start[c]);
-  for (int c = 0; c < 2 && fusable; ++c)
-    if (mesh->dm.n_elem[c] && !mesh->case_scale[c]) {
-      void* q = nullptr;
-      HX_TRY(ctx, hipMalloc(&q, size_t(mesh->dm.n_elem[c]) * mesh->dm.nq * sizeof(double)), "hipMalloc(case scale)");
-      mesh->allocs.push_back(q);
-      mesh->case_scale[c] = static_cast<double*>(q);
-    }
-  auto& cache = mesh->case_cache;
-  bool pending = fusable && cache.valid && cache.gen == gen_at_entry && cache.cc == cc && cache.cd == cd &&
-                 cache.use_ldg == o->use_ldg && !std::memcmp(&cache.visc, &o->visc, sizeof(hx_transport)) &&
-                 !std::memcmp(&cache.cond, &o->cond, sizeof(hx_transport));
-  cache.valid = false;
+  bool pending = c.pending;
   for (int i_flow = 0; i_flow < o->flow_iters; ++i_flow) {
-    for (int i_cheby = 0; i_cheby < n_cheby; ++i_cheby) {
-      hipLaunchKernelGGL(k_fill_u64, dim3((nc + 255) / 256), dim3(256), 0, l.stream, mesh->case_min,
-                         (unsigned long long)__builtin_bit_cast(long long, DBL_MAX), nc);
-      HX_TRY(ctx, hipGetLastError(), "fill");
-      for (int c = 0; c < 2; ++c) {
-        if (!mesh->dm.n_elem[c]) continue;
-        Timer tm(ctx, std::string(cls_name(c)) + "/compute time step/" + name, mesh->dm.n_elem[c]);
-        if (mesh->case_start[c]) {
-          // each point's 1/scale into the time step scale (Max_dt in local mode), or the fused values of the previous
-          // stage 1, then a streaming minimum per case
-          if (!pending) HX_TRY(ctx, mesh->ks->max_dt(l, c, visc, 1, cc, cd), "max_dt");
-          hipLaunchKernelGGL(k_case_min_tss, dim3(nc * mesh->case_split[c]), dim3(256), 0, l.stream,
-                             pending ? mesh->case_scale[c] : mesh->dm.tss[c], mesh->case_start[c], mesh->dm.nq,
-                             mesh->case_split[c], mesh->case_min);
-          HX_TRY(ctx, hipGetLastError(), "case minimum");
-        } else {
-          HX_TRY(ctx, mesh->ks->max_dt_cases(l, c, visc, cc, cd, mesh->case_id[c], mesh->case_min), "max_dt_cases");
-        }
-      }
-      hipLaunchKernelGGL(k_case_dt, dim3((nc + 255) / 256), dim3(256), 0, l.stream, mesh->case_min, mesh->case_dt, nc,
-                         o->max_time_step, cheby(n_cheby, i_cheby));
-      HX_TRY(ctx, hipGetLastError(), "case dt");
-      for (int c = 0; c < 2; ++c) {
-        const size_t n = size_t(mesh->dm.n_elem[c]) * mesh->dm.nq;
-        if (!n) continue;
-        hipLaunchKernelGGL(k_case_tss, dim3(unsigned(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, l.stream,
-                           mesh->dm.tss[c], mesh->case_id[c], mesh->case_dt, mesh->dm.nq, n);
-        HX_TRY(ctx, hipGetLastError(), "case tss");
-      }
-      for (int i = 0; i < 2; ++i) {
-        hx_kernel_options opts{1., i, 0, o->use_filter};
-        hx_status s;
-        if ((s = hx_apply_state_bcs(mesh))) return s;
-        if (o->use_ldg && !i) {
-          s = hx_compute_navier_stokes(mesh, &opts, o->visc, o->cond);
-        } else {
-          if (i == 1 && fusable) {
-            mesh->fuse = Dt_fuse{};
-            mesh->fuse.visc = o->use_ldg ? 1 : 0;
-            mesh->fuse.max_cfl_c = cc;
-            mesh->fuse.max_cfl_d = cd;
-            mesh->fuse.ns = l.ns;
-            mesh->fuse.point_scale[0] = mesh->case_scale[0];
-            mesh->fuse.point_scale[1] = mesh->case_scale[1];
-          }
-          s = hx_compute_euler(mesh, &opts);
-          mesh->fuse = Dt_fuse{};
-        }
-        if (s) return s;
-      }
-      pending = fusable;
-      HX_TRY(ctx, hipMemcpyAsync(dts.data(), mesh->case_dt, nc * sizeof(double), hipMemcpyDeviceToHost, l.stream),
+    for (int i_cheby = 0; i_cheby < c.n_cheby; ++i_cheby) {
+      c.pending = pending;
+      if (hx_status s = cases_dt(mesh, o, c, i_cheby)) return s;
+      for (int i = 0; i < 2; ++i)
+        if (hx_status s = cases_stage(mesh, o, c, i)) return s;
+      pending = c.fusable;
+      HX_TRY(ctx, hipMemcpyAsync(dts.data(), mesh->case_dt, nc * sizeof(double), hipMemcpyDeviceToHost, c.l.stream),
              "case dt readback");
-      HX_TRY(ctx, hipStreamSynchronize(l.stream), "case dt sync");
-      for (int k = 0; k < nc; ++k) {
-        st[k].time_step = dts[k];
-        st[k].flow_time += dts[k];
-      }
+      HX_TRY(ctx, hipStreamSynchronize(c.l.stream), "case dt sync");
+      cases_status(mesh, dts.data(), st);
     }
   }
-  for (int k = 0; k < nc; ++k) ++st[k].iteration;
-  if (fusable && pending) {
-    cache.valid = true;
-    cache.gen = mesh->gen;
-    cache.cc = cc;
-    cache.cd = cd;
-    cache.use_ldg = o->use_ldg;
-    cache.visc = o->visc;
-    cache.cond = o->cond;
+  cases_end(mesh, o, c, pending, st);
+  return HX_OK;
+}
+
+// ---- groups of meshes: fused launches over meshes, classes and row sizes ----------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+// the plan of one fused kernel: a segment per (mesh, class) with work, tiles from the fused set
+void build_plan(hx_group* g, int kind, hx::Fused_plan& p)
+{
+  p = hx::Fused_plan{};
+  for (int k = 0; k < int(g->meshes.size()); ++k)
+    for (int c = 0; c < (kind == hx::fused_state_bcs ? 1 : 2); ++c) {
+      const Dev_mesh& m = g->meshes[k]->dm;
+      const int t = g->fs->tiles(kind, m, c);
+      if (!t) continue;
+      p.first[p.n_seg] = p.n_tiles;
+      p.member[p.n_seg] = (signed char)k;
+      p.cls[p.n_seg] = (signed char)c;
+      p.rs[p.n_seg] = (signed char)m.rs;
+      ++p.n_seg;
+      p.n_tiles += t;
+    }
+  p.first[p.n_seg] = p.n_tiles;
+}
+
+// a fused launch of plan `kind` with the group's launch shape (the Local's shape; the face kernels one tile per
+// workgroup)
+hx::Fused_launch group_launch(hx_group* g, int kind, int stage)
+{
+  hx::Fused_launch l{};
+  l.stream = g->ctx->stream;
+  l.mem = g->mem + (stage ? g->meshes.size() : 0);
+  l.plan = g->plans + kind;
+  l.n_tiles = g->hplan[kind].n_tiles;
+  l.order = kind == hx::fused_local ? g->order : 2;
+  const long long cap = (long long)g->per_cu * g->n_cu;
+  l.grid = unsigned(l.order == 2 || cap <= 0 ? l.n_tiles : std::min<long long>(cap, l.n_tiles));
+  l.chunk = l.grid ? (l.n_tiles + int(l.grid) - 1) / int(l.grid) : 0;
+  if (l.order == 1 && l.chunk) l.grid = unsigned((l.n_tiles + l.chunk - 1) / l.chunk);
+  l.host_ops = g->hops;
+  return l;
+}
+
+// the member tables: every mesh's device description and operator tables, and for stage 1 its Local's fused time step
+hx_status upload_members(hx_group* g, const std::vector<Dt_fuse>& fuse)
+{
+  const int n = int(g->meshes.size());
+  std::vector<hx::Fused_member> h(2 * n);
+  for (int k = 0; k < n; ++k) {
+    hx_mesh* m = g->meshes[k];
+    h[k] = hx::Fused_member{m->dm, g->ctx->eops[m->dm.rs], g->ctx->ops[m->dm.rs], Dt_fuse{}};
+    h[n + k] = h[k];
+    h[n + k].fuse = fuse[k];
   }
+  if (g->hmem.size() == h.size() && !std::memcmp(g->hmem.data(), h.data(), h.size() * sizeof(h[0]))) return HX_OK;
+  HX_TRY(g->ctx, hipMemcpyAsync(g->mem, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice, g->ctx->stream),
+         "upload group members");
+  HX_TRY(g->ctx, hipStreamSynchronize(g->ctx->stream), "upload group members");
+  g->hmem = h;
+  return HX_OK;
+}
+
+// one fused Euler stage of every mesh: state boundary conditions, Neighbor, (hanging faces per mesh) Local, prolong
+hx_status group_stage(hx_group* g, const hx_update_options* o, int i)
+{
+  hx_ctx* ctx = g->ctx;
+  const long long n_elem = [&] {
+    long long s = 0;
+    for (hx_mesh* m : g->meshes) s += (long long)m->dm.n_elem[0] + m->dm.n_elem[1];
+    return s;
+  }();
+  hx_pde p{};
+  p.kind = HX_PDE_EULER;
+  for (hx_mesh* m : g->meshes) ++m->gen;
+  {
+    Timer tm(ctx, "fused/boundary conditions/state", g->hplan[hx::fused_state_bcs].n_tiles);
+    HX_TRY(ctx, g->fs->state_bcs(group_launch(g, hx::fused_state_bcs, i)), "fused state BCs");
+  }
+  {
+    Timer tm(ctx, "fused/neighbor/euler", g->hplan[hx::fused_neighbor].n_tiles);
+    HX_TRY(ctx, g->fs->neighbor(group_launch(g, hx::fused_neighbor, i)), "fused neighbor");
+  }
+  for (hx_mesh* m : g->meshes)
+    if (m->dm.n_ref)
+      if (hx_status s = hx_restrict_refined(m, &p, 1, 0)) return s;
+  {
+    Timer tm(ctx, "fused/local/euler", n_elem);
+    const double coef = i ? hx_basis_step_ratio(0) : 1.;
+    HX_TRY(ctx, g->fs->local_euler(group_launch(g, hx::fused_local, i), g->meshes[0]->precision, g->with_def, coef,
+                                   nullptr, i, 0, o->use_filter),
+           "fused local");
+  }
+  for (hx_mesh* m : g->meshes)
+    if (m->dm.n_ref)
+      if (hx_status s = hx_prolong_refined(m, &p, 0, 0)) return s;
+  return HX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hx_status hx_group_create(hx_ctx* ctx, hx_mesh* const* meshes, int n_meshes, hx_group** out)
+{
+  if (!ctx || !meshes || !out || n_meshes < 1) return HX_ERR_INVALID;
+  *out = nullptr;
+  if (n_meshes > hx::fused_max_seg / 2) return fail(ctx, HX_ERR_INVALID, "a group holds at most 8 meshes");
+  for (int k = 0; k < n_meshes; ++k) {
+    if (!live(meshes[k]) || meshes[k]->ctx != ctx) return fail(ctx, HX_ERR_INVALID, "group meshes must be live meshes of the context");
+    if (meshes[k]->n_cases < 1) return fail(ctx, HX_ERR_INVALID, "every mesh of a group needs its cases registered (hx_mesh_set_cases)");
+    for (int j = 0; j < k; ++j)
+      if (meshes[j] == meshes[k]) return fail(ctx, HX_ERR_INVALID, "a mesh appears twice in the group");
+    if (meshes[k]->dm.nd != meshes[0]->dm.nd) return fail(ctx, HX_ERR_INVALID, "group meshes must share n_dim");
+  }
+  auto* g = new hx_group;
+  g->ctx = ctx;
+  g->meshes.assign(meshes, meshes + n_meshes);
+  bool fusable = meshes[0]->dm.nd == 3;
+  for (hx_mesh* m : g->meshes) {
+    g->n_cases += m->n_cases;
+    g->with_def = g->with_def || m->dm.n_elem[HX_DEFORMED] > 0;
+    fusable = fusable && hx::fused_set_nd3()->supports(m->dm.rs) && !ensure_ops(ctx, m->dm.rs);
+    g->hops[m->dm.rs] = ctx->host_ops[m->dm.rs].data();
+  }
+  auto bail = [&](hx_status s) {
+    release_group(g);
+    delete g;
+    return s;
+  };
+  if (hipHostMalloc(reinterpret_cast<void**>(&g->dts), std::max(1, g->n_cases) * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return bail(fail(ctx, HX_ERR_HIP, "hipHostMalloc(group time steps)"));
+  if (fusable) {
+    g->fs = hx::fused_set_nd3();
+    if (hipMalloc(reinterpret_cast<void**>(&g->mem), 2 * n_meshes * sizeof(hx::Fused_member)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&g->plans), 3 * sizeof(hx::Fused_plan)) != hipSuccess)
+      return bail(fail(ctx, HX_ERR_HIP, "hipMalloc(group tables)"));
+    for (int kind = 0; kind < 3; ++kind) build_plan(g, kind, g->hplan[kind]);
+    if (hipMemcpy(g->plans, g->hplan, sizeof(g->hplan), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(ctx, HX_ERR_HIP, "upload group plans"));
+    if (hx_status s = upload_members(g, std::vector<Dt_fuse>(n_meshes))) return bail(s);
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+      return bail(fail(ctx, HX_ERR_HIP, "hipGetDeviceProperties"));
+    g->n_cu = prop.multiProcessorCount;
+  }
+  for (hx_mesh* m : g->meshes) ++m->n_groups;
+  ctx->groups.push_back(g);
+  *out = g;
+  return HX_OK;
+}
+
+hx_status hx_group_destroy(hx_group* g)
+{
+  if (!g) return HX_OK;
+  if (hx_ctx* ctx = g->ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->groups.erase(std::remove(ctx->groups.begin(), ctx->groups.end(), g), ctx->groups.end());
+    release_group(g);
+  }
+  delete g;
+  return HX_OK;
+}
+
+int hx_group_fused(const hx_group* g) { return g && g->ctx && g->fs ? 1 : 0; }
+
+hx_status hx_group_set_shape(hx_group* g, int order, int workgroups_per_cu)
+{
+  if (!g || !g->ctx) return HX_ERR_INVALID;
+  if (order < 0 || order > 2 || (order != 2 && workgroups_per_cu < 1))
+    return fail(g->ctx, HX_ERR_INVALID, "launch shape: order 0 or 1 with workgroups_per_cu >= 1, or order 2");
+  g->order = order;
+  g->per_cu = order == 2 ? 0 : workgroups_per_cu;
+  return HX_OK;
+}
+
+hx_status hx_group_update(hx_group* g, const hx_update_options* o, hx_update_status* st)
+{
+  if (!g || !g->ctx || !o || !st) return HX_ERR_INVALID;
+  hx_ctx* ctx = g->ctx;
+  const int n = int(g->meshes.size());
+  for (hx_mesh* m : g->meshes)
+    if (m->precision != g->meshes[0]->precision)
+      return fail(ctx, HX_ERR_INVALID, "group meshes must share the arithmetic form");
+  // the fused launches cover Euler stages with a global time step; anything else steps the meshes one at a time
+  if (!g->fs || o->use_ldg || o->local_time || o->fix_admis) {
+    for (int k = 0, off = 0; k < n; off += g->meshes[k]->n_cases, ++k)
+      if (hx_status s = hx_update_cases(g->meshes[k], o, st + off)) return s;
+    return HX_OK;
+  }
+  std::vector<Cases_step> c(n);
+  std::vector<Dt_fuse> fuse(n);
+  std::vector<char> pending(n);
+  for (int k = 0; k < n; ++k) {
+    if (hx_status s = cases_setup(g->meshes[k], o, c[k])) return s;
+    fuse[k] = c[k].fusable ? cases_fuse(g->meshes[k], o, c[k]) : Dt_fuse{};
+    pending[k] = c[k].pending;
+  }
+  if (hx_status s = upload_members(g, fuse)) return s;
+  for (int i_flow = 0; i_flow < o->flow_iters; ++i_flow) {
+    for (int i_cheby = 0; i_cheby < c[0].n_cheby; ++i_cheby) {
+      for (int k = 0; k < n; ++k) {
+        c[k].pending = pending[k];
+        if (hx_status s = cases_dt(g->meshes[k], o, c[k], i_cheby)) return s;
+      }
+      for (int i = 0; i < 2; ++i)
+        if (hx_status s = group_stage(g, o, i)) return s;
+      for (int k = 0, off = 0; k < n; off += g->meshes[k]->n_cases, ++k) {
+        pending[k] = c[k].fusable;
+        HX_TRY(ctx, hipMemcpyAsync(g->dts + off, g->meshes[k]->case_dt, g->meshes[k]->n_cases * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream), "case dt readback");
+      }
+      HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "case dt sync");
+      for (int k = 0, off = 0; k < n; off += g->meshes[k]->n_cases, ++k) cases_status(g->meshes[k], g->dts + off, st + off);
+    }
+  }
+  for (int k = 0, off = 0; k < n; off += g->meshes[k]->n_cases, ++k) cases_end(g->meshes[k], o, c[k], pending[k], st + off);
+  return HX_OK;
+}
+
+hx_status hx_group_autotune(hx_group* g, int use_filter, int reps, hx_launch_shape* table, int capacity, int* n_table)
+{
+  if (!g || !g->ctx || reps < 1) return HX_ERR_INVALID;
+  hx_ctx* ctx = g->ctx;
+  if (!g->fs) return fail(ctx, HX_ERR_UNSUPPORTED, "the group has no fused launches to tune");
+  // candidates: one tile per workgroup, and persistent grids of 2 to 16 workgroups per compute unit walking the tiles
+  // strided or in contiguous chunks
+  std::vector<hx_launch_shape> cand{{2, 0, 0.}};
+  for (int order = 0; order < 2; ++order)
+    for (int per_cu : {2, 4, 8, 16}) cand.push_back({order, per_cu, 0.});
+  hipEvent_t a = nullptr, b = nullptr;
+  HX_TRY(ctx, hipEventCreate(&a), "hipEventCreate");
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(ctx, HX_ERR_HIP, "hipEventCreate");
+  }
+  const int keep_order = g->order, keep_per_cu = g->per_cu;
+  hx_status status = HX_OK;
+  for (auto& cs : cand) {
+    g->order = cs.order;
+    g->per_cu = cs.workgroups_per_cu;
+    // the Local in residual mode (stage 0 tables, no time-step fusion): the state is left as it is, the residual cache
+    // is overwritten and the faces get the extrapolation of the unchanged state
+    auto run = [&] {
+      return g->fs->local_euler(group_launch(g, hx::fused_local, 0), g->meshes[0]->precision, g->with_def, 1., nullptr,
+                                0, 1, use_filter);
+    };
+    hipError_t e = run();  // warm-up
+    float best = 0.f;
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+      float ms = 0.f;
+      e = hipEventRecord(a, ctx->stream);
+      if (e == hipSuccess) e = run();
+      if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+      if (e == hipSuccess && (r == 0 || ms < best)) best = ms;
+    }
+    if (e != hipSuccess) {
+      status = fail(ctx, HX_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+      break;
+    }
+    cs.ms = best;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  for (hx_mesh* m : g->meshes) ++m->gen;
+  if (status) {
+    g->order = keep_order;
+    g->per_cu = keep_per_cu;
+    return status;
+  }
+  const auto best = std::min_element(cand.begin(), cand.end(), [](const auto& x, const auto& y) { return x.ms < y.ms; });
+  g->order = best->order;
+  g->per_cu = best->workgroups_per_cu;
+  if (n_table) *n_table = int(cand.size());
+  if (table)
+    for (int k = 0; k < std::min(capacity, int(cand.size())); ++k) table[k] = cand[k];
   return HX_OK;
 }
 
@@ -1607,14 +2275,17 @@ static hx_status fta_bcs(hx_mesh* mesh, int mode)
 
 // ---- Solver::calc_jacobian (Solver.cpp:273-381) ---------------------------------------------------------------------
 
-hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
+}  // extern "C"
+
+namespace {
+
+// every vertex index a vertex-sharing kernel follows: vertex ids and the hanging-vertex matchers
+hx_status check_vertex_data(hx_mesh* mesh, const hx_vertex_data* v)
 {
-  if (!live(mesh) || !v) return HX_ERR_INVALID;
   hx_ctx* ctx = mesh->ctx;
-  Dev_mesh& dm = mesh->dm;
-  const int nd = dm.nd, rs = dm.rs, nvert = 1 << nd, nfq = dm.nfq;
+  const Dev_mesh& dm = mesh->dm;
+  const int nd = dm.nd, nvert = 1 << nd;
   const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1];
-  // every index the kernels follow is checked here
   if (v->n_vertex < 0 || (n_tot && v->n_vertex == 0)) return fail(ctx, HX_ERR_INVALID, "vertex count missing");
   for (int c = 0; c < 2; ++c) {
     if (dm.n_elem[c] && !v->vertex_id[c]) return fail(ctx, HX_ERR_INVALID, "vertex ids missing");
@@ -1622,7 +2293,6 @@ hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
       if (v->vertex_id[c][k] < 0 || v->vertex_id[c][k] >= v->n_vertex)
         return fail(ctx, HX_ERR_INVALID, "vertex id out of range");
   }
-  if (dm.n_elem[HX_DEFORMED] && !v->position) return fail(ctx, HX_ERR_INVALID, "vertex positions missing");
   if (v->n_hanging < 0 || (v->n_hanging && !v->hanging)) return fail(ctx, HX_ERR_INVALID, "hanging vertices missing");
   if (v->n_hanging && nd < 2) return fail(ctx, HX_ERR_INVALID, "hanging vertices need n_dim >= 2");
   for (int i = 0; i < v->n_hanging; ++i) {
@@ -1636,23 +2306,87 @@ hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
       if (c < 0 || c > 1 || e < 0 || e >= dm.n_elem[c]) return fail(ctx, HX_ERR_INVALID, "hanging-vertex element out of range");
     }
   }
-  ++mesh->gen;
-  Scratch sc;
-  int* vid = nullptr;
-  int* hang = nullptr;
-  double *pos = nullptr, *adj = nullptr, *vt = nullptr, *shared = nullptr, *vals = nullptr;
-  const long long n_def = dm.n_elem[HX_DEFORMED];
-  HX_TRY(ctx, sc.put(&vid, (const int*)nullptr, size_t(n_tot * nvert)), "hipMalloc(vertex ids)");
+  return HX_OK;
+}
+
+// the per-element vertex copies `vert` [n_cartesian + n_deformed][2^nd] shared over every vertex (by minimum or by
+// maximum), then the hanging vertices matched; the buffers come from `sc`
+hx_status share_vertices(hx_mesh* mesh, const hx_vertex_data* v, double* vert, bool maximum, Scratch& sc)
+{
+  hx_ctx* ctx = mesh->ctx;
+  const Dev_mesh& dm = mesh->dm;
+  const int nvert = 1 << dm.nd;
+  const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1], n = n_tot * nvert;
+  int *vid = nullptr, *hang = nullptr;
+  double *shared = nullptr, *vals = nullptr;
+  HX_TRY(ctx, sc.put(&vid, (const int*)nullptr, size_t(n)), "hipMalloc(vertex ids)");
   for (int c = 0, off = 0; c < 2; off += dm.n_elem[c] * nvert, ++c)
     if (dm.n_elem[c])
       HX_TRY(ctx, hipMemcpy(vid + off, v->vertex_id[c], size_t(dm.n_elem[c]) * nvert * sizeof(int), hipMemcpyHostToDevice),
              "upload vertex ids");
-  HX_TRY(ctx, sc.put(&pos, v->position, size_t(n_def * nvert * nd)), "upload vertex positions");
-  if (v->node_adj) HX_TRY(ctx, sc.put(&adj, v->node_adj, size_t(n_def * 2 * nd * nfq)), "upload node adjustments");
   HX_TRY(ctx, sc.put(&hang, v->hanging, size_t(12LL * v->n_hanging)), "upload hanging vertices");
-  HX_TRY(ctx, sc.put(&vt, (const double*)nullptr, size_t(n_tot * nvert)), "hipMalloc(vertex scale)");
   HX_TRY(ctx, sc.put(&shared, (const double*)nullptr, size_t(v->n_vertex)), "hipMalloc(shared vertices)");
   HX_TRY(ctx, sc.put(&vals, (const double*)nullptr, size_t(9LL * v->n_hanging)), "hipMalloc(hanging values)");
+  if (n) {
+    const double inf = std::numeric_limits<double>::infinity();
+    hipLaunchKernelGGL(k_fill_n, dim3(unsigned((v->n_vertex + 255) / 256)), dim3(256), 0, ctx->stream, shared,
+                       maximum ? -inf : inf, (long long)v->n_vertex);
+    if (maximum)
+      hipLaunchKernelGGL(k_vertex_push_max, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vert, n, shared);
+    else
+      hipLaunchKernelGGL(k_vertex_push_min, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vert, n, shared);
+    hipLaunchKernelGGL(k_vertex_fetch, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vert, n, shared);
+    HX_TRY(ctx, hipGetLastError(), "vertex sharing");
+  }
+  Launch l = make_launch(mesh);
+  HX_TRY(ctx, mesh->ks->hanging_match(l, hang, v->n_hanging, vert, vals, 0), "hanging vertices");
+  HX_TRY(ctx, mesh->ks->hanging_match(l, hang, v->n_hanging, vert, vals, 1), "hanging vertices");
+  return HX_OK;
+}
+
+// Stab_art_visc (stabilizing_art_visc.cpp:8-61) over both classes into a device array [n_tot][2^nd] (each element's
+// value in all its vertex slots), and the element values copied to `uncert` when it is not null
+hx_status stab_into(hx_mesh* mesh, double char_speed, double* vert, double* uncert)
+{
+  hx_ctx* ctx = mesh->ctx;
+  const Dev_mesh& dm = mesh->dm;
+  const int nvert = 1 << dm.nd;
+  const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1];
+  Launch l = make_launch(mesh);
+  for (int cls = 0; cls < 2; ++cls) {
+    Timer tm(ctx, std::string(cls_name(cls)) + "/stabilizing art visc/indicator", dm.n_elem[cls]);
+    HX_TRY(ctx, mesh->ks->stab_indicator(l, cls, char_speed, vert), "stab_art_visc");
+  }
+  if (uncert && n_tot) {
+    std::vector<double> h(size_t(n_tot) * nvert);
+    HX_TRY(ctx, hipMemcpyAsync(h.data(), vert, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+           "element values readback");
+    HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "stab_art_visc sync");
+    for (long long e = 0; e < n_tot; ++e) uncert[e] = h[size_t(e) * nvert];
+  }
+  return HX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
+{
+  if (!live(mesh) || !v) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  Dev_mesh& dm = mesh->dm;
+  const int nd = dm.nd, rs = dm.rs, nvert = 1 << nd, nfq = dm.nfq;
+  const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1];
+  if (hx_status s = check_vertex_data(mesh, v)) return s;
+  if (dm.n_elem[HX_DEFORMED] && !v->position) return fail(ctx, HX_ERR_INVALID, "vertex positions missing");
+  ++mesh->gen;
+  Scratch sc;
+  double *pos = nullptr, *adj = nullptr, *vt = nullptr;
+  const long long n_def = dm.n_elem[HX_DEFORMED];
+  HX_TRY(ctx, sc.put(&pos, v->position, size_t(n_def * nvert * nd)), "upload vertex positions");
+  if (v->node_adj) HX_TRY(ctx, sc.put(&adj, v->node_adj, size_t(n_def * 2 * nd * nfq)), "upload node adjustments");
+  HX_TRY(ctx, sc.put(&vt, (const double*)nullptr, size_t(n_tot * nvert)), "hipMalloc(vertex scale)");
   Launch l = make_launch(mesh);
   {
     Timer tm(ctx, "compute jacobian/elements", n_tot);
@@ -1666,22 +2400,50 @@ hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
   }
   {
     Timer tm(ctx, "compute jacobian/vertex time step scale", n_tot);
-    const long long n = n_tot * nvert;
-    if (n) {
-      hipLaunchKernelGGL(k_fill_n, dim3(unsigned((v->n_vertex + 255) / 256)), dim3(256), 0, ctx->stream, shared,
-                         std::numeric_limits<double>::infinity(), (long long)v->n_vertex);
-      hipLaunchKernelGGL(k_vertex_push_min, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vt, n, shared);
-      hipLaunchKernelGGL(k_vertex_fetch, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vt, n, shared);
-      HX_TRY(ctx, hipGetLastError(), "vertex sharing");
-    }
-    HX_TRY(ctx, mesh->ks->hanging_match(l, hang, v->n_hanging, vt, vals, 0), "hanging vertices");
-    HX_TRY(ctx, mesh->ks->hanging_match(l, hang, v->n_hanging, vt, vals, 1), "hanging vertices");
+    if (hx_status s = share_vertices(mesh, v, vt, false, sc)) return s;
   }
   for (int c = 0, off = 0; c < 2; off += dm.n_elem[c] * nvert, ++c)
     if (dm.n_elem[c])
       HX_TRY(ctx, hipMemcpyAsync(dm.vertex_tss[c], vt + off, size_t(dm.n_elem[c]) * nvert * sizeof(double),
                                  hipMemcpyDeviceToDevice, ctx->stream), "vertex scale copy");
   HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "calc_jacobian sync");
+  return HX_OK;
+}
+
+
+hx_status hx_stab_art_visc(hx_mesh* mesh, double char_speed, double* uncert)
+{
+  if (!live(mesh) || !uncert) return HX_ERR_INVALID;
+  const long long n_tot = (long long)mesh->dm.n_elem[0] + mesh->dm.n_elem[1];
+  Scratch sc;
+  double* vert = nullptr;
+  HX_TRY(mesh->ctx, sc.put(&vert, (const double*)nullptr, size_t(n_tot) << mesh->dm.nd), "hipMalloc(element values)");
+  return stab_into(mesh, char_speed, vert, uncert);
+}
+
+hx_status hx_set_art_visc_admis(hx_mesh* mesh, const hx_vertex_data* v, double char_speed, double* uncert)
+{
+  if (!live(mesh) || !v) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  const Dev_mesh& dm = mesh->dm;
+  for (int c = 0; c < 2; ++c)
+    if (dm.n_elem[c] && !dm.av[c]) return fail(ctx, HX_ERR_INVALID, "set_art_visc_admis needs the artificial viscosity arrays");
+  if (hx_status s = check_vertex_data(mesh, v)) return s;
+  const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1];
+  Scratch sc;
+  double* vert = nullptr;
+  HX_TRY(ctx, sc.put(&vert, (const double*)nullptr, size_t(n_tot) << dm.nd), "hipMalloc(vertex values)");
+  if (hx_status s = stab_into(mesh, char_speed, vert, uncert)) return s;
+  {
+    Timer tm(ctx, "stabilizing art visc/vertices", n_tot);
+    if (hx_status s = share_vertices(mesh, v, vert, true, sc)) return s;
+  }
+  Launch l = make_launch(mesh);
+  for (int cls = 0; cls < 2; ++cls) {
+    Timer tm(ctx, std::string(cls_name(cls)) + "/stabilizing art visc/interpolate", dm.n_elem[cls]);
+    HX_TRY(ctx, mesh->ks->vertex_interp_lap(l, cls, vert), "vertex interpolation");
+  }
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "set_art_visc_admis sync");
   return HX_OK;
 }
 

@@ -65,7 +65,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_av_write_face(Dev_me
   for (int v = 0; v < NVAR; ++v)
     lds[el * NVAR * NQP + v * NQP + S::pad(q)] = valid ? av_extrap<ND, RS, KIND>(m, cls, e, v, q) : 0.;
   __syncthreads();
-  write_faces_from<ND, RS, S::BLOCK, NVAR>(m, cls, o, lds, NVAR * NQP);
+  write_faces_from<ND, RS, S::BLOCK, NVAR>(m, cls, blockIdx.x, o, lds, NVAR * NQP);
 }
 
 // ---- Neighbor<Advection / Smooth_art_visc / Fix_therm_admis> (Spatial.hpp:613-704) -----------------------------
@@ -222,7 +222,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_adv_local(Dev_mesh m
   for (int j = 0; j < ND; ++j) my[L::U + j * NQP + qp] = vel[j];
   for (int v = 0; v < RS; ++v) my[L::U + (ND + v) * NQP + qp] = a[v];
   __syncthreads();
-  write_faces_from<ND, RS, NT, ND + RS>(m, cls, o, lds + L::U, L::PER);
+  write_faces_from<ND, RS, NT, ND + RS>(m, cls, blockIdx.x, o, lds + L::U, L::PER);
 }
 
 // ---- Local<Smooth_art_visc / Fix_therm_admis> (Spatial.hpp:326-509, stage 0) -----------------------------------
@@ -423,7 +423,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_reconcile(Dev_me
   }
   for (int v = 0; v < NS; ++v) my[PF + v * NQP + S::pad(q)] = fv[v];
   __syncthreads();
-  write_faces_from<ND, RS, S::BLOCK, NS>(m, cls, o, lds + PF, PER);
+  write_faces_from<ND, RS, S::BLOCK, NS>(m, cls, blockIdx.x, o, lds + PF, PER);
 }
 
 // ---- Max_dt<Advection / Smooth_art_visc> (Spatial.hpp:784-828) ------------------------------------------------
@@ -653,6 +653,86 @@ __global__ __launch_bounds__(256) void k_av_points(Dev_mesh m, Av_consts<RS> ac,
     for (int w = 0; w < 256 / 64; ++w) b += wsum[w];
     atomicAdd(resid, b);
   }
+}
+
+// ---- stabilizing artificial viscosity (stabilizing_art_visc.cpp:8-61, Solver.cpp:636-659) --------------------------
+// Per element, the specific volume's share of L2 energy in the top Legendre mode along each dimension,
+//   nonsmooth = sum_dims sum_rows (row . orth_{rs-1} w)^2 face_weight / (n_dim ||1/rho||^2),
+// mapped to [0, 1] by a sine ramp of half width 0.5 around log10 = -4.25 log10(rs - 1), times (rs - 1) char_speed h.
+// One thread per point stages 1/rho in LDS, one thread per row forms its squared projection, and one thread per element
+// sums both in the reference's order (points, then dimensions and rows) and writes the value to every vertex slot of
+// the element (out[e][2^nd], ready for sharing by maximum).
+template <int ND, int RS>
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_stab_indicator(Dev_mesh m, Av_consts<RS> c, int cls,
+                                                                        double char_speed, double* __restrict__ out)
+{
+  using S = Shape<ND, RS>;
+  using R = Row<ND, RS>;
+  constexpr int NQ = S::NQ, NFQ = S::NFQ, EPB = S::EPB, NR = ND * NFQ;
+  __shared__ double iv[EPB][NQ];
+  __shared__ double rowv[EPB][NR];
+  const int t = threadIdx.x, el = t / NQ, q = t % NQ;
+  const int n = m.n_elem[cls];
+  const int e = blockIdx.x * EPB + el;
+  const bool valid = el < EPB && e < n;
+  if (el < EPB) iv[el][q] = valid ? 1. / m.state[cls][size_t(e) * S::NV * NQ + ND * NQ + q] : 0.;
+  __syncthreads();
+  for (int k = t; k < EPB * NR; k += S::BLOCK) {
+    const int el2 = k / NR, r = k % NR, i_dim = r / NFQ, f = r % NFQ;
+    const int base = R::base_of_fq(f, i_dim), sd = R::stride(i_dim);
+    double dot = 0.;
+    for (int j = 0; j < RS; ++j) dot += iv[el2][base + j * sd] * (c.orth[j] * c.weight[j]);
+    double fw = 1.;
+    for (int d = 0; d < ND - 1; ++d) fw *= c.weight[(f / ipow(RS, ND - 2 - d)) % RS];
+    rowv[el2][r] = dot * dot * fw;
+  }
+  __syncthreads();
+  if (!valid || q != 0) return;
+  double norm_sq = 0.;
+  for (int p = 0; p < NQ; ++p) {
+    double qw = 1.;
+    for (int d = 0; d < ND; ++d) qw *= c.weight[(p / ipow(RS, ND - 1 - d)) % RS];
+    norm_sq += iv[el][p] * iv[el][p] * qw;
+  }
+  double nonsmooth = 0.;
+  for (int r = 0; r < NR; ++r) nonsmooth += rowv[el][r];
+  nonsmooth /= norm_sq * ND;
+  const double center = -4.25 * log(double(RS - 1)) / log(10.), half_width = .5;
+  double ind = log(nonsmooth) / log(10.);
+  if (ind <= center - half_width) ind = 0.;
+  else if (ind < center + half_width) ind = .5 * (1. + sin(M_PI * (ind - center) / 2. / half_width));
+  else ind = 1.;
+  const double val = (RS - 1) * char_speed * m.nom_size[cls][e] * ind;
+  for (int v = 0; v < (1 << ND); ++v) out[size_t(e) * (1 << ND) + v] = val;
+}
+
+// Laplacian artificial viscosity at every point from the element's vertex values vert[e][2^nd] (first vertex index
+// slowest): Gauss_lobatto(2)'s linear interpolant (1 - x, x) at the nodes along each dimension, applied as
+// math::hypercube_matvec does (the last dimension contracted first).
+template <int ND, int RS>
+__global__ __launch_bounds__(256) void k_vertex_interp_lap(Dev_mesh m, Av_consts<RS> c, int cls,
+                                                           const double* __restrict__ vert)
+{
+  constexpr int NQ = Shape<ND, RS>::NQ, NVERT = 1 << ND;
+  const size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (gid >= size_t(m.n_elem[cls]) * NQ) return;
+  const size_t e = gid / NQ;
+  const int q = int(gid % NQ);
+  double v[NVERT];
+  for (int k = 0; k < NVERT; ++k) v[k] = vert[e * NVERT + k];
+  // contract dimensions from the last to the first: after step d the first 2^d entries hold the partial results
+  int len = NVERT;
+  for (int d = ND - 1; d >= 0; --d) {
+    const double x = c.node[(q / ipow(RS, ND - 1 - d)) % RS];
+    len /= 2;
+    for (int k = 0; k < len; ++k) {
+      double acc = 0.;
+      acc += (1. - x) * v[2 * k];
+      acc += x * v[2 * k + 1];
+      v[k] = acc;
+    }
+  }
+  m.av[cls][e * 2 * NQ + NQ + q] = v[0];
 }
 
 }  // namespace hx

@@ -389,14 +389,14 @@ __device__ inline void filter_columns(const Eops<RS, double>& o, T* buf, int per
 // Loads block `ldg` (0: state/flux, 1: LDG) of every face of the workgroup's elements into dst[EPB][2ND][NV][NFQ]
 // (element stride `per`).
 template <int ND, int RS, int NT>
-__device__ inline void load_faces(const Dev_mesh& m, int cls, int ldg, double* dst, int per)
+__device__ inline void load_faces(const Dev_mesh& m, int cls, int blk, int ldg, double* dst, int per)
 {
   using S = Shape<ND, RS>;
   constexpr int PF = 2 * ND * S::NV * S::NFQ;
   const int n = m.n_elem[cls];
   for (int k = threadIdx.x; k < S::EPB * PF; k += NT) {
     const int el = k / PF, r = k % PF, f = r / (S::NV * S::NFQ), vq = r % (S::NV * S::NFQ);
-    const int e = blockIdx.x * S::EPB + el;
+    const int e = blk * S::EPB + el;
     dst[el * per + r] = e < n ? slot_ptr(m, m.face_slot[cls][e * 2 * ND + f])[ldg * S::NV * S::NFQ + vq] : 0.;
   }
 }
@@ -404,13 +404,14 @@ __device__ inline void load_faces(const Dev_mesh& m, int cls, int ldg, double* d
 // Write_face of the FP64 padded [NVAR][NQP] arrays at `buf` (element stride `per`) into block 0 of every face
 // (Spatial.hpp:29-71)
 template <int ND, int RS, int NT, int NVAR = Shape<ND, RS>::NV>
-__device__ inline void write_faces_from(const Dev_mesh& m, int cls, const Eops<RS, double>& o, const double* buf, int per)
+__device__ inline void write_faces_from(const Dev_mesh& m, int cls, int blk, const Eops<RS, double>& o, const double* buf,
+                                        int per)
 {
   using S = Shape<ND, RS>;
   const int n = m.n_elem[cls];
   for (int i = 0; i < ND; ++i) {
     for_columns<ND, RS, NVAR, NT, false>(i, [&](bool, int el, int v, int r, int base, int sd) {
-      const int e = blockIdx.x * S::EPB + el;
+      const int e = blk * S::EPB + el;
       if (e >= n) return;
       double x[RS];
       load_row(buf + el * per + v * S::NQP + base, sd, x);
@@ -538,6 +539,10 @@ __device__ inline double max_dt_scale_u(const Dev_mesh& m, const Ops<RS>* __rest
   return scale;
 }
 
+// The stage coefficient: `coef` itself, or with a device-resident time step (dtp not null) coef times that time step,
+// the product the host forms when it passes the time step by value (coef = dt, or dt times the stage-1 step ratio)
+__device__ inline double step_coef(double coef, const double* dtp) { return dtp ? coef * *dtp : coef; }
+
 // ---- Local, Euler (Spatial.hpp:326-509 with Navier_stokes<false>) -----------------------------------------
 template <int ND, int RS, bool DEF, class P>
 struct Euler_lds
@@ -575,23 +580,34 @@ constexpr int euler_waves()
   return w < 1 ? 1 : (w > 5 ? 5 : w);
 }
 
+// LDS bytes of one Euler Local workgroup: the arrays of T, then (8-byte aligned) the arrays of double
 template <int ND, int RS, bool DEF, class P>
-__global__ __launch_bounds__((euler_block<ND, RS>())) __attribute__((amdgpu_waves_per_eu(euler_waves<ND, RS, DEF, P>())))
-void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, double coef, int stage,
-                   int compute_residual, int use_filter, Dt_fuse fuse, const Ops<RS>* __restrict__ gops)
+constexpr int euler_lds_bytes()
+{
+  using L = Euler_lds<ND, RS, DEF, P>;
+  constexpr int t = Shape<ND, RS>::EPB * int(sizeof(typename P::T)) * L::PER_T;
+  return (t + 7) / 8 * 8 + Shape<ND, RS>::EPB * 8 * L::PER_D;
+}
+
+// One workgroup's worth of the Euler Local: the EPB elements of tile `blk` of class DEF, with the LDS arrays lt (T) and
+// ld (double) of Euler_lds. Shared by k_local_euler (one tile per workgroup) and the fused launch (tiles of several
+// meshes, classes and row sizes per kernel).
+template <int ND, int RS, bool DEF, class P>
+__device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typename P::T* lt, double* ld,
+                                                 const Eops<RS, double>& o, const Eops<RS, double>* od, double coef,
+                                                 const double* __restrict__ dtp, int stage, int compute_residual,
+                                                 int use_filter, const Dt_fuse& fuse, const Ops<RS>* __restrict__ gops)
 {
   using S = Shape<ND, RS>;
   using L = Euler_lds<ND, RS, DEF, P>;
   using T = typename P::T;
   constexpr int NQ = S::NQ, NQP = S::NQP, NFQ = S::NFQ, NV = S::NV, EPB = S::EPB, NT = euler_block<ND, RS>();
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
-  __shared__ T lt[EPB * L::PER_T];
-  __shared__ double ld[EPB * L::PER_D];
   const int t = threadIdx.x;
   const bool pt = t < EPB * NQ;
   const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0, qp = S::pad(q);
   const int n = m.n_elem[cls];
-  const int e = blockIdx.x * EPB + el;
+  const int e = blk * EPB + el;
   const bool valid = pt && e < n;
   T* my = lt + el * L::PER_T;
   const Contraction<RS, P> con(o, od);
@@ -603,7 +619,7 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
   // are not held in registers across the row phases
   const double* mt = m.metrics + (DEF ? size_t(e) * (ND * ND + 1) * NQ + q : 0);
   const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
-  load_faces<ND, RS, NT>(m, cls, 0, ld + L::SB, L::PER_D);
+  load_faces<ND, RS, NT>(m, cls, blk, 0, ld + L::SB, L::PER_D);
   for (int i = 0; i < ND; ++i) {
     // volume flux along reference direction i (Spatial.hpp:404-419), in FP64
     double nrm[ND];
@@ -637,7 +653,7 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
   }
   // stage update (Spatial.hpp:483-503)
   if (valid) {
-    double mult = coef * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
+    double mult = step_coef(coef, dtp) * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
     if constexpr (DEF) mult /= mt[ND * ND * NQ];
     double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
@@ -668,7 +684,7 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
   if (pt)
     for (int v = 0; v < NV; ++v) ld[el * L::PER_D + L::U + v * NQP + qp] = u[v];
   __syncthreads();
-  write_faces_from<ND, RS, NT>(m, cls, o, ld + L::U, L::PER_D);
+  write_faces_from<ND, RS, NT>(m, cls, blk, o, ld + L::U, L::PER_D);
   if (fuse.block_min) {
     for (int off = 32; off > 0; off /= 2) {
       const double x = __shfl_xor(c, off, 64);
@@ -683,9 +699,21 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
     if (t == 0) {
       double b = wmin[0];
       for (int w = 1; w < NT / 64; ++w) b = (wmin[w] < b) ? wmin[w] : b;
-      fuse.block_min[fuse.block_offset + blockIdx.x] = b;
+      fuse.block_min[fuse.block_offset + blk] = b;
     }
   }
+}
+
+template <int ND, int RS, bool DEF, class P>
+__global__ __launch_bounds__((euler_block<ND, RS>())) __attribute__((amdgpu_waves_per_eu(euler_waves<ND, RS, DEF, P>())))
+void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, double coef, const double* __restrict__ dtp,
+                   int stage, int compute_residual, int use_filter, Dt_fuse fuse, const Ops<RS>* __restrict__ gops)
+{
+  using L = Euler_lds<ND, RS, DEF, P>;
+  constexpr int EPB = Shape<ND, RS>::EPB;
+  __shared__ typename P::T lt[EPB * L::PER_T];
+  __shared__ double ld[EPB * L::PER_D];
+  local_euler_tile<ND, RS, DEF, P>(m, blockIdx.x, lt, ld, o, od, coef, dtp, stage, compute_residual, use_filter, fuse, gops);
 }
 
 // ---- Local, Navier-Stokes stage 0 (Spatial.hpp:326-509 with Navier_stokes<true>) -------------------------
@@ -735,9 +763,8 @@ constexpr int ns_waves()
 template <int ND, int RS, bool DEF, class P>
 __global__ __launch_bounds__((form_block<ND, RS, P>())) __attribute__((amdgpu_waves_per_eu(ns_waves<ND, RS, DEF, P>())))
 void k_local_ns(Dev_mesh m, Eops<RS, double> o,
-                                                                        const Eops<RS, double>* od, Ns_params ns,
-                                                                        double coef, int compute_residual,
-                                                                        int use_filter)
+                const Eops<RS, double>* od, Ns_params ns, double coef, const double* __restrict__ dtp,
+                int compute_residual, int use_filter)
 {
   using S = Shape<ND, RS>;
   using L = Ns_lds<ND, RS, DEF, P>;
@@ -792,7 +819,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
     if constexpr (DEF) return metd[i][j];
     else return double(i == j);
   };
-  load_faces<ND, RS, NT>(m, cls, 1, ld + L::SL, L::PER_D);
+  load_faces<ND, RS, NT>(m, cls, blockIdx.x, 1, ld + L::SL, L::PER_D);
   if constexpr (DEF) {
     for (int k = t; k < EPB * 2 * ND * ND * NFQ; k += NT) {
       const int el2 = k / (2 * ND * ND * NFQ), r = k % (2 * ND * ND * NFQ), f = r / (ND * NFQ), jq = r % (ND * NFQ);
@@ -971,7 +998,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
   }
   // stage-0 update (Spatial.hpp:483-503): the convective part goes to the residual cache
   if (valid) {
-    double mult = coef * m.tss[cls][size_t(e) * NQ + q] * inv_h;
+    double mult = step_coef(coef, dtp) * m.tss[cls][size_t(e) * NQ + q] * inv_h;
     if constexpr (DEF) mult /= det;
     double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
@@ -1009,8 +1036,8 @@ constexpr int reconcile_waves()
 
 template <int ND, int RS, bool DEF>
 __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) __attribute__((amdgpu_waves_per_eu(reconcile_waves<ND, RS>())))
-void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt,
-                                                                        int compute_residual, int use_filter)
+void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __restrict__ dtp, int compute_residual,
+                     int use_filter)
 {
   using S = Shape<ND, RS>;
   using R = Row<ND, RS>;
@@ -1026,7 +1053,7 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt,
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : 0.;
-  load_faces<ND, RS, S::BLOCK>(m, cls, 1, lds + L::SL, L::PER);
+  load_faces<ND, RS, S::BLOCK>(m, cls, blockIdx.x, 1, lds + L::SL, L::PER);
   __syncthreads();
   // -boundary_term(jump) (Spatial.hpp:560-570)
   double rate[NV] = {};
@@ -1042,7 +1069,7 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt,
     for (int v = 0; v < NV; ++v) rate[v] = my[L::U + v * S::NQP + S::pad(q)];
   }
   if (valid) {
-    double mult = dt * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
+    double mult = step_coef(dt, dtp) * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
     if constexpr (DEF) mult /= m.metrics[size_t(e) * (ND * ND + 1) * NQ + ND * ND * NQ + q];
     if (compute_residual) {
       double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
@@ -1059,7 +1086,7 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt,
   __syncthreads();
   for (int v = 0; v < NV; ++v) my[L::U + v * S::NQP + S::pad(q)] = u[v];
   __syncthreads();
-  write_faces_from<ND, RS, S::BLOCK>(m, cls, o, lds + L::U, L::PER);
+  write_faces_from<ND, RS, S::BLOCK>(m, cls, blockIdx.x, o, lds + L::U, L::PER);
 }
 
 // ---- Write_face (Spatial.hpp:60-70) ----------------------------------------------------------------------
@@ -1080,12 +1107,12 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh 
 }
 
 // ---- Neighbor (Spatial.hpp:613-704) ----------------------------------------------------------------------
+// one thread per (connection, face point) item `gid` of the class's connections
 template <int ND, int RS, bool DEF, bool VISC>
-__global__ __launch_bounds__(256) void k_neighbor(Dev_mesh m)
+__device__ __forceinline__ void neighbor_at(const Dev_mesh& m, int gid)
 {
   constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
-  const int gid = blockIdx.x * 256 + threadIdx.x;
   const int con = gid / NFQ, f = gid % NFQ;
   if (con >= m.n_con[cls]) return;
   const Dir dir(m.con_dir[cls][con]);
@@ -1126,6 +1153,12 @@ __global__ __launch_bounds__(256) void k_neighbor(Dev_mesh m)
     f0[v * NFQ + f] = sg0 * fl;
     f1[v * NFQ + p1] = sg1 * fl;
   }
+}
+
+template <int ND, int RS, bool DEF, bool VISC>
+__global__ __launch_bounds__(256) void k_neighbor(Dev_mesh m)
+{
+  neighbor_at<ND, RS, DEF, VISC>(m, blockIdx.x * 256 + threadIdx.x);
 }
 
 // ---- Neighbor_reconcile (Spatial.hpp:716-759) ------------------------------------------------------------
@@ -1439,11 +1472,11 @@ __device__ inline void reflect_normal(double* g, const double* nrml, int f)
   for (int d = 0; d < ND; ++d) g[d * NFQ + f] -= 2 * dot * nrml[d * NFQ + f] / norm_sq;
 }
 
+// one thread per (boundary connection, face point) item `gid`
 template <int ND, int RS>
-__global__ __launch_bounds__(256) void k_state_bcs(Dev_mesh m)
+__device__ __forceinline__ void state_bcs_at(const Dev_mesh& m, int gid)
 {
   constexpr int NFQ = ipow(RS, ND - 1), NV = ND + 2;
-  const int gid = blockIdx.x * 256 + threadIdx.x;
   const int ib = gid / NFQ, f = gid % NFQ;
   if (ib >= m.n_bc_con) return;
   const int con = m.bc_con[ib];
@@ -1517,6 +1550,12 @@ __global__ __launch_bounds__(256) void k_state_bcs(Dev_mesh m)
     default:
       break;
   }
+}
+
+template <int ND, int RS>
+__global__ __launch_bounds__(256) void k_state_bcs(Dev_mesh m)
+{
+  state_bcs_at<ND, RS>(m, blockIdx.x * 256 + threadIdx.x);
 }
 
 template <int ND, int RS>
@@ -1629,16 +1668,17 @@ struct Launchers
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
     const Eops<RS, double>* od = static_cast<const Eops<RS, double>*>(l.eops);
     if (visc) {
-      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_local_ns<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, cr, uf);
-      else hipLaunchKernelGGL((k_local_ns<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, cr, uf);
+      if (cls == HX_DEFORMED)
+        hipLaunchKernelGGL((k_local_ns<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, l.dtp, cr, uf);
+      else hipLaunchKernelGGL((k_local_ns<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, l.dtp, cr, uf);
     } else {
       const dim3 be(euler_block<ND, RS>());
       Dt_fuse f = l.fuse;  // the deformed class's workgroups store after the Cartesian class's
       if (f.block_min && cls == HX_DEFORMED) f.block_offset += blocks_for(l.m.n_elem[HX_CARTESIAN], S::EPB);
       if (cls == HX_DEFORMED)
-        hipLaunchKernelGGL((k_local_euler<ND, RS, true, P>), g, be, 0, l.stream, l.m, o, od, coef, stage, cr, uf, f, ops(l));
+        hipLaunchKernelGGL((k_local_euler<ND, RS, true, P>), g, be, 0, l.stream, l.m, o, od, coef, l.dtp, stage, cr, uf, f, ops(l));
       else
-        hipLaunchKernelGGL((k_local_euler<ND, RS, false, P>), g, be, 0, l.stream, l.m, o, od, coef, stage, cr, uf, f, ops(l));
+        hipLaunchKernelGGL((k_local_euler<ND, RS, false, P>), g, be, 0, l.stream, l.m, o, od, coef, l.dtp, stage, cr, uf, f, ops(l));
     }
     return hipGetLastError();
   }
@@ -1662,8 +1702,8 @@ struct Launchers
     if (n == 0) return hipSuccess;
     dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
-    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, cr, uf);
-    else hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, cr, uf);
+    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, l.dtp, cr, uf);
+    else hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, l.dtp, cr, uf);
     return hipGetLastError();
   }
   static hipError_t max_dt(const Launch& l, int cls, bool visc, int is_local, double cc, double cd)
@@ -1896,13 +1936,36 @@ struct Launchers
                        cls, what, mult, us_max, resid);
     return hipGetLastError();
   }
+  // Stab_art_visc into per-element vertex slots (both classes: the deformed class after the Cartesian one), and the
+  // vertex values interpolated into the Laplacian artificial viscosity
+  static hipError_t stab_indicator(const Launch& l, int cls, double char_speed, double* vert)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    const Av_consts<RS> c = av_consts(l, RS);
+    double* out = vert + (cls == HX_DEFORMED ? size_t(l.m.n_elem[HX_CARTESIAN]) << ND : 0);
+    hipLaunchKernelGGL((k_stab_indicator<ND, RS>), dim3(blocks_for(n, S::EPB)), dim3(S::BLOCK), 0, l.stream, l.m, c, cls,
+                       char_speed, out);
+    return hipGetLastError();
+  }
+  static hipError_t vertex_interp_lap(const Launch& l, int cls, const double* vert)
+  {
+    const int n = l.m.n_elem[cls];
+    if (n == 0) return hipSuccess;
+    const Av_consts<RS> c = av_consts(l, RS);
+    const double* in = vert + (cls == HX_DEFORMED ? size_t(l.m.n_elem[HX_CARTESIAN]) << ND : 0);
+    hipLaunchKernelGGL((k_vertex_interp_lap<ND, RS>), dim3(blocks_for(size_t(n) * S::NQ, 256)), dim3(256), 0, l.stream,
+                       l.m, c, cls, in);
+    return hipGetLastError();
+  }
   static const Kernel_set* set()
   {
     static const Kernel_set s{&write_face,  &neighbor,      &local,     &neighbor_reconcile, &reconcile_ldg_flux,
                               &max_dt,      &prolong,       &restrict_, &state_bcs,          &flux_bcs,
                               &av_write_face, &av_neighbor, &adv_local, &diff_local,         &diff_neighbor_reconcile,
                               &diff_reconcile, &av_max_dt,  &av_bcs,    &av_points,          &max_dt_cases,
-                              &admissible,    &fix_av,      &set_jacobian, &connection_normals, &hanging_match};
+                              &admissible,    &fix_av,      &set_jacobian, &connection_normals, &hanging_match,
+                              &stab_indicator, &vertex_interp_lap};
     return &s;
   }
 };

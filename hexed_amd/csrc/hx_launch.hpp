@@ -19,6 +19,7 @@ struct Launch
   double* scalar;    // device scratch: [0] time step reduction
   int precision;     // hx_precision
   Dt_fuse fuse;      // next step's Max_dt fused into the Euler Local (slots == nullptr: not fused)
+  const double* dtp; // device-resident time step the stage kernels multiply their coefficient by, or null
 };
 
 struct Kernel_set
@@ -57,7 +58,60 @@ struct Kernel_set
                              const double* diff, const double* bound);
   hipError_t (*connection_normals)(const Launch&);
   hipError_t (*hanging_match)(const Launch&, const int* hanging, int n_hanging, double* vtss, double* vals, int mode);
+  // Stab_art_visc's element values into vert [n_cartesian + n_deformed][2^nd], and Solver::set_art_visc_admis's
+  // interpolation of vertex values into the Laplacian artificial viscosity
+  hipError_t (*stab_indicator)(const Launch&, int cls, double char_speed, double* vert);
+  hipError_t (*vertex_interp_lap)(const Launch&, int cls, const double* vert);
 };
+
+// ---- fused launches over a group of meshes (hx_group) ----------------------------------------------------------
+constexpr int fused_max_seg = 16;  // (mesh, class) segments of one plan: 8 meshes x 2 classes
+
+// what a fused kernel reads per mesh (device memory)
+struct Fused_member
+{
+  Dev_mesh m;
+  const void* eops;  // Eops<rs, double>
+  const void* ops;   // Ops<rs>
+  Dt_fuse fuse;      // the Euler Local's fused time-step work (stage 1 of a batch step), or none
+};
+
+// tiles of one fused launch (device memory): segment k covers tiles [first[k], first[k + 1]) of member[k], class cls[k],
+// row size rs[k]
+struct Fused_plan
+{
+  int n_seg, n_tiles;
+  int first[fused_max_seg + 1];
+  signed char member[fused_max_seg], cls[fused_max_seg], rs[fused_max_seg];
+};
+
+struct Fused_launch
+{
+  hipStream_t stream;
+  const Fused_member* mem;  // device
+  const Fused_plan* plan;   // device
+  int n_tiles;
+  unsigned grid;            // workgroups
+  int order;                // 0: tiles strided over the grid, 1: each workgroup one contiguous chunk of `chunk` tiles
+  int chunk;
+  const void* const* host_ops;  // host Ops<rs> tables indexed by row size (the context's)
+};
+
+// kinds of fused launch, for Fused_set::tiles
+enum { fused_state_bcs = 0, fused_neighbor = 1, fused_local = 2 };
+
+struct Fused_set
+{
+  bool (*supports)(int rs);
+  hipError_t (*state_bcs)(const Fused_launch&);
+  hipError_t (*neighbor)(const Fused_launch&);
+  hipError_t (*local_euler)(const Fused_launch&, int precision, bool with_def, double coef, const double* dtp, int stage,
+                            int compute_residual, int use_filter);
+  int (*tiles)(int kind, const Dev_mesh& m, int cls);  // tiles of one (mesh, class) segment
+};
+
+// n_dim 3, row sizes 4, 5 and 6
+const Fused_set* fused_set_nd3();
 
 const Kernel_set* kernel_set_nd1(int rs);
 const Kernel_set* kernel_set_nd2(int rs);

@@ -26,6 +26,7 @@ class Context:
         self.lib = abi.load_library()
         self.ptr = ctypes.c_void_p()
         self._meshes = weakref.WeakSet()  # live meshes; a context outlives every mesh made on it
+        self._groups = weakref.WeakSet()  # live groups, closed before the meshes they hold
         rc = self.lib.hx_ctx_create(device, ctypes.byref(self.ptr))
         if rc != abi.HX_OK:
             raise HxError(f"hx_ctx_create failed with status {rc}")
@@ -69,6 +70,8 @@ class Context:
 
     def close(self):
         if self.ptr:
+            for g in list(self._groups):
+                g.close()
             for m in list(self._meshes):
                 m.close()
             self.lib.hx_ctx_destroy(self.ptr)
@@ -97,7 +100,8 @@ class Device_mesh:
         # valid in either order: a mesh whose context was destroyed first holds no device memory any more, and
         # hx_mesh_destroy then releases only its handle
         if self.ptr:
-            self.lib.hx_mesh_destroy(self.ptr)
+            # refused while the mesh belongs to a live group (close the group first): the handle is kept
+            self.ctx.check(self.lib.hx_mesh_destroy(self.ptr), "hx_mesh_destroy")
         self.ptr = ctypes.c_void_p()
 
     def __del__(self):
@@ -207,6 +211,21 @@ class Device_mesh:
         vd = vertex_data if vertex_data is not None else self.host.vertex_data()
         self.ctx.check(self.lib.hx_calc_jacobian(self.ptr, ctypes.byref(vd)), "calc_jacobian")
 
+    def stab_art_visc(self, char_speed):
+        """Stab_art_visc's element values (Cartesian elements first)."""
+        out = np.zeros(sum(self.host.n_elem))
+        self.ctx.check(self.lib.hx_stab_art_visc(self.ptr, char_speed, out.ctypes.data_as(abi.c_double_p)),
+                       "stab_art_visc")
+        return out
+
+    def set_art_visc_admis(self, char_speed, vertex_data=None):
+        """Solver::set_art_visc_admis into the Laplacian artificial viscosity; returns the element values."""
+        vd = vertex_data if vertex_data is not None else self.host.vertex_data()
+        out = np.zeros(sum(self.host.n_elem))
+        self.ctx.check(self.lib.hx_set_art_visc_admis(self.ptr, ctypes.byref(vd), char_speed,
+                                                      out.ctypes.data_as(abi.c_double_p)), "set_art_visc_admis")
+        return out
+
     # state files (Solver::write_state / read_state, Solver.cpp:232-271) -----------------------------------------
     def _field_present(self, name, cls):
         n, p = ctypes.c_size_t(), ctypes.c_void_p()
@@ -264,6 +283,18 @@ class Device_mesh:
         self.ctx.check(self.lib.hx_update(self.ptr, ctypes.byref(opts), ctypes.byref(status)), "update")
         return status
 
+    def update_async(self, opts, n_updates=1, start=None):
+        """n_updates Solver::update calls enqueued with the time step kept on the device (the steady-state update
+        replayed from a captured graph); `start`, an hx_update_status, sets the device status first."""
+        self.ctx.check(self.lib.hx_update_async(self.ptr, ctypes.byref(opts), int(n_updates),
+                                                ctypes.byref(start) if start is not None else None), "update_async")
+
+    def status_sync(self, status=None):
+        """Waits for the device and returns the status that update_async accumulated."""
+        status = status if status is not None else abi.hx_update_status()
+        self.ctx.check(self.lib.hx_update_status_sync(self.ptr, ctypes.byref(status)), "update_status_sync")
+        return status
+
     # batch of independent cases --------------------------------------------------------------------------------
     def set_cases(self, n_cases, case_id):
         """case_id: per class, the case of every element (int32 arrays; None for an empty class)."""
@@ -280,3 +311,52 @@ class Device_mesh:
 
     def compute_residual(self, opts):
         self.ctx.check(self.lib.hx_compute_residual(self.ptr, ctypes.byref(opts)), "compute_residual")
+
+
+class Device_group:
+    """Meshes stepped together (hx_group): each kernel of an Euler stage launched once for all of them, both element
+    classes and every row size (3D, row sizes 4 to 6). Every mesh needs its cases registered (Device_mesh.set_cases);
+    the group must be closed before its meshes."""
+
+    def __init__(self, ctx, meshes):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.meshes = list(meshes)
+        self.n_cases = sum(m.n_cases for m in self.meshes)
+        ptrs = (ctypes.c_void_p * len(self.meshes))(*[m.ptr.value for m in self.meshes])
+        self.ptr = ctypes.c_void_p()
+        ctx.check(self.lib.hx_group_create(ctx.ptr, ptrs, len(self.meshes), ctypes.byref(self.ptr)), "hx_group_create")
+        ctx._groups.add(self)
+
+    @property
+    def fused(self):
+        return bool(self.lib.hx_group_fused(self.ptr))
+
+    def set_shape(self, order, workgroups_per_cu=0):
+        self.ctx.check(self.lib.hx_group_set_shape(self.ptr, order, workgroups_per_cu), "hx_group_set_shape")
+
+    def update(self, opts, status=None):
+        """One Solver::update of every case; `status` is a ctypes array of n_cases hx_update_status (meshes in order)."""
+        status = status if status is not None else (abi.hx_update_status * self.n_cases)()
+        self.ctx.check(self.lib.hx_group_update(self.ptr, ctypes.byref(opts), status), "hx_group_update")
+        return status
+
+    def autotune(self, use_filter=False, reps=5):
+        """Times the fused Local under every candidate launch shape and keeps the fastest; returns
+        [(order, workgroups_per_cu, ms)] of all candidates."""
+        table = (abi.hx_launch_shape * 16)()
+        n = ctypes.c_int()
+        self.ctx.check(self.lib.hx_group_autotune(self.ptr, int(use_filter), reps, table, 16, ctypes.byref(n)),
+                       "hx_group_autotune")
+        return [(table[k].order, table[k].workgroups_per_cu, table[k].ms) for k in range(n.value)]
+
+    def close(self):
+        if self.ptr:
+            self.lib.hx_group_destroy(self.ptr)
+        self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -268,8 +268,9 @@ hx_status hx_ctx_timing(hx_ctx* ctx, const char* category, double* total_ms, lon
 hx_status hx_ctx_timing_list(hx_ctx* ctx, char* out, size_t capacity);
 
 /* Diagnostic with no counterpart in the reference: a plain device-to-device copy kernel ("hx_stream_copy") over
- * `bytes` bytes, run `reps` times; returns the achieved bandwidth counting one read and one write of every byte.
- * Used to measure the achievable HBM bandwidth of the box and to calibrate hardware byte counters. */
+ * `bytes` bytes (four 16-byte loads in flight per lane before their stores), run `reps` times at each of four grid
+ * sizes; returns the best achieved bandwidth counting one read and one write of every byte. Used to measure the
+ * achievable HBM bandwidth of the box and to calibrate hardware byte counters. */
 hx_status hx_stream_copy(hx_ctx* ctx, size_t bytes, int reps, double* gbs);
 
 /* ---- basis (Basis.hpp:14-63, Gauss_legendre.hpp, Basis.cpp:6-14) ------------------------------------- */
@@ -355,8 +356,54 @@ hx_status hx_compute_restrict(hx_mesh* mesh, int scale, int offset);
  * state BCs + stage 1], each stage followed by fix_admissibility when opts->fix_admis is set (a fix ends the Chebyshev
  * sequence without advancing the time, Solver.cpp:867-870). `status` is advanced in place. */
 hx_status hx_update(hx_mesh* mesh, const hx_update_options* opts, hx_update_status* status);
+/* Solver::update (Solver.cpp:834-886) n_updates times, enqueued without waiting for the device: each step's time
+ * step is formed and kept on the device (the reference passes it by value into every kernel constructor,
+ * Solver.cpp:848-849, Spatial.hpp:311), and the status (time_step, flow_time, iteration) accumulates on the device
+ * until hx_update_status_sync reads it. `from`, when not NULL, sets the device status first. The first update of a
+ * sequence runs Max_dt itself; the steady-state update (starting from the next step's Max_dt that stage 1 fused, or
+ * any update with local time stepping) is captured once into a HIP graph and replayed, one graph launch per update,
+ * recaptured when the options or the arithmetic form change. Time steps and states are bit-identical to hx_update's.
+ * No counterpart of the stage sequence changes; fix_admis (whose verdict is read on the host) and batch meshes are
+ * rejected. */
+hx_status hx_update_async(hx_mesh* mesh, const hx_update_options* opts, int n_updates, const hx_update_status* from);
+/* Waits for the mesh's stream and writes the device status of hx_update_async into `status` (time_step, flow_time,
+ * iteration; fix_admis_iters is left as it is). */
+hx_status hx_update_status_sync(hx_mesh* mesh, hx_update_status* status);
 /* Solver::compute_residual (Solver.cpp:899-913): writes the scaled residual into the residual cache. */
 hx_status hx_compute_residual(hx_mesh* mesh, const hx_update_options* opts);
+
+/* ---- groups of meshes stepped together ---------------------------------------------------------------------------- */
+/* No counterpart in the reference, which runs one Solver per mesh: a group of up to 8 batch meshes (each with its cases
+ * registered by hx_mesh_set_cases; a single case is a batch of one) of one n_dim, stepped together. When every mesh is
+ * 3D with row size 4, 5 or 6, each kernel of an Euler stage -- the state boundary conditions, Neighbor and Local
+ * (Spatial.hpp:293-510, 601-705) -- is launched once for all meshes, both element classes and every row size: a
+ * workgroup runs one tile (one per-class workgroup's worth of work) of the (mesh, class) segment it falls in, with the
+ * row size as a template argument, so results are bit-identical to stepping each mesh with hx_update_cases. Hanging
+ * faces (Prolong/Restrict_refined) and the per-case time steps stay per mesh. Navier-Stokes options, local time stepping
+ * and fix_admis step the meshes one at a time. Meshes must outlive the group (hx_mesh_destroy refuses a mesh in a
+ * group); hx_ctx_destroy detaches live groups. */
+typedef struct hx_group hx_group;
+/* A launch shape of the fused Local: order 2 = one tile per workgroup (workgroups_per_cu unused); order 0 = a
+ * persistent grid of workgroups_per_cu workgroups per compute unit, tiles strided over the grid; order 1 = the same
+ * grid, each workgroup one contiguous chunk of tiles. `ms` is the time the autotuner measured for one launch. */
+typedef struct hx_launch_shape {
+  int order;
+  int workgroups_per_cu;
+  double ms;
+} hx_launch_shape;
+hx_status hx_group_create(hx_ctx* ctx, hx_mesh* const* meshes, int n_meshes, hx_group** out);
+hx_status hx_group_destroy(hx_group* group);
+/* 1 when the group's Euler stages run as fused launches, 0 when its meshes step one at a time. */
+int hx_group_fused(const hx_group* group);
+hx_status hx_group_set_shape(hx_group* group, int order, int workgroups_per_cu);
+/* One Solver::update of every case of every mesh; `status` holds one entry per case, meshes in order. */
+hx_status hx_group_update(hx_group* group, const hx_update_options* opts, hx_update_status* status);
+/* Times the fused Local under each candidate launch shape (one tile per workgroup; persistent grids of 2, 4, 8 and 16
+ * workgroups per compute unit, strided and chunked), `reps` launches each after a warm-up, the best of them counted, and
+ * sets the fastest. The Local runs in residual mode: states are left as they are, the residual cache is overwritten and
+ * the faces are rewritten with the extrapolation of the unchanged state, so call it between updates. `table` (up to
+ * `capacity` entries, may be NULL) receives every candidate with its time, n_table their number. */
+hx_status hx_group_autotune(hx_group* group, int use_filter, int reps, hx_launch_shape* table, int capacity, int* n_table);
 
 /* ---- mesh metrics ------------------------------------------------------------------------------------------------ */
 /* Solver::calc_jacobian (Solver.cpp:273-381) without snapping: Element::set_jacobian (Element.cpp:99-112) writes unit
@@ -423,6 +470,17 @@ hx_status hx_max_dt_smooth_av(hx_mesh* mesh, const hx_kernel_options* opts, doub
 /* Solver::update_art_visc_smoothness (Solver.cpp:457-582): writes the bulk coefficient av[.][0] and, if `residual`
  * is not NULL, the art_visc_residual it assigns (Solver.cpp:575). */
 hx_status hx_update_art_visc_smoothness(hx_mesh* mesh, const hx_av_options* opts, double* residual);
+/* Stab_art_visc (stabilizing_art_visc.cpp:8-61; the stabilizing_art_visc free function over a Kernel_mesh): per
+ * element the specific volume's share of L2 energy in the top Legendre mode along each dimension, on a sine ramp of
+ * half width 0.5 around log10 = -4.25 log10(row_size - 1), times (row_size - 1) char_speed nominal_size, written to
+ * uncert[n_cartesian + n_deformed] (Kernel_element::uncert(); Cartesian elements first). */
+hx_status hx_stab_art_visc(hx_mesh* mesh, double char_speed, double* uncert);
+/* Solver::set_art_visc_admis (Solver.cpp:636-659) with char_speed = freestream speed + freestream sound speed:
+ * Stab_art_visc's element values on every vertex, shared by maximum over the elements of each vertex (hanging
+ * vertices matched, as in hx_calc_jacobian; `vertices` needs the ids and matchers, not the positions), interpolated
+ * n-linearly to the quadrature points into the Laplacian coefficient av[.][1]. `uncert`, when not NULL, receives the
+ * element values. */
+hx_status hx_set_art_visc_admis(hx_mesh* mesh, const hx_vertex_data* vertices, double char_speed, double* uncert);
 
 #ifdef __cplusplus
 }

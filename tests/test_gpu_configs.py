@@ -1,0 +1,137 @@
+"""Two configurations stepped on the HIP path against the oracle, end to end:
+
+- config 1, the 2D isentropic vortex (16x16 Cartesian quads on [-5, 5]^2, row size 6, global time step at
+  max_safety 0.5): stage by stage through the kernel calls for the first steps, then 100 hx_update steps compared
+  with the oracle after every step (state <= 1e-12 relative L2 per variable, the same time step), and the final
+  state against the exact translated vortex;
+- config 5, a batch of independent Euler box cases of uneven sizes and Mach numbers at row sizes 4, 5 and 6 (one
+  batch mesh per row size, as the batch workload runs them): each case of every batch is also stepped alone by the
+  oracle, and its time step and state after every step must match (time step to 1e-13, state to 1e-12).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from hexed_amd import abi, configs, mesh
+from hexed_amd.device import Context, Device_mesh
+
+from helpers import init_faces, o_call, per_var_rel_l2, say
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def state_err(dev, host):
+    worst = 0.0
+    for c in (0, 1):
+        if host.n_elem[c]:
+            got = dev.download("state", c, out=np.zeros_like(host.state[c]))
+            worst = max(worst, max(per_var_rel_l2(got, host.state[c])))
+    return worst
+
+
+def test_vortex_config1_stages(ctx):
+    """Three time steps of config 1 through compute_euler's kernels, compared after every kernel call."""
+    say("[gpu] config 1 vortex, stage by stage")
+    host, _ = configs.isentropic_vortex()
+    init_faces(host)
+    dev = Device_mesh(ctx, host.copy())
+    s = host.struct()
+    for step in range(3):
+        dt_o = ctypes.c_double()
+        o_call("oracle_max_dt_euler", ctypes.byref(s), None, 0.5 / 1.0, 0.5, 0, ctypes.byref(dt_o))
+        dt_d = dev.max_dt(0, safety_conv=0.5, safety_diff=0.5)
+        assert abs(dt_d - dt_o.value) <= 1e-13 * dt_o.value, (step, dt_d, dt_o.value)
+        dt = dt_o.value
+        for stage in (0, 1):
+            o_call("oracle_apply_state_bcs", ctypes.byref(s))
+            dev.apply_state_bcs()
+            for c in (0, 1):
+                o_call("oracle_neighbor", ctypes.byref(s), c, None, stage)
+                dev.neighbor(c, i_stage=stage)
+            got = dev.download("faces", 0, out=np.zeros_like(host.faces))
+            err = np.linalg.norm(got - host.faces) / np.linalg.norm(host.faces)
+            assert err <= TOL, f"step {step} stage {stage} neighbor: faces {err:.3e}"
+            for c in (0, 1):
+                o_call("oracle_local", ctypes.byref(s), c, None, dt, stage, 0, 0)
+                dev.local(c, dt, i_stage=stage)
+            err = state_err(dev, host)
+            assert err <= TOL, f"step {step} stage {stage} local: state {err:.3e}"
+    dev.close()
+
+
+def test_vortex_config1_update(ctx):
+    """Config 1's 100 steps: hx_update vs oracle_update after every step, then the exact translated vortex."""
+    say("[gpu] config 1 vortex, 100 updates")
+    host, exact = configs.isentropic_vortex()
+    init_faces(host)
+    dev = Device_mesh(ctx, host.copy())
+    s = host.struct()
+    opts = abi.update_options(max_safety=0.5)
+    st_d, st_o = abi.hx_update_status(), abi.hx_update_status()
+    worst = 0.0
+    for step in range(100):
+        dev.update(opts, st_d)
+        o_call("oracle_update", ctypes.byref(s), ctypes.byref(opts), ctypes.byref(st_o))
+        assert abs(st_d.time_step - st_o.time_step) <= 1e-13 * st_o.time_step, (step, st_d.time_step, st_o.time_step)
+        err = state_err(dev, host)
+        worst = max(worst, err)
+        assert err <= TOL, f"step {step}: state {err:.3e}"
+    t = st_d.flow_time
+    assert t > 0.25
+    got = dev.download("state", 0, out=np.zeros_like(host.state[0]))
+    ref = host.copy()
+    mesh.set_state(ref, lambda pos: exact(pos, t))
+    exact_err = np.linalg.norm(got - ref.state[0]) / np.linalg.norm(ref.state[0])
+    say(f"  worst per-step rel L2 vs oracle {worst:.2e}; vs exact vortex at t={t:.4f}: {exact_err:.2e}")
+    assert exact_err < 1e-4, exact_err
+    dev.close()
+
+
+def small_batch():
+    """Nine cases at row sizes 4, 5 and 6 with element counts log-uniform in [8, 216] and Mach in [0.3, 0.9]."""
+    return configs.batch_cases(n_cases=9, seed=3, min_elem=8, max_elem=216)
+
+
+@pytest.mark.parametrize("precision", [abi.HX_FP64], ids=["fp64"])
+def test_batch_config5_vs_oracle(ctx, precision):
+    cases = small_batch()
+    assert {c["rs"] for c in cases} == {4, 5, 6}
+    assert len({c["n"] for c in cases}) > 1
+    groups = configs.euler_batch(cases)
+    opts = abi.update_options(max_safety=0.6)
+    for rs, (b, ks) in groups.items():
+        say(f"[gpu] config 5 batch vs oracle: row size {rs}, cases {ks}, {b.n_elem[0] + b.n_elem[1]} elements")
+        init_faces(b)
+        dev = Device_mesh(ctx, b.copy())
+        dev.set_precision(precision)
+        dev.set_cases(len(ks), [c.astype(np.int32) for c in b.case_id])
+        st = (abi.hx_update_status * len(ks))()
+        singles = []
+        for k in ks:
+            m = configs.euler_box(cases[k]["n"], rs, cases[k]["mach"], cases[k]["seed"])
+            init_faces(m)
+            singles.append((m, m.struct(), abi.hx_update_status()))
+        for step in range(3):
+            dev.update_cases(opts, st)
+            got = [dev.download("state", c, out=np.zeros_like(b.state[c])) for c in (0, 1)]
+            off = [0, 0]
+            for j, (m, s, so) in enumerate(singles):
+                o_call("oracle_update", ctypes.byref(s), ctypes.byref(opts), ctypes.byref(so))
+                assert abs(st[j].time_step - so.time_step) <= 1e-13 * so.time_step, (rs, j, step)
+                for c in (0, 1):
+                    n = m.n_elem[c]
+                    if n:
+                        err = max(per_var_rel_l2(got[c][off[c]:off[c] + n], m.state[c]))
+                        assert err <= TOL, f"row size {rs} case {ks[j]} step {step}: rel L2 {err:.3e}"
+                    off[c] += n
+        dev.close()

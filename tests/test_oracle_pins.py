@@ -536,3 +536,35 @@ def test_forcing_projection_top_legendre_coefficient(rs, av_rs):
     o_call("oracle_update_art_visc_smoothness", ctypes.byref(m.struct()), ctypes.byref(opts), ctypes.byref(r))
     want = c_top ** 2 * 2 * energy / rho
     np.testing.assert_allclose(m.forcing[0][:, 0], want, rtol=1e-12)
+
+
+def test_art_visc_saturation_and_residual():
+    """update_art_visc_smoothness's clean-up (Solver.cpp:552-575): with no advection or diffusion iterations the last
+    forcing component keeps its input f3, and the bulk coefficient becomes us_max F / (us_max + F) with
+    F = visc_mult * L * f3 and us_max = L * unscaled_max * sqrt(2 E_inf / rho_inf); the residual is the L2 norm (Gauss
+    weights times the element volume h^n_dim) of the coefficient's change. The unsaturated F, or a residual without
+    the volume, would land elsewhere."""
+    rs, n, L, visc_mult, unscaled_max, mass, energy = 5, 4, 0.2, 3.0, 0.7, 1.0, 2.5
+    m = av_line(rs, n, h=0.5)
+    rng = np.random.default_rng(17)
+    m.state[0][:, 0] = 0.3
+    m.state[0][:, 1] = 1.0 + 0.2 * rng.random((n, m.nq))
+    m.state[0][:, 2] = 2.0 + rng.random((n, m.nq))
+    f3 = rng.uniform(0.0, 2.0, size=(n, m.nq))
+    m.forcing[0][:, 3] = f3
+    old = rng.uniform(0.0, 0.5, size=(n, m.nq))
+    m.av[0][:, 0] = old
+    init_faces(m)
+    opts = abi.av_options(L, rs, mass, energy, advect_iters=0, diff_iters=0, visc_mult=visc_mult,
+                          unscaled_max=unscaled_max)
+    r = ctypes.c_double()
+    o_call("oracle_update_art_visc_smoothness", ctypes.byref(m.struct()), ctypes.byref(opts), ctypes.byref(r))
+    us_max = L * unscaled_max * np.sqrt(2 * energy / mass)
+    big_f = visc_mult * L * f3
+    want = us_max * big_f / (us_max + big_f)
+    np.testing.assert_allclose(m.av[0][:, 0], want, rtol=1e-13)
+    assert np.abs(m.av[0][:, 0] - big_f).max() > 1e-2
+    w = BASIS(rs)["weight"]
+    resid = np.sqrt(np.sum((old - want) ** 2 * w[None, :] * 0.5))
+    assert r.value == pytest.approx(resid, rel=1e-12)
+    assert abs(r.value - np.sqrt(np.sum((old - want) ** 2 * w[None, :]))) > 1e-3 * resid
