@@ -222,6 +222,7 @@ def load_library(path=LIB_PATH):
         "hx_update_async": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.c_int,
                                            ctypes.POINTER(hx_update_status)]),
         "hx_update_status_sync": (ctypes.c_int, [P, ctypes.POINTER(hx_update_status)]),
+        "hx_update_no_alloc": (ctypes.c_int, [P, ctypes.POINTER(hx_update_options), ctypes.POINTER(hx_update_status)]),
         "hx_group_create": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)]),
         "hx_group_destroy": (ctypes.c_int, [P]),
         "hx_group_fused": (ctypes.c_int, [P]),
@@ -273,7 +274,7 @@ EXPORTED_SYMBOLS = [
     "hx_neighbor", "hx_local", "hx_neighbor_reconcile", "hx_reconcile_ldg_flux", "hx_max_dt", "hx_prolong_refined",
     "hx_restrict_refined", "hx_apply_state_bcs", "hx_apply_flux_bcs", "hx_compute_euler", "hx_compute_navier_stokes",
     "hx_max_dt_euler", "hx_max_dt_navier_stokes", "hx_compute_write_face", "hx_compute_prolong", "hx_compute_restrict",
-    "hx_update", "hx_update_async", "hx_update_status_sync", "hx_group_create", "hx_group_destroy",
+    "hx_update", "hx_update_async", "hx_update_status_sync", "hx_update_no_alloc", "hx_group_create", "hx_group_destroy",
     "hx_group_fused", "hx_group_set_shape", "hx_group_update", "hx_group_autotune", "hx_compute_residual", "hx_apply_advection_bcs", "hx_apply_avc_diff_bcs", "hx_apply_avc_diff_flux_bcs",
     "hx_compute_advection", "hx_compute_smooth_av", "hx_compute_write_face_advection", "hx_compute_write_face_smooth_av",
     "hx_compute_prolong_advection", "hx_max_dt_advection", "hx_max_dt_smooth_av", "hx_update_art_visc_smoothness",

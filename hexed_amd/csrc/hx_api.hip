@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,6 +16,51 @@
 #include "hx_launch.hpp"
 
 using namespace hx;
+
+// Scratch memory of hx_update_no_alloc, for callers that cannot allocate device memory: [0, 32) holds the reduction
+// scalar of the time step, [32, 32 + no_alloc_max_blocks) one fused Max_dt minimum per Euler Local workgroup (a mesh
+// of up to 262,144 workgroups: 262,144 elements at 3D row size 5 to 8). Each GPU holds its own copy (2 MiB) until the
+// process ends. Host code uses the address hipGetSymbolAddress returns for the current device (hx_ctx::scratch), never
+// the array's name.
+namespace {
+constexpr long long no_alloc_max_blocks = 1LL << 18;
+constexpr long long scratch_minima_offset = 32;
+}  // namespace
+static __device__ __align__(256) double scratch[scratch_minima_offset + no_alloc_max_blocks];
+
+namespace {
+constexpr int max_devices = 64;
+std::mutex scratch_mutex;
+double* scratch_address[max_devices] = {};       // per device, once found
+hipError_t scratch_error[max_devices] = {};
+// advanced by every hx_update_no_alloc on the device: a mesh's fused minima in the array are current while the
+// epoch is the one its own last call left
+std::atomic<long long> scratch_epochs[max_devices];
+
+// the scratch array's address on `device` (the current device), looked up once per device
+double* scratch_on(int device, hipError_t* err)
+{
+  if (device < 0 || device >= max_devices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_address[device] && scratch_error[device] == hipSuccess) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p) {
+      scratch_address[device] = static_cast<double*>(p);
+    } else {
+      scratch_error[device] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+      (void)hipGetLastError();
+    }
+  }
+  *err = scratch_error[device];
+  return scratch_address[device];
+}
+
+long long scratch_epoch(int device) { return scratch_epochs[device].load(); }
+}  // namespace
 
 struct hx_ctx
 {
@@ -44,6 +91,8 @@ struct hx_ctx
   std::vector<hx_mesh*> meshes;  // live meshes of this context, detached by hx_ctx_destroy
   std::vector<hx_group*> groups; // live groups of this context, detached by hx_ctx_destroy
   bool events_off = false;       // no per-launch events (a graph capture, or a call that may not create any)
+  double* scratch = nullptr;     // this device's address of the file-scope scratch array (hx_update_no_alloc)
+  hipError_t scratch_err = hipSuccess;
 };
 
 // status of the updates enqueued by hx_update_async, kept on the device
@@ -86,6 +135,7 @@ struct hx_mesh
     double msc = 0, msd = 0;
     int use_ldg = 0;
     hx_transport visc{}, cond{};
+    bool in_scratch = false;  // the minima are in the scratch array (hx_update_no_alloc), not in dt_block_min
   } dt_cache;
   // the same fusion for hx_update_cases (cases listed in order): stage 1's Local writes each point's 1/scale into
   // case_scale, which the next call's per-case minimum reads instead of running Max_dt
@@ -102,6 +152,11 @@ struct hx_mesh
   // (null otherwise), and the captured steady-state update with the options and form it was captured for
   Step_dev* step = nullptr;
   const double* dt_dev = nullptr;
+  // the reduction scalar the mesh's calls use: the context's, or the file-scope scratch array during
+  // hx_update_no_alloc; and the scratch epoch of the last hx_update_no_alloc on this mesh (its fused minima sit in the
+  // scratch array and are current only while no other call has used the array since)
+  double* scalar = nullptr;
+  long long scratch_epoch = -1;
   int n_groups = 0;  // groups (hx_group) holding the mesh, which must be destroyed first
   hipGraphExec_t graph = nullptr;
   hx_update_options graph_opts{};
@@ -391,7 +446,7 @@ Launch make_launch(hx_mesh* mesh, bool writes = true)
   l.ops = mesh->ctx->ops[mesh->dm.rs];
   l.hops = mesh->ctx->host_ops[mesh->dm.rs].data();
   l.eops = mesh->ctx->eops[mesh->dm.rs];
-  l.scalar = mesh->ctx->scalar;
+  l.scalar = mesh->scalar ? mesh->scalar : mesh->ctx->scalar;
   l.precision = mesh->precision;
   l.fuse = mesh->fuse;
   l.dtp = mesh->dt_dev;
@@ -714,6 +769,8 @@ hx_status hx_ctx_create(int device, hx_ctx** out)
     delete ctx;
     return HX_ERR_HIP;
   }
+  // the scratch array of hx_update_no_alloc, with this device current; a failure is reported by that entry point
+  ctx->scratch = scratch_on(device, &ctx->scratch_err);
   *out = ctx;
   return HX_OK;
 }
@@ -1291,8 +1348,9 @@ struct Update_setup
   int n_cheby;
   bool fusable;      // with a global time step, stage 1's Local also computes the next step's Max_dt
   Ns_params ns;
-  double* block_min; // the fused per-workgroup minima
+  double* block_min; // the fused per-workgroup minima: the mesh's buffer, or the scratch array
   long long n_blocks;
+  bool in_scratch;
   double* scalar;    // the reduction scalar
 };
 
@@ -1303,22 +1361,32 @@ long long euler_local_blocks(const hx_mesh* mesh)
   return (mesh->dm.n_elem[0] + epb - 1) / epb + (mesh->dm.n_elem[1] + epb - 1) / epb;
 }
 
-hx_status update_setup(hx_mesh* mesh, const hx_update_options* o, Update_setup* u)
+// no_alloc: the minima and the reduction scalar go to the file-scope scratch array, and nothing is allocated
+hx_status update_setup(hx_mesh* mesh, const hx_update_options* o, Update_setup* u, bool no_alloc = false)
 {
   hx_ctx* ctx = mesh->ctx;
   // the state that stage 1's Local writes is the state that Max_dt would read next; local time stepping writes the
   // time step scale in Max_dt, so it is not fused
   u->fusable = !o->local_time;
   u->n_blocks = euler_local_blocks(mesh);
-  if (u->fusable && !mesh->dt_block_min) {
-    mesh->n_dt_blocks = u->n_blocks;
-    void* p = nullptr;
-    HX_TRY(ctx, hipMalloc(&p, std::max<long long>(mesh->n_dt_blocks, 1) * sizeof(double)), "hipMalloc(time step minima)");
-    mesh->allocs.push_back(p);
-    mesh->dt_block_min = static_cast<double*>(p);
+  u->in_scratch = no_alloc;
+  if (no_alloc) {
+    if (u->fusable && u->n_blocks > no_alloc_max_blocks)
+      return fail(ctx, HX_ERR_INVALID, "mesh too large for hx_update_no_alloc: " + std::to_string(u->n_blocks) +
+                                           " Euler Local workgroups, at most " + std::to_string(no_alloc_max_blocks));
+    u->block_min = ctx->scratch + scratch_minima_offset;
+    u->scalar = ctx->scratch;
+  } else {
+    if (u->fusable && !mesh->dt_block_min) {
+      mesh->n_dt_blocks = u->n_blocks;
+      void* p = nullptr;
+      HX_TRY(ctx, hipMalloc(&p, std::max<long long>(mesh->n_dt_blocks, 1) * sizeof(double)), "hipMalloc(time step minima)");
+      mesh->allocs.push_back(p);
+      mesh->dt_block_min = static_cast<double*>(p);
+    }
+    u->block_min = mesh->dt_block_min;
+    u->scalar = ctx->scalar;
   }
-  u->block_min = mesh->dt_block_min;
-  u->scalar = ctx->scalar;
   u->n_cheby = o->n_cheby_flow;
   const double max_cheby = cheby_step(u->n_cheby, u->n_cheby - 1);
   u->msc = o->max_safety / max_cheby;
@@ -1342,7 +1410,8 @@ bool fused_pending(hx_mesh* mesh, const hx_update_options* o, const Update_setup
 {
   const auto& c = mesh->dt_cache;
   return u.fusable && c.valid && c.gen == mesh->gen && c.msc == u.msc && c.msd == u.msd && c.use_ldg == o->use_ldg &&
-         same_transport(c.visc, o->visc) && same_transport(c.cond, o->cond);
+         same_transport(c.visc, o->visc) && same_transport(c.cond, o->cond) && c.in_scratch == u.in_scratch &&
+         (!u.in_scratch || mesh->scratch_epoch == scratch_epoch(mesh->ctx->device));
 }
 
 void remember_fused(hx_mesh* mesh, const hx_update_options* o, const Update_setup& u)
@@ -1355,6 +1424,7 @@ void remember_fused(hx_mesh* mesh, const hx_update_options* o, const Update_setu
   c.use_ldg = o->use_ldg;
   c.visc = o->visc;
   c.cond = o->cond;
+  c.in_scratch = u.in_scratch;
 }
 
 // Max_dt of both classes into the context's reduction scalar, without reading it back (kernels_max_dt.cpp:8-17)
@@ -1549,6 +1619,40 @@ hx_status hx_update_async(hx_mesh* mesh, const hx_update_options* o, int n_updat
     ++mesh->gen;
     pending = u.fusable;
   }
+  if (u.fusable && pending) remember_fused(mesh, o, u);
+  return HX_OK;
+}
+
+hx_status hx_update_no_alloc(hx_mesh* mesh, const hx_update_options* o, hx_update_status* st)
+{
+  if (!live(mesh) || !o || !st) return HX_ERR_INVALID;
+  hx_ctx* ctx = mesh->ctx;
+  if (!ctx->scratch)
+    return fail(ctx, HX_ERR_HIP, std::string("the scratch array has no device address: ") + hipGetErrorString(ctx->scratch_err));
+  if (o->fix_admis)
+    return fail(ctx, HX_ERR_UNSUPPORTED, "the admissibility fix allocates its flags; use hx_update");
+  Update_setup u;
+  if (hx_status s = update_setup(mesh, o, &u, true)) return s;
+  bool pending = fused_pending(mesh, o, u);
+  mesh->scratch_epoch = ++scratch_epochs[ctx->device];
+  mesh->dt_cache.valid = false;
+  // the kernels' reduction scalar is the array's, and no event is created for timing
+  struct Scoped
+  {
+    hx_mesh* mesh;
+    bool events;
+    Scoped(hx_mesh* m) : mesh{m}, events{m->ctx->events_off}
+    {
+      mesh->scalar = mesh->ctx->scratch;
+      mesh->ctx->events_off = true;
+    }
+    ~Scoped()
+    {
+      mesh->scalar = nullptr;
+      mesh->ctx->events_off = events;
+    }
+  } scoped(mesh);
+  if (hx_status s = update_once(mesh, o, u, st, pending)) return s;
   if (u.fusable && pending) remember_fused(mesh, o, u);
   return HX_OK;
 }

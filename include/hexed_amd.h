@@ -366,6 +366,15 @@ hx_status hx_update(hx_mesh* mesh, const hx_update_options* opts, hx_update_stat
  * No counterpart of the stage sequence changes; fix_admis (whose verdict is read on the host) and batch meshes are
  * rejected. */
 hx_status hx_update_async(hx_mesh* mesh, const hx_update_options* opts, int n_updates, const hx_update_status* from);
+/* No counterpart in the reference: hx_update for callers that cannot allocate device memory (code under graph capture,
+ * serving loops, harnesses that forbid hipMalloc). The same kernels and results as hx_update, with its scratch -- the
+ * time step's reduction scalar and the fused Max_dt minimum of every Euler Local workgroup -- in a file-scope device
+ * array of 32 + 262,144 doubles (2 MiB per GPU, held until the process ends) instead of memory the library allocates.
+ * Meshes with more than 262,144 Euler Local workgroups (elements / max(1, 256 / row_size^n_dim), both classes) are
+ * rejected with HX_ERR_INVALID; fix_admis is rejected with HX_ERR_UNSUPPORTED (the fix allocates its flags). Every
+ * call on one GPU shares the array: the fused Max_dt minima a call leaves are used by the mesh's next call only if no
+ * other call used the array in between, and calls that overlap on one GPU race. No timing events are recorded. */
+hx_status hx_update_no_alloc(hx_mesh* mesh, const hx_update_options* opts, hx_update_status* status);
 /* Waits for the mesh's stream and writes the device status of hx_update_async into `status` (time_step, flow_time,
  * iteration; fix_admis_iters is left as it is). */
 hx_status hx_update_status_sync(hx_mesh* mesh, hx_update_status* status);
