@@ -309,6 +309,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps for the CPU baseline (0 to skip)")
     ap.add_argument("--err-steps", type=int, default=10, help="steps after which FP32-MFMA is compared with FP64")
     ap.add_argument("--av", type=int, default=1, help="ns48: also time the artificial-viscosity kernels (0 to skip)")
+    ap.add_argument("--face-fold", type=int, default=-1, choices=(-1, 0, 1),
+                    help="fold the face fluxes into the Local: 1 always, 0 never, -1 where 2 n_dim <= row_size")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -336,6 +338,7 @@ def main():
     groups = []
     for m, nc, _ in hosts:
         dev = Device_mesh(ctx, m)
+        dev.set_face_fold(args.face_fold)
         if nc is not None:
             dev.set_cases(nc, m.case_id)
         groups.append((m, dev, nc, None))
@@ -461,7 +464,8 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32-mfma+f64" if args.precision == "fp32_mfma" else "f64", "data": "synthetic",
         "config": {"workload": wl["desc"], "name": args.config, "n_elem": n_elem, "dof": n_dof,
-                   "row_size": sorted({m.row_size for m, _, _ in hosts}), "precision": args.precision},
+                   "row_size": sorted({m.row_size for m, _, _ in hosts}), "precision": args.precision,
+                   "face_fold": args.face_fold},
         "forms": {
             args.precision: {"value": round(value, 1), "ms_per_step": round(1e3 * elapsed / args.steps, 4),
                              "eager": {"value": round(n_dof * 2 * args.steps / chosen["eager"], 1),

@@ -195,6 +195,7 @@ def load_library(path=LIB_PATH):
         "hx_mesh_device_ptr": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(P),
                                               ctypes.POINTER(ctypes.c_size_t)]),
         "hx_mesh_mark_changed": (ctypes.c_int, [P]),
+        "hx_mesh_set_face_fold": (ctypes.c_int, [P, ctypes.c_int]),
         "hx_write_face": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(hx_pde)]),
         "hx_neighbor": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(hx_pde), ctypes.c_int]),
         "hx_local": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(hx_pde), ctypes.c_double, ctypes.c_int,
@@ -269,7 +270,7 @@ EXPORTED_SYMBOLS = [
     "hx_ctx_create", "hx_ctx_destroy", "hx_last_error", "hx_ctx_synchronize", "hx_ctx_stream", "hx_kernels_built",
     "hx_ctx_set_timing", "hx_ctx_timing_reset", "hx_ctx_timing", "hx_ctx_timing_list", "hx_stream_copy",
     "hx_basis_table", "hx_basis_max_cfl", "hx_basis_step_ratio", "hx_mesh_create", "hx_mesh_destroy", "hx_mesh_validate",
-    "hx_mesh_upload", "hx_mesh_download", "hx_mesh_set_precision", "hx_mesh_device_ptr", "hx_mesh_mark_changed",
+    "hx_mesh_upload", "hx_mesh_download", "hx_mesh_set_precision", "hx_mesh_device_ptr", "hx_mesh_mark_changed", "hx_mesh_set_face_fold",
     "hx_write_face",
     "hx_neighbor", "hx_local", "hx_neighbor_reconcile", "hx_reconcile_ldg_flux", "hx_max_dt", "hx_prolong_refined",
     "hx_restrict_refined", "hx_apply_state_bcs", "hx_apply_flux_bcs", "hx_compute_euler", "hx_compute_navier_stokes",

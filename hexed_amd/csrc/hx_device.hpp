@@ -51,7 +51,25 @@ struct Dev_mesh
   int n_bc;
   hx_bc* bcs;
   double* bc_cache;  // [n_bc_con][nv][nfq] boundary-face state cache
+  // Face-flux folding (the update path): each element computes its faces' numerical fluxes itself from its own and its
+  // neighbour's extrapolated states. partner[cls][e * 2nd + face] = {neighbour slot, info, connection normal slot, 0}
+  // with info = packed direction | side << 6 | connection class << 7 | boundary << 8, or neighbour slot -1 where the
+  // flux is computed before the Local (faces of hanging-face mortars, coarse hanging faces). The connections that
+  // touch a mortar are listed per class in mortar_con. The state blocks of the faces are double-buffered: block 0 of a
+  // slot is read from faces (alt_rd 0) or faces_alt [n_slot][nv][nfq] (alt_rd 1) and written to the one alt_wr names;
+  // the LDG blocks stay in faces. alt_rd = alt_wr = 0 everywhere else.
+  int* partner[2];
+  int n_mortar_con[2];
+  int* mortar_con[2];
+  double* faces_alt;
+  int alt_rd, alt_wr;
 };
+
+// the state (block 0) of face slot `slot` in the buffer `alt` selects
+__device__ inline double* state_ptr(const Dev_mesh& m, int slot, int alt)
+{
+  return alt ? m.faces_alt + size_t(slot) * m.nv * m.nfq : m.faces + size_t(slot) * m.face_sz;
+}
 
 // Operator tables of one row size, passed by value as a kernel argument.
 template <int RS>
@@ -155,6 +173,27 @@ __device__ inline int face1_index(const Dir& dir, int i)
     return dir.flip_tangential() ? RS - 1 - i : i;
   } else {
     return i;
+  }
+}
+
+// the face-0 point whose value face1_index moves to face-1 storage position j (the inverse permutation: the transpose
+// and the flip undone in reverse order)
+template <int ND, int RS>
+__device__ inline int face0_index(const Dir& dir, int j)
+{
+  if constexpr (ND == 3) {
+    const bool tr = dir.transpose();
+    int c = tr ? j % RS : j / RS, r = tr ? j / RS : j % RS;
+    if (dir.flip_tangential()) {
+      bool colwise = (dir.i_dim0 > 3 - dir.i_dim0 - dir.i_dim1) != tr;
+      if (colwise) r = RS - 1 - r;
+      else c = RS - 1 - c;
+    }
+    return c * RS + r;
+  } else if constexpr (ND == 2) {
+    return dir.flip_tangential() ? RS - 1 - j : j;
+  } else {
+    return j;
   }
 }
 

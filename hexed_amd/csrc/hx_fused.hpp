@@ -1,6 +1,7 @@
 // Fused launches of an Euler stage over a group of meshes (hx_group): one launch of the state boundary conditions, one
 // of Neighbor and one of Local cover every mesh of the group, both element classes and every row size of the set RSS,
-// where the per-mesh path launches each kernel once per mesh and class.
+// where the per-mesh path launches each kernel once per mesh and class. The stages fold the face fluxes into the Local
+// (face_flux), as the per-mesh batch stage does, so Neighbor runs only on connections touching a hanging-face mortar.
 //
 // Work is cut into tiles -- one workgroup's worth of the per-class kernel (256 face items, or the EPB elements of an
 // Euler Local workgroup) -- listed per (mesh, class) segment in a plan. A workgroup finds its tile's segment, and runs
@@ -93,6 +94,7 @@ __global__ __launch_bounds__(256) void k_fused_state_bcs(const Fused_member* __r
   });
 }
 
+// with face-flux folding only the connections touching a hanging-face mortar (Dev_mesh::mortar_con) run Neighbor
 template <int ND, int... RSS>
 __global__ __launch_bounds__(256) void k_fused_neighbor(const Fused_member* __restrict__ mem,
                                                        const Fused_plan* __restrict__ plan, int order, int chunk)
@@ -101,11 +103,14 @@ __global__ __launch_bounds__(256) void k_fused_neighbor(const Fused_member* __re
     const int s = fused_segment(plan, tile);
     const Dev_mesh& m = mem[plan->member[s]].m;
     const int gid = (tile - plan->first[s]) * 256 + int(threadIdx.x);
-    const bool def = plan->cls[s] == HX_DEFORMED;
+    const int cls = plan->cls[s];
     dispatch_rs<RSS...>(plan->rs[s], [&](auto rs_c) {
       constexpr int RS = decltype(rs_c)::value;
-      if (def) neighbor_at<ND, RS, true, false>(m, gid);
-      else neighbor_at<ND, RS, false, false>(m, gid);
+      constexpr int NFQ = Shape<ND, RS>::NFQ;
+      const int i = gid / NFQ, f = gid % NFQ;
+      if (i >= m.n_mortar_con[cls]) return;
+      if (cls == HX_DEFORMED) neighbor_con<ND, RS, true, false>(m, m.mortar_con[cls][i], f);
+      else neighbor_con<ND, RS, false, false>(m, m.mortar_con[cls][i], f);
     });
   });
 }
@@ -117,16 +122,16 @@ constexpr int fused_local_lds()
   int b = 0;
   for (int x : {euler_lds_bytes<ND, RSS, false, P>()...}) b = x > b ? x : b;
   if constexpr (WITH_DEF)
-    for (int x : {euler_lds_bytes<ND, RSS, true, P>()...}) b = x > b ? x : b;
+    for (int x : {euler_lds_bytes<ND, RSS, true, euler_form<P, true>>()...}) b = x > b ? x : b;
   return b;
 }
 template <int ND, class P, bool WITH_DEF, int... RSS>
 constexpr int fused_local_waves()
 {
   int w = 8;
-  for (int x : {euler_waves<ND, RSS, false, P>()...}) w = x < w ? x : w;
+  for (int x : {euler_waves<ND, RSS, false, P, true>()...}) w = x < w ? x : w;
   if constexpr (WITH_DEF)
-    for (int x : {euler_waves<ND, RSS, true, P>()...}) w = x < w ? x : w;
+    for (int x : {euler_waves<ND, RSS, true, euler_form<P, true>, true>()...}) w = x < w ? x : w;
   return w;
 }
 
@@ -150,18 +155,20 @@ void k_fused_local_euler(const Fused_member* __restrict__ mem, const Fused_plan*
       const Ops<RS>* gops = static_cast<const Ops<RS>*>(mb.ops);
       if (def) {
         if constexpr (WITH_DEF) {
-          constexpr int tb = Shape<ND, RS>::EPB * int(sizeof(T)) * Euler_lds<ND, RS, true, P>::PER_T;
-          T* lt = reinterpret_cast<T*>(fused_lds);
+          using PD = euler_form<P, true>;
+          using TD = typename PD::T;
+          constexpr int tb = Shape<ND, RS>::EPB * int(sizeof(TD)) * Euler_lds<ND, RS, true, PD>::PER_T;
+          TD* lt = reinterpret_cast<TD*>(fused_lds);
           double* ld = fused_lds + (tb + 7) / 8;
-          local_euler_tile<ND, RS, true, P>(mb.m, blk, lt, ld, o, od, coef, dtp, stage, compute_residual, use_filter,
-                                            mb.fuse, gops);
+          local_euler_tile<ND, RS, true, PD, true>(mb.m, blk, lt, ld, o, od, coef, dtp, stage, compute_residual,
+                                                   use_filter, mb.fuse, gops);
         }
       } else {
         constexpr int tb = Shape<ND, RS>::EPB * int(sizeof(T)) * Euler_lds<ND, RS, false, P>::PER_T;
         T* lt = reinterpret_cast<T*>(fused_lds);
         double* ld = fused_lds + (tb + 7) / 8;
-        local_euler_tile<ND, RS, false, P>(mb.m, blk, lt, ld, o, od, coef, dtp, stage, compute_residual, use_filter,
-                                           mb.fuse, gops);
+        local_euler_tile<ND, RS, false, P, true>(mb.m, blk, lt, ld, o, od, coef, dtp, stage, compute_residual,
+                                                 use_filter, mb.fuse, gops);
       }
     });
     // the next tile reuses the LDS
@@ -175,7 +182,7 @@ inline int fused_tiles(int kind, const Dev_mesh& m, int cls)
 {
   using S = Shape<ND, RS>;
   if (kind == fused_state_bcs) return int((size_t(m.n_bc_con) * S::NFQ + 255) / 256);
-  if (kind == fused_neighbor) return int((size_t(m.n_con[cls]) * S::NFQ + 255) / 256);
+  if (kind == fused_neighbor) return int((size_t(m.n_mortar_con[cls]) * S::NFQ + 255) / 256);
   return (m.n_elem[cls] + S::EPB - 1) / S::EPB;
 }
 

@@ -61,7 +61,8 @@ __device__ inline void extrapolate_faces(const Dev_mesh& m, int cls, const Ops<R
     const double* row = src + (el * S::NV + v) * S::NQ + b;
     double acc = 0;
     for (int kk = 0; kk < RS; ++kk) acc += ops.bound[s][kk] * row[kk * sd];
-    slot_ptr(m, m.face_slot[cls][e * 2 * ND + f])[ldg * S::NV * S::NFQ + v * S::NFQ + fq] = acc;
+    const int slot = m.face_slot[cls][e * 2 * ND + f];
+    (ldg ? slot_ptr(m, slot) + S::NV * S::NFQ : state_ptr(m, slot, m.alt_wr))[v * S::NFQ + fq] = acc;
   }
 }
 
@@ -108,6 +109,12 @@ struct Fp32_mfma_form
   using T = float;
   static constexpr bool mfma = true;
 };
+
+// The arithmetic form of the Euler Local for form P: the FP32-MFMA form runs the FP64 kernel on deformed elements, where
+// its FP64 pressure rows (below) cost more LDS traffic and FP64 work than the matrix cores save (measured on the 48^3
+// deformed mesh: 1.82 against 1.71 ms per launch)
+template <class P, bool DEF>
+using euler_form = std::conditional_t<DEF, Fp64_form, P>;
 
 // Threads per workgroup: one per quadrature point, rounded up to whole wavefronts for the matrix-core form so that
 // every lane of a wavefront takes part in its matrix instructions.
@@ -397,7 +404,12 @@ __device__ inline void load_faces(const Dev_mesh& m, int cls, int blk, int ldg, 
   for (int k = threadIdx.x; k < S::EPB * PF; k += NT) {
     const int el = k / PF, r = k % PF, f = r / (S::NV * S::NFQ), vq = r % (S::NV * S::NFQ);
     const int e = blk * S::EPB + el;
-    dst[el * per + r] = e < n ? slot_ptr(m, m.face_slot[cls][e * 2 * ND + f])[ldg * S::NV * S::NFQ + vq] : 0.;
+    if (e < n) {
+      const int slot = m.face_slot[cls][e * 2 * ND + f];
+      dst[el * per + r] = (ldg ? slot_ptr(m, slot) + S::NV * S::NFQ : state_ptr(m, slot, m.alt_rd))[vq];
+    } else {
+      dst[el * per + r] = 0.;
+    }
   }
 }
 
@@ -416,9 +428,169 @@ __device__ inline void write_faces_from(const Dev_mesh& m, int cls, int blk, con
       double x[RS];
       load_row(buf + el * per + v * S::NQP + base, sd, x);
       const int* fs = m.face_slot[cls] + e * 2 * ND + 2 * i;
-      slot_ptr(m, fs[0])[v * S::NFQ + r] = row_bound(o, 0, x);
-      slot_ptr(m, fs[1])[v * S::NFQ + r] = row_bound(o, 1, x);
+      state_ptr(m, fs[0], m.alt_wr)[v * S::NFQ + r] = row_bound(o, 0, x);
+      state_ptr(m, fs[1], m.alt_wr)[v * S::NFQ + r] = row_bound(o, 1, x);
     });
+  }
+}
+
+// ---- Neighbor (Spatial.hpp:613-704) ----------------------------------------------------------------------
+// the numerical flux at face point f of connection con of class DEF
+template <int ND, int RS, bool DEF, bool VISC>
+__device__ __forceinline__ void neighbor_con(const Dev_mesh& m, int con, int f)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
+  constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
+  const Dir dir(m.con_dir[cls][con]);
+  const int s0 = m.con_slot[cls][2 * con], s1 = m.con_slot[cls][2 * con + 1];
+  double* f0 = state_ptr(m, s0, m.alt_rd);
+  double* f1 = state_ptr(m, s1, m.alt_rd);
+  int p1 = f;
+  double sg0 = 1., sg1 = 1.;
+  double n[ND];
+  if constexpr (DEF) {
+    p1 = face1_index<ND, RS>(dir, f);
+    sg0 = 1 - 2 * int(dir.flip_normal(0));
+    sg1 = 1 - 2 * int(dir.flip_normal(1));
+    const double* nn = m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ;
+    for (int j = 0; j < ND; ++j) n[j] = sg0 * nn[j * NFQ + f];
+  } else {
+    for (int j = 0; j < ND; ++j) n[j] = (j == dir.i_dim0);
+  }
+  double u0[NV], u1[NV];
+  for (int v = 0; v < NV; ++v) {
+    u0[v] = f0[v * NFQ + f];
+    u1[v] = f1[v * NFQ + p1];
+  }
+  if constexpr (VISC) {
+    double* l0 = slot_ptr(m, s0) + NV * NFQ;
+    double* l1 = slot_ptr(m, s1) + NV * NFQ;
+    for (int v = 0; v < NV; ++v) {
+      const double avg = .5 * (u0[v] + u1[v]);
+      l0[v * NFQ + f] = avg;
+      l1[v * NFQ + p1] = avg;
+    }
+  }
+  double F0[NV], F1[NV];
+  flux_conv<ND>(u0, n, F0);
+  flux_conv<ND>(u1, n, F1);
+  double nsq = 0;
+  for (int j = 0; j < ND; ++j) nsq += n[j] * n[j];
+  const double c = fmax(char_speed<ND>(u0), char_speed<ND>(u1)) * sqrt(nsq);
+  for (int v = 0; v < NV; ++v) {
+    const double fl = .5 * (F0[v] + F1[v] + c * (u0[v] - u1[v]));
+    f0[v * NFQ + f] = sg0 * fl;
+    f1[v * NFQ + p1] = sg1 * fl;
+  }
+}
+
+// one thread per (connection, face point) item `gid` of the class's connections
+template <int ND, int RS, bool DEF, bool VISC>
+__device__ __forceinline__ void neighbor_at(const Dev_mesh& m, int gid)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ;
+  const int con = gid / NFQ, f = gid % NFQ;
+  if (con < m.n_con[DEF ? HX_DEFORMED : HX_CARTESIAN]) neighbor_con<ND, RS, DEF, VISC>(m, con, f);
+}
+
+template <int ND, int RS, bool DEF, bool VISC>
+__global__ __launch_bounds__(256) void k_neighbor(Dev_mesh m)
+{
+  neighbor_at<ND, RS, DEF, VISC>(m, blockIdx.x * 256 + threadIdx.x);
+}
+
+// Neighbor over the connections that touch a hanging-face mortar (Dev_mesh::mortar_con), whose fluxes the face-flux
+// folding leaves to a launch before the Local
+template <int ND, int RS, bool DEF, bool VISC>
+__global__ __launch_bounds__(256) void k_neighbor_mortar(Dev_mesh m)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ;
+  constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid / NFQ, f = gid % NFQ;
+  if (i < m.n_mortar_con[cls]) neighbor_con<ND, RS, DEF, VISC>(m, m.mortar_con[cls][i], f);
+}
+
+// Face-flux folding: the numerical flux of face point p of face `face` of element e (class cls), times this side's
+// sign, computed from the two extrapolated states exactly as Neighbor computes it for the connection (Spatial.hpp:
+// 613-704, the operands in the connection's side order), and with VISC the LDG average of the two states. A face whose
+// flux a launch before the Local computed (hanging faces) reads flux and average from its own slot. On a boundary
+// connection the element also writes the ghost side's flux into the ghost slot's block 0 in the primary buffer, where
+// Neighbor leaves it.
+template <int ND, int RS, bool VISC>
+__device__ inline void face_flux(const Dev_mesh& m, int cls, int e, int face, int p, double (&fl)[ND + 2],
+                                 double (&avg)[ND + 2])
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
+  const size_t k = size_t(e) * 2 * ND + face;
+  const int* pi = m.partner[cls] + 4 * k;
+  const int own_slot = m.face_slot[cls][k];
+  const double* own = state_ptr(m, own_slot, m.alt_rd);
+  const int ps = pi[0];
+  if (ps < 0) {
+    for (int v = 0; v < NV; ++v) fl[v] = own[v * NFQ + p];
+    if constexpr (VISC) {
+      const double* l = slot_ptr(m, own_slot) + NV * NFQ;
+      for (int v = 0; v < NV; ++v) avg[v] = l[v * NFQ + p];
+    }
+    return;
+  }
+  const int info = pi[1];
+  const Dir dir(info & 63);
+  const bool side1 = (info >> 6) & 1, def = (info >> 7) & 1, bc = (info >> 8) & 1;
+  const double* par = state_ptr(m, ps, m.alt_rd);
+  int f0 = p, f1 = p;
+  double sg0 = 1., sg1 = 1.;
+  double n[ND];
+  if (def) {
+    if (side1) f0 = face0_index<ND, RS>(dir, p);
+    else f1 = face1_index<ND, RS>(dir, p);
+    sg0 = 1 - 2 * int(dir.flip_normal(0));
+    sg1 = 1 - 2 * int(dir.flip_normal(1));
+    const double* nn = m.normals + size_t(pi[2]) * ND * NFQ;
+    for (int j = 0; j < ND; ++j) n[j] = sg0 * nn[j * NFQ + f0];
+  } else {
+    for (int j = 0; j < ND; ++j) n[j] = (j == dir.i_dim0);
+  }
+  const double* a0 = side1 ? par : own;
+  const double* a1 = side1 ? own : par;
+  double u0[NV], u1[NV];
+  for (int v = 0; v < NV; ++v) {
+    u0[v] = a0[v * NFQ + f0];
+    u1[v] = a1[v * NFQ + f1];
+  }
+  if constexpr (VISC)
+    for (int v = 0; v < NV; ++v) avg[v] = .5 * (u0[v] + u1[v]);
+  double F0[NV], F1[NV];
+  flux_conv<ND>(u0, n, F0);
+  flux_conv<ND>(u1, n, F1);
+  double nsq = 0;
+  for (int j = 0; j < ND; ++j) nsq += n[j] * n[j];
+  const double c = fmax(char_speed<ND>(u0), char_speed<ND>(u1)) * sqrt(nsq);
+  double* ghost = (bc && !side1) ? slot_ptr(m, ps) : nullptr;
+  for (int v = 0; v < NV; ++v) {
+    const double x = .5 * (F0[v] + F1[v] + c * (u0[v] - u1[v]));
+    fl[v] = (side1 ? sg1 : sg0) * x;
+    if (ghost) ghost[v * NFQ + f1] = sg1 * x;
+  }
+}
+
+// The faces' numerical fluxes of the workgroup's elements into dst[EPB][2ND][NV][NFQ] (element stride `per`), one thread
+// per face point (face-flux folding, in place of load_faces)
+template <int ND, int RS, int NT>
+__device__ inline void fold_fluxes(const Dev_mesh& m, int cls, int blk, double* dst, int per)
+{
+  using S = Shape<ND, RS>;
+  constexpr int NFP = 2 * ND * S::NFQ, NV = S::NV, NFQ = S::NFQ;
+  const int n = m.n_elem[cls];
+  for (int k = threadIdx.x; k < S::EPB * NFP; k += NT) {
+    const int el = k / NFP, r = k % NFP, face = r / NFQ, p = r % NFQ;
+    const int e = blk * S::EPB + el;
+    double fl[NV], avg[NV];
+    if (e < n) face_flux<ND, RS, false>(m, cls, e, face, p, fl, avg);
+    else
+      for (int v = 0; v < NV; ++v) fl[v] = 0.;
+    for (int v = 0; v < NV; ++v) dst[el * per + (face * NV + v) * NFQ + p] = fl[v];
   }
 }
 
@@ -570,14 +742,17 @@ constexpr int euler_block()
 }
 
 // Waves per SIMD that the workgroup's LDS allows (160 KB per CU, 4 SIMDs), at most 5: the register target given to
-// the compiler, so that registers do not lower the occupancy below what LDS permits
-template <int ND, int RS, bool DEF, class P>
+// the compiler, so that registers do not lower the occupancy below what LDS permits. With face-flux folding, where a
+// thread computes more than one face point's flux, at most 4 (the registers of two face points' states and fluxes
+// would spill at 5).
+template <int ND, int RS, bool DEF, class P, bool FOLD = false>
 constexpr int euler_waves()
 {
   using L = Euler_lds<ND, RS, DEF, P>;
   constexpr int lds = Shape<ND, RS>::EPB * (int(sizeof(typename P::T)) * L::PER_T + 8 * L::PER_D);
   constexpr int w = (163840 / lds) * (euler_block<ND, RS>() / 64) / 4;
-  return w < 1 ? 1 : (w > 5 ? 5 : w);
+  constexpr int cap = FOLD && Shape<ND, RS>::EPB * 2 * ND * Shape<ND, RS>::NFQ > euler_block<ND, RS>() ? 4 : 5;
+  return w < 1 ? 1 : (w > cap ? cap : w);
 }
 
 // LDS bytes of one Euler Local workgroup: the arrays of T, then (8-byte aligned) the arrays of double
@@ -592,7 +767,8 @@ constexpr int euler_lds_bytes()
 // One workgroup's worth of the Euler Local: the EPB elements of tile `blk` of class DEF, with the LDS arrays lt (T) and
 // ld (double) of Euler_lds. Shared by k_local_euler (one tile per workgroup) and the fused launch (tiles of several
 // meshes, classes and row sizes per kernel).
-template <int ND, int RS, bool DEF, class P>
+// FOLD: the faces' numerical fluxes are computed here from both sides' states (face_flux) instead of read from the slots
+template <int ND, int RS, bool DEF, class P, bool FOLD = false>
 __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typename P::T* lt, double* ld,
                                                  const Eops<RS, double>& o, const Eops<RS, double>* od, double coef,
                                                  const double* __restrict__ dtp, int stage, int compute_residual,
@@ -619,7 +795,8 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   // are not held in registers across the row phases
   const double* mt = m.metrics + (DEF ? size_t(e) * (ND * ND + 1) * NQ + q : 0);
   const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
-  load_faces<ND, RS, NT>(m, cls, blk, 0, ld + L::SB, L::PER_D);
+  if constexpr (FOLD) fold_fluxes<ND, RS, NT>(m, cls, blk, ld + L::SB, L::PER_D);
+  else load_faces<ND, RS, NT>(m, cls, blk, 0, ld + L::SB, L::PER_D);
   for (int i = 0; i < ND; ++i) {
     // volume flux along reference direction i (Spatial.hpp:404-419), in FP64
     double nrm[ND];
@@ -704,8 +881,8 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   }
 }
 
-template <int ND, int RS, bool DEF, class P>
-__global__ __launch_bounds__((euler_block<ND, RS>())) __attribute__((amdgpu_waves_per_eu(euler_waves<ND, RS, DEF, P>())))
+template <int ND, int RS, bool DEF, class P, bool FOLD>
+__global__ __launch_bounds__((euler_block<ND, RS>())) __attribute__((amdgpu_waves_per_eu(euler_waves<ND, RS, DEF, P, FOLD>())))
 void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, double coef, const double* __restrict__ dtp,
                    int stage, int compute_residual, int use_filter, Dt_fuse fuse, const Ops<RS>* __restrict__ gops)
 {
@@ -713,7 +890,8 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
   constexpr int EPB = Shape<ND, RS>::EPB;
   __shared__ typename P::T lt[EPB * L::PER_T];
   __shared__ double ld[EPB * L::PER_D];
-  local_euler_tile<ND, RS, DEF, P>(m, blockIdx.x, lt, ld, o, od, coef, dtp, stage, compute_residual, use_filter, fuse, gops);
+  local_euler_tile<ND, RS, DEF, P, FOLD>(m, blockIdx.x, lt, ld, o, od, coef, dtp, stage, compute_residual, use_filter,
+                                         fuse, gops);
 }
 
 // ---- Local, Navier-Stokes stage 0 (Spatial.hpp:326-509 with Navier_stokes<true>) -------------------------
@@ -760,7 +938,8 @@ constexpr int ns_waves()
   return w < 1 ? 1 : (w > cap ? cap : w);
 }
 
-template <int ND, int RS, bool DEF, class P>
+// FOLD: the faces' numerical fluxes and LDG averages are computed here from both sides' states (face_flux)
+template <int ND, int RS, bool DEF, class P, bool FOLD>
 __global__ __launch_bounds__((form_block<ND, RS, P>())) __attribute__((amdgpu_waves_per_eu(ns_waves<ND, RS, DEF, P>())))
 void k_local_ns(Dev_mesh m, Eops<RS, double> o,
                 const Eops<RS, double>* od, Ns_params ns, double coef, const double* __restrict__ dtp,
@@ -784,13 +963,34 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
 
   // the numerical face fluxes are needed only after the gradient; their loads are issued now and held in
   // registers so that their latency hides behind the gradient phase
-  constexpr int PF = 2 * ND * NV * NFQ, NSB = (EPB * PF + NT - 1) / NT;
+  constexpr int PF = 2 * ND * NV * NFQ, NSB = FOLD ? 1 : (EPB * PF + NT - 1) / NT;
   double sb[NSB];
+  if constexpr (!FOLD) {
 #pragma unroll
-  for (int k = 0; k < NSB; ++k) {
-    const int kk = t + k * NT, el2 = kk / PF, r = kk % PF, f = r / (NV * NFQ), vq = r % (NV * NFQ);
-    const int e2 = blockIdx.x * EPB + el2;
-    sb[k] = (kk < EPB * PF && e2 < n) ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[vq] : 0.;
+    for (int k = 0; k < NSB; ++k) {
+      const int kk = t + k * NT, el2 = kk / PF, r = kk % PF, f = r / (NV * NFQ), vq = r % (NV * NFQ);
+      const int e2 = blockIdx.x * EPB + el2;
+      sb[k] = (kk < EPB * PF && e2 < n) ? state_ptr(m, m.face_slot[cls][e2 * 2 * ND + f], m.alt_rd)[vq] : 0.;
+    }
+  }
+  // folded: one thread per face point computes its flux (held in registers until the convective pass, whose LDS it
+  // shares with the gradient phase) and its LDG average (into LDS now)
+  constexpr int NFP = 2 * ND * NFQ, FPT = FOLD ? (EPB * NFP + NT - 1) / NT : 1;
+  double sbf[FPT][NV];
+  if constexpr (FOLD) {
+#pragma unroll
+    for (int k = 0; k < FPT; ++k) {
+      const int kk = t + k * NT, el2 = kk / NFP, r = kk % NFP, face = r / NFQ, p = r % NFQ;
+      const int e2 = blockIdx.x * EPB + el2;
+      double avg[NV];
+      if (kk < EPB * NFP && e2 < n) {
+        face_flux<ND, RS, true>(m, cls, e2, face, p, sbf[k], avg);
+      } else {
+        for (int v = 0; v < NV; ++v) sbf[k][v] = avg[v] = 0.;
+      }
+      if (kk < EPB * NFP)
+        for (int v = 0; v < NV; ++v) ld[el2 * L::PER_D + L::SL + (face * NV + v) * NFQ + p] = avg[v];
+    }
   }
 
   double u[NV];
@@ -819,7 +1019,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
     if constexpr (DEF) return metd[i][j];
     else return double(i == j);
   };
-  load_faces<ND, RS, NT>(m, cls, blockIdx.x, 1, ld + L::SL, L::PER_D);
+  if constexpr (!FOLD) load_faces<ND, RS, NT>(m, cls, blockIdx.x, 1, ld + L::SL, L::PER_D);
   if constexpr (DEF) {
     for (int k = t; k < EPB * 2 * ND * ND * NFQ; k += NT) {
       const int el2 = k / (2 * ND * ND * NFQ), r = k % (2 * ND * ND * NFQ), f = r / (ND * NFQ), jq = r % (ND * NFQ);
@@ -924,12 +1124,21 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
     for (int v = 0; v < NV; ++v) ut[v] = T(u[v]);
     flux_visc<ND>(ns, ut, bulk, lap, grad, fvis);
   }
+  if constexpr (FOLD) {
 #pragma unroll
-  for (int k = 0; k < NSB; ++k)
-    if (t + k * NT < EPB * PF) {
-      const int kk = t + k * NT;
-      ld[(kk / PF) * L::PER_D + L::SB + kk % PF] = sb[k];
+    for (int k = 0; k < FPT; ++k) {
+      const int kk = t + k * NT, el2 = kk / NFP, r = kk % NFP, face = r / NFQ, p = r % NFQ;
+      if (kk < EPB * NFP)
+        for (int v = 0; v < NV; ++v) ld[el2 * L::PER_D + L::SB + (face * NV + v) * NFQ + p] = sbf[k][v];
     }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NSB; ++k)
+      if (t + k * NT < EPB * PF) {
+        const int kk = t + k * NT;
+        ld[(kk / PF) * L::PER_D + L::SB + kk % PF] = sb[k];
+      }
+  }
   // residual along each direction (Spatial.hpp:444-470). The viscous flux goes first (it is also extrapolated to the
   // LDG blocks of the faces), then the convective flux with its numerical face fluxes. The two rates accumulate in
   // separate arrays, each over the directions in order, so the order of the two passes leaves the result unchanged;
@@ -1104,61 +1313,6 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh 
   for (int v = 0; v < NV; ++v) su[el][v][q] = e < m.n_elem[cls] ? st[v * NQ + q] : 0.;
   __syncthreads();
   extrapolate_faces<ND, RS>(m, cls, ops, &su[0][0][0], 0);
-}
-
-// ---- Neighbor (Spatial.hpp:613-704) ----------------------------------------------------------------------
-// one thread per (connection, face point) item `gid` of the class's connections
-template <int ND, int RS, bool DEF, bool VISC>
-__device__ __forceinline__ void neighbor_at(const Dev_mesh& m, int gid)
-{
-  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
-  constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
-  const int con = gid / NFQ, f = gid % NFQ;
-  if (con >= m.n_con[cls]) return;
-  const Dir dir(m.con_dir[cls][con]);
-  double* f0 = slot_ptr(m, m.con_slot[cls][2 * con]);
-  double* f1 = slot_ptr(m, m.con_slot[cls][2 * con + 1]);
-  int p1 = f;
-  double sg0 = 1., sg1 = 1.;
-  double n[ND];
-  if constexpr (DEF) {
-    p1 = face1_index<ND, RS>(dir, f);
-    sg0 = 1 - 2 * int(dir.flip_normal(0));
-    sg1 = 1 - 2 * int(dir.flip_normal(1));
-    const double* nn = m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ;
-    for (int j = 0; j < ND; ++j) n[j] = sg0 * nn[j * NFQ + f];
-  } else {
-    for (int j = 0; j < ND; ++j) n[j] = (j == dir.i_dim0);
-  }
-  double u0[NV], u1[NV];
-  for (int v = 0; v < NV; ++v) {
-    u0[v] = f0[v * NFQ + f];
-    u1[v] = f1[v * NFQ + p1];
-  }
-  if constexpr (VISC) {
-    for (int v = 0; v < NV; ++v) {
-      const double avg = .5 * (u0[v] + u1[v]);
-      f0[NV * NFQ + v * NFQ + f] = avg;
-      f1[NV * NFQ + v * NFQ + p1] = avg;
-    }
-  }
-  double F0[NV], F1[NV];
-  flux_conv<ND>(u0, n, F0);
-  flux_conv<ND>(u1, n, F1);
-  double nsq = 0;
-  for (int j = 0; j < ND; ++j) nsq += n[j] * n[j];
-  const double c = fmax(char_speed<ND>(u0), char_speed<ND>(u1)) * sqrt(nsq);
-  for (int v = 0; v < NV; ++v) {
-    const double fl = .5 * (F0[v] + F1[v] + c * (u0[v] - u1[v]));
-    f0[v * NFQ + f] = sg0 * fl;
-    f1[v * NFQ + p1] = sg1 * fl;
-  }
-}
-
-template <int ND, int RS, bool DEF, bool VISC>
-__global__ __launch_bounds__(256) void k_neighbor(Dev_mesh m)
-{
-  neighbor_at<ND, RS, DEF, VISC>(m, blockIdx.x * 256 + threadIdx.x);
 }
 
 // ---- Neighbor_reconcile (Spatial.hpp:716-759) ------------------------------------------------------------
@@ -1394,8 +1548,9 @@ __global__ __launch_bounds__(256) void k_prolong(Dev_mesh m, const Ops<RS>* __re
   const size_t gi = size_t(blockIdx.x) * G + g;
   const bool active = g < G && gi < size_t(m.n_ref) * n_var;
   const int i_ref = active ? int(gi / n_var) : 0, v = active ? int(gi % n_var) : 0;
-  const int off = (offset * (ND + 2) + v) * NFQ;
-  if (active) stage[g][p] = slot_ptr(m, m.ref_slot[5 * i_ref])[off + p];
+  // block 0 of the slots in the buffer being written (the prolongation follows the Local), or the LDG block
+  auto at = [&](int slot) { return offset ? slot_ptr(m, slot) + (ND + 2 + v) * NFQ : state_ptr(m, slot, m.alt_wr) + v * NFQ; };
+  if (active) stage[g][p] = at(m.ref_slot[5 * i_ref])[p];
   __syncthreads();
   if (!active) return;
   Face_rows<RS, false> fr;
@@ -1403,16 +1558,16 @@ __global__ __launch_bounds__(256) void k_prolong(Dev_mesh m, const Ops<RS>* __re
   const int a = ND > 2 ? p / RS : p, b = ND > 2 ? p % RS : 0;
   face_rows<ND, RS, false>(fr, ops->prolong, a, b, scale);
   if constexpr (ND == 1) {
-    slot_ptr(m, m.ref_slot[5 * i_ref + 1])[off + p] = stage[g][p];
+    at(m.ref_slot[5 * i_ref + 1])[p] = stage[g][p];
   } else if constexpr (ND == 2) {
-    for (int f = 0; f < nf; ++f) slot_ptr(m, m.ref_slot[5 * i_ref + 1 + f])[off + p] = fr.line(stage[g], a, half_of<ND>(f, 0, fr.str));
+    for (int f = 0; f < nf; ++f) at(m.ref_slot[5 * i_ref + 1 + f])[p] = fr.line(stage[g], a, half_of<ND>(f, 0, fr.str));
   } else {
     double row_0[RS], row_1[RS];
     fr.dim0(stage[g], a, false, row_0);
     fr.dim0(stage[g], a, !fr.str[0], row_1);
     for (int f = 0; f < nf; ++f) {
       const bool h0 = half_of<ND>(f, 0, fr.str), h1 = half_of<ND>(f, 1, fr.str);
-      slot_ptr(m, m.ref_slot[5 * i_ref + 1 + f])[off + p] = fr.dim1(row_0, row_1, h0, b, h1);
+      at(m.ref_slot[5 * i_ref + 1 + f])[p] = fr.dim1(row_0, row_1, h0, b, h1);
     }
   }
 }
@@ -1434,8 +1589,9 @@ __global__ __launch_bounds__(256) void k_restrict(Dev_mesh m, const Ops<RS>* __r
   Face_rows<RS, true> fr;
   fr.str[0] = fr.str[1] = false;
   const int nf = active ? fine_faces<ND>(m, i_ref, fr.str) : 0;
-  const int off = (offset * (ND + 2) + v) * NFQ;
-  for (int f = 0; f < nf; ++f) stage[g][f][p] = slot_ptr(m, m.ref_slot[5 * i_ref + 1 + f])[off + p];
+  // block 0 of the slots in the buffer being read (the restriction precedes the Local), or the LDG block
+  auto at = [&](int slot) { return offset ? slot_ptr(m, slot) + (ND + 2 + v) * NFQ : state_ptr(m, slot, m.alt_rd) + v * NFQ; };
+  for (int f = 0; f < nf; ++f) stage[g][f][p] = at(m.ref_slot[5 * i_ref + 1 + f])[p];
   __syncthreads();
   if (!active) return;
   const int a = ND > 2 ? p / RS : p, b = ND > 2 ? p % RS : 0;
@@ -1452,10 +1608,10 @@ __global__ __launch_bounds__(256) void k_restrict(Dev_mesh m, const Ops<RS>* __r
       fr.dim0(stage[g][f], a, half_of<ND>(f, 0, fr.str), row);
       r = fr.dim1(row, row, false, b, half_of<ND>(f, 1, fr.str));
     }
-    slot_ptr(m, m.ref_slot[5 * i_ref + 1 + f])[off + p] = r;
+    at(m.ref_slot[5 * i_ref + 1 + f])[p] = r;
     acc += r;
   }
-  slot_ptr(m, m.ref_slot[5 * i_ref])[off + p] = acc;
+  at(m.ref_slot[5 * i_ref])[p] = acc;
 }
 
 // ---- flow boundary conditions (Boundary_condition.cpp) ---------------------------------------------------
@@ -1481,16 +1637,18 @@ __device__ __forceinline__ void state_bcs_at(const Dev_mesh& m, int gid)
   if (ib >= m.n_bc_con) return;
   const int con = m.bc_con[ib];
   const hx_bc bc = m.bcs[m.bc_serial[ib]];
-  const double* in_f = slot_ptr(m, m.con_slot[HX_DEFORMED][2 * con]);
-  double* gh_f = slot_ptr(m, m.con_slot[HX_DEFORMED][2 * con + 1]);
+  const int s_in = m.con_slot[HX_DEFORMED][2 * con], s_gh = m.con_slot[HX_DEFORMED][2 * con + 1];
+  // the states (block 0) in the buffer being read
+  const double* in_f = state_ptr(m, s_in, m.alt_rd);
+  double* gh_f = state_ptr(m, s_gh, m.alt_rd);
   switch (bc.kind) {
     case HX_BC_FREESTREAM:
       for (int v = 0; v < NV; ++v) gh_f[v * NFQ + f] = bc.state[v];
       break;
     case HX_BC_COPY:
     case HX_BC_OUTFLOW:
-      for (int l = 0; l < 2; ++l)
-        for (int v = 0; v < NV; ++v) gh_f[(l * NV + v) * NFQ + f] = in_f[(l * NV + v) * NFQ + f];
+      for (int v = 0; v < NV; ++v) gh_f[v * NFQ + f] = in_f[v * NFQ + f];
+      for (int v = 0; v < NV; ++v) slot_ptr(m, s_gh)[(NV + v) * NFQ + f] = slot_ptr(m, s_in)[(NV + v) * NFQ + f];
       break;
     case HX_BC_NONPENETRATION:
       for (int v = 0; v < NV; ++v) gh_f[v * NFQ + f] = in_f[v * NFQ + f];
@@ -1647,6 +1805,20 @@ struct Launchers
   }
   static hipError_t neighbor(const Launch& l, int cls, bool visc)
   {
+    if (l.fold) {
+      // face-flux folding: only the connections touching a hanging-face mortar
+      const int n = l.m.n_mortar_con[cls];
+      if (n == 0) return hipSuccess;
+      dim3 g(blocks_for(size_t(n) * S::NFQ, 256)), b(256);
+      if (cls == HX_DEFORMED) {
+        if (visc) hipLaunchKernelGGL((k_neighbor_mortar<ND, RS, true, true>), g, b, 0, l.stream, l.m);
+        else hipLaunchKernelGGL((k_neighbor_mortar<ND, RS, true, false>), g, b, 0, l.stream, l.m);
+      } else {
+        if (visc) hipLaunchKernelGGL((k_neighbor_mortar<ND, RS, false, true>), g, b, 0, l.stream, l.m);
+        else hipLaunchKernelGGL((k_neighbor_mortar<ND, RS, false, false>), g, b, 0, l.stream, l.m);
+      }
+      return hipGetLastError();
+    }
     const int n = l.m.n_con[cls];
     if (n == 0) return hipSuccess;
     dim3 g(blocks_for(size_t(n) * S::NFQ, 256)), b(256);
@@ -1667,18 +1839,27 @@ struct Launchers
     dim3 g(blocks_for(n, S::EPB)), b(form_block<ND, RS, P>());
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
     const Eops<RS, double>* od = static_cast<const Eops<RS, double>*>(l.eops);
+    if (l.fold) return local_fold<P, true>(l, cls, visc, coef, stage, cr, uf, g, b, o, od);
+    return local_fold<P, false>(l, cls, visc, coef, stage, cr, uf, g, b, o, od);
+  }
+  template <class P, bool FOLD>
+  static hipError_t local_fold(const Launch& l, int cls, bool visc, double coef, int stage, int cr, int uf, dim3 g, dim3 b,
+                               const Eops<RS, double>& o, const Eops<RS, double>* od)
+  {
     if (visc) {
       if (cls == HX_DEFORMED)
-        hipLaunchKernelGGL((k_local_ns<ND, RS, true, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, l.dtp, cr, uf);
-      else hipLaunchKernelGGL((k_local_ns<ND, RS, false, P>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, l.dtp, cr, uf);
+        hipLaunchKernelGGL((k_local_ns<ND, RS, true, P, FOLD>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, l.dtp, cr, uf);
+      else hipLaunchKernelGGL((k_local_ns<ND, RS, false, P, FOLD>), g, b, 0, l.stream, l.m, o, od, l.ns, coef, l.dtp, cr, uf);
     } else {
       const dim3 be(euler_block<ND, RS>());
       Dt_fuse f = l.fuse;  // the deformed class's workgroups store after the Cartesian class's
       if (f.block_min && cls == HX_DEFORMED) f.block_offset += blocks_for(l.m.n_elem[HX_CARTESIAN], S::EPB);
       if (cls == HX_DEFORMED)
-        hipLaunchKernelGGL((k_local_euler<ND, RS, true, P>), g, be, 0, l.stream, l.m, o, od, coef, l.dtp, stage, cr, uf, f, ops(l));
+        hipLaunchKernelGGL((k_local_euler<ND, RS, true, euler_form<P, true>, FOLD>), g, be, 0, l.stream, l.m, o, od, coef,
+                           l.dtp, stage, cr, uf, f, ops(l));
       else
-        hipLaunchKernelGGL((k_local_euler<ND, RS, false, P>), g, be, 0, l.stream, l.m, o, od, coef, l.dtp, stage, cr, uf, f, ops(l));
+        hipLaunchKernelGGL((k_local_euler<ND, RS, false, P, FOLD>), g, be, 0, l.stream, l.m, o, od, coef, l.dtp, stage, cr,
+                           uf, f, ops(l));
     }
     return hipGetLastError();
   }

@@ -20,6 +20,7 @@ struct Launch
   int precision;     // hx_precision
   Dt_fuse fuse;      // next step's Max_dt fused into the Euler Local (slots == nullptr: not fused)
   const double* dtp; // device-resident time step the stage kernels multiply their coefficient by, or null
+  int fold;          // face-flux folding: the Local computes its faces' fluxes, Neighbor covers only mortar connections
 };
 
 struct Kernel_set

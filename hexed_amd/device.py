@@ -113,6 +113,11 @@ class Device_mesh:
     def set_precision(self, precision):
         self.ctx.check(self.lib.hx_mesh_set_precision(self.ptr, precision), "set_precision")
 
+    def set_face_fold(self, mode):
+        """Whether update folds the face fluxes into the Local: 1 always, 0 never (Neighbor then Local, as the
+        reference sequences them), -1 where 2 n_dim <= row_size (the default)."""
+        self.ctx.check(self.lib.hx_mesh_set_face_fold(self.ptr, int(mode)), "set_face_fold")
+
     def _field_array(self, name, cls):
         h = self.host
         if name in ("faces", "normals", "metrics"):
@@ -322,7 +327,7 @@ class Device_group:
         self.ctx = ctx
         self.lib = ctx.lib
         self.meshes = list(meshes)
-        self.n_cases = sum(m.n_cases for m in self.meshes)
+        self.n_cases = sum(getattr(m, "n_cases", 0) for m in self.meshes)
         ptrs = (ctypes.c_void_p * len(self.meshes))(*[m.ptr.value for m in self.meshes])
         self.ptr = ctypes.c_void_p()
         ctx.check(self.lib.hx_group_create(ctx.ptr, ptrs, len(self.meshes), ctypes.byref(self.ptr)), "hx_group_create")

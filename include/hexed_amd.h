@@ -308,6 +308,13 @@ hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size
  * call it after writing through a pointer from hx_mesh_device_ptr, so that the next hx_update computes Max_dt afresh
  * instead of reusing the minimum its previous call fused into the last stage. */
 hx_status hx_mesh_mark_changed(hx_mesh* mesh);
+/* No counterpart in the reference: whether hx_update, hx_update_async and hx_update_no_alloc fold the numerical face
+ * fluxes into the Local (each element computes its faces' fluxes from its own and its neighbour's extrapolated state,
+ * and Neighbor runs only on connections touching a hanging-face mortar). mode 1 folds, 0 does not (the reference's
+ * Neighbor-then-Local sequence), -1 (the default) folds where 2 n_dim <= row_size, where it measured faster. Results
+ * agree to rounding (the flux is the same expression); the faces an update leaves are the same. Batch and group
+ * updates always fold. */
+hx_status hx_mesh_set_face_fold(hx_mesh* mesh, int mode);
 
 /* ---- kernels: one per reference kernel class (Spatial.hpp) ------------------------------------------ */
 /* Spatial<Pde, def>::Write_face (Spatial.hpp:29-71): extrapolate element state to its faces. */
@@ -410,7 +417,8 @@ hx_status hx_group_update(hx_group* group, const hx_update_options* opts, hx_upd
 /* Times the fused Local under each candidate launch shape (one tile per workgroup; persistent grids of 2, 4, 8 and 16
  * workgroups per compute unit, strided and chunked), `reps` launches each after a warm-up, the best of them counted, and
  * sets the fastest. The Local runs in residual mode: states are left as they are, the residual cache is overwritten and
- * the faces are rewritten with the extrapolation of the unchanged state, so call it between updates. `table` (up to
+ * the face extrapolations go to the second face-state buffer (the primary faces keep theirs; boundary ghost slots get
+ * fluxes that the next update's state boundary conditions overwrite), so call it between updates. `table` (up to
  * `capacity` entries, may be NULL) receives every candidate with its time, n_table their number. */
 hx_status hx_group_autotune(hx_group* group, int use_filter, int reps, hx_launch_shape* table, int capacity, int* n_table);
 

@@ -48,6 +48,7 @@ def run_batch_vs_single(ctx, cases, opts, precision, relabel=None):
     for m in cases:
         d = Device_mesh(ctx, m.copy())
         d.set_precision(precision)
+        d.set_face_fold(1)  # a batch update folds the face fluxes at every row size
         singles.append((d, abi.hx_update_status()))
     for it in range(STEPS):
         dev.update_cases(opts, st)

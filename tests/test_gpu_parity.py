@@ -194,6 +194,30 @@ def test_update_navier_stokes(ctx, cfg, bc_kind):
         compare_mesh(f"ns update {it}", host, dev)
 
 
+@pytest.mark.parametrize("cfg", [(3, 6, True, 3), (3, 4, True, 3), (2, 3, False, 5), (3, 5, False, 3)], ids=ids)
+@pytest.mark.parametrize("fold", [0, 1])
+@pytest.mark.parametrize("ns", [False, True], ids=["euler", "ns"])
+def test_update_face_fold_modes(ctx, cfg, fold, ns):
+    """hx_update with the face fluxes folded into the Local (1) and with the Neighbor-then-Local sequence (0), each
+    forced on row sizes where the default picks the other, against the oracle after every step."""
+    say(f"[gpu] update face fold {fold} {'ns' if ns else 'euler'} {ids(cfg)}")
+    host = build(cfg, with_av=ns, seed=9)
+    dev = Device_mesh(ctx, host.copy())
+    dev.set_face_fold(fold)
+    s = host.struct()
+    if ns:
+        visc, cond = sutherland_air()
+        opts = abi.update_options(max_safety=0.3, visc=visc, cond=cond)
+    else:
+        opts = abi.update_options(max_safety=0.5)
+    st_o, st_d = abi.hx_update_status(), abi.hx_update_status()
+    for it in range(2):
+        o_call("oracle_update", ctypes.byref(s), ctypes.byref(opts), ctypes.byref(st_o))
+        dev.update(opts, st_d)
+        assert abs(st_d.time_step - st_o.time_step) <= 1e-13 * st_o.time_step
+        compare_mesh(f"fold {fold} update {it}", host, dev)
+
+
 @pytest.mark.parametrize("cfg", [(3, 6, True, 3), (2, 4, False, 4)], ids=ids)
 def test_residual(ctx, cfg):
     """Solver::compute_residual: residual cache only, state untouched."""
