@@ -117,14 +117,22 @@ def algorithmic_bytes(category, nd, nq, n_elem):
 FORMS = {"fp64": (abi.HX_FP64, "Fp64_form"), "fp32_mfma": (abi.HX_FP32_MFMA, "Fp32_mfma_form")}
 
 
-def kernel_symbol(category, nd, rs, form):
+def folds_faces(nd, rs, mode):
+    """Whether hx_update folds the face fluxes into the Local (hx_mesh_set_face_fold)."""
+    return mode == 1 or (mode == -1 and rs >= 2 * nd)
+
+
+def kernel_symbol(category, nd, rs, form, fold):
     """The HIP kernel a timing category launches (the name rocprofv3 reports, without the namespace), for local
-    categories."""
+    categories. The deformed Euler Local runs the FP64 form under either precision."""
     cls, kernel, pde = category.split("/")
     if kernel != "local":
         return None
     name = "k_local_ns" if pde == "navier_stokes" else "k_local_euler"
-    return f"{name}<{nd}, {rs}, {'true' if cls == 'deformed' else 'false'}, {FORMS[form][1]}>"
+    deformed = cls == "deformed"
+    form_name = "Fp64_form" if deformed and pde == "euler" else FORMS[form][1]
+    return (f"{name}<{nd}, {rs}, {'true' if deformed else 'false'}, {form_name}, "
+            f"{'true' if fold else 'false'}>")
 
 
 def measured_traffic(config, symbol):
@@ -388,7 +396,9 @@ def main():
         cls = abi.HX_DEFORMED if dominant.startswith("deformed") else abi.HX_CARTESIAN
         per_round = [algorithmic_bytes(dominant, nd, m.nq, m.n_elem[cls]) for m, _, _ in hosts]
         launches_per_mesh = launches / len(hosts)
-    symbol = kernel_symbol(dominant, nd, hosts[0][0].row_size, args.precision) if len(hosts) == 1 else None
+    rs0 = hosts[0][0].row_size
+    symbol = (kernel_symbol(dominant, nd, rs0, args.precision, folds_faces(nd, rs0, args.face_fold))
+              if len(hosts) == 1 else None)
     roofline = {"bound": "hbm", "kernel": dominant, "symbol": symbol, "launch_ms": round(ms / launches, 5),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "traffic": measured_traffic(args.config, symbol) if symbol else None,
