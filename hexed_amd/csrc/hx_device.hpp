@@ -94,6 +94,12 @@ struct Eops
   T lift[RS][2];   // L = W^{-1} B^T diag(-1, 1)
   T bound[2][RS];  // B
   T filt[RS][RS];  // modal filter
+  // The FP32 matrix-core operands of the row contraction, per lane row rr (lane mod 4): entry t * RS + k holds
+  // coefficient k of row 4t + rr of the stacked [D; B] (zero past its rs + 2 rows), entry MF_TA * RS + 2t + s holds
+  // L[4t + rr][s] (zero past row rs - 1); each lane row padded to whole 16-byte vectors
+  static constexpr int MF_TA = (RS + 5) / 4, MF_TL = (RS + 3) / 4, MF_N = MF_TA * RS + 2 * MF_TL;
+  static constexpr int MF_STRIDE = (MF_N + 3) / 4 * 4;
+  float mfma[4][MF_STRIDE];
 };
 
 template <int RS, class T>
@@ -109,6 +115,19 @@ Eops<RS, T> make_eops(const Ops<RS>& o)
     e.lift[i][1] = T(o.lift[i][1]);
     e.bound[0][i] = T(o.bound[0][i]);
     e.bound[1][i] = T(o.bound[1][i]);
+  }
+  using E = Eops<RS, T>;
+  for (int rr = 0; rr < 4; ++rr) {
+    for (int c = 0; c < E::MF_STRIDE; ++c) e.mfma[rr][c] = 0.f;
+    for (int t = 0; t < E::MF_TA; ++t)
+      for (int k = 0; k < RS; ++k) {
+        const int row = 4 * t + rr;
+        if (row < RS) e.mfma[rr][t * RS + k] = float(o.diff[row][k]);
+        else if (row < RS + 2) e.mfma[rr][t * RS + k] = float(o.bound[row - RS][k]);
+      }
+    for (int t = 0; t < E::MF_TL; ++t)
+      for (int s = 0; s < 2; ++s)
+        if (4 * t + rr < RS) e.mfma[rr][E::MF_TA * RS + 2 * t + s] = float(o.lift[4 * t + rr][s]);
   }
   return e;
 }

@@ -370,6 +370,66 @@ struct Contraction<RS, Fp32_mfma_form, RELOAD>
   }
 };
 
+// The same contraction with the operands read from a workgroup copy of Eops::mfma in LDS (lds_mfma_operands) at every
+// call, instead of being formed in registers from the FP64 constants: each lane reads its lane row's tile with 16-byte
+// reads, which frees the scalar registers the constants took and the conversions and selects that formed them. The
+// values, products and their order are the ones of Contraction<RS, Fp32_mfma_form>.
+template <int RS>
+struct Contraction_lds
+{
+  using E = Eops<RS, double>;
+  static constexpr int TA = E::MF_TA, TL = E::MF_TL;
+  const float* row;  // this lane's row of the LDS copy
+  __device__ explicit Contraction_lds(const float* tab) : row(tab + (threadIdx.x & 3) * E::MF_STRIDE) {}
+  __device__ void faces_if(bool with_faces, const float (&x)[RS], double b0, double b1, float (&y)[RS],
+                           float (&bx)[2]) const
+  {
+    float a[E::MF_STRIDE];
+    const hx_f4* r4 = reinterpret_cast<const hx_f4*>(row);
+#pragma unroll
+    for (int c = 0; c < E::MF_STRIDE / 4; ++c) {
+      const hx_f4 v = r4[c];
+      a[4 * c] = v[0];
+      a[4 * c + 1] = v[1];
+      a[4 * c + 2] = v[2];
+      a[4 * c + 3] = v[3];
+    }
+    hx_f4 acc[TA];
+#pragma unroll
+    for (int t = 0; t < TA; ++t) acc[t] = hx_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < RS; ++k)
+#pragma unroll
+      for (int t = 0; t < TA; ++t) acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[t * RS + k], x[k], acc[t], 0, 0, 0);
+    bx[0] = acc[RS / 4][RS % 4];
+    bx[1] = acc[(RS + 1) / 4][(RS + 1) % 4];
+    const float j0 = with_faces ? float(b0 - double(bx[0])) : 0.f;
+    const float j1 = with_faces ? float(b1 - double(bx[1])) : 0.f;
+#pragma unroll
+    for (int t = 0; t < TL; ++t) {
+      acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[TA * RS + 2 * t], j0, acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_4x4x1f32(a[TA * RS + 2 * t + 1], j1, acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < RS; ++i) y[i] = acc[i / 4][i % 4];
+  }
+  __device__ void faces(const float (&x)[RS], double b0, double b1, float (&y)[RS]) const
+  {
+    float bx[2];
+    faces_if(true, x, b0, b1, y, bx);
+  }
+};
+
+// Copies the device operator table's matrix-core operands into the workgroup's LDS (4 * Eops::MF_STRIDE floats at
+// tab); the caller synchronizes before the first contraction
+template <int RS, int NT>
+__device__ inline void lds_mfma_operands(const Eops<RS, double>* od, float* tab)
+{
+  using E = Eops<RS, double>;
+  const float* src = &od->mfma[0][0];
+  for (int k = threadIdx.x; k < 4 * E::MF_STRIDE; k += NT) tab[k] = src[k];
+}
+
 // Modal filter of NVAR padded [NQP] arrays per element at `buf` (element stride `per`), in place, one direction at a
 // time (Spatial.hpp:472-481). Ends synchronized.
 template <int ND, int RS, int NVAR, int NT, class T>
@@ -952,6 +1012,8 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
   __shared__ T lt[EPB * L::PER_T];
   __shared__ double ld[EPB * L::PER_D];
+  // FP32-MFMA form: the matrix-core operands, read from LDS at each contraction (Contraction_lds)
+  __shared__ __align__(16) float mtab[P::mfma ? 4 * Eops<RS, double>::MF_STRIDE : 4];
   const int t = threadIdx.x;
   const bool pt = t < EPB * NQ;
   const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0, qp = S::pad(q);
@@ -959,7 +1021,11 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
   const int e = blockIdx.x * EPB + el;
   const bool valid = pt && e < n;
   T* my = lt + el * L::PER_T;
-  const Contraction<RS, P, true> con(o, od);
+  if constexpr (P::mfma) lds_mfma_operands<RS, NT>(od, mtab);
+  const auto con = [&] {
+    if constexpr (P::mfma) return Contraction_lds<RS>(mtab);
+    else return Contraction<RS, P, true>(o, od);
+  }();
 
   // the numerical face fluxes are needed only after the gradient; their loads are issued now and held in
   // registers so that their latency hides behind the gradient phase

@@ -111,6 +111,35 @@ def test_graph_recapture_and_mixing(ctx):
     b.close()
 
 
+@pytest.mark.parametrize("local_time", [False, True], ids=["global", "local-time"])
+def test_graph_after_state_upload(ctx, local_time):
+    """A state uploaded between asynchronous sequences, its faces left as they were: the face slots then no longer hold
+    the state's extrapolation, and the next update must read them as the reference does (the graph, which forms each
+    element's own face states from its state, waits until an update has rewritten them)."""
+    say(f"[gpu] graph after a state upload, local time {local_time}")
+    m, a, b = twins(ctx, configs.euler_box(3, 6, mach=0.8, seed=6))
+    opts = abi.update_options(max_safety=0.6, local_time=local_time)
+    sa = abi.hx_update_status()
+    for _ in range(3):
+        a.update(opts, sa)
+    b.update_async(opts, 3)
+    assert_status("before upload", sa, b.status_sync())
+    assert_same("before upload", m, a, b)
+    for c in (0, 1):
+        if m.n_elem[c]:
+            s = a.download("state", c, out=np.zeros_like(m.state[c]))
+            s[:, 0] *= 1.5
+            a.upload("state", c, s)
+            b.upload("state", c, s)
+    for _ in range(3):
+        a.update(opts, sa)
+    b.update_async(opts, 3)
+    assert_status("after upload", sa, b.status_sync())
+    assert_same("after upload", m, a, b)
+    a.close()
+    b.close()
+
+
 def test_graph_rejected_uses(ctx):
     say("[gpu] graph entry point errors")
     m = configs.euler_box(2, 4, seed=1)
