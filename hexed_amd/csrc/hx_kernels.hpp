@@ -1309,7 +1309,8 @@ constexpr int reconcile_waves()
   return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
-template <int ND, int RS, bool DEF>
+// FOLD: the jumps are formed here from both sides' LDG blocks (ldg_jump) instead of read from the slots
+template <int ND, int RS, bool DEF, bool FOLD = false>
 __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) __attribute__((amdgpu_waves_per_eu(reconcile_waves<ND, RS>())))
 void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __restrict__ dtp, int compute_residual,
                      int use_filter)
@@ -1328,7 +1329,20 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : 0.;
-  load_faces<ND, RS, S::BLOCK>(m, cls, blockIdx.x, 1, lds + L::SL, L::PER);
+  if constexpr (FOLD) {
+    constexpr int NFP = 2 * ND * NFQ;
+    for (int k = t; k < EPB * NFP; k += S::BLOCK) {
+      const int el2 = k / NFP, r = k % NFP, face = r / NFQ, p = r % NFQ;
+      const int e2 = blockIdx.x * EPB + el2;
+      double jv[NV];
+      if (e2 < n) ldg_jump<ND, RS>(m, cls, e2, face, p, jv);
+      else
+        for (int v = 0; v < NV; ++v) jv[v] = 0.;
+      for (int v = 0; v < NV; ++v) lds[el2 * L::PER + L::SL + (face * NV + v) * NFQ + p] = jv[v];
+    }
+  } else {
+    load_faces<ND, RS, S::BLOCK>(m, cls, blockIdx.x, 1, lds + L::SL, L::PER);
+  }
   __syncthreads();
   // -boundary_term(jump) (Spatial.hpp:560-570)
   double rate[NV] = {};
@@ -1382,14 +1396,12 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh 
 }
 
 // ---- Neighbor_reconcile (Spatial.hpp:716-759) ------------------------------------------------------------
-template <int ND, int RS, bool DEF, int NVAR = ND + 2>
-__global__ __launch_bounds__(256) void k_neighbor_reconcile(Dev_mesh m)
+// face point f of connection con of class DEF
+template <int ND, int RS, bool DEF, int NVAR>
+__device__ __forceinline__ void neighbor_reconcile_con(const Dev_mesh& m, int con, int f)
 {
   constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;  // LDG block offset NV * NFQ; NVAR variables reconciled
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
-  const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int con = gid / NFQ, f = gid % NFQ;
-  if (con >= m.n_con[cls]) return;
   const Dir dir(m.con_dir[cls][con]);
   double* f0 = slot_ptr(m, m.con_slot[cls][2 * con]) + NV * NFQ;
   double* f1 = slot_ptr(m, m.con_slot[cls][2 * con + 1]) + NV * NFQ;
@@ -1407,6 +1419,64 @@ __global__ __launch_bounds__(256) void k_neighbor_reconcile(Dev_mesh m)
     avg += .5 * sg1 * a1;
     f0[v * NFQ + f] = sg0 * avg - a0;
     f1[v * NFQ + p1] = sg1 * avg - a1;
+  }
+}
+
+template <int ND, int RS, bool DEF, int NVAR = ND + 2>
+__global__ __launch_bounds__(256) void k_neighbor_reconcile(Dev_mesh m)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ;
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int con = gid / NFQ, f = gid % NFQ;
+  if (con < m.n_con[DEF ? HX_DEFORMED : HX_CARTESIAN]) neighbor_reconcile_con<ND, RS, DEF, NVAR>(m, con, f);
+}
+
+// Neighbor_reconcile over the connections that touch a hanging-face mortar (Dev_mesh::mortar_con), the ones whose
+// jumps the folded Reconcile_ldg_flux does not form itself
+template <int ND, int RS, bool DEF>
+__global__ __launch_bounds__(256) void k_neighbor_reconcile_mortar(Dev_mesh m)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ;
+  constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid / NFQ, f = gid % NFQ;
+  if (i < m.n_mortar_con[cls]) neighbor_reconcile_con<ND, RS, DEF, ND + 2>(m, m.mortar_con[cls][i], f);
+}
+
+// Neighbor_reconcile folded into Reconcile_ldg_flux: the reconciled viscous-flux jump of face point p of face `face`
+// of element e, this side's value of what Neighbor_reconcile writes (the same operations on the two sides' LDG blocks,
+// in the connection's side order), from the partner table (Dev_mesh::partner). A face whose partner is -1 (hanging
+// faces) reads the jump a launch before wrote into its own slot.
+template <int ND, int RS>
+__device__ __forceinline__ void ldg_jump(const Dev_mesh& m, int cls, int e, int face, int p, double (&out)[ND + 2])
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
+  const size_t k = size_t(e) * 2 * ND + face;
+  const int* pi = m.partner[cls] + 4 * k;
+  const double* own = slot_ptr(m, m.face_slot[cls][k]) + NV * NFQ;
+  const int ps = pi[0];
+  if (ps < 0) {
+    for (int v = 0; v < NV; ++v) out[v] = own[v * NFQ + p];
+    return;
+  }
+  const int info = pi[1];
+  const Dir dir(info & 63);
+  const bool side1 = (info >> 6) & 1, def = (info >> 7) & 1;
+  int fp = p;
+  double sg0 = 1., sg1 = 1.;
+  if (def) {
+    fp = side1 ? face0_index<ND, RS>(dir, p) : face1_index<ND, RS>(dir, p);
+    sg0 = 1 - 2 * int(dir.flip_normal(0));
+    sg1 = 1 - 2 * int(dir.flip_normal(1));
+  }
+  const double* par = slot_ptr(m, ps) + NV * NFQ;
+  for (int v = 0; v < NV; ++v) {
+    const double ao = own[v * NFQ + p], ap = par[v * NFQ + fp];
+    const double a0 = side1 ? ap : ao, a1 = side1 ? ao : ap;
+    double avg = 0;
+    avg += .5 * sg0 * a0;
+    avg += .5 * sg1 * a1;
+    out[v] = side1 ? sg1 * avg - a1 : sg0 * avg - a0;
   }
 }
 
@@ -1936,6 +2006,15 @@ struct Launchers
   }
   static hipError_t neighbor_reconcile(const Launch& l, int cls)
   {
+    if (l.fold) {
+      // folded into Reconcile_ldg_flux: only the connections touching a hanging-face mortar
+      const int n = l.m.n_mortar_con[cls];
+      if (n == 0) return hipSuccess;
+      dim3 g(blocks_for(size_t(n) * S::NFQ, 256)), b(256);
+      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_neighbor_reconcile_mortar<ND, RS, true>), g, b, 0, l.stream, l.m);
+      else hipLaunchKernelGGL((k_neighbor_reconcile_mortar<ND, RS, false>), g, b, 0, l.stream, l.m);
+      return hipGetLastError();
+    }
     const int n = l.m.n_con[cls];
     if (n == 0) return hipSuccess;
     dim3 g(blocks_for(size_t(n) * S::NFQ, 256)), b(256);
@@ -1949,8 +2028,15 @@ struct Launchers
     if (n == 0) return hipSuccess;
     dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
-    if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, l.dtp, cr, uf);
-    else hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, l.dtp, cr, uf);
+    if (l.fold) {
+      if (cls == HX_DEFORMED)
+        hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, true, true>), g, b, 0, l.stream, l.m, o, dt, l.dtp, cr, uf);
+      else
+        hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, false, true>), g, b, 0, l.stream, l.m, o, dt, l.dtp, cr, uf);
+    } else {
+      if (cls == HX_DEFORMED) hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, true>), g, b, 0, l.stream, l.m, o, dt, l.dtp, cr, uf);
+      else hipLaunchKernelGGL((k_reconcile_ldg<ND, RS, false>), g, b, 0, l.stream, l.m, o, dt, l.dtp, cr, uf);
+    }
     return hipGetLastError();
   }
   static hipError_t max_dt(const Launch& l, int cls, bool visc, int is_local, double cc, double cd)

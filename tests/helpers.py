@@ -65,6 +65,23 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / nb)
 
 
+def update_folds(m, mode=-1):
+    """Whether hx_update folds the face fluxes into the element kernels for mesh m (hx_mesh_set_face_fold `mode`)."""
+    return mode == 1 or (mode == -1 and m.row_size >= 2 * m.n_dim)
+
+
+def faces_rel_l2(got, ref, m, ldg=True):
+    """Relative L2 of two face buffers [n_slot][face_sz]. ldg False leaves out the LDG blocks, which a folded
+    Navier-Stokes update leaves holding the viscous-flux extrapolations instead of the reconciled jumps (values the
+    reference overwrites before it reads them again)."""
+    if ldg:
+        return rel_l2(got, ref)
+    k = m.nv * m.nfq
+    keep = np.ones(got.shape[1], bool)
+    keep[k:2 * k] = False
+    return rel_l2(got[:, keep], ref[:, keep])
+
+
 def per_var_rel_l2(a, b):
     """Relative L2 per conserved variable of [n][nv][nq] arrays."""
     return [rel_l2(a[:, v], b[:, v]) for v in range(a.shape[1])]
