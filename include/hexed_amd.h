@@ -310,9 +310,11 @@ hx_status hx_mesh_device_ptr(hx_mesh* mesh, int field, int cls, void** out, size
 hx_status hx_mesh_mark_changed(hx_mesh* mesh);
 /* No counterpart in the reference: whether hx_update, hx_update_async and hx_update_no_alloc fold the numerical face
  * fluxes into the Local (each element computes its faces' fluxes from its own and its neighbour's extrapolated state,
- * and Neighbor runs only on connections touching a hanging-face mortar). mode 1 folds, 0 does not (the reference's
- * Neighbor-then-Local sequence), -1 (the default) folds where 2 n_dim <= row_size, where it measured faster. Results
- * agree to rounding (the flux is the same expression); the faces an update leaves are the same. Batch and group
+ * and Neighbor runs only on connections touching a hanging-face mortar), and likewise the Navier-Stokes jumps into
+ * Reconcile_ldg_flux (Neighbor_reconcile only on those connections). mode 1 folds, 0 does not (the reference's
+ * sequence), -1 (the default) folds where 2 n_dim <= row_size, where it measured faster. Results agree to rounding
+ * (the same expressions); the face state blocks an update leaves are the same, while after a folded Navier-Stokes
+ * update the LDG blocks hold the viscous-flux extrapolations instead of the reconciled jumps. Batch and group
  * updates always fold. */
 hx_status hx_mesh_set_face_fold(hx_mesh* mesh, int mode);
 
