@@ -115,7 +115,19 @@ __global__ __launch_bounds__(256) void k_fused_neighbor(const Fused_member* __re
   });
 }
 
-// largest LDS footprint and smallest register target of the Euler Local tiles the fused kernel may run
+// the FP32-MFMA form's operand table at the start of the fused Local's LDS (the largest of the set's row sizes),
+// in doubles so that the tile arrays after it stay 8-byte aligned
+template <class P, int... RSS>
+constexpr int fused_table_doubles()
+{
+  int n = 0;
+  if constexpr (P::mfma)
+    for (int x : {4 * Eops<RSS, double>::MF_STRIDE...}) n = x > n ? x : n;
+  return (n + 1) / 2;
+}
+
+// largest LDS footprint and smallest register target of the Euler Local tiles the fused kernel may run (the operand
+// table included)
 template <int ND, class P, bool WITH_DEF, int... RSS>
 constexpr int fused_local_lds()
 {
@@ -123,7 +135,7 @@ constexpr int fused_local_lds()
   for (int x : {euler_lds_bytes<ND, RSS, false, P>()...}) b = x > b ? x : b;
   if constexpr (WITH_DEF)
     for (int x : {euler_lds_bytes<ND, RSS, true, euler_form<P, true>>()...}) b = x > b ? x : b;
-  return b;
+  return b + 8 * fused_table_doubles<P, RSS...>();
 }
 template <int ND, class P, bool WITH_DEF, int... RSS>
 constexpr int fused_local_waves()
@@ -141,7 +153,9 @@ void k_fused_local_euler(const Fused_member* __restrict__ mem, const Fused_plan*
                          Eops_set<RSS...> ops, double coef, const double* __restrict__ dtp, int stage,
                          int compute_residual, int use_filter)
 {
-  extern __shared__ double fused_lds[];
+  extern __shared__ double fused_lds_all[];
+  float* mtab = reinterpret_cast<float*>(fused_lds_all);
+  double* fused_lds = fused_lds_all + fused_table_doubles<P, RSS...>();
   for_tiles<PERSIST>(plan->n_tiles, order, chunk, [&](int tile) {
     const int s = fused_segment(plan, tile);
     const Fused_member& mb = mem[plan->member[s]];
@@ -160,14 +174,14 @@ void k_fused_local_euler(const Fused_member* __restrict__ mem, const Fused_plan*
           constexpr int tb = Shape<ND, RS>::EPB * int(sizeof(TD)) * Euler_lds<ND, RS, true, PD>::PER_T;
           TD* lt = reinterpret_cast<TD*>(fused_lds);
           double* ld = fused_lds + (tb + 7) / 8;
-          local_euler_tile<ND, RS, true, PD, true>(mb.m, blk, lt, ld, o, od, coef, dtp, stage, compute_residual,
-                                                   use_filter, mb.fuse, gops);
+          local_euler_tile<ND, RS, true, PD, true>(mb.m, blk, lt, ld, mtab, o, od, coef, dtp, stage,
+                                                   compute_residual, use_filter, mb.fuse, gops);
         }
       } else {
         constexpr int tb = Shape<ND, RS>::EPB * int(sizeof(T)) * Euler_lds<ND, RS, false, P>::PER_T;
         T* lt = reinterpret_cast<T*>(fused_lds);
         double* ld = fused_lds + (tb + 7) / 8;
-        local_euler_tile<ND, RS, false, P, true>(mb.m, blk, lt, ld, o, od, coef, dtp, stage, compute_residual,
+        local_euler_tile<ND, RS, false, P, true>(mb.m, blk, lt, ld, mtab, o, od, coef, dtp, stage, compute_residual,
                                                  use_filter, mb.fuse, gops);
       }
     });

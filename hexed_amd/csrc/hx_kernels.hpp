@@ -828,8 +828,9 @@ constexpr int euler_lds_bytes()
 // ld (double) of Euler_lds. Shared by k_local_euler (one tile per workgroup) and the fused launch (tiles of several
 // meshes, classes and row sizes per kernel).
 // FOLD: the faces' numerical fluxes are computed here from both sides' states (face_flux) instead of read from the slots
+// mtab: the FP32-MFMA form's LDS copy of the matrix-core operands (4 * Eops::MF_STRIDE floats, 16-byte aligned)
 template <int ND, int RS, bool DEF, class P, bool FOLD = false>
-__device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typename P::T* lt, double* ld,
+__device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typename P::T* lt, double* ld, float* mtab,
                                                  const Eops<RS, double>& o, const Eops<RS, double>* od, double coef,
                                                  const double* __restrict__ dtp, int stage, int compute_residual,
                                                  int use_filter, const Dt_fuse& fuse, const Ops<RS>* __restrict__ gops)
@@ -846,7 +847,12 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   const int e = blk * EPB + el;
   const bool valid = pt && e < n;
   T* my = lt + el * L::PER_T;
-  const Contraction<RS, P> con(o, od);
+  // (the first contraction follows a barrier)
+  if constexpr (P::mfma) lds_mfma_operands<RS, NT>(od, mtab);
+  const auto con = [&] {
+    if constexpr (P::mfma) return Contraction_lds<RS>(mtab);
+    else return Contraction<RS, P>(o, od);
+  }();
 
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
@@ -950,8 +956,9 @@ void k_local_euler(Dev_mesh m, Eops<RS, double> o, const Eops<RS, double>* od, d
   constexpr int EPB = Shape<ND, RS>::EPB;
   __shared__ typename P::T lt[EPB * L::PER_T];
   __shared__ double ld[EPB * L::PER_D];
-  local_euler_tile<ND, RS, DEF, P, FOLD>(m, blockIdx.x, lt, ld, o, od, coef, dtp, stage, compute_residual, use_filter,
-                                         fuse, gops);
+  __shared__ __align__(16) float mtab[P::mfma ? 4 * Eops<RS, double>::MF_STRIDE : 4];
+  local_euler_tile<ND, RS, DEF, P, FOLD>(m, blockIdx.x, lt, ld, mtab, o, od, coef, dtp, stage, compute_residual,
+                                         use_filter, fuse, gops);
 }
 
 // ---- Local, Navier-Stokes stage 0 (Spatial.hpp:326-509 with Navier_stokes<true>) -------------------------
