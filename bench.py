@@ -397,8 +397,16 @@ def main():
         per_round = [algorithmic_bytes(dominant, nd, m.nq, m.n_elem[cls]) for m, _, _ in hosts]
         launches_per_mesh = launches / len(hosts)
     rs0 = hosts[0][0].row_size
-    symbol = (kernel_symbol(dominant, nd, rs0, args.precision, folds_faces(nd, rs0, args.face_fold))
-              if len(hosts) == 1 else None)
+    if dominant == "fused/local/euler":
+        # the fused Local over the group's row sizes, in the launch shape the autotuner kept
+        table = chosen.get("autotune") or []
+        persist = bool(table) and min(table, key=lambda r: r[2])[0] != 2  # (order, workgroups per CU, ms)
+        with_def = any(m.n_elem[1] for m, _, _ in hosts)
+        symbol = (f"k_fused_local_euler<{nd}, {FORMS[args.precision][1]}, {'true' if with_def else 'false'}, "
+                  f"{'true' if persist else 'false'}, 4, 5, 6>")
+    else:
+        symbol = (kernel_symbol(dominant, nd, rs0, args.precision, folds_faces(nd, rs0, args.face_fold))
+                  if len(hosts) == 1 else None)
     roofline = {"bound": "hbm", "kernel": dominant, "symbol": symbol, "launch_ms": round(ms / launches, 5),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "traffic": measured_traffic(args.config, symbol) if symbol else None,

@@ -1679,7 +1679,11 @@ __device__ inline void face_rows(Face_rows<RS, DIVIDE>& fr, const double (*mats)
 }
 
 // one thread per (refined face, variable, face point): the coarse face is staged in LDS and every fine face's value
-// at the point computed from it (the dimension-0 sums shared by the fine faces of the same half)
+// at the point computed from it. In 3D in two phases: each thread first forms the dimension-0 sums at its point
+// (a, k) for both halves into LDS (Face_rows::dim0's sums, shared by the fine faces of the same half, so each is
+// formed once instead of once per b), then after a barrier its own point's dimension-1 sum for every fine face
+// (Face_rows::dim1's). The operator matrices are staged in LDS once per workgroup. The products and their order
+// are those of Face_rows.
 template <int ND, int RS>
 __global__ __launch_bounds__(256) void k_prolong(Dev_mesh m, const Ops<RS>* __restrict__ ops, int n_var, int scale,
                                                  int offset)
@@ -1694,29 +1698,65 @@ __global__ __launch_bounds__(256) void k_prolong(Dev_mesh m, const Ops<RS>* __re
   // block 0 of the slots in the buffer being written (the prolongation follows the Local), or the LDG block
   auto at = [&](int slot) { return offset ? slot_ptr(m, slot) + (ND + 2 + v) * NFQ : state_ptr(m, slot, m.alt_wr) + v * NFQ; };
   if (active) stage[g][p] = at(m.ref_slot[5 * i_ref])[p];
-  __syncthreads();
-  if (!active) return;
-  Face_rows<RS, false> fr;
-  const int nf = fine_faces<ND>(m, i_ref, fr.str);
-  const int a = ND > 2 ? p / RS : p, b = ND > 2 ? p % RS : 0;
-  face_rows<ND, RS, false>(fr, ops->prolong, a, b, scale);
-  if constexpr (ND == 1) {
-    at(m.ref_slot[5 * i_ref + 1])[p] = stage[g][p];
-  } else if constexpr (ND == 2) {
-    for (int f = 0; f < nf; ++f) at(m.ref_slot[5 * i_ref + 1 + f])[p] = fr.line(stage[g], a, half_of<ND>(f, 0, fr.str));
-  } else {
-    double row_0[RS], row_1[RS];
-    fr.dim0(stage[g], a, false, row_0);
-    fr.dim0(stage[g], a, !fr.str[0], row_1);
+  if constexpr (ND == 3) {
+    __shared__ double mat[2][RS][RS];
+    __shared__ double rows[G][2][NFQ];
+    for (int k = threadIdx.x; k < 2 * RS * RS; k += 256) (&mat[0][0][0])[k] = (&ops->prolong[0][0][0])[k];
+    bool str[2] = {false, false};
+    const int nf = active ? fine_faces<ND>(m, i_ref, str) : 0;
+    const double fac = 1. + scale;
+    const int a = p / RS, c = p % RS;  // phase 1: point (a, k = c); phase 2: point (a, b = c)
+    __syncthreads();
+    if (active) {
+      // row h: the dimension-0 half 0, and half 1 unless dimension 0 is stretched (then both are the scaled row)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double val;
+        if (str[0]) {
+          val = stage[g][a * RS + c] * fac;
+        } else {
+          const int half = h;
+          val = 0.;
+#pragma unroll
+          for (int j = 0; j < RS; ++j) val += mat[half][a][j] * stage[g][j * RS + c];
+        }
+        rows[g][h][p] = val;
+      }
+    }
+    __syncthreads();
+    if (!active) return;
     for (int f = 0; f < nf; ++f) {
-      const bool h0 = half_of<ND>(f, 0, fr.str), h1 = half_of<ND>(f, 1, fr.str);
-      at(m.ref_slot[5 * i_ref + 1 + f])[p] = fr.dim1(row_0, row_1, h0, b, h1);
+      const bool h0 = half_of<ND>(f, 0, str), h1 = half_of<ND>(f, 1, str);
+      const double* x = rows[g][h0 && !str[0] ? 1 : 0] + a * RS;
+      double r;
+      if (str[1]) {
+        r = x[c] * fac;
+      } else {
+        r = 0.;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) r += mat[h1 ? 1 : 0][c][k] * x[k];
+      }
+      at(m.ref_slot[5 * i_ref + 1 + f])[p] = r;
+    }
+  } else {
+    __syncthreads();
+    if (!active) return;
+    Face_rows<RS, false> fr;
+    const int nf = fine_faces<ND>(m, i_ref, fr.str);
+    const int a = ND > 2 ? p / RS : p, b = ND > 2 ? p % RS : 0;
+    face_rows<ND, RS, false>(fr, ops->prolong, a, b, scale);
+    if constexpr (ND == 1) {
+      at(m.ref_slot[5 * i_ref + 1])[p] = stage[g][p];
+    } else {
+      for (int f = 0; f < nf; ++f) at(m.ref_slot[5 * i_ref + 1 + f])[p] = fr.line(stage[g], a, half_of<ND>(f, 0, fr.str));
     }
   }
 }
 
 // one thread per (refined face, variable, face point): the fine faces are staged in LDS, restricted in place
-// (Restrict_refined leaves the restricted data in the fine faces) and summed into the coarse face in fine-face order
+// (Restrict_refined leaves the restricted data in the fine faces) and summed into the coarse face in fine-face order.
+// In 3D in two phases as k_prolong: each thread forms the dimension-0 sums at its point (a, k) of every fine face
+// into LDS, then its own point's dimension-1 sums.
 template <int ND, int RS>
 __global__ __launch_bounds__(256) void k_restrict(Dev_mesh m, const Ops<RS>* __restrict__ ops, int n_var, int scale,
                                                   int offset)
@@ -1735,26 +1775,58 @@ __global__ __launch_bounds__(256) void k_restrict(Dev_mesh m, const Ops<RS>* __r
   // block 0 of the slots in the buffer being read (the restriction precedes the Local), or the LDG block
   auto at = [&](int slot) { return offset ? slot_ptr(m, slot) + (ND + 2 + v) * NFQ : state_ptr(m, slot, m.alt_rd) + v * NFQ; };
   for (int f = 0; f < nf; ++f) stage[g][f][p] = at(m.ref_slot[5 * i_ref + 1 + f])[p];
-  __syncthreads();
-  if (!active) return;
-  const int a = ND > 2 ? p / RS : p, b = ND > 2 ? p % RS : 0;
-  face_rows<ND, RS, true>(fr, ops->restrict_, a, b, scale);
-  double acc = 0.;
-  for (int f = 0; f < nf; ++f) {
-    double r;
-    if constexpr (ND == 1) {
-      r = stage[g][f][p];
-    } else if constexpr (ND == 2) {
-      r = fr.line(stage[g][f], a, half_of<ND>(f, 0, fr.str));
-    } else {
-      double row[RS];
-      fr.dim0(stage[g][f], a, half_of<ND>(f, 0, fr.str), row);
-      r = fr.dim1(row, row, false, b, half_of<ND>(f, 1, fr.str));
+  if constexpr (ND == 3) {
+    __shared__ double mat[2][RS][RS];
+    __shared__ double rows[G][N_FACE][NFQ];
+    for (int k = threadIdx.x; k < 2 * RS * RS; k += 256) (&mat[0][0][0])[k] = (&ops->restrict_[0][0][0])[k];
+    const double fac = 1. + scale;
+    const int a = p / RS, c = p % RS;  // phase 1: point (a, k = c); phase 2: point (a, b = c)
+    __syncthreads();
+    for (int f = 0; f < nf; ++f) {
+      double val;
+      if (fr.str[0]) {
+        val = stage[g][f][a * RS + c] / fac;
+      } else {
+        const int h0 = half_of<ND>(f, 0, fr.str);
+        val = 0.;
+#pragma unroll
+        for (int j = 0; j < RS; ++j) val += mat[h0][a][j] * stage[g][f][j * RS + c];
+      }
+      rows[g][f][p] = val;
     }
-    at(m.ref_slot[5 * i_ref + 1 + f])[p] = r;
-    acc += r;
+    __syncthreads();
+    if (!active) return;
+    double acc = 0.;
+    for (int f = 0; f < nf; ++f) {
+      const double* x = rows[g][f] + a * RS;
+      double r;
+      if (fr.str[1]) {
+        r = x[c] / fac;
+      } else {
+        const int h1 = half_of<ND>(f, 1, fr.str);
+        r = 0.;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) r += mat[h1][c][k] * x[k];
+      }
+      at(m.ref_slot[5 * i_ref + 1 + f])[p] = r;
+      acc += r;
+    }
+    at(m.ref_slot[5 * i_ref])[p] = acc;
+  } else {
+    __syncthreads();
+    if (!active) return;
+    const int a = ND > 2 ? p / RS : p, b = ND > 2 ? p % RS : 0;
+    face_rows<ND, RS, true>(fr, ops->restrict_, a, b, scale);
+    double acc = 0.;
+    for (int f = 0; f < nf; ++f) {
+      double r;
+      if constexpr (ND == 1) r = stage[g][f][p];
+      else r = fr.line(stage[g][f], a, half_of<ND>(f, 0, fr.str));
+      at(m.ref_slot[5 * i_ref + 1 + f])[p] = r;
+      acc += r;
+    }
+    at(m.ref_slot[5 * i_ref])[p] = acc;
   }
-  at(m.ref_slot[5 * i_ref])[p] = acc;
 }
 
 // ---- flow boundary conditions (Boundary_condition.cpp) ---------------------------------------------------
