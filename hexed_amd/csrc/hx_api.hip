@@ -162,6 +162,7 @@ struct hx_mesh
   // stage reads (alt_rd) and the one its face extrapolations go to (alt_wr)
   int fold = 0, alt_rd = 0, alt_wr = 0;
   int face_fold_mode = -1;  // hx_mesh_set_face_fold: 1 fold, 0 do not, -1 fold where rs >= 2 n_dim
+  bool mortar_fusable = false;  // every refined face's mortars are in connections (k_mortar_restrict applies)
   hipGraphExec_t graph = nullptr;
   hx_update_options graph_opts{};
   int graph_precision = -1;
@@ -444,7 +445,11 @@ std::string validate(const hx_host_mesh* d, const Sizes& z)
 // that touches no hanging-face mortar gets its neighbour slot, the packed direction with its side, the connection class
 // and a boundary bit, and (deformed connections) the connection's normal slot; every other face (mortar neighbours,
 // coarse hanging faces) is marked precomputed. Connections touching a mortar are listed per class.
-void build_partners(const hx_host_mesh* d, const Sizes& z, std::vector<int> (&partner)[2], std::vector<int> (&mortar)[2])
+// ref_con[4 * i + f]: the connection (class << 30 | index) of mortar f of refined face i, -1 where the face has none.
+// Returns whether the mortars of every refined face are exactly the fine faces its stretch leaves (Restrict_refined's)
+// and each is in a connection, which k_mortar_restrict requires.
+bool build_partners(const hx_host_mesh* d, const Sizes& z, std::vector<int> (&partner)[2], std::vector<int> (&mortar)[2],
+                    std::vector<int>& ref_con)
 {
   const int nf = 2 * z.nd;
   std::vector<int> owner(std::max(d->n_slot, 1), -1);  // element face owning each slot: cls | (e * 2nd + face) << 1
@@ -460,12 +465,14 @@ void build_partners(const hx_host_mesh* d, const Sizes& z, std::vector<int> (&pa
     for (int f = 1; f < 5; ++f)
       if (d->ref_slot[5 * i + f] >= 0) is_mortar[d->ref_slot[5 * i + f]] = 1;
   for (int i = 0; i < d->n_bc_con; ++i) is_bc[d->bc_con[i]] = 1;
+  std::vector<int> con_of_slot(std::max(d->n_slot, 1), -1);  // the connection of each mortar slot
   for (int cc = 0; cc < 2; ++cc) {
     mortar[cc].clear();
     for (int con = 0; con < d->n_con[cc]; ++con) {
       const int s0 = d->con_slot[cc][2 * con], s1 = d->con_slot[cc][2 * con + 1];
       if (is_mortar[s0] || is_mortar[s1]) {
         mortar[cc].push_back(con);
+        con_of_slot[is_mortar[s0] ? s0 : s1] = cc << 30 | con;
         continue;
       }
       const int bc = cc == HX_DEFORMED ? is_bc[con] : 0;
@@ -479,6 +486,20 @@ void build_partners(const hx_host_mesh* d, const Sizes& z, std::vector<int> (&pa
       }
     }
   }
+  ref_con.assign(size_t(d->n_ref) * 4, -1);
+  bool complete = true;
+  for (int i = 0; i < d->n_ref; ++i) {
+    int nf = 1 << (z.nd - 1);
+    for (int k = 0; k < z.nd - 1 && k < 2; ++k)
+      if (d->ref_stretch[2 * i + k]) nf /= 2;
+    for (int f = 0; f < 4; ++f) {
+      const int slot = d->ref_slot[5 * i + 1 + f];
+      if (slot >= 0) ref_con[4 * size_t(i) + f] = con_of_slot[slot];
+      // (a mortar past the stretch's fine faces would not be restricted but still needs its flux)
+      complete = complete && (f < nf ? ref_con[4 * size_t(i) + f] >= 0 : slot < 0);
+    }
+  }
+  return complete;
 }
 
 // `writes`: the launch may change mesh data (every launch except read-only checks), which retires a fused Max_dt
@@ -1035,8 +1056,9 @@ hx_status hx_mesh_create(hx_ctx* ctx, const hx_host_mesh* d, hx_mesh** out)
   if ((s = device_copy(mesh, &m.bcs, d->bcs, (long long)d->n_bc, true))) return bail(s);
   if ((s = device_copy(mesh, &m.bc_cache, d->bc_cache, (long long)d->n_bc_con * z.nv * z.nfq, true))) return bail(s);
   {
-    std::vector<int> partner[2], mortar[2];
-    build_partners(d, z, partner, mortar);
+    std::vector<int> partner[2], mortar[2], ref_con;
+    mesh->mortar_fusable = build_partners(d, z, partner, mortar, ref_con);
+    if ((s = device_copy(mesh, &m.ref_con, ref_con.data(), (long long)ref_con.size(), true))) return bail(s);
     for (int c = 0; c < 2; ++c) {
       if ((s = device_copy(mesh, &m.partner[c], partner[c].data(), (long long)partner[c].size(), true))) return bail(s);
       m.n_mortar_con[c] = int(mortar[c].size());
@@ -1309,9 +1331,16 @@ hx_status hx_compute_euler(hx_mesh* mesh, const hx_kernel_options* o)
   hx_pde p{};
   p.kind = HX_PDE_EULER;
   hx_status s;
-  if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, o->i_stage))) return s;
-  if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, o->i_stage))) return s;
-  if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
+  if (mesh->fold && mesh->dm.nd == 3 && mesh->dm.n_ref > 0 && mesh->mortar_fusable) {
+    // the folded stage's Neighbor covers only the mortar connections: one launch with the restriction
+    Launch l = make_launch(mesh);
+    Timer tm(mesh->ctx, "prolong/restrict/mortar flux and restrict", mesh->dm.n_ref);
+    HX_TRY(mesh->ctx, mesh->ks->mortar_restrict(l), "mortar flux and restrict");
+  } else {
+    if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, o->i_stage))) return s;
+    if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, o->i_stage))) return s;
+    if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
+  }
   if ((s = hx_local(mesh, HX_CARTESIAN, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
   if ((s = hx_local(mesh, HX_DEFORMED, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
   return hx_prolong_refined(mesh, &p, 0, 0);

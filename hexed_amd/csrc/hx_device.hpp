@@ -63,6 +63,8 @@ struct Dev_mesh
   int* mortar_con[2];
   double* faces_alt;
   int alt_rd, alt_wr;
+  // [n_ref][4]: the connection of each mortar of a refined face, connection class << 30 | index (-1: no mortar)
+  int* ref_con;
 };
 
 // the state (block 0) of face slot `slot` in the buffer `alt` selects

@@ -495,32 +495,45 @@ __device__ inline void write_faces_from(const Dev_mesh& m, int cls, int blk, con
 }
 
 // ---- Neighbor (Spatial.hpp:613-704) ----------------------------------------------------------------------
-// the numerical flux at face point f of connection con of class DEF
+// The numerical flux at face point f of connection con of class DEF, and where each side keeps it: side 0 at f, side
+// 1 at p1, each times its normal sign. With VISC the LDG average of the two states goes into both LDG blocks.
+template <int ND, int RS>
+struct Con_flux
+{
+  double fl[ND + 2];
+  double* f0;
+  double* f1;
+  int s0, p1;
+  double sg0, sg1;
+};
+
 template <int ND, int RS, bool DEF, bool VISC>
-__device__ __forceinline__ void neighbor_con(const Dev_mesh& m, int con, int f)
+__device__ __forceinline__ Con_flux<ND, RS> con_flux(const Dev_mesh& m, int con, int f)
 {
   constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
+  Con_flux<ND, RS> r;
   const Dir dir(m.con_dir[cls][con]);
   const int s0 = m.con_slot[cls][2 * con], s1 = m.con_slot[cls][2 * con + 1];
-  double* f0 = state_ptr(m, s0, m.alt_rd);
-  double* f1 = state_ptr(m, s1, m.alt_rd);
-  int p1 = f;
-  double sg0 = 1., sg1 = 1.;
+  r.s0 = s0;
+  r.f0 = state_ptr(m, s0, m.alt_rd);
+  r.f1 = state_ptr(m, s1, m.alt_rd);
+  r.p1 = f;
+  r.sg0 = r.sg1 = 1.;
   double n[ND];
   if constexpr (DEF) {
-    p1 = face1_index<ND, RS>(dir, f);
-    sg0 = 1 - 2 * int(dir.flip_normal(0));
-    sg1 = 1 - 2 * int(dir.flip_normal(1));
+    r.p1 = face1_index<ND, RS>(dir, f);
+    r.sg0 = 1 - 2 * int(dir.flip_normal(0));
+    r.sg1 = 1 - 2 * int(dir.flip_normal(1));
     const double* nn = m.normals + size_t(m.con_nslot[2 * con]) * ND * NFQ;
-    for (int j = 0; j < ND; ++j) n[j] = sg0 * nn[j * NFQ + f];
+    for (int j = 0; j < ND; ++j) n[j] = r.sg0 * nn[j * NFQ + f];
   } else {
     for (int j = 0; j < ND; ++j) n[j] = (j == dir.i_dim0);
   }
   double u0[NV], u1[NV];
   for (int v = 0; v < NV; ++v) {
-    u0[v] = f0[v * NFQ + f];
-    u1[v] = f1[v * NFQ + p1];
+    u0[v] = r.f0[v * NFQ + f];
+    u1[v] = r.f1[v * NFQ + r.p1];
   }
   if constexpr (VISC) {
     double* l0 = slot_ptr(m, s0) + NV * NFQ;
@@ -528,7 +541,7 @@ __device__ __forceinline__ void neighbor_con(const Dev_mesh& m, int con, int f)
     for (int v = 0; v < NV; ++v) {
       const double avg = .5 * (u0[v] + u1[v]);
       l0[v * NFQ + f] = avg;
-      l1[v * NFQ + p1] = avg;
+      l1[v * NFQ + r.p1] = avg;
     }
   }
   double F0[NV], F1[NV];
@@ -537,10 +550,18 @@ __device__ __forceinline__ void neighbor_con(const Dev_mesh& m, int con, int f)
   double nsq = 0;
   for (int j = 0; j < ND; ++j) nsq += n[j] * n[j];
   const double c = fmax(char_speed<ND>(u0), char_speed<ND>(u1)) * sqrt(nsq);
+  for (int v = 0; v < NV; ++v) r.fl[v] = .5 * (F0[v] + F1[v] + c * (u0[v] - u1[v]));
+  return r;
+}
+
+template <int ND, int RS, bool DEF, bool VISC>
+__device__ __forceinline__ void neighbor_con(const Dev_mesh& m, int con, int f)
+{
+  constexpr int NFQ = Shape<ND, RS>::NFQ, NV = ND + 2;
+  const Con_flux<ND, RS> r = con_flux<ND, RS, DEF, VISC>(m, con, f);
   for (int v = 0; v < NV; ++v) {
-    const double fl = .5 * (F0[v] + F1[v] + c * (u0[v] - u1[v]));
-    f0[v * NFQ + f] = sg0 * fl;
-    f1[v * NFQ + p1] = sg1 * fl;
+    r.f0[v * NFQ + f] = r.sg0 * r.fl[v];
+    r.f1[v * NFQ + r.p1] = r.sg1 * r.fl[v];
   }
 }
 
@@ -1829,6 +1850,84 @@ __global__ __launch_bounds__(256) void k_restrict(Dev_mesh m, const Ops<RS>* __r
   }
 }
 
+// Neighbor on the connections of a refined face's mortars fused with the face's Restrict_refined, for the Euler stages
+// of the update path in 3D (scale 1, state blocks): one workgroup per refined face. Phase 1: each (mortar, connection
+// point) item forms the connection's numerical flux (con_flux, as Neighbor does) and stores the fine face's side in its
+// slot, the mortar's side in LDS instead of the mortar slot; phases 2 and 3: k_restrict's two phases from LDS, storing
+// the restricted mortar values and their sum. The values are those of k_neighbor_mortar followed by k_restrict; the
+// mortar slots' unrestricted fluxes, which the restriction overwrites, are never written.
+template <int ND, int RS>
+__global__ __launch_bounds__(256) void k_mortar_restrict(Dev_mesh m, const Ops<RS>* __restrict__ ops)
+{
+  constexpr int NFQ = ipow(RS, ND - 1), NV = ND + 2, N_FACE = 1 << (ND - 1);
+  __shared__ double stage[N_FACE][NV][NFQ];
+  __shared__ double rows[N_FACE][NV][NFQ];
+  __shared__ double mat[2][RS][RS];
+  const int i_ref = blockIdx.x, t = threadIdx.x;
+  for (int k = t; k < 2 * RS * RS; k += 256) (&mat[0][0][0])[k] = (&ops->restrict_[0][0][0])[k];
+  bool str[2] = {false, false};
+  const int nf = fine_faces<ND>(m, i_ref, str);
+  for (int it = t; it < nf * NFQ; it += 256) {
+    const int f = it / NFQ, q = it % NFQ;
+    const int code = m.ref_con[4 * i_ref + f];
+    if (code < 0) continue;  // (the host launches this only where every such mortar is in a connection)
+    const int con = code & 0x3fffffff;
+    const int mortar = m.ref_slot[5 * i_ref + 1 + f];
+    auto keep = [&](const Con_flux<ND, RS>& r) {
+      if (r.s0 == mortar) {
+        for (int v = 0; v < NV; ++v) {
+          stage[f][v][q] = r.sg0 * r.fl[v];
+          r.f1[v * NFQ + r.p1] = r.sg1 * r.fl[v];
+        }
+      } else {
+        for (int v = 0; v < NV; ++v) {
+          r.f0[v * NFQ + q] = r.sg0 * r.fl[v];
+          stage[f][v][r.p1] = r.sg1 * r.fl[v];
+        }
+      }
+    };
+    if (code >> 30) keep(con_flux<ND, RS, true, false>(m, con, q));
+    else keep(con_flux<ND, RS, false, false>(m, con, q));
+  }
+  constexpr double fac = 2.;  // 1 + scale
+  __syncthreads();
+  for (int it = t; it < nf * NV * NFQ; it += 256) {
+    const int f = it / (NV * NFQ), v = (it / NFQ) % NV, p = it % NFQ;
+    const int a = p / RS, c = p % RS;
+    double val;
+    if (str[0]) {
+      val = stage[f][v][a * RS + c] / fac;
+    } else {
+      const int h0 = half_of<ND>(f, 0, str);
+      val = 0.;
+#pragma unroll
+      for (int j = 0; j < RS; ++j) val += mat[h0][a][j] * stage[f][v][j * RS + c];
+    }
+    rows[f][v][p] = val;
+  }
+  __syncthreads();
+  for (int it = t; it < NV * NFQ; it += 256) {
+    const int v = it / NFQ, p = it % NFQ;
+    const int a = p / RS, c = p % RS;
+    double acc = 0.;
+    for (int f = 0; f < nf; ++f) {
+      const double* x = rows[f][v] + a * RS;
+      double r;
+      if (str[1]) {
+        r = x[c] / fac;
+      } else {
+        const int h1 = half_of<ND>(f, 1, str);
+        r = 0.;
+#pragma unroll
+        for (int k = 0; k < RS; ++k) r += mat[h1][c][k] * x[k];
+      }
+      state_ptr(m, m.ref_slot[5 * i_ref + 1 + f], m.alt_rd)[v * NFQ + p] = r;
+      acc += r;
+    }
+    state_ptr(m, m.ref_slot[5 * i_ref], m.alt_rd)[v * NFQ + p] = acc;
+  }
+}
+
 // ---- flow boundary conditions (Boundary_condition.cpp) ---------------------------------------------------
 template <int ND, int RS>
 __device__ inline void reflect_normal(double* g, const double* nrml, int f)
@@ -2155,6 +2254,16 @@ struct Launchers
                        l.m, ops(l), n_var, scale, offset);
     return hipGetLastError();
   }
+  static hipError_t mortar_restrict(const Launch& l)
+  {
+    if constexpr (ND != 3) {
+      return hipErrorNotSupported;
+    } else {
+      if (l.m.n_ref == 0) return hipSuccess;
+      hipLaunchKernelGGL((k_mortar_restrict<ND, RS>), dim3(l.m.n_ref), dim3(256), 0, l.stream, l.m, ops(l));
+      return hipGetLastError();
+    }
+  }
   static hipError_t state_bcs(const Launch& l)
   {
     if (l.m.n_bc_con == 0) return hipSuccess;
@@ -2377,7 +2486,7 @@ struct Launchers
                               &av_write_face, &av_neighbor, &adv_local, &diff_local,         &diff_neighbor_reconcile,
                               &diff_reconcile, &av_max_dt,  &av_bcs,    &av_points,          &max_dt_cases,
                               &admissible,    &fix_av,      &set_jacobian, &connection_normals, &hanging_match,
-                              &stab_indicator, &vertex_interp_lap};
+                              &stab_indicator, &vertex_interp_lap, &mortar_restrict};
     return &s;
   }
 };

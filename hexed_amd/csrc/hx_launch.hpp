@@ -63,6 +63,8 @@ struct Kernel_set
   // interpolation of vertex values into the Laplacian artificial viscosity
   hipError_t (*stab_indicator)(const Launch&, int cls, double char_speed, double* vert);
   hipError_t (*vertex_interp_lap)(const Launch&, int cls, const double* vert);
+  // 3D, face-flux folding: Neighbor on the mortar connections fused with Restrict_refined (k_mortar_restrict)
+  hipError_t (*mortar_restrict)(const Launch&);
 };
 
 // ---- fused launches over a group of meshes (hx_group) ----------------------------------------------------------

@@ -119,6 +119,27 @@ def test_update_navier_stokes_tree(ctx, t):
     dev.close()
 
 
+@pytest.mark.parametrize("fold", [0, 1])
+@pytest.mark.parametrize("t", TREES[:2], ids=ids)
+def test_update_euler_tree_fold_modes(ctx, t, fold):
+    """hx_update on 3D tree meshes with the face fluxes folded (where the Neighbor on the mortar connections and the
+    restriction run as one launch per stage, k_mortar_restrict) and not, each forced at both row sizes, against the
+    oracle after every step."""
+    say(f"[gpu] tree update euler fold {fold} {ids(t)}")
+    host = tree(t)
+    dev = Device_mesh(ctx, host.copy())
+    dev.set_face_fold(fold)
+    s = host.struct()
+    opts = abi.update_options(max_safety=0.5)
+    st_o, st_d = abi.hx_update_status(), abi.hx_update_status()
+    for it in range(2):
+        o_call("oracle_update", ctypes.byref(s), ctypes.byref(opts), ctypes.byref(st_o))
+        dev.update(opts, st_d)
+        assert abs(st_d.time_step - st_o.time_step) <= 1e-13 * st_o.time_step
+        compare(f"euler tree update fold {fold} {it}", host, dev)
+    dev.close()
+
+
 def test_freestream_tree(ctx):
     fs = configs.freestream_state(3, 1.0, [0.8, 0.1, -0.2], 1.0 / 1.4)
     host = configs.tree_mesh(base=3, rs=6, levels=2, frac=0.3, seed=7, bc=(abi.HX_BC_FREESTREAM, fs))
