@@ -1043,10 +1043,11 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
   // FP32-MFMA form: the matrix-core operands, read from LDS at each contraction (Contraction_lds)
   __shared__ __align__(16) float mtab[P::mfma ? 4 * Eops<RS, double>::MF_STRIDE : 4];
   const int t = threadIdx.x;
+  const int blk = blockIdx.x;
   const bool pt = t < EPB * NQ;
   const int el = pt ? t / NQ : 0, q = pt ? t % NQ : 0, qp = S::pad(q);
   const int n = m.n_elem[cls];
-  const int e = blockIdx.x * EPB + el;
+  const int e = blk * EPB + el;
   const bool valid = pt && e < n;
   T* my = lt + el * L::PER_T;
   if constexpr (P::mfma) lds_mfma_operands<RS, NT>(od, mtab);
@@ -1063,7 +1064,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
 #pragma unroll
     for (int k = 0; k < NSB; ++k) {
       const int kk = t + k * NT, el2 = kk / PF, r = kk % PF, f = r / (NV * NFQ), vq = r % (NV * NFQ);
-      const int e2 = blockIdx.x * EPB + el2;
+      const int e2 = blk * EPB + el2;
       sb[k] = (kk < EPB * PF && e2 < n) ? state_ptr(m, m.face_slot[cls][e2 * 2 * ND + f], m.alt_rd)[vq] : 0.;
     }
   }
@@ -1075,7 +1076,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
 #pragma unroll
     for (int k = 0; k < FPT; ++k) {
       const int kk = t + k * NT, el2 = kk / NFP, r = kk % NFP, face = r / NFQ, p = r % NFQ;
-      const int e2 = blockIdx.x * EPB + el2;
+      const int e2 = blk * EPB + el2;
       double avg[NV];
       if (kk < EPB * NFP && e2 < n) {
         face_flux<ND, RS, true>(m, cls, e2, face, p, sbf[k], avg);
@@ -1113,11 +1114,11 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
     if constexpr (DEF) return metd[i][j];
     else return double(i == j);
   };
-  if constexpr (!FOLD) load_faces<ND, RS, NT>(m, cls, blockIdx.x, 1, ld + L::SL, L::PER_D);
+  if constexpr (!FOLD) load_faces<ND, RS, NT>(m, cls, blk, 1, ld + L::SL, L::PER_D);
   if constexpr (DEF) {
     for (int k = t; k < EPB * 2 * ND * ND * NFQ; k += NT) {
       const int el2 = k / (2 * ND * ND * NFQ), r = k % (2 * ND * ND * NFQ), f = r / (ND * NFQ), jq = r % (ND * NFQ);
-      const int e2 = blockIdx.x * EPB + el2;
+      const int e2 = blk * EPB + el2;
       double val = 0.;
       if (e2 < n) {
         const int ns_ = m.face_nslot[e2 * 2 * ND + f];
@@ -1256,7 +1257,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
       T x[RS], y[RS], bx[2];
       load_row(b + L::F + v * NQP + base, sd, x);
       con.faces_if(false, x, 0., 0., y, bx);
-      const int e2 = blockIdx.x * EPB + el2;
+      const int e2 = blk * EPB + el2;
       if (ok && e2 < n) {
         const int* fs = m.face_slot[cls] + e2 * 2 * ND + 2 * i;
         slot_ptr(m, fs[0])[(NV + v) * NFQ + r] = double(bx[0]);
@@ -1337,6 +1338,9 @@ constexpr int reconcile_waves()
   return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
+template <int ND, int RS>
+__device__ __forceinline__ void ldg_jump(const Dev_mesh& m, int cls, int e, int face, int p, double (&out)[ND + 2]);
+
 // FOLD: the jumps are formed here from both sides' LDG blocks (ldg_jump) instead of read from the slots
 template <int ND, int RS, bool DEF, bool FOLD = false>
 __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) __attribute__((amdgpu_waves_per_eu(reconcile_waves<ND, RS>())))
@@ -1350,8 +1354,9 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __
   constexpr int cls = DEF ? HX_DEFORMED : HX_CARTESIAN;
   __shared__ double lds[EPB * L::PER];
   const int t = threadIdx.x, el = t / NQ, q = t % NQ;
+  const int blk = blockIdx.x;
   const int n = m.n_elem[cls];
-  const int e = blockIdx.x * EPB + el;
+  const int e = blk * EPB + el;
   const bool valid = e < n;
   double* my = lds + el * L::PER;
   double u[NV];
@@ -1361,7 +1366,7 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __
     constexpr int NFP = 2 * ND * NFQ;
     for (int k = t; k < EPB * NFP; k += S::BLOCK) {
       const int el2 = k / NFP, r = k % NFP, face = r / NFQ, p = r % NFQ;
-      const int e2 = blockIdx.x * EPB + el2;
+      const int e2 = blk * EPB + el2;
       double jv[NV];
       if (e2 < n) ldg_jump<ND, RS>(m, cls, e2, face, p, jv);
       else
@@ -1369,7 +1374,7 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __
       for (int v = 0; v < NV; ++v) lds[el2 * L::PER + L::SL + (face * NV + v) * NFQ + p] = jv[v];
     }
   } else {
-    load_faces<ND, RS, S::BLOCK>(m, cls, blockIdx.x, 1, lds + L::SL, L::PER);
+    load_faces<ND, RS, S::BLOCK>(m, cls, blk, 1, lds + L::SL, L::PER);
   }
   __syncthreads();
   // -boundary_term(jump) (Spatial.hpp:560-570)
@@ -1403,7 +1408,7 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __
   __syncthreads();
   for (int v = 0; v < NV; ++v) my[L::U + v * S::NQP + S::pad(q)] = u[v];
   __syncthreads();
-  write_faces_from<ND, RS, S::BLOCK>(m, cls, blockIdx.x, o, lds + L::U, L::PER);
+  write_faces_from<ND, RS, S::BLOCK>(m, cls, blk, o, lds + L::U, L::PER);
 }
 
 // ---- Write_face (Spatial.hpp:60-70) ----------------------------------------------------------------------
