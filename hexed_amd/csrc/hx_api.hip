@@ -1162,6 +1162,8 @@ hx_status hx_mesh_set_face_fold(hx_mesh* mesh, int mode)
 {
   if (!live(mesh)) return HX_ERR_INVALID;
   if (mode < -1 || mode > 1) return fail(mesh->ctx, HX_ERR_INVALID, "face fold mode: -1, 0 or 1");
+  // a graph captured by hx_update_async holds the launches of the previous mode
+  if (mode != mesh->face_fold_mode) release_graph(mesh);
   mesh->face_fold_mode = mode;
   return HX_OK;
 }

@@ -111,6 +111,25 @@ def test_graph_recapture_and_mixing(ctx):
     b.close()
 
 
+def test_graph_after_face_fold_change(ctx):
+    """Changing the face-fold mode between asynchronous sequences recaptures the graph: after each change the replays
+    stay bit-identical to hx_update in the new mode (folded and unfolded results differ in rounding)."""
+    say("[gpu] graph after a face-fold mode change")
+    m, a, b = twins(ctx, configs.euler_box(3, 6, mach=0.8, seed=8))
+    opts = abi.update_options(max_safety=0.6)
+    sa = abi.hx_update_status()
+    for mode in (-1, 0, 1, -1):
+        a.set_face_fold(mode)
+        b.set_face_fold(mode)
+        for _ in range(3):
+            a.update(opts, sa)
+        b.update_async(opts, 3)
+        assert_status(f"fold mode {mode}", sa, b.status_sync())
+        assert_same(f"fold mode {mode}", m, a, b)
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("local_time", [False, True], ids=["global", "local-time"])
 def test_graph_after_state_upload(ctx, local_time):
     """A state uploaded between asynchronous sequences, its faces left as they were: the face slots then no longer hold
