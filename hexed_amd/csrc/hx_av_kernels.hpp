@@ -615,11 +615,13 @@ __global__ __launch_bounds__(256) void k_av_points(Dev_mesh m, Av_consts<RS> ac,
 {
   constexpr int NQ = Shape<ND, RS>::NQ, NV = ND + 2;
   __shared__ double wsum[256 / 64];
-  const size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x;
-  const size_t e = gid / NQ;
-  const int q = int(gid % NQ);
+  // grid-stride over the points (the grid is capped, av_points_blocks), so that the residual takes one atomic per
+  // workgroup of a small grid rather than one per 256 points on a single word
   double part = 0.;
-  if (e < size_t(m.n_elem[cls])) {
+  const size_t n_pts = size_t(m.n_elem[cls]) * NQ;
+  for (size_t gid = size_t(blockIdx.x) * 256 + threadIdx.x; gid < n_pts; gid += size_t(gridDim.x) * 256) {
+    const size_t e = gid / NQ;
+    const int q = int(gid % NQ);
     double* st = m.state[cls] + e * NV * NQ;
     if (what == 0) {
       const double scale = sqrt(2 * st[ND * NQ + q] * st[(ND + 1) * NQ + q]);
@@ -638,7 +640,7 @@ __global__ __launch_bounds__(256) void k_av_points(Dev_mesh m, Av_consts<RS> ac,
       double volume = 1;
       for (int d = 0; d < ND; ++d) volume *= m.nom_size[cls][e];
       const double diff = av[q] - new_av;
-      part = (1. * diff * diff) * w * volume;
+      part += (1. * diff * diff) * w * volume;
       av[q] = new_av;
       const double scale = sqrt(2 * st[ND * NQ + q] * st[(ND + 1) * NQ + q]);
       for (int d = 0; d < ND; ++d) st[d * NQ + q] *= scale;

@@ -2458,8 +2458,9 @@ struct Launchers
     const int n = l.m.n_elem[cls];
     if (n == 0) return hipSuccess;
     const Av_consts<RS> c = av_consts(l, av_rs);
-    hipLaunchKernelGGL((k_av_points<ND, RS>), dim3(blocks_for(size_t(n) * S::NQ, 256)), dim3(256), 0, l.stream, l.m, c,
-                       cls, what, mult, us_max, resid);
+    constexpr unsigned av_points_blocks = 2048;
+    hipLaunchKernelGGL((k_av_points<ND, RS>), dim3(std::min(blocks_for(size_t(n) * S::NQ, 256), av_points_blocks)),
+                       dim3(256), 0, l.stream, l.m, c, cls, what, mult, us_max, resid);
     return hipGetLastError();
   }
   // Stab_art_visc into per-element vertex slots (both classes: the deformed class after the Cartesian one), and the
