@@ -236,8 +236,8 @@ struct Sav_lds
   static constexpr int NQP = S::NQP, NFQ = S::NFQ;
   static constexpr int U = 0;                                   // [NS][NQP] the diffused variables
   static constexpr int G = U + NS * NQP;                        // [NS][NQP] one gradient component
-  static constexpr int SN = G + NS * NQP;                       // [ND][ND][NQP] reference-level normals
-  static constexpr int SL = SN + (DEF ? ND * ND * NQP : 0);     // [2ND][NS][NFQ] LDG face state
+  static constexpr int SN = G + NS * NQP;                       // [ND][NQP] reference-level normals n_ij of one j
+  static constexpr int SL = SN + (DEF ? ND * NQP : 0);          // [2ND][NS][NFQ] LDG face state
   static constexpr int SFN = SL + 2 * ND * NS * NFQ;            // [2ND][ND][NFQ] face normals
   static constexpr int P1 = SFN + (DEF ? 2 * ND * ND * NFQ : 0);
   static constexpr int F = 0;                                   // [NS][NQP] viscous flux along the current direction
@@ -272,10 +272,9 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
   if constexpr (DEF) {
     const double* mt = m.metrics + size_t(e) * (ND * ND + 1) * NQ;
     for (int i = 0; i < ND; ++i)
-      for (int j = 0; j < ND; ++j) {
-        met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
-        my[L::SN + (i * ND + j) * NQP + qp] = met[i][j];
-      }
+      for (int j = 0; j < ND; ++j) met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
+    // the column of the normals that gradient component 0 needs (each later component's replaces it)
+    for (int i = 0; i < ND; ++i) my[L::SN + i * NQP + qp] = met[i][0];
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
   for (int k = t; k < EPB * 2 * ND * NS * NFQ; k += NT) {
@@ -300,6 +299,13 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
   const double inv_h = 1. / d_pos;
   double grad[NS][ND];
   for (int j = 0; j < ND; ++j) {
+    if constexpr (DEF) {
+      if (j) {
+        // (every read of the previous column ended at the barrier after its gradient component)
+        for (int i = 0; i < ND; ++i) my[L::SN + i * NQP + qp] = met[i][j];
+        __syncthreads();
+      }
+    }
     for (int i = DEF ? 0 : j; i < (DEF ? ND : j + 1); ++i) {
       for_columns<ND, RS, NS, NT, false>(i, [&](bool, int el2, int v, int r, int base, int sd) {
         double* b = lds + el2 * L::PER;
@@ -308,7 +314,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
         double b0 = b[L::SL + ((2 * i) * NS + v) * NFQ + r], b1 = b[L::SL + ((2 * i + 1) * NS + v) * NFQ + r];
         if constexpr (DEF) {
           double nk[RS];
-          load_row(b + L::SN + (i * ND + j) * NQP + base, sd, nk);
+          load_row(b + L::SN + i * NQP + base, sd, nk);
 #pragma unroll
           for (int k = 0; k < RS; ++k) x[k] *= nk[k];
           b0 *= b[L::SFN + ((2 * i) * ND + j) * NFQ + r];
