@@ -135,8 +135,8 @@ struct Adv_lds
   static constexpr int F = 0;                                  // [RS][NQP] flux along the current direction
   static constexpr int RT = F + RS * S::NQP;                   // [RS][NQP] rate
   static constexpr int U = 0;                                  // [ND + RS][NQP] updated state (after F, RT)
-  static constexpr int SB = (2 * RS > ND + RS ? 2 * RS : ND + RS) * S::NQP;  // [2ND][RS][NFQ] face fluxes
-  static constexpr int PER = SB + 2 * ND * RS * S::NFQ;
+  static constexpr int SB = (2 * RS > ND + RS ? 2 * RS : ND + RS) * S::NQP;  // [2][RS][NFQ] the direction's face fluxes
+  static constexpr int PER = SB + 2 * RS * S::NFQ;
 };
 
 template <int ND, int RS, bool DEF>
@@ -167,12 +167,20 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_adv_local(Dev_mesh m
       for (int j = 0; j < ND; ++j) met[i][j] = valid ? mt[(i * ND + j) * NQ + q] : double(i == j);
     det = valid ? mt[ND * ND * NQ + q] : 1.;
   }
-  // numerical face fluxes: the first rs variables of block 0 of every face
-  for (int k = t; k < EPB * 2 * ND * RS * NFQ; k += NT) {
-    const int el2 = k / (2 * ND * RS * NFQ), r = k % (2 * ND * RS * NFQ), f = r / (RS * NFQ), vq = r % (RS * NFQ);
-    const int e2 = blockIdx.x * EPB + el2;
-    lds[el2 * L::PER + L::SB + r] = e2 < n ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + f])[vq] : 0.;
-  }
+  // numerical face fluxes (the first rs variables of block 0 of every face), loaded now into registers, direction by
+  // direction, and staged into LDS one direction at a time: entry k of direction i's block [EPB][2][RS][NFQ] is held by
+  // thread k mod NT
+  constexpr int DB = EPB * 2 * RS * NFQ, FPD = (DB + NT - 1) / NT;
+  double sbr[ND][FPD];
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+#pragma unroll
+    for (int j = 0; j < FPD; ++j) {
+      const int k = t + j * NT, el2 = k / (2 * RS * NFQ), r = k % (2 * RS * NFQ);
+      const int e2 = blockIdx.x * EPB + el2;
+      sbr[i][j] = k < DB && e2 < n ? slot_ptr(m, m.face_slot[cls][e2 * 2 * ND + 2 * i + r / (RS * NFQ)])[r % (RS * NFQ)]
+                                   : 0.;
+    }
   for (int i = 0; i < ND; ++i) {
     double vn = 0;
     for (int j = 0; j < ND; ++j) {
@@ -181,12 +189,17 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_adv_local(Dev_mesh m
     }
     if (i) __syncthreads();
     for (int v = 0; v < RS; ++v) my[L::F + v * NQP + qp] = ac.adv_node[v] * vn * a[v];
+#pragma unroll
+    for (int j = 0; j < FPD; ++j) {
+      const int k = t + j * NT;
+      if (k < DB) lds[(k / (2 * RS * NFQ)) * L::PER + L::SB + k % (2 * RS * NFQ)] = sbr[i][j];
+    }
     __syncthreads();
     for_columns<ND, RS, RS, NT, false>(i, [&](bool, int el2, int v, int r, int base, int sd) {
       double* b = lds + el2 * L::PER;
       double x[RS], y[RS];
       load_row(b + L::F + v * NQP + base, sd, x);
-      con.faces(x, b[L::SB + ((2 * i) * RS + v) * NFQ + r], b[L::SB + ((2 * i + 1) * RS + v) * NFQ + r], y);
+      con.faces(x, b[L::SB + v * NFQ + r], b[L::SB + (RS + v) * NFQ + r], y);
       sub_row(b + L::RT + v * NQP + base, sd, y, i == 0);
     });
   }
