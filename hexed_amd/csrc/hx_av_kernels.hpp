@@ -680,9 +680,9 @@ __global__ __launch_bounds__(256) void k_av_points(Dev_mesh m, Av_consts<RS> ac,
 // Per element, the specific volume's share of L2 energy in the top Legendre mode along each dimension,
 //   nonsmooth = sum_dims sum_rows (row . orth_{rs-1} w)^2 face_weight / (n_dim ||1/rho||^2),
 // mapped to [0, 1] by a sine ramp of half width 0.5 around log10 = -4.25 log10(rs - 1), times (rs - 1) char_speed h.
-// One thread per point stages 1/rho in LDS, one thread per row forms its squared projection, and one thread per element
-// sums both in the reference's order (points, then dimensions and rows) and writes the value to every vertex slot of
-// the element (out[e][2^nd], ready for sharing by maximum).
+// One thread per point stages 1/rho (and its weighted square) in LDS, one thread per row forms its squared projection,
+// and one wavefront per element sums both and writes the value to every vertex slot of the element (out[e][2^nd],
+// ready for sharing by maximum).
 template <int ND, int RS>
 __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_stab_indicator(Dev_mesh m, Av_consts<RS> c, int cls,
                                                                         double char_speed, double* __restrict__ out)
@@ -691,12 +691,19 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_stab_indicator(Dev_m
   using R = Row<ND, RS>;
   constexpr int NQ = S::NQ, NFQ = S::NFQ, EPB = S::EPB, NR = ND * NFQ;
   __shared__ double iv[EPB][NQ];
+  __shared__ double pw[EPB][NQ];  // each point's (1/rho)^2 times its quadrature weight
   __shared__ double rowv[EPB][NR];
   const int t = threadIdx.x, el = t / NQ, q = t % NQ;
   const int n = m.n_elem[cls];
   const int e = blockIdx.x * EPB + el;
   const bool valid = el < EPB && e < n;
-  if (el < EPB) iv[el][q] = valid ? 1. / m.state[cls][size_t(e) * S::NV * NQ + ND * NQ + q] : 0.;
+  if (el < EPB) {
+    const double x = valid ? 1. / m.state[cls][size_t(e) * S::NV * NQ + ND * NQ + q] : 0.;
+    double qw = 1.;
+    for (int d = 0; d < ND; ++d) qw *= c.weight[(q / ipow(RS, ND - 1 - d)) % RS];
+    iv[el][q] = x;
+    pw[el][q] = x * x * qw;
+  }
   __syncthreads();
   for (int k = t; k < EPB * NR; k += S::BLOCK) {
     const int el2 = k / NR, r = k % NR, i_dim = r / NFQ, f = r % NFQ;
@@ -708,23 +715,31 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_stab_indicator(Dev_m
     rowv[el2][r] = dot * dot * fw;
   }
   __syncthreads();
-  if (!valid || q != 0) return;
-  double norm_sq = 0.;
-  for (int p = 0; p < NQ; ++p) {
-    double qw = 1.;
-    for (int d = 0; d < ND; ++d) qw *= c.weight[(p / ipow(RS, ND - 1 - d)) % RS];
-    norm_sq += iv[el][p] * iv[el][p] * qw;
+  // one wavefront per element: the point and row sums lane-strided, then across the lanes (a fixed order, not the
+  // reference's sequential one; the sums agree to rounding). Only whole wavefronts take part (a block's last one may be
+  // partial, and a shuffle must not read a lane the block does not have); every block has at least two.
+  constexpr int NWAVE = S::BLOCK / 64;
+  static_assert(NWAVE >= 1, "the block holds a whole wavefront");
+  const int lane = t & 63;
+  for (int el2 = t / 64; el2 < EPB && t / 64 < NWAVE; el2 += NWAVE) {
+    const int e2 = blockIdx.x * EPB + el2;
+    double norm_sq = 0., nonsmooth = 0.;
+    for (int p = lane; p < NQ; p += 64) norm_sq += pw[el2][p];
+    for (int r = lane; r < NR; r += 64) nonsmooth += rowv[el2][r];
+    for (int off = 32; off > 0; off /= 2) {
+      norm_sq += __shfl_xor(norm_sq, off, 64);
+      nonsmooth += __shfl_xor(nonsmooth, off, 64);
+    }
+    if (lane != 0 || e2 >= n) continue;
+    nonsmooth /= norm_sq * ND;
+    const double center = -4.25 * log(double(RS - 1)) / log(10.), half_width = .5;
+    double ind = log(nonsmooth) / log(10.);
+    if (ind <= center - half_width) ind = 0.;
+    else if (ind < center + half_width) ind = .5 * (1. + sin(M_PI * (ind - center) / 2. / half_width));
+    else ind = 1.;
+    const double val = (RS - 1) * char_speed * m.nom_size[cls][e2] * ind;
+    for (int v = 0; v < (1 << ND); ++v) out[size_t(e2) * (1 << ND) + v] = val;
   }
-  double nonsmooth = 0.;
-  for (int r = 0; r < NR; ++r) nonsmooth += rowv[el][r];
-  nonsmooth /= norm_sq * ND;
-  const double center = -4.25 * log(double(RS - 1)) / log(10.), half_width = .5;
-  double ind = log(nonsmooth) / log(10.);
-  if (ind <= center - half_width) ind = 0.;
-  else if (ind < center + half_width) ind = .5 * (1. + sin(M_PI * (ind - center) / 2. / half_width));
-  else ind = 1.;
-  const double val = (RS - 1) * char_speed * m.nom_size[cls][e] * ind;
-  for (int v = 0; v < (1 << ND); ++v) out[size_t(e) * (1 << ND) + v] = val;
 }
 
 // Laplacian artificial viscosity at every point from the element's vertex values vert[e][2^nd] (first vertex index
