@@ -137,6 +137,10 @@ constexpr int fused_local_lds()
     for (int x : {euler_lds_bytes<ND, RSS, true, euler_form<P, true>>()...}) b = x > b ? x : b;
   return b + 8 * fused_table_doubles<P, RSS...>();
 }
+// The FP32-MFMA form takes 5 waves per SIMD where the LDS allows them, although the row sizes whose tiles give each
+// thread two face points (4 and 5 in 3D) then spill a few registers: the larger row sizes carry most of the work
+// (measured on a batch of row sizes 4 to 6: 2.71 -> 2.59 ms per step). Otherwise the target is the smallest of the
+// per-class targets (the FP64 form, which the LDS holds to 4 workgroups per compute unit, only spills at 5).
 template <int ND, class P, bool WITH_DEF, int... RSS>
 constexpr int fused_local_waves()
 {
@@ -144,6 +148,9 @@ constexpr int fused_local_waves()
   for (int x : {euler_waves<ND, RSS, false, P, true>()...}) w = x < w ? x : w;
   if constexpr (WITH_DEF)
     for (int x : {euler_waves<ND, RSS, true, euler_form<P, true>, true>()...}) w = x < w ? x : w;
+  // workgroups of 4 waves (one per SIMD) that the LDS holds per compute unit
+  if constexpr (P::mfma)
+    if (163840 / fused_local_lds<ND, P, WITH_DEF, RSS...>() >= 5 && w < 5) w = 5;
   return w;
 }
 
