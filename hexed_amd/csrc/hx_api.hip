@@ -166,6 +166,22 @@ struct hx_mesh
   hipGraphExec_t graph = nullptr;
   hx_update_options graph_opts{};
   int graph_precision = -1;
+  // the vertex topology of the last call that shared vertex values (hx_calc_jacobian, hx_set_art_visc_admis): a host
+  // copy, against which a repeated topology is recognised and neither validated nor uploaded again, its device copy,
+  // and the device work arrays of the vertex sharing and of the element values, kept for the next call
+  struct
+  {
+    std::vector<int> ids;      // vertex ids of the Cartesian elements, then of the deformed ones
+    std::vector<int> hanging;  // hx_vertex_data::hanging
+    long long n_vertex = -1;   // -1: nothing cached
+    int* vid = nullptr;
+    int* hang = nullptr;
+    double* shared = nullptr;  // [n_vertex]
+    double* vals = nullptr;    // [9 n_hanging]
+    double* vert = nullptr;    // [n_cartesian + n_deformed][2^n_dim] per-element vertex copies
+    double* elem = nullptr;    // [n_cartesian + n_deformed] element values
+    long long cap_vertex = 0, cap_hanging = 0;  // sizes `shared` and `hang` / `vals` are allocated for
+  } vc;
 };
 
 
@@ -202,9 +218,23 @@ void release_graph(hx_mesh* mesh)
   mesh->graph_precision = -1;
 }
 
+void release_vertex_cache(hx_mesh* mesh)
+{
+  auto& vc = mesh->vc;
+  for (void* p : {(void*)vc.vid, (void*)vc.hang, (void*)vc.shared, (void*)vc.vals, (void*)vc.vert, (void*)vc.elem})
+    if (p) (void)hipFree(p);
+  vc.vid = vc.hang = nullptr;
+  vc.shared = vc.vals = vc.vert = vc.elem = nullptr;
+  vc.ids.clear();
+  vc.hanging.clear();
+  vc.n_vertex = -1;
+  vc.cap_vertex = vc.cap_hanging = 0;
+}
+
 void release_mesh_memory(hx_mesh* mesh)
 {
   release_graph(mesh);
+  release_vertex_cache(mesh);
   for (void* p : mesh->allocs) (void)hipFree(p);
   mesh->allocs.clear();
   mesh->dm = Dev_mesh{};
@@ -786,6 +816,13 @@ __global__ __launch_bounds__(256) void k_fill_n(double* p, double v, long long n
 {
   const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
   if (gid < n) p[gid] = v;
+}
+
+// dst[i] = src[i * stride], i < n
+__global__ __launch_bounds__(256) void k_gather_stride(const double* src, int stride, long long n, double* dst)
+{
+  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (gid < n) dst[gid] = src[gid * stride];
 }
 
 // device copies of a call's inputs, freed when the call returns
@@ -2560,38 +2597,89 @@ hx_status check_vertex_data(hx_mesh* mesh, const hx_vertex_data* v)
   return HX_OK;
 }
 
-// the per-element vertex copies `vert` [n_cartesian + n_deformed][2^nd] shared over every vertex (by minimum or by
-// maximum), then the hanging vertices matched; the buffers come from `sc`
-hx_status share_vertices(hx_mesh* mesh, const hx_vertex_data* v, double* vert, bool maximum, Scratch& sc)
+// device buffer `*p` of `count` elements, (re)allocated when `*p` is null or `realloc`
+template <class T>
+hipError_t vc_buffer(T** p, size_t count, bool realloc)
+{
+  if (*p && !realloc) return hipSuccess;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  return count ? hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)) : hipSuccess;
+}
+
+// makes the mesh's vertex cache hold the topology `v`: a topology equal to the cached one (same vertex count, ids and
+// hanging vertices) is taken as it is; another is validated, copied and uploaded. Also allocates the element work
+// arrays (`vert`, `elem`) on first use.
+hx_status sync_vertices(hx_mesh* mesh, const hx_vertex_data* v)
 {
   hx_ctx* ctx = mesh->ctx;
   const Dev_mesh& dm = mesh->dm;
+  auto& vc = mesh->vc;
   const int nvert = 1 << dm.nd;
   const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1], n = n_tot * nvert;
-  int *vid = nullptr, *hang = nullptr;
-  double *shared = nullptr, *vals = nullptr;
-  HX_TRY(ctx, sc.put(&vid, (const int*)nullptr, size_t(n)), "hipMalloc(vertex ids)");
-  for (int c = 0, off = 0; c < 2; off += dm.n_elem[c] * nvert, ++c)
+  HX_TRY(ctx, vc_buffer(&vc.vert, size_t(n), false), "hipMalloc(vertex values)");
+  HX_TRY(ctx, vc_buffer(&vc.elem, size_t(n_tot), false), "hipMalloc(element values)");
+  bool same = vc.n_vertex >= 0 && v->n_vertex == vc.n_vertex && v->n_hanging >= 0 &&
+              (long long)vc.hanging.size() == 12LL * v->n_hanging && (long long)vc.ids.size() == n;
+  for (int c = 0, off = 0; same && c < 2; off += dm.n_elem[c] * nvert, ++c)
     if (dm.n_elem[c])
-      HX_TRY(ctx, hipMemcpy(vid + off, v->vertex_id[c], size_t(dm.n_elem[c]) * nvert * sizeof(int), hipMemcpyHostToDevice),
-             "upload vertex ids");
-  HX_TRY(ctx, sc.put(&hang, v->hanging, size_t(12LL * v->n_hanging)), "upload hanging vertices");
-  HX_TRY(ctx, sc.put(&shared, (const double*)nullptr, size_t(v->n_vertex)), "hipMalloc(shared vertices)");
-  HX_TRY(ctx, sc.put(&vals, (const double*)nullptr, size_t(9LL * v->n_hanging)), "hipMalloc(hanging values)");
+      same = v->vertex_id[c] && !std::memcmp(vc.ids.data() + off, v->vertex_id[c], size_t(dm.n_elem[c]) * nvert * sizeof(int));
+  if (same && v->n_hanging)
+    same = v->hanging && !std::memcmp(vc.hanging.data(), v->hanging, vc.hanging.size() * sizeof(int));
+  if (same) return HX_OK;
+  if (hx_status s = check_vertex_data(mesh, v)) return s;
+  // the stream's earlier work may still read the buffers replaced below; from here on the cache is invalid until the
+  // new topology is on the device
+  HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "vertex upload sync");
+  vc.n_vertex = -1;
+  vc.ids.resize(size_t(n));
+  for (int c = 0, off = 0; c < 2; off += dm.n_elem[c] * nvert, ++c)
+    if (dm.n_elem[c]) std::memcpy(vc.ids.data() + off, v->vertex_id[c], size_t(dm.n_elem[c]) * nvert * sizeof(int));
+  vc.hanging.assign(v->hanging, v->hanging + 12LL * v->n_hanging);
+  HX_TRY(ctx, vc_buffer(&vc.vid, size_t(n), false), "hipMalloc(vertex ids)");
+  if (vc.cap_vertex < v->n_vertex) {
+    vc.cap_vertex = 0;
+    HX_TRY(ctx, vc_buffer(&vc.shared, size_t(v->n_vertex), true), "hipMalloc(shared vertices)");
+    vc.cap_vertex = v->n_vertex;
+  }
+  if (vc.cap_hanging < v->n_hanging) {
+    vc.cap_hanging = 0;
+    HX_TRY(ctx, vc_buffer(&vc.hang, size_t(12LL * v->n_hanging), true), "hipMalloc(hanging vertices)");
+    HX_TRY(ctx, vc_buffer(&vc.vals, size_t(9LL * v->n_hanging), true), "hipMalloc(hanging values)");
+    vc.cap_hanging = v->n_hanging;
+  }
+  if (n) HX_TRY(ctx, hipMemcpy(vc.vid, vc.ids.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice), "upload vertex ids");
+  if (v->n_hanging)
+    HX_TRY(ctx, hipMemcpy(vc.hang, vc.hanging.data(), vc.hanging.size() * sizeof(int), hipMemcpyHostToDevice),
+           "upload hanging vertices");
+  vc.n_vertex = v->n_vertex;
+  return HX_OK;
+}
+
+// the per-element vertex copies `vert` [n_cartesian + n_deformed][2^nd] shared over every vertex (by minimum or by
+// maximum), then the hanging vertices matched, over the topology in the mesh's vertex cache (sync_vertices)
+hx_status share_vertices(hx_mesh* mesh, double* vert, bool maximum)
+{
+  hx_ctx* ctx = mesh->ctx;
+  const Dev_mesh& dm = mesh->dm;
+  const auto& vc = mesh->vc;
+  const int nvert = 1 << dm.nd;
+  const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1], n = n_tot * nvert;
+  const int n_hanging = int(vc.hanging.size() / 12);
   if (n) {
     const double inf = std::numeric_limits<double>::infinity();
-    hipLaunchKernelGGL(k_fill_n, dim3(unsigned((v->n_vertex + 255) / 256)), dim3(256), 0, ctx->stream, shared,
-                       maximum ? -inf : inf, (long long)v->n_vertex);
+    hipLaunchKernelGGL(k_fill_n, dim3(unsigned((vc.n_vertex + 255) / 256)), dim3(256), 0, ctx->stream, vc.shared,
+                       maximum ? -inf : inf, vc.n_vertex);
     if (maximum)
-      hipLaunchKernelGGL(k_vertex_push_max, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vert, n, shared);
+      hipLaunchKernelGGL(k_vertex_push_max, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vc.vid, vert, n, vc.shared);
     else
-      hipLaunchKernelGGL(k_vertex_push_min, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vert, n, shared);
-    hipLaunchKernelGGL(k_vertex_fetch, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vid, vert, n, shared);
+      hipLaunchKernelGGL(k_vertex_push_min, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vc.vid, vert, n, vc.shared);
+    hipLaunchKernelGGL(k_vertex_fetch, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, vc.vid, vert, n, vc.shared);
     HX_TRY(ctx, hipGetLastError(), "vertex sharing");
   }
   Launch l = make_launch(mesh);
-  HX_TRY(ctx, mesh->ks->hanging_match(l, hang, v->n_hanging, vert, vals, 0), "hanging vertices");
-  HX_TRY(ctx, mesh->ks->hanging_match(l, hang, v->n_hanging, vert, vals, 1), "hanging vertices");
+  HX_TRY(ctx, mesh->ks->hanging_match(l, vc.hang, n_hanging, vert, vc.vals, 0), "hanging vertices");
+  HX_TRY(ctx, mesh->ks->hanging_match(l, vc.hang, n_hanging, vert, vc.vals, 1), "hanging vertices");
   return HX_OK;
 }
 
@@ -2609,11 +2697,14 @@ hx_status stab_into(hx_mesh* mesh, double char_speed, double* vert, double* unce
     HX_TRY(ctx, mesh->ks->stab_indicator(l, cls, char_speed, vert), "stab_art_visc");
   }
   if (uncert && n_tot) {
-    std::vector<double> h(size_t(n_tot) * nvert);
-    HX_TRY(ctx, hipMemcpyAsync(h.data(), vert, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+    // each element's value is in all its vertex slots: the first slot of each, gathered on the device
+    double* elem = mesh->vc.elem;
+    hipLaunchKernelGGL(k_gather_stride, dim3(unsigned((n_tot + 255) / 256)), dim3(256), 0, ctx->stream, vert, nvert,
+                       n_tot, elem);
+    HX_TRY(ctx, hipGetLastError(), "element values");
+    HX_TRY(ctx, hipMemcpyAsync(uncert, elem, size_t(n_tot) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
            "element values readback");
     HX_TRY(ctx, hipStreamSynchronize(ctx->stream), "stab_art_visc sync");
-    for (long long e = 0; e < n_tot; ++e) uncert[e] = h[size_t(e) * nvert];
   }
   return HX_OK;
 }
@@ -2629,7 +2720,7 @@ hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
   Dev_mesh& dm = mesh->dm;
   const int nd = dm.nd, rs = dm.rs, nvert = 1 << nd, nfq = dm.nfq;
   const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1];
-  if (hx_status s = check_vertex_data(mesh, v)) return s;
+  if (hx_status s = sync_vertices(mesh, v)) return s;
   if (dm.n_elem[HX_DEFORMED] && !v->position) return fail(ctx, HX_ERR_INVALID, "vertex positions missing");
   ++mesh->gen;
   Scratch sc;
@@ -2651,7 +2742,7 @@ hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
   }
   {
     Timer tm(ctx, "compute jacobian/vertex time step scale", n_tot);
-    if (hx_status s = share_vertices(mesh, v, vt, false, sc)) return s;
+    if (hx_status s = share_vertices(mesh, vt, false)) return s;
   }
   for (int c = 0, off = 0; c < 2; off += dm.n_elem[c] * nvert, ++c)
     if (dm.n_elem[c])
@@ -2665,11 +2756,11 @@ hx_status hx_calc_jacobian(hx_mesh* mesh, const hx_vertex_data* v)
 hx_status hx_stab_art_visc(hx_mesh* mesh, double char_speed, double* uncert)
 {
   if (!live(mesh) || !uncert) return HX_ERR_INVALID;
+  auto& vc = mesh->vc;
   const long long n_tot = (long long)mesh->dm.n_elem[0] + mesh->dm.n_elem[1];
-  Scratch sc;
-  double* vert = nullptr;
-  HX_TRY(mesh->ctx, sc.put(&vert, (const double*)nullptr, size_t(n_tot) << mesh->dm.nd), "hipMalloc(element values)");
-  return stab_into(mesh, char_speed, vert, uncert);
+  HX_TRY(mesh->ctx, vc_buffer(&vc.vert, size_t(n_tot) << mesh->dm.nd, false), "hipMalloc(vertex values)");
+  HX_TRY(mesh->ctx, vc_buffer(&vc.elem, size_t(n_tot), false), "hipMalloc(element values)");
+  return stab_into(mesh, char_speed, vc.vert, uncert);
 }
 
 hx_status hx_set_art_visc_admis(hx_mesh* mesh, const hx_vertex_data* v, double char_speed, double* uncert)
@@ -2679,15 +2770,13 @@ hx_status hx_set_art_visc_admis(hx_mesh* mesh, const hx_vertex_data* v, double c
   const Dev_mesh& dm = mesh->dm;
   for (int c = 0; c < 2; ++c)
     if (dm.n_elem[c] && !dm.av[c]) return fail(ctx, HX_ERR_INVALID, "set_art_visc_admis needs the artificial viscosity arrays");
-  if (hx_status s = check_vertex_data(mesh, v)) return s;
+  if (hx_status s = sync_vertices(mesh, v)) return s;
   const long long n_tot = (long long)dm.n_elem[0] + dm.n_elem[1];
-  Scratch sc;
-  double* vert = nullptr;
-  HX_TRY(ctx, sc.put(&vert, (const double*)nullptr, size_t(n_tot) << dm.nd), "hipMalloc(vertex values)");
+  double* vert = mesh->vc.vert;
   if (hx_status s = stab_into(mesh, char_speed, vert, uncert)) return s;
   {
     Timer tm(ctx, "stabilizing art visc/vertices", n_tot);
-    if (hx_status s = share_vertices(mesh, v, vert, true, sc)) return s;
+    if (hx_status s = share_vertices(mesh, vert, true)) return s;
   }
   Launch l = make_launch(mesh);
   for (int cls = 0; cls < 2; ++cls) {

@@ -498,7 +498,10 @@ hx_status hx_stab_art_visc(hx_mesh* mesh, double char_speed, double* uncert);
  * Stab_art_visc's element values on every vertex, shared by maximum over the elements of each vertex (hanging
  * vertices matched, as in hx_calc_jacobian; `vertices` needs the ids and matchers, not the positions), interpolated
  * n-linearly to the quadrature points into the Laplacian coefficient av[.][1]. `uncert`, when not NULL, receives the
- * element values. */
+ * element values. The mesh keeps the vertex ids and matchers of its last call here or to hx_calc_jacobian on the device,
+ * with a host copy: a call whose topology equals it (compared by content, not by pointer) neither validates nor uploads
+ * them again, and the work arrays stay allocated, so repeated calls (Case.cpp:432 calls it before every update) cost
+ * the kernels and one readback of `uncert`. */
 hx_status hx_set_art_visc_admis(hx_mesh* mesh, const hx_vertex_data* vertices, double char_speed, double* uncert);
 
 #ifdef __cplusplus

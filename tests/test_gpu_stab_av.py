@@ -107,3 +107,65 @@ def test_stab_art_visc_tree(ctx, nd, rs):
             m.av[c] = np.zeros((m.n_elem[c], 2, m.nq))
     perturb(m, seed=7)
     compare(ctx, m, f"tree nd{nd} rs{rs}")
+
+
+def set_topology(m, kind, ids0, n0, perm):
+    """Vertex ids of `m` rewritten in place: the original ids, the same sharing under renumbered ids, or every element
+    with vertices of its own."""
+    nvert = 2 ** m.n_dim
+    off = 0
+    for c in (0, 1):
+        if kind == "original":
+            m.vertex_ids[c][...] = ids0[c]
+        elif kind == "renumbered":
+            m.vertex_ids[c][...] = perm[ids0[c]]
+        else:
+            m.vertex_ids[c][...] = off + np.arange(m.n_elem[c] * nvert, dtype=np.int32).reshape(-1, nvert)
+        off += m.n_elem[c] * nvert
+    m.n_vertex = off if kind == "unshared" else n0
+
+
+@pytest.mark.parametrize("shape", ["box3", "tree2"])
+def test_set_art_visc_admis_vertex_cache(ctx, shape):
+    """The mesh keeps the vertex topology of its last call on the device: repeated calls, the vertex ids renumbered in
+    place (the same host arrays, the same sharing), every element given vertices of its own (no sharing, more
+    vertices) and the original ids again each match the oracle given the same topology."""
+    say(f"[gpu] set_art_visc_admis vertex cache: {shape}")
+    if shape == "box3":
+        m = mesh.box(3, 4, 3, deformed=True, jitter=0.1, seed=3, bc=(abi.HX_BC_FREESTREAM, freestream(3)),
+                     with_av=True)
+    else:
+        m = configs.tree_mesh(base=3, rs=5, levels=2, frac=0.3, seed=4, nd=2)
+        assert len(m.hanging) > 0
+        for c in (0, 1):
+            if m.n_elem[c]:
+                m.av[c] = np.zeros((m.n_elem[c], 2, m.nq))
+    perturb(m, seed=11)
+    dev = Device_mesh(ctx, m.copy())
+    h = dev.host
+    ids0 = [m.vertex_ids[c].copy() for c in (0, 1)]
+    n0 = int(m.n_vertex)
+    perm = np.random.default_rng(5).permutation(n0).astype(np.int32)
+    char_speed = 3.0
+    lap = {}
+    for kind in ("original", "original", "renumbered", "unshared", "original"):
+        for x in (m, h):
+            set_topology(x, kind, ids0, n0, perm)
+        set_vertices(m)
+        s = m.struct()
+        u_o = np.zeros(sum(m.n_elem))
+        o_call("oracle_set_art_visc_admis", ctypes.byref(s), char_speed, u_o.ctypes.data_as(DP), None)
+        u_d = dev.set_art_visc_admis(char_speed)
+        scale = np.max(np.abs(u_o))
+        assert scale > 0
+        assert np.max(np.abs(u_d - u_o)) <= TOL * scale, f"{shape} {kind}: element values"
+        got = [dev.download("av", c, out=np.zeros_like(m.av[c]))[:, 1] if m.n_elem[c] else None for c in (0, 1)]
+        for c in (0, 1):
+            if m.n_elem[c]:
+                err = np.max(np.abs(got[c] - m.av[c][:, 1]))
+                assert err <= TOL * scale, f"{shape} {kind}: class {c} Laplacian AV {err / scale:.3e}"
+        lap[kind] = np.concatenate([g.ravel() for g in got if g is not None])
+    # the topologies lead to different viscosities, so a stale device topology would have failed above
+    assert not np.array_equal(lap["original"], lap["unshared"])
+    assert np.array_equal(lap["original"], lap["renumbered"])
+    dev.close()
