@@ -878,6 +878,22 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
+  // what the stage update at the end reads from memory -- the residual cache (stage 1), the time step scale, the
+  // nominal size. Cartesian elements load it at the start of the last direction, so that its latency overlaps that
+  // direction's phases (Euler Local 0.82 -> 0.78 ms per step on the 32^3 box, 0.48 -> 0.45 on the 16^3 tree);
+  // deformed ones at the update, where the earlier loads cost more registers than their latency saves
+  // (1.78 -> 1.84 ms on the 48^3 deformed mesh)
+  constexpr bool EARLY = !DEF && FOLD && RS <= 6;
+  double rcv[NV];
+  double tss_q = 0., nom = 1.;
+  const auto load_update_operands = [&] {
+    const double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
+    for (int v = 0; v < NV; ++v) rcv[v] = (valid && stage) ? rc[v * NQ + q] : 0.;
+    if (valid) {
+      tss_q = m.tss[cls][size_t(e) * NQ + q];
+      nom = m.nom_size[cls][e];
+    }
+  };
   // deformed: each direction's reference-level normals are loaded in that direction's point phase, so that they
   // are not held in registers across the row phases
   const double* mt = m.metrics + (DEF ? size_t(e) * (ND * ND + 1) * NQ + q : 0);
@@ -885,6 +901,7 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   if constexpr (FOLD) fold_fluxes<ND, RS, NT>(m, cls, blk, ld + L::SB, L::PER_D);
   else load_faces<ND, RS, NT>(m, cls, blk, 0, ld + L::SB, L::PER_D);
   for (int i = 0; i < ND; ++i) {
+    if (EARLY && i == ND - 1) load_update_operands();
     // volume flux along reference direction i (Spatial.hpp:404-419), in FP64
     double nrm[ND];
     for (int j = 0; j < ND; ++j) {
@@ -916,18 +933,19 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
     __syncthreads();
   }
   // stage update (Spatial.hpp:483-503)
+  if constexpr (!EARLY) load_update_operands();
   if (valid) {
-    double mult = step_coef(coef, dtp) * m.tss[cls][size_t(e) * NQ + q] / m.nom_size[cls][e];
+    double mult = step_coef(coef, dtp) * tss_q / nom;
     if constexpr (DEF) mult /= mt[ND * ND * NQ];
-    double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
+    double* rcw = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
       double r = double(my[L::RT + v * NQP + qp]);
       if (has_pressure<ND, DEF>(v) && P::mfma) r -= ld[el * L::PER_D + L::RP + v * NQP + qp];
-      if (stage) r -= rc[v * NQ + q];
-      else rc[v * NQ + q] = r;
+      if (stage) r -= rcv[v];
+      else rcw[v * NQ + q] = r;
       r *= mult;
-      if (compute_residual) rc[v * NQ + q] = r;
+      if (compute_residual) rcw[v * NQ + q] = r;
       else u[v] += r;
     }
     if (!compute_residual)
