@@ -885,13 +885,16 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   // (1.78 -> 1.84 ms on the 48^3 deformed mesh)
   constexpr bool EARLY = !DEF && FOLD && RS <= 6;
   double rcv[NV];
-  double tss_q = 0., nom = 1.;
+  double tss_q = 0., nom = 1., det_q = 1.;
   const auto load_update_operands = [&] {
     const double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     for (int v = 0; v < NV; ++v) rcv[v] = (valid && stage) ? rc[v * NQ + q] : 0.;
+    if (valid) nom = m.nom_size[cls][e];
+  };
+  const auto load_scale = [&] {
     if (valid) {
       tss_q = m.tss[cls][size_t(e) * NQ + q];
-      nom = m.nom_size[cls][e];
+      if constexpr (DEF) det_q = m.metrics[size_t(e) * (ND * ND + 1) * NQ + ND * ND * NQ + q];
     }
   };
   // deformed: each direction's reference-level normals are loaded in that direction's point phase, so that they
@@ -901,7 +904,10 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   if constexpr (FOLD) fold_fluxes<ND, RS, NT>(m, cls, blk, ld + L::SB, L::PER_D);
   else load_faces<ND, RS, NT>(m, cls, blk, 0, ld + L::SB, L::PER_D);
   for (int i = 0; i < ND; ++i) {
-    if (EARLY && i == ND - 1) load_update_operands();
+    if (i == ND - 1) {
+      if constexpr (EARLY) load_update_operands();
+      if constexpr (FOLD && RS <= 6) load_scale();
+    }
     // volume flux along reference direction i (Spatial.hpp:404-419), in FP64
     double nrm[ND];
     for (int j = 0; j < ND; ++j) {
@@ -934,9 +940,10 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   }
   // stage update (Spatial.hpp:483-503)
   if constexpr (!EARLY) load_update_operands();
+  if constexpr (!(FOLD && RS <= 6)) load_scale();
   if (valid) {
     double mult = step_coef(coef, dtp) * tss_q / nom;
-    if constexpr (DEF) mult /= mt[ND * ND * NQ];
+    if constexpr (DEF) mult /= det_q;
     double* rcw = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
     for (int v = 0; v < NV; ++v) {
@@ -1292,7 +1299,11 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
     for (int v = 0; v < NV; ++v) vrate[v] = my[L::RV + v * NQP + qp];
   }
   const double p64 = P::mfma ? pressure_of<ND>(u) : 0.;
+  // the time step scale of the stage update, loaded with the last direction's convective pass so that its latency
+  // overlaps that pass
+  double tss_q = 0.;
   for (int i = 0; i < ND; ++i) {
+    if (i == ND - 1 && valid) tss_q = m.tss[cls][size_t(e) * NQ + q];
     double nrmd[ND];
     for (int j = 0; j < ND; ++j) nrmd[j] = metric(i, j);
     __syncthreads();
@@ -1320,7 +1331,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
   }
   // stage-0 update (Spatial.hpp:483-503): the convective part goes to the residual cache
   if (valid) {
-    double mult = step_coef(coef, dtp) * m.tss[cls][size_t(e) * NQ + q] * inv_h;
+    double mult = step_coef(coef, dtp) * tss_q * inv_h;
     if constexpr (DEF) mult /= det;
     double* rc = m.rcache[cls] + size_t(e) * S::NRC * NQ;
     double* sw = m.state[cls] + size_t(e) * NV * NQ;
