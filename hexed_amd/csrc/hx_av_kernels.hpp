@@ -280,6 +280,19 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
   double uv[NS];
   for (int v = 0; v < NS; ++v) uv[v] = valid ? diff_vars<ND, RS, KIND>(m, cls, e)[v * NQ + q] : 0.;
   for (int v = 0; v < NS; ++v) my[L::U + v * NQP + qp] = uv[v];
+  // read with the first loads rather than where they are used: the source's forcing[0] (its forcing[1] and [2] are
+  // the first two diffused variables, held above), the update's time step scale, and the elements' face slots (into
+  // LDS, for the flux extrapolations)
+  const double f0 = (SAV && valid) ? m.forcing[cls][size_t(e) * 4 * NQ + q] : 0.;
+  const double tss_q = valid ? m.tss[cls][size_t(e) * NQ + q] : 0.;
+  // (the slots are staged where their LDS leaves the workgroups per compute unit as they are)
+  constexpr int LDS = 8 * EPB * L::PER, SLOTS = 4 * EPB * 2 * ND;
+  constexpr bool STAGE = 163840 / (LDS + SLOTS) == 163840 / LDS;
+  __shared__ int fsl[STAGE ? EPB * 2 * ND : 1];
+  if (STAGE && t < EPB * 2 * ND) {
+    const int e2 = blockIdx.x * EPB + t / (2 * ND);
+    fsl[t] = e2 < n ? m.face_slot[cls][e2 * 2 * ND + t % (2 * ND)] : 0;
+  }
   double det = 1.;
   double met[ND][ND];
   if constexpr (DEF) {
@@ -351,7 +364,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
     double src_mult = d_pos;
     if constexpr (DEF) src_mult *= det;
     for (int v = 0; v < NS; ++v) {
-      const double f = valid ? fabs(m.forcing[cls][(size_t(e) * 4 + v) * NQ + q]) : 0.;
+      const double f = valid ? fabs(v == 0 ? f0 : uv[v - 1]) : 0.;
       my[L::RT + v * NQP + qp] = src_mult * (((v == 1) ? sqrt(f) : f) / diff_time);
     }
   } else {
@@ -375,7 +388,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
       con.faces_if(false, x, 0., 0., y, bx);
       const int e2 = blockIdx.x * EPB + el2;
       if (e2 < n) {
-        const int* fs = m.face_slot[cls] + e2 * 2 * ND + 2 * i;
+        const int* fs = (STAGE ? fsl + el2 * 2 * ND : m.face_slot[cls] + e2 * 2 * ND) + 2 * i;
         slot_ptr(m, fs[0])[(NV + v) * NFQ + r] = bx[0];
         slot_ptr(m, fs[1])[(NV + v) * NFQ + r] = bx[1];
       }
@@ -384,7 +397,7 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_local(Dev_mesh m
   }
   __syncthreads();
   if (valid) {
-    double mult = coef * m.tss[cls][size_t(e) * NQ + q] / d_pos;
+    double mult = coef * tss_q / d_pos;
     if constexpr (DEF) mult /= det;
     double* out = diff_vars<ND, RS, KIND>(m, cls, e);
     for (int v = 0; v < NS; ++v) {
