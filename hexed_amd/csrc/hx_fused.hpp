@@ -137,11 +137,12 @@ constexpr int fused_local_lds()
     for (int x : {euler_lds_bytes<ND, RSS, true, euler_form<P, true>>()...}) b = x > b ? x : b;
   return b + 8 * fused_table_doubles<P, RSS...>();
 }
-// The FP32-MFMA form takes 5 waves per SIMD where the LDS allows them, although the row sizes whose tiles give each
-// thread two face points (4 and 5 in 3D) then spill a few registers: the larger row sizes carry most of the work
-// (measured on a batch of row sizes 4 to 6: 2.71 -> 2.59 ms per step). Otherwise the target is the smallest of the
-// per-class targets (the FP64 form, which the LDS holds to 4 workgroups per compute unit, only spills at 5).
-template <int ND, class P, bool WITH_DEF, int... RSS>
+// The FP32-MFMA form with one tile per workgroup takes 5 waves per SIMD where the LDS allows them, although the row
+// sizes whose tiles give each thread two face points (4 and 5 in 3D) then spill a few registers: the larger row sizes
+// carry most of the work (measured on a batch of row sizes 4 to 6: 2.71 -> 2.59 ms per step). Otherwise the target is
+// the smallest of the per-class targets (the FP64 form, which the LDS holds to 4 workgroups per compute unit, only
+// spills at 5; a persistent grid, whose tile loop holds more registers, spills far more: 1.9 -> 3.0 ms per launch).
+template <int ND, class P, bool WITH_DEF, bool PERSIST, int... RSS>
 constexpr int fused_local_waves()
 {
   int w = 8;
@@ -149,13 +150,13 @@ constexpr int fused_local_waves()
   if constexpr (WITH_DEF)
     for (int x : {euler_waves<ND, RSS, true, euler_form<P, true>, true>()...}) w = x < w ? x : w;
   // workgroups of 4 waves (one per SIMD) that the LDS holds per compute unit
-  if constexpr (P::mfma)
+  if constexpr (P::mfma && !PERSIST)
     if (163840 / fused_local_lds<ND, P, WITH_DEF, RSS...>() >= 5 && w < 5) w = 5;
   return w;
 }
 
 template <int ND, class P, bool WITH_DEF, bool PERSIST, int... RSS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_local_waves<ND, P, WITH_DEF, RSS...>())))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_local_waves<ND, P, WITH_DEF, PERSIST, RSS...>())))
 void k_fused_local_euler(const Fused_member* __restrict__ mem, const Fused_plan* __restrict__ plan, int order, int chunk,
                          Eops_set<RSS...> ops, double coef, const double* __restrict__ dtp, int stage,
                          int compute_residual, int use_filter)
