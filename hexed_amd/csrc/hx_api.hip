@@ -1220,7 +1220,7 @@ hx_status hx_write_face(hx_mesh* mesh, int cls, const hx_pde* pde)
   if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
   Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/write face/" + pde_name(pde), mesh->dm.n_elem[cls]);
   if (is_aux(pde_kind(pde))) HX_TRY(mesh->ctx, mesh->ks->av_write_face(l, cls, pde->kind), "write_face");
-  else HX_TRY(mesh->ctx, mesh->ks->write_face(l, cls), "write_face");
+  else HX_TRY(mesh->ctx, mesh->ks->write_face(l, cls, 0), "write_face");
   return HX_OK;
 }
 
@@ -2955,9 +2955,12 @@ hx_status hx_update_art_visc_smoothness(hx_mesh* mesh, const hx_av_options* o, d
   hx_status s;
   hx_kernel_options ko{1., 0, 0, 0};
   double dummy;
-  if ((s = points(0))) return s;
+  // the advection velocity and the Write_face of the scaled state in one pass over the state
+  for (int c = 0; c < 2; ++c) {
+    Timer tm(ctx, std::string(cls_name(c)) + "/write face/advection velocity", dm.n_elem[c]);
+    HX_TRY(ctx, mesh->ks->write_face(l, c, 1), "advection velocity and write_face");
+  }
   // enforce CFL condition
-  if ((s = hx_compute_write_face(mesh))) return s;
   if ((s = hx_compute_prolong(mesh, 0, 0))) return s;
   if ((s = hx_max_dt_advection(mesh, &ko, o->advect_max_safety, 1., 1, L, &dummy))) return s;
   for (int iter = 0; iter < o->advect_iters; ++iter) {
@@ -2986,7 +2989,11 @@ hx_status hx_update_art_visc_smoothness(hx_mesh* mesh, const hx_av_options* o, d
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, l.stream, resid_dev, 0.);
   HX_TRY(ctx, hipGetLastError(), "fill");
   if ((s = points(2))) return s;
-  if ((s = hx_compute_write_face(mesh))) return s;
+  // the flow state put back (momentum times sqrt(2 rho E)) and its Write_face, in one pass
+  for (int c = 0; c < 2; ++c) {
+    Timer tm(ctx, std::string(cls_name(c)) + "/write face/restored state", dm.n_elem[c]);
+    HX_TRY(ctx, mesh->ks->write_face(l, c, 2), "restore state and write_face");
+  }
   if ((s = hx_compute_prolong(mesh, 0, 0))) return s;
   if (residual) {
     double r = 0;

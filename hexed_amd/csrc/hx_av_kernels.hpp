@@ -638,9 +638,10 @@ __global__ __launch_bounds__(256) void k_fix_av(Dev_mesh m, int cls, int mode)
   }
 }
 
-// ---- point loops of update_art_visc_smoothness (Solver.cpp:465-477,525-540,549-574) ---------------------------
-// what 0: momentum /= sqrt(2 rho E); 1: Legendre projection of the advected scalars into forcing[0];
-// 2: new coefficient us_max f/(us_max + f), squared change weighted into `resid`, momentum restored
+// ---- point loops of update_art_visc_smoothness (Solver.cpp:525-540,549-574) ------------------------------------
+// what 1: Legendre projection of the advected scalars into forcing[0]; 2: new coefficient us_max f/(us_max + f),
+// squared change weighted into `resid`. (The momentum's division by sqrt(2 rho E) before the advection and its
+// restoration in the last loop run in the Write_face that follows each: k_write_face with scale_velocity.)
 template <int ND, int RS>
 __global__ __launch_bounds__(256) void k_av_points(Dev_mesh m, Av_consts<RS> ac, int cls, int what, double mult,
                                                    double us_max, double* resid)
@@ -655,10 +656,7 @@ __global__ __launch_bounds__(256) void k_av_points(Dev_mesh m, Av_consts<RS> ac,
     const size_t e = gid / NQ;
     const int q = int(gid % NQ);
     double* st = m.state[cls] + e * NV * NQ;
-    if (what == 0) {
-      const double scale = sqrt(2 * st[ND * NQ + q] * st[(ND + 1) * NQ + q]);
-      for (int d = 0; d < ND; ++d) st[d * NQ + q] /= scale;
-    } else if (what == 1) {
+    if (what == 1) {
       const double* adv = m.advection[cls] + e * RS * NQ;
       double proj = 0;
       for (int i = 0; i < RS; ++i) proj += adv[i * NQ + q] * ac.weight[i] * ac.orth[i];
@@ -674,8 +672,6 @@ __global__ __launch_bounds__(256) void k_av_points(Dev_mesh m, Av_consts<RS> ac,
       const double diff = av[q] - new_av;
       part += (1. * diff * diff) * w * volume;
       av[q] = new_av;
-      const double scale = sqrt(2 * st[ND * NQ + q] * st[(ND + 1) * NQ + q]);
-      for (int d = 0; d < ND; ++d) st[d * NQ + q] *= scale;
     }
   }
   if (what != 2) return;

@@ -1450,7 +1450,11 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __
 
 // ---- Write_face (Spatial.hpp:60-70) ----------------------------------------------------------------------
 template <int ND, int RS>
-__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh m, const Ops<RS>* __restrict__ gops, int cls)
+// scale_velocity 1: first divide the momentum by sqrt(2 rho E) in the state itself, the advection velocity of
+// Solver::update_art_visc_smoothness (Solver.cpp:468-478); 2: multiply it back (Solver.cpp:570-574). Either way the
+// Write_face that follows it in the reference reads the state once.
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh m, const Ops<RS>* __restrict__ gops, int cls,
+                                                                       int scale_velocity)
 {
   using S = Shape<ND, RS>;
   constexpr int NQ = S::NQ, NV = S::NV, EPB = S::EPB;
@@ -1458,9 +1462,16 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh 
   __shared__ double su[EPB][NV][NQ];
   const int t = threadIdx.x, el = t / NQ, q = t % NQ;
   const int e = blockIdx.x * EPB + el;
+  const bool valid = e < m.n_elem[cls];
   load_ops(ops, gops, t, S::BLOCK);
-  const double* st = m.state[cls] + size_t(e) * NV * NQ;
-  for (int v = 0; v < NV; ++v) su[el][v][q] = e < m.n_elem[cls] ? st[v * NQ + q] : 0.;
+  double* st = m.state[cls] + size_t(e) * NV * NQ;
+  double u[NV];
+  for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : 0.;
+  if (scale_velocity && valid) {
+    const double scale = sqrt(2 * u[ND] * u[ND + 1]);
+    for (int d = 0; d < ND; ++d) st[d * NQ + q] = u[d] = scale_velocity == 1 ? u[d] / scale : u[d] * scale;
+  }
+  for (int v = 0; v < NV; ++v) su[el][v][q] = u[v];
   __syncthreads();
   extrapolate_faces<ND, RS>(m, cls, ops, &su[0][0][0], 0);
 }
@@ -2152,11 +2163,12 @@ struct Launchers
   static const Ops<RS>* ops(const Launch& l) { return static_cast<const Ops<RS>*>(l.ops); }
   static const Ops<RS>& hops(const Launch& l) { return *static_cast<const Ops<RS>*>(l.hops); }
 
-  static hipError_t write_face(const Launch& l, int cls)
+  static hipError_t write_face(const Launch& l, int cls, int scale_velocity)
   {
     const int n = l.m.n_elem[cls];
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_write_face<ND, RS>), dim3(blocks_for(n, S::EPB)), dim3(S::BLOCK), 0, l.stream, l.m, ops(l), cls);
+    hipLaunchKernelGGL((k_write_face<ND, RS>), dim3(blocks_for(n, S::EPB)), dim3(S::BLOCK), 0, l.stream, l.m, ops(l), cls,
+                       scale_velocity);
     return hipGetLastError();
   }
   static hipError_t neighbor(const Launch& l, int cls, bool visc)

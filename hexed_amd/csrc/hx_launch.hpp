@@ -25,7 +25,7 @@ struct Launch
 
 struct Kernel_set
 {
-  hipError_t (*write_face)(const Launch&, int cls);
+  hipError_t (*write_face)(const Launch&, int cls, int scale_velocity);
   hipError_t (*neighbor)(const Launch&, int cls, bool visc);
   hipError_t (*local)(const Launch&, int cls, bool visc, double coef, int stage, int compute_residual, int use_filter);
   hipError_t (*neighbor_reconcile)(const Launch&, int cls);
