@@ -3,12 +3,12 @@
 Random numbers come from splitmix64 (Steele, Lea & Flood 2014) mapped to [0, 1) by the top 53 bits, so a
 configuration is the same on every platform and library version.
 
-  config 1  2D Euler isentropic vortex, 16x16 Cartesian quads on [-5, 5]^2, row size 6
-  config 2  3D Euler, 32^3 Cartesian hexes on [0, 1]^3, row size 6, Mach 0.8 + 1% Fourier perturbations
-  config 3  3D Navier-Stokes, 48^3 deformed hexes (interior vertices jittered +-10% of spacing), row size 6,
+  vortex    2D Euler isentropic vortex, 16x16 Cartesian quads on [-5, 5]^2, row size 6
+  euler32   3D Euler, 32^3 Cartesian hexes on [0, 1]^3, row size 6, Mach 0.8 + 1% Fourier perturbations
+  ns48      3D Navier-Stokes, 48^3 deformed hexes (interior vertices jittered +-10% of spacing), row size 6,
             Re 1e4 on unit length, Mach 0.3, Sutherland air, constant artificial viscosity
-  config 4  3D Euler on a seeded 2:1-balanced tree with hanging faces
-  config 5  a batch of 64 independent 3D Euler boxes of log-uniform size, row size 4-6, Mach 0.3-0.9
+  amr16     3D Euler on a seeded 2:1-balanced tree with hanging faces (16^3 base), row size 6 or 4
+  batch64   a batch of 64 independent 3D Euler boxes of log-uniform size, row size 4-6, Mach 0.3-0.9
 """
 import math
 

@@ -1,4 +1,4 @@
-"""Batches of independent cases (config 5): the case list, the concatenated mesh, and -- through the oracle -- that
+"""Batches of independent cases: the case list, the concatenated mesh, and -- through the oracle -- that
 the cases of a concatenated mesh do not interact. The GPU batch update is compared with single-case updates in
 test_gpu_batch.py."""
 import ctypes

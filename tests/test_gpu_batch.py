@@ -1,4 +1,4 @@
-"""hx_update_cases (a batch of independent cases in one mesh, config 5) against hx_update on a mesh holding each
+"""hx_update_cases (a batch of independent cases in one mesh) against hx_update on a mesh holding each
 case alone: the same time step per case and bit-identical states, in both arithmetic forms, for Euler and
 Navier-Stokes (single cases are themselves compared with the oracle in test_gpu_parity.py); plus the errors of
 the registration."""

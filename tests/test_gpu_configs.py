@@ -1,10 +1,10 @@
-"""Two configurations stepped on the HIP path against the oracle, end to end:
+"""Two workloads stepped on the HIP path against the oracle, end to end:
 
-- config 1, the 2D isentropic vortex (16x16 Cartesian quads on [-5, 5]^2, row size 6, global time step at
+- the 2D isentropic vortex (16x16 Cartesian quads on [-5, 5]^2, row size 6, global time step at
   max_safety 0.5): stage by stage through the kernel calls for the first steps, then 100 hx_update steps compared
   with the oracle after every step (state <= 1e-12 relative L2 per variable, the same time step), and the final
   state against the exact translated vortex;
-- config 5, a batch of independent Euler box cases of uneven sizes and Mach numbers at row sizes 4, 5 and 6 (one
+- a batch of independent Euler box cases of uneven sizes and Mach numbers at row sizes 4, 5 and 6 (one
   batch mesh per row size, as the batch workload runs them): each case of every batch is also stepped alone by the
   oracle, and its time step and state after every step must match (time step to 1e-13, state to 1e-12).
 """
@@ -39,9 +39,9 @@ def state_err(dev, host):
     return worst
 
 
-def test_vortex_config1_stages(ctx):
-    """Three time steps of config 1 through compute_euler's kernels, compared after every kernel call."""
-    say("[gpu] config 1 vortex, stage by stage")
+def test_vortex_stages(ctx):
+    """Three time steps of the vortex through compute_euler's kernels, compared after every kernel call."""
+    say("[gpu] isentropic vortex, stage by stage")
     host, _ = configs.isentropic_vortex()
     init_faces(host)
     dev = Device_mesh(ctx, host.copy())
@@ -69,9 +69,9 @@ def test_vortex_config1_stages(ctx):
     dev.close()
 
 
-def test_vortex_config1_update(ctx):
+def test_vortex_update(ctx):
     """Config 1's 100 steps: hx_update vs oracle_update after every step, then the exact translated vortex."""
-    say("[gpu] config 1 vortex, 100 updates")
+    say("[gpu] isentropic vortex, 100 updates")
     host, exact = configs.isentropic_vortex()
     init_faces(host)
     dev = Device_mesh(ctx, host.copy())
@@ -103,14 +103,14 @@ def small_batch():
 
 
 @pytest.mark.parametrize("precision", [abi.HX_FP64], ids=["fp64"])
-def test_batch_config5_vs_oracle(ctx, precision):
+def test_batch_vs_oracle(ctx, precision):
     cases = small_batch()
     assert {c["rs"] for c in cases} == {4, 5, 6}
     assert len({c["n"] for c in cases}) > 1
     groups = configs.euler_batch(cases)
     opts = abi.update_options(max_safety=0.6)
     for rs, (b, ks) in groups.items():
-        say(f"[gpu] config 5 batch vs oracle: row size {rs}, cases {ks}, {b.n_elem[0] + b.n_elem[1]} elements")
+        say(f"[gpu] batch vs oracle: row size {rs}, cases {ks}, {b.n_elem[0] + b.n_elem[1]} elements")
         init_faces(b)
         dev = Device_mesh(ctx, b.copy())
         dev.set_precision(precision)

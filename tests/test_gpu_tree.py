@@ -1,4 +1,4 @@
-"""Hanging faces (config 4): the HIP path against the oracle on seeded 2:1-balanced tree meshes, through
+"""Hanging faces: the HIP path against the oracle on seeded 2:1-balanced tree meshes, through
 Prolong_refined / Restrict_refined (Spatial.hpp:138-286) alone and inside whole time steps.
 
 Bar: relative L2 <= 1e-12 per conserved variable (floating point; summation order differs between host and gfx950)."""

@@ -263,7 +263,7 @@ def test_free_stream_preservation(nd, rs, deformed, visc):
     assert dev <= 1e-12 * max(abs(x) for x in fs)
 
 
-# ---- isentropic vortex (config 1): exact translation and convergence ----------------------------------------------
+# ---- 2D isentropic vortex: exact translation and convergence ----------------------------------------------
 
 def vortex_error(n, rs, steps, safety, max_time_step=1e300):
     m, exact = configs.isentropic_vortex(n=n, rs=rs)
@@ -278,7 +278,7 @@ def vortex_error(n, rs, steps, safety, max_time_step=1e300):
     return np.abs(m.state[0] - want).max(), st.flow_time
 
 
-def test_isentropic_vortex_config1():
+def test_isentropic_vortex_translation():
     """Config 1: 16x16 quads, row size 6, 100 steps at max_safety 0.5 track the translated exact vortex."""
     err, t = vortex_error(16, 6, 100, 0.5)
     assert t > 0.25

@@ -1,4 +1,4 @@
-"""Tree meshes with hanging faces (config 4): the builder's wiring of Refined_connection (connection.hpp:168-286)
+"""Tree meshes with hanging faces: the builder's wiring of Refined_connection (connection.hpp:168-286)
 and the oracle on it. The GPU side is compared with the oracle in test_gpu_tree.py."""
 import ctypes
 import collections
