@@ -322,7 +322,7 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU figures are not produced "
+        print("bench.py: the benchmark runs on one GPU only; multi-GPU figures are not produced "
               f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
         sys.exit(2)
     wl = WORKLOADS[args.config]

@@ -76,7 +76,7 @@ def main(argv=None):
         p.add_argument("--command", action="append", default=[])
     a = ap.parse_args(argv)
     w = sys.stdout.write
-    w(f"# {a.title}\n\nCommands (repository root on the GPU box, `export TMPDIR=/tmp`):\n\n```\n")
+    w(f"# {a.title}\n\nCommands (run from the repository root on an MI355X, `export TMPDIR=/tmp`):\n\n```\n")
     for c in a.command:
         w(c + "\n")
     w("```\n\n")
