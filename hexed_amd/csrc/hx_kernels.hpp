@@ -40,32 +40,6 @@ __device__ inline void load_ops(Ops<RS>& s, const Ops<RS>* g, int t, int nthread
 
 __device__ inline double* slot_ptr(const Dev_mesh& m, int slot) { return m.faces + size_t(slot) * m.face_sz; }
 
-// ---- face extrapolation of element data held in LDS (Spatial.hpp:41-57) ----------------------------------
-// src: [EPB][NV][NQ] in LDS; writes block `ldg` (0: state/flux, 1: LDG) of every face slot.
-template <int ND, int RS>
-__device__ inline void extrapolate_faces(const Dev_mesh& m, int cls, const Ops<RS>& ops, const double* src, int ldg)
-{
-  using S = Shape<ND, RS>;
-  using R = Row<ND, RS>;
-  const int n = m.n_elem[cls];
-  for (int k = threadIdx.x; k < S::EPB * S::NFACE_DOF; k += S::BLOCK) {
-    const int el = k / S::NFACE_DOF;
-    const int r = k % S::NFACE_DOF;
-    const int f = r / (S::NV * S::NFQ);
-    const int v = (r / S::NFQ) % S::NV;
-    const int fq = r % S::NFQ;
-    const int e = blockIdx.x * S::EPB + el;
-    if (e >= n) continue;
-    const int i = f / 2, s = f % 2;
-    const int b = R::base_of_fq(fq, i), sd = R::stride(i);
-    const double* row = src + (el * S::NV + v) * S::NQ + b;
-    double acc = 0;
-    for (int kk = 0; kk < RS; ++kk) acc += ops.bound[s][kk] * row[kk * sd];
-    const int slot = m.face_slot[cls][e * 2 * ND + f];
-    (ldg ? slot_ptr(m, slot) + S::NV * S::NFQ : state_ptr(m, slot, m.alt_wr))[v * S::NFQ + fq] = acc;
-  }
-}
-
 // Modal filter along every dimension in turn (Spatial.hpp:472-481): buf [EPB][nvar][NQ] in LDS, vals per thread.
 template <int ND, int RS, int NVAR>
 __device__ inline void filter_rows(const Ops<RS>& ops, double* buf, int el, int q, double (&vals)[NVAR])
@@ -1449,21 +1423,21 @@ void k_reconcile_ldg(Dev_mesh m, Eops<RS, double> o, double dt, const double* __
 }
 
 // ---- Write_face (Spatial.hpp:60-70) ----------------------------------------------------------------------
+// Write_face (Spatial.hpp:29-71) of the state: staged in LDS in the padded layout, then one thread per column of each
+// direction in turn extrapolating it to both of that direction's faces (write_faces_from, the operator constants
+// taken by value). scale_velocity 1: first divide the momentum by sqrt(2 rho E) in the state itself, the advection
+// velocity of Solver::update_art_visc_smoothness (Solver.cpp:468-478); 2: multiply it back (Solver.cpp:570-574).
+// Either way the Write_face that follows it in the reference reads the state once.
 template <int ND, int RS>
-// scale_velocity 1: first divide the momentum by sqrt(2 rho E) in the state itself, the advection velocity of
-// Solver::update_art_visc_smoothness (Solver.cpp:468-478); 2: multiply it back (Solver.cpp:570-574). Either way the
-// Write_face that follows it in the reference reads the state once.
-__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh m, const Ops<RS>* __restrict__ gops, int cls,
+__global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh m, Eops<RS, double> o, int cls,
                                                                        int scale_velocity)
 {
   using S = Shape<ND, RS>;
-  constexpr int NQ = S::NQ, NV = S::NV, EPB = S::EPB;
-  __shared__ Ops<RS> ops;
-  __shared__ double su[EPB][NV][NQ];
+  constexpr int NQ = S::NQ, NQP = S::NQP, NV = S::NV, EPB = S::EPB;
+  __shared__ double su[EPB * NV * NQP];
   const int t = threadIdx.x, el = t / NQ, q = t % NQ;
   const int e = blockIdx.x * EPB + el;
   const bool valid = e < m.n_elem[cls];
-  load_ops(ops, gops, t, S::BLOCK);
   double* st = m.state[cls] + size_t(e) * NV * NQ;
   double u[NV];
   for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : 0.;
@@ -1471,9 +1445,9 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_write_face(Dev_mesh 
     const double scale = sqrt(2 * u[ND] * u[ND + 1]);
     for (int d = 0; d < ND; ++d) st[d * NQ + q] = u[d] = scale_velocity == 1 ? u[d] / scale : u[d] * scale;
   }
-  for (int v = 0; v < NV; ++v) su[el][v][q] = u[v];
+  for (int v = 0; v < NV; ++v) su[el * NV * NQP + v * NQP + S::pad(q)] = u[v];
   __syncthreads();
-  extrapolate_faces<ND, RS>(m, cls, ops, &su[0][0][0], 0);
+  write_faces_from<ND, RS, S::BLOCK>(m, cls, blockIdx.x, o, su, NV * NQP);
 }
 
 // ---- Neighbor_reconcile (Spatial.hpp:716-759) ------------------------------------------------------------
@@ -2167,7 +2141,8 @@ struct Launchers
   {
     const int n = l.m.n_elem[cls];
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_write_face<ND, RS>), dim3(blocks_for(n, S::EPB)), dim3(S::BLOCK), 0, l.stream, l.m, ops(l), cls,
+    const Eops<RS, double> o = make_eops<RS, double>(hops(l));
+    hipLaunchKernelGGL((k_write_face<ND, RS>), dim3(blocks_for(n, S::EPB)), dim3(S::BLOCK), 0, l.stream, l.m, o, cls,
                        scale_velocity);
     return hipGetLastError();
   }
