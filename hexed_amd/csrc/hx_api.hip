@@ -849,6 +849,50 @@ struct Scratch
   }
 };
 
+// hx_local; project_av_rs > 0 (Advection only): its update also forms the Legendre projection of degree
+// project_av_rs - 1 into forcing[0] (the projection pass of update_art_visc_smoothness)
+hx_status local_impl(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual, int use_filter,
+                     int project_av_rs)
+{
+  if (!live(mesh) || !valid_cls(cls)) return HX_ERR_INVALID;
+  Launch l = make_launch(mesh);
+  bool visc;
+  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
+  const int kind = pde_kind(pde);
+  // HEXED_ASSERTs of the Local constructor (Spatial.hpp:322-323)
+  if ((visc || kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS) && i_stage)
+    return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
+  if (i_stage && compute_residual)
+    return fail(mesh->ctx, HX_ERR_INVALID, "residual calculation is a single-stage operation");
+  const double coef = i_stage ? dt * hx_basis_step_ratio(mesh->dm.rs) : dt;
+  if (is_aux(kind) && use_filter)
+    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the artificial-viscosity and admissibility PDEs are built without the modal filter");
+  if ((kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS) && compute_residual)
+    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the diffusion-only PDEs are built without compute_residual");
+  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/local/" + pde_name(pde), mesh->dm.n_elem[cls]);
+  if (kind == HX_PDE_ADVECTION)
+    HX_TRY(mesh->ctx, mesh->ks->adv_local(l, cls, coef, i_stage, compute_residual, pde->advect_length, project_av_rs), "local");
+  else if (kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS)
+    HX_TRY(mesh->ctx, mesh->ks->diff_local(l, cls, kind, coef, pde->diff_time), "local");
+  else HX_TRY(mesh->ctx, mesh->ks->local(l, cls, visc, coef, i_stage, compute_residual, use_filter), "local");
+  return HX_OK;
+}
+
+// compute_advection (kernels_convective.cpp:19, the COMPUTE_CONVECTION sequence); project_av_rs > 0: as local_impl
+hx_status compute_advection_impl(hx_mesh* mesh, const hx_kernel_options* o, double advect_length, int project_av_rs)
+{
+  hx_pde p{};
+  p.kind = HX_PDE_ADVECTION;
+  p.advect_length = advect_length;
+  hx_status s;
+  if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, o->i_stage))) return s;
+  if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, o->i_stage))) return s;
+  if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
+  for (int c = 0; c < 2; ++c)
+    if ((s = local_impl(mesh, c, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter, project_av_rs))) return s;
+  return hx_prolong_refined(mesh, &p, 0, 0);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1238,28 +1282,7 @@ hx_status hx_neighbor(hx_mesh* mesh, int cls, const hx_pde* pde, int)
 
 hx_status hx_local(hx_mesh* mesh, int cls, const hx_pde* pde, double dt, int i_stage, int compute_residual, int use_filter)
 {
-  if (!live(mesh) || !valid_cls(cls)) return HX_ERR_INVALID;
-  Launch l = make_launch(mesh);
-  bool visc;
-  if (hx_status s = pde_check(mesh, pde, &visc, &l)) return s;
-  const int kind = pde_kind(pde);
-  // HEXED_ASSERTs of the Local constructor (Spatial.hpp:322-323)
-  if ((visc || kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS) && i_stage)
-    return fail(mesh->ctx, HX_ERR_INVALID, "two-stage stabilization is not applicable to diffusion equations");
-  if (i_stage && compute_residual)
-    return fail(mesh->ctx, HX_ERR_INVALID, "residual calculation is a single-stage operation");
-  const double coef = i_stage ? dt * hx_basis_step_ratio(mesh->dm.rs) : dt;
-  if (is_aux(kind) && use_filter)
-    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the artificial-viscosity and admissibility PDEs are built without the modal filter");
-  if ((kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS) && compute_residual)
-    return fail(mesh->ctx, HX_ERR_UNSUPPORTED, "the diffusion-only PDEs are built without compute_residual");
-  Timer tm(mesh->ctx, std::string(cls_name(cls)) + "/local/" + pde_name(pde), mesh->dm.n_elem[cls]);
-  if (kind == HX_PDE_ADVECTION)
-    HX_TRY(mesh->ctx, mesh->ks->adv_local(l, cls, coef, i_stage, compute_residual, pde->advect_length), "local");
-  else if (kind == HX_PDE_SMOOTH_AV || kind == HX_PDE_FIX_ADMIS)
-    HX_TRY(mesh->ctx, mesh->ks->diff_local(l, cls, kind, coef, pde->diff_time), "local");
-  else HX_TRY(mesh->ctx, mesh->ks->local(l, cls, visc, coef, i_stage, compute_residual, use_filter), "local");
-  return HX_OK;
+  return local_impl(mesh, cls, pde, dt, i_stage, compute_residual, use_filter, 0);
 }
 
 hx_status hx_neighbor_reconcile(hx_mesh* mesh, int cls, const hx_pde* pde)
@@ -2454,16 +2477,7 @@ hx_status hx_apply_avc_diff_flux_bcs(hx_mesh* mesh)
 hx_status hx_compute_advection(hx_mesh* mesh, const hx_kernel_options* o, double advect_length)
 {
   if (!live(mesh) || !o) return HX_ERR_INVALID;
-  hx_pde p{};
-  p.kind = HX_PDE_ADVECTION;
-  p.advect_length = advect_length;
-  hx_status s;
-  if ((s = hx_neighbor(mesh, HX_CARTESIAN, &p, o->i_stage))) return s;
-  if ((s = hx_neighbor(mesh, HX_DEFORMED, &p, o->i_stage))) return s;
-  if ((s = hx_restrict_refined(mesh, &p, 1, 0))) return s;
-  if ((s = hx_local(mesh, HX_CARTESIAN, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
-  if ((s = hx_local(mesh, HX_DEFORMED, &p, o->dt, o->i_stage, o->compute_residual, o->use_filter))) return s;
-  return hx_prolong_refined(mesh, &p, 0, 0);
+  return compute_advection_impl(mesh, o, advect_length, 0);
 }
 
 hx_status hx_compute_smooth_av(hx_mesh* mesh, const hx_kernel_options* o, double diff_time, double cheby_step)
@@ -2969,10 +2983,14 @@ hx_status hx_update_art_visc_smoothness(hx_mesh* mesh, const hx_av_options* o, d
     for (int i = 0; i < 2; ++i) {
       if ((s = hx_apply_advection_bcs(mesh))) return s;
       ko.i_stage = i;
-      if ((s = hx_compute_advection(mesh, &ko, L))) return s;
+      // the last stage's Locals also form the Legendre projection into forcing[0]
+      const bool last = iter + 1 == o->advect_iters && i == 1;
+      if ((s = compute_advection_impl(mesh, &ko, L, last ? o->av_rs : 0))) return s;
     }
   }
-  if ((s = points(1))) return s;
+  // the projection pass, unless the last advection stage formed it
+  if (o->advect_iters == 0)
+    if ((s = points(1))) return s;
   // root-smear-square
   const double diff_time = o->diff_ratio * L * L / n_real;
   ko = hx_kernel_options{0., 0, 0, 0};

@@ -142,7 +142,7 @@ struct Adv_lds
 template <int ND, int RS, bool DEF>
 __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_adv_local(Dev_mesh m, Eops<RS, double> o, Av_consts<RS> ac,
                                                                       double coef, int stage, int compute_residual,
-                                                                      double advect_length)
+                                                                      double advect_length, int project)
 {
   using S = Shape<ND, RS>;
   using L = Adv_lds<ND, RS>;
@@ -228,6 +228,14 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_adv_local(Dev_mesh m
       if (!stage) a[v] = (a[v] + du) / pseudo;
       else a[v] += du / pseudo;
       m.advection[cls][(size_t(e) * RS + v) * NQ + q] = a[v];
+    }
+    // project: the advection's last stage also forms the forcing from the advected scalars just written, as the
+    // Legendre projection pass after the advection does (Solver.cpp:525-540; ac.orth is of the projection's degree)
+    if (project && !compute_residual) {
+      double proj = 0;
+      for (int i = 0; i < RS; ++i) proj += a[i] * ac.weight[i] * ac.orth[i];
+      const double* st = m.state[cls] + size_t(e) * (ND + 2) * NQ;
+      m.forcing[cls][size_t(e) * 4 * NQ + q] = proj * proj * 2 * st[(ND + 1) * NQ + q] / st[ND * NQ + q];
     }
   }
   // Write_face<Advection> of the updated state (Spatial.hpp:507)

@@ -2346,16 +2346,20 @@ struct Launchers
     }
     return hipGetLastError();
   }
-  static hipError_t adv_local(const Launch& l, int cls, double coef, int stage, int cr, double advect_length)
+  // project_av_rs > 0: the Legendre projection of degree project_av_rs - 1 into forcing[0] after the update
+  static hipError_t adv_local(const Launch& l, int cls, double coef, int stage, int cr, double advect_length,
+                              int project_av_rs)
   {
     const int n = l.m.n_elem[cls];
     if (n == 0) return hipSuccess;
     const Eops<RS, double> o = make_eops<RS, double>(hops(l));
-    const Av_consts<RS> c = av_consts(l, RS);
+    const Av_consts<RS> c = av_consts(l, project_av_rs > 0 ? project_av_rs : RS);
+    const int project = project_av_rs > 0;
     dim3 g(blocks_for(n, S::EPB)), b(S::BLOCK);
     if (cls == HX_DEFORMED)
-      hipLaunchKernelGGL((k_adv_local<ND, RS, true>), g, b, 0, l.stream, l.m, o, c, coef, stage, cr, advect_length);
-    else hipLaunchKernelGGL((k_adv_local<ND, RS, false>), g, b, 0, l.stream, l.m, o, c, coef, stage, cr, advect_length);
+      hipLaunchKernelGGL((k_adv_local<ND, RS, true>), g, b, 0, l.stream, l.m, o, c, coef, stage, cr, advect_length, project);
+    else
+      hipLaunchKernelGGL((k_adv_local<ND, RS, false>), g, b, 0, l.stream, l.m, o, c, coef, stage, cr, advect_length, project);
     return hipGetLastError();
   }
   template <int KIND>

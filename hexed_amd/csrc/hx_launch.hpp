@@ -39,7 +39,8 @@ struct Kernel_set
   // HX_PDE_FIX_ADMIS; the diff_* entries take the two diffusion-only kinds)
   hipError_t (*av_write_face)(const Launch&, int cls, int kind);
   hipError_t (*av_neighbor)(const Launch&, int cls, int kind);
-  hipError_t (*adv_local)(const Launch&, int cls, double coef, int stage, int compute_residual, double advect_length);
+  hipError_t (*adv_local)(const Launch&, int cls, double coef, int stage, int compute_residual, double advect_length,
+                          int project_av_rs);
   hipError_t (*diff_local)(const Launch&, int cls, int kind, double dt, double diff_time);
   hipError_t (*diff_neighbor_reconcile)(const Launch&, int cls, int kind);
   hipError_t (*diff_reconcile)(const Launch&, int cls, int kind, double dt, double diff_time, double cheby);
