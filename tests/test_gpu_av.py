@@ -145,15 +145,20 @@ def test_smooth_av_kernels(ctx, cfg):
     dev.close()
 
 
-@pytest.mark.parametrize("cfg", [(3, 4, "deformed"), (3, 6, "tree"), (2, 6, "cartesian")], ids=ids)
-def test_update_art_visc_smoothness(ctx, cfg):
-    say(f"[gpu] update_art_visc_smoothness {ids(cfg)}")
+SMOOTHNESS_CASES = [((3, 4, "deformed"), 4), ((3, 6, "tree"), 4), ((2, 6, "cartesian"), 4),
+                    # no advection (the projection then runs as its own pass) and a single advection iteration
+                    ((3, 4, "deformed"), 0), ((3, 4, "deformed"), 1)]
+
+
+@pytest.mark.parametrize("cfg,advect_iters", SMOOTHNESS_CASES, ids=lambda x: ids(x) if isinstance(x, tuple) else f"adv{x}")
+def test_update_art_visc_smoothness(ctx, cfg, advect_iters):
+    say(f"[gpu] update_art_visc_smoothness {ids(cfg)}, {advect_iters} advection iterations")
     nd, rs, kind = cfg
     host = build(cfg, seed=9)
     host.state = [configs_state(host, c) for c in (0, 1)]
     init_faces(host)
     dev = Device_mesh(ctx, host.copy())
-    opts = abi.av_options(0.2, rs, 1.0, 2.5, advect_iters=4, diff_iters=3, n_cheby=2)
+    opts = abi.av_options(0.2, rs, 1.0, 2.5, advect_iters=advect_iters, diff_iters=3, n_cheby=2)
     r_o = ctypes.c_double()
     s = host.struct()
     oracle.check(oracle.load().oracle_update_art_visc_smoothness(ctypes.byref(s), ctypes.byref(opts), ctypes.byref(r_o)))
