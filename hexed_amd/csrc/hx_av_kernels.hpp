@@ -436,6 +436,11 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_reconcile(Dev_me
   const int e = blockIdx.x * EPB + el;
   const bool valid = e < n;
   double* my = lds + el * PER;
+  // the point's own data are read with the face data, so that their latency overlaps it
+  double fv[NS];
+  for (int v = 0; v < NS; ++v) fv[v] = valid ? diff_vars<ND, RS, KIND>(m, cls, e)[v * NQ + q] : 0.;
+  const double tss = valid ? m.tss[cls][size_t(e) * NQ + q] : 0., nom = valid ? m.nom_size[cls][e] : 1.;
+  const double det = (DEF && valid) ? m.metrics[size_t(e) * (ND * ND + 1) * NQ + ND * ND * NQ + q] : 1.;
   for (int k = t; k < EPB * PF; k += S::BLOCK) {
     const int el2 = k / PF, r = k % PF, f = r / (NS * NFQ), vq = r % (NS * NFQ);
     const int e2 = blockIdx.x * EPB + el2;
@@ -448,12 +453,9 @@ __global__ __launch_bounds__((Shape<ND, RS>::BLOCK)) void k_sav_reconcile(Dev_me
     for (int v = 0; v < NS; ++v)
       rate[v] -= o.lift[ki][0] * my[((2 * i) * NS + v) * NFQ + fq] + o.lift[ki][1] * my[((2 * i + 1) * NS + v) * NFQ + fq];
   }
-  double fv[NS];
-  for (int v = 0; v < NS; ++v) fv[v] = valid ? diff_vars<ND, RS, KIND>(m, cls, e)[v * NQ + q] : 0.;
   if (valid) {
-    const double tss = m.tss[cls][size_t(e) * NQ + q];
-    double mult = dt * tss / m.nom_size[cls][e];
-    if constexpr (DEF) mult /= m.metrics[size_t(e) * (ND * ND + 1) * NQ + ND * ND * NQ + q];
+    double mult = dt * tss / nom;
+    if constexpr (DEF) mult /= det;
     double* out = diff_vars<ND, RS, KIND>(m, cls, e);
     for (int v = 0; v < NS; ++v) {
       fv[v] += rate[v] * mult;
