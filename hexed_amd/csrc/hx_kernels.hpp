@@ -852,11 +852,12 @@ __device__ __forceinline__ void local_euler_tile(const Dev_mesh& m, int blk, typ
   double u[NV];
   const double* st = m.state[cls] + size_t(e) * NV * NQ;
   for (int v = 0; v < NV; ++v) u[v] = valid ? st[v * NQ + q] : (v >= ND ? 1. : 0.);
-  // what the stage update at the end reads from memory -- the residual cache (stage 1), the time step scale, the
-  // nominal size. Cartesian elements load it at the start of the last direction, so that its latency overlaps that
-  // direction's phases (Euler Local 0.82 -> 0.78 ms per step on the 32^3 box, 0.48 -> 0.45 on the 16^3 tree);
-  // deformed ones at the update, where the earlier loads cost more registers than their latency saves
-  // (1.78 -> 1.84 ms on the 48^3 deformed mesh)
+  // what the stage update at the end reads from memory -- the residual cache (stage 1), the nominal size, the time
+  // step scale and (deformed) the Jacobian determinant. Folded kernels up to row size 6 load the last two at the start
+  // of the last direction, so that their latency overlaps that direction's phases; on Cartesian elements the first two
+  // as well (Euler Local 0.82 -> 0.78 ms per step on the 32^3 box, 0.48 -> 0.45 on the 16^3 tree). On deformed
+  // elements those are read at the update: earlier, they cost more registers than their latency saves (1.78 -> 1.84 ms
+  // on the 48^3 deformed mesh).
   constexpr bool EARLY = !DEF && FOLD && RS <= 6;
   double rcv[NV];
   double tss_q = 0., nom = 1., det_q = 1.;
