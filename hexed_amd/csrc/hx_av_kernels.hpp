@@ -649,7 +649,8 @@ __global__ __launch_bounds__(256) void k_fix_av(Dev_mesh m, int cls, int mode)
 }
 
 // ---- point loops of update_art_visc_smoothness (Solver.cpp:525-540,549-574) ------------------------------------
-// what 1: Legendre projection of the advected scalars into forcing[0]; 2: new coefficient us_max f/(us_max + f),
+// what 1: Legendre projection of the advected scalars into forcing[0] (only without advection iterations: otherwise
+// the last Advection stage's Local forms it, k_adv_local with project); 2: new coefficient us_max f/(us_max + f),
 // squared change weighted into `resid`. (The momentum's division by sqrt(2 rho E) before the advection and its
 // restoration in the last loop run in the Write_face that follows each: k_write_face with scale_velocity.)
 template <int ND, int RS>
