@@ -180,6 +180,57 @@ def time_artificial_viscosity(ctx, m, reps=3):
     return out
 
 
+def time_setup_paths(ctx, m, opts, reps=3):
+    """The paths around the time step on the workload's mesh, outside the timed region: Solver::calc_jacobian (element
+    metrics, face normals and vertex time step scales on the device; Solver.cpp:273-381), with the oracle's on the host
+    cores beside it; Solver::is_admissible (Solver.cpp:921-959); and Solver::fix_admissibility (Solver.cpp:961-1095)
+    on the initial state with the energy made negative at three quadrature points of three elements, with its
+    iteration count. Milliseconds per call (the fix: one call)."""
+    import oracle
+    out = {}
+    host = m.copy()
+    dev = Device_mesh(ctx, host)
+    reset(dev, host)
+    vd = host.vertex_data()
+
+    def timed(call, n):
+        call()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            r = call()
+        ctx.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / n, r
+
+    ms, _ = timed(lambda: dev.calc_jacobian(vd), reps)
+    lib = oracle.load()
+    ref = m.copy()
+    s, vdr = ref.struct(), ref.vertex_data()
+    t0 = time.perf_counter()
+    oracle.check(lib.oracle_calc_jacobian(ctypes.byref(s), ctypes.byref(vdr)))
+    out["calc_jacobian"] = {"ms_per_call": round(ms, 3), "cpu_oracle_ms": round(1e3 * (time.perf_counter() - t0), 1),
+                            "cpu_threads": lib.oracle_get_threads()}
+    ms, ok = timed(dev.is_admissible, 10 * reps)
+    out["is_admissible"] = {"ms_per_call": round(ms, 3), "admissible": ok}
+    # three points of three elements made inadmissible (energy < 0), then one fix
+    nv, nd = host.nv, host.n_dim
+    c = 1 if host.n_elem[1] else 0
+    bad = host.state[c].copy()
+    n = host.n_elem[c]
+    for k, e in enumerate((n // 7, n // 2, (9 * n) // 10)):
+        bad[e, nd + 1, (17 * k + 5) % host.nq] = -0.02
+    dev.upload("state", c, bad)
+    dev.compute_write_face()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    iters = dev.fix_admissibility(opts, 0.7)
+    ctx.synchronize()
+    out["fix_admissibility"] = {"ms_per_call": round(1e3 * (time.perf_counter() - t0), 3), "iterations": iters,
+                                "admissible_after": dev.is_admissible()}
+    dev.close()
+    return out
+
+
 def host_cpu():
     """The host's CPU model (lscpu), its logical CPU count (nproc) and the CPUs this process may run on."""
     import subprocess
@@ -317,6 +368,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None, help="oracle steps for the CPU baseline (0 to skip)")
     ap.add_argument("--err-steps", type=int, default=10, help="steps after which FP32-MFMA is compared with FP64")
     ap.add_argument("--av", type=int, default=1, help="ns48: also time the artificial-viscosity kernels (0 to skip)")
+    ap.add_argument("--setup", type=int, default=1,
+                    help="ns48: also time calc_jacobian, is_admissible and fix_admissibility (0 to skip)")
     ap.add_argument("--face-fold", type=int, default=-1, choices=(-1, 0, 1),
                     help="fold the face fluxes into the Local: 1 always, 0 never, -1 where 2 n_dim <= row_size")
     args = ap.parse_args()
@@ -380,6 +433,11 @@ def main():
     if args.config == "ns48" and args.av:
         aux = time_artificial_viscosity(ctx, hosts[0][0])
         print("[bench] artificial viscosity: " + ", ".join(f"{k} {v['ms_per_call']} ms" for k, v in aux.items()),
+              file=sys.stderr, flush=True)
+    setup_paths = None
+    if args.config == "ns48" and args.setup:
+        setup_paths = time_setup_paths(ctx, hosts[0][0], opts)
+        print("[bench] setup paths: " + ", ".join(f"{k} {v['ms_per_call']} ms" for k, v in setup_paths.items()),
               file=sys.stderr, flush=True)
 
     # dominant kernel of the chosen form and its roofline: algorithmic bytes of every launch in its category over the
@@ -501,7 +559,7 @@ def main():
                                                   "max": float(f"{max(fp32_err):.3e}")},
         },
         "roofline": roofline, "cpu_baseline": cpu, "parity_vs_cpu": parity, "kernel_breakdown": breakdown,
-        "artificial_viscosity": aux,
+        "artificial_viscosity": aux, "setup_paths": setup_paths,
     }
     print(json.dumps(line))
     if group is not None:
