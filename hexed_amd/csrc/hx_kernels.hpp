@@ -1042,13 +1042,15 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
   __shared__ double ld[EPB * L::PER_D];
   // FP32-MFMA form: the matrix-core operands, read from LDS at each contraction (Contraction_lds)
   __shared__ __align__(16) float mtab[P::mfma ? 4 * Eops<RS, double>::MF_STRIDE : 4];
-  // FP64 form: the elements' face slots, read with the first loads so that the viscous flux extrapolations below
-  // need no global read ahead of their writes (3.13 -> 3.09 ms on the 48^3 deformed mesh; the FP32-MFMA form, whose
-  // registers are full, lost 1% with it)
-  __shared__ int fsl[P::mfma ? 1 : EPB * 2 * ND];
+  // FP64 form, where the LDS keeps the workgroups per compute unit: the elements' face slots, read with the first
+  // loads so that the viscous flux extrapolations below need no global read ahead of their writes (3.13 -> 3.09 ms on
+  // the 48^3 deformed mesh; the FP32-MFMA form, whose registers are full, lost 1% with it)
+  constexpr int LDS_B = EPB * (int(sizeof(T)) * L::PER_T + 8 * L::PER_D), SLOTS_B = 4 * EPB * 2 * ND;
+  constexpr bool STAGE = !P::mfma && 163840 / (LDS_B + SLOTS_B) == 163840 / LDS_B;
+  __shared__ int fsl[STAGE ? EPB * 2 * ND : 1];
   const int t = threadIdx.x;
   const int blk = blockIdx.x;
-  if (!P::mfma && t < EPB * 2 * ND) {
+  if (STAGE && t < EPB * 2 * ND) {
     const int e2 = blk * EPB + t / (2 * ND);
     fsl[t] = e2 < m.n_elem[cls] ? m.face_slot[cls][e2 * 2 * ND + t % (2 * ND)] : 0;
   }
@@ -1267,7 +1269,7 @@ void k_local_ns(Dev_mesh m, Eops<RS, double> o,
       con.faces_if(false, x, 0., 0., y, bx);
       const int e2 = blk * EPB + el2;
       if (ok && e2 < n) {
-        const int* fs = (P::mfma ? m.face_slot[cls] + e2 * 2 * ND : fsl + el2 * 2 * ND) + 2 * i;
+        const int* fs = (STAGE ? fsl + el2 * 2 * ND : m.face_slot[cls] + e2 * 2 * ND) + 2 * i;
         slot_ptr(m, fs[0])[(NV + v) * NFQ + r] = double(bx[0]);
         slot_ptr(m, fs[1])[(NV + v) * NFQ + r] = double(bx[1]);
       }
